@@ -1,0 +1,323 @@
+// context.hip -- context, device memory accounting, vectors, shared vector kernels.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "internal.hpp"
+#include "reduce.hpp"
+
+namespace fvg {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int status, const std::string& msg) {
+    g_last_error = msg;
+    return status;
+}
+
+int dev_alloc(fvg_ctx* ctx, size_t bytes, void** out) {
+    *out = nullptr;
+    if (ctx->budget && ctx->in_use + bytes > ctx->budget)
+        return fail(FVG_E_BUDGET, "allocation of " + std::to_string(bytes) + " bytes exceeds the HBM budget (" +
+                                      std::to_string(ctx->in_use) + " of " + std::to_string(ctx->budget) +
+                                      " in use)");
+    hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(FVG_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    ctx->in_use += bytes;
+    return FVG_OK;
+}
+
+void dev_free(fvg_ctx* ctx, void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    ctx->in_use = ctx->in_use >= bytes ? ctx->in_use - bytes : 0;
+}
+
+// ---- kernels ------------------------------------------------------------------------
+
+__global__ void k_fill(int64_t n, double* __restrict__ x, double v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+__global__ void k_copy(int64_t n, const double* __restrict__ s, double* __restrict__ d) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+// Each block sums a contiguous tile of kBlock*kItems elements (item-major per thread,
+// then the block tree); the tile decomposition depends only on n.
+constexpr int kItems = 8;
+
+__global__ void __launch_bounds__(kBlock) k_dot_partial(int64_t n, const double* __restrict__ a,
+                                                        const double* __restrict__ b,
+                                                        double* __restrict__ partials) {
+    __shared__ double lds[kBlock / 64];
+    const int64_t tiles = gridDim.x;
+    const int64_t per = (n + tiles - 1) / tiles;
+    const int64_t lo = (int64_t)blockIdx.x * per;
+    const int64_t hi = lo + per < n ? lo + per : n;
+    double s = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) s += a[i] * b[i];
+    s = block_sum<kBlock>(s, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kBlock) k_reduce_final(const double* __restrict__ partials, int count,
+                                                         double* __restrict__ out) {
+    __shared__ double lds[kBlock / 64];
+    double s = sum_partials<kBlock>(partials, count, lds);
+    if (threadIdx.x == 0) *out = s;
+}
+
+int reduce_blocks_for(int64_t n) {
+    int64_t g = (n + (int64_t)kBlock * kItems - 1) / ((int64_t)kBlock * kItems);
+    if (g < 1) g = 1;
+    if (g > kMaxReduceBlocks) g = kMaxReduceBlocks;
+    return (int)g;
+}
+
+int launch_dot(fvg_ctx* ctx, int64_t n, const double* a, const double* b, double* out_dev) {
+    int g = reduce_blocks_for(n);
+    hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(kBlock), 0, ctx->stream, n, a, b, ctx->partials);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, g, out_dev);
+    FVG_HIP(hipGetLastError());
+    return FVG_OK;
+}
+
+int launch_fill(fvg_ctx* ctx, int64_t n, double* x, double value) {
+    if (n <= 0) return FVG_OK;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, x, value);
+    FVG_HIP(hipGetLastError());
+    return FVG_OK;
+}
+
+int launch_copy(fvg_ctx* ctx, int64_t n, const double* src, double* dst) {
+    if (n <= 0) return FVG_OK;
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, src, dst);
+    FVG_HIP(hipGetLastError());
+    return FVG_OK;
+}
+
+int read_scalars(fvg_ctx* ctx, const double* dev, double* host, int count) {
+    FVG_HIP(hipMemcpyAsync(ctx->pinned, dev, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
+    FVG_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(host, ctx->pinned, sizeof(double) * count);
+    return FVG_OK;
+}
+
+int timing_begin(fvg_ctx* ctx) {
+    if (!ctx->timing) return -1;
+    fvg_ctx::Span s{};
+    if (hipEventCreate(&s.a) != hipSuccess || hipEventCreate(&s.b) != hipSuccess) return -1;
+    (void)hipEventRecord(s.a, ctx->stream);
+    ctx->spans.push_back(s);
+    return (int)ctx->spans.size() - 1;
+}
+
+void timing_end(fvg_ctx* ctx, int token, int kind, int64_t launches) {
+    if (token < 0 || token >= (int)ctx->spans.size()) return;
+    auto& s = ctx->spans[(size_t)token];
+    (void)hipEventRecord(s.b, ctx->stream);
+    s.kind = kind;
+    s.launches = launches;
+}
+
+static int collect_spans(fvg_ctx* ctx) {
+    if (ctx->spans.empty()) return FVG_OK;
+    FVG_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto& s : ctx->spans) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, s.a, s.b) == hipSuccess) {
+            if (s.kind == kSpanSpmv) {
+                ctx->stats.spmv_ms += ms;
+                ctx->stats.spmv_launches += s.launches;
+            } else {
+                ctx->stats.sweep_ms += ms;
+                ctx->stats.sweep_launches += s.launches;
+            }
+        }
+        (void)hipEventDestroy(s.a);
+        (void)hipEventDestroy(s.b);
+    }
+    ctx->spans.clear();
+    return FVG_OK;
+}
+
+}  // namespace fvg
+
+using namespace fvg;
+
+extern "C" {
+
+const char* fvg_last_error(void) { return g_last_error.c_str(); }
+
+const char* fvg_status_string(int s) {
+    switch (s) {
+    case FVG_OK: return "ok";
+    case FVG_E_INVALID: return "invalid argument";
+    case FVG_E_HIP: return "HIP runtime error";
+    case FVG_E_BUDGET: return "HBM budget exceeded";
+    case FVG_E_ZERO_PIVOT: return "zero pivot in ILU";
+    case FVG_E_NOT_SYMMETRIC: return "matrix not symmetric";
+    case FVG_E_MISSING_ENTRY: return "sparse matrix entry not found";
+    case FVG_E_MESH: return "invalid mesh";
+    case FVG_E_TIMEOUT: return "device wait timed out";
+    case FVG_E_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+    }
+}
+
+int fvg_ctx_create(int device, uint64_t budget, fvg_ctx** out) {
+    if (!out) return fail(FVG_E_INVALID, "null output");
+    *out = nullptr;
+    int count = 0;
+    FVG_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(FVG_E_INVALID, "no such device");
+    FVG_HIP(hipSetDevice(device));
+    auto* c = new fvg_ctx();
+    c->device = device;
+    c->budget = budget;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(FVG_E_HIP, "hipStreamCreate failed");
+    }
+    int s = dev_alloc_n(c, (int64_t)kMaxReduceBlocks * 4, &c->partials);
+    c->partials_cap = kMaxReduceBlocks;
+    if (s == FVG_OK) s = dev_alloc_n(c, 64, &c->scalar);
+    if (s == FVG_OK && hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault) != hipSuccess)
+        s = fail(FVG_E_HIP, "hipHostMalloc failed");
+    if (s != FVG_OK) {
+        fvg_ctx_destroy(c);
+        return s;
+    }
+    *out = c;
+    return FVG_OK;
+}
+
+int fvg_ctx_destroy(fvg_ctx* c) {
+    if (!c) return FVG_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& s : c->spans) {
+        (void)hipEventDestroy(s.a);
+        (void)hipEventDestroy(s.b);
+    }
+    dev_free_n(c, c->partials, c->partials_cap * 4);
+    dev_free_n(c, c->scalar, 64);
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FVG_OK;
+}
+
+int fvg_ctx_sync(fvg_ctx* c) {
+    if (!c) return fail(FVG_E_INVALID, "null context");
+    FVG_HIP(hipStreamSynchronize(c->stream));
+    return FVG_OK;
+}
+
+int fvg_ctx_bytes_in_use(const fvg_ctx* c, uint64_t* bytes) {
+    if (!c || !bytes) return fail(FVG_E_INVALID, "null argument");
+    *bytes = c->in_use;
+    return FVG_OK;
+}
+
+int fvg_ctx_stream(const fvg_ctx* c, void** stream) {
+    if (!c || !stream) return fail(FVG_E_INVALID, "null argument");
+    *stream = (void*)c->stream;
+    return FVG_OK;
+}
+
+int fvg_ctx_enable_timing(fvg_ctx* c, int enable) {
+    if (!c) return fail(FVG_E_INVALID, "null context");
+    c->timing = enable != 0;
+    return FVG_OK;
+}
+
+int fvg_ctx_kernel_stats(fvg_ctx* c, fvg_kernel_stats* out, int reset) {
+    if (!c || !out) return fail(FVG_E_INVALID, "null argument");
+    FVG_TRY(collect_spans(c));
+    *out = c->stats;
+    if (reset) c->stats = fvg_kernel_stats{};
+    return FVG_OK;
+}
+
+int fvg_vec_create(fvg_ctx* c, int64_t n, fvg_vec** out) {
+    if (!c || !out || n < 0) return fail(FVG_E_INVALID, "bad vector arguments");
+    auto* v = new fvg_vec();
+    v->n = n;
+    int s = dev_alloc_n(c, n, &v->d);
+    if (s != FVG_OK) {
+        delete v;
+        return s;
+    }
+    if (n > 0) {
+        s = launch_fill(c, n, v->d, 0.0);
+        if (s != FVG_OK) {
+            dev_free_n(c, v->d, n);
+            delete v;
+            return s;
+        }
+    }
+    *out = v;
+    return FVG_OK;
+}
+
+int fvg_vec_wrap(fvg_ctx* c, int64_t n, double* ptr, fvg_vec** out) {
+    if (!c || !out || n < 0 || (!ptr && n > 0)) return fail(FVG_E_INVALID, "bad vector arguments");
+    auto* v = new fvg_vec();
+    v->n = n;
+    v->d = ptr;
+    v->owned = false;
+    *out = v;
+    return FVG_OK;
+}
+
+int fvg_vec_destroy(fvg_ctx* c, fvg_vec* v) {
+    if (!v) return FVG_OK;
+    if (v->owned && c) {
+        (void)hipStreamSynchronize(c->stream);
+        dev_free_n(c, v->d, v->n);
+    }
+    delete v;
+    return FVG_OK;
+}
+
+int fvg_vec_len(const fvg_vec* v, int64_t* n) {
+    if (!v || !n) return fail(FVG_E_INVALID, "null argument");
+    *n = v->n;
+    return FVG_OK;
+}
+
+int fvg_vec_device_ptr(const fvg_vec* v, double** p) {
+    if (!v || !p) return fail(FVG_E_INVALID, "null argument");
+    *p = v->d;
+    return FVG_OK;
+}
+
+int fvg_vec_upload(fvg_ctx* c, fvg_vec* v, const double* host) {
+    if (!c || !v || (!host && v->n)) return fail(FVG_E_INVALID, "null argument");
+    FVG_TRY(upload(c, v->d, host, v->n));
+    FVG_HIP(hipStreamSynchronize(c->stream));
+    return FVG_OK;
+}
+
+int fvg_vec_download(fvg_ctx* c, const fvg_vec* v, double* host) {
+    if (!c || !v || (!host && v->n)) return fail(FVG_E_INVALID, "null argument");
+    return download(c, host, v->d, v->n);
+}
+
+int fvg_vec_dot(fvg_ctx* c, const fvg_vec* a, const fvg_vec* b, double* out) {
+    if (!c || !a || !b || !out) return fail(FVG_E_INVALID, "null argument");
+    if (a->n != b->n) return fail(FVG_E_INVALID, "vector lengths differ");
+    FVG_TRY(launch_dot(c, a->n, a->d, b->d, c->scalar));
+    return read_scalars(c, c->scalar, out, 1);
+}
+
+}  // extern "C"

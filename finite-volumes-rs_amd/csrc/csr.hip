@@ -1,0 +1,197 @@
+// csr.hip -- DistributedMatrix<f64> on the device and its SpMV (dmatrix.rs:136-146).
+//
+// SpMV is a row-block CSR-stream kernel: a workgroup owns kSpmvRows consecutive rows,
+// streams their contiguous (column, value) range from HBM with fully coalesced
+// loads, forms each product v*x[col] once into LDS, and then every thread sums its
+// own row from LDS in ascending column order starting from 0.0 -- the reference's
+// sequential `result += value * x` loop, so the result is bit-identical to it.
+#include <cstring>
+
+#include "reduce.hpp"
+#include "spmv.hpp"
+
+namespace fvg {
+
+constexpr int kSpmvRows = 256;     // rows (and threads) per workgroup
+constexpr int kSpmvStage = 3072;   // staged products per workgroup (24 KiB of LDS)
+
+int spmv_blocks(int64_t nrows) { return grid_for(nrows, kSpmvRows); }
+
+int64_t partial_stride(const fvg_ctx* ctx) { return ctx->partials_cap; }
+
+int ensure_partials(fvg_ctx* ctx, int64_t count) {
+    if (count <= ctx->partials_cap) return FVG_OK;
+    FVG_HIP(hipStreamSynchronize(ctx->stream));
+    dev_free_n(ctx, ctx->partials, ctx->partials_cap * 4);
+    ctx->partials_cap = 0;
+    FVG_TRY(dev_alloc_n(ctx, count * 4, &ctx->partials));
+    ctx->partials_cap = count;
+    return FVG_OK;
+}
+
+template <bool EPI>
+__global__ void __launch_bounds__(kSpmvRows) k_spmv(int n, const int32_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ ci, const double* __restrict__ v,
+                                                    const double* __restrict__ x, double* __restrict__ y,
+                                                    SpmvEpilogue epi, double* __restrict__ partials,
+                                                    int64_t pstride) {
+    __shared__ double prod[kSpmvStage];
+    __shared__ double red[kSpmvRows / 64];
+    if (EPI && epi.skip && *epi.skip) return;
+    const int r0 = blockIdx.x * kSpmvRows;
+    const int r1 = min(r0 + kSpmvRows, n);
+    const int base = rp[r0];
+    const int cnt = rp[r1] - base;
+    const int r = r0 + (int)threadIdx.x;
+    double sum = 0.0;
+    if (cnt <= kSpmvStage) {
+        for (int k = threadIdx.x; k < cnt; k += kSpmvRows) prod[k] = v[base + k] * x[ci[base + k]];
+        __syncthreads();
+        if (r < r1) {
+            const int e = rp[r + 1] - base;
+            for (int k = rp[r] - base; k < e; ++k) sum += prod[k];
+        }
+    } else if (r < r1) {
+        const int e = rp[r + 1];
+        for (int k = rp[r]; k < e; ++k) sum += v[k] * x[ci[k]];
+    }
+    if (!EPI) {
+        if (r < r1) y[r] = sum;
+        return;
+    }
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    if (r < r1) {
+        if (epi.b) sum = (sum - epi.b[r]) * -1.0;
+        y[r] = sum;
+        if (epi.copy1) epi.copy1[r] = sum;
+        if (epi.copy2) epi.copy2[r] = sum;
+        if (epi.dotw) d0 = epi.dotw[r] * sum;
+        if (epi.dot_self) d1 = sum * sum;
+        if (epi.dotw2) d2 = epi.dotw2[r] * sum;
+    }
+    if (epi.dotw) {
+        d0 = block_sum<kSpmvRows>(d0, red);
+        if (threadIdx.x == 0) partials[blockIdx.x] = d0;
+    }
+    if (epi.dot_self) {
+        d1 = block_sum<kSpmvRows>(d1, red);
+        if (threadIdx.x == 0) partials[pstride + blockIdx.x] = d1;
+    }
+    if (epi.dotw2) {
+        d2 = block_sum<kSpmvRows>(d2, red);
+        if (threadIdx.x == 0) partials[2 * pstride + blockIdx.x] = d2;
+    }
+}
+
+int launch_spmv(fvg_ctx* ctx, const fvg_csr* a, const double* x, double* y, const SpmvEpilogue& epi) {
+    const int64_t n = a->nrows;
+    if (n <= 0) return FVG_OK;
+    const int g = spmv_blocks(n);
+    const bool has_epi = epi.b || epi.copy1 || epi.copy2 || epi.dotw || epi.dotw2 || epi.dot_self || epi.skip;
+    if (has_epi) FVG_TRY(ensure_partials(ctx, g));
+    const int tok = timing_begin(ctx);
+    if (has_epi)
+        hipLaunchKernelGGL(k_spmv<true>, dim3(g), dim3(kSpmvRows), 0, ctx->stream, (int)n, a->row_ptr, a->col, a->val,
+                           x, y, epi, ctx->partials, ctx->partials_cap);
+    else
+        hipLaunchKernelGGL(k_spmv<false>, dim3(g), dim3(kSpmvRows), 0, ctx->stream, (int)n, a->row_ptr, a->col,
+                           a->val, x, y, epi, ctx->partials, ctx->partials_cap);
+    timing_end(ctx, tok, kSpanSpmv, 1);
+    FVG_HIP(hipGetLastError());
+    return FVG_OK;
+}
+
+static void free_csr(fvg_ctx* c, fvg_csr* a) {
+    dev_free_n(c, a->row_ptr, a->nrows + 1);
+    dev_free_n(c, a->col, a->nnz);
+    dev_free_n(c, a->val, a->nnz);
+}
+
+}  // namespace fvg
+
+using namespace fvg;
+
+extern "C" {
+
+int fvg_csr_create(fvg_ctx* c, int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* row_ptr,
+                   const int32_t* col_idx, const double* values, fvg_csr** out) {
+    if (!c || !out || !row_ptr || (nnz > 0 && !col_idx)) return fail(FVG_E_INVALID, "null argument");
+    if (nrows < 0 || ncols < 0 || nnz < 0 || nnz >= INT32_MAX || nrows >= INT32_MAX)
+        return fail(FVG_E_INVALID, "bad CSR sizes");
+    if (row_ptr[0] != 0 || row_ptr[nrows] != nnz) return fail(FVG_E_INVALID, "row_ptr must run from 0 to nnz");
+    for (int64_t i = 0; i < nrows; ++i) {
+        if (row_ptr[i + 1] < row_ptr[i]) return fail(FVG_E_INVALID, "row_ptr must be non-decreasing");
+        for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+            if (col_idx[k] < 0 || col_idx[k] >= ncols) return fail(FVG_E_INVALID, "column index out of range");
+            if (k > row_ptr[i] && col_idx[k] <= col_idx[k - 1])
+                return fail(FVG_E_INVALID, "columns must be strictly increasing within a row");
+        }
+    }
+    *out = nullptr;
+    auto* a = new fvg_csr();
+    a->nrows = nrows;
+    a->ncols = ncols;
+    a->nnz = nnz;
+    a->h_row_ptr.assign(row_ptr, row_ptr + nrows + 1);
+    a->h_col.assign(col_idx, col_idx + nnz);
+    int s;
+    if ((s = dev_alloc_n(c, nrows + 1, &a->row_ptr)) || (s = dev_alloc_n(c, nnz, &a->col)) ||
+        (s = dev_alloc_n(c, nnz, &a->val)) || (s = upload(c, a->row_ptr, row_ptr, nrows + 1)) ||
+        (s = upload(c, a->col, col_idx, nnz))) {
+        free_csr(c, a);
+        delete a;
+        return s;
+    }
+    if (values) s = upload(c, a->val, values, nnz);
+    else if (nnz > 0) s = launch_fill(c, nnz, a->val, 0.0);
+    if (s == FVG_OK && hipStreamSynchronize(c->stream) != hipSuccess) s = fail(FVG_E_HIP, "CSR upload failed");
+    if (s != FVG_OK) {
+        free_csr(c, a);
+        delete a;
+        return s;
+    }
+    *out = a;
+    return FVG_OK;
+}
+
+int fvg_csr_destroy(fvg_ctx* c, fvg_csr* a) {
+    if (!a) return FVG_OK;
+    if (c) {
+        (void)hipStreamSynchronize(c->stream);
+        free_csr(c, a);
+    }
+    delete a;
+    return FVG_OK;
+}
+
+int fvg_csr_sizes(const fvg_csr* a, int64_t* nrows, int64_t* ncols, int64_t* nnz) {
+    if (!a) return fail(FVG_E_INVALID, "null matrix");
+    if (nrows) *nrows = a->nrows;
+    if (ncols) *ncols = a->ncols;
+    if (nnz) *nnz = a->nnz;
+    return FVG_OK;
+}
+
+int fvg_csr_download(fvg_ctx* c, const fvg_csr* a, int32_t* row_ptr, int32_t* col_idx, double* values) {
+    if (!c || !a) return fail(FVG_E_INVALID, "null argument");
+    if (row_ptr) std::memcpy(row_ptr, a->h_row_ptr.data(), sizeof(int32_t) * (size_t)(a->nrows + 1));
+    if (col_idx) std::memcpy(col_idx, a->h_col.data(), sizeof(int32_t) * (size_t)a->nnz);
+    if (values) FVG_TRY(download(c, values, a->val, a->nnz));
+    return FVG_OK;
+}
+
+int fvg_csr_upload_values(fvg_ctx* c, fvg_csr* a, const double* values) {
+    if (!c || !a || (!values && a->nnz)) return fail(FVG_E_INVALID, "null argument");
+    FVG_TRY(upload(c, a->val, values, a->nnz));
+    FVG_HIP(hipStreamSynchronize(c->stream));
+    return FVG_OK;
+}
+
+int fvg_spmv(fvg_ctx* c, const fvg_csr* a, const fvg_vec* x, fvg_vec* y) {
+    if (!c || !a || !x || !y) return fail(FVG_E_INVALID, "null argument");
+    if (x->n < a->ncols || y->n != a->nrows) return fail(FVG_E_INVALID, "vector lengths do not match the matrix");
+    if (x->d == y->d) return fail(FVG_E_INVALID, "x and y must not alias");
+    return launch_spmv(c, a, x->d, y->d, SpmvEpilogue{});
+}
+
+}  // extern "C"

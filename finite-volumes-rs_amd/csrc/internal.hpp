@@ -1,0 +1,113 @@
+// internal.hpp -- shared definitions of libfvg (not part of the public ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fvg.h"
+
+namespace fvg {
+
+// ---- error reporting ------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int status, const std::string& msg);
+
+#define FVG_HIP(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::fvg::fail(FVG_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define FVG_TRY(expr)              \
+    do {                           \
+        int s_ = (expr);           \
+        if (s_ != FVG_OK) return s_; \
+    } while (0)
+
+// ---- launch geometry ------------------------------------------------------------
+constexpr int kBlock = 256;                 // threads per workgroup (4 wave64)
+constexpr int kMaxReduceBlocks = 2048;      // partial sums per reduction
+inline int grid_for(int64_t n, int per_block = kBlock) {
+    int64_t g = (n + per_block - 1) / per_block;
+    return (int)(g < 1 ? 1 : g);
+}
+
+}  // namespace fvg
+
+// ---- public handle definitions -----------------------------------------------------
+struct fvg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;       // 0 = unlimited
+    uint64_t in_use = 0;
+    double* partials = nullptr;        // 4 slots of partials_cap doubles of reduction scratch
+    int64_t partials_cap = 0;
+    double* scalar = nullptr;          // small device scalar area
+    void* pinned = nullptr;            // pinned host staging (4 KiB)
+    bool timing = false;
+    struct Span { hipEvent_t a, b; int kind; int64_t launches; };
+    std::vector<Span> spans;           // event pairs recorded while timing is enabled
+    fvg_kernel_stats stats{};
+};
+
+struct fvg_vec {
+    int64_t n = 0;
+    double* d = nullptr;
+    bool owned = true;
+};
+
+struct fvg_csr {
+    int64_t nrows = 0, ncols = 0, nnz = 0;
+    int32_t* row_ptr = nullptr;
+    int32_t* col = nullptr;
+    double* val = nullptr;
+    // host copy of the pattern, used by the preconditioner analysis
+    std::vector<int32_t> h_row_ptr, h_col;
+    uint64_t pattern_id = 0;           // identifies a mesh pattern (assembly target check)
+};
+
+namespace fvg {
+
+// ---- device memory accounted against the context budget ---------------------------
+int dev_alloc(fvg_ctx* ctx, size_t bytes, void** out);
+void dev_free(fvg_ctx* ctx, void* p, size_t bytes);
+template <class T>
+int dev_alloc_n(fvg_ctx* ctx, int64_t n, T** out) {
+    return dev_alloc(ctx, (size_t)(n > 0 ? n : 1) * sizeof(T), (void**)out);
+}
+template <class T>
+void dev_free_n(fvg_ctx* ctx, T*& p, int64_t n) {
+    if (p) dev_free(ctx, (void*)p, (size_t)(n > 0 ? n : 1) * sizeof(T));
+    p = nullptr;
+}
+template <class T>
+int upload(fvg_ctx* ctx, T* dst, const T* src, int64_t n) {
+    if (n <= 0) return FVG_OK;
+    FVG_HIP(hipMemcpyAsync(dst, src, (size_t)n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return FVG_OK;
+}
+template <class T>
+int download(fvg_ctx* ctx, T* dst, const T* src, int64_t n) {
+    if (n <= 0) return FVG_OK;
+    FVG_HIP(hipMemcpyAsync(dst, src, (size_t)n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    FVG_HIP(hipStreamSynchronize(ctx->stream));
+    return FVG_OK;
+}
+
+// ---- shared kernels (defined in vector_ops.hip) ------------------------------------
+// deterministic dot product: partials -> fixed-order final sum into *out_dev
+int launch_dot(fvg_ctx* ctx, int64_t n, const double* a, const double* b, double* out_dev);
+int launch_fill(fvg_ctx* ctx, int64_t n, double* x, double value);
+int launch_copy(fvg_ctx* ctx, int64_t n, const double* src, double* dst);
+int read_scalars(fvg_ctx* ctx, const double* dev, double* host, int count);
+
+// timing of launch groups (kind: 0 SpMV, 1 sweep); no-ops unless timing is enabled
+enum { kSpanSpmv = 0, kSpanSweep = 1 };
+int timing_begin(fvg_ctx* ctx);
+void timing_end(fvg_ctx* ctx, int token, int kind, int64_t launches);
+
+}  // namespace fvg

@@ -1,0 +1,479 @@
+// precond.hip -- IncompleteCholesky / IncompleteLowerUpper / Jacobi / identity on the device.
+//
+// Analysis (host, once per pattern): dependency levels of the strictly lower and
+// strictly upper triangles, rows sorted by level, and the slot maps that gather a
+// factor into level-ordered sweep matrices.
+//
+// Factorisation (device): one launch per lower level, one thread per row, in the
+// row-oriented (left-looking) form of each reference algorithm.  For every factor
+// entry the sequence of floating-point operations is the reference's own, so the
+// factors are bit-identical:
+//   IC(0)  incomplete_cholesky.rs:87-145 -- l_ij = (1/l_jj) * (a_ij - sum_m l_im l_jm)
+//          with the subtractions in ascending m, l_ii = sqrt(a_ii - sum_m l_im^2);
+//   ILU(0) incomplete_lower_upper.rs:200-263 -- IKJ row elimination, then the drop
+//          rule |v| <= 1e-6 |u_ii| (:255-258) applied as an explicit zero.
+// Sweeps (device): one launch per level; each row subtracts its entries in ascending
+// column order, as the reference's sequential loops do (incomplete_cholesky.rs:217-254,
+// incomplete_lower_upper.rs:291-328).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "precond.hpp"
+
+namespace fvg {
+
+// ---- kernels ------------------------------------------------------------------------
+
+__global__ void k_sym_check(int64_t nnz, const int32_t* __restrict__ tpos, const double* __restrict__ a,
+                            int32_t* __restrict__ flags) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const int32_t t = tpos[k];
+    if (t < 0) return;
+    if (!(fabs(a[k] - a[t]) < 1e-14)) atomicAdd(&flags[2], 1);   // ApproximateCmp for f64, linalg/mod.rs:73-77
+}
+
+__global__ void k_ic0_level(int64_t begin, int64_t end, const int32_t* __restrict__ rows,
+                            const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                            const int32_t* __restrict__ diag, const double* __restrict__ a, double* __restrict__ l,
+                            double* __restrict__ inv_diag, int32_t* __restrict__ flags) {
+    const int64_t p = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= end) return;
+    const int i = rows[p];
+    const int s = rp[i], d = diag[i];
+    for (int k = s; k < d; ++k) {
+        const int j = col[k];
+        double v = a[k];
+        int x = s, y = rp[j];
+        const int ye = diag[j];
+        while (x < k && y < ye) {
+            const int cx = col[x], cy = col[y];
+            if (cx == cy) {
+                v -= l[x] * l[y];
+                ++x;
+                ++y;
+            } else if (cx < cy) {
+                ++x;
+            } else {
+                ++y;
+            }
+        }
+        l[k] = inv_diag[j] * v;
+    }
+    double dd = a[d];
+    for (int k = s; k < d; ++k) dd -= l[k] * l[k];
+    if (!(dd > 0.0)) atomicAdd(&flags[1], 1);
+    const double lkk = sqrt(dd);
+    l[d] = lkk;
+    inv_diag[i] = 1.0 / lkk;
+}
+
+__global__ void k_ilu0_level(int64_t begin, int64_t end, const int32_t* __restrict__ rows,
+                             const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                             const int32_t* __restrict__ diag, const double* __restrict__ a, double* __restrict__ lu,
+                             int32_t* __restrict__ flags) {
+    const int64_t p = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= end) return;
+    const int i = rows[p];
+    const int s = rp[i], e = rp[i + 1], d = diag[i];
+    for (int k = s; k < e; ++k) lu[k] = a[k];
+    for (int k = s; k < d; ++k) {
+        const int kk = col[k];
+        const double ukk = lu[diag[kk]];
+        if (fabs(ukk) < 1e-16) atomicAdd(&flags[0], 1);
+        const double inv = 1.0 / ukk;
+        lu[k] = inv * lu[k];
+        const double lik = lu[k];
+        int x = k + 1, y = diag[kk] + 1;
+        const int ye = rp[kk + 1];
+        while (x < e && y < ye) {
+            const int cx = col[x], cy = col[y];
+            if (cx == cy) {
+                lu[x] -= lik * lu[y];
+                ++x;
+                ++y;
+            } else if (cx < cy) {
+                ++x;
+            } else {
+                ++y;
+            }
+        }
+    }
+    const double tol = fabs(lu[d]) * 1e-6;
+    for (int k = s; k < e; ++k)
+        if (k != d && !(fabs(lu[k]) > tol)) lu[k] = 0.0;
+}
+
+__global__ void k_inv_diag(int64_t n, const int32_t* __restrict__ diag, const double* __restrict__ v,
+                           double* __restrict__ inv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inv[i] = 1.0 / v[diag[i]];
+}
+
+__global__ void k_gather(int64_t nnz, const int32_t* __restrict__ src, const double* __restrict__ fac,
+                         double* __restrict__ dst) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnz) dst[k] = fac[src[k]];
+}
+
+__global__ void k_sweep_level(int64_t begin, int64_t end, const int32_t* __restrict__ rows,
+                              const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                              const double* __restrict__ val, const double* __restrict__ scale,
+                              const double* __restrict__ b, double* __restrict__ x,
+                              const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int64_t p = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= end) return;
+    const int i = rows[p];
+    double s = b[i];
+    const int e = ptr[p + 1];
+    for (int k = ptr[p]; k < e; ++k) s -= val[k] * x[col[k]];
+    x[i] = scale ? scale[i] * s : s;
+}
+
+// Jacobi (d = 1/a_ii, DiagonalPreconditioner mod.rs:60-66) or identity copy (d == nullptr, mod.rs:25-31)
+__global__ void k_scale(int64_t n, const double* __restrict__ d, const double* __restrict__ r,
+                        double* __restrict__ z, const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = d ? d[i] * r[i] : r[i];
+}
+
+// ---- host analysis ----------------------------------------------------------------------
+
+static void free_sweep(fvg_ctx* c, Sweep& s) {
+    dev_free_n(c, s.rows, s.n);
+    dev_free_n(c, s.ptr, s.n + 1);
+    dev_free_n(c, s.col, s.nnz);
+    dev_free_n(c, s.src, s.nnz);
+    dev_free_n(c, s.val, s.nnz);
+}
+
+// upper=false: entries j < i; upper=true: entries i < j < n.  src_map (optional)
+// maps a matrix slot to the factor slot holding that entry's value.
+static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
+                       bool upper, const std::vector<int32_t>* src_map, Sweep& sw) {
+    std::vector<int32_t> lev((size_t)n, 0);
+    int maxlev = 0;
+    auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
+    if (!upper) {
+        for (int64_t i = 0; i < n; ++i) {
+            int l = 0;
+            for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k)
+                if (take(i, col[(size_t)k])) l = std::max(l, lev[(size_t)col[(size_t)k]] + 1);
+            lev[(size_t)i] = l;
+            maxlev = std::max(maxlev, l);
+        }
+    } else {
+        for (int64_t i = n - 1; i >= 0; --i) {
+            int l = 0;
+            for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k)
+                if (take(i, col[(size_t)k])) l = std::max(l, lev[(size_t)col[(size_t)k]] + 1);
+            lev[(size_t)i] = l;
+            maxlev = std::max(maxlev, l);
+        }
+    }
+    sw.n = n;
+    sw.nlevels = maxlev + 1;
+    sw.lvl_ptr.assign((size_t)sw.nlevels + 1, 0);
+    for (int64_t i = 0; i < n; ++i) sw.lvl_ptr[(size_t)lev[(size_t)i] + 1]++;
+    for (int l = 0; l < sw.nlevels; ++l) sw.lvl_ptr[(size_t)l + 1] += sw.lvl_ptr[(size_t)l];
+    std::vector<int32_t> rows((size_t)n);
+    {
+        std::vector<int64_t> cur(sw.lvl_ptr.begin(), sw.lvl_ptr.end() - 1);
+        for (int64_t i = 0; i < n; ++i) rows[(size_t)cur[(size_t)lev[(size_t)i]]++] = (int32_t)i;   // ascending rows
+    }
+    std::vector<int32_t> ptr((size_t)n + 1, 0), ecol, esrc;
+    ecol.reserve(col.size() / 2 + 1);
+    esrc.reserve(col.size() / 2 + 1);
+    for (int64_t p = 0; p < n; ++p) {
+        const int32_t i = rows[(size_t)p];
+        for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
+            const int32_t j = col[(size_t)k];
+            if (!take(i, j)) continue;
+            ecol.push_back(j);
+            esrc.push_back(src_map ? (*src_map)[(size_t)k] : k);
+        }
+        ptr[(size_t)p + 1] = (int32_t)ecol.size();
+    }
+    sw.nnz = (int64_t)ecol.size();
+    int s;
+    if ((s = dev_alloc_n(c, n, &sw.rows)) || (s = dev_alloc_n(c, n + 1, &sw.ptr)) ||
+        (s = dev_alloc_n(c, sw.nnz, &sw.col)) || (s = dev_alloc_n(c, sw.nnz, &sw.src)) ||
+        (s = dev_alloc_n(c, sw.nnz, &sw.val)) || (s = upload(c, sw.rows, rows.data(), n)) ||
+        (s = upload(c, sw.ptr, ptr.data(), n + 1)) || (s = upload(c, sw.col, ecol.data(), sw.nnz)) ||
+        (s = upload(c, sw.src, esrc.data(), sw.nnz)))
+        return s;
+    return FVG_OK;
+}
+
+static void free_prec(fvg_ctx* c, fvg_prec* p) {
+    dev_free_n(c, p->d_rp, p->n + 1);
+    dev_free_n(c, p->d_col, p->nnz);
+    dev_free_n(c, p->d_diag, p->n);
+    dev_free_n(c, p->d_tpos, p->nnz);
+    dev_free_n(c, p->d_fac, p->nnz);
+    dev_free_n(c, p->d_inv_diag, p->n);
+    dev_free_n(c, p->d_tmp, p->n);
+    dev_free_n(c, p->d_flags, 4);
+    free_sweep(c, p->lower);
+    free_sweep(c, p->upper);
+}
+
+// pattern, diagonal slots and device copies shared by every kind
+static int setup_pattern(fvg_ctx* c, const fvg_csr* a, fvg_prec* p, bool need_transpose) {
+    const int64_t n = a->nrows;
+    p->n = n;
+    p->nnz = a->nnz;
+    p->h_rp = a->h_row_ptr;
+    p->h_col = a->h_col;
+    p->h_diag.assign((size_t)n, -1);
+    for (int64_t i = 0; i < n; ++i) {
+        auto b = p->h_col.begin() + p->h_rp[(size_t)i], e = p->h_col.begin() + p->h_rp[(size_t)i + 1];
+        auto it = std::lower_bound(b, e, (int32_t)i);
+        if (it == e || *it != (int32_t)i)
+            return fail(FVG_E_MISSING_ENTRY, "Sparse matrix index (" + std::to_string(i) + ", " + std::to_string(i) +
+                                                 ") not found");
+        p->h_diag[(size_t)i] = (int32_t)(it - p->h_col.begin());
+    }
+    if (need_transpose) {
+        p->h_tpos.assign((size_t)p->nnz, -1);
+        for (int64_t i = 0; i < n; ++i) {
+            for (int32_t k = p->h_rp[(size_t)i]; k < p->h_rp[(size_t)i + 1]; ++k) {
+                const int32_t j = p->h_col[(size_t)k];
+                if (j >= n) continue;
+                auto b = p->h_col.begin() + p->h_rp[(size_t)j], e = p->h_col.begin() + p->h_rp[(size_t)j + 1];
+                auto it = std::lower_bound(b, e, (int32_t)i);
+                if (it == e || *it != (int32_t)i)
+                    return fail(FVG_E_MISSING_ENTRY, "Sparse matrix index (" + std::to_string(j) + ", " +
+                                                         std::to_string(i) + ") not found");
+                p->h_tpos[(size_t)k] = (int32_t)(it - p->h_col.begin());
+            }
+        }
+    }
+    int s;
+    if ((s = dev_alloc_n(c, n + 1, &p->d_rp)) || (s = dev_alloc_n(c, p->nnz, &p->d_col)) ||
+        (s = dev_alloc_n(c, n, &p->d_diag)) || (s = dev_alloc_n(c, n, &p->d_inv_diag)) ||
+        (s = dev_alloc_n(c, 4, &p->d_flags)) || (s = upload(c, p->d_rp, p->h_rp.data(), n + 1)) ||
+        (s = upload(c, p->d_col, p->h_col.data(), p->nnz)) || (s = upload(c, p->d_diag, p->h_diag.data(), n)))
+        return s;
+    if (need_transpose) {
+        if ((s = dev_alloc_n(c, p->nnz, &p->d_tpos)) || (s = upload(c, p->d_tpos, p->h_tpos.data(), p->nnz))) return s;
+    }
+    return FVG_OK;
+}
+
+static int read_flags(fvg_ctx* c, const fvg_prec* p, int32_t* host4) {
+    FVG_HIP(hipMemcpyAsync(c->pinned, p->d_flags, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    FVG_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(host4, c->pinned, 4 * sizeof(int32_t));
+    return FVG_OK;
+}
+
+// numeric factorisation for the current values of a (pattern already analysed)
+static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a) {
+    FVG_HIP(hipMemsetAsync(p->d_flags, 0, 4 * sizeof(int32_t), c->stream));
+    const Sweep& lo = p->lower;
+    if (p->kind == kIC) {
+        hipLaunchKernelGGL(k_sym_check, dim3(grid_for(p->nnz)), dim3(kBlock), 0, c->stream, p->nnz, p->d_tpos, a->val,
+                           p->d_flags);
+        int32_t fl[4];
+        FVG_TRY(read_flags(c, p, fl));
+        if (fl[2]) return fail(FVG_E_NOT_SYMMETRIC, "IncompleteCholesky requires a symmetric matrix");
+        for (int l = 0; l < lo.nlevels; ++l) {
+            const int64_t b = lo.lvl_ptr[(size_t)l], e = lo.lvl_ptr[(size_t)l + 1];
+            hipLaunchKernelGGL(k_ic0_level, dim3(grid_for(e - b)), dim3(kBlock), 0, c->stream, b, e, lo.rows, p->d_rp,
+                               p->d_col, p->d_diag, a->val, p->d_fac, p->d_inv_diag, p->d_flags);
+        }
+    } else {
+        for (int l = 0; l < lo.nlevels; ++l) {
+            const int64_t b = lo.lvl_ptr[(size_t)l], e = lo.lvl_ptr[(size_t)l + 1];
+            hipLaunchKernelGGL(k_ilu0_level, dim3(grid_for(e - b)), dim3(kBlock), 0, c->stream, b, e, lo.rows,
+                               p->d_rp, p->d_col, p->d_diag, a->val, p->d_fac, p->d_flags);
+        }
+        hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_diag, p->d_fac,
+                           p->d_inv_diag);
+    }
+    FVG_HIP(hipGetLastError());
+    if (p->kind == kILU) {
+        int32_t fl[4];
+        FVG_TRY(read_flags(c, p, fl));
+        if (fl[0]) return fail(FVG_E_ZERO_PIVOT, "error, lu[k, k] too small");
+    }
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(p->lower.nnz)), dim3(kBlock), 0, c->stream, p->lower.nnz, p->lower.src,
+                       p->d_fac, p->lower.val);
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(p->upper.nnz)), dim3(kBlock), 0, c->stream, p->upper.nnz, p->upper.src,
+                       p->d_fac, p->upper.val);
+    FVG_HIP(hipGetLastError());
+    FVG_HIP(hipStreamSynchronize(c->stream));
+    return FVG_OK;
+}
+
+static int sweep(fvg_ctx* c, const Sweep& sw, const double* scale, const double* b, double* x,
+                 const int32_t* skip) {
+    const int tok = timing_begin(c);
+    for (int l = 0; l < sw.nlevels; ++l) {
+        const int64_t lb = sw.lvl_ptr[(size_t)l], le = sw.lvl_ptr[(size_t)l + 1];
+        hipLaunchKernelGGL(k_sweep_level, dim3(grid_for(le - lb)), dim3(kBlock), 0, c->stream, lb, le, sw.rows, sw.ptr,
+                           sw.col, sw.val, scale, b, x, skip);
+    }
+    timing_end(c, tok, kSpanSweep, sw.nlevels);
+    FVG_HIP(hipGetLastError());
+    return FVG_OK;
+}
+
+int apply_prec(fvg_ctx* c, const fvg_prec* p, const double* r, double* z, const int32_t* skip) {
+    switch (p->kind) {
+    case kIdentity:
+        hipLaunchKernelGGL(k_scale, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, nullptr, r, z, skip);
+        FVG_HIP(hipGetLastError());
+        return FVG_OK;
+    case kDiagonal:
+        hipLaunchKernelGGL(k_scale, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_inv_diag, r, z, skip);
+        FVG_HIP(hipGetLastError());
+        return FVG_OK;
+    case kIC:
+        FVG_TRY(sweep(c, p->lower, p->d_inv_diag, r, p->d_tmp, skip));
+        return sweep(c, p->upper, p->d_inv_diag, p->d_tmp, z, skip);
+    case kILU:
+        FVG_TRY(sweep(c, p->lower, nullptr, r, p->d_tmp, skip));
+        return sweep(c, p->upper, p->d_inv_diag, p->d_tmp, z, skip);
+    }
+    return fail(FVG_E_INVALID, "unknown preconditioner");
+}
+
+static int create_factored(fvg_ctx* c, const fvg_csr* a, int32_t level, int kind, fvg_prec** out) {
+    if (!c || !a || !out) return fail(FVG_E_INVALID, "null argument");
+    *out = nullptr;
+    if (level != 0)
+        return fail(FVG_E_UNSUPPORTED, "only level 0 incomplete factorisations are implemented on the device");
+    if (a->nrows <= 0) return fail(FVG_E_INVALID, "empty matrix");
+    auto* p = new fvg_prec();
+    p->kind = kind;
+    int s = setup_pattern(c, a, p, kind == kIC);
+    const int64_t n = a->nrows;
+    if (s == FVG_OK) s = dev_alloc_n(c, p->nnz, &p->d_fac);
+    if (s == FVG_OK) s = dev_alloc_n(c, n, &p->d_tmp);
+    if (s == FVG_OK) s = build_sweep(c, p->h_rp, p->h_col, n, false, nullptr, p->lower);
+    if (s == FVG_OK) s = build_sweep(c, p->h_rp, p->h_col, n, true, kind == kIC ? &p->h_tpos : nullptr, p->upper);
+    if (s == FVG_OK) s = launch_fill(c, p->nnz, p->d_fac, 0.0);
+    if (s == FVG_OK) s = factorize(c, p, a);
+    if (s != FVG_OK) {
+        (void)hipStreamSynchronize(c->stream);
+        free_prec(c, p);
+        delete p;
+        return s;
+    }
+    *out = p;
+    return FVG_OK;
+}
+
+}  // namespace fvg
+
+using namespace fvg;
+
+extern "C" {
+
+int fvg_identity_create(fvg_ctx* c, int64_t n, fvg_prec** out) {
+    if (!c || !out || n < 0) return fail(FVG_E_INVALID, "bad argument");
+    auto* p = new fvg_prec();
+    p->kind = kIdentity;
+    p->n = n;
+    *out = p;
+    return FVG_OK;
+}
+
+int fvg_diagonal_create(fvg_ctx* c, const fvg_csr* a, fvg_prec** out) {
+    if (!c || !a || !out) return fail(FVG_E_INVALID, "null argument");
+    *out = nullptr;
+    auto* p = new fvg_prec();
+    p->kind = kDiagonal;
+    int s = setup_pattern(c, a, p, false);
+    if (s == FVG_OK) {
+        hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_diag, a->val,
+                           p->d_inv_diag);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+            s = fail(FVG_E_HIP, "diagonal preconditioner launch failed");
+    }
+    if (s != FVG_OK) {
+        free_prec(c, p);
+        delete p;
+        return s;
+    }
+    *out = p;
+    return FVG_OK;
+}
+
+int fvg_ic_create(fvg_ctx* c, const fvg_csr* a, int32_t level, fvg_prec** out) {
+    return create_factored(c, a, level, kIC, out);
+}
+
+int fvg_ilu_create(fvg_ctx* c, const fvg_csr* a, int32_t level, fvg_prec** out) {
+    return create_factored(c, a, level, kILU, out);
+}
+
+int fvg_prec_refactor(fvg_ctx* c, fvg_prec* p, const fvg_csr* a) {
+    if (!c || !p || !a) return fail(FVG_E_INVALID, "null argument");
+    if (a->nrows != p->n || a->nnz != p->nnz || a->h_col != p->h_col)
+        return fail(FVG_E_INVALID, "matrix pattern differs from the analysed one");
+    if (p->kind == kIdentity) return FVG_OK;
+    if (p->kind == kDiagonal) {
+        hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_diag, a->val,
+                           p->d_inv_diag);
+        FVG_HIP(hipGetLastError());
+        return FVG_OK;
+    }
+    return factorize(c, p, a);
+}
+
+int fvg_prec_destroy(fvg_ctx* c, fvg_prec* p) {
+    if (!p) return FVG_OK;
+    if (c) {
+        (void)hipStreamSynchronize(c->stream);
+        free_prec(c, p);
+    }
+    delete p;
+    return FVG_OK;
+}
+
+int fvg_precondition(fvg_ctx* c, const fvg_prec* p, fvg_vec* solution, const fvg_vec* rhs) {
+    if (!c || !p || !solution || !rhs) return fail(FVG_E_INVALID, "null argument");
+    if (solution->n != p->n || rhs->n != p->n) return fail(FVG_E_INVALID, "vector length mismatch");
+    if (solution->d == rhs->d) return fail(FVG_E_INVALID, "solution and rhs must not alias");
+    FVG_TRY(apply_prec(c, p, rhs->d, solution->d, nullptr));
+    FVG_HIP(hipStreamSynchronize(c->stream));
+    return FVG_OK;
+}
+
+int fvg_prec_factor_download(fvg_ctx* c, const fvg_prec* p, double* values, double* inv_diag) {
+    if (!c || !p) return fail(FVG_E_INVALID, "null argument");
+    if (values) {
+        if (!p->d_fac) return fail(FVG_E_INVALID, "preconditioner has no factor");
+        FVG_TRY(download(c, values, p->d_fac, p->nnz));
+    }
+    if (inv_diag) {
+        if (!p->d_inv_diag) return fail(FVG_E_INVALID, "preconditioner has no diagonal");
+        FVG_TRY(download(c, inv_diag, p->d_inv_diag, p->n));
+    }
+    return FVG_OK;
+}
+
+int fvg_prec_levels(const fvg_prec* p, int32_t* lower_levels, int32_t* upper_levels) {
+    if (!p) return fail(FVG_E_INVALID, "null argument");
+    if (lower_levels) *lower_levels = p->lower.nlevels;
+    if (upper_levels) *upper_levels = p->upper.nlevels;
+    return FVG_OK;
+}
+
+int fvg_prec_nonpositive_pivots(fvg_ctx* c, const fvg_prec* p, int64_t* count) {
+    if (!c || !p || !count) return fail(FVG_E_INVALID, "null argument");
+    *count = 0;
+    if (p->kind != kIC) return FVG_OK;
+    int32_t fl[4];
+    FVG_TRY(read_flags(c, p, fl));
+    *count = fl[1];
+    return FVG_OK;
+}
+
+}  // extern "C"

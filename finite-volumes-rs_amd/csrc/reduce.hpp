@@ -1,0 +1,49 @@
+// reduce.hpp -- deterministic wave64 / workgroup reductions shared by the kernels.
+//
+// Every reduction here has a fixed summation tree for a given launch shape, so a
+// dot product of the same vectors gives the same bits on every run and does not
+// depend on what else is resident on the GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fvg {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;  // lane 0 holds the wave total
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Sum over a workgroup of NT threads; the total is valid in thread 0.
+template <int NT, class T>
+__device__ __forceinline__ T block_sum(T v, T* lds) {
+    static_assert(NT % 64 == 0, "wave64 multiple");
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    T s = T(0);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NT / 64; ++i) s += lds[i];
+    }
+    __syncthreads();
+    return s;
+}
+
+// Fixed-order sum of `count` partials (stride-NT per thread, then block tree).
+template <int NT>
+__device__ __forceinline__ double sum_partials(const double* __restrict__ p, int count, double* lds) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < count; i += NT) s += p[i];
+    return block_sum<NT>(s, lds);
+}
+
+}  // namespace fvg

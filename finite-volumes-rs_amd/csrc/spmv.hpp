@@ -1,0 +1,32 @@
+// spmv.hpp -- SpMV launch interface shared by the solvers (internal).
+#pragma once
+
+#include "internal.hpp"
+
+namespace fvg {
+
+// Optional work fused after y = A x, per row r:
+//   if b:        y_r = (y_r - b[r]) * -1      (residual r = b - A x, solvers' setup)
+//   copy1/copy2: also store y_r there
+//   partials slot 0: sum of dotw[r] * y_r     (if dotw)
+//   partials slot 1: sum of y_r * y_r         (if dot_self)
+//   partials slot 2: sum of dotw2[r] * y_r    (if dotw2)
+struct SpmvEpilogue {
+    const double* b = nullptr;
+    double* copy1 = nullptr;
+    double* copy2 = nullptr;
+    const double* dotw = nullptr;
+    const double* dotw2 = nullptr;
+    bool dot_self = false;
+    const int32_t* skip = nullptr;   // if set and *skip != 0 the launch does nothing
+};
+
+// Number of partial sums one fused SpMV writes per dot (one per workgroup).
+int spmv_blocks(int64_t nrows);
+// Ensures ctx->partials holds 4 slots of `count` partials each.
+int ensure_partials(fvg_ctx* ctx, int64_t count);
+// y = A x with the epilogue; partials go to ctx->partials[slot * stride + block].
+int launch_spmv(fvg_ctx* ctx, const fvg_csr* a, const double* x, double* y, const SpmvEpilogue& epi);
+int64_t partial_stride(const fvg_ctx* ctx);
+
+}  // namespace fvg

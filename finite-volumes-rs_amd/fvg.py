@@ -1,0 +1,662 @@
+"""Python host mirror of the finite-volumes assemble-and-solve API over libfvg.so.
+
+Names and argument meanings follow the crate:
+
+    mesh = Mesh.from_arrays(ctx, raw)                  # Mesh::new + add_* + compute
+    eq = -laplacian(Orthogonal(), k_f) - source(f)     # fvm::terms algebra
+    lhs, rhs = assemble(eq, bc, mesh)                  # fvm::assembly::assemble
+    ic = IncompleteCholesky.from_matrix(lhs, 0)        # preconditioners
+    info = conjugate_gradient(x, lhs, rhs, ic, LinearSolverOptions(...))
+
+Everything numeric runs in libfvg.so on the GPU; this module only marshals host
+arrays.  Importing it on a machine without the library or without a GPU raises:
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfvg.so")
+
+FVG_OK = 0
+STATUS = {1: "FVG_E_INVALID", 2: "FVG_E_HIP", 3: "FVG_E_BUDGET", 4: "FVG_E_ZERO_PIVOT", 5: "FVG_E_NOT_SYMMETRIC",
+          6: "FVG_E_MISSING_ENTRY", 7: "FVG_E_MESH", 8: "FVG_E_TIMEOUT", 9: "FVG_E_UNSUPPORTED"}
+
+OP_LAPLACIAN_ORTHOGONAL, OP_CONVECTION_UPWIND, OP_SOURCE, OP_LINEAR_SOURCE, OP_TIME_EULER, OP_DIVERGENCE = 1, 2, 3, 4, 5, 6
+OP_ADD, OP_SUB, OP_NEG = 16, 17, 18
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+
+class FvgError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class ZeroPivotError(FvgError):
+    """IncompleteLowerUpper hit |u_kk| < 1e-16 (the reference panics)."""
+
+
+class NotSymmetricError(FvgError):
+    """IncompleteCholesky on a nonsymmetric matrix (the reference asserts)."""
+
+
+class BudgetError(FvgError):
+    """An allocation would exceed the context's HBM budget."""
+
+
+_ERRORS = {3: BudgetError, 4: ZeroPivotError, 5: NotSymmetricError}
+
+
+class TermOp(C.Structure):
+    _fields_ = [("op", C.c_int32), ("arg0", C.c_int32), ("arg1", C.c_int32), ("arg2", C.c_int32),
+                ("scalar", C.c_double)]
+
+
+class SolverOptions(C.Structure):
+    _fields_ = [("relative_tolerance", C.c_double), ("absolute_tolerance", C.c_double),
+                ("max_iterations", C.c_int64), ("precision", C.c_int32), ("reserved", C.c_int32),
+                ("inner_tolerance", C.c_double)]
+
+
+class SolverInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int64), ("initial_residual", C.c_double), ("final_residual", C.c_double),
+                ("history_len", C.c_int64), ("restarts", C.c_int64), ("outer_iterations", C.c_int64),
+                ("true_residual", C.c_double)]
+
+
+class KernelStats(C.Structure):
+    _fields_ = [("spmv_ms", C.c_double), ("sweep_ms", C.c_double), ("spmv_launches", C.c_int64),
+                ("sweep_launches", C.c_int64)]
+
+
+_lib = None
+P = C.c_void_p
+
+
+def lib():
+    """Load libfvg.so; raises when it is missing (build it with __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FvgError(1, f"{LIB_PATH} is not built")
+    L = C.CDLL(LIB_PATH)
+    L.fvg_last_error.restype = C.c_char_p
+    L.fvg_status_string.restype = C.c_char_p
+    sig = {
+        "fvg_ctx_create": [C.c_int, C.c_uint64, C.POINTER(P)],
+        "fvg_ctx_destroy": [P],
+        "fvg_ctx_sync": [P],
+        "fvg_ctx_bytes_in_use": [P, C.POINTER(C.c_uint64)],
+        "fvg_ctx_stream": [P, C.POINTER(P)],
+        "fvg_ctx_enable_timing": [P, C.c_int],
+        "fvg_ctx_kernel_stats": [P, C.POINTER(KernelStats), C.c_int],
+        "fvg_vec_create": [P, C.c_int64, C.POINTER(P)],
+        "fvg_vec_wrap": [P, C.c_int64, P, C.POINTER(P)],
+        "fvg_vec_destroy": [P, P],
+        "fvg_vec_len": [P, C.POINTER(C.c_int64)],
+        "fvg_vec_device_ptr": [P, C.POINTER(P)],
+        "fvg_vec_upload": [P, P, P],
+        "fvg_vec_download": [P, P, P],
+        "fvg_vec_dot": [P, P, P, C.POINTER(C.c_double)],
+        "fvg_mesh_create": [P, C.c_int, C.c_int64, P, C.c_int64, P, P, P, C.c_int64, P, P, C.POINTER(P)],
+        "fvg_mesh_destroy": [P, P],
+        "fvg_mesh_sizes": [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                           C.POINTER(C.c_int64), C.POINTER(C.c_int32)],
+        "fvg_mesh_geometry": [P, P, P, P, P, P, P, P],
+        "fvg_csr_from_mesh": [P, P, C.POINTER(P)],
+        "fvg_csr_create": [P, C.c_int64, C.c_int64, C.c_int64, P, P, P, C.POINTER(P)],
+        "fvg_csr_destroy": [P, P],
+        "fvg_csr_sizes": [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+        "fvg_csr_download": [P, P, P, P, P],
+        "fvg_csr_upload_values": [P, P, P],
+        "fvg_spmv": [P, P, P, P],
+        "fvg_assemble": [P, P, P, C.c_int32, P, C.c_int32, P, C.c_int32, P, P, P, P],
+        "fvg_identity_create": [P, C.c_int64, C.POINTER(P)],
+        "fvg_diagonal_create": [P, P, C.POINTER(P)],
+        "fvg_ic_create": [P, P, C.c_int32, C.POINTER(P)],
+        "fvg_ilu_create": [P, P, C.c_int32, C.POINTER(P)],
+        "fvg_prec_refactor": [P, P, P],
+        "fvg_prec_destroy": [P, P],
+        "fvg_precondition": [P, P, P, P],
+        "fvg_prec_factor_download": [P, P, P, P],
+        "fvg_prec_levels": [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+        "fvg_prec_nonpositive_pivots": [P, P, C.POINTER(C.c_int64)],
+        "fvg_cg": [P, P, P, P, P, C.POINTER(SolverOptions), C.POINTER(SolverInfo), P, C.c_int64],
+        "fvg_bicgstab": [P, P, P, P, P, C.POINTER(SolverOptions), C.POINTER(SolverInfo), P, C.c_int64],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != FVG_OK:
+        msg = lib().fvg_last_error().decode()
+        raise _ERRORS.get(status, FvgError)(status, msg)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+# ---------------------------------------------------------------------------------
+# context, vectors
+# ---------------------------------------------------------------------------------
+
+class Context:
+    """One device, one HIP stream, an optional HBM budget in bytes (0 = unlimited)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        h = P()
+        check(lib().fvg_ctx_create(device, int(hbm_budget), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fvg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def sync(self):
+        check(lib().fvg_ctx_sync(self.h))
+
+    def bytes_in_use(self):
+        v = C.c_uint64()
+        check(lib().fvg_ctx_bytes_in_use(self.h, C.byref(v)))
+        return v.value
+
+    def enable_timing(self, on=True):
+        check(lib().fvg_ctx_enable_timing(self.h, 1 if on else 0))
+
+    def kernel_stats(self, reset=True):
+        s = KernelStats()
+        check(lib().fvg_ctx_kernel_stats(self.h, C.byref(s), 1 if reset else 0))
+        return dict(spmv_ms=s.spmv_ms, sweep_ms=s.sweep_ms, spmv_launches=s.spmv_launches,
+                    sweep_launches=s.sweep_launches)
+
+
+class DistributedVector:
+    """DistributedVector<f64> (src/linalg/dvector.rs:10-179), device resident."""
+
+    def __init__(self, ctx, n):
+        self.ctx = ctx
+        self.n = int(n)
+        h = P()
+        check(lib().fvg_vec_create(ctx.h, self.n, C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_size(cls, ctx, n):
+        return cls(ctx, n)
+
+    @classmethod
+    def from_data(cls, ctx, data):
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        v = cls(ctx, data.shape[0])
+        v.upload(data)
+        return v
+
+    def upload(self, data):
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        assert data.shape == (self.n,)
+        check(lib().fvg_vec_upload(self.ctx.h, self.h, _ptr(data)))
+
+    def numpy(self):
+        out = np.empty(self.n)
+        check(lib().fvg_vec_download(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    def dot(self, other):
+        r = C.c_double()
+        check(lib().fvg_vec_dot(self.ctx.h, self.h, other.h, C.byref(r)))
+        return r.value
+
+    def __len__(self):
+        return self.n
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().fvg_vec_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+# ---------------------------------------------------------------------------------
+# mesh
+# ---------------------------------------------------------------------------------
+
+class Mesh:
+    """Mesh<DIM> (src/core/mesh/mod.rs:200-234) built from raw topology and computed."""
+
+    def __init__(self, ctx, raw):
+        self.ctx = ctx
+        self.dim = int(raw["dim"])
+        nodes = np.ascontiguousarray(raw["nodes"], dtype=np.float64)
+        fptr = np.ascontiguousarray(raw["face_ptr"], dtype=np.int64)
+        fn = np.ascontiguousarray(raw["face_nodes"], dtype=np.int32)
+        fp = np.ascontiguousarray(raw["face_patch"], dtype=np.int32)
+        cptr = np.ascontiguousarray(raw["cell_ptr"], dtype=np.int64)
+        cf = np.ascontiguousarray(raw["cell_faces"], dtype=np.int32)
+        h = P()
+        check(lib().fvg_mesh_create(ctx.h, self.dim, nodes.shape[0], _ptr(nodes), fptr.shape[0] - 1, _ptr(fptr),
+                                    _ptr(fn), _ptr(fp), cptr.shape[0] - 1, _ptr(cptr), _ptr(cf), C.byref(h)))
+        self.h = h
+        nc, nf, nb, nnz, ncol = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+        check(lib().fvg_mesh_sizes(h, C.byref(nc), C.byref(nf), C.byref(nb), C.byref(nnz), C.byref(ncol)))
+        self.n_cells, self.n_faces, self.n_boundary_faces, self.nnz, self.n_colors = (
+            nc.value, nf.value, nb.value, nnz.value, ncol.value)
+        self.face_patch = fp
+        self._geom = None
+
+    @classmethod
+    def from_arrays(cls, ctx, raw):
+        return cls(ctx, raw)
+
+    def geometry(self):
+        """face_area, face_center, face_normal, owner, neighbor, cell_volume, cell_center."""
+        if self._geom is None:
+            F, N, D = self.n_faces, self.n_cells, self.dim
+            g = dict(face_area=np.empty(F), face_center=np.empty((F, D)), face_normal=np.empty((F, D)),
+                     owner=np.empty(F, np.int32), neighbor=np.empty(F, np.int32), cell_volume=np.empty(N),
+                     cell_center=np.empty((N, D)))
+            check(lib().fvg_mesh_geometry(self.h, *[_ptr(g[k]) for k in (
+                "face_area", "face_center", "face_normal", "owner", "neighbor", "cell_volume", "cell_center")]))
+            self._geom = g
+        return self._geom
+
+    def boundary_faces(self):
+        return np.nonzero(self.face_patch >= 0)[0]
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().fvg_mesh_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+# ---------------------------------------------------------------------------------
+# matrix
+# ---------------------------------------------------------------------------------
+
+class DistributedMatrix:
+    """DistributedMatrix<f64> in CSR (src/linalg/dmatrix.rs:14-301), device resident."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+        nr, nc, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().fvg_csr_sizes(handle, C.byref(nr), C.byref(nc), C.byref(nnz)))
+        self.nrows, self.ncols, self.nnz = nr.value, nc.value, nnz.value
+
+    @classmethod
+    def from_mesh(cls, ctx, mesh):
+        """from_cut_sparsity(mesh.cell_to_cell_sparsity(), n_cells) (dmatrix.rs:38-50)."""
+        h = P()
+        check(lib().fvg_csr_from_mesh(ctx.h, mesh.h, C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_csr(cls, ctx, row_ptr, col, values=None, ncols=None):
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col, dtype=np.int32)
+        va = None if values is None else np.ascontiguousarray(values, dtype=np.float64)
+        n = rp.shape[0] - 1
+        ncols = n if ncols is None else ncols
+        h = P()
+        check(lib().fvg_csr_create(ctx.h, n, ncols, ci.shape[0], _ptr(rp), _ptr(ci), _ptr(va), C.byref(h)))
+        return cls(ctx, h)
+
+    def csr(self):
+        rp = np.empty(self.nrows + 1, np.int32)
+        ci = np.empty(self.nnz, np.int32)
+        va = np.empty(self.nnz)
+        check(lib().fvg_csr_download(self.ctx.h, self.h, _ptr(rp), _ptr(ci), _ptr(va)))
+        return rp, ci, va
+
+    def set_values(self, values):
+        va = np.ascontiguousarray(values, dtype=np.float64)
+        check(lib().fvg_csr_upload_values(self.ctx.h, self.h, _ptr(va)))
+
+    def imul(self, result, rhs):
+        """result = A rhs (dmatrix.rs:136-146)."""
+        check(lib().fvg_spmv(self.ctx.h, self.h, rhs.h, result.h))
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().fvg_csr_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+# ---------------------------------------------------------------------------------
+# terms (src/fvm/terms) -> postfix program
+# ---------------------------------------------------------------------------------
+
+class Orthogonal:
+    """facengrad::Orthogonal (src/fvm/schemes/facengrad/orthogonal.rs)."""
+
+
+class Upwind:
+    """faceinterp::Upwind(&flux) (src/fvm/schemes/faceinterp/upwind.rs)."""
+
+    def __init__(self, flux):
+        self.flux = np.ascontiguousarray(flux, dtype=np.float64)
+
+
+class Euler:
+    """time::Euler (src/fvm/schemes/time/euler.rs)."""
+
+    def __init__(self, previous, dt, density=None, density_previous=None):
+        self.previous = np.ascontiguousarray(previous, dtype=np.float64)
+        self.dt = float(dt)
+        self.density = density
+        self.density_previous = density_previous
+
+
+class Term:
+    """An expression tree of reference terms; +, - and unary - build TermsBinaryOp / OpNeg."""
+
+    def __init__(self, kind, **kw):
+        self.kind = kind
+        self.kw = kw
+
+    def __add__(self, other):
+        return Term("add", a=self, b=other)
+
+    def __sub__(self, other):
+        return Term("sub", a=self, b=other)
+
+    def __neg__(self):
+        return Term("neg", a=self)
+
+    def compile(self):
+        """Returns (ops, face_fields, cell_fields) with fields deduplicated by identity."""
+        ops, ff, cf = [], [], []
+
+        def fidx(lst, arr):
+            for i, a in enumerate(lst):
+                if a is arr:
+                    return i
+            lst.append(arr)
+            return len(lst) - 1
+
+        def walk(t):
+            k = t.kind
+            if k in ("add", "sub"):
+                walk(t.kw["a"])
+                walk(t.kw["b"])
+                ops.append((OP_ADD if k == "add" else OP_SUB, 0, 0, 0, 0.0))
+            elif k == "neg":
+                walk(t.kw["a"])
+                ops.append((OP_NEG, 0, 0, 0, 0.0))
+            elif k == "laplacian":
+                ops.append((OP_LAPLACIAN_ORTHOGONAL, fidx(ff, t.kw["k"]), 0, 0, 0.0))
+            elif k == "convection":
+                ops.append((OP_CONVECTION_UPWIND, fidx(ff, t.kw["flux"]), fidx(ff, t.kw["upwind"]), 0, 0.0))
+            elif k == "divergence":
+                ops.append((OP_DIVERGENCE, fidx(ff, t.kw["flux"]), 0, 0, 0.0))
+            elif k == "source":
+                ops.append((OP_SOURCE, fidx(cf, t.kw["f"]), 0, 0, 0.0))
+            elif k == "linear_source":
+                ops.append((OP_LINEAR_SOURCE, fidx(cf, t.kw["c"]), 0, 0, 0.0))
+            elif k == "time":
+                e = t.kw["scheme"]
+                a1 = a2 = -1
+                if e.density is not None:
+                    a1 = fidx(cf, np.ascontiguousarray(e.density, dtype=np.float64))
+                    a2 = fidx(cf, np.ascontiguousarray(e.density_previous, dtype=np.float64))
+                ops.append((OP_TIME_EULER, fidx(cf, e.previous), a1, a2, e.dt))
+            else:
+                raise ValueError(k)
+
+        walk(self)
+        return ops, ff, cf
+
+
+def laplacian(scheme, diffusivity):
+    """laplacian(scheme, &k_f) (src/fvm/terms/laplacian.rs:31-46); Orthogonal scheme."""
+    if not isinstance(scheme, Orthogonal):
+        raise FvgError(9, "only the Orthogonal face-normal-gradient scheme is implemented")
+    return Term("laplacian", k=np.ascontiguousarray(diffusivity, dtype=np.float64))
+
+
+def convection(scheme, flux):
+    """convection(Upwind(&w), &flux) (src/fvm/terms/convection.rs:32-45)."""
+    if not isinstance(scheme, Upwind):
+        raise FvgError(9, "only the Upwind face-interpolation scheme is implemented")
+    return Term("convection", flux=np.ascontiguousarray(flux, dtype=np.float64), upwind=scheme.flux)
+
+
+def divergence(flux):
+    return Term("divergence", flux=np.ascontiguousarray(flux, dtype=np.float64))
+
+
+def source(f):
+    """source(&f) (src/fvm/terms/source.rs:30-39)."""
+    return Term("source", f=np.ascontiguousarray(f, dtype=np.float64))
+
+
+def linear_source(c):
+    return Term("linear_source", c=np.ascontiguousarray(c, dtype=np.float64))
+
+
+def time(scheme):
+    return Term("time", scheme=scheme)
+
+
+class FaceConstraints:
+    """Boundary closure per boundary face: value_j = bndi * value_i + bnde (src/fvm/bcs/mod.rs)."""
+
+    def __init__(self, bndi, bnde):
+        self.bndi = np.ascontiguousarray(bndi, dtype=np.float64)
+        self.bnde = np.ascontiguousarray(bnde, dtype=np.float64)
+
+    def as_bc(self):
+        return self
+
+
+def dirichlet(values):
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    return FaceConstraints(np.zeros_like(v), v)
+
+
+def assemble(equation, boundary_condition, mesh, lhs=None, rhs=None):
+    """fvm::assembly::assemble (src/fvm/assembly.rs:11-79) on the GPU.
+
+    boundary_condition: FaceConstraints, a (bndi, bnde) pair of arrays over the
+    boundary faces in face order, or a callable taking the mesh geometry dict and
+    the boundary face ids and returning such a pair.  lhs/rhs may be passed to reuse
+    device storage from an earlier assembly on the same mesh.
+    """
+    ops, ff, cf = equation.compile()
+    bfaces = mesh.boundary_faces()
+    bc = boundary_condition
+    if callable(bc) and not isinstance(bc, FaceConstraints):
+        bc = bc(mesh.geometry(), bfaces)
+    if not isinstance(bc, FaceConstraints):
+        bc = FaceConstraints(*bc)
+    return assemble_program(mesh, ops, ff, cf, bc.bndi, bc.bnde, lhs, rhs)
+
+
+def assemble_program(mesh, ops, face_fields, cell_fields, bndi, bnde, lhs=None, rhs=None):
+    """assemble() with the equation already in postfix form: ops is a list of
+    (opcode, arg0, arg1, arg2, scalar) over the given face and cell field arrays."""
+    ctx = mesh.ctx
+    nb = mesh.n_boundary_faces
+    bndi = np.ascontiguousarray(bndi, dtype=np.float64)
+    bnde = np.ascontiguousarray(bnde, dtype=np.float64)
+    if bndi.shape != (nb,) or bnde.shape != (nb,):
+        raise FvgError(1, "boundary closure arrays must have one entry per boundary face")
+    fvecs = [DistributedVector.from_data(ctx, a) for a in face_fields]
+    cvecs = [DistributedVector.from_data(ctx, a) for a in cell_fields]
+    bi = DistributedVector.from_data(ctx, bndi) if nb else None
+    be = DistributedVector.from_data(ctx, bnde) if nb else None
+    if lhs is None:
+        lhs = DistributedMatrix.from_mesh(ctx, mesh)
+    if rhs is None:
+        rhs = DistributedVector.from_size(ctx, mesh.n_cells)
+    prog = (TermOp * len(ops))(*[TermOp(*o) for o in ops])
+    fv = (P * max(1, len(fvecs)))(*[v.h for v in fvecs])
+    cv = (P * max(1, len(cvecs)))(*[v.h for v in cvecs])
+    check(lib().fvg_assemble(ctx.h, mesh.h, C.cast(prog, P), len(ops), C.cast(fv, P), len(fvecs), C.cast(cv, P),
+                             len(cvecs), bi.h if bi else None, be.h if be else None, lhs.h, rhs.h))
+    ctx.sync()
+    return lhs, rhs
+
+
+# ---------------------------------------------------------------------------------
+# preconditioners
+# ---------------------------------------------------------------------------------
+
+class Preconditioner:
+    def __init__(self, ctx, handle, n):
+        self.ctx = ctx
+        self.h = handle
+        self.n = n
+
+    def precondition(self, solution, rhs):
+        check(lib().fvg_precondition(self.ctx.h, self.h, solution.h, rhs.h))
+
+    def refactor(self, matrix):
+        check(lib().fvg_prec_refactor(self.ctx.h, self.h, matrix.h))
+        self.ctx.sync()
+
+    def factor_values(self):
+        v = np.empty(self._nnz)
+        d = np.empty(self.n)
+        check(lib().fvg_prec_factor_download(self.ctx.h, self.h, _ptr(v), _ptr(d)))
+        return v, d
+
+    def levels(self):
+        lo, up = C.c_int32(), C.c_int32()
+        check(lib().fvg_prec_levels(self.h, C.byref(lo), C.byref(up)))
+        return lo.value, up.value
+
+    def nonpositive_pivots(self):
+        c = C.c_int64()
+        check(lib().fvg_prec_nonpositive_pivots(self.ctx.h, self.h, C.byref(c)))
+        return c.value
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().fvg_prec_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class IdentityPreconditioner(Preconditioner):
+    @classmethod
+    def new(cls, ctx, n):
+        h = P()
+        check(lib().fvg_identity_create(ctx.h, n, C.byref(h)))
+        return cls(ctx, h, n)
+
+
+class DiagonalPreconditioner(Preconditioner):
+    @classmethod
+    def from_matrix(cls, matrix):
+        h = P()
+        check(lib().fvg_diagonal_create(matrix.ctx.h, matrix.h, C.byref(h)))
+        return cls(matrix.ctx, h, matrix.nrows)
+
+
+class IncompleteCholesky(Preconditioner):
+    """IncompleteCholesky::from_matrix(&A, level) (incomplete_cholesky.rs:205-211)."""
+
+    @classmethod
+    def from_matrix(cls, matrix, level=0):
+        h = P()
+        check(lib().fvg_ic_create(matrix.ctx.h, matrix.h, int(level), C.byref(h)))
+        p = cls(matrix.ctx, h, matrix.nrows)
+        p._nnz = matrix.nnz
+        return p
+
+
+class IncompleteLowerUpper(Preconditioner):
+    """IncompleteLowerUpper::from_matrix(&A, level) (incomplete_lower_upper.rs:266-282)."""
+
+    @classmethod
+    def from_matrix(cls, matrix, level=0):
+        h = P()
+        check(lib().fvg_ilu_create(matrix.ctx.h, matrix.h, int(level), C.byref(h)))
+        p = cls(matrix.ctx, h, matrix.nrows)
+        p._nnz = matrix.nnz
+        return p
+
+
+# ---------------------------------------------------------------------------------
+# solvers
+# ---------------------------------------------------------------------------------
+
+@dataclass
+class LinearSolverOptions:
+    """LinearSolverOptions (src/linalg/solvers/mod.rs:20-36)."""
+    relative_tolerance: float = 1e-6
+    absolute_tolerance: float = 1e-6
+    max_iterations: int = 1000
+    precision: int = PRECISION_FP64
+    inner_tolerance: float = 0.0
+
+    def c(self):
+        return SolverOptions(self.relative_tolerance, self.absolute_tolerance, self.max_iterations, self.precision,
+                             0, self.inner_tolerance)
+
+
+@dataclass
+class LinearSolverInfo:
+    """LinearSolverInfo (src/linalg/solvers/mod.rs:11-17)."""
+    solver_identifier: str
+    iterations: int
+    initial_residual: float
+    final_residual: float
+    history: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    restarts: int = 0
+    outer_iterations: int = 0
+    true_residual: float = 0.0
+
+    def __str__(self):
+        return (f"(solver={self.solver_identifier}, iterations={self.iterations}, "
+                f"initial_residual={self.initial_residual:.3e}, final_residual={self.final_residual:.3e})")
+
+
+def _solve(fn, name, solution, lhs, rhs, preconditioner, options):
+    opts = options.c()
+    info = SolverInfo()
+    cap = max(2, options.max_iterations + 2)
+    hist = np.zeros(cap)
+    check(fn(lhs.ctx.h, lhs.h, preconditioner.h, rhs.h, solution.h, C.byref(opts), C.byref(info), _ptr(hist), cap))
+    hl = min(info.history_len, cap)
+    return LinearSolverInfo(name, info.iterations, info.initial_residual, info.final_residual, hist[:hl].copy(),
+                            info.restarts, info.outer_iterations, info.true_residual)
+
+
+def conjugate_gradient(solution, lhs, rhs, preconditioner, options=None):
+    """conjugate_gradient (src/linalg/solvers/conjugate_gradient.rs:12-112)."""
+    return _solve(lib().fvg_cg, "ConjugateGradient", solution, lhs, rhs, preconditioner,
+                  options or LinearSolverOptions())
+
+
+def bi_conjugate_gradient_stab(solution, lhs, rhs, preconditioner, options=None):
+    """bi_conjugate_gradient_stab (src/linalg/solvers/bi_conjugate_gradient_stab.rs:13-179)."""
+    return _solve(lib().fvg_bicgstab, "BiConjugateGradientStab", solution, lhs, rhs, preconditioner,
+                  options or LinearSolverOptions())

@@ -1,0 +1,237 @@
+/*
+ * fvg.h -- C ABI of the MI355X assemble-and-solve library (libfvg.so).
+ *
+ * The library replaces, behind the same call shapes, the hot path of the
+ * finite-volumes crate: cell-centred face-flux assembly into CSR and the
+ * preconditioned Krylov solves.  Each entry point names the reference item it
+ * stands for (paths relative to the crate root).  Everything is plain C: POD
+ * structs, raw pointers and sizes; no exceptions cross the boundary.
+ *
+ * Conventions
+ *   - Every function returns an int status (FVG_OK == 0).  fvg_last_error()
+ *     returns a message for the most recent failure on the calling thread.
+ *   - Handles are owned by the library until the matching *_destroy call.
+ *   - Host pointers passed to *_create / *_upload are read during the call
+ *     and not retained.
+ *   - A context owns one HIP stream on one device.  It is not thread safe:
+ *     use one context per host thread.  All work on a context is ordered.
+ *   - Indices are int32 (rows, columns, faces, cells < 2^31).
+ */
+#ifndef FVG_H
+#define FVG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    FVG_OK = 0,
+    FVG_E_INVALID = 1,        /* bad argument, shape or handle */
+    FVG_E_HIP = 2,            /* HIP runtime failure */
+    FVG_E_BUDGET = 3,         /* allocation would exceed the context's HBM budget */
+    FVG_E_ZERO_PIVOT = 4,     /* ILU |u_kk| < 1e-16; the reference panics
+                                 (src/linalg/preconditioners/incomplete_lower_upper.rs:234-236) */
+    FVG_E_NOT_SYMMETRIC = 5,  /* IC on a matrix failing |a_ij - a_ji| < 1e-14; the reference asserts
+                                 (src/linalg/preconditioners/incomplete_cholesky.rs:153,
+                                  src/linalg/dmatrix.rs:76-88) */
+    FVG_E_MISSING_ENTRY = 6,  /* a needed CSR slot is absent; the reference panics
+                                 (src/linalg/dmatrix.rs:262-286) */
+    FVG_E_MESH = 7,           /* invalid mesh topology (src/core/mesh/compute.rs:15-31) */
+    FVG_E_TIMEOUT = 8,        /* a device-side wait exceeded its bounded spin limit */
+    FVG_E_UNSUPPORTED = 9     /* valid request outside what this build implements */
+};
+
+const char* fvg_last_error(void);
+const char* fvg_status_string(int status);
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct fvg_ctx fvg_ctx;
+
+/* hbm_budget_bytes == 0 means unlimited.  Allocations made through the
+ * context are counted against the budget; exceeding it returns FVG_E_BUDGET. */
+int fvg_ctx_create(int device, uint64_t hbm_budget_bytes, fvg_ctx** out);
+int fvg_ctx_destroy(fvg_ctx* ctx);
+int fvg_ctx_sync(fvg_ctx* ctx);
+int fvg_ctx_bytes_in_use(const fvg_ctx* ctx, uint64_t* bytes);
+/* The hipStream_t of the context, as void*, for callers that interleave work. */
+int fvg_ctx_stream(const fvg_ctx* ctx, void** stream);
+
+/* ---- dense vectors: DistributedVector<f64> (src/linalg/dvector.rs:10-179) ---- */
+typedef struct fvg_vec fvg_vec;
+
+/* from_size (dvector.rs:19-21): zero-filled device vector of length n. */
+int fvg_vec_create(fvg_ctx* ctx, int64_t n, fvg_vec** out);
+/* Non-owning view of caller-owned device memory (n doubles). */
+int fvg_vec_wrap(fvg_ctx* ctx, int64_t n, double* device_ptr, fvg_vec** out);
+int fvg_vec_destroy(fvg_ctx* ctx, fvg_vec* v);
+int fvg_vec_len(const fvg_vec* v, int64_t* n);
+int fvg_vec_device_ptr(const fvg_vec* v, double** device_ptr);
+int fvg_vec_upload(fvg_ctx* ctx, fvg_vec* v, const double* host);     /* from_data (dvector.rs:23-25) */
+int fvg_vec_download(fvg_ctx* ctx, const fvg_vec* v, double* host);
+/* DistributedVector::dot (dvector.rs:63-71); deterministic fixed-order tree sum. */
+int fvg_vec_dot(fvg_ctx* ctx, const fvg_vec* a, const fvg_vec* b, double* out);
+
+/* ---- mesh: Mesh<DIM> built with add_node / add_face / add_cell and
+ *      Mesh::compute (src/core/mesh/mod.rs:444-527, src/core/mesh/compute.rs:12-261).
+ *      Geometry is computed on the host, bit-for-bit as the reference does,
+ *      including its face-normal scaling in 3D (compute.rs:305) and the cell
+ *      volume that follows from it (compute.rs:359-361).  Face data, the CSR
+ *      pattern, per-face CSR slots and the face colouring used by assembly are
+ *      then resident on the device. ------------------------------------------ */
+typedef struct fvg_mesh fvg_mesh;
+
+/* face_patch[f] >= 0 marks a boundary face of that patch, -1 an interior face.
+ * Cells name their faces; the first cell naming a face owns it, the second is
+ * its neighbour (mod.rs:511-525). */
+int fvg_mesh_create(fvg_ctx* ctx, int dim, int64_t n_nodes, const double* nodes, int64_t n_faces,
+                    const int64_t* face_ptr, const int32_t* face_nodes, const int32_t* face_patch,
+                    int64_t n_cells, const int64_t* cell_ptr, const int32_t* cell_faces, fvg_mesh** out);
+int fvg_mesh_destroy(fvg_ctx* ctx, fvg_mesh* mesh);
+/* Sizes: cells, faces, boundary faces, nnz of the cell-to-cell CSR pattern, colours. */
+int fvg_mesh_sizes(const fvg_mesh* mesh, int64_t* n_cells, int64_t* n_faces, int64_t* n_boundary_faces,
+                   int64_t* nnz, int32_t* n_colors);
+/* Host copies of Mesh::compute results; any output may be NULL.
+ * face_center/face_normal/cell_center are (count x dim) row-major.
+ * neighbor[f] is -1 on boundary faces (FaceRef::owner/neighbor mod.rs:339-351). */
+int fvg_mesh_geometry(const fvg_mesh* mesh, double* face_area, double* face_center, double* face_normal,
+                      int32_t* owner, int32_t* neighbor, double* cell_volume, double* cell_center);
+
+/* ---- sparse matrix: DistributedMatrix<f64> in CSR (src/linalg/dmatrix.rs:14-301) ---- */
+typedef struct fvg_csr fvg_csr;
+
+/* DistributedMatrix::from_cut_sparsity(mesh.cell_to_cell_sparsity(), n_cells)
+ * (dmatrix.rs:38-50): sorted pattern, zero values. */
+int fvg_csr_from_mesh(fvg_ctx* ctx, const fvg_mesh* mesh, fvg_csr** out);
+/* General CSR (columns sorted per row).  values may be NULL (zeros). */
+int fvg_csr_create(fvg_ctx* ctx, int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* row_ptr,
+                   const int32_t* col_idx, const double* values, fvg_csr** out);
+int fvg_csr_destroy(fvg_ctx* ctx, fvg_csr* a);
+int fvg_csr_sizes(const fvg_csr* a, int64_t* nrows, int64_t* ncols, int64_t* nnz);
+int fvg_csr_download(fvg_ctx* ctx, const fvg_csr* a, int32_t* row_ptr, int32_t* col_idx, double* values);
+int fvg_csr_upload_values(fvg_ctx* ctx, fvg_csr* a, const double* values);
+/* DistributedMatrix::imul (dmatrix.rs:136-146): y = A x, each row summed in
+ * ascending column order, bit-identical to the sequential loop. */
+int fvg_spmv(fvg_ctx* ctx, const fvg_csr* a, const fvg_vec* x, fvg_vec* y);
+
+/* ---- assembly: fvm::assembly::assemble (src/fvm/assembly.rs:11-79) -------------
+ * The equation (an `impl Term` expression built from laplacian, convection,
+ * source, ... and + - neg, src/fvm/terms/operations.rs) is passed as a postfix
+ * program.  Field arguments index the face_fields / cell_fields arrays. */
+enum {
+    FVG_OP_LAPLACIAN_ORTHOGONAL = 1,  /* laplacian(Orthogonal, k_f): terms/laplacian.rs:35-45,
+                                         schemes/facengrad/orthogonal.rs:26-44.  arg0 = face field k */
+    FVG_OP_CONVECTION_UPWIND = 2,     /* convection(Upwind(&w), &flux): terms/convection.rs:36-44,
+                                         schemes/faceinterp/upwind.rs:26-33.  arg0 = face field flux,
+                                         arg1 = face field w deciding the upwind side */
+    FVG_OP_SOURCE = 3,                /* source(&f): terms/source.rs:34-39.  arg0 = cell field f */
+    FVG_OP_LINEAR_SOURCE = 4,         /* linear_source(&c): terms/source.rs:72-77.  arg0 = cell field c */
+    FVG_OP_TIME_EULER = 5,            /* time(Euler): terms/time.rs:31-34, schemes/time/euler.rs:27-37.
+                                         arg0 = cell field previous, scalar = dt, arg1/arg2 = cell
+                                         fields density/density_previous or -1 */
+    FVG_OP_DIVERGENCE = 6,            /* divergence(&flux): terms/divergence.rs:36-45.  arg0 = face field */
+    FVG_OP_ADD = 16,                  /* a + b (operations.rs:86-120) */
+    FVG_OP_SUB = 17,                  /* a - b (operations.rs:123-157) */
+    FVG_OP_NEG = 18                   /* -a    (operations.rs:190-199) */
+};
+
+typedef struct fvg_term_op {
+    int32_t op;
+    int32_t arg0;
+    int32_t arg1;
+    int32_t arg2;
+    double scalar;
+} fvg_term_op;
+
+#define FVG_MAX_TERM_OPS 32
+
+/* Writes every value of `lhs` (whose pattern must be fvg_csr_from_mesh(mesh))
+ * and every entry of `rhs`.  bc_i / bc_e carry the boundary closure
+ * (value_outside = bc_i * value_inside + bc_e, assembly.rs:66-72) for each
+ * boundary face in face order; they may be NULL when the mesh has none. */
+int fvg_assemble(fvg_ctx* ctx, const fvg_mesh* mesh, const fvg_term_op* program, int32_t n_ops,
+                 const fvg_vec* const* face_fields, int32_t n_face_fields, const fvg_vec* const* cell_fields,
+                 int32_t n_cell_fields, const fvg_vec* bc_i, const fvg_vec* bc_e, fvg_csr* lhs, fvg_vec* rhs);
+
+/* ---- preconditioners: Preconditioner<T> (src/linalg/preconditioners/mod.rs:13-17) ---- */
+typedef struct fvg_prec fvg_prec;
+
+/* IdentityPreconditioner (mod.rs:22-33) */
+int fvg_identity_create(fvg_ctx* ctx, int64_t n, fvg_prec** out);
+/* DiagonalPreconditioner::from_matrix (mod.rs:38-68) */
+int fvg_diagonal_create(fvg_ctx* ctx, const fvg_csr* a, fvg_prec** out);
+/* IncompleteCholesky::from_matrix(&A, level) (incomplete_cholesky.rs:205-211).
+ * Factorised on the device.  level must be 0 in this build (FVG_E_UNSUPPORTED
+ * otherwise).  FVG_E_NOT_SYMMETRIC mirrors the reference's assert. */
+int fvg_ic_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out);
+/* IncompleteLowerUpper::from_matrix(&A, level) (incomplete_lower_upper.rs:266-282),
+ * with its drop rule |v| <= 1e-6 |u_ii| (:255-258).  level must be 0. */
+int fvg_ilu_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out);
+/* Re-run the numeric factorisation for new values on the same pattern. */
+int fvg_prec_refactor(fvg_ctx* ctx, fvg_prec* p, const fvg_csr* a);
+int fvg_prec_destroy(fvg_ctx* ctx, fvg_prec* p);
+/* Preconditioner::precondition(solution, rhs): solution = M^-1 rhs. */
+int fvg_precondition(fvg_ctx* ctx, const fvg_prec* p, fvg_vec* solution, const fvg_vec* rhs);
+/* Factor download (for checks): IC -> L (lower with diagonal, A's lower pattern) and
+ * inv_diag; ILU -> LU values on A's pattern (dropped entries read 0). */
+int fvg_prec_factor_download(fvg_ctx* ctx, const fvg_prec* p, double* values, double* inv_diag);
+/* Number of levels of the lower and upper triangular dependency graphs. */
+int fvg_prec_levels(const fvg_prec* p, int32_t* lower_levels, int32_t* upper_levels);
+/* Count of IC pivots that were not positive (the reference yields NaN silently). */
+int fvg_prec_nonpositive_pivots(fvg_ctx* ctx, const fvg_prec* p, int64_t* count);
+
+/* ---- Krylov solvers (src/linalg/solvers/) ------------------------------------ */
+enum { FVG_PRECISION_FP64 = 0, FVG_PRECISION_MIXED = 1 };
+
+/* LinearSolverOptions (solvers/mod.rs:20-36); defaults rel 1e-6, abs 1e-6, 1000. */
+typedef struct fvg_solver_options {
+    double relative_tolerance;
+    double absolute_tolerance;
+    int64_t max_iterations;
+    int32_t precision;          /* FVG_PRECISION_FP64: the reference's algorithm in FP64.
+                                   FVG_PRECISION_MIXED: FP32 inner preconditioned iterations
+                                   inside an FP64 iterative-refinement loop. */
+    int32_t reserved;
+    double inner_tolerance;     /* mixed only: relative reduction per inner solve (0 = default 1e-4) */
+} fvg_solver_options;
+
+/* LinearSolverInfo (solvers/mod.rs:11-17) plus counters. */
+typedef struct fvg_solver_info {
+    int64_t iterations;         /* FP64: reference iteration count; mixed: total inner iterations */
+    double initial_residual;
+    double final_residual;
+    int64_t history_len;
+    int64_t restarts;           /* BiCGStab restarts (bi_conjugate_gradient_stab.rs:155-169) */
+    int64_t outer_iterations;   /* mixed only: refinement steps */
+    double true_residual;       /* mixed only: ||b - A x|| / ||b|| in FP64 at exit */
+} fvg_solver_info;
+
+/* conjugate_gradient (solvers/conjugate_gradient.rs:12-112).  x is the initial
+ * guess on entry and the solution on exit.  history (may be NULL) receives up to
+ * history_capacity residuals. */
+int fvg_cg(fvg_ctx* ctx, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
+           const fvg_solver_options* opts, fvg_solver_info* info, double* history, int64_t history_capacity);
+/* bi_conjugate_gradient_stab (solvers/bi_conjugate_gradient_stab.rs:13-179), right
+ * preconditioned, including its residual scaling in the loop test (:139, :171). */
+int fvg_bicgstab(fvg_ctx* ctx, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
+                 const fvg_solver_options* opts, fvg_solver_info* info, double* history,
+                 int64_t history_capacity);
+
+/* ---- device timing of the most recent solve ------------------------------- */
+typedef struct fvg_kernel_stats {
+    double spmv_ms;             /* total time of the SpMV kernels */
+    double sweep_ms;            /* total time of the triangular-sweep kernels */
+    int64_t spmv_launches;
+    int64_t sweep_launches;
+} fvg_kernel_stats;
+/* Enable HIP-event timing of SpMV / sweep kernels on this context (adds events). */
+int fvg_ctx_enable_timing(fvg_ctx* ctx, int enable);
+int fvg_ctx_kernel_stats(fvg_ctx* ctx, fvg_kernel_stats* out, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVG_H */

@@ -1,0 +1,1115 @@
+// oracle.cpp -- CPU FP64 restatement of the finite-volumes assemble-and-solve path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker the GPU library is compared
+// against; it is loaded by tests/, by __graft_entry__.smoke() and by the CPU-baseline
+// leg of bench.py, and by nothing else.  The product (libfvg.so) never links it.
+//
+// Every function restates the reference crate's Rust code in the same operation order
+// (sequential sums, reciprocal-multiply, no fused multiply-add: build with
+// -ffp-contract=off), citing the file:line it follows.  File paths are relative to the
+// root of the finite-volumes-rs crate.
+//
+// Exported C functions (prefix orc_) are bound from Python through ctypes.
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------------
+// Sparsity<T>  (src/core/sparsity.rs:8-161)
+// ---------------------------------------------------------------------------------
+struct Sparsity {
+    std::vector<size_t> minors;
+    std::vector<size_t> major_starts{0};
+    size_t max_minor = 0;
+
+    void push_to_major(size_t m) {                       // sparsity.rs:25-28
+        minors.push_back(m);
+        max_minor = std::max(max_minor, m);
+    }
+    void close_major() { major_starts.push_back(minors.size()); }   // :31-33
+    size_t major_start(size_t i) const { return major_starts[i]; }  // :41-43
+    size_t major_end(size_t i) const { return major_starts[i + 1]; }
+    size_t major_len() const { return major_starts.size() - 1; }
+    size_t minor_len() const { return minors.size(); }
+    // sparsity.rs:102-111 (linear search)
+    long find_flat_index(size_t major, size_t minor) const {
+        for (size_t k = major_start(major); k < major_end(major); ++k)
+            if (minors[k] == minor) return (long)k;
+        return -1;
+    }
+    // sparsity.rs:123-138: per-row sort
+    void sort() {
+        for (size_t i = 0; i < major_len(); ++i)
+            std::sort(minors.begin() + major_start(i), minors.begin() + major_end(i));
+    }
+    // sparsity.rs:141-157: per-row sort carrying values (stable, like Rust's sort_by)
+    template <class D>
+    void sort_with(std::vector<D>& values) {
+        for (size_t i = 0; i < major_len(); ++i) {
+            size_t rs = major_start(i), re = major_end(i);
+            std::vector<std::pair<size_t, D>> merged;
+            merged.reserve(re - rs);
+            for (size_t k = rs; k < re; ++k) merged.emplace_back(minors[k], values[k]);
+            std::stable_sort(merged.begin(), merged.end(),
+                             [](const auto& a, const auto& b) { return a.first < b.first; });
+            for (size_t k = rs; k < re; ++k) {
+                minors[k] = merged[k - rs].first;
+                values[k] = merged[k - rs].second;
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------
+// Small vector  (src/core/vector.rs)
+// ---------------------------------------------------------------------------------
+template <int N>
+struct Vec {
+    double d[N];
+    Vec() { for (int i = 0; i < N; ++i) d[i] = 0.0; }
+    double& operator[](int i) { return d[i]; }
+    double operator[](int i) const { return d[i]; }
+};
+template <int N> Vec<N> operator+(Vec<N> a, const Vec<N>& b) { for (int i = 0; i < N; ++i) a[i] += b[i]; return a; }   // vector.rs:195-203
+template <int N> Vec<N> operator-(Vec<N> a, const Vec<N>& b) { for (int i = 0; i < N; ++i) a[i] -= b[i]; return a; }   // :206-214
+template <int N> Vec<N> operator*(Vec<N> a, double s) { for (int i = 0; i < N; ++i) a[i] *= s; return a; }             // :216-224
+template <int N> Vec<N> operator/(Vec<N> a, double s) { for (int i = 0; i < N; ++i) a[i] /= s; return a; }             // :227-235
+template <int N> double dot(const Vec<N>& a, const Vec<N>& b) {   // vector.rs:37-43
+    double x = 0.0;
+    for (int i = 0; i < N; ++i) x += a[i] * b[i];
+    return x;
+}
+template <int N> double norm(const Vec<N>& a) { return std::sqrt(dot(a, a)); }  // vector.rs:45-47
+
+// Normal::normal  (vector.rs:301-329)
+template <int N> Vec<N> normal_of(const Vec<N>& s, const Vec<N>& r) {
+    Vec<N> n;
+    if (N == 1) {
+        n[0] = 1.0;
+    } else if (N == 2) {
+        n[0] = -s[1];
+        n[1] = s[0];
+    } else {
+        n[0] = s[1 % N] * r[2 % N] - s[2 % N] * r[1 % N];
+        n[1 % N] = s[2 % N] * r[0] - s[0] * r[2 % N];
+        n[2 % N] = s[0] * r[1 % N] - s[1 % N] * r[0];
+        for (int i = 0; i < N; ++i) n[i] *= 0.5;
+    }
+    return n;
+}
+
+// ---------------------------------------------------------------------------------
+// Mesh<DIM>  (src/core/mesh/mod.rs, src/core/mesh/compute.rs)
+// ---------------------------------------------------------------------------------
+constexpr size_t NONE = (size_t)-1;
+
+template <int DIM>
+struct FaceData {
+    double area = 0.0;
+    Vec<DIM> center, normal;
+    size_t owner = NONE;
+    int kind = 0;          // 0 none, 1 cell, 2 boundary  (InternalFaceNeighbor, mod.rs:117-122)
+    size_t neighbor = 0;   // cell index or patch id
+};
+template <int DIM>
+struct CellData {
+    double volume = 0.0;
+    Vec<DIM> center;
+};
+
+template <int DIM>
+struct Mesh {
+    std::vector<Vec<DIM>> nodes;
+    Sparsity face_nodes, cell_faces, cell_to_cell;
+    std::vector<FaceData<DIM>> face_data;
+    std::vector<CellData<DIM>> cell_data;
+
+    void add_node(const Vec<DIM>& p) { nodes.push_back(p); }   // mod.rs:444-449
+
+    // mod.rs:451-476: boundary faces are tagged now, interior faces get their
+    // neighbour from the second add_cell that names them.
+    void add_face(const int32_t* nd, size_t len, int32_t patch) {
+        for (size_t k = 0; k < len; ++k) face_nodes.push_to_major((size_t)nd[k]);
+        face_nodes.close_major();
+        FaceData<DIM> fd;
+        if (patch >= 0) { fd.kind = 2; fd.neighbor = (size_t)patch; }
+        face_data.push_back(fd);
+    }
+
+    // mod.rs:486-527
+    void add_cell(const int32_t* fc, size_t len) {
+        for (size_t k = 0; k < len; ++k) cell_faces.push_to_major((size_t)fc[k]);
+        cell_faces.close_major();
+        cell_data.push_back(CellData<DIM>());
+        size_t me = cell_data.size() - 1;
+        for (size_t k = 0; k < len; ++k) {
+            FaceData<DIM>& f = face_data[(size_t)fc[k]];
+            if (f.owner == NONE) {
+                f.owner = me;
+            } else if (f.kind == 0) {
+                f.kind = 1;
+                f.neighbor = me;
+            }
+        }
+    }
+
+    size_t n_cells() const { return cell_faces.major_len(); }
+    size_t n_faces() const { return face_nodes.major_len(); }
+
+    // compute.rs:267-321
+    void calc_face_data(size_t f, const Vec<DIM>& node_ave) {
+        FaceData<DIM>& fd = face_data[f];
+        size_t s = face_nodes.major_start(f), len = face_nodes.major_end(f) - s;
+        if (DIM == 2) {
+            if (len != 2) throw std::runtime_error("2D face must have two nodes");
+            Vec<DIM> n0 = nodes[face_nodes.minors[s]];
+            Vec<DIM> n1 = nodes[face_nodes.minors[s + 1]];
+            Vec<DIM> dn = n1 - n0;
+            Vec<DIM> dsn = normal_of(dn, dn);
+            fd.area = norm(dsn);
+            fd.normal = dsn / fd.area;
+            fd.center = (n0 + n1) * 0.5;
+        } else {
+            Vec<DIM> center, normal;
+            double area = 0.0;
+            for (size_t i = 0; i < len; ++i) {
+                Vec<DIM> n0 = nodes[face_nodes.minors[s + i]];
+                Vec<DIM> n1 = nodes[face_nodes.minors[s + (i == len - 1 ? 0 : i + 1)]];
+                Vec<DIM> dn0 = n0 - node_ave;
+                Vec<DIM> dn1 = n1 - node_ave;
+                Vec<DIM> dsn = normal_of(dn0, dn1);
+                double area_i = norm(dsn);
+                area += area_i;
+                normal = dsn;                                  // compute.rs:305 (assignment)
+                center = center + ((n0 + n1 + node_ave) * 1.0 / 3.0) * area_i;
+            }
+            if (!(area > 2.220446049250313e-16)) throw std::runtime_error("degenerate face");
+            normal = normal / area;
+            center = center / area;
+            fd.area = area;
+            fd.center = center;
+            fd.normal = normal;
+        }
+    }
+
+    // compute.rs:325-375
+    void calc_cell_data(size_t c, const Vec<DIM>& facec_ave) {
+        Vec<DIM> center;
+        double volume = 0.0;
+        for (size_t k = cell_faces.major_start(c); k < cell_faces.major_end(c); ++k) {
+            const FaceData<DIM>& face = face_data[cell_faces.minors[k]];
+            Vec<DIM> dfc = face.center - facec_ave;
+            double height = std::fabs(dot(dfc, face.normal));
+            double volume_i, w;
+            Vec<DIM> center_i;
+            if (DIM == 2) {
+                volume_i = height * face.area / 2.0;
+                center_i = (face.center * 2.0 + facec_ave) / 3.0;
+            } else {
+                volume_i = height * face.area / 3.0;
+                center_i = (face.center * 3.0 + facec_ave) / 4.0;
+            }
+            (void)w;
+            center = center + center_i * volume_i;
+            volume += volume_i;
+        }
+        cell_data[c].volume = volume;
+        cell_data[c].center = center / volume;
+    }
+
+    // compute.rs:12-152 (serial: every cell and face owned)
+    void compute() {
+        for (size_t f = 0; f < n_faces(); ++f) {                       // :15-18
+            if (face_data[f].owner == NONE || face_data[f].owner >= n_cells())
+                throw std::runtime_error("face without owner cell");
+            if (face_data[f].kind == 0)                                 // :21-31
+                throw std::runtime_error("face without neighbour in a serial mesh");
+        }
+        for (size_t f = 0; f < n_faces(); ++f) {                       // :35-45
+            size_t s = face_nodes.major_start(f), size = face_nodes.major_end(f) - s;
+            Vec<DIM> c;
+            for (size_t k = 0; k < size; ++k) c = c + nodes[face_nodes.minors[s + k]];
+            c = c / (double)size;
+            calc_face_data(f, c);
+        }
+        for (size_t c = 0; c < n_cells(); ++c) {                       // :52-69
+            size_t s = cell_faces.major_start(c), size = cell_faces.major_end(c) - s;
+            Vec<DIM> center;
+            for (size_t k = 0; k < size; ++k) center = center + face_data[cell_faces.minors[s + k]].center;
+            center = center / (double)size;
+            calc_cell_data(c, center);
+        }
+        for (size_t f = 0; f < n_faces(); ++f) {                       // :97-131
+            Vec<DIM> inside = cell_data[face_data[f].owner].center;
+            if (dot(face_data[f].normal, face_data[f].center - inside) < 0.0)
+                face_data[f].normal = face_data[f].normal * -1.0;
+        }
+        for (size_t c = 0; c < n_cells(); ++c) {                       // :135-152
+            std::set<size_t> others;   // HashSet in the reference; order is removed by the later sort
+            for (size_t k = cell_faces.major_start(c); k < cell_faces.major_end(c); ++k) {
+                const FaceData<DIM>& fd = face_data[cell_faces.minors[k]];
+                if (fd.kind == 1) others.insert(fd.neighbor == c ? fd.owner : fd.neighbor);  // FaceRef::other_cell mod.rs:326-338
+            }
+            cell_to_cell.push_to_major(c);
+            for (size_t o : others) cell_to_cell.push_to_major(o);
+            cell_to_cell.close_major();
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------
+// DistributedMatrix<f64>  (src/linalg/dmatrix.rs)
+// ---------------------------------------------------------------------------------
+struct Matrix {
+    Sparsity sp;
+    std::vector<double> values;
+    size_t nrows = 0, ncols = 0;
+
+    // dmatrix.rs:38-50
+    static Matrix from_cut_sparsity(const Sparsity& s, size_t nrows) {
+        Matrix m;
+        for (size_t i = 0; i < nrows; ++i) {
+            for (size_t k = s.major_start(i); k < s.major_end(i); ++k) m.sp.push_to_major(s.minors[k]);
+            m.sp.close_major();
+        }
+        m.ncols = m.sp.max_minor + 1;
+        m.values.assign(m.sp.minor_len(), 0.0);
+        m.sp.sort();
+        m.nrows = nrows;
+        return m;
+    }
+    // dmatrix.rs:53-58
+    static Matrix from_sparsity_and_values(Sparsity s, std::vector<double> v) {
+        Matrix m;
+        m.nrows = s.major_len();
+        m.ncols = s.max_minor + 1;
+        s.sort_with(v);
+        m.sp = std::move(s);
+        m.values = std::move(v);
+        return m;
+    }
+    // binary search inside a sorted row: Index<[usize;2]> dmatrix.rs:262-286
+    long pos(size_t i, size_t j) const {
+        if (i >= sp.major_len()) return -1;
+        auto b = sp.minors.begin() + sp.major_start(i), e = sp.minors.begin() + sp.major_end(i);
+        auto it = std::lower_bound(b, e, j);
+        if (it == e || *it != j) return -1;
+        return (long)(it - sp.minors.begin());
+    }
+    double& at(size_t i, size_t j) {
+        long p = pos(i, j);
+        if (p < 0) throw std::runtime_error("Sparse matrix index not found");
+        return values[(size_t)p];
+    }
+    double get(size_t i, size_t j) const {   // dmatrix.rs:105-117
+        long p = pos(i, j);
+        return p < 0 ? 0.0 : values[(size_t)p];
+    }
+    // dmatrix.rs:136-146
+    void imul(std::vector<double>& result, const std::vector<double>& rhs) const {
+        for (size_t i = 0; i < nrows; ++i) {
+            result[i] = 0.0;
+            for (size_t k = sp.major_start(i); k < sp.major_end(i); ++k)
+                result[i] += values[k] * rhs[sp.minors[k]];
+        }
+    }
+    // dmatrix.rs:214-229
+    Matrix square_block() const {
+        Sparsity s;
+        std::vector<double> v;
+        for (size_t i = 0; i < nrows; ++i) {
+            for (size_t k = sp.major_start(i); k < sp.major_end(i); ++k) {
+                if (sp.minors[k] < nrows) { s.push_to_major(sp.minors[k]); v.push_back(values[k]); }
+            }
+            s.close_major();
+        }
+        return from_sparsity_and_values(std::move(s), std::move(v));
+    }
+    // dmatrix.rs:76-88 with ApproximateCmp for f64 (linalg/mod.rs:73-77)
+    bool is_symmetric() const {
+        for (size_t i = 0; i < nrows; ++i)
+            for (size_t k = sp.major_start(i); k < sp.major_end(i); ++k) {
+                size_t j = sp.minors[k];
+                if (j < nrows) {
+                    long p = pos(j, i);
+                    if (p < 0) throw std::runtime_error("Sparse matrix index not found");
+                    if (!(std::fabs(values[k] - values[(size_t)p]) < 1e-14)) return false;
+                }
+            }
+        return true;
+    }
+};
+
+// ---------------------------------------------------------------------------------
+// Terms and assembly (src/fvm/terms/*, src/fvm/schemes/*, src/fvm/assembly.rs)
+// The expression tree arrives as a postfix program; each opcode is one reference term.
+// ---------------------------------------------------------------------------------
+enum {
+    OP_LAPLACIAN_ORTHOGONAL = 1,
+    OP_CONVECTION_UPWIND = 2,
+    OP_SOURCE = 3,
+    OP_LINEAR_SOURCE = 4,
+    OP_TIME_EULER = 5,
+    OP_DIVERGENCE = 6,
+    OP_ADD = 16,
+    OP_SUB = 17,
+    OP_NEG = 18,
+};
+struct TermOp { int32_t op, arg0, arg1, arg2; double scalar; };
+
+struct Triple { double li, lj, r; };
+struct Pair { double l, r; };
+
+template <int DIM>
+struct Assembler {
+    const Mesh<DIM>& mesh;
+    const TermOp* prog;
+    int nops;
+    const double* const* face_fields;
+    const double* const* cell_fields;
+
+    // Term::face_terms of the whole expression for face f
+    Triple face_terms(size_t f) const {
+        Triple st[32];
+        int sp = 0;
+        const FaceData<DIM>& fd = mesh.face_data[f];
+        for (int k = 0; k < nops; ++k) {
+            const TermOp& o = prog[k];
+            switch (o.op) {
+            case OP_LAPLACIAN_ORTHOGONAL: {
+                // orthogonal.rs:26-44 then laplacian.rs:35-45
+                double dx;
+                if (fd.kind == 1) dx = norm(mesh.cell_data[fd.neighbor].center - mesh.cell_data[fd.owner].center);
+                else dx = norm(fd.center - mesh.cell_data[fd.owner].center);
+                double t = 1.0 / dx;
+                double l0 = 1.0 * (-t), l1 = 1.0 * t, r = 0.0;
+                double diff = face_fields[o.arg0][f];
+                st[sp++] = {diff * l0, diff * l1, -(diff * r)};
+                break;
+            }
+            case OP_CONVECTION_UPWIND: {
+                // upwind.rs:26-33 then convection.rs:36-44; arg0 = convected flux, arg1 = upwind flux
+                double uflux = face_fields[o.arg1][f];
+                double l0, l1, r = 0.0;
+                if (uflux > 0.0) { l0 = 1.0; l1 = 0.0; } else { l0 = 0.0; l1 = 1.0; }
+                double flux = face_fields[o.arg0][f];
+                st[sp++] = {l0 * flux, l1 * flux, (-r) * flux};
+                break;
+            }
+            case OP_DIVERGENCE: {   // divergence.rs:36-45
+                double flux = face_fields[o.arg0][f];
+                st[sp++] = {0.0, 0.0, -flux};
+                break;
+            }
+            case OP_SOURCE: case OP_LINEAR_SOURCE: case OP_TIME_EULER:
+                st[sp++] = {0.0, 0.0, 0.0};   // Term::face_terms default (terms/mod.rs:32-34)
+                break;
+            case OP_ADD: { Triple b = st[--sp], a = st[--sp]; st[sp++] = {a.li + b.li, a.lj + b.lj, a.r + b.r}; break; }   // operations.rs:34-41, 56-62
+            case OP_SUB: { Triple b = st[--sp], a = st[--sp]; st[sp++] = {a.li - b.li, a.lj - b.lj, a.r - b.r}; break; }   // operations.rs:71-77
+            case OP_NEG: { Triple a = st[--sp]; st[sp++] = {-a.li, -a.lj, -a.r}; break; }                                  // operations.rs:170-176
+            default: throw std::runtime_error("bad term opcode");
+            }
+        }
+        return st[0];
+    }
+
+    // Term::cell_terms of the whole expression for cell c
+    Pair cell_terms(size_t c) const {
+        Pair st[32];
+        int sp = 0;
+        for (int k = 0; k < nops; ++k) {
+            const TermOp& o = prog[k];
+            switch (o.op) {
+            case OP_SOURCE:          // source.rs:34-39
+                st[sp++] = {0.0, -cell_fields[o.arg0][c]};
+                break;
+            case OP_LINEAR_SOURCE:   // source.rs:72-77
+                st[sp++] = {cell_fields[o.arg0][c], 0.0};
+                break;
+            case OP_TIME_EULER: {    // terms/time.rs:31-34 -> schemes/time/euler.rs:27-37
+                double r0 = 1.0, r1 = 1.0;
+                if (o.arg1 >= 0) { r0 = cell_fields[o.arg1][c]; r1 = cell_fields[o.arg2][c]; }
+                double dt = o.scalar;
+                st[sp++] = {1.0 * (1.0 / dt) * r0, cell_fields[o.arg0][c] / dt * r1};
+                break;
+            }
+            case OP_LAPLACIAN_ORTHOGONAL: case OP_CONVECTION_UPWIND: case OP_DIVERGENCE:
+                st[sp++] = {0.0, 0.0};   // Term::cell_terms default (terms/mod.rs:28-30)
+                break;
+            case OP_ADD: { Pair b = st[--sp], a = st[--sp]; st[sp++] = {a.l + b.l, a.r + b.r}; break; }
+            case OP_SUB: { Pair b = st[--sp], a = st[--sp]; st[sp++] = {a.l - b.l, a.r - b.r}; break; }
+            case OP_NEG: { Pair a = st[--sp]; st[sp++] = {-a.l, -a.r}; break; }
+            default: throw std::runtime_error("bad term opcode");
+            }
+        }
+        return st[0];
+    }
+
+    // assembly.rs:11-79.  bc_i/bc_e are the boundary closure values, in boundary-face order.
+    void assemble(Matrix& lhs, std::vector<double>& rhs, const double* bc_i, const double* bc_e) const {
+        size_t n = mesh.n_cells();
+        lhs = Matrix::from_cut_sparsity(mesh.cell_to_cell, n);
+        rhs.assign(n, 0.0);
+        for (size_t c = 0; c < n; ++c) {
+            Pair p = cell_terms(c);
+            lhs.at(c, c) = p.l * mesh.cell_data[c].volume;
+            rhs[c] = p.r * mesh.cell_data[c].volume;
+        }
+        size_t b = 0;
+        for (size_t f = 0; f < mesh.n_faces(); ++f) {
+            Triple t = face_terms(f);
+            const FaceData<DIM>& fd = mesh.face_data[f];
+            double li = t.li * fd.area, lj = t.lj * fd.area, r = t.r * fd.area;
+            size_t i = fd.owner;
+            if (fd.kind == 1) {
+                size_t j = fd.neighbor;
+                lhs.at(i, i) += li;
+                lhs.at(i, j) += lj;
+                rhs[i] += r;
+                lhs.at(j, i) -= li;
+                lhs.at(j, j) -= lj;
+                rhs[j] -= r;
+            } else {
+                double bndi = bc_i[b], bnde = bc_e[b];
+                ++b;
+                lhs.at(i, i) += li;
+                lhs.at(i, i) += lj * bndi;
+                rhs[i] += r - lj * bnde;
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------
+// Preconditioners  (src/linalg/preconditioners/*)
+// ---------------------------------------------------------------------------------
+struct Prec {
+    virtual ~Prec() {}
+    virtual void precondition(std::vector<double>& sol, const std::vector<double>& rhs) const = 0;
+};
+
+struct IdentityPrec : Prec {   // preconditioners/mod.rs:22-33
+    void precondition(std::vector<double>& sol, const std::vector<double>& rhs) const override {
+        for (size_t i = 0; i < rhs.size(); ++i) sol[i] = rhs[i];
+    }
+};
+
+struct DiagonalPrec : Prec {   // preconditioners/mod.rs:38-68
+    std::vector<double> diagonal;
+    explicit DiagonalPrec(const Matrix& m) {
+        diagonal.resize(m.nrows);
+        for (size_t i = 0; i < m.nrows; ++i) {
+            long p = m.pos(i, i);
+            if (p < 0) throw std::runtime_error("Sparse matrix index not found");
+            diagonal[i] = 1.0 / m.values[(size_t)p];
+        }
+    }
+    void precondition(std::vector<double>& sol, const std::vector<double>& rhs) const override {
+        for (size_t i = 0; i < diagonal.size(); ++i) sol[i] = diagonal[i] * rhs[i];
+    }
+};
+
+struct ICPrec : Prec {   // incomplete_cholesky.rs:19-255
+    Matrix l;
+    Sparsity sp_t;
+    std::vector<double> l_vt, inv_diag;
+
+    // incomplete_cholesky.rs:35-85 (stack-ordered walk, HashMap visited/length)
+    static void row_i_sparsity_edit(std::set<size_t>& s, size_t i, const Matrix& m, size_t level) {
+        s.clear();
+        std::vector<size_t> q{i};
+        std::unordered_map<size_t, size_t> visited{{i, i}}, length{{i, 0}};
+        while (!q.empty()) {
+            size_t k = q.back();
+            q.pop_back();
+            size_t lk = length.at(k);
+            for (size_t kj = m.sp.major_start(k); kj < m.sp.major_end(k); ++kj) {
+                size_t j = m.sp.minors[kj];
+                if (j == k) continue;
+                auto it = visited.find(j);
+                if (it != visited.end() && it->second == i) continue;
+                visited[j] = i;
+                if (lk < level) {
+                    q.push_back(j);
+                    length[j] = lk + 1;
+                    s.insert(j);
+                }
+            }
+        }
+    }
+
+    // incomplete_cholesky.rs:148-202
+    ICPrec(const Matrix& matrix_in, size_t level) {
+        Matrix matrix = matrix_in.square_block();
+        if (!matrix.is_symmetric()) throw std::invalid_argument("not symmetric");
+        Sparsity sp;
+        std::vector<double> values;
+        std::set<size_t> row_sp;
+        for (size_t i = 0; i < matrix.nrows; ++i) {
+            row_i_sparsity_edit(row_sp, i, matrix, level);
+            for (size_t k = matrix.sp.major_start(i); k < matrix.sp.major_end(i); ++k) {
+                size_t j = matrix.sp.minors[k];
+                if (j <= i) { sp.push_to_major(j); values.push_back(matrix.values[k]); }
+                if (j >= i) sp_t.push_to_major(j);
+                row_sp.erase(j);
+            }
+            for (size_t j : row_sp) {
+                if (j <= i) { sp.push_to_major(j); values.push_back(0.0); }
+                if (j >= i) sp_t.push_to_major(j);
+            }
+            sp.close_major();
+            sp_t.close_major();
+        }
+        inv_diag.assign(matrix.nrows, 0.0);
+        l_vt.assign(values.size(), 0.0);
+        l = Matrix::from_sparsity_and_values(std::move(sp), std::move(values));
+        compute();
+    }
+
+    // incomplete_cholesky.rs:87-145
+    void compute() {
+        for (size_t k = 0; k < l.nrows; ++k) {
+            double lkk = l.at(k, k);
+            lkk = std::sqrt(lkk);
+            double lkkinv = 1.0 / lkk;
+            inv_diag[k] = lkkinv;
+            l.at(k, k) = lkk;
+            for (size_t k_i = sp_t.major_start(k); k_i < sp_t.major_end(k); ++k_i) {
+                size_t i = sp_t.minors[k_i];
+                if (i <= k) continue;
+                long p = l.pos(i, k);
+                if (p >= 0) l.values[(size_t)p] = lkkinv * l.values[(size_t)p];
+            }
+            for (size_t k_j = sp_t.major_start(k); k_j < sp_t.major_end(k); ++k_j) {
+                for (size_t k_i = sp_t.major_start(k); k_i < sp_t.major_end(k); ++k_i) {
+                    size_t i = sp_t.minors[k_i];
+                    size_t j = sp_t.minors[k_j];
+                    if (i < j) continue;
+                    if (j <= k) continue;
+                    double lik = l.get(i, k);
+                    double ljk = l.get(j, k);
+                    long p = l.pos(i, j);
+                    if (p >= 0) l.values[(size_t)p] -= lik * ljk;
+                }
+            }
+        }
+        for (size_t row = 0; row < l.nrows; ++row)
+            for (size_t k = l.sp.major_start(row); k < l.sp.major_end(row); ++k) {
+                long kk = sp_t.find_flat_index(l.sp.minors[k], row);
+                if (kk < 0) throw std::runtime_error("transpose entry missing");
+                l_vt[(size_t)kk] = l.values[k];
+            }
+    }
+
+    // incomplete_cholesky.rs:217-254
+    void precondition(std::vector<double>& solution, const std::vector<double>& rhs) const override {
+        for (size_t i = 0; i < l.nrows; ++i) {
+            double si = rhs[i];
+            for (size_t k = l.sp.major_start(i); k < l.sp.major_end(i); ++k) {
+                size_t j = l.sp.minors[k];
+                if (i != j) si -= l.values[k] * solution[j];
+            }
+            solution[i] = inv_diag[i] * si;
+        }
+        for (size_t ii = l.nrows; ii-- > 0;) {
+            size_t i = ii;
+            double si = solution[i];
+            for (size_t k = sp_t.major_start(i); k < sp_t.major_end(i); ++k) {
+                size_t j = sp_t.minors[k];
+                if (i != j) si -= l_vt[k] * solution[j];
+            }
+            solution[i] = inv_diag[i] * si;
+        }
+    }
+};
+
+struct ILUPrec : Prec {   // incomplete_lower_upper.rs:17-328
+    Matrix lu;
+
+    // incomplete_lower_upper.rs:30-81
+    static std::set<size_t> row_i_u_sparsity(size_t i, const Matrix& m, size_t level) {
+        std::set<size_t> s;
+        std::vector<size_t> q{i};
+        std::map<size_t, size_t> visited{{i, i}}, length{{i, 0}};
+        while (!q.empty()) {
+            size_t k = q.back();
+            q.pop_back();
+            size_t lk = length.at(k);
+            for (size_t kj = m.sp.major_start(k); kj < m.sp.major_end(k); ++kj) {
+                size_t j = m.sp.minors[kj];
+                if (j == k) continue;
+                auto it = visited.find(j);
+                if (it != visited.end() && it->second == i) continue;
+                visited[j] = i;
+                if (j < i && lk < level) { q.push_back(j); length[j] = lk + 1; }
+                else if (j > i) s.insert(j);
+            }
+        }
+        return s;
+    }
+    // incomplete_lower_upper.rs:84-137
+    static std::set<size_t> row_i_l_sparsity(size_t i, const Matrix& m, size_t level) {
+        std::set<size_t> s;
+        std::vector<size_t> q{i};
+        std::map<size_t, size_t> visited{{i, i}}, length{{i, 0}};
+        while (!q.empty()) {
+            size_t k = q.back();
+            q.pop_back();
+            size_t lk = length.at(k);
+            for (size_t kj = m.sp.major_start(k); kj < m.sp.major_end(k); ++kj) {
+                size_t j = m.sp.minors[kj];
+                if (j == k) continue;
+                auto it = visited.find(j);
+                if (it != visited.end() && it->second == i) continue;
+                visited[j] = i;
+                if (j > i && lk < level) { q.push_back(j); length[j] = lk + 1; }
+                else if (j < i) s.insert(j);
+            }
+        }
+        return s;
+    }
+
+    // incomplete_lower_upper.rs:200-263.  A zero pivot throws (the reference panics).
+    ILUPrec(const Matrix& matrix_in, size_t level) {
+        Matrix matrix = matrix_in.square_block();
+        size_t n = matrix.nrows;
+        lu.nrows = 0;
+        for (size_t i = 0; i < n; ++i) {
+            std::set<size_t> l_row_sp = row_i_l_sparsity(i, matrix, level);
+            std::set<size_t> u_row_sp = row_i_u_sparsity(i, matrix, level);
+            std::unordered_map<size_t, double> lu_row;   // SparseArray (sparse_array.rs:8-84)
+            for (size_t k : l_row_sp) lu_row[k] = matrix.get(i, k);
+            lu_row[i] = matrix.get(i, i);
+            for (size_t k : u_row_sp) lu_row[k] = matrix.get(i, k);
+            for (size_t k : l_row_sp) {
+                if (k >= i) continue;
+                double lukk = lu.at(k, k);
+                if (std::fabs(lukk) < 1e-16) throw std::domain_error("zero pivot");
+                double lukkinv = 1.0 / lukk;
+                lu_row.at(k) = lukkinv * lu_row.at(k);
+                double lik = lu_row.at(k);
+                for (size_t q = lu.sp.major_start(k); q < lu.sp.major_end(k); ++q) {
+                    size_t j = lu.sp.minors[q];
+                    if (j <= k) continue;
+                    double term = lik * lu.values[q];
+                    auto it = lu_row.find(j);
+                    if (it == lu_row.end()) continue;
+                    it->second -= term;
+                }
+            }
+            // sparsify_btreeset: keep the diagonal, drop |v| <= |u_ii|*1e-6 (sparse_array.rs:48-50)
+            double tol = std::fabs(lu_row.at(i)) * 1e-6;
+            std::vector<std::pair<size_t, double>> row;
+            for (auto& kv : lu_row)
+                if (kv.first == i || std::fabs(kv.second) > tol) row.push_back(kv);
+            std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+            for (auto& kv : row) { lu.sp.push_to_major(kv.first); lu.values.push_back(kv.second); }   // push_row dmatrix.rs:191-200
+            lu.sp.close_major();
+            lu.nrows += 1;
+        }
+        lu.ncols = lu.sp.max_minor + 1;
+    }
+
+    // incomplete_lower_upper.rs:291-328
+    void precondition(std::vector<double>& solution, const std::vector<double>& rhs) const override {
+        for (size_t i = 0; i < lu.nrows; ++i) {
+            double si = rhs[i];
+            for (size_t k = lu.sp.major_start(i); k < lu.sp.major_end(i); ++k) {
+                size_t j = lu.sp.minors[k];
+                if (j < i) si -= lu.values[k] * solution[j];
+            }
+            solution[i] = si;
+        }
+        for (size_t ii = lu.nrows; ii-- > 0;) {
+            size_t i = ii;
+            double si = solution[i];
+            for (size_t k = lu.sp.major_start(i); k < lu.sp.major_end(i); ++k) {
+                size_t j = lu.sp.minors[k];
+                if (j > i) si -= lu.values[k] * solution[j];
+            }
+            double uii = lu.values[(size_t)lu.pos(i, i)];
+            solution[i] = (1.0 / uii) * si;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------
+// Solvers  (src/linalg/solvers/*)
+// ---------------------------------------------------------------------------------
+static double vdot(const std::vector<double>& a, const std::vector<double>& b) {   // dvector.rs:63-71
+    double out = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) out += a[i] * b[i];
+    return out;
+}
+
+struct Info {
+    int64_t iterations = 0;
+    double initial_residual = 0, final_residual = 0;
+    int64_t restarts = 0;
+    std::vector<double> history;
+};
+
+// conjugate_gradient.rs:12-112
+static Info conjugate_gradient(std::vector<double>& solution, const Matrix& lhs, const std::vector<double>& rhs,
+                               const Prec& prec, double rtol, double atol, int64_t max_iter) {
+    Info info;
+    double tolerance = atol, relative_tolerance = rtol;
+    double bnorm = std::sqrt(vdot(rhs, rhs));
+    if (bnorm < 1e-14) {
+        for (auto& v : solution) v = 0.0;
+        info.initial_residual = info.final_residual = bnorm;
+        info.history = {bnorm};
+        return info;
+    }
+    size_t n = solution.size();
+    std::vector<double> p(n, 0.0), r(rhs.size(), 0.0), z(n, 0.0), ap(rhs.size(), 0.0);
+    lhs.imul(r, solution);
+    for (size_t i = 0; i < r.size(); ++i) r[i] -= rhs[i];
+    for (size_t i = 0; i < r.size(); ++i) r[i] *= -1.0;
+    double initial_residual = std::sqrt(vdot(r, r)) / bnorm;
+    prec.precondition(z, r);
+    p = z;
+    double error = 1.0;
+    int64_t iter = 0;
+    while (error > tolerance) {
+        if (iter > max_iter) break;
+        lhs.imul(ap, p);
+        double r_dot_z = vdot(r, z);
+        double p_dot_ap = vdot(p, ap);
+        double alpha = r_dot_z / p_dot_ap;
+        for (size_t i = 0; i < ap.size(); ++i) {
+            solution[i] += p[i] * alpha;
+            r[i] -= ap[i] * alpha;
+        }
+        double r_dot_r = vdot(r, r);
+        error = std::sqrt(r_dot_r) / bnorm;
+        if ((error < tolerance) || (error / initial_residual < relative_tolerance)) {
+            info.history.push_back(error);
+            iter += 1;
+            break;
+        }
+        prec.precondition(z, r);
+        double r_dot_z_new = vdot(r, z);
+        double beta = r_dot_z_new / r_dot_z;
+        for (size_t i = 0; i < ap.size(); ++i) p[i] = z[i] + p[i] * beta;
+        info.history.push_back(error);
+        iter += 1;
+    }
+    info.iterations = iter;
+    info.initial_residual = initial_residual;
+    info.final_residual = error;
+    return info;
+}
+
+// bi_conjugate_gradient_stab.rs:13-179 (right preconditioned; Q4 double division kept)
+static Info bicgstab(std::vector<double>& solution, const Matrix& lhs, const std::vector<double>& rhs,
+                     const Prec& prec, double rtol, double atol, int64_t max_iter) {
+    Info info;
+    double tolerance = atol, relative_tolerance = rtol;
+    double bnorm = std::sqrt(vdot(rhs, rhs));
+    if (bnorm < 1e-14) {
+        for (auto& v : solution) v = 0.0;
+        info.initial_residual = info.final_residual = bnorm;
+        info.history = {bnorm};
+        return info;
+    }
+    size_t n = solution.size(), m = rhs.size();
+    std::vector<double> p(n, 0.0), r(m, 0.0), rh(m, 0.0), z(n, 0.0), y(n, 0.0), v(m, 0.0), h(m, 0.0),
+        s(m, 0.0), t(m, 0.0);
+    lhs.imul(r, solution);
+    for (size_t i = 0; i < m; ++i) r[i] -= rhs[i];
+    for (size_t i = 0; i < m; ++i) r[i] *= -1.0;
+    for (size_t i = 0; i < m; ++i) rh[i] = r[i];
+    double initial_residual = std::sqrt(vdot(r, r)) / bnorm;
+    if (initial_residual < tolerance) {
+        info.initial_residual = info.final_residual = initial_residual;
+        info.history = {initial_residual};
+        return info;
+    }
+    for (size_t i = 0; i < std::min(n, m); ++i) p[i] = r[i];
+    double rho = vdot(rh, r);
+    double error = 1.0;
+    int64_t iter = 0;
+    while (error > tolerance) {
+        if (iter > max_iter) break;
+        prec.precondition(y, p);
+        lhs.imul(v, y);
+        double rh_dot_v = vdot(rh, v);
+        double alpha = rho / rh_dot_v;
+        for (size_t i = 0; i < m; ++i) {
+            h[i] = solution[i] + y[i] * alpha;
+            s[i] = r[i] - v[i] * alpha;
+        }
+        double s_dot_s_sqrt = std::sqrt(vdot(s, s)) / bnorm;
+        if ((s_dot_s_sqrt < tolerance) || (s_dot_s_sqrt / initial_residual < relative_tolerance)) {
+            error = s_dot_s_sqrt;
+            info.history.push_back(error);
+            for (size_t i = 0; i < m; ++i) solution[i] = h[i];
+            iter += 1;
+            break;
+        }
+        prec.precondition(z, s);
+        lhs.imul(t, z);
+        double t_dot_s = vdot(t, s);
+        double t_dot_t = vdot(t, t);
+        double omega = t_dot_s / t_dot_t;
+        for (size_t i = 0; i < m; ++i) {
+            solution[i] = h[i] + z[i] * omega;
+            r[i] = s[i] - t[i] * omega;
+        }
+        double r_dot_r_sqrt = std::sqrt(vdot(r, r)) / bnorm;
+        if ((r_dot_r_sqrt < tolerance) || (r_dot_r_sqrt / initial_residual < relative_tolerance)) {
+            error = r_dot_r_sqrt;
+            info.history.push_back(error);
+            iter += 1;
+            break;
+        }
+        double rho_new = vdot(rh, r);
+        double beta = (rho_new / rho) * (alpha / omega);
+        rho = rho_new;
+        if (std::fabs(rho) > (1e-15 * bnorm)) {
+            for (size_t i = 0; i < m; ++i) p[i] = r[i] + (p[i] - v[i] * omega) * beta;
+        } else {
+            info.restarts += 1;
+            for (size_t i = 0; i < m; ++i) { rh[i] = r[i]; p[i] = r[i]; }
+            rho = vdot(rh, r);
+        }
+        error = r_dot_r_sqrt / bnorm;
+        info.history.push_back(error);
+        iter += 1;
+    }
+    info.iterations = iter;
+    info.initial_residual = initial_residual;
+    info.final_residual = error;
+    return info;
+}
+
+}  // namespace orc
+
+// =================================================================================
+// C interface (ctypes)
+// =================================================================================
+using namespace orc;
+
+namespace {
+thread_local std::string g_err;
+
+struct MeshHandle {
+    int dim;
+    Mesh<2>* m2 = nullptr;
+    Mesh<3>* m3 = nullptr;
+    std::vector<size_t> bnd_of_face;   // boundary ordinal per face, NONE for interior
+    size_t n_bnd = 0;
+    ~MeshHandle() { delete m2; delete m3; }
+};
+
+template <int DIM>
+Mesh<DIM>* build_mesh(int64_t nn, const double* nodes, int64_t nf, const int64_t* fptr, const int32_t* fnodes,
+                      const int32_t* fpatch, int64_t nc, const int64_t* cptr, const int32_t* cfaces) {
+    auto* m = new Mesh<DIM>();
+    for (int64_t i = 0; i < nn; ++i) {
+        Vec<DIM> p;
+        for (int d = 0; d < DIM; ++d) p[d] = nodes[i * DIM + d];
+        m->add_node(p);
+    }
+    for (int64_t f = 0; f < nf; ++f) m->add_face(fnodes + fptr[f], (size_t)(fptr[f + 1] - fptr[f]), fpatch[f]);
+    for (int64_t c = 0; c < nc; ++c) m->add_cell(cfaces + cptr[c], (size_t)(cptr[c + 1] - cptr[c]));
+    m->compute();
+    return m;
+}
+
+struct MatrixHandle { Matrix m; };
+struct PrecHandle { Prec* p = nullptr; ~PrecHandle() { delete p; } };
+
+Matrix csr_to_matrix(int64_t n, const int32_t* row_ptr, const int32_t* col, const double* val) {
+    Matrix m;
+    for (int64_t i = 0; i < n; ++i) {
+        for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+            m.sp.push_to_major((size_t)col[k]);
+            m.values.push_back(val[k]);
+        }
+        m.sp.close_major();
+    }
+    m.nrows = (size_t)n;
+    m.ncols = m.sp.max_minor + 1;
+    return m;
+}
+}  // namespace
+
+template <int DIM>
+static void geometry_out(const Mesh<DIM>& m, double* farea, double* fcen, double* fnorm, int32_t* owner, int32_t* neigh,
+                         double* cvol, double* ccen) {
+    for (size_t f = 0; f < m.n_faces(); ++f) {
+        const auto& fd = m.face_data[f];
+        if (farea) farea[f] = fd.area;
+        for (int d = 0; d < DIM; ++d) {
+            if (fcen) fcen[f * DIM + d] = fd.center[d];
+            if (fnorm) fnorm[f * DIM + d] = fd.normal[d];
+        }
+        if (owner) owner[f] = (int32_t)fd.owner;
+        if (neigh) neigh[f] = fd.kind == 1 ? (int32_t)fd.neighbor : -1;
+    }
+    for (size_t c = 0; c < m.n_cells(); ++c) {
+        if (cvol) cvol[c] = m.cell_data[c].volume;
+        for (int d = 0; d < DIM; ++d)
+            if (ccen) ccen[c * DIM + d] = m.cell_data[c].center[d];
+    }
+}
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+// Mesh::add_node/add_face/add_cell + Mesh::compute.  Returns NULL on error.
+void* orc_mesh_new(int dim, int64_t nn, const double* nodes, int64_t nf, const int64_t* fptr, const int32_t* fnodes,
+                   const int32_t* fpatch, int64_t nc, const int64_t* cptr, const int32_t* cfaces) {
+    try {
+        auto* h = new MeshHandle();
+        h->dim = dim;
+        if (dim == 2) h->m2 = build_mesh<2>(nn, nodes, nf, fptr, fnodes, fpatch, nc, cptr, cfaces);
+        else if (dim == 3) h->m3 = build_mesh<3>(nn, nodes, nf, fptr, fnodes, fpatch, nc, cptr, cfaces);
+        else { delete h; g_err = "invalid dimension"; return nullptr; }
+        h->bnd_of_face.assign((size_t)nf, NONE);
+        for (int64_t f = 0; f < nf; ++f)
+            if (fpatch[f] >= 0) h->bnd_of_face[(size_t)f] = h->n_bnd++;
+        return h;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void orc_mesh_free(void* h) { delete (MeshHandle*)h; }
+
+int64_t orc_mesh_nnz(void* hv) {
+    auto* h = (MeshHandle*)hv;
+    return h->dim == 2 ? (int64_t)h->m2->cell_to_cell.minor_len() : (int64_t)h->m3->cell_to_cell.minor_len();
+}
+
+void orc_mesh_geometry(void* hv, double* farea, double* fcen, double* fnorm, int32_t* owner, int32_t* neigh,
+                       double* cvol, double* ccen) {
+    auto* h = (MeshHandle*)hv;
+    if (h->dim == 2) geometry_out(*h->m2, farea, fcen, fnorm, owner, neigh, cvol, ccen);
+    else geometry_out(*h->m3, farea, fcen, fnorm, owner, neigh, cvol, ccen);
+}
+
+// assemble(): outputs the sorted CSR pattern (int32), values and rhs.
+int orc_assemble(void* hv, const TermOp* prog, int nops, const double* const* face_fields,
+                 const double* const* cell_fields, const double* bc_i, const double* bc_e, int32_t* row_ptr,
+                 int32_t* col_idx, double* values, double* rhs) {
+    try {
+        auto* h = (MeshHandle*)hv;
+        Matrix A;
+        std::vector<double> b;
+        if (h->dim == 2) Assembler<2>{*h->m2, prog, nops, face_fields, cell_fields}.assemble(A, b, bc_i, bc_e);
+        else Assembler<3>{*h->m3, prog, nops, face_fields, cell_fields}.assemble(A, b, bc_i, bc_e);
+        for (size_t i = 0; i <= A.nrows; ++i) row_ptr[i] = (int32_t)A.sp.major_starts[i];
+        for (size_t k = 0; k < A.sp.minor_len(); ++k) { col_idx[k] = (int32_t)A.sp.minors[k]; values[k] = A.values[k]; }
+        for (size_t i = 0; i < b.size(); ++i) rhs[i] = b[i];
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+void orc_spmv(int64_t n, const int32_t* row_ptr, const int32_t* col, const double* val, const double* x, double* y) {
+    Matrix m = csr_to_matrix(n, row_ptr, col, val);
+    std::vector<double> xx(x, x + m.ncols), yy((size_t)n);
+    m.imul(yy, xx);
+    for (int64_t i = 0; i < n; ++i) y[i] = yy[(size_t)i];
+}
+
+// kind: 0 identity, 1 diagonal, 2 IC(level), 3 ILU(level).
+// status: 0 ok, 4 zero pivot, 5 not symmetric, 1 other.
+void* orc_prec_new(int kind, int64_t n, const int32_t* row_ptr, const int32_t* col, const double* val, int level,
+                   int* status) {
+    *status = 0;
+    try {
+        Matrix m = csr_to_matrix(n, row_ptr, col, val);
+        auto* h = new PrecHandle();
+        if (kind == 0) h->p = new IdentityPrec();
+        else if (kind == 1) h->p = new DiagonalPrec(m);
+        else if (kind == 2) h->p = new ICPrec(m, (size_t)level);
+        else if (kind == 3) h->p = new ILUPrec(m, (size_t)level);
+        else { delete h; *status = 1; g_err = "bad preconditioner kind"; return nullptr; }
+        return h;
+    } catch (std::invalid_argument& e) {
+        g_err = e.what(); *status = 5; return nullptr;
+    } catch (std::domain_error& e) {
+        g_err = e.what(); *status = 4; return nullptr;
+    } catch (std::exception& e) {
+        g_err = e.what(); *status = 1; return nullptr;
+    }
+}
+
+void orc_prec_free(void* h) { delete (PrecHandle*)h; }
+
+// Factor access: IC -> L (lower, with diagonal) and inv_diag; ILU -> LU rows.
+int64_t orc_prec_factor_nnz(void* hv) {
+    auto* h = (PrecHandle*)hv;
+    if (auto* ic = dynamic_cast<ICPrec*>(h->p)) return (int64_t)ic->l.sp.minor_len();
+    if (auto* ilu = dynamic_cast<ILUPrec*>(h->p)) return (int64_t)ilu->lu.sp.minor_len();
+    return 0;
+}
+int orc_prec_factor(void* hv, int32_t* row_ptr, int32_t* col, double* val, double* inv_diag) {
+    auto* h = (PrecHandle*)hv;
+    const Matrix* m = nullptr;
+    if (auto* ic = dynamic_cast<ICPrec*>(h->p)) {
+        m = &ic->l;
+        if (inv_diag) for (size_t i = 0; i < ic->inv_diag.size(); ++i) inv_diag[i] = ic->inv_diag[i];
+    } else if (auto* ilu = dynamic_cast<ILUPrec*>(h->p)) {
+        m = &ilu->lu;
+    } else {
+        return 1;
+    }
+    for (size_t i = 0; i <= m->nrows; ++i) row_ptr[i] = (int32_t)m->sp.major_starts[i];
+    for (size_t k = 0; k < m->sp.minor_len(); ++k) { col[k] = (int32_t)m->sp.minors[k]; val[k] = m->values[k]; }
+    return 0;
+}
+
+void orc_precondition(void* hv, int64_t n, const double* rhs, double* sol) {
+    auto* h = (PrecHandle*)hv;
+    std::vector<double> r(rhs, rhs + n), s(sol, sol + n);
+    h->p->precondition(s, r);
+    for (int64_t i = 0; i < n; ++i) sol[i] = s[(size_t)i];
+}
+
+// solver: 0 CG, 1 BiCGStab.  out_info: [iterations, initial_residual, final_residual, restarts, history_len]
+int orc_solve(int solver, int64_t n, const int32_t* row_ptr, const int32_t* col, const double* val, void* prec,
+              const double* b, double* x, double rtol, double atol, int64_t max_iter, double* out_info,
+              double* history, int64_t history_cap) {
+    try {
+        Matrix m = csr_to_matrix(n, row_ptr, col, val);
+        std::vector<double> bb(b, b + n), xx(x, x + n);
+        const Prec& p = *((PrecHandle*)prec)->p;
+        Info info = solver == 0 ? conjugate_gradient(xx, m, bb, p, rtol, atol, max_iter)
+                                : bicgstab(xx, m, bb, p, rtol, atol, max_iter);
+        for (int64_t i = 0; i < n; ++i) x[i] = xx[(size_t)i];
+        out_info[0] = (double)info.iterations;
+        out_info[1] = info.initial_residual;
+        out_info[2] = info.final_residual;
+        out_info[3] = (double)info.restarts;
+        out_info[4] = (double)info.history.size();
+        if (history)
+            for (size_t k = 0; k < info.history.size() && (int64_t)k < history_cap; ++k) history[k] = info.history[k];
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+}  // extern "C"

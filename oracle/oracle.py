@@ -1,0 +1,197 @@
+"""ctypes binding of the CPU FP64 oracle (oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the
+CPU-baseline leg of bench.py.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+OP = dict(LAPLACIAN_ORTHOGONAL=1, CONVECTION_UPWIND=2, SOURCE=3, LINEAR_SOURCE=4, TIME_EULER=5,
+          DIVERGENCE=6, ADD=16, SUB=17, NEG=18)
+
+
+class TermOp(C.Structure):
+    _fields_ = [("op", C.c_int32), ("arg0", C.c_int32), ("arg1", C.c_int32), ("arg2", C.c_int32),
+                ("scalar", C.c_double)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-B", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.c_void_p
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_mesh_new.restype = P
+        L.orc_mesh_new.argtypes = [C.c_int, C.c_int64, P, C.c_int64, P, P, P, C.c_int64, P, P]
+        L.orc_mesh_free.argtypes = [P]
+        L.orc_mesh_nnz.restype = C.c_int64
+        L.orc_mesh_nnz.argtypes = [P]
+        L.orc_mesh_geometry.argtypes = [P] * 8
+        L.orc_assemble.restype = C.c_int
+        L.orc_assemble.argtypes = [P, P, C.c_int, P, P, P, P, P, P, P, P]
+        L.orc_spmv.argtypes = [C.c_int64, P, P, P, P, P]
+        L.orc_prec_new.restype = P
+        L.orc_prec_new.argtypes = [C.c_int, C.c_int64, P, P, P, C.c_int, C.POINTER(C.c_int)]
+        L.orc_prec_free.argtypes = [P]
+        L.orc_prec_factor_nnz.restype = C.c_int64
+        L.orc_prec_factor_nnz.argtypes = [P]
+        L.orc_prec_factor.restype = C.c_int
+        L.orc_prec_factor.argtypes = [P, P, P, P, P]
+        L.orc_precondition.argtypes = [P, C.c_int64, P, P]
+        L.orc_solve.restype = C.c_int
+        L.orc_solve.argtypes = [C.c_int, C.c_int64, P, P, P, P, P, P, C.c_double, C.c_double, C.c_int64, P, P,
+                                C.c_int64]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _err():
+    return lib().orc_last_error().decode()
+
+
+class OracleMesh:
+    """Mesh built with add_node/add_face/add_cell and Mesh::compute (compute.rs:12-261)."""
+
+    def __init__(self, m):
+        self.dim = int(m["dim"])
+        self._keep = [np.ascontiguousarray(m["nodes"], dtype=np.float64),
+                      np.ascontiguousarray(m["face_ptr"], dtype=np.int64),
+                      np.ascontiguousarray(m["face_nodes"], dtype=np.int32),
+                      np.ascontiguousarray(m["face_patch"], dtype=np.int32),
+                      np.ascontiguousarray(m["cell_ptr"], dtype=np.int64),
+                      np.ascontiguousarray(m["cell_faces"], dtype=np.int32)]
+        nodes, fptr, fn, fp, cptr, cf = self._keep
+        self.n_faces = fptr.shape[0] - 1
+        self.n_cells = cptr.shape[0] - 1
+        self.n_bnd = int((fp >= 0).sum())
+        h = lib().orc_mesh_new(self.dim, nodes.shape[0], _p(nodes), self.n_faces, _p(fptr), _p(fn), _p(fp),
+                               self.n_cells, _p(cptr), _p(cf))
+        if not h:
+            raise RuntimeError("oracle mesh: " + _err())
+        self.h = h
+        self.nnz = int(lib().orc_mesh_nnz(h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_free(self.h)
+            self.h = None
+
+    def geometry(self):
+        F, N, D = self.n_faces, self.n_cells, self.dim
+        g = dict(face_area=np.empty(F), face_center=np.empty((F, D)), face_normal=np.empty((F, D)),
+                 owner=np.empty(F, np.int32), neighbor=np.empty(F, np.int32), cell_volume=np.empty(N),
+                 cell_center=np.empty((N, D)))
+        lib().orc_mesh_geometry(self.h, *[_p(g[k]) for k in ("face_area", "face_center", "face_normal", "owner",
+                                                               "neighbor", "cell_volume", "cell_center")])
+        return g
+
+    def assemble(self, program, face_fields, cell_fields, bc_i, bc_e):
+        """program: list of (op, arg0, arg1, arg2, scalar). Returns (row_ptr, col, values, rhs)."""
+        ops = (TermOp * max(1, len(program)))(*[TermOp(*t) for t in program])
+        ff = [np.ascontiguousarray(a, dtype=np.float64) for a in face_fields]
+        cf = [np.ascontiguousarray(a, dtype=np.float64) for a in cell_fields]
+        ffp = (C.c_void_p * max(1, len(ff)))(*[a.ctypes.data for a in ff])
+        cfp = (C.c_void_p * max(1, len(cf)))(*[a.ctypes.data for a in cf])
+        bi = np.ascontiguousarray(bc_i, dtype=np.float64)
+        be = np.ascontiguousarray(bc_e, dtype=np.float64)
+        rp = np.empty(self.n_cells + 1, np.int32)
+        ci = np.empty(self.nnz, np.int32)
+        va = np.empty(self.nnz)
+        rhs = np.empty(self.n_cells)
+        rc = lib().orc_assemble(self.h, C.cast(ops, C.c_void_p), len(program), C.cast(ffp, C.c_void_p),
+                                C.cast(cfp, C.c_void_p), _p(bi), _p(be), _p(rp), _p(ci), _p(va), _p(rhs))
+        if rc:
+            raise RuntimeError("oracle assemble: " + _err())
+        return rp, ci, va, rhs
+
+
+def spmv(row_ptr, col, val, x):
+    n = row_ptr.shape[0] - 1
+    y = np.empty(n)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    lib().orc_spmv(n, _p(row_ptr), _p(col), _p(val), _p(x), _p(y))
+    return y
+
+
+class ZeroPivot(RuntimeError):
+    pass
+
+
+class NotSymmetric(RuntimeError):
+    pass
+
+
+class OraclePrec:
+    """kind: 'identity', 'diagonal', 'ic', 'ilu' (preconditioners/*.rs)."""
+
+    KINDS = dict(identity=0, diagonal=1, ic=2, ilu=3)
+
+    def __init__(self, kind, row_ptr, col, val, level=0):
+        self.kind = kind
+        self.n = row_ptr.shape[0] - 1
+        self._keep = (np.ascontiguousarray(row_ptr, np.int32), np.ascontiguousarray(col, np.int32),
+                      np.ascontiguousarray(val, np.float64))
+        st = C.c_int(0)
+        h = lib().orc_prec_new(self.KINDS[kind], self.n, *[_p(a) for a in self._keep], level, C.byref(st))
+        if not h:
+            if st.value == 4:
+                raise ZeroPivot(_err())
+            if st.value == 5:
+                raise NotSymmetric(_err())
+            raise RuntimeError("oracle preconditioner: " + _err())
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_prec_free(self.h)
+            self.h = None
+
+    def factor(self):
+        nnz = int(lib().orc_prec_factor_nnz(self.h))
+        rp = np.empty(self.n + 1, np.int32)
+        ci = np.empty(nnz, np.int32)
+        va = np.empty(nnz)
+        inv = np.empty(self.n)
+        lib().orc_prec_factor(self.h, _p(rp), _p(ci), _p(va), _p(inv))
+        return rp, ci, va, inv
+
+    def apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros(self.n)
+        lib().orc_precondition(self.h, self.n, _p(r), _p(z))
+        return z
+
+
+def solve(solver, row_ptr, col, val, prec, b, x0=None, rtol=1e-8, atol=0.0, max_iter=1000):
+    """solver: 'cg' or 'bicgstab'. Returns (x, info dict)."""
+    n = row_ptr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.zeros(5)
+    hist = np.zeros(max_iter + 2)
+    rc = lib().orc_solve(0 if solver == "cg" else 1, n, _p(row_ptr), _p(col), _p(val), prec.h, _p(b), _p(x), rtol,
+                         atol, max_iter, _p(out), _p(hist), hist.shape[0])
+    if rc:
+        raise RuntimeError("oracle solve: " + _err())
+    hl = int(out[4])
+    return x, dict(iterations=int(out[0]), initial_residual=float(out[1]), final_residual=float(out[2]),
+                   restarts=int(out[3]), history=hist[:hl].copy())

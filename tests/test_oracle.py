@@ -1,0 +1,336 @@
+"""Pins the CPU FP64 oracle before it is trusted as the parity reference.
+
+The reference crate's own tests hold no golden vectors for assembly or the solvers;
+its single mesh test (src/core/mesh/io.rs:569-609) runs on square.su2.  Everything
+else is pinned by what the mathematics fixes: closed forms (including the crate's
+3D face-normal and volume scaling, compute.rs:305 and :359-361), defining properties
+of the factorisations, exact solutions of TPFA on orthogonal grids, and an
+independent direct solver (scipy).
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+import meshgen
+import oracle as O
+import problems
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EPS = np.finfo(np.float64).eps
+
+
+def _su2():
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    import su2
+    return su2.read(os.path.join(HERE, "golden", "square.su2"))
+
+
+def test_square_su2_reference_invariants():
+    """io.rs:569-589: 121 nodes, 220 faces, 100 cells, sum vol = 1 and sum_bnd n*A = 0 within 10 eps."""
+    m = _su2()
+    assert m["nodes"].shape[0] == 121
+    om = O.OracleMesh(m)
+    assert om.n_faces == 220 and om.n_cells == 100
+    g = om.geometry()
+    assert abs(g["cell_volume"].sum() - 1.0) < EPS * 10.0
+    b = m["face_patch"] >= 0
+    assert np.linalg.norm((g["face_normal"][b] * g["face_area"][b, None]).sum(axis=0)) < EPS * 10.0
+
+
+def _single_cell_mesh(nodes, faces):
+    fptr = np.zeros(len(faces) + 1, np.int64)
+    fptr[1:] = np.cumsum([len(f) for f in faces])
+    return dict(dim=nodes.shape[1], nodes=np.asarray(nodes, np.float64), face_ptr=fptr,
+                face_nodes=np.array([n for f in faces for n in f], np.int32),
+                face_patch=np.zeros(len(faces), np.int32), cell_ptr=np.array([0, len(faces)], np.int64),
+                cell_faces=np.arange(len(faces), dtype=np.int32))
+
+
+def test_closed_form_triangle():
+    m = _single_cell_mesh(np.array([[0, 0], [1, 0], [0, 1]], float), [(0, 1), (1, 2), (2, 0)])
+    g = O.OracleMesh(m).geometry()
+    assert np.allclose(g["cell_volume"], [0.5], rtol=0, atol=1e-15)
+    assert np.allclose(g["cell_center"][0], [1 / 3, 1 / 3], atol=1e-15)
+    assert np.allclose(np.linalg.norm(g["face_normal"], axis=1), 1.0, atol=1e-15)   # 2D normals are unit
+    assert np.allclose(sorted(g["face_area"]), [1, 1, np.sqrt(2)], atol=1e-15)
+    # outward normals
+    d = g["face_center"] - g["cell_center"][0]
+    assert np.all((d * g["face_normal"]).sum(1) > 0)
+
+
+def test_closed_form_tet_normal_and_volume_scaling():
+    """3D normals have |n| = 1/3 on triangles (compute.rs:305); volume = true/3 (compute.rs:359-361)."""
+    nodes = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    m = _single_cell_mesh(nodes, [(0, 2, 1), (0, 1, 3), (1, 2, 3), (0, 3, 2)])
+    g = O.OracleMesh(m).geometry()
+    assert np.allclose(np.linalg.norm(g["face_normal"], axis=1), 1.0 / 3.0, atol=1e-15)
+    assert np.allclose(sorted(g["face_area"]), [0.5, 0.5, 0.5, np.sqrt(3) / 2], atol=1e-15)
+    assert abs(g["cell_volume"][0] - (1.0 / 6.0) / 3.0) < 1e-16
+    assert np.allclose(g["cell_center"][0], [0.25, 0.25, 0.25], atol=1e-15)
+
+
+def test_closed_form_prism_and_hex_scaling():
+    """Quad faces carry |n| = 1/4; a right prism's volume is 5/18 of true, a cube's 1/4."""
+    pr = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [0, 1, 1]], float)
+    m = _single_cell_mesh(pr, [(0, 1, 2), (3, 4, 5), (0, 1, 4, 3), (1, 2, 5, 4), (2, 0, 3, 5)])
+    g = O.OracleMesh(m).geometry()
+    nn = np.linalg.norm(g["face_normal"], axis=1)
+    assert np.allclose(nn[:2], 1 / 3, atol=1e-15) and np.allclose(nn[2:], 1 / 4, atol=1e-15)
+    assert abs(g["cell_volume"][0] - 0.5 * 5.0 / 18.0) < 1e-15
+    hx = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    m = _single_cell_mesh(hx, [(0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)])
+    g = O.OracleMesh(m).geometry()
+    assert abs(g["cell_volume"][0] - 0.25) < 1e-15
+
+
+@pytest.mark.parametrize("gen", [lambda: meshgen.tri2d(12, seed=1), lambda: meshgen.tet3d(3, seed=2),
+                                 lambda: meshgen.prism3d(5, layers=3, seed=3)])
+def test_generated_meshes_volume_and_closure(gen):
+    m = gen()
+    g = O.OracleMesh(m).geometry()
+    scale = {2: 1.0}.get(m["dim"], None)
+    vol = g["cell_volume"].sum()
+    if m["dim"] == 2:
+        assert abs(vol - 1.0) < 1e-13
+    elif "tet" in m["name"]:
+        assert abs(vol - 1.0 / 3.0) < 1e-13
+    else:
+        assert abs(vol - 5.0 / 18.0) < 1e-13
+    # every cell closed: sum of outward |n|-scaled area vectors is zero up to the per-face scaling
+    b = m["face_patch"] >= 0
+    assert scale is None or np.linalg.norm((g["face_normal"][b] * g["face_area"][b, None]).sum(0)) < 1e-13
+
+
+def test_pattern_brute_force_and_counts():
+    """Row i = sorted({i} U face neighbours); nnz = n + 2 F_int (cfg1 size from the workload table)."""
+    m = meshgen.tri2d(64, seed=1)
+    s = meshgen.mesh_sizes(m)
+    assert (s["n"], s["f_int"], s["f_bnd"], s["nnz"]) == (8192, 12160, 256, 32512)
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    prob = problems.diffusion(m, g)
+    rp, ci, va, b = om.assemble(prob["prog"], prob["face_fields"], prob["cell_fields"], prob["bc_i"], prob["bc_e"])
+    nbrs = [{i} for i in range(s["n"])]
+    for o, nb in zip(g["owner"], g["neighbor"]):
+        if nb >= 0:
+            nbrs[o].add(nb)
+            nbrs[nb].add(o)
+    for i in range(s["n"]):
+        assert list(ci[rp[i]:rp[i + 1]]) == sorted(nbrs[i])
+
+
+def test_tpfa_five_point_stencil_on_square_su2():
+    """Uniform 10x10 quads, k = 1: A_ij = -1 for face neighbours, A_ii = #faces + #boundary faces."""
+    m = _su2()
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    n = om.n_cells
+    prob = problems.diffusion(m, g, kc=np.ones(n), f=np.zeros(n))
+    rp, ci, va, b = om.assemble(prob["prog"], prob["face_fields"], prob["cell_fields"], prob["bc_i"], prob["bc_e"])
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n)).toarray()
+    nb = np.bincount(g["owner"][m["face_patch"] >= 0], minlength=n)
+    off = A - np.diag(np.diag(A))
+    assert np.allclose(off[off != 0], -1.0, atol=1e-11)
+    assert np.allclose(np.diag(A), 4.0 + nb, atol=1e-11)
+    assert np.abs(A - A.T).max() == 0.0
+
+
+def test_tpfa_exact_for_linear_solution():
+    """u = x with Dirichlet u = x_f on an orthogonal grid: TPFA reproduces u_c = x_c."""
+    m = _su2()
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    n = om.n_cells
+    prob = problems.diffusion(m, g, kc=np.ones(n), f=np.zeros(n), bc="x")
+    rp, ci, va, b = om.assemble(prob["prog"], prob["face_fields"], prob["cell_fields"], prob["bc_i"], prob["bc_e"])
+    P = O.OraclePrec("ic", rp, ci, va, 0)
+    x, info = O.solve("cg", rp, ci, va, P, b, rtol=1e-12, atol=0.0)
+    assert np.abs(x - g["cell_center"][:, 0]).max() < 1e-10
+
+
+def test_laplacian_properties_on_jittered_mesh():
+    m = meshgen.tet3d(5, seed=4)
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    n = om.n_cells
+    kc = meshgen.lognormal_cells(n, 1.5, seed=3)
+    prob = problems.diffusion(m, g, kc=kc, f=np.zeros(n))
+    rp, ci, va, b = om.assemble(prob["prog"], prob["face_fields"], prob["cell_fields"], prob["bc_i"], prob["bc_e"])
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    assert abs(A - A.T).max() == 0.0
+    d = A.diagonal()
+    off = A - sp.diags(d)
+    assert (off.data <= 0).all() and (d > 0).all()       # M-matrix sign pattern
+    interior = np.ones(n, bool)
+    interior[g["owner"][m["face_patch"] >= 0]] = False
+    rs = np.asarray(A.sum(axis=1)).ravel()
+    assert np.abs(rs[interior]).max() < 1e-12 * d.max()   # zero interior row sums
+
+
+def test_upwind_convection_column_sums():
+    """Upwind convection is conservative: every interior face adds +F and -F to the same column, so
+    column c sums to the outflow through c's boundary faces (inflow closed by bndi = bnde = 0)."""
+    m = meshgen.tri2d(8, seed=2)
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    F = om.n_faces
+    flux = np.random.default_rng(1).normal(size=F)
+    nb = om.n_bnd
+    prog = [(problems.CONV, 0, 0, 0, 0.0)]
+    rp, ci, va, b = om.assemble(prog, [flux], [], np.zeros(nb), np.zeros(nb))
+    n = om.n_cells
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    fa = flux * g["face_area"]
+    bo = g["neighbor"] < 0
+    net = np.zeros(n)
+    np.add.at(net, g["owner"][bo], np.where(fa[bo] > 0, fa[bo], 0.0))
+    cs = np.asarray(A.sum(axis=0)).ravel()
+    assert np.allclose(cs, net, atol=1e-13)
+    assert np.abs(net).max() > 0.1
+
+
+def test_spmv_matches_dense_product():
+    m = meshgen.tri2d(10, seed=2)
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    prob = problems.convection_diffusion(m, g, seed=1)
+    rp, ci, va, b = om.assemble(prob["prog"], prob["face_fields"], prob["cell_fields"], prob["bc_i"], prob["bc_e"])
+    n = om.n_cells
+    x = np.random.default_rng(3).normal(size=n)
+    y = O.spmv(rp, ci, va, x)
+    ref = sp.csr_matrix((va, ci, rp), shape=(n, n)).toarray() @ x
+    assert np.allclose(y, ref, rtol=1e-14, atol=1e-14 * np.abs(ref).max())
+
+
+def _tridiag(n, seed):
+    rng = np.random.default_rng(seed)
+    off = -rng.uniform(0.5, 1.0, n - 1)
+    d = np.abs(np.concatenate([[0], off])) + np.abs(np.concatenate([off, [0]])) + rng.uniform(0.1, 1.0, n)
+    A = np.diag(d) + np.diag(off, 1) + np.diag(off, -1)
+    S = sp.csr_matrix(A)
+    S.sort_indices()
+    return A, S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.copy()
+
+
+def test_ic0_equals_cholesky_on_a_chain():
+    """No fill on a tridiagonal matrix: IC(0) is the exact Cholesky factor; PCG converges in one step."""
+    A, rp, ci, va = _tridiag(40, 1)
+    P = O.OraclePrec("ic", rp, ci, va, 0)
+    lrp, lci, lva, inv = P.factor()
+    L = sp.csr_matrix((lva, lci, lrp), shape=A.shape).toarray()
+    Lref = np.linalg.cholesky(A)
+    assert np.allclose(L, Lref, rtol=1e-14, atol=1e-14)
+    assert np.allclose(inv, 1.0 / np.diag(Lref), rtol=1e-14)
+    b = np.random.default_rng(2).normal(size=40)
+    x, info = O.solve("cg", rp, ci, va, P, b, rtol=1e-10, atol=0.0)
+    assert info["iterations"] == 1
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-12)
+
+
+def test_ic0_defining_property_on_mesh_matrix():
+    """(L L^T)_ij = A_ij on the pattern of A."""
+    m = meshgen.tri2d(10, seed=5)
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    prob = problems.diffusion(m, g, kc=meshgen.lognormal_cells(om.n_cells, 1.0, 1))
+    rp, ci, va, b = om.assemble(prob["prog"], prob["face_fields"], prob["cell_fields"], prob["bc_i"], prob["bc_e"])
+    n = om.n_cells
+    P = O.OraclePrec("ic", rp, ci, va, 0)
+    lrp, lci, lva, _ = P.factor()
+    L = sp.csr_matrix((lva, lci, lrp), shape=(n, n)).toarray()
+    LLt = L @ L.T
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n)).toarray()
+    mask = A != 0
+    assert np.allclose(LLt[mask], A[mask], rtol=1e-12, atol=1e-12 * np.abs(A).max())
+    assert not np.allclose(LLt, A)   # but not a complete factorisation
+    # preconditioner apply = (L L^T)^-1 r
+    r = np.random.default_rng(4).normal(size=n)
+    assert np.allclose(P.apply(r), np.linalg.solve(LLt, r), rtol=1e-10)
+
+
+def test_ilu0_equals_lu_on_a_chain_and_defining_property():
+    A, rp, ci, va = _tridiag(30, 3)
+    A = A + np.diag(0.3 * np.ones(29), 1)   # make it nonsymmetric
+    S = sp.csr_matrix(A)
+    S.sort_indices()
+    rp, ci, va = S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.copy()
+    P = O.OraclePrec("ilu", rp, ci, va, 0)
+    lrp, lci, lva, _ = P.factor()
+    LU = sp.csr_matrix((lva, lci, lrp), shape=A.shape).toarray()
+    L = np.tril(LU, -1) + np.eye(30)
+    U = np.triu(LU)
+    assert np.allclose(L @ U, A, rtol=1e-13, atol=1e-13)   # exact LU (no fill on a chain)
+    r = np.random.default_rng(5).normal(size=30)
+    assert np.allclose(P.apply(r), np.linalg.solve(A, r), rtol=1e-11)
+
+
+def test_ilu0_drop_rule():
+    """Entries with |v| <= 1e-6 |u_ii| are removed after the row is eliminated (incomplete_lower_upper.rs:255-258)."""
+    A = np.array([[4.0, 1e-7, 1.0], [1e-7 * 4, 4.0, 1.0], [1.0, 1.0, 4.0]])
+    S = sp.csr_matrix(A)
+    S.sort_indices()
+    P = O.OraclePrec("ilu", S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.copy(), 0)
+    lrp, lci, lva, _ = P.factor()
+    assert list(lci[lrp[0]:lrp[1]]) == [0, 2]       # 1e-7 <= 1e-6 * 4 dropped from row 0
+    assert list(lci[lrp[1]:lrp[2]]) == [1, 2]       # l_10 = 1e-7 dropped from row 1
+    assert list(lci[lrp[2]:lrp[3]]) == [0, 1, 2]
+
+
+@pytest.mark.parametrize("solver,prec,kind", [("cg", "ic", "sym"), ("cg", "diagonal", "sym"),
+                                              ("bicgstab", "ilu", "nonsym"), ("bicgstab", "identity", "nonsym")])
+def test_solvers_against_direct_solve(solver, prec, kind):
+    m = meshgen.tri2d(24, seed=6)
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    if kind == "sym":
+        prob = problems.diffusion(m, g, kc=meshgen.lognormal_cells(om.n_cells, 1.0, 2), seed=5)
+    else:
+        prob = problems.convection_diffusion(m, g, pe=(1, 20), seed=2)
+    rp, ci, va, b = om.assemble(prob["prog"], prob["face_fields"], prob["cell_fields"], prob["bc_i"], prob["bc_e"])
+    n = om.n_cells
+    P = O.OraclePrec(prec, rp, ci, va, 0)
+    x, info = O.solve(solver, rp, ci, va, P, b, rtol=1e-10, atol=0.0, max_iter=5000)
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    xs = spl.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xs) <= 1e-6 * np.linalg.norm(xs)
+    assert np.linalg.norm(b - A @ x) <= 1e-8 * np.linalg.norm(b)
+    assert info["history"].shape[0] == info["iterations"]
+
+
+def test_bicgstab_divides_by_bnorm_twice_in_loop_test():
+    """Q4: the while test sees |r|/|b|^2 (bi_conjugate_gradient_stab.rs:139, 171); with |b| = 1e4 and
+    absolute_tolerance 1e-6 the loop leaves while |r|/|b| is still ~1e-2."""
+    m = meshgen.tri2d(24, seed=6)
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    prob = problems.convection_diffusion(m, g, pe=(1, 20), seed=2)
+    rp, ci, va, b = om.assemble(prob["prog"], prob["face_fields"], prob["cell_fields"], prob["bc_i"], prob["bc_e"])
+    b = b / np.linalg.norm(b) * 1e4
+    P = O.OraclePrec("identity", rp, ci, va, 0)
+    x, info = O.solve("bicgstab", rp, ci, va, P, b, rtol=0.0, atol=1e-6, max_iter=5000)
+    n = om.n_cells
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    rel = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    assert info["final_residual"] <= 1e-6
+    assert rel > 1e-5   # a single division would have iterated down to 1e-6
+    # the recorded history is |r|/|b|^2
+    assert info["history"][-1] <= 1e-6 and info["history"][-1] * 1e4 == pytest.approx(rel, rel=1e-6)
+
+
+def test_cg_runs_max_iterations_plus_one():
+    A, rp, ci, va = _tridiag(200, 7)
+    b = np.random.default_rng(1).normal(size=200)
+    P = O.OraclePrec("identity", rp, ci, va, 0)
+    x, info = O.solve("cg", rp, ci, va, P, b, rtol=1e-30, atol=0.0, max_iter=5)
+    assert info["iterations"] == 6
+
+
+def test_ic_rejects_nonsymmetric():
+    A = np.array([[4.0, 1.0], [1.5, 4.0]])
+    S = sp.csr_matrix(A)
+    with pytest.raises(O.NotSymmetric):
+        O.OraclePrec("ic", S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.copy(), 0)
