@@ -378,6 +378,7 @@ int fvg_assemble(fvg_ctx* c, const fvg_mesh* m, const fvg_term_op* program, int3
                            rhs->d);
     }
     FVG_HIP(hipGetLastError());
+    lhs->version += 1;
     return FVG_OK;
 }
 

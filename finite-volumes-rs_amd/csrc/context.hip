@@ -137,9 +137,11 @@ static int collect_spans(fvg_ctx* ctx) {
             if (s.kind == kSpanSpmv) {
                 ctx->stats.spmv_ms += ms;
                 ctx->stats.spmv_launches += s.launches;
+                ctx->stats.spmv_calls += 1;
             } else {
                 ctx->stats.sweep_ms += ms;
                 ctx->stats.sweep_launches += s.launches;
+                ctx->stats.sweep_calls += 1;
             }
         }
         (void)hipEventDestroy(s.a);
@@ -311,6 +313,11 @@ int fvg_vec_upload(fvg_ctx* c, fvg_vec* v, const double* host) {
 int fvg_vec_download(fvg_ctx* c, const fvg_vec* v, double* host) {
     if (!c || !v || (!host && v->n)) return fail(FVG_E_INVALID, "null argument");
     return download(c, host, v->d, v->n);
+}
+
+int fvg_vec_fill(fvg_ctx* c, fvg_vec* v, double value) {
+    if (!c || !v) return fail(FVG_E_INVALID, "null argument");
+    return launch_fill(c, v->n, v->d, value);
 }
 
 int fvg_vec_dot(fvg_ctx* c, const fvg_vec* a, const fvg_vec* b, double* out) {

@@ -102,6 +102,7 @@ int launch_spmv(fvg_ctx* ctx, const fvg_csr* a, const double* x, double* y, cons
 }
 
 static void free_csr(fvg_ctx* c, fvg_csr* a) {
+    dev_free_n(c, a->val32, a->nnz);
     dev_free_n(c, a->row_ptr, a->nrows + 1);
     dev_free_n(c, a->col, a->nnz);
     dev_free_n(c, a->val, a->nnz);
@@ -184,6 +185,7 @@ int fvg_csr_upload_values(fvg_ctx* c, fvg_csr* a, const double* values) {
     if (!c || !a || (!values && a->nnz)) return fail(FVG_E_INVALID, "null argument");
     FVG_TRY(upload(c, a->val, values, a->nnz));
     FVG_HIP(hipStreamSynchronize(c->stream));
+    a->version += 1;
     return FVG_OK;
 }
 

@@ -68,6 +68,9 @@ struct fvg_csr {
     // host copy of the pattern, used by the preconditioner analysis
     std::vector<int32_t> h_row_ptr, h_col;
     uint64_t pattern_id = 0;           // identifies a mesh pattern (assembly target check)
+    uint64_t version = 0;              // bumped whenever the values change
+    float* val32 = nullptr;            // FP32 copy of the values for mixed precision (made on use)
+    uint64_t val32_version = ~0ull;
 };
 
 namespace fvg {

@@ -365,8 +365,8 @@ static int rhs_norm(fvg_ctx* c, const fvg_vec* b, double* bnorm) {
 
 // iterations enqueued between two polls of the device state
 static int poll_batch(const fvg_prec* p) {
-    const int levels = p->lower.nlevels + p->upper.nlevels;
-    return levels > 16 ? 1 : 4;
+    (void)p;
+    return 4;
 }
 
 int run_cg(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
@@ -424,6 +424,7 @@ int run_cg(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fv
             if (h.done || h.halt) break;
         }
     }
+    FVG_TRY(check_sweep_status(c, p));
     return K.finish(info, history, cap);
 }
 
@@ -503,8 +504,13 @@ int run_bicgstab(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec*
             if (h.done || h.halt) break;
         }
     }
+    FVG_TRY(check_sweep_status(c, p));
     return K.finish(info, history, cap);
 }
+
+// mixed.hip
+int run_mixed(fvg_ctx* c, fvg_csr* a, fvg_prec* p, const fvg_vec* b, fvg_vec* x, const fvg_solver_options* o,
+              fvg_solver_info* info, double* history, int64_t cap, bool bicgstab);
 
 }  // namespace fvg
 
@@ -515,14 +521,17 @@ extern "C" {
 int fvg_cg(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
            const fvg_solver_options* o, fvg_solver_info* info, double* history, int64_t cap) {
     FVG_TRY(check_solve_args(c, a, p, b, x, o));
-    if (o->precision == FVG_PRECISION_MIXED) return fail(FVG_E_UNSUPPORTED, "mixed precision not available yet");
+    // the FP32 copies of the matrix values and factors are caches owned by a and p
+    if (o->precision == FVG_PRECISION_MIXED)
+        return run_mixed(c, const_cast<fvg_csr*>(a), const_cast<fvg_prec*>(p), b, x, o, info, history, cap, false);
     return run_cg(c, a, p, b, x, o, info, history, cap);
 }
 
 int fvg_bicgstab(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
                  const fvg_solver_options* o, fvg_solver_info* info, double* history, int64_t cap) {
     FVG_TRY(check_solve_args(c, a, p, b, x, o));
-    if (o->precision == FVG_PRECISION_MIXED) return fail(FVG_E_UNSUPPORTED, "mixed precision not available yet");
+    if (o->precision == FVG_PRECISION_MIXED)
+        return run_mixed(c, const_cast<fvg_csr*>(a), const_cast<fvg_prec*>(p), b, x, o, info, history, cap, true);
     return run_bicgstab(c, a, p, b, x, o, info, history, cap);
 }
 

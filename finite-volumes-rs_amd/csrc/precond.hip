@@ -17,6 +17,7 @@
 // incomplete_lower_upper.rs:291-328).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "precond.hpp"
@@ -41,6 +42,7 @@ __global__ void k_ic0_level(int64_t begin, int64_t end, const int32_t* __restric
     const int64_t p = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= end) return;
     const int i = rows[p];
+    if (i < 0) return;
     const int s = rp[i], d = diag[i];
     for (int k = s; k < d; ++k) {
         const int j = col[k];
@@ -76,6 +78,7 @@ __global__ void k_ilu0_level(int64_t begin, int64_t end, const int32_t* __restri
     const int64_t p = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= end) return;
     const int i = rows[p];
+    if (i < 0) return;
     const int s = rp[i], e = rp[i + 1], d = diag[i];
     for (int k = s; k < e; ++k) lu[k] = a[k];
     for (int k = s; k < d; ++k) {
@@ -126,10 +129,127 @@ __global__ void k_sweep_level(int64_t begin, int64_t end, const int32_t* __restr
     const int64_t p = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= end) return;
     const int i = rows[p];
+    if (i < 0) return;
     double s = b[i];
     const int e = ptr[p + 1];
     for (int k = ptr[p]; k < e; ++k) s -= val[k] * x[col[k]];
     x[i] = scale ? scale[i] * s : s;
+}
+
+// ---- single-launch sweep -----------------------------------------------------------
+// Every row of the sweep is one lane; a wave64 owns one chunk of 64 positions of one
+// level, and chunks are numbered in level order, so every row a lane waits for lives
+// in a chunk with a smaller number.  A workgroup holds kWavesPerGroup consecutive
+// chunks and workgroups are dispatched in increasing order on each XCD, so whatever a
+// running wave waits for has already been dispatched and is itself only waiting on
+// earlier chunks: the sweep always drains.
+//
+// The output vector doubles as the completion flag: it is filled with a signalling
+// NaN bit pattern (kPending) that no arithmetic result can carry, and each row's value
+// is published with one 64-bit agent-scope store; dependants poll that word with
+// agent-scope loads, which are coherent across XCDs.  Every wait is bounded: after
+// kSpinLimit polls (or once another lane has timed out) the wait gives up, flags the
+// error and lets the sweep finish, so a kernel never hangs.
+constexpr uint64_t kPending = 0x7FF4DEADBEEF0F64ull;
+constexpr int kSpinLimit = 1 << 18;
+constexpr int kWavesPerGroup = 4;
+
+__global__ void k_fill_pending(int64_t n, uint64_t* __restrict__ x, int32_t* __restrict__ err,
+                               const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = kPending;
+    (void)err;
+}
+
+__device__ __forceinline__ double wait_value(const double* x, int j, int32_t* err) {
+    const uint64_t* p = reinterpret_cast<const uint64_t*>(x) + j;
+    uint64_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int spins = 0;
+    while (v == kPending) {
+        if (++spins > kSpinLimit ||
+            ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return __longlong_as_double((long long)v);
+}
+
+__global__ void __launch_bounds__(kChunk * kWavesPerGroup)
+    k_sweep_syncfree(int64_t npos, const int32_t* __restrict__ rows, const int32_t* __restrict__ ptr,
+                     const int32_t* __restrict__ col, const double* __restrict__ val,
+                     const double* __restrict__ scale, const double* __restrict__ b, double* x,
+                     int32_t* __restrict__ err, const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int64_t p = (int64_t)blockIdx.x * (kChunk * kWavesPerGroup) + threadIdx.x;
+    if (p >= npos) return;
+    const int i = rows[p];
+    if (i < 0) return;
+    double s = b[i];
+    const int e = ptr[p + 1];
+    for (int k = ptr[p]; k < e; ++k) s -= val[k] * wait_value(x, col[k], err);
+    const double r = scale ? scale[i] * s : s;
+    __hip_atomic_store(reinterpret_cast<uint64_t*>(x) + i, (uint64_t)__double_as_longlong(r), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// FP32 form of the single-launch sweep (mixed-precision inner iterations); the pending
+// marker is a signalling-NaN float pattern.
+constexpr uint32_t kPending32 = 0x7FB0DEADu;
+
+__global__ void k_fill_pending32(int64_t n, uint32_t* __restrict__ x, const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = kPending32;
+}
+
+__device__ __forceinline__ float wait_value32(const float* x, int j, int32_t* err) {
+    const uint32_t* p = reinterpret_cast<const uint32_t*>(x) + j;
+    uint32_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int spins = 0;
+    while (v == kPending32) {
+        if (++spins > kSpinLimit ||
+            ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return __uint_as_float(v);
+}
+
+__global__ void __launch_bounds__(kChunk * kWavesPerGroup)
+    k_sweep_syncfree32(int64_t npos, const int32_t* __restrict__ rows, const int32_t* __restrict__ ptr,
+                       const int32_t* __restrict__ col, const float* __restrict__ val,
+                       const float* __restrict__ scale, const float* __restrict__ b, float* x,
+                       int32_t* __restrict__ err, const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int64_t p = (int64_t)blockIdx.x * (kChunk * kWavesPerGroup) + threadIdx.x;
+    if (p >= npos) return;
+    const int i = rows[p];
+    if (i < 0) return;
+    float s = b[i];
+    const int e = ptr[p + 1];
+    for (int k = ptr[p]; k < e; ++k) s -= val[k] * wait_value32(x, col[k], err);
+    const float r = scale ? scale[i] * s : s;
+    __hip_atomic_store(reinterpret_cast<uint32_t*>(x) + i, __float_as_uint(r), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_to_f32(int64_t n, const double* __restrict__ s, float* __restrict__ d) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (float)s[i];
+}
+
+__global__ void k_scale32(int64_t n, const float* __restrict__ d, const float* __restrict__ r, float* __restrict__ z,
+                          const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = d ? d[i] * r[i] : r[i];
 }
 
 // Jacobi (d = 1/a_ii, DiagonalPreconditioner mod.rs:60-66) or identity copy (d == nullptr, mod.rs:25-31)
@@ -143,8 +263,9 @@ __global__ void k_scale(int64_t n, const double* __restrict__ d, const double* _
 // ---- host analysis ----------------------------------------------------------------------
 
 static void free_sweep(fvg_ctx* c, Sweep& s) {
-    dev_free_n(c, s.rows, s.n);
-    dev_free_n(c, s.ptr, s.n + 1);
+    dev_free_n(c, s.val32, s.nnz);
+    dev_free_n(c, s.rows, s.npos);
+    dev_free_n(c, s.ptr, s.npos + 1);
     dev_free_n(c, s.col, s.nnz);
     dev_free_n(c, s.src, s.nnz);
     dev_free_n(c, s.val, s.nnz);
@@ -176,33 +297,38 @@ static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::ve
     }
     sw.n = n;
     sw.nlevels = maxlev + 1;
+    std::vector<int64_t> count((size_t)sw.nlevels, 0);
+    for (int64_t i = 0; i < n; ++i) count[(size_t)lev[(size_t)i]]++;
     sw.lvl_ptr.assign((size_t)sw.nlevels + 1, 0);
-    for (int64_t i = 0; i < n; ++i) sw.lvl_ptr[(size_t)lev[(size_t)i] + 1]++;
-    for (int l = 0; l < sw.nlevels; ++l) sw.lvl_ptr[(size_t)l + 1] += sw.lvl_ptr[(size_t)l];
-    std::vector<int32_t> rows((size_t)n);
+    for (int l = 0; l < sw.nlevels; ++l)
+        sw.lvl_ptr[(size_t)l + 1] = sw.lvl_ptr[(size_t)l] + (count[(size_t)l] + kChunk - 1) / kChunk * kChunk;
+    sw.npos = sw.lvl_ptr[(size_t)sw.nlevels];
+    std::vector<int32_t> rows((size_t)sw.npos, -1);
     {
         std::vector<int64_t> cur(sw.lvl_ptr.begin(), sw.lvl_ptr.end() - 1);
         for (int64_t i = 0; i < n; ++i) rows[(size_t)cur[(size_t)lev[(size_t)i]]++] = (int32_t)i;   // ascending rows
     }
-    std::vector<int32_t> ptr((size_t)n + 1, 0), ecol, esrc;
+    std::vector<int32_t> ptr((size_t)sw.npos + 1, 0), ecol, esrc;
     ecol.reserve(col.size() / 2 + 1);
     esrc.reserve(col.size() / 2 + 1);
-    for (int64_t p = 0; p < n; ++p) {
+    for (int64_t p = 0; p < sw.npos; ++p) {
         const int32_t i = rows[(size_t)p];
-        for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
-            const int32_t j = col[(size_t)k];
-            if (!take(i, j)) continue;
-            ecol.push_back(j);
-            esrc.push_back(src_map ? (*src_map)[(size_t)k] : k);
+        if (i >= 0) {
+            for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
+                const int32_t j = col[(size_t)k];
+                if (!take(i, j)) continue;
+                ecol.push_back(j);
+                esrc.push_back(src_map ? (*src_map)[(size_t)k] : k);
+            }
         }
         ptr[(size_t)p + 1] = (int32_t)ecol.size();
     }
     sw.nnz = (int64_t)ecol.size();
     int s;
-    if ((s = dev_alloc_n(c, n, &sw.rows)) || (s = dev_alloc_n(c, n + 1, &sw.ptr)) ||
+    if ((s = dev_alloc_n(c, sw.npos, &sw.rows)) || (s = dev_alloc_n(c, sw.npos + 1, &sw.ptr)) ||
         (s = dev_alloc_n(c, sw.nnz, &sw.col)) || (s = dev_alloc_n(c, sw.nnz, &sw.src)) ||
-        (s = dev_alloc_n(c, sw.nnz, &sw.val)) || (s = upload(c, sw.rows, rows.data(), n)) ||
-        (s = upload(c, sw.ptr, ptr.data(), n + 1)) || (s = upload(c, sw.col, ecol.data(), sw.nnz)) ||
+        (s = dev_alloc_n(c, sw.nnz, &sw.val)) || (s = upload(c, sw.rows, rows.data(), sw.npos)) ||
+        (s = upload(c, sw.ptr, ptr.data(), sw.npos + 1)) || (s = upload(c, sw.col, ecol.data(), sw.nnz)) ||
         (s = upload(c, sw.src, esrc.data(), sw.nnz)))
         return s;
     return FVG_OK;
@@ -217,6 +343,8 @@ static void free_prec(fvg_ctx* c, fvg_prec* p) {
     dev_free_n(c, p->d_inv_diag, p->n);
     dev_free_n(c, p->d_tmp, p->n);
     dev_free_n(c, p->d_flags, 4);
+    dev_free_n(c, p->d_inv_diag32, p->n);
+    dev_free_n(c, p->d_tmp32, p->n);
     free_sweep(c, p->lower);
     free_sweep(c, p->upper);
 }
@@ -307,19 +435,106 @@ static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a) {
                        p->d_fac, p->upper.val);
     FVG_HIP(hipGetLastError());
     FVG_HIP(hipStreamSynchronize(c->stream));
+    p->fac_version += 1;
     return FVG_OK;
 }
 
-static int sweep(fvg_ctx* c, const Sweep& sw, const double* scale, const double* b, double* x,
+// FVG_SWEEP=levels selects one launch per level instead of the single-launch sweep.
+static bool use_level_launches() {
+    static const bool v = [] {
+        const char* e = std::getenv("FVG_SWEEP");
+        return e && std::strcmp(e, "levels") == 0;
+    }();
+    return v;
+}
+
+static int sweep(fvg_ctx* c, const Sweep& sw, const double* scale, const double* b, double* x, int32_t* err,
                  const int32_t* skip) {
     const int tok = timing_begin(c);
-    for (int l = 0; l < sw.nlevels; ++l) {
-        const int64_t lb = sw.lvl_ptr[(size_t)l], le = sw.lvl_ptr[(size_t)l + 1];
-        hipLaunchKernelGGL(k_sweep_level, dim3(grid_for(le - lb)), dim3(kBlock), 0, c->stream, lb, le, sw.rows, sw.ptr,
-                           sw.col, sw.val, scale, b, x, skip);
+    int64_t launches = 0;
+    if (use_level_launches()) {
+        for (int l = 0; l < sw.nlevels; ++l) {
+            const int64_t lb = sw.lvl_ptr[(size_t)l], le = sw.lvl_ptr[(size_t)l + 1];
+            hipLaunchKernelGGL(k_sweep_level, dim3(grid_for(le - lb)), dim3(kBlock), 0, c->stream, lb, le, sw.rows,
+                               sw.ptr, sw.col, sw.val, scale, b, x, skip);
+        }
+        launches = sw.nlevels;
+    } else {
+        hipLaunchKernelGGL(k_fill_pending, dim3(grid_for(sw.n)), dim3(kBlock), 0, c->stream, sw.n,
+                           reinterpret_cast<uint64_t*>(x), err, skip);
+        const int64_t per = (int64_t)kChunk * kWavesPerGroup;
+        hipLaunchKernelGGL(k_sweep_syncfree, dim3((unsigned)((sw.npos + per - 1) / per)), dim3((unsigned)per), 0,
+                           c->stream, sw.npos, sw.rows, sw.ptr, sw.col, sw.val, scale, b, x, err, skip);
+        launches = 2;
     }
-    timing_end(c, tok, kSpanSweep, sw.nlevels);
+    timing_end(c, tok, kSpanSweep, launches);
     FVG_HIP(hipGetLastError());
+    return FVG_OK;
+}
+
+static int sweep32(fvg_ctx* c, const Sweep& sw, const float* scale, const float* b, float* x, int32_t* err,
+                   const int32_t* skip) {
+    const int tok = timing_begin(c);
+    hipLaunchKernelGGL(k_fill_pending32, dim3(grid_for(sw.n)), dim3(kBlock), 0, c->stream, sw.n,
+                       reinterpret_cast<uint32_t*>(x), skip);
+    const int64_t per = (int64_t)kChunk * kWavesPerGroup;
+    hipLaunchKernelGGL(k_sweep_syncfree32, dim3((unsigned)((sw.npos + per - 1) / per)), dim3((unsigned)per), 0,
+                       c->stream, sw.npos, sw.rows, sw.ptr, sw.col, sw.val32, scale, b, x, err, skip);
+    timing_end(c, tok, kSpanSweep, 2);
+    FVG_HIP(hipGetLastError());
+    return FVG_OK;
+}
+
+int ensure_prec32(fvg_ctx* c, fvg_prec* p) {
+    if (p->kind == kIdentity) return FVG_OK;
+    if (p->fac32_version == p->fac_version && p->d_inv_diag32) return FVG_OK;
+    if (!p->d_inv_diag32) FVG_TRY(dev_alloc_n(c, p->n, &p->d_inv_diag32));
+    if (!p->d_tmp32) FVG_TRY(dev_alloc_n(c, p->n, &p->d_tmp32));
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_inv_diag,
+                       p->d_inv_diag32);
+    for (Sweep* sw : {&p->lower, &p->upper}) {
+        if (sw->nnz == 0 && sw->n == 0) continue;
+        if (!sw->val32) FVG_TRY(dev_alloc_n(c, sw->nnz, &sw->val32));
+        if (sw->nnz > 0)
+            hipLaunchKernelGGL(k_to_f32, dim3(grid_for(sw->nnz)), dim3(kBlock), 0, c->stream, sw->nnz, sw->val,
+                               sw->val32);
+    }
+    FVG_HIP(hipGetLastError());
+    p->fac32_version = p->fac_version;
+    return FVG_OK;
+}
+
+int apply_prec32(fvg_ctx* c, const fvg_prec* p, const float* r, float* z, const int32_t* skip) {
+    switch (p->kind) {
+    case kIdentity:
+        hipLaunchKernelGGL(k_scale32, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, nullptr, r, z, skip);
+        break;
+    case kDiagonal:
+        hipLaunchKernelGGL(k_scale32, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_inv_diag32, r, z,
+                           skip);
+        break;
+    case kIC:
+        FVG_TRY(sweep32(c, p->lower, p->d_inv_diag32, r, p->d_tmp32, p->d_flags + 3, skip));
+        return sweep32(c, p->upper, p->d_inv_diag32, p->d_tmp32, z, p->d_flags + 3, skip);
+    case kILU:
+        FVG_TRY(sweep32(c, p->lower, nullptr, r, p->d_tmp32, p->d_flags + 3, skip));
+        return sweep32(c, p->upper, p->d_inv_diag32, p->d_tmp32, z, p->d_flags + 3, skip);
+    default:
+        return fail(FVG_E_INVALID, "unknown preconditioner");
+    }
+    FVG_HIP(hipGetLastError());
+    return FVG_OK;
+}
+
+int check_sweep_status(fvg_ctx* c, const fvg_prec* p) {
+    if (!p->d_flags) return FVG_OK;
+    int32_t fl[4];
+    FVG_TRY(read_flags(c, p, fl));
+    if (fl[3]) {
+        FVG_HIP(hipMemsetAsync(p->d_flags + 3, 0, sizeof(int32_t), c->stream));
+        FVG_HIP(hipStreamSynchronize(c->stream));
+        return fail(FVG_E_TIMEOUT, "a triangular-sweep wait exceeded its spin limit");
+    }
     return FVG_OK;
 }
 
@@ -334,11 +549,11 @@ int apply_prec(fvg_ctx* c, const fvg_prec* p, const double* r, double* z, const 
         FVG_HIP(hipGetLastError());
         return FVG_OK;
     case kIC:
-        FVG_TRY(sweep(c, p->lower, p->d_inv_diag, r, p->d_tmp, skip));
-        return sweep(c, p->upper, p->d_inv_diag, p->d_tmp, z, skip);
+        FVG_TRY(sweep(c, p->lower, p->d_inv_diag, r, p->d_tmp, p->d_flags + 3, skip));
+        return sweep(c, p->upper, p->d_inv_diag, p->d_tmp, z, p->d_flags + 3, skip);
     case kILU:
-        FVG_TRY(sweep(c, p->lower, nullptr, r, p->d_tmp, skip));
-        return sweep(c, p->upper, p->d_inv_diag, p->d_tmp, z, skip);
+        FVG_TRY(sweep(c, p->lower, nullptr, r, p->d_tmp, p->d_flags + 3, skip));
+        return sweep(c, p->upper, p->d_inv_diag, p->d_tmp, z, p->d_flags + 3, skip);
     }
     return fail(FVG_E_INVALID, "unknown preconditioner");
 }
@@ -422,6 +637,7 @@ int fvg_prec_refactor(fvg_ctx* c, fvg_prec* p, const fvg_csr* a) {
         hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_diag, a->val,
                            p->d_inv_diag);
         FVG_HIP(hipGetLastError());
+        p->fac_version += 1;
         return FVG_OK;
     }
     return factorize(c, p, a);
@@ -443,7 +659,7 @@ int fvg_precondition(fvg_ctx* c, const fvg_prec* p, fvg_vec* solution, const fvg
     if (solution->d == rhs->d) return fail(FVG_E_INVALID, "solution and rhs must not alias");
     FVG_TRY(apply_prec(c, p, rhs->d, solution->d, nullptr));
     FVG_HIP(hipStreamSynchronize(c->stream));
-    return FVG_OK;
+    return check_sweep_status(c, p);
 }
 
 int fvg_prec_factor_download(fvg_ctx* c, const fvg_prec* p, double* values, double* inv_diag) {

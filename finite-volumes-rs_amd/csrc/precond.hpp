@@ -6,17 +6,20 @@
 namespace fvg {
 
 // One triangular sweep x = scale .* (b - T x) over a strictly triangular T, stored
-// in level order: positions are rows sorted by (dependency level, row index), so
-// each level is a contiguous range and its rows' entries are contiguous in HBM.
+// in level order: positions are rows sorted by (dependency level, row index), each
+// level padded to a whole number of kChunk-row chunks (padding rows are -1), so a
+// level is a contiguous range of chunks and its rows' entries are contiguous in HBM.
+constexpr int kChunk = 64;              // rows per wave64 chunk
 struct Sweep {
-    int64_t n = 0, nnz = 0;
+    int64_t n = 0, npos = 0, nnz = 0;
     int nlevels = 0;
     std::vector<int64_t> lvl_ptr;       // host: level l holds positions [lvl_ptr[l], lvl_ptr[l+1])
-    int32_t* rows = nullptr;            // row of each position
-    int32_t* ptr = nullptr;             // entry range of each position (n+1)
+    int32_t* rows = nullptr;            // row of each position, -1 for padding (npos)
+    int32_t* ptr = nullptr;             // entry range of each position (npos+1)
     int32_t* col = nullptr;             // column of each entry, ascending within a row
     int32_t* src = nullptr;             // slot of each entry in the factor value array
     double* val = nullptr;              // entry values gathered from the factor
+    float* val32 = nullptr;             // FP32 copy for the mixed-precision inner solver
 };
 
 enum PrecKind { kIdentity = 0, kDiagonal = 1, kIC = 2, kILU = 3 };
@@ -35,12 +38,23 @@ struct fvg_prec {
     double* d_fac = nullptr;            // factor values on the matrix pattern
     double* d_inv_diag = nullptr;       // IC 1/l_ii, ILU 1/u_ii, Jacobi 1/a_ii
     double* d_tmp = nullptr;            // intermediate vector of the two sweeps
-    int32_t* d_flags = nullptr;         // [0] zero pivots, [1] non-positive IC pivots, [2] asymmetry
+    int32_t* d_flags = nullptr;         // [0] zero pivots, [1] non-positive IC pivots, [2] asymmetry,
+                                        // [3] sweep wait timed out
     fvg::Sweep lower, upper;
+    // FP32 copies used by the mixed-precision inner iterations (made on first use)
+    float* d_inv_diag32 = nullptr;
+    float* d_tmp32 = nullptr;
+    uint64_t fac_version = 0, fac32_version = 0;
 };
 
 namespace fvg {
 // solution = M^-1 rhs on the context stream (no host synchronisation).
 // skip: optional device flag; when *skip != 0 the launches do nothing.
 int apply_prec(fvg_ctx* ctx, const fvg_prec* p, const double* rhs, double* solution, const int32_t* skip);
+// FVG_E_TIMEOUT if a sweep wait gave up since the last check (synchronises).
+int check_sweep_status(fvg_ctx* ctx, const fvg_prec* p);
+// FP32 factor copies for the mixed-precision inner solver (refreshed after a refactor).
+int ensure_prec32(fvg_ctx* ctx, fvg_prec* p);
+// solution = M^-1 rhs with FP32 factors, vectors and arithmetic.
+int apply_prec32(fvg_ctx* ctx, const fvg_prec* p, const float* rhs, float* solution, const int32_t* skip);
 }  // namespace fvg

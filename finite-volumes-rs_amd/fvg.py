@@ -72,7 +72,7 @@ class SolverInfo(C.Structure):
 
 class KernelStats(C.Structure):
     _fields_ = [("spmv_ms", C.c_double), ("sweep_ms", C.c_double), ("spmv_launches", C.c_int64),
-                ("sweep_launches", C.c_int64)]
+                ("sweep_launches", C.c_int64), ("spmv_calls", C.c_int64), ("sweep_calls", C.c_int64)]
 
 
 _lib = None
@@ -105,6 +105,7 @@ def lib():
         "fvg_vec_upload": [P, P, P],
         "fvg_vec_download": [P, P, P],
         "fvg_vec_dot": [P, P, P, C.POINTER(C.c_double)],
+        "fvg_vec_fill": [P, P, C.c_double],
         "fvg_mesh_create": [P, C.c_int, C.c_int64, P, C.c_int64, P, P, P, C.c_int64, P, P, C.POINTER(P)],
         "fvg_mesh_destroy": [P, P],
         "fvg_mesh_sizes": [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -184,7 +185,7 @@ class Context:
         s = KernelStats()
         check(lib().fvg_ctx_kernel_stats(self.h, C.byref(s), 1 if reset else 0))
         return dict(spmv_ms=s.spmv_ms, sweep_ms=s.sweep_ms, spmv_launches=s.spmv_launches,
-                    sweep_launches=s.sweep_launches)
+                    sweep_launches=s.sweep_launches, spmv_calls=s.spmv_calls, sweep_calls=s.sweep_calls)
 
 
 class DistributedVector:
@@ -217,6 +218,9 @@ class DistributedVector:
         out = np.empty(self.n)
         check(lib().fvg_vec_download(self.ctx.h, self.h, _ptr(out)))
         return out
+
+    def fill(self, value):
+        check(lib().fvg_vec_fill(self.ctx.h, self.h, float(value)))
 
     def dot(self, other):
         r = C.c_double()
@@ -496,29 +500,45 @@ def assemble(equation, boundary_condition, mesh, lhs=None, rhs=None):
     return assemble_program(mesh, ops, ff, cf, bc.bndi, bc.bnde, lhs, rhs)
 
 
+class Assembler:
+    """A compiled equation with its fields and boundary closure resident on the device;
+    run() re-assembles into the given lhs/rhs without host transfers."""
+
+    def __init__(self, mesh, ops, face_fields, cell_fields, bndi, bnde):
+        ctx = mesh.ctx
+        self.mesh = mesh
+        nb = mesh.n_boundary_faces
+        bndi = np.ascontiguousarray(bndi, dtype=np.float64)
+        bnde = np.ascontiguousarray(bnde, dtype=np.float64)
+        if bndi.shape != (nb,) or bnde.shape != (nb,):
+            raise FvgError(1, "boundary closure arrays must have one entry per boundary face")
+        self.fvecs = [DistributedVector.from_data(ctx, a) for a in face_fields]
+        self.cvecs = [DistributedVector.from_data(ctx, a) for a in cell_fields]
+        self.bi = DistributedVector.from_data(ctx, bndi) if nb else None
+        self.be = DistributedVector.from_data(ctx, bnde) if nb else None
+        self.ops = list(ops)
+        self._prog = (TermOp * len(self.ops))(*[TermOp(*o) for o in self.ops])
+        self._fv = (P * max(1, len(self.fvecs)))(*[v.h for v in self.fvecs])
+        self._cv = (P * max(1, len(self.cvecs)))(*[v.h for v in self.cvecs])
+
+    def run(self, lhs, rhs, sync=True):
+        m = self.mesh
+        check(lib().fvg_assemble(m.ctx.h, m.h, C.cast(self._prog, P), len(self.ops), C.cast(self._fv, P),
+                                 len(self.fvecs), C.cast(self._cv, P), len(self.cvecs),
+                                 self.bi.h if self.bi else None, self.be.h if self.be else None, lhs.h, rhs.h))
+        if sync:
+            m.ctx.sync()
+
+
 def assemble_program(mesh, ops, face_fields, cell_fields, bndi, bnde, lhs=None, rhs=None):
     """assemble() with the equation already in postfix form: ops is a list of
     (opcode, arg0, arg1, arg2, scalar) over the given face and cell field arrays."""
-    ctx = mesh.ctx
-    nb = mesh.n_boundary_faces
-    bndi = np.ascontiguousarray(bndi, dtype=np.float64)
-    bnde = np.ascontiguousarray(bnde, dtype=np.float64)
-    if bndi.shape != (nb,) or bnde.shape != (nb,):
-        raise FvgError(1, "boundary closure arrays must have one entry per boundary face")
-    fvecs = [DistributedVector.from_data(ctx, a) for a in face_fields]
-    cvecs = [DistributedVector.from_data(ctx, a) for a in cell_fields]
-    bi = DistributedVector.from_data(ctx, bndi) if nb else None
-    be = DistributedVector.from_data(ctx, bnde) if nb else None
+    asm = Assembler(mesh, ops, face_fields, cell_fields, bndi, bnde)
     if lhs is None:
-        lhs = DistributedMatrix.from_mesh(ctx, mesh)
+        lhs = DistributedMatrix.from_mesh(mesh.ctx, mesh)
     if rhs is None:
-        rhs = DistributedVector.from_size(ctx, mesh.n_cells)
-    prog = (TermOp * len(ops))(*[TermOp(*o) for o in ops])
-    fv = (P * max(1, len(fvecs)))(*[v.h for v in fvecs])
-    cv = (P * max(1, len(cvecs)))(*[v.h for v in cvecs])
-    check(lib().fvg_assemble(ctx.h, mesh.h, C.cast(prog, P), len(ops), C.cast(fv, P), len(fvecs), C.cast(cv, P),
-                             len(cvecs), bi.h if bi else None, be.h if be else None, lhs.h, rhs.h))
-    ctx.sync()
+        rhs = DistributedVector.from_size(mesh.ctx, mesh.n_cells)
+    asm.run(lhs, rhs)
     return lhs, rhs
 
 

@@ -1,0 +1,99 @@
+"""The synthetic assemble-and-solve workloads, built the way a crate user builds them.
+
+Each builder returns the raw mesh (meshgen form) and a ``fields(geom)`` function that
+evaluates the equation's fields and boundary closure from Mesh::compute geometry --
+exactly as user code written against the crate would (e.g. a flux ``u . face.normal()``
+as in examples/ex4/ex4.rs:59-65, a Dirichlet value from ``face.center()``).
+
+Equation programs are postfix lists of (opcode, arg0, arg1, arg2, scalar); see
+include/fvg.h for the opcodes.
+
+Workloads (sizes as in the benchmark definition):
+  diffusion_2d     64x64 quads -> 8,192 triangles, jitter 0.25h, k = 1, f ~ U[0,1],
+                   u = 0 on the boundary, CG + IC(0)
+  diffusion_3d     64^3 hexes -> 1,572,864 Kuhn tets, jitter 0.2h, ln k ~ N(0, 1.5^2)
+                   box-filtered over 3x3x3 hexes, u = x on the boundary, f ~ N(0,1), CG + IC(0)
+  convdiff_prism   128x128x2 triangles extruded through 64 layers -> 2,097,152 prisms,
+                   velocity (1,0.5,0.25) scaled per cell to Peclet U[1,100], k = 1e-2,
+                   upwind, inflow u = 1 / outflow zero gradient, BiCGStab + ILU(0)
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import meshgen
+
+LAP, CONV, SRC, LIN, TIME, DIV, ADD, SUB, NEG = 1, 2, 3, 4, 5, 6, 16, 17, 18
+
+# -laplacian(Orthogonal, &k) - source(&f)      (face field 0 = k, cell field 0 = f)
+DIFFUSION_PROGRAM = [(LAP, 0, 0, 0, 0.0), (NEG, 0, 0, 0, 0.0), (SRC, 0, 0, 0, 0.0), (SUB, 0, 0, 0, 0.0)]
+# convection(Upwind(&F), &F) - laplacian(Orthogonal, &k)   (face fields 0 = F, 1 = k)
+CONVDIFF_PROGRAM = [(CONV, 0, 0, 0, 0.0), (LAP, 1, 0, 0, 0.0), (SUB, 0, 0, 0, 0.0)]
+
+
+def diffusion_fields(raw, geom, kc, f, bc="zero"):
+    """Face k = harmonic mean of the cells (owner value on the boundary); Dirichlet
+    u = 0 (bc='zero') or u = x_f (bc='x') as (bndi, bnde) = (0, g)."""
+    kf = meshgen.harmonic_face_values(kc, geom["owner"], geom["neighbor"])
+    bnd = np.nonzero(raw["face_patch"] >= 0)[0]
+    bci = np.zeros(bnd.size)
+    bce = np.zeros(bnd.size) if bc == "zero" else geom["face_center"][bnd, 0].copy()
+    return dict(prog=DIFFUSION_PROGRAM, face_fields=[kf], cell_fields=[np.asarray(f, np.float64)], bc_i=bci,
+                bc_e=bce, solver="cg", prec="ic")
+
+
+def convdiff_fields(raw, geom, kdiff=1e-2, velocity=(1.0, 0.5, 0.25), pe=(1.0, 100.0), seed=0):
+    """Per-cell velocity scaled to |u| h / k ~ U[pe]; face flux = mean cell velocity . face.normal();
+    inflow faces (flux < 0) Dirichlet u = 1 -> (0, 1), others zero gradient -> (1, 0) (ex4.rs:76-86)."""
+    n = raw["cell_ptr"].shape[0] - 1
+    dim = raw["dim"]
+    rng = np.random.default_rng(seed)
+    u = np.asarray(velocity[:dim], dtype=np.float64)
+    u = u / np.linalg.norm(u)
+    speed = rng.uniform(pe[0], pe[1], n) * kdiff / raw["h"]
+    uc = speed[:, None] * u[None, :]
+    own, nb = geom["owner"], geom["neighbor"]
+    uf = uc[own].copy()
+    it = nb >= 0
+    uf[it] = 0.5 * (uc[own[it]] + uc[nb[it]])
+    flux = (uf * geom["face_normal"]).sum(axis=1)
+    kf = np.full(own.shape[0], kdiff)
+    bnd = np.nonzero(raw["face_patch"] >= 0)[0]
+    inflow = flux[bnd] < 0.0
+    return dict(prog=CONVDIFF_PROGRAM, face_fields=[flux, kf], cell_fields=[], bc_i=np.where(inflow, 0.0, 1.0),
+                bc_e=np.where(inflow, 1.0, 0.0), solver="bicgstab", prec="ilu")
+
+
+def diffusion_2d(nx=64, seed=1):
+    raw = meshgen.tri2d(nx, jitter=0.25, seed=seed)
+    n = raw["cell_ptr"].shape[0] - 1
+    f = np.random.default_rng(seed + 1000).uniform(0.0, 1.0, n)
+
+    def fields(geom):
+        return diffusion_fields(raw, geom, np.ones(n), f, bc="zero")
+
+    return raw, fields
+
+
+def diffusion_3d(nx=64, seed=1, sigma=1.5):
+    raw = meshgen.tet3d(nx, jitter=0.2, seed=seed)
+    n = raw["cell_ptr"].shape[0] - 1
+    kc = meshgen.lognormal_hex_boxfiltered(nx, nx, nx, sigma, seed + 2000)
+    f = np.random.default_rng(seed + 1000).standard_normal(n)
+
+    def fields(geom):
+        return diffusion_fields(raw, geom, kc, f, bc="x")
+
+    return raw, fields
+
+
+def convdiff_prism(nx=128, layers=64, seed=1, pe=(1.0, 100.0)):
+    raw = meshgen.prism3d(nx, layers=layers, jitter=0.25, seed=seed)
+
+    def fields(geom):
+        return convdiff_fields(raw, geom, pe=pe, seed=seed + 3000)
+
+    return raw, fields
+
+
+WORKLOADS = {"diffusion_2d": diffusion_2d, "diffusion_3d": diffusion_3d, "convdiff_prism": convdiff_prism}

@@ -72,6 +72,8 @@ int fvg_vec_len(const fvg_vec* v, int64_t* n);
 int fvg_vec_device_ptr(const fvg_vec* v, double** device_ptr);
 int fvg_vec_upload(fvg_ctx* ctx, fvg_vec* v, const double* host);     /* from_data (dvector.rs:23-25) */
 int fvg_vec_download(fvg_ctx* ctx, const fvg_vec* v, double* host);
+/* Set every entry to value on the device (asynchronous on the context stream). */
+int fvg_vec_fill(fvg_ctx* ctx, fvg_vec* v, double value);
 /* DistributedVector::dot (dvector.rs:63-71); deterministic fixed-order tree sum. */
 int fvg_vec_dot(fvg_ctx* ctx, const fvg_vec* a, const fvg_vec* b, double* out);
 
@@ -223,9 +225,11 @@ int fvg_bicgstab(fvg_ctx* ctx, const fvg_csr* a, const fvg_prec* p, const fvg_ve
 /* ---- device timing of the most recent solve ------------------------------- */
 typedef struct fvg_kernel_stats {
     double spmv_ms;             /* total time of the SpMV kernels */
-    double sweep_ms;            /* total time of the triangular-sweep kernels */
+    double sweep_ms;            /* total time of the triangular sweeps (all levels of each) */
     int64_t spmv_launches;
-    int64_t sweep_launches;
+    int64_t sweep_launches;     /* kernel launches spent in sweeps */
+    int64_t spmv_calls;         /* SpMV products timed */
+    int64_t sweep_calls;        /* whole triangular sweeps timed */
 } fvg_kernel_stats;
 /* Enable HIP-event timing of SpMV / sweep kernels on this context (adds events). */
 int fvg_ctx_enable_timing(fvg_ctx* ctx, int enable);
