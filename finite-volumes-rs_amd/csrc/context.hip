@@ -184,6 +184,7 @@ int fvg_ctx_create(int device, uint64_t budget, fvg_ctx** out) {
     FVG_HIP(hipSetDevice(device));
     auto* c = new fvg_ctx();
     c->device = device;
+    (void)hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device);
     c->budget = budget;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -234,6 +235,29 @@ int fvg_ctx_stream(const fvg_ctx* c, void** stream) {
     if (!c || !stream) return fail(FVG_E_INVALID, "null argument");
     *stream = (void*)c->stream;
     return FVG_OK;
+}
+
+int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
+    if (!c || value < 0) return fail(FVG_E_INVALID, "bad tuning argument");
+    switch (knob) {
+    case FVG_TUNE_SWEEP_MODE:
+        if (value > 1) return fail(FVG_E_INVALID, "sweep mode must be 0 or 1");
+        c->tune_sweep_mode = value;
+        return FVG_OK;
+    case FVG_TUNE_SWEEP_WAVES:
+        c->tune_sweep_waves = value;
+        return FVG_OK;
+    }
+    return fail(FVG_E_INVALID, "unknown tuning knob");
+}
+
+int fvg_ctx_get_tuning(const fvg_ctx* c, int32_t knob, int64_t* value) {
+    if (!c || !value) return fail(FVG_E_INVALID, "null argument");
+    switch (knob) {
+    case FVG_TUNE_SWEEP_MODE: *value = c->tune_sweep_mode; return FVG_OK;
+    case FVG_TUNE_SWEEP_WAVES: *value = c->tune_sweep_waves; return FVG_OK;
+    }
+    return fail(FVG_E_INVALID, "unknown tuning knob");
 }
 
 int fvg_ctx_enable_timing(fvg_ctx* c, int enable) {

@@ -52,6 +52,9 @@ struct fvg_ctx {
     struct Span { hipEvent_t a, b; int kind; int64_t launches; };
     std::vector<Span> spans;           // event pairs recorded while timing is enabled
     fvg_kernel_stats stats{};
+    int64_t tune_sweep_mode = 0;       // FVG_TUNE_SWEEP_MODE
+    int64_t tune_sweep_waves = 0;      // FVG_TUNE_SWEEP_WAVES
+    int num_cus = 0;
 };
 
 struct fvg_vec {

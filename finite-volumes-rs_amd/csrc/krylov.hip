@@ -16,6 +16,8 @@
 
 namespace fvg {
 
+constexpr int kScalarBlock = 1024;   // threads of the one-workgroup scalar kernels
+
 struct KState {
     double bnorm, init_res, error, atol, rtol;
     double rz, alpha, beta, rho, omega;
@@ -51,9 +53,9 @@ __device__ __forceinline__ void loop_test(KState* st) {
 }
 
 template <int STEP>
-__global__ void __launch_bounds__(kBlock) k_scalar(KState* __restrict__ st, const double* __restrict__ partials,
+__global__ void __launch_bounds__(kScalarBlock) k_scalar(KState* __restrict__ st, const double* __restrict__ partials,
                                                    int64_t stride, int count, double* __restrict__ hist) {
-    __shared__ double lds[kBlock / 64];
+    __shared__ double lds[kScalarBlock / 64];
     __shared__ int skip;
     if (threadIdx.x == 0) {
         skip = 0;
@@ -66,12 +68,12 @@ __global__ void __launch_bounds__(kBlock) k_scalar(KState* __restrict__ st, cons
     }
     __syncthreads();
     if (skip) return;
-    const double s0 = sum_partials<kBlock>(partials, count, lds);
+    const double s0 = sum_partials<kScalarBlock>(partials, count, lds);
     double s1 = 0.0, s2 = 0.0;
     if (STEP == BI_OMEGA || STEP == BI_R) {
-        s1 = sum_partials<kBlock>(partials + stride, count, lds);
+        s1 = sum_partials<kScalarBlock>(partials + stride, count, lds);
     }
-    if (STEP == BI_R) s2 = sum_partials<kBlock>(partials + 2 * stride, count, lds);
+    if (STEP == BI_R) s2 = sum_partials<kScalarBlock>(partials + 2 * stride, count, lds);
     if (threadIdx.x != 0) return;
     switch (STEP) {
     case CG_INIT:   // s0 = r.r   (conjugate_gradient.rs:52, 62-66)
@@ -395,12 +397,12 @@ int run_cg(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fv
     e0.b = b->d;
     e0.dot_self = true;
     FVG_TRY(launch_spmv(c, a, x->d, r, e0));
-    hipLaunchKernelGGL(k_scalar<CG_INIT>, dim3(1), dim3(kBlock), 0, c->stream, K.st, part + stride, stride, nb, K.hist);
+    hipLaunchKernelGGL(k_scalar<CG_INIT>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part + stride, stride, nb, K.hist);
     // z = M^-1 r; p = z; r.z
     FVG_TRY(apply_prec(c, p, r, z, nullptr));
     FVG_TRY(launch_copy(c, n, z, pp));
     hipLaunchKernelGGL(k_dot_g, dim3(nb), dim3(kBlock), 0, c->stream, n, (const KState*)nullptr, r, z, part);
-    hipLaunchKernelGGL(k_scalar<CG_RZ0>, dim3(1), dim3(kBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
+    hipLaunchKernelGGL(k_scalar<CG_RZ0>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
     FVG_HIP(hipGetLastError());
 
     const int batch = poll_batch(p);
@@ -410,12 +412,12 @@ int run_cg(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fv
         e1.dotw = pp;
         e1.skip = halt;
         FVG_TRY(launch_spmv(c, a, pp, ap, e1));
-        hipLaunchKernelGGL(k_scalar<CG_ALPHA>, dim3(1), dim3(kBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
+        hipLaunchKernelGGL(k_scalar<CG_ALPHA>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
         hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, pp, ap, x->d, r, part);
-        hipLaunchKernelGGL(k_scalar<CG_ERROR>, dim3(1), dim3(kBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
+        hipLaunchKernelGGL(k_scalar<CG_ERROR>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
         FVG_TRY(apply_prec(c, p, r, z, done));
         hipLaunchKernelGGL(k_dot_g, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, r, z, part);
-        hipLaunchKernelGGL(k_scalar<CG_BETA>, dim3(1), dim3(kBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
+        hipLaunchKernelGGL(k_scalar<CG_BETA>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
         hipLaunchKernelGGL(k_cg_p, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, z, pp);
         FVG_HIP(hipGetLastError());
         if ((it + 1) % batch == 0) {
@@ -460,7 +462,7 @@ int run_bicgstab(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec*
     e0.copy2 = pp;
     e0.dot_self = true;
     FVG_TRY(launch_spmv(c, a, x->d, r, e0));
-    hipLaunchKernelGGL(k_scalar<BI_INIT>, dim3(1), dim3(kBlock), 0, c->stream, K.st, part + stride, stride, nb, K.hist);
+    hipLaunchKernelGGL(k_scalar<BI_INIT>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part + stride, stride, nb, K.hist);
     FVG_HIP(hipGetLastError());
     {
         KState h;
@@ -484,18 +486,18 @@ int run_bicgstab(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec*
         e1.dotw = rh;
         e1.skip = halt;
         FVG_TRY(launch_spmv(c, a, y, v, e1));
-        hipLaunchKernelGGL(k_scalar<BI_ALPHA>, dim3(1), dim3(kBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
+        hipLaunchKernelGGL(k_scalar<BI_ALPHA>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
         hipLaunchKernelGGL(k_bi_hs, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, y, v, r, x->d, s, part);
-        hipLaunchKernelGGL(k_scalar<BI_S>, dim3(1), dim3(kBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
+        hipLaunchKernelGGL(k_scalar<BI_S>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
         FVG_TRY(apply_prec(c, p, s, z, done));                   // z = M^-1 s
         SpmvEpilogue e2;                                         // t = A z, t.s, t.t
         e2.dotw = s;
         e2.dot_self = true;
         e2.skip = done;
         FVG_TRY(launch_spmv(c, a, z, t, e2));
-        hipLaunchKernelGGL(k_scalar<BI_OMEGA>, dim3(1), dim3(kBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
+        hipLaunchKernelGGL(k_scalar<BI_OMEGA>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
         hipLaunchKernelGGL(k_bi_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, z, t, s, rh, x->d, r, part, stride);
-        hipLaunchKernelGGL(k_scalar<BI_R>, dim3(1), dim3(kBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
+        hipLaunchKernelGGL(k_scalar<BI_R>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
         hipLaunchKernelGGL(k_bi_p, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, r, v, pp, rh);
         FVG_HIP(hipGetLastError());
         if ((it + 1) % batch == 0) {

@@ -20,6 +20,8 @@
 
 namespace fvg {
 
+constexpr int kScalarBlock = 1024;   // threads of the one-workgroup scalar kernels
+
 constexpr int kRows32 = 256;
 constexpr int kStage32 = 3072;
 
@@ -96,16 +98,19 @@ enum MStep {
 };
 
 template <int STEP>
-__global__ void __launch_bounds__(kBlock) k_mscalar(MState* __restrict__ st, const double* __restrict__ partials,
+__global__ void __launch_bounds__(kScalarBlock) k_mscalar(MState* __restrict__ st, const double* __restrict__ partials,
                                                     int64_t stride, int count) {
-    __shared__ double lds[kBlock / 64];
+    __shared__ double lds[kScalarBlock / 64];
     __shared__ int skip;
     if (threadIdx.x == 0) skip = STEP == M_OUTER ? st->done : (st->done | st->inner_done);
     __syncthreads();
     if (skip) return;
-    const double s0 = sum_partials<kBlock>(partials, count, lds);
-    const double s1 = sum_partials<kBlock>(partials + stride, count, lds);
-    const double s2 = sum_partials<kBlock>(partials + 2 * stride, count, lds);
+    constexpr bool need0 = STEP != M_OUTER;
+    constexpr bool need1 = STEP == M_OUTER || STEP == M_BI_OMEGA || STEP == M_BI_R;
+    constexpr bool need2 = STEP == M_IN_INIT;
+    const double s0 = need0 ? sum_partials<kScalarBlock>(partials, count, lds) : 0.0;
+    const double s1 = need1 ? sum_partials<kScalarBlock>(partials + stride, count, lds) : 0.0;
+    const double s2 = need2 ? sum_partials<kScalarBlock>(partials + 2 * stride, count, lds) : 0.0;
     if (threadIdx.x != 0) return;
     switch (STEP) {
     case M_OUTER: {
@@ -425,7 +430,7 @@ int run_mixed(fvg_ctx* c, fvg_csr* a, fvg_prec* p, const fvg_vec* b, fvg_vec* x,
         e.dot_self = true;
         e.skip = &st->done;
         FVG_TRY(launch_spmv(c, a, x->d, r64, e));
-        hipLaunchKernelGGL(k_mscalar<M_OUTER>, dim3(1), dim3(kBlock), 0, c->stream, st, part, stride, nb);
+        hipLaunchKernelGGL(k_mscalar<M_OUTER>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
         FVG_HIP(hipMemcpyAsync(c->pinned, st, sizeof(MState), hipMemcpyDeviceToHost, c->stream));
         FVG_HIP(hipStreamSynchronize(c->stream));
         std::memcpy(&h, c->pinned, sizeof(MState));
@@ -439,30 +444,30 @@ int run_mixed(fvg_ctx* c, fvg_csr* a, fvg_prec* p, const fvg_vec* b, fvg_vec* x,
             hipLaunchKernelGGL(k_m_copy, dim3(nb), dim3(kBlock), 0, c->stream, n, st, z, pp);
             hipLaunchKernelGGL(k_m_dot, dim3(nb), dim3(kBlock), 0, c->stream, n, st, 0, r, z, part + 2 * stride);
         }
-        hipLaunchKernelGGL(k_mscalar<M_IN_INIT>, dim3(1), dim3(kBlock), 0, c->stream, st, part, stride, nb);
+        hipLaunchKernelGGL(k_mscalar<M_IN_INIT>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
         FVG_HIP(hipGetLastError());
         for (int64_t it = 0;; ++it) {
             if (!bicgstab) {
                 FVG_TRY(launch_spmv32(c, a, pp, ap, pp, 0, inner_skip));
-                hipLaunchKernelGGL(k_mscalar<M_CG_ALPHA>, dim3(1), dim3(kBlock), 0, c->stream, st, part, stride, nb);
+                hipLaunchKernelGGL(k_mscalar<M_CG_ALPHA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
                 hipLaunchKernelGGL(k_m_cg_dr, dim3(nb), dim3(kBlock), 0, c->stream, n, st, pp, ap, d, r, part);
-                hipLaunchKernelGGL(k_mscalar<M_CG_ERR>, dim3(1), dim3(kBlock), 0, c->stream, st, part, stride, nb);
+                hipLaunchKernelGGL(k_mscalar<M_CG_ERR>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
                 FVG_TRY(apply_prec32(c, p, r, z, inner_skip));
                 hipLaunchKernelGGL(k_m_dot, dim3(nb), dim3(kBlock), 0, c->stream, n, st, 1, r, z, part);
-                hipLaunchKernelGGL(k_mscalar<M_CG_BETA>, dim3(1), dim3(kBlock), 0, c->stream, st, part, stride, nb);
+                hipLaunchKernelGGL(k_mscalar<M_CG_BETA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
                 hipLaunchKernelGGL(k_m_cg_p, dim3(nb), dim3(kBlock), 0, c->stream, n, st, z, pp);
             } else {
                 FVG_TRY(apply_prec32(c, p, pp, y, inner_skip));
                 FVG_TRY(launch_spmv32(c, a, y, v, rh, 0, inner_skip));
-                hipLaunchKernelGGL(k_mscalar<M_BI_ALPHA>, dim3(1), dim3(kBlock), 0, c->stream, st, part, stride, nb);
+                hipLaunchKernelGGL(k_mscalar<M_BI_ALPHA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
                 hipLaunchKernelGGL(k_m_bi_hs, dim3(nb), dim3(kBlock), 0, c->stream, n, st, y, v, r, d, s, part);
-                hipLaunchKernelGGL(k_mscalar<M_BI_S>, dim3(1), dim3(kBlock), 0, c->stream, st, part, stride, nb);
+                hipLaunchKernelGGL(k_mscalar<M_BI_S>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
                 FVG_TRY(apply_prec32(c, p, s, z, inner_skip));
                 FVG_TRY(launch_spmv32(c, a, z, t, s, 1, inner_skip));
-                hipLaunchKernelGGL(k_mscalar<M_BI_OMEGA>, dim3(1), dim3(kBlock), 0, c->stream, st, part, stride, nb);
+                hipLaunchKernelGGL(k_mscalar<M_BI_OMEGA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
                 hipLaunchKernelGGL(k_m_bi_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, st, z, t, s, rh, d, r, part,
                                    stride);
-                hipLaunchKernelGGL(k_mscalar<M_BI_R>, dim3(1), dim3(kBlock), 0, c->stream, st, part, stride, nb);
+                hipLaunchKernelGGL(k_mscalar<M_BI_R>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
                 hipLaunchKernelGGL(k_m_bi_p, dim3(nb), dim3(kBlock), 0, c->stream, n, st, r, v, pp, rh);
             }
             FVG_HIP(hipGetLastError());

@@ -139,105 +139,123 @@ __global__ void k_sweep_level(int64_t begin, int64_t end, const int32_t* __restr
 // ---- single-launch sweep -----------------------------------------------------------
 // Every row of the sweep is one lane; a wave64 owns one chunk of 64 positions of one
 // level, and chunks are numbered in level order, so every row a lane waits for lives
-// in a chunk with a smaller number.  A workgroup holds kWavesPerGroup consecutive
-// chunks and workgroups are dispatched in increasing order on each XCD, so whatever a
-// running wave waits for has already been dispatched and is itself only waiting on
-// earlier chunks: the sweep always drains.
+// in a chunk with a smaller number.  The kernel is persistent: a cooperative launch
+// makes all its W waves co-resident, and wave w takes chunks w, w+W, w+2W, ... in
+// order.  The smallest unfinished chunk therefore always belongs to a running wave
+// whose earlier chunks are done and whose dependencies are all finished: the sweep
+// always drains.  W bounds how many waves poll at once (and how far ahead of the
+// dependency front they run).
 //
 // The output vector doubles as the completion flag: it is filled with a signalling
-// NaN bit pattern (kPending) that no arithmetic result can carry, and each row's value
-// is published with one 64-bit agent-scope store; dependants poll that word with
+// NaN bit pattern (pending) that no arithmetic result can carry, and each row's value
+// is published with one agent-scope store; dependants read that word with
 // agent-scope loads, which are coherent across XCDs.  Every wait is bounded: after
 // kSpinLimit polls (or once another lane has timed out) the wait gives up, flags the
 // error and lets the sweep finish, so a kernel never hangs.
-constexpr uint64_t kPending = 0x7FF4DEADBEEF0F64ull;
+template <class T>
+struct Bits;
+template <>
+struct Bits<double> {
+    using U = uint64_t;
+    static constexpr U pending = 0x7FF4DEADBEEF0F64ull;
+    __device__ static double value(U u) { return __longlong_as_double((long long)u); }
+    __device__ static U bits(double v) { return (U)__double_as_longlong(v); }
+};
+template <>
+struct Bits<float> {
+    using U = uint32_t;
+    static constexpr U pending = 0x7FB0DEADu;
+    __device__ static float value(U u) { return __uint_as_float(u); }
+    __device__ static U bits(float v) { return __float_as_uint(v); }
+};
+
 constexpr int kSpinLimit = 1 << 18;
 constexpr int kWavesPerGroup = 4;
+constexpr int kMaxRowDeps = 6;     // dependencies loaded together before any wait
 
-__global__ void k_fill_pending(int64_t n, uint64_t* __restrict__ x, int32_t* __restrict__ err,
-                               const int32_t* __restrict__ skip) {
+template <class T>
+__global__ void k_fill_pending(int64_t n, T* __restrict__ x, const int32_t* __restrict__ skip) {
     if (skip && *skip) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = kPending;
-    (void)err;
+    if (i < n) reinterpret_cast<typename Bits<T>::U*>(x)[i] = Bits<T>::pending;
 }
 
-__device__ __forceinline__ double wait_value(const double* x, int j, int32_t* err) {
-    const uint64_t* p = reinterpret_cast<const uint64_t*>(x) + j;
-    uint64_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+template <class T>
+__device__ __forceinline__ typename Bits<T>::U load_word(const T* x, int j) {
+    return __hip_atomic_load(reinterpret_cast<const typename Bits<T>::U*>(x) + j, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class T>
+__device__ __forceinline__ T wait_value(const T* x, int j, int32_t* err) {
+    using U = typename Bits<T>::U;
+    U v = load_word(x, j);
     int spins = 0;
-    while (v == kPending) {
+    while (v == Bits<T>::pending) {
         if (++spins > kSpinLimit ||
             ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
         }
         __builtin_amdgcn_s_sleep(1);
-        v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v = load_word(x, j);
     }
-    return __longlong_as_double((long long)v);
+    return Bits<T>::value(v);
 }
 
+template <class T>
 __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
-    k_sweep_syncfree(int64_t npos, const int32_t* __restrict__ rows, const int32_t* __restrict__ ptr,
-                     const int32_t* __restrict__ col, const double* __restrict__ val,
-                     const double* __restrict__ scale, const double* __restrict__ b, double* x,
-                     int32_t* __restrict__ err, const int32_t* __restrict__ skip) {
-    if (skip && *skip) return;
-    const int64_t p = (int64_t)blockIdx.x * (kChunk * kWavesPerGroup) + threadIdx.x;
-    if (p >= npos) return;
-    const int i = rows[p];
-    if (i < 0) return;
-    double s = b[i];
-    const int e = ptr[p + 1];
-    for (int k = ptr[p]; k < e; ++k) s -= val[k] * wait_value(x, col[k], err);
-    const double r = scale ? scale[i] * s : s;
-    __hip_atomic_store(reinterpret_cast<uint64_t*>(x) + i, (uint64_t)__double_as_longlong(r), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// FP32 form of the single-launch sweep (mixed-precision inner iterations); the pending
-// marker is a signalling-NaN float pattern.
-constexpr uint32_t kPending32 = 0x7FB0DEADu;
-
-__global__ void k_fill_pending32(int64_t n, uint32_t* __restrict__ x, const int32_t* __restrict__ skip) {
-    if (skip && *skip) return;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = kPending32;
-}
-
-__device__ __forceinline__ float wait_value32(const float* x, int j, int32_t* err) {
-    const uint32_t* p = reinterpret_cast<const uint32_t*>(x) + j;
-    uint32_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    int spins = 0;
-    while (v == kPending32) {
-        if (++spins > kSpinLimit ||
-            ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return __uint_as_float(v);
-}
-
-__global__ void __launch_bounds__(kChunk * kWavesPerGroup)
-    k_sweep_syncfree32(int64_t npos, const int32_t* __restrict__ rows, const int32_t* __restrict__ ptr,
-                       const int32_t* __restrict__ col, const float* __restrict__ val,
-                       const float* __restrict__ scale, const float* __restrict__ b, float* x,
+    k_sweep_persistent(int64_t nchunks, int64_t nwaves, const int32_t* __restrict__ rows,
+                       const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                       const T* __restrict__ val, const T* __restrict__ scale, const T* __restrict__ b, T* x,
                        int32_t* __restrict__ err, const int32_t* __restrict__ skip) {
+    using U = typename Bits<T>::U;
     if (skip && *skip) return;
-    const int64_t p = (int64_t)blockIdx.x * (kChunk * kWavesPerGroup) + threadIdx.x;
-    if (p >= npos) return;
-    const int i = rows[p];
-    if (i < 0) return;
-    float s = b[i];
-    const int e = ptr[p + 1];
-    for (int k = ptr[p]; k < e; ++k) s -= val[k] * wait_value32(x, col[k], err);
-    const float r = scale ? scale[i] * s : s;
-    __hip_atomic_store(reinterpret_cast<uint32_t*>(x) + i, __float_as_uint(r), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    const int64_t w0 = (int64_t)blockIdx.x * kWavesPerGroup + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    for (int64_t c = w0; c < nchunks; c += nwaves) {
+        const int64_t p = c * kChunk + lane;
+        const int i = rows[p];
+        if (i < 0) continue;
+        T s = b[i];
+        const int kb = ptr[p], ke = ptr[p + 1];
+        if (ke - kb <= kMaxRowDeps) {
+            // load every dependency at once and re-poll all still-pending ones together, so a
+            // row waits one memory round trip after its last dependency lands; then subtract
+            // in ascending column order as the sequential sweep does
+            U w[kMaxRowDeps];
+            int cj[kMaxRowDeps];
+#pragma unroll
+            for (int q = 0; q < kMaxRowDeps; ++q)
+                if (kb + q < ke) {
+                    cj[q] = col[kb + q];
+                    w[q] = load_word(x, cj[q]);
+                }
+            int spins = 0;
+            for (;;) {
+                bool pend = false;
+#pragma unroll
+                for (int q = 0; q < kMaxRowDeps; ++q) pend |= (kb + q < ke) && w[q] == Bits<T>::pending;
+                if (!pend) break;
+                if (++spins > kSpinLimit ||
+                    ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                for (int q = 0; q < kMaxRowDeps; ++q)
+                    if ((kb + q < ke) && w[q] == Bits<T>::pending) w[q] = load_word(x, cj[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < kMaxRowDeps; ++q)
+                if (kb + q < ke) s -= val[kb + q] * Bits<T>::value(w[q]);
+        } else {
+            for (int k = kb; k < ke; ++k) s -= val[k] * wait_value(x, col[k], err);
+        }
+        const T r = scale ? scale[i] * s : s;
+        __hip_atomic_store(reinterpret_cast<U*>(x) + i, Bits<T>::bits(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 __global__ void k_to_f32(int64_t n, const double* __restrict__ s, float* __restrict__ d) {
@@ -439,20 +457,50 @@ static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a) {
     return FVG_OK;
 }
 
-// FVG_SWEEP=levels selects one launch per level instead of the single-launch sweep.
-static bool use_level_launches() {
-    static const bool v = [] {
-        const char* e = std::getenv("FVG_SWEEP");
-        return e && std::strcmp(e, "levels") == 0;
-    }();
-    return v;
+// largest grid of the persistent sweep that is guaranteed co-resident
+template <class T>
+static int max_coresident_groups(fvg_ctx* c) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent<T>, kChunk * kWavesPerGroup, 0) !=
+        hipSuccess)
+        return 1;
+    const int m = per_cu * c->num_cus;
+    return m > 0 ? m : 1;
+}
+
+// waves of the persistent sweep: about two levels' worth of chunks in flight
+static int64_t sweep_waves(const fvg_ctx* c, const Sweep& sw, int max_groups) {
+    int64_t w = c->tune_sweep_waves;
+    if (w <= 0) {
+        const int64_t chunks = sw.npos / kChunk;
+        w = 2 * ((chunks + sw.nlevels - 1) / std::max(1, sw.nlevels));
+    }
+    w = std::max<int64_t>(w, kWavesPerGroup);
+    w = (w + kWavesPerGroup - 1) / kWavesPerGroup * kWavesPerGroup;
+    return std::min<int64_t>(w, (int64_t)max_groups * kWavesPerGroup);
+}
+
+template <class T>
+static int launch_sweep(fvg_ctx* c, const Sweep& sw, const T* val, const T* scale, const T* b, T* x, int32_t* err,
+                        const int32_t* skip) {
+    if (sw.n == 0) return FVG_OK;
+    hipLaunchKernelGGL(k_fill_pending<T>, dim3(grid_for(sw.n)), dim3(kBlock), 0, c->stream, sw.n, x, skip);
+    int64_t nchunks = sw.npos / kChunk;
+    int64_t nwaves = sweep_waves(c, sw, max_coresident_groups<T>(c));
+    nwaves = std::min(nwaves, (nchunks + kWavesPerGroup - 1) / kWavesPerGroup * kWavesPerGroup);
+    const int64_t groups = nwaves / kWavesPerGroup;
+    void* args[] = {&nchunks, &nwaves, (void*)&sw.rows, (void*)&sw.ptr, (void*)&sw.col, (void*)&val, (void*)&scale,
+                    (void*)&b, (void*)&x, (void*)&err, (void*)&skip};
+    FVG_HIP(hipLaunchCooperativeKernel((const void*)k_sweep_persistent<T>, dim3((unsigned)groups),
+                                       dim3(kChunk * kWavesPerGroup), args, 0, c->stream));
+    return FVG_OK;
 }
 
 static int sweep(fvg_ctx* c, const Sweep& sw, const double* scale, const double* b, double* x, int32_t* err,
                  const int32_t* skip) {
     const int tok = timing_begin(c);
     int64_t launches = 0;
-    if (use_level_launches()) {
+    if (c->tune_sweep_mode == 1) {
         for (int l = 0; l < sw.nlevels; ++l) {
             const int64_t lb = sw.lvl_ptr[(size_t)l], le = sw.lvl_ptr[(size_t)l + 1];
             hipLaunchKernelGGL(k_sweep_level, dim3(grid_for(le - lb)), dim3(kBlock), 0, c->stream, lb, le, sw.rows,
@@ -460,11 +508,7 @@ static int sweep(fvg_ctx* c, const Sweep& sw, const double* scale, const double*
         }
         launches = sw.nlevels;
     } else {
-        hipLaunchKernelGGL(k_fill_pending, dim3(grid_for(sw.n)), dim3(kBlock), 0, c->stream, sw.n,
-                           reinterpret_cast<uint64_t*>(x), err, skip);
-        const int64_t per = (int64_t)kChunk * kWavesPerGroup;
-        hipLaunchKernelGGL(k_sweep_syncfree, dim3((unsigned)((sw.npos + per - 1) / per)), dim3((unsigned)per), 0,
-                           c->stream, sw.npos, sw.rows, sw.ptr, sw.col, sw.val, scale, b, x, err, skip);
+        FVG_TRY(launch_sweep<double>(c, sw, sw.val, scale, b, x, err, skip));
         launches = 2;
     }
     timing_end(c, tok, kSpanSweep, launches);
@@ -475,11 +519,7 @@ static int sweep(fvg_ctx* c, const Sweep& sw, const double* scale, const double*
 static int sweep32(fvg_ctx* c, const Sweep& sw, const float* scale, const float* b, float* x, int32_t* err,
                    const int32_t* skip) {
     const int tok = timing_begin(c);
-    hipLaunchKernelGGL(k_fill_pending32, dim3(grid_for(sw.n)), dim3(kBlock), 0, c->stream, sw.n,
-                       reinterpret_cast<uint32_t*>(x), skip);
-    const int64_t per = (int64_t)kChunk * kWavesPerGroup;
-    hipLaunchKernelGGL(k_sweep_syncfree32, dim3((unsigned)((sw.npos + per - 1) / per)), dim3((unsigned)per), 0,
-                       c->stream, sw.npos, sw.rows, sw.ptr, sw.col, sw.val32, scale, b, x, err, skip);
+    FVG_TRY(launch_sweep<float>(c, sw, sw.val32, scale, b, x, err, skip));
     timing_end(c, tok, kSpanSweep, 2);
     FVG_HIP(hipGetLastError());
     return FVG_OK;

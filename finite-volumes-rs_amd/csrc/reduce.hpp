@@ -38,12 +38,20 @@ __device__ __forceinline__ T block_sum(T v, T* lds) {
     return s;
 }
 
-// Fixed-order sum of `count` partials (stride-NT per thread, then block tree).
+// Fixed-order sum of `count` partials: thread t owns elements t, t+NT, t+2NT, ...
+// summed with four independent accumulators (so the loads overlap), then the block tree.
 template <int NT>
 __device__ __forceinline__ double sum_partials(const double* __restrict__ p, int count, double* lds) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < count; i += NT) s += p[i];
-    return block_sum<NT>(s, lds);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3 * NT < count; i += 4 * NT) {
+        a0 += p[i];
+        a1 += p[i + NT];
+        a2 += p[i + 2 * NT];
+        a3 += p[i + 3 * NT];
+    }
+    for (; i < count; i += NT) a0 += p[i];
+    return block_sum<NT>((a0 + a1) + (a2 + a3), lds);
 }
 
 }  // namespace fvg

@@ -30,6 +30,7 @@ STATUS = {1: "FVG_E_INVALID", 2: "FVG_E_HIP", 3: "FVG_E_BUDGET", 4: "FVG_E_ZERO_
 OP_LAPLACIAN_ORTHOGONAL, OP_CONVECTION_UPWIND, OP_SOURCE, OP_LINEAR_SOURCE, OP_TIME_EULER, OP_DIVERGENCE = 1, 2, 3, 4, 5, 6
 OP_ADD, OP_SUB, OP_NEG = 16, 17, 18
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+TUNE_SWEEP_MODE, TUNE_SWEEP_WAVES = 1, 2
 
 
 class FvgError(RuntimeError):
@@ -96,6 +97,8 @@ def lib():
         "fvg_ctx_bytes_in_use": [P, C.POINTER(C.c_uint64)],
         "fvg_ctx_stream": [P, C.POINTER(P)],
         "fvg_ctx_enable_timing": [P, C.c_int],
+        "fvg_ctx_set_tuning": [P, C.c_int32, C.c_int64],
+        "fvg_ctx_get_tuning": [P, C.c_int32, C.POINTER(C.c_int64)],
         "fvg_ctx_kernel_stats": [P, C.POINTER(KernelStats), C.c_int],
         "fvg_vec_create": [P, C.c_int64, C.POINTER(P)],
         "fvg_vec_wrap": [P, C.c_int64, P, C.POINTER(P)],
@@ -176,6 +179,14 @@ class Context:
     def bytes_in_use(self):
         v = C.c_uint64()
         check(lib().fvg_ctx_bytes_in_use(self.h, C.byref(v)))
+        return v.value
+
+    def set_tuning(self, knob, value):
+        check(lib().fvg_ctx_set_tuning(self.h, int(knob), int(value)))
+
+    def get_tuning(self, knob):
+        v = C.c_int64()
+        check(lib().fvg_ctx_get_tuning(self.h, int(knob), C.byref(v)))
         return v.value
 
     def enable_timing(self, on=True):

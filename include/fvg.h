@@ -60,6 +60,15 @@ int fvg_ctx_bytes_in_use(const fvg_ctx* ctx, uint64_t* bytes);
 /* The hipStream_t of the context, as void*, for callers that interleave work. */
 int fvg_ctx_stream(const fvg_ctx* ctx, void** stream);
 
+/* Launch-shape knobs (no counterpart in the reference; used by the autotuner).
+ * Values of 0 restore the built-in choice. */
+enum {
+    FVG_TUNE_SWEEP_MODE = 1,    /* 0: one persistent launch per triangular sweep; 1: one launch per level */
+    FVG_TUNE_SWEEP_WAVES = 2    /* waves of the persistent sweep (0: about two levels of chunks) */
+};
+int fvg_ctx_set_tuning(fvg_ctx* ctx, int32_t knob, int64_t value);
+int fvg_ctx_get_tuning(const fvg_ctx* ctx, int32_t knob, int64_t* value);
+
 /* ---- dense vectors: DistributedVector<f64> (src/linalg/dvector.rs:10-179) ---- */
 typedef struct fvg_vec fvg_vec;
 

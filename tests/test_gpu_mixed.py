@@ -3,10 +3,6 @@
 The mixed solver is not the reference algorithm, so its bar is the FP64 oracle's
 solution: true relative residual <= 1e-8 and x within 1e-6 relative of the oracle's x
 (stated for every case below)."""
-import os
-import subprocess
-import sys
-
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -16,7 +12,6 @@ import oracle as O
 from test_gpu_parity import _oracle_system
 
 pytestmark = pytest.mark.gpu
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 @pytest.mark.parametrize("name,kind,solver,prec", [("tri64", "sym", "cg", "ic"), ("tet10", "sym", "cg", "ic"),
@@ -45,24 +40,21 @@ def test_mixed_precision_reaches_fp64_accuracy(ctx, name, kind, solver, prec):
     assert info.iterations <= 3 * io["iterations"] + 20
 
 
-def test_level_launch_sweeps_match_single_launch(ctx):
-    """FVG_SWEEP=levels (one launch per level) gives the same bits as the single-launch sweep."""
-    code = r'''
-import sys, numpy as np
-sys.path[:0] = [sys.argv[1] + "/finite-volumes-rs_amd", sys.argv[1] + "/oracle", sys.argv[1] + "/tests"]
-import fvg, oracle as O
-from test_gpu_parity import _oracle_system
-rp, ci, va, b = _oracle_system("tet10", "sym")
-ctx = fvg.Context(0)
-A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
-ic = fvg.IncompleteCholesky.from_matrix(A, 0)
-r = np.random.default_rng(1).normal(size=rp.shape[0] - 1)
-z = fvg.DistributedVector(ctx, r.shape[0])
-ic.precondition(z, fvg.DistributedVector.from_data(ctx, r))
-assert np.array_equal(z.numpy(), O.OraclePrec("ic", rp, ci, va, 0).apply(r))
-print("levels-ok")
-'''
-    env = dict(os.environ, FVG_SWEEP="levels")
-    out = subprocess.run([sys.executable, "-c", code, ROOT], env=env, capture_output=True, text=True, timeout=300)
-    assert out.returncode == 0, out.stderr[-2000:]
-    assert "levels-ok" in out.stdout
+@pytest.mark.parametrize("mode,waves", [(1, 0), (0, 4), (0, 64), (0, 0)])
+def test_sweep_launch_shapes_give_identical_bits(ctx, mode, waves):
+    """One launch per level, and the persistent sweep at several wave counts, all give the
+    oracle's bits (the schedule never changes the arithmetic)."""
+    rp, ci, va, b = _oracle_system("tet10", "sym")
+    n = rp.shape[0] - 1
+    A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+    ic = fvg.IncompleteCholesky.from_matrix(A, 0)
+    ref = O.OraclePrec("ic", rp, ci, va, 0).apply(np.random.default_rng(1).normal(size=n))
+    try:
+        ctx.set_tuning(fvg.TUNE_SWEEP_MODE, mode)
+        ctx.set_tuning(fvg.TUNE_SWEEP_WAVES, waves)
+        z = fvg.DistributedVector(ctx, n)
+        ic.precondition(z, fvg.DistributedVector.from_data(ctx, np.random.default_rng(1).normal(size=n)))
+        assert np.array_equal(z.numpy(), ref)
+    finally:
+        ctx.set_tuning(fvg.TUNE_SWEEP_MODE, 0)
+        ctx.set_tuning(fvg.TUNE_SWEEP_WAVES, 0)
