@@ -247,6 +247,10 @@ int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
     case FVG_TUNE_SWEEP_WAVES:
         c->tune_sweep_waves = value;
         return FVG_OK;
+    case FVG_TUNE_SWEEP_POLL:
+        if (value > 2) return fail(FVG_E_INVALID, "poll variant must be 0, 1 or 2");
+        c->tune_sweep_poll = value;
+        return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }
@@ -256,6 +260,7 @@ int fvg_ctx_get_tuning(const fvg_ctx* c, int32_t knob, int64_t* value) {
     switch (knob) {
     case FVG_TUNE_SWEEP_MODE: *value = c->tune_sweep_mode; return FVG_OK;
     case FVG_TUNE_SWEEP_WAVES: *value = c->tune_sweep_waves; return FVG_OK;
+    case FVG_TUNE_SWEEP_POLL: *value = c->tune_sweep_poll; return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }

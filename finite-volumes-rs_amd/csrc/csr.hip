@@ -37,7 +37,7 @@ __global__ void __launch_bounds__(kSpmvRows) k_spmv(int n, const int32_t* __rest
                                                     int64_t pstride) {
     __shared__ double prod[kSpmvStage];
     __shared__ double red[kSpmvRows / 64];
-    if (EPI && epi.skip && *epi.skip) return;
+    if (EPI && epi.skip && epi.skip[(epi.blk_inst ? epi.blk_inst[blockIdx.x] : 0) * epi.skip_stride]) return;
     const int r0 = blockIdx.x * kSpmvRows;
     const int r1 = min(r0 + kSpmvRows, n);
     const int base = rp[r0];

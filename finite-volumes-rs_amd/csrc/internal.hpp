@@ -54,6 +54,7 @@ struct fvg_ctx {
     fvg_kernel_stats stats{};
     int64_t tune_sweep_mode = 0;       // FVG_TUNE_SWEEP_MODE
     int64_t tune_sweep_waves = 0;      // FVG_TUNE_SWEEP_WAVES
+    int64_t tune_sweep_poll = 0;       // FVG_TUNE_SWEEP_POLL
     int num_cus = 0;
 };
 

@@ -1,41 +1,37 @@
-// krylov.hip -- conjugate_gradient and bi_conjugate_gradient_stab on the device.
+// krylov.hip -- conjugate_gradient and bi_conjugate_gradient_stab on the device,
+// for one system or a batch of independent systems.
 //
 // The iteration is the reference's, step for step (conjugate_gradient.rs:12-112,
 // bi_conjugate_gradient_stab.rs:13-179): same updates in the same operation order,
 // same stopping tests on the recursive residual, same iteration counting and
-// history.  All scalars live on the device: each reduction ends in a one-workgroup
-// "scalar" kernel that sums the per-workgroup partials in a fixed order and runs the
-// scalar part of the algorithm, so the host only enqueues work and polls a flag.
-// Vector updates are fused with the dot products that follow them.
+// history.  All scalars live on the device: each reduction ends in a "scalar" kernel
+// with one workgroup per system that sums that system's per-workgroup partials in a
+// fixed order and runs the scalar part of the algorithm, so the host only enqueues
+// work and polls one flag.  Vector updates are fused with the dot products after them.
+//
+// A batch is a block-diagonal system whose members start on 256-row boundaries, so
+// every 256-row workgroup belongs to exactly one member and each member's partial
+// sums (and so every result bit) are the same as when that member is solved alone.
 #include <cmath>
 #include <cstring>
 
+#include "krylov.hpp"
 #include "precond.hpp"
 #include "reduce.hpp"
 #include "spmv.hpp"
 
 namespace fvg {
 
-constexpr int kScalarBlock = 1024;   // threads of the one-workgroup scalar kernels
-
-struct KState {
-    double bnorm, init_res, error, atol, rtol;
-    double rz, alpha, beta, rho, omega;
-    double ts, tt;
-    int64_t iter, max_iter, hist_len, hist_cap, restarts;
-    int32_t done;   // converged or stopped inside an iteration: the rest of it is skipped
-    int32_t halt;   // the loop test failed before an iteration began
-    int32_t mode;   // BiCGStab direction update: 0 update, 1 restart
-    int32_t pad;
-};
+constexpr int kScalarBlock = 1024;   // threads per scalar workgroup
 
 enum ScalarStep {
-    CG_INIT = 0,    // initial residual, loop test
+    ST_BNORM = 0,   // |b|, zero right-hand side exit (conjugate_gradient.rs:29-36)
+    CG_INIT,        // initial residual, loop test
     CG_RZ0,         // r.z of the first iteration
     CG_ALPHA,       // alpha = rz / p.Ap
     CG_ERROR,       // error = |r| / |b| and the convergence test
     CG_BETA,        // beta = rz_new / rz, history, loop test
-    BI_INIT,        // initial residual, rho = rh.r
+    BI_INIT,        // initial residual, rho = rh.r, early exit (bi_conjugate_gradient_stab.rs:174-190)
     BI_ALPHA,       // alpha = rho / rh.v
     BI_S,           // |s| test
     BI_OMEGA,       // omega = t.s / t.t
@@ -47,42 +43,66 @@ __device__ __forceinline__ void push_history(KState* st, double* hist, double e)
     st->hist_len += 1;
 }
 
-__device__ __forceinline__ void loop_test(KState* st) {
+// first time a system stops iterating: retire it from the batch
+__device__ __forceinline__ void retire(KState* st, int32_t* active, int32_t* all_done) {
+    st->halt = 1;
+    if (!st->counted) {
+        st->counted = 1;
+        if (atomicSub(active, 1) == 1) *all_done = 1;
+    }
+}
+
+__device__ __forceinline__ void loop_test(KState* st, int32_t* active, int32_t* all_done) {
     // while error > tolerance { if iter > max_iter { break } ... }
-    if (!(st->error > st->atol) || st->iter > st->max_iter) st->halt = 1;
+    if (!(st->error > st->atol) || st->iter > st->max_iter) retire(st, active, all_done);
 }
 
 template <int STEP>
-__global__ void __launch_bounds__(kScalarBlock) k_scalar(KState* __restrict__ st, const double* __restrict__ partials,
-                                                   int64_t stride, int count, double* __restrict__ hist) {
+__global__ void __launch_bounds__(kScalarBlock) k_scalar(KState* __restrict__ states, const double* __restrict__ partials,
+                                                         int64_t stride, int count, const int32_t* __restrict__ blk_ptr,
+                                                         double* __restrict__ hist_all, int64_t hcap,
+                                                         int32_t* __restrict__ active, int32_t* __restrict__ all_done) {
     __shared__ double lds[kScalarBlock / 64];
     __shared__ int skip;
+    KState* st = states + blockIdx.x;
+    double* hist = hist_all + (int64_t)blockIdx.x * hcap;
     if (threadIdx.x == 0) {
         skip = 0;
         if (STEP == CG_ALPHA || STEP == BI_ALPHA) {
             if (st->halt) st->done = 1;
             skip = st->done;
-        } else if (STEP != CG_INIT && STEP != BI_INIT) {
+        } else if (STEP == CG_INIT || STEP == BI_INIT) {
+            skip = st->halt;   // zero right-hand side already answered
+        } else if (STEP != ST_BNORM) {
             skip = st->done;
         }
     }
     __syncthreads();
     if (skip) return;
-    const double s0 = sum_partials<kScalarBlock>(partials, count, lds);
-    double s1 = 0.0, s2 = 0.0;
-    if (STEP == BI_OMEGA || STEP == BI_R) {
-        s1 = sum_partials<kScalarBlock>(partials + stride, count, lds);
-    }
-    if (STEP == BI_R) s2 = sum_partials<kScalarBlock>(partials + 2 * stride, count, lds);
+    const int b0 = blk_ptr ? blk_ptr[blockIdx.x] : 0;
+    const int nb = blk_ptr ? blk_ptr[blockIdx.x + 1] - b0 : count;
+    constexpr bool need0 = !(STEP == CG_INIT || STEP == BI_INIT);
+    constexpr bool need1 = STEP == CG_INIT || STEP == BI_INIT || STEP == BI_OMEGA || STEP == BI_R;
+    const double s0 = need0 ? sum_partials<kScalarBlock>(partials + b0, nb, lds) : 0.0;
+    const double s1 = need1 ? sum_partials<kScalarBlock>(partials + stride + b0, nb, lds) : 0.0;
     if (threadIdx.x != 0) return;
     switch (STEP) {
-    case CG_INIT:   // s0 = r.r   (conjugate_gradient.rs:52, 62-66)
-        st->init_res = sqrt(s0) / st->bnorm;
+    case ST_BNORM:
+        st->bnorm = sqrt(s0);
+        if (st->bnorm < 1e-14) {
+            st->zero_rhs = 1;
+            st->init_res = st->bnorm;
+            st->error = st->bnorm;
+            push_history(st, hist, st->bnorm);
+            st->done = 1;
+            retire(st, active, all_done);
+        }
+        break;
+    case CG_INIT:   // s1 = r.r   (conjugate_gradient.rs:52, 62-66)
+        st->init_res = sqrt(s1) / st->bnorm;
         st->error = 1.0;
         st->iter = 0;
-        st->done = 0;
-        st->halt = 0;
-        loop_test(st);
+        loop_test(st, active, all_done);
         break;
     case CG_RZ0:    // s0 = r.z (conjugate_gradient.rs:71)
         st->rz = s0;
@@ -96,7 +116,7 @@ __global__ void __launch_bounds__(kScalarBlock) k_scalar(KState* __restrict__ st
             push_history(st, hist, st->error);
             st->iter += 1;
             st->done = 1;
-            st->halt = 1;
+            retire(st, active, all_done);
         }
         break;
     }
@@ -105,17 +125,21 @@ __global__ void __launch_bounds__(kScalarBlock) k_scalar(KState* __restrict__ st
         st->rz = s0;
         push_history(st, hist, st->error);
         st->iter += 1;
-        loop_test(st);
+        loop_test(st, active, all_done);
         break;
-    case BI_INIT:   // s0 = r.r; rho = rh.r with rh == r (bi_conjugate_gradient_stab.rs:174, 190)
-        st->init_res = sqrt(s0) / st->bnorm;
-        st->rho = s0;
+    case BI_INIT:   // s1 = r.r; rho = rh.r with rh == r (bi_conjugate_gradient_stab.rs:174-190)
+        st->init_res = sqrt(s1) / st->bnorm;
+        st->rho = s1;
         st->error = 1.0;
         st->iter = 0;
-        st->done = 0;
-        st->halt = 0;
-        st->restarts = 0;
-        loop_test(st);
+        if (st->init_res < st->atol) {   // :176-178
+            st->error = st->init_res;
+            push_history(st, hist, st->init_res);
+            st->done = 1;
+            retire(st, active, all_done);
+            break;
+        }
+        loop_test(st, active, all_done);
         break;
     case BI_ALPHA:  // s0 = rh.v (:208-209)
         st->alpha = st->rho / s0;
@@ -127,13 +151,11 @@ __global__ void __launch_bounds__(kScalarBlock) k_scalar(KState* __restrict__ st
             push_history(st, hist, ss);
             st->iter += 1;
             st->done = 1;
-            st->halt = 1;
+            retire(st, active, all_done);
         }
         break;
     }
     case BI_OMEGA:  // s0 = t.s, s1 = t.t (:239-242)
-        st->ts = s0;
-        st->tt = s1;
         st->omega = s0 / s1;
         break;
     case BI_R: {    // s1 = r.r, s0 = rh.r (:252-287)
@@ -143,7 +165,7 @@ __global__ void __launch_bounds__(kScalarBlock) k_scalar(KState* __restrict__ st
             push_history(st, hist, rr);
             st->iter += 1;
             st->done = 1;
-            st->halt = 1;
+            retire(st, active, all_done);
             break;
         }
         const double rho_new = s0;
@@ -159,23 +181,48 @@ __global__ void __launch_bounds__(kScalarBlock) k_scalar(KState* __restrict__ st
         st->error = rr / st->bnorm;
         push_history(st, hist, st->error);
         st->iter += 1;
-        loop_test(st);
-        (void)s2;
+        loop_test(st, active, all_done);
         break;
     }
     }
 }
 
-// ---- fused vector kernels (one element per thread, kBlock per workgroup) --------------
+// ---- fused vector kernels (one element per thread, kBlock rows per workgroup) -----------
+#define K_STATE const KState& S = states[blk_inst ? blk_inst[blockIdx.x] : 0]
+
+// partial a.b (slot 0); skipped when `skip_done` and the system is done
+__global__ void __launch_bounds__(kBlock) k_dot_g(int64_t n, const KState* __restrict__ states,
+                                                  const int32_t* __restrict__ blk_inst, int skip_done,
+                                                  const double* __restrict__ a, const double* __restrict__ b,
+                                                  double* __restrict__ partials) {
+    __shared__ double lds[kBlock / 64];
+    K_STATE;
+    if (skip_done && S.done) return;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double d = i < n ? a[i] * b[i] : 0.0;
+    d = block_sum<kBlock>(d, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = d;
+}
+
+// x = 0 where the right-hand side was zero (conjugate_gradient.rs:31-36)
+__global__ void k_zero_rhs_x(int64_t n, const KState* __restrict__ states, const int32_t* __restrict__ blk_inst,
+                             double* __restrict__ x) {
+    K_STATE;
+    if (!S.zero_rhs) return;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
 
 // CG (conjugate_gradient.rs:77-80): x += p*alpha; r -= ap*alpha; partial r.r
-__global__ void __launch_bounds__(kBlock) k_cg_xr(int64_t n, const KState* __restrict__ st, const double* __restrict__ p,
+__global__ void __launch_bounds__(kBlock) k_cg_xr(int64_t n, const KState* __restrict__ states,
+                                                  const int32_t* __restrict__ blk_inst, const double* __restrict__ p,
                                                   const double* __restrict__ ap, double* __restrict__ x,
                                                   double* __restrict__ r, double* __restrict__ partials) {
     __shared__ double lds[kBlock / 64];
-    if (st->done) return;
+    K_STATE;
+    if (S.done) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const double alpha = st->alpha;
+    const double alpha = S.alpha;
     double d = 0.0;
     if (i < n) {
         x[i] += p[i] * alpha;
@@ -187,34 +234,45 @@ __global__ void __launch_bounds__(kBlock) k_cg_xr(int64_t n, const KState* __res
     if (threadIdx.x == 0) partials[blockIdx.x] = d;
 }
 
-// partial a.b, skipped when done
-__global__ void __launch_bounds__(kBlock) k_dot_g(int64_t n, const KState* __restrict__ st, const double* __restrict__ a,
-                                                  const double* __restrict__ b, double* __restrict__ partials) {
-    __shared__ double lds[kBlock / 64];
-    if (st && st->done) return;
+// CG (conjugate_gradient.rs:98-100): p = z + p*beta
+__global__ void __launch_bounds__(kBlock) k_cg_p(int64_t n, const KState* __restrict__ states,
+                                                 const int32_t* __restrict__ blk_inst, const double* __restrict__ z,
+                                                 double* __restrict__ p) {
+    K_STATE;
+    if (S.done) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    double d = i < n ? a[i] * b[i] : 0.0;
+    if (i < n) p[i] = z[i] + p[i] * S.beta;
+}
+
+// CG start: p = z (conjugate_gradient.rs:59), partial r.z
+__global__ void __launch_bounds__(kBlock) k_cg_start(int64_t n, const KState* __restrict__ states,
+                                                     const int32_t* __restrict__ blk_inst, const double* __restrict__ r,
+                                                     const double* __restrict__ z, double* __restrict__ p,
+                                                     double* __restrict__ partials) {
+    __shared__ double lds[kBlock / 64];
+    K_STATE;
+    if (S.halt) return;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double d = 0.0;
+    if (i < n) {
+        p[i] = z[i];
+        d = r[i] * z[i];
+    }
     d = block_sum<kBlock>(d, lds);
     if (threadIdx.x == 0) partials[blockIdx.x] = d;
 }
 
-// CG (conjugate_gradient.rs:98-100): p = z + p*beta
-__global__ void __launch_bounds__(kBlock) k_cg_p(int64_t n, const KState* __restrict__ st, const double* __restrict__ z,
-                                                 double* __restrict__ p) {
-    if (st->done) return;
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) p[i] = z[i] + p[i] * st->beta;
-}
-
 // BiCGStab (bi_conjugate_gradient_stab.rs:213-216): h = x + y*alpha (kept in x); s = r - v*alpha; partial s.s
-__global__ void __launch_bounds__(kBlock) k_bi_hs(int64_t n, const KState* __restrict__ st, const double* __restrict__ y,
+__global__ void __launch_bounds__(kBlock) k_bi_hs(int64_t n, const KState* __restrict__ states,
+                                                  const int32_t* __restrict__ blk_inst, const double* __restrict__ y,
                                                   const double* __restrict__ v, const double* __restrict__ r,
                                                   double* __restrict__ x, double* __restrict__ s,
                                                   double* __restrict__ partials) {
     __shared__ double lds[kBlock / 64];
-    if (st->done) return;
+    K_STATE;
+    if (S.done) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const double alpha = st->alpha;
+    const double alpha = S.alpha;
     double d = 0.0;
     if (i < n) {
         x[i] = x[i] + y[i] * alpha;
@@ -227,15 +285,17 @@ __global__ void __launch_bounds__(kBlock) k_bi_hs(int64_t n, const KState* __res
 }
 
 // BiCGStab (:246-249): x = h + z*omega; r = s - t*omega; partials rh.r (slot 0) and r.r (slot 1)
-__global__ void __launch_bounds__(kBlock) k_bi_xr(int64_t n, const KState* __restrict__ st, const double* __restrict__ z,
+__global__ void __launch_bounds__(kBlock) k_bi_xr(int64_t n, const KState* __restrict__ states,
+                                                  const int32_t* __restrict__ blk_inst, const double* __restrict__ z,
                                                   const double* __restrict__ t, const double* __restrict__ s,
                                                   const double* __restrict__ rh, double* __restrict__ x,
                                                   double* __restrict__ r, double* __restrict__ partials,
                                                   int64_t stride) {
     __shared__ double lds[kBlock / 64];
-    if (st->done) return;
+    K_STATE;
+    if (S.done) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const double omega = st->omega;
+    const double omega = S.omega;
     double d0 = 0.0, d1 = 0.0;
     if (i < n) {
         x[i] = x[i] + z[i] * omega;
@@ -251,14 +311,16 @@ __global__ void __launch_bounds__(kBlock) k_bi_xr(int64_t n, const KState* __res
 }
 
 // BiCGStab (:268-282): p = r + (p - v*omega)*beta, or restart rh = r, p = r
-__global__ void __launch_bounds__(kBlock) k_bi_p(int64_t n, const KState* __restrict__ st, const double* __restrict__ r,
+__global__ void __launch_bounds__(kBlock) k_bi_p(int64_t n, const KState* __restrict__ states,
+                                                 const int32_t* __restrict__ blk_inst, const double* __restrict__ r,
                                                  const double* __restrict__ v, double* __restrict__ p,
                                                  double* __restrict__ rh) {
-    if (st->done) return;
+    K_STATE;
+    if (S.done) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    if (st->mode == 0) {
-        p[i] = r[i] + (p[i] - v[i] * st->omega) * st->beta;
+    if (S.mode == 0) {
+        p[i] = r[i] + (p[i] - v[i] * S.omega) * S.beta;
     } else {
         rh[i] = r[i];
         p[i] = r[i];
@@ -281,161 +343,137 @@ struct Workspace {
     }
 };
 
-static int check_solve_args(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, const fvg_vec* x,
-                            const fvg_solver_options* o) {
-    if (!c || !a || !p || !b || !x || !o) return fail(FVG_E_INVALID, "null argument");
-    if (a->nrows != a->ncols) return fail(FVG_E_INVALID, "solver needs a square matrix");
-    if (b->n != a->nrows || x->n != a->nrows || p->n != a->nrows)
-        return fail(FVG_E_INVALID, "vector / preconditioner size does not match the matrix");
-    if (o->max_iterations < 0) return fail(FVG_E_INVALID, "max_iterations must be non-negative");
+int KRun::init(fvg_ctx* ctx, int ninst, const fvg_solver_options* o) {
+    c = ctx;
+    n_inst = ninst;
+    hist_cap = o->max_iterations + 2;
+    FVG_TRY(dev_alloc_n(c, n_inst, &st));
+    FVG_TRY(dev_alloc_n(c, hist_cap * n_inst, &hist));
+    FVG_TRY(dev_alloc_n(c, 2, &flags));
+    std::vector<KState> h((size_t)n_inst);
+    for (auto& s : h) {
+        std::memset(&s, 0, sizeof(s));
+        s.atol = o->absolute_tolerance;
+        s.rtol = o->relative_tolerance;
+        s.max_iter = o->max_iterations;
+        s.hist_cap = hist_cap;
+    }
+    FVG_TRY(upload(c, st, h.data(), n_inst));
+    int32_t f[2] = {n_inst, 0};
+    FVG_TRY(upload(c, flags, f, 2));
     return FVG_OK;
 }
 
-struct KHost {
-    fvg_ctx* c;
-    KState* st = nullptr;
-    double* hist = nullptr;
-    int64_t hist_cap = 0;
-    ~KHost() {
-        if (st) {
-            (void)hipStreamSynchronize(c->stream);
-            dev_free_n(c, st, 1);
-            dev_free_n(c, hist, hist_cap);
-        }
+KRun::~KRun() {
+    if (c) {
+        (void)hipStreamSynchronize(c->stream);
+        dev_free_n(c, st, n_inst);
+        dev_free_n(c, hist, hist_cap * n_inst);
+        dev_free_n(c, flags, 2);
     }
-    int init(double bnorm, const fvg_solver_options* o) {
-        hist_cap = o->max_iterations + 2;
-        FVG_TRY(dev_alloc_n(c, 1, &st));
-        FVG_TRY(dev_alloc_n(c, hist_cap, &hist));
-        KState h{};
-        h.bnorm = bnorm;
-        h.atol = o->absolute_tolerance;
-        h.rtol = o->relative_tolerance;
-        h.max_iter = o->max_iterations;
-        h.hist_cap = hist_cap;
-        FVG_HIP(hipMemcpyAsync(st, &h, sizeof(KState), hipMemcpyHostToDevice, c->stream));
-        FVG_HIP(hipStreamSynchronize(c->stream));
-        return FVG_OK;
-    }
-    int read(KState* h) {
-        FVG_HIP(hipMemcpyAsync(c->pinned, st, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
-        FVG_HIP(hipStreamSynchronize(c->stream));
-        std::memcpy(h, c->pinned, sizeof(KState));
-        return FVG_OK;
-    }
-    int finish(fvg_solver_info* info, double* history, int64_t cap) {
-        KState h;
-        FVG_TRY(read(&h));
-        if (info) {
-            std::memset(info, 0, sizeof(*info));
-            info->iterations = h.iter;
-            info->initial_residual = h.init_res;
-            info->final_residual = h.error;
-            info->history_len = h.hist_len;
-            info->restarts = h.restarts;
-        }
-        if (history && cap > 0) {
-            int64_t m = h.hist_len < cap ? h.hist_len : cap;
-            m = m < hist_cap ? m : hist_cap;
-            FVG_TRY(download(c, history, hist, m));
-        }
-        return FVG_OK;
-    }
-};
+}
 
-// zero right-hand side early exit shared by both solvers (conjugate_gradient.rs:31-36)
-static int zero_rhs_exit(fvg_ctx* c, fvg_vec* x, double bnorm, fvg_solver_info* info, double* history, int64_t cap) {
-    FVG_TRY(launch_fill(c, x->n, x->d, 0.0));
+int KRun::all_done(bool* out) {
+    FVG_HIP(hipMemcpyAsync(c->pinned, flags + 1, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     FVG_HIP(hipStreamSynchronize(c->stream));
-    if (info) {
-        std::memset(info, 0, sizeof(*info));
-        info->initial_residual = bnorm;
-        info->final_residual = bnorm;
-        info->history_len = 1;
+    int32_t v;
+    std::memcpy(&v, c->pinned, sizeof(v));
+    *out = v != 0;
+    return FVG_OK;
+}
+
+int KRun::finish(fvg_solver_info* infos, double* const* histories, int64_t cap) {
+    std::vector<KState> h((size_t)n_inst);
+    FVG_TRY(download(c, h.data(), st, n_inst));
+    for (int i = 0; i < n_inst; ++i) {
+        const KState& s = h[(size_t)i];
+        if (infos) {
+            fvg_solver_info& info = infos[i];
+            std::memset(&info, 0, sizeof(info));
+            info.iterations = s.iter;
+            info.initial_residual = s.init_res;
+            info.final_residual = s.error;
+            info.history_len = s.hist_len;
+            info.restarts = s.restarts;
+        }
+        if (histories && histories[i] && cap > 0) {
+            int64_t m = s.hist_len < cap ? s.hist_len : cap;
+            m = m < hist_cap ? m : hist_cap;
+            FVG_TRY(download(c, histories[i], hist + (int64_t)i * hist_cap, m));
+        }
     }
-    if (history && cap > 0) history[0] = bnorm;
     return FVG_OK;
 }
 
-static int rhs_norm(fvg_ctx* c, const fvg_vec* b, double* bnorm) {
-    FVG_TRY(launch_dot(c, b->n, b->d, b->d, c->scalar));
-    double bb;
-    FVG_TRY(read_scalars(c, c->scalar, &bb, 1));
-    *bnorm = std::sqrt(bb);
-    return FVG_OK;
-}
+#define SCALAR(step, part0)                                                                                \
+    hipLaunchKernelGGL(k_scalar<step>, dim3(L.n_inst), dim3(kScalarBlock), 0, c->stream, K.st, part0, stride, \
+                       nb, L.blk_ptr, K.hist, K.hist_cap, K.flags, K.flags + 1)
 
-// iterations enqueued between two polls of the device state
-static int poll_batch(const fvg_prec* p) {
-    (void)p;
-    return 4;
-}
+// iterations enqueued between two polls of the "all systems done" flag
+constexpr int kPollBatch = 4;
 
-int run_cg(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
-           const fvg_solver_options* o, fvg_solver_info* info, double* history, int64_t cap) {
+int run_cg_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* b, double* x, const Layout& L,
+                 KRun& K) {
     const int64_t n = a->nrows;
-    double bnorm;
-    FVG_TRY(rhs_norm(c, b, &bnorm));
-    if (bnorm < 1e-14) return zero_rhs_exit(c, x, bnorm, info, history, cap);
     Workspace ws{c};
     double *r, *z, *pp, *ap;
     FVG_TRY(ws.get(n, &r));
     FVG_TRY(ws.get(n, &z));
     FVG_TRY(ws.get(n, &pp));
     FVG_TRY(ws.get(n, &ap));
-    KHost K{c};
-    FVG_TRY(K.init(bnorm, o));
     const int nb = spmv_blocks(n);
     FVG_TRY(ensure_partials(c, nb));
     const int64_t stride = c->partials_cap;
     double* part = c->partials;
-    const int32_t* done = &K.st->done;
-    const int32_t* halt = &K.st->halt;
+    const int32_t* all_done = K.flags + 1;
+    const int32_t sstride = (int32_t)(sizeof(KState) / sizeof(int32_t));
 
+    // |b| and the zero right-hand side exit
+    hipLaunchKernelGGL(k_dot_g, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, 0, b, b, part);
+    SCALAR(ST_BNORM, part);
+    hipLaunchKernelGGL(k_zero_rhs_x, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, x);
     // r = (A x - b) * -1 and r.r (conjugate_gradient.rs:47-52)
     SpmvEpilogue e0;
-    e0.b = b->d;
+    e0.b = b;
     e0.dot_self = true;
-    FVG_TRY(launch_spmv(c, a, x->d, r, e0));
-    hipLaunchKernelGGL(k_scalar<CG_INIT>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part + stride, stride, nb, K.hist);
+    e0.skip = &K.st[0].halt;
+    e0.skip_stride = sstride;
+    e0.blk_inst = L.blk_inst;
+    FVG_TRY(launch_spmv(c, a, x, r, e0));
+    SCALAR(CG_INIT, part);
     // z = M^-1 r; p = z; r.z
-    FVG_TRY(apply_prec(c, p, r, z, nullptr));
-    FVG_TRY(launch_copy(c, n, z, pp));
-    hipLaunchKernelGGL(k_dot_g, dim3(nb), dim3(kBlock), 0, c->stream, n, (const KState*)nullptr, r, z, part);
-    hipLaunchKernelGGL(k_scalar<CG_RZ0>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
+    FVG_TRY(apply_prec(c, p, r, z, all_done));
+    hipLaunchKernelGGL(k_cg_start, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, r, z, pp, part);
+    SCALAR(CG_RZ0, part);
     FVG_HIP(hipGetLastError());
 
-    const int batch = poll_batch(p);
     for (int64_t it = 0;; ++it) {
-        // ap = A p with p.ap
-        SpmvEpilogue e1;
+        SpmvEpilogue e1;   // ap = A p with p.ap
         e1.dotw = pp;
-        e1.skip = halt;
+        e1.skip = &K.st[0].halt;
+        e1.skip_stride = sstride;
+        e1.blk_inst = L.blk_inst;
         FVG_TRY(launch_spmv(c, a, pp, ap, e1));
-        hipLaunchKernelGGL(k_scalar<CG_ALPHA>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
-        hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, pp, ap, x->d, r, part);
-        hipLaunchKernelGGL(k_scalar<CG_ERROR>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
-        FVG_TRY(apply_prec(c, p, r, z, done));
-        hipLaunchKernelGGL(k_dot_g, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, r, z, part);
-        hipLaunchKernelGGL(k_scalar<CG_BETA>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
-        hipLaunchKernelGGL(k_cg_p, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, z, pp);
+        SCALAR(CG_ALPHA, part);
+        hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, pp, ap, x, r, part);
+        SCALAR(CG_ERROR, part);
+        FVG_TRY(apply_prec(c, p, r, z, all_done));
+        hipLaunchKernelGGL(k_dot_g, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, 1, r, z, part);
+        SCALAR(CG_BETA, part);
+        hipLaunchKernelGGL(k_cg_p, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, z, pp);
         FVG_HIP(hipGetLastError());
-        if ((it + 1) % batch == 0) {
-            KState h;
-            FVG_TRY(K.read(&h));
-            if (h.done || h.halt) break;
+        if ((it + 1) % kPollBatch == 0) {
+            bool d;
+            FVG_TRY(K.all_done(&d));
+            if (d) break;
         }
     }
-    FVG_TRY(check_sweep_status(c, p));
-    return K.finish(info, history, cap);
+    return check_sweep_status(c, p);
 }
 
-int run_bicgstab(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
-                 const fvg_solver_options* o, fvg_solver_info* info, double* history, int64_t cap) {
+int run_bicgstab_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* b, double* x, const Layout& L,
+                       KRun& K) {
     const int64_t n = a->nrows;
-    double bnorm;
-    FVG_TRY(rhs_norm(c, b, &bnorm));
-    if (bnorm < 1e-14) return zero_rhs_exit(c, x, bnorm, info, history, cap);
     Workspace ws{c};
     double *r, *rh, *pp, *y, *v, *s, *z, *t;
     FVG_TRY(ws.get(n, &r));
@@ -446,73 +484,87 @@ int run_bicgstab(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec*
     FVG_TRY(ws.get(n, &s));
     FVG_TRY(ws.get(n, &z));
     FVG_TRY(ws.get(n, &t));
-    KHost K{c};
-    FVG_TRY(K.init(bnorm, o));
     const int nb = spmv_blocks(n);
     FVG_TRY(ensure_partials(c, nb));
     const int64_t stride = c->partials_cap;
     double* part = c->partials;
-    const int32_t* done = &K.st->done;
-    const int32_t* halt = &K.st->halt;
+    const int32_t* all_done = K.flags + 1;
+    const int32_t sstride = (int32_t)(sizeof(KState) / sizeof(int32_t));
 
+    hipLaunchKernelGGL(k_dot_g, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, 0, b, b, part);
+    SCALAR(ST_BNORM, part);
+    hipLaunchKernelGGL(k_zero_rhs_x, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, x);
     // r = b - A x, rh = r, p = r, r.r (bi_conjugate_gradient_stab.rs:165-190)
     SpmvEpilogue e0;
-    e0.b = b->d;
+    e0.b = b;
     e0.copy1 = rh;
     e0.copy2 = pp;
     e0.dot_self = true;
-    FVG_TRY(launch_spmv(c, a, x->d, r, e0));
-    hipLaunchKernelGGL(k_scalar<BI_INIT>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part + stride, stride, nb, K.hist);
+    e0.skip = &K.st[0].halt;
+    e0.skip_stride = sstride;
+    e0.blk_inst = L.blk_inst;
+    FVG_TRY(launch_spmv(c, a, x, r, e0));
+    SCALAR(BI_INIT, part);
     FVG_HIP(hipGetLastError());
-    {
-        KState h;
-        FVG_TRY(K.read(&h));
-        if (h.init_res < h.atol) {   // :176-178
-            if (info) {
-                std::memset(info, 0, sizeof(*info));
-                info->initial_residual = h.init_res;
-                info->final_residual = h.init_res;
-                info->history_len = 1;
-            }
-            if (history && cap > 0) history[0] = h.init_res;
-            return FVG_OK;
-        }
-    }
 
-    const int batch = poll_batch(p);
     for (int64_t it = 0;; ++it) {
-        FVG_TRY(apply_prec(c, p, pp, y, halt));                  // y = M^-1 p
-        SpmvEpilogue e1;                                         // v = A y, rh.v
+        FVG_TRY(apply_prec(c, p, pp, y, all_done));   // y = M^-1 p
+        SpmvEpilogue e1;                                // v = A y, rh.v
         e1.dotw = rh;
-        e1.skip = halt;
+        e1.skip = &K.st[0].halt;
+        e1.skip_stride = sstride;
+        e1.blk_inst = L.blk_inst;
         FVG_TRY(launch_spmv(c, a, y, v, e1));
-        hipLaunchKernelGGL(k_scalar<BI_ALPHA>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
-        hipLaunchKernelGGL(k_bi_hs, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, y, v, r, x->d, s, part);
-        hipLaunchKernelGGL(k_scalar<BI_S>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
-        FVG_TRY(apply_prec(c, p, s, z, done));                   // z = M^-1 s
-        SpmvEpilogue e2;                                         // t = A z, t.s, t.t
+        SCALAR(BI_ALPHA, part);
+        hipLaunchKernelGGL(k_bi_hs, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, y, v, r, x, s, part);
+        SCALAR(BI_S, part);
+        FVG_TRY(apply_prec(c, p, s, z, all_done));    // z = M^-1 s
+        SpmvEpilogue e2;                                // t = A z, t.s, t.t
         e2.dotw = s;
         e2.dot_self = true;
-        e2.skip = done;
+        e2.skip = &K.st[0].done;
+        e2.skip_stride = sstride;
+        e2.blk_inst = L.blk_inst;
         FVG_TRY(launch_spmv(c, a, z, t, e2));
-        hipLaunchKernelGGL(k_scalar<BI_OMEGA>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
-        hipLaunchKernelGGL(k_bi_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, z, t, s, rh, x->d, r, part, stride);
-        hipLaunchKernelGGL(k_scalar<BI_R>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part, stride, nb, K.hist);
-        hipLaunchKernelGGL(k_bi_p, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, r, v, pp, rh);
+        SCALAR(BI_OMEGA, part);
+        hipLaunchKernelGGL(k_bi_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, z, t, s, rh, x, r, part,
+                           stride);
+        SCALAR(BI_R, part);
+        hipLaunchKernelGGL(k_bi_p, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, r, v, pp, rh);
         FVG_HIP(hipGetLastError());
-        if ((it + 1) % batch == 0) {
-            KState h;
-            FVG_TRY(K.read(&h));
-            if (h.done || h.halt) break;
+        if ((it + 1) % kPollBatch == 0) {
+            bool d;
+            FVG_TRY(K.all_done(&d));
+            if (d) break;
         }
     }
-    FVG_TRY(check_sweep_status(c, p));
-    return K.finish(info, history, cap);
+    return check_sweep_status(c, p);
+}
+
+static int check_solve_args(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, const fvg_vec* x,
+                            const fvg_solver_options* o) {
+    if (!c || !a || !p || !b || !x || !o) return fail(FVG_E_INVALID, "null argument");
+    if (a->nrows != a->ncols) return fail(FVG_E_INVALID, "solver needs a square matrix");
+    if (b->n != a->nrows || x->n != a->nrows || p->n != a->nrows)
+        return fail(FVG_E_INVALID, "vector / preconditioner size does not match the matrix");
+    if (o->max_iterations < 0) return fail(FVG_E_INVALID, "max_iterations must be non-negative");
+    if (b->d == x->d) return fail(FVG_E_INVALID, "b and x must not alias");
+    return FVG_OK;
 }
 
 // mixed.hip
 int run_mixed(fvg_ctx* c, fvg_csr* a, fvg_prec* p, const fvg_vec* b, fvg_vec* x, const fvg_solver_options* o,
               fvg_solver_info* info, double* history, int64_t cap, bool bicgstab);
+
+static int run_single(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
+                      const fvg_solver_options* o, fvg_solver_info* info, double* history, int64_t cap, bool bicg) {
+    KRun K;
+    FVG_TRY(K.init(c, 1, o));
+    Layout L;
+    FVG_TRY(bicg ? run_bicgstab_multi(c, a, p, b->d, x->d, L, K) : run_cg_multi(c, a, p, b->d, x->d, L, K));
+    double* hs[1] = {history};
+    return K.finish(info, hs, cap);
+}
 
 }  // namespace fvg
 
@@ -526,7 +578,7 @@ int fvg_cg(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fv
     // the FP32 copies of the matrix values and factors are caches owned by a and p
     if (o->precision == FVG_PRECISION_MIXED)
         return run_mixed(c, const_cast<fvg_csr*>(a), const_cast<fvg_prec*>(p), b, x, o, info, history, cap, false);
-    return run_cg(c, a, p, b, x, o, info, history, cap);
+    return run_single(c, a, p, b, x, o, info, history, cap, false);
 }
 
 int fvg_bicgstab(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
@@ -534,7 +586,7 @@ int fvg_bicgstab(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec*
     FVG_TRY(check_solve_args(c, a, p, b, x, o));
     if (o->precision == FVG_PRECISION_MIXED)
         return run_mixed(c, const_cast<fvg_csr*>(a), const_cast<fvg_prec*>(p), b, x, o, info, history, cap, true);
-    return run_bicgstab(c, a, p, b, x, o, info, history, cap);
+    return run_single(c, a, p, b, x, o, info, history, cap, true);
 }
 
 }  // extern "C"

@@ -1,0 +1,49 @@
+// krylov.hpp -- Krylov solver state and drivers shared by the single and batched paths (internal).
+#pragma once
+
+#include "internal.hpp"
+
+struct fvg_prec;
+
+namespace fvg {
+
+// Per-system solver state, resident on the device.
+struct KState {
+    double bnorm, init_res, error, atol, rtol;
+    double rz, alpha, beta, rho, omega;
+    int64_t iter, max_iter, hist_len, hist_cap, restarts;
+    int32_t done;      // converged or stopped inside an iteration: the rest of it is skipped
+    int32_t halt;      // the system no longer iterates
+    int32_t mode;      // BiCGStab direction update: 0 update, 1 restart
+    int32_t counted;   // already retired from the batch's active count
+    int32_t zero_rhs;  // |b| < 1e-14: x = 0 answered without iterating
+    int32_t pad;
+};
+
+// How the rows of the system split into members.  Default: one member owning all rows.
+struct Layout {
+    int n_inst = 1;
+    const int32_t* blk_inst = nullptr;   // member of each 256-row workgroup (device)
+    const int32_t* blk_ptr = nullptr;    // member i owns workgroups [blk_ptr[i], blk_ptr[i+1]) (device)
+};
+
+// Device state of one solve over one or more members.
+struct KRun {
+    fvg_ctx* c = nullptr;
+    int n_inst = 0;
+    KState* st = nullptr;
+    double* hist = nullptr;
+    int64_t hist_cap = 0;
+    int32_t* flags = nullptr;   // [0] members still iterating, [1] all done
+    int init(fvg_ctx* ctx, int ninst, const fvg_solver_options* o);
+    int all_done(bool* out);
+    int finish(fvg_solver_info* infos, double* const* histories, int64_t cap);
+    ~KRun();
+};
+
+int run_cg_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* b, double* x, const Layout& L,
+                 KRun& K);
+int run_bicgstab_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* b, double* x, const Layout& L,
+                       KRun& K);
+
+}  // namespace fvg

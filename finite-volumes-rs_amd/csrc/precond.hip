@@ -203,7 +203,9 @@ __device__ __forceinline__ T wait_value(const T* x, int j, int32_t* err) {
     return Bits<T>::value(v);
 }
 
-template <class T>
+// POLL: 0 re-poll pending words after a short sleep; 1 re-poll without sleeping;
+// 2 keep two polls in flight, issued a short sleep apart.
+template <class T, int POLL>
 __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
     k_sweep_persistent(int64_t nchunks, int64_t nwaves, const int32_t* __restrict__ rows,
                        const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
@@ -242,10 +244,24 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
                     __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
-                __builtin_amdgcn_s_sleep(1);
+                if (POLL == 2) {
+                    U w2[kMaxRowDeps];
 #pragma unroll
-                for (int q = 0; q < kMaxRowDeps; ++q)
-                    if ((kb + q < ke) && w[q] == Bits<T>::pending) w[q] = load_word(x, cj[q]);
+                    for (int q = 0; q < kMaxRowDeps; ++q)
+                        if ((kb + q < ke) && w[q] == Bits<T>::pending) w2[q] = load_word(x, cj[q]);
+                    __builtin_amdgcn_s_sleep(2);
+#pragma unroll
+                    for (int q = 0; q < kMaxRowDeps; ++q)
+                        if ((kb + q < ke) && w[q] == Bits<T>::pending) w[q] = load_word(x, cj[q]);
+#pragma unroll
+                    for (int q = 0; q < kMaxRowDeps; ++q)
+                        if ((kb + q < ke) && w2[q] != Bits<T>::pending && w[q] == Bits<T>::pending) w[q] = w2[q];
+                } else {
+                    if (POLL == 0) __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                    for (int q = 0; q < kMaxRowDeps; ++q)
+                        if ((kb + q < ke) && w[q] == Bits<T>::pending) w[q] = load_word(x, cj[q]);
+                }
             }
 #pragma unroll
             for (int q = 0; q < kMaxRowDeps; ++q)
@@ -461,7 +477,7 @@ static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a) {
 template <class T>
 static int max_coresident_groups(fvg_ctx* c) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent<T>, kChunk * kWavesPerGroup, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent<T, 0>, kChunk * kWavesPerGroup, 0) !=
         hipSuccess)
         return 1;
     const int m = per_cu * c->num_cus;
@@ -491,8 +507,10 @@ static int launch_sweep(fvg_ctx* c, const Sweep& sw, const T* val, const T* scal
     const int64_t groups = nwaves / kWavesPerGroup;
     void* args[] = {&nchunks, &nwaves, (void*)&sw.rows, (void*)&sw.ptr, (void*)&sw.col, (void*)&val, (void*)&scale,
                     (void*)&b, (void*)&x, (void*)&err, (void*)&skip};
-    FVG_HIP(hipLaunchCooperativeKernel((const void*)k_sweep_persistent<T>, dim3((unsigned)groups),
-                                       dim3(kChunk * kWavesPerGroup), args, 0, c->stream));
+    const void* fn = c->tune_sweep_poll == 1   ? (const void*)k_sweep_persistent<T, 1>
+                     : c->tune_sweep_poll == 2 ? (const void*)k_sweep_persistent<T, 2>
+                                               : (const void*)k_sweep_persistent<T, 0>;
+    FVG_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)groups), dim3(kChunk * kWavesPerGroup), args, 0, c->stream));
     return FVG_OK;
 }
 

@@ -18,7 +18,11 @@ struct SpmvEpilogue {
     const double* dotw = nullptr;
     const double* dotw2 = nullptr;
     bool dot_self = false;
-    const int32_t* skip = nullptr;   // if set and *skip != 0 the launch does nothing
+    // if set, a workgroup does nothing when skip[member * skip_stride] != 0, member being
+    // blk_inst[workgroup] (or 0 without blk_inst)
+    const int32_t* skip = nullptr;
+    int32_t skip_stride = 0;
+    const int32_t* blk_inst = nullptr;
 };
 
 // Number of partial sums one fused SpMV writes per dot (one per workgroup).

@@ -40,8 +40,8 @@ def test_mixed_precision_reaches_fp64_accuracy(ctx, name, kind, solver, prec):
     assert info.iterations <= 3 * io["iterations"] + 20
 
 
-@pytest.mark.parametrize("mode,waves", [(1, 0), (0, 4), (0, 64), (0, 0)])
-def test_sweep_launch_shapes_give_identical_bits(ctx, mode, waves):
+@pytest.mark.parametrize("mode,waves,poll", [(1, 0, 0), (0, 4, 0), (0, 64, 0), (0, 0, 0), (0, 0, 1), (0, 0, 2)])
+def test_sweep_launch_shapes_give_identical_bits(ctx, mode, waves, poll):
     """One launch per level, and the persistent sweep at several wave counts, all give the
     oracle's bits (the schedule never changes the arithmetic)."""
     rp, ci, va, b = _oracle_system("tet10", "sym")
@@ -52,9 +52,11 @@ def test_sweep_launch_shapes_give_identical_bits(ctx, mode, waves):
     try:
         ctx.set_tuning(fvg.TUNE_SWEEP_MODE, mode)
         ctx.set_tuning(fvg.TUNE_SWEEP_WAVES, waves)
+        ctx.set_tuning(fvg.TUNE_SWEEP_POLL, poll)
         z = fvg.DistributedVector(ctx, n)
         ic.precondition(z, fvg.DistributedVector.from_data(ctx, np.random.default_rng(1).normal(size=n)))
         assert np.array_equal(z.numpy(), ref)
     finally:
         ctx.set_tuning(fvg.TUNE_SWEEP_MODE, 0)
         ctx.set_tuning(fvg.TUNE_SWEEP_WAVES, 0)
+        ctx.set_tuning(fvg.TUNE_SWEEP_POLL, 0)
