@@ -134,6 +134,15 @@ def lib():
         "fvg_prec_nonpositive_pivots": [P, P, C.POINTER(C.c_int64)],
         "fvg_cg": [P, P, P, P, P, C.POINTER(SolverOptions), C.POINTER(SolverInfo), P, C.c_int64],
         "fvg_bicgstab": [P, P, P, P, P, C.POINTER(SolverOptions), C.POINTER(SolverInfo), P, C.c_int64],
+        "fvg_batch_create": [P, C.c_int32, P, P, C.c_int32, C.POINTER(SolverOptions), C.POINTER(P)],
+        "fvg_batch_destroy": [P, P],
+        "fvg_batch_size": [P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)],
+        "fvg_batch_launch": [P, P, P, P],
+        "fvg_batch_query": [P, P, C.POINTER(C.c_int32)],
+        "fvg_batch_finish": [P, P, P, P, C.c_int64],
+        "fvg_batch_solve": [P, P, P, P, P, P, C.c_int64],
+        "fvg_batch_set_threads": [P, P, C.c_int32],
+        "fvg_batch_get_threads": [P, C.POINTER(C.c_int32)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -691,3 +700,96 @@ def bi_conjugate_gradient_stab(solution, lhs, rhs, preconditioner, options=None)
     """bi_conjugate_gradient_stab (src/linalg/solvers/bi_conjugate_gradient_stab.rs:13-179)."""
     return _solve(lib().fvg_bicgstab, "BiConjugateGradientStab", solution, lhs, rhs, preconditioner,
                   options or LinearSolverOptions())
+
+
+# ---------------------------------------------------------------------------------
+# batched solves (no reference counterpart)
+# ---------------------------------------------------------------------------------
+
+SOLVER_CG = 0
+SOLVER_BICGSTAB = 1
+
+
+class Batch:
+    """Many independent systems solved by one launch, one workgroup per system.
+
+    ``matrices[k]`` / ``preconditioners[k]`` stay owned by the caller and must outlive
+    the batch.  ``solver`` is ``"cg"`` or ``"bicgstab"``; ``options.precision`` selects the
+    FP64 reference algorithm or mixed precision.  Each system's result does not depend on
+    the rest of the batch.
+    """
+
+    def __init__(self, ctx, matrices, preconditioners, solver="cg", options=None):
+        if len(matrices) != len(preconditioners) or not matrices:
+            raise ValueError("one preconditioner per matrix, at least one system")
+        self.ctx = ctx
+        self.options = options or LinearSolverOptions()
+        self.solver = SOLVER_CG if solver == "cg" else SOLVER_BICGSTAB
+        self.n = len(matrices)
+        self._keep = (list(matrices), list(preconditioners))
+        am = (P * self.n)(*[m.h for m in matrices])
+        pm = (P * self.n)(*[p.h for p in preconditioners])
+        h = P()
+        opts = self.options.c()
+        check(lib().fvg_batch_create(ctx.h, self.n, am, pm, self.solver, C.byref(opts), C.byref(h)))
+        self.h = h
+        self._inflight = None
+
+    @property
+    def threads(self):
+        t = C.c_int32()
+        check(lib().fvg_batch_get_threads(self.h, C.byref(t)))
+        return t.value
+
+    @threads.setter
+    def threads(self, value):
+        check(lib().fvg_batch_set_threads(self.ctx.h, self.h, int(value)))
+
+    def launch(self, rhs, solutions):
+        """Enqueue the solves (solutions: initial guesses in, answers out); returns at once."""
+        if len(rhs) != self.n or len(solutions) != self.n:
+            raise ValueError("one right-hand side and one solution per system")
+        bv = (P * self.n)(*[v.h for v in rhs])
+        xv = (P * self.n)(*[v.h for v in solutions])
+        check(lib().fvg_batch_launch(self.ctx.h, self.h, bv, xv))
+        self._inflight = (rhs, solutions)
+
+    def done(self):
+        d = C.c_int32()
+        check(lib().fvg_batch_query(self.ctx.h, self.h, C.byref(d)))
+        return bool(d.value)
+
+    def finish(self, history=True):
+        """Wait for the launched solves; returns one LinearSolverInfo per system."""
+        infos = (SolverInfo * self.n)()
+        cap = max(2, self.options.max_iterations + 2) if self.options.precision == PRECISION_FP64 else 64
+        hists = [np.zeros(cap) for _ in range(self.n)] if history else None
+        hp = (P * self.n)(*[_ptr(hh) for hh in hists]) if history else None
+        check(lib().fvg_batch_finish(self.ctx.h, self.h, infos, hp, cap if history else 0))
+        self._inflight = None
+        name = "ConjugateGradient" if self.solver == SOLVER_CG else "BiConjugateGradientStab"
+        out = []
+        for k in range(self.n):
+            i = infos[k]
+            hl = min(i.history_len, cap)
+            out.append(LinearSolverInfo(name, i.iterations, i.initial_residual, i.final_residual,
+                                        hists[k][:hl].copy() if history else np.zeros(0), i.restarts,
+                                        i.outer_iterations, i.true_residual))
+        return out
+
+    def solve(self, rhs, solutions, history=True):
+        self.launch(rhs, solutions)
+        return self.finish(history)
+
+    def close(self):
+        if getattr(self, "h", None):
+            if self._inflight is not None:
+                lib().fvg_batch_finish(self.ctx.h, self.h, None, None, 0)
+            lib().fvg_batch_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

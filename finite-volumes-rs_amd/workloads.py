@@ -16,6 +16,8 @@ Workloads (sizes as in the benchmark definition):
   convdiff_prism   128x128x2 triangles extruded through 64 layers -> 2,097,152 prisms,
                    velocity (1,0.5,0.25) scaled per cell to Peclet U[1,100], k = 1e-2,
                    upwind, inflow u = 1 / outflow zero gradient, BiCGStab + ILU(0)
+  batched_2d       256 independent jittered triangle meshes, cells log-uniform in
+                   [1k, 64k], ln k ~ N(0, 1) per cell, f ~ N(0,1), u = 0, CG + IC(0)
 """
 from __future__ import annotations
 
@@ -94,6 +96,29 @@ def convdiff_prism(nx=128, layers=64, seed=1, pe=(1.0, 100.0)):
         return convdiff_fields(raw, geom, pe=pe, seed=seed + 3000)
 
     return raw, fields
+
+
+def batched_2d(n_inst=256, seed=1, lo=1000, hi=64000, sigma=1.0):
+    """Independent 2D diffusion instances with cell counts log-uniform in [lo, hi]:
+    jittered triangle meshes (2 nx^2 cells with nx = round(sqrt(n/2))), per-cell log-normal
+    k with the given sigma, f ~ N(0,1) per cell (each instance its own right-hand side),
+    u = 0 on the boundary, CG + IC(0).  Returns a list of (raw mesh, fields) pairs."""
+    rng = np.random.default_rng(seed)
+    targets = np.exp(rng.uniform(np.log(lo), np.log(hi), n_inst))
+    out = []
+    for k, t in enumerate(targets):
+        nx = max(2, int(round(np.sqrt(t / 2.0))))
+        s = seed * 100003 + k
+        raw = meshgen.tri2d(nx, jitter=0.25, seed=s)
+        n = raw["cell_ptr"].shape[0] - 1
+        kc = meshgen.lognormal_cells(n, sigma, s + 1)
+        f = np.random.default_rng(s + 2).standard_normal(n)
+
+        def fields(geom, raw=raw, kc=kc, f=f):
+            return diffusion_fields(raw, geom, kc, f, bc="zero")
+
+        out.append((raw, fields))
+    return out
 
 
 WORKLOADS = {"diffusion_2d": diffusion_2d, "diffusion_3d": diffusion_3d, "convdiff_prism": convdiff_prism}

@@ -232,6 +232,36 @@ int fvg_bicgstab(fvg_ctx* ctx, const fvg_csr* a, const fvg_prec* p, const fvg_ve
                  const fvg_solver_options* opts, fvg_solver_info* info, double* history,
                  int64_t history_capacity);
 
+/* ---- batched solves (no reference counterpart: many independent systems) ----- */
+/* A batch holds n independent systems A_k x_k = b_k, each with its own matrix and
+ * preconditioner (identity, Jacobi, IC(0) or ILU(0)), solved by one kernel launch in
+ * which one workgroup runs one system's whole solve.  FP64 batches run the reference
+ * algorithm of fvg_cg / fvg_bicgstab per system; mixed batches run FP32 inner
+ * iterations inside FP64 refinement.  Each system's result is independent of the rest
+ * of the batch.  Matrices and preconditioners stay owned by the caller and must
+ * outlive the batch; values assembled or refactored later are picked up at the next
+ * launch.  A batch launches on a stream of its own, after all work enqueued on the
+ * context stream so far, so batches overlap on the device. */
+enum { FVG_SOLVER_CG = 0, FVG_SOLVER_BICGSTAB = 1 };
+typedef struct fvg_batch fvg_batch;
+int fvg_batch_create(fvg_ctx* ctx, int32_t n, const fvg_csr* const* a, const fvg_prec* const* p, int32_t solver,
+                     const fvg_solver_options* opts, fvg_batch** out);
+int fvg_batch_destroy(fvg_ctx* ctx, fvg_batch* batch);
+int fvg_batch_size(const fvg_batch* batch, int32_t* n_systems, int64_t* total_rows);
+/* Enqueues the solve of every system (x_k: initial guess in, solution out). */
+int fvg_batch_launch(fvg_ctx* ctx, fvg_batch* batch, const fvg_vec* const* b, fvg_vec* const* x);
+/* 1 in *done once a launched batch has finished (does not block). */
+int fvg_batch_query(fvg_ctx* ctx, fvg_batch* batch, int32_t* done);
+/* Waits for the launched batch; infos[k] and histories[k] (each may be NULL) per system. */
+int fvg_batch_finish(fvg_ctx* ctx, fvg_batch* batch, fvg_solver_info* infos, double* const* histories,
+                     int64_t history_capacity);
+/* launch + finish */
+int fvg_batch_solve(fvg_ctx* ctx, fvg_batch* batch, const fvg_vec* const* b, fvg_vec* const* x,
+                    fvg_solver_info* infos, double* const* histories, int64_t history_capacity);
+/* Threads per system (256, 512 or 1024; default 1024). */
+int fvg_batch_set_threads(fvg_ctx* ctx, fvg_batch* batch, int32_t threads);
+int fvg_batch_get_threads(const fvg_batch* batch, int32_t* threads);
+
 /* ---- device timing of the most recent solve ------------------------------- */
 typedef struct fvg_kernel_stats {
     double spmv_ms;             /* total time of the SpMV kernels */

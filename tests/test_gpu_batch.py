@@ -1,0 +1,206 @@
+"""Batched solves: many independent systems in one launch, one workgroup per system.
+
+Bars:
+- FP64 batches run the reference algorithm per system. Each system's solution is within
+  1e-6 relative of the oracle's at a 1e-8 relative residual, and its iteration count is
+  within +-2 of the oracle's.
+- Mixed batches (FP32 inner, FP64 refinement) reach a true relative residual <= 1e-8, and x
+  is within 1e-6 relative of the FP64 oracle solution.
+- Every system gives the same bits alone and in any mix.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import fvg
+import meshgen
+import oracle as O
+import problems
+
+pytestmark = pytest.mark.gpu
+
+OPTS = dict(relative_tolerance=1e-8, absolute_tolerance=0.0, max_iterations=3000)
+
+
+def _systems(kind, meshes, seed0):
+    out = []
+    for j, raw in enumerate(meshes):
+        om = O.OracleMesh(raw)
+        geom = om.geometry()
+        n = om.n_cells
+        if kind == "sym":
+            f = np.random.default_rng(seed0 + j).normal(size=n)
+            prob = problems.diffusion(raw, geom, kc=meshgen.lognormal_cells(n, 1.0, seed0 + j), f=f, bc="zero")
+        else:
+            prob = problems.convection_diffusion(raw, geom, pe=(1.0, 50.0), seed=seed0 + j)
+        out.append(om.assemble(prob["prog"], prob["face_fields"], prob["cell_fields"], prob["bc_i"], prob["bc_e"]))
+    return out
+
+
+def _sym_meshes():
+    return [meshgen.tri2d(12, seed=1), meshgen.tri2d(40, seed=2), meshgen.tet3d(5, seed=3), meshgen.tri2d(7, seed=4),
+            meshgen.tri2d(25, seed=5), meshgen.tri2d(3, seed=6)]
+
+
+def _nonsym_meshes():
+    return [meshgen.tri2d(20, seed=7), meshgen.prism3d(6, layers=3, seed=8), meshgen.tri2d(9, seed=9)]
+
+
+class _Batch:
+    """Device objects for a list of systems plus a batch over them."""
+
+    def __init__(self, ctx, systems, prec, solver, precision=fvg.PRECISION_FP64, order=None):
+        self.systems = systems
+        order = list(range(len(systems))) if order is None else order
+        self.order = order
+        self.A, self.M, self.b, self.x = [], [], [], []
+        for k in order:
+            rp, ci, va, b = systems[k]
+            A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+            kinds = prec if isinstance(prec, list) else [prec] * len(systems)
+            pk = kinds[k]
+            if pk == "ic":
+                M = fvg.IncompleteCholesky.from_matrix(A, 0)
+            elif pk == "ilu":
+                M = fvg.IncompleteLowerUpper.from_matrix(A, 0)
+            elif pk == "diagonal":
+                M = fvg.DiagonalPreconditioner.from_matrix(A)
+            else:
+                M = fvg.IdentityPreconditioner.new(ctx, rp.shape[0] - 1)
+            self.A.append(A)
+            self.M.append(M)
+            self.b.append(fvg.DistributedVector.from_data(ctx, b))
+            self.x.append(fvg.DistributedVector(ctx, rp.shape[0] - 1))
+        self.opts = fvg.LinearSolverOptions(**OPTS, precision=precision)
+        self.batch = fvg.Batch(ctx, self.A, self.M, solver, self.opts)
+
+    def solve(self):
+        for x in self.x:
+            x.fill(0.0)
+        infos = self.batch.solve(self.b, self.x)
+        return {k: (self.x[j].numpy(), infos[j]) for j, k in enumerate(self.order)}
+
+
+def _check_against_oracle(systems, results, prec, solver):
+    kinds = prec if isinstance(prec, list) else [prec] * len(systems)
+    for k, (rp, ci, va, b) in enumerate(systems):
+        P = O.OraclePrec(kinds[k], rp, ci, va, 0)
+        xo, io = O.solve(solver, rp, ci, va, P, b, rtol=1e-8, atol=0.0, max_iter=3000)
+        xg, ig = results[k]
+        assert abs(ig.iterations - io["iterations"]) <= 2, (k, ig.iterations, io["iterations"])
+        assert np.linalg.norm(xg - xo) <= 1e-6 * np.linalg.norm(xo), k
+        assert ig.final_residual < 1e-8 * ig.initial_residual
+        assert abs(ig.initial_residual - io["initial_residual"]) <= 1e-12 * io["initial_residual"]
+        assert ig.history.shape[0] == ig.iterations
+        assert ig.history[-1] == ig.final_residual
+
+
+def test_batch_cg_ic0_matches_oracle_per_system(ctx):
+    systems = _systems("sym", _sym_meshes(), 10)
+    B = _Batch(ctx, systems, "ic", "cg")
+    _check_against_oracle(systems, B.solve(), "ic", "cg")
+
+
+def test_batch_bicgstab_ilu0_matches_oracle_per_system(ctx):
+    systems = _systems("nonsym", _nonsym_meshes(), 20)
+    B = _Batch(ctx, systems, "ilu", "bicgstab")
+    _check_against_oracle(systems, B.solve(), "ilu", "bicgstab")
+
+
+def test_batch_mixes_preconditioner_kinds(ctx):
+    systems = _systems("sym", [meshgen.tri2d(10, seed=31), meshgen.tri2d(14, seed=32), meshgen.tri2d(6, seed=33)], 30)
+    kinds = ["ic", "diagonal", "identity"]
+    B = _Batch(ctx, systems, kinds, "cg")
+    _check_against_oracle(systems, B.solve(), kinds, "cg")
+
+
+@pytest.mark.parametrize("solver,kind,prec,precision", [("cg", "sym", "ic", fvg.PRECISION_FP64),
+                                                        ("cg", "sym", "ic", fvg.PRECISION_MIXED),
+                                                        ("bicgstab", "nonsym", "ilu", fvg.PRECISION_FP64),
+                                                        ("bicgstab", "nonsym", "ilu", fvg.PRECISION_MIXED)])
+def test_batch_result_is_the_same_alone_and_in_the_mix(ctx, solver, kind, prec, precision):
+    meshes = _sym_meshes() if kind == "sym" else _nonsym_meshes()
+    systems = _systems(kind, meshes, 40)
+    full = _Batch(ctx, systems, prec, solver, precision).solve()
+    rev = _Batch(ctx, systems, prec, solver, precision, order=list(reversed(range(len(systems))))).solve()
+    for k in range(len(systems)):
+        alone = _Batch(ctx, [systems[k]], prec, solver, precision).solve()[0]
+        for other in (full[k], rev[k]):
+            assert np.array_equal(alone[0], other[0]), k
+            assert alone[1].iterations == other[1].iterations
+            assert alone[1].final_residual == other[1].final_residual
+            assert np.array_equal(alone[1].history, other[1].history)
+
+
+@pytest.mark.parametrize("solver,kind,prec", [("cg", "sym", "ic"), ("bicgstab", "nonsym", "ilu")])
+def test_batch_mixed_precision_reaches_fp64_accuracy(ctx, solver, kind, prec):
+    meshes = _sym_meshes() if kind == "sym" else _nonsym_meshes()
+    systems = _systems(kind, meshes, 50)
+    res = _Batch(ctx, systems, prec, solver, fvg.PRECISION_MIXED).solve()
+    for k, (rp, ci, va, b) in enumerate(systems):
+        n = rp.shape[0] - 1
+        xg, ig = res[k]
+        S = sp.csr_matrix((va, ci, rp), shape=(n, n))
+        true_rel = np.linalg.norm(b - S @ xg) / np.linalg.norm(b)
+        assert true_rel <= 1e-8, (k, true_rel)
+        assert abs(ig.true_residual - true_rel) <= 1e-3 * true_rel + 1e-15
+        xo, _ = O.solve(solver, rp, ci, va, O.OraclePrec(prec, rp, ci, va, 0), b, rtol=1e-8, atol=0.0, max_iter=3000)
+        assert np.linalg.norm(xg - xo) <= 1e-6 * np.linalg.norm(xo), k
+
+
+def test_batch_graph_replays_are_identical(ctx):
+    """The batch launch is a captured graph; replays with eager work in between give the same bits."""
+    systems = _systems("sym", _sym_meshes()[:4], 60)
+    B = _Batch(ctx, systems, "ic", "cg")
+    first = B.solve()
+    scratch = fvg.DistributedVector(ctx, systems[0][0].shape[0] - 1)
+    for _ in range(3):
+        B.A[0].imul(scratch, B.b[0])   # eager work on the context stream between replays
+        ctx.sync()
+        again = B.solve()
+        for k in first:
+            assert np.array_equal(first[k][0], again[k][0])
+            assert first[k][1].iterations == again[k][1].iterations
+
+
+def test_batch_picks_up_new_values_after_refactor(ctx):
+    systems = _systems("sym", _sym_meshes()[:3], 70)
+    B = _Batch(ctx, systems, "ic", "cg")
+    B.solve()
+    scaled = []
+    for j, (rp, ci, va, b) in enumerate(systems):
+        va2 = va * (1.5 + j)
+        B.A[j].set_values(va2)
+        B.M[j].refactor(B.A[j])
+        scaled.append((rp, ci, va2, b))
+    _check_against_oracle(scaled, B.solve(), "ic", "cg")
+
+
+@pytest.mark.parametrize("threads", [256, 512])
+def test_batch_workgroup_sizes(ctx, threads):
+    systems = _systems("sym", _sym_meshes(), 80)
+    B = _Batch(ctx, systems, "ic", "cg")
+    B.batch.threads = threads
+    assert B.batch.threads == threads
+    _check_against_oracle(systems, B.solve(), "ic", "cg")
+
+
+def test_batch_zero_rhs_and_errors(ctx):
+    systems = _systems("sym", [meshgen.tri2d(8, seed=90), meshgen.tri2d(5, seed=91)], 90)
+    rp, ci, va, b = systems[1]
+    systems[1] = (rp, ci, va, np.zeros_like(b))
+    B = _Batch(ctx, systems, "ic", "cg")
+    for x in B.x:
+        x.fill(3.0)
+    infos = B.batch.solve(B.b, B.x)
+    assert np.all(B.x[1].numpy() == 0.0)
+    assert infos[1].iterations == 0 and infos[1].final_residual == 0.0
+    assert infos[0].iterations > 0
+    # a structurally unsymmetric pattern is refused for factorised preconditioners
+    rp3 = np.array([0, 2, 3, 4], np.int32)
+    ci3 = np.array([0, 1, 1, 2], np.int32)
+    va3 = np.array([2.0, -1.0, 2.0, 2.0])
+    A3 = fvg.DistributedMatrix.from_csr(ctx, rp3, ci3, va3)
+    M3 = fvg.IncompleteLowerUpper.from_matrix(A3, 0)
+    with pytest.raises(fvg.FvgError):
+        fvg.Batch(ctx, [A3], [M3], "bicgstab", fvg.LinearSolverOptions(**OPTS))
