@@ -5,15 +5,25 @@
 // single persistent workgroup: SpMV, triangular sweeps, vector updates and dot products
 // all run inside it, separated by workgroup barriers, so a system never waits on another
 // workgroup and the whole batch is one kernel launch (replayed from a captured hipGraph).
-// That makes each result independent of what else is in the batch: a system gives the
+// Each result is therefore independent of what else is in the batch: a system gives the
 // same bits alone and in any mix, and batches on different streams overlap freely.
 //
 // Storage is permuted into "positions": rows sorted by the level of the lower triangular
 // sweep (dependency depth), so a level is a contiguous range.  The upper sweep runs the
 // same levels in reverse (valid for structurally symmetric patterns, which mesh matrices
-// are), so both sweeps, the SpMV and every vector pass stream contiguous ranges.  Entries
-// keep the original ascending column order, so per-row sums are the reference's
-// (dmatrix.rs:136-146, incomplete_cholesky.rs:217-254, incomplete_lower_upper.rs:287-328).
+// are).  Entries keep the original ascending column order, so per-row sums are the
+// reference's (dmatrix.rs:136-146, incomplete_cholesky.rs:217-254,
+// incomplete_lower_upper.rs:287-328).
+//
+// Triangular sweeps inside the workgroup: the rows of a sweep are cut into steps (a
+// level, or a slice of a level) and the steps into groups that fit one LDS stage buffer.
+// While a group is computed step by step (one barrier per step), the next group's static
+// data (step bounds, row pointers, dependency positions, factor values, scales) and its
+// right-hand side are already in flight into registers; they land in the second stage
+// buffer at the end of the group.  Solved values go to an LDS ring indexed by position,
+// from which later rows read their dependencies (when every dependency lies within the
+// ring's span, which the host checks per sweep), and to HBM for the passes after the
+// sweep.
 //
 // Solver variants (the algorithm steps are those of krylov.hip / mixed.hip):
 //   CG64 / BI64   conjugate_gradient / bi_conjugate_gradient_stab in FP64, with the
@@ -24,6 +34,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "precond.hpp"
 #include "reduce.hpp"
@@ -32,34 +43,63 @@ namespace fvg {
 
 enum TeamVariant { kCG64 = 0, kBI64 = 1, kCG32 = 2, kBI32 = 3 };
 
+constexpr int kRing64 = 16384;  // LDS ring of values by position (sweeps, SpMV window), FP64
+constexpr int kRing32 = 32768;  // FP32
+constexpr int kES = 4;          // factor entries per row held in registers (the rest read on use)
+constexpr int kEA = 4;          // matrix entries per row held in registers (the rest read on use)
+constexpr int kMaxGroup = 1024; // largest workgroup: the host's ring checks assume groups this large
+
+struct TeamSweep {
+    const int32_t* lvl;      // level of each processing index (non-decreasing)
+    int32_t nnz1;            // index of the last entry (dep / val hold at least one element)
+    int32_t pad1;
+    const int32_t* ptr;      // entries of processing index q: [ptr[q], ptr[q+1])
+    const int32_t* dep;      // position of each dependency
+    const double* val;
+    const float* val32;
+    int32_t ring64;          // every dependency lies within the FP64 ring's span
+    int32_t ring32;
+    int32_t upper;           // processing index q is position n-1-q
+    int32_t pad;
+};
+
 // device-side description of one system (uploaded once per batch, updated when b/x move)
 struct TeamDesc {
-    int32_t n, nlev, kind, pad;
-    const int32_t* perm;    // natural row of each position
-    const int32_t* lvl;     // lower-sweep level l holds positions [lvl[l], lvl[l+1])
-    const int32_t* a_ptr;   // A by position: entries [a_ptr[p], a_ptr[p+1])
-    const int32_t* a_dep;   // position of each entry's column
+    int32_t n, npad, kind, wspan;   // wspan: largest |position - column position| of A
+    int32_t win64, win32, a_nnz1, pad1;   // the SpMV x window fits the FP64 / FP32 ring; last entry of A
+    const int32_t* perm;     // natural row of each position
+    const int32_t* a_ptr;    // A by position
+    const int32_t* a_dep;
     const double* a_val;
     const float* a_val32;
-    const int32_t* l_ptr;   // strict lower factor by position
-    const int32_t* l_dep;
-    const double* l_val;
-    const float* l_val32;
-    const int32_t* u_ptr;   // strict upper factor by processing index q (position n-1-q)
-    const int32_t* u_dep;
-    const double* u_val;
-    const float* u_val32;
-    const double* sc;       // per position: IC 1/l_ii, ILU 1/u_ii, Jacobi 1/a_ii
+    TeamSweep lo, up;
+    const double* sc;        // per position: IC 1/l_ii, ILU 1/u_ii, Jacobi 1/a_ii
     const float* sc32;
-    const double* b;        // caller vectors, natural order
+    const double* b;         // caller vectors, natural order
     double* x;
-    double* w64;            // workspace
+    double* w64;             // workspace, vectors of npad elements
     float* w32;
     double* hist;
     int64_t hcap;
     fvg_solver_info* info;
     double atol, rtol, inner_tol;
     int64_t max_iter;
+    int64_t* prof;           // optional: wall-clock ticks spent in SpMV, lower sweep, upper sweep, total
+};
+
+// ---- optional phase timing (thread 0's view of its workgroup) ----------------------------
+__shared__ long long g_prof[4];
+__shared__ int g_prof_on;
+
+struct ProfScope {
+    int ph;
+    long long t0;
+    __device__ explicit ProfScope(int p) : ph(p), t0(0) {
+        if (threadIdx.x == 0 && g_prof_on) t0 = wall_clock64();
+    }
+    __device__ ~ProfScope() {
+        if (threadIdx.x == 0 && g_prof_on) g_prof[ph] += wall_clock64() - t0;
+    }
 };
 
 // ---- workgroup reductions (result in every thread, fixed order) -------------------------
@@ -106,40 +146,125 @@ __device__ __forceinline__ const double* pick<double>(const double* d, const flo
 template <>
 __device__ __forceinline__ const float* pick<float>(const double*, const float* f) { return f; }
 
-// ---- preconditioner apply: z = M^-1 r (ends with a barrier) -----------------------------
+// ---- triangular sweeps: one row per thread per group of NT rows ------------------------
+// Rows are taken in processing order in groups of NT; thread t owns row t of each group.
+// A group spans levels [La, Lb] of the sweep; the group is computed level by level with
+// one barrier per level, each thread solving its row at its level.  A row's data (first
+// kES entries, right-hand side, scale, level) is fetched into registers while the
+// previous group is being computed, and its entry range two groups ahead.  Every fetch is
+// unconditional (indices clamped into range), so each stage issues the same number of
+// loads and the wait for a stage's data never includes the loads issued after it.
+// Solved values go to the LDS ring (indexed by position) and to HBM.
+template <class T>
+struct SweepRow {
+    int kb, cnt, lvl;
+    int dep[kES];
+    T val[kES];
+    T b, sc;
+};
+
+template <int NT>
+__device__ __forceinline__ void sweep_ptr(const TeamSweep& S, int n, int g, int& kb, int& ke) {
+    const int q = g * NT + (int)threadIdx.x;
+    const int qc = min(q, n - 1);
+    kb = S.ptr[qc];
+    ke = S.ptr[qc + 1];
+    if (q >= n) ke = kb;
+}
+
 template <int NT, class T>
-__device__ void team_lower(const TeamDesc& D, const T* __restrict__ val, const T* sc, const T* b, T* x) {
-    for (int l = 0; l < D.nlev; ++l) {
-        const int p0 = D.lvl[l], p1 = D.lvl[l + 1];
-        for (int p = p0 + (int)threadIdx.x; p < p1; p += NT) {
-            T s = b[p];
-            const int ke = D.l_ptr[p + 1];
-            for (int k = D.l_ptr[p]; k < ke; ++k) s -= val[k] * x[D.l_dep[k]];
-            x[p] = sc ? sc[p] * s : s;
+__device__ __forceinline__ void sweep_row(SweepRow<T>& R, const TeamSweep& S, const T* __restrict__ val,
+                                          const T* __restrict__ sc, const T* __restrict__ b, int n, int g, int kb,
+                                          int ke) {
+    const int q = g * NT + (int)threadIdx.x;
+    const int qc = min(q, n - 1);
+    const int pos = S.upper ? n - 1 - qc : qc;
+    R.kb = kb;
+    R.cnt = ke - kb;
+    const int lv = S.lvl[qc];
+    R.lvl = q < n ? lv : -1;
+    R.b = b[pos];
+    R.sc = sc[pos];
+    const int last = max(R.cnt - 1, 0);
+#pragma unroll
+    for (int e = 0; e < kES; ++e) {
+        const int k = min(kb + min(e, last), S.nnz1);
+        R.dep[e] = S.dep[k];
+        R.val[e] = val[k];
+    }
+}
+
+// levels spanned by group g
+template <int NT>
+__device__ __forceinline__ void group_levels(const TeamSweep& S, int n, int g, int& la, int& lb) {
+    const int q0 = g * NT;
+    const int q1 = min(q0 + NT, n);
+    la = S.lvl[q0];
+    lb = S.lvl[q1 - 1];
+}
+
+template <int NT, class T, int RING>
+__device__ __forceinline__ void sweep_compute(const SweepRow<T>& R, const TeamSweep& S, const T* __restrict__ val,
+                                              bool has_sc, bool ring_ok, int n, int g, int la, int lb, T* out,
+                                              T* ring) {
+    const int q = g * NT + (int)threadIdx.x;
+    const int pos = S.upper ? n - 1 - q : q;
+    for (int L = la; L <= lb; ++L) {
+        if (R.lvl == L) {
+            T s = R.b;
+            if (ring_ok) {
+#pragma unroll
+                for (int e = 0; e < kES; ++e)
+                    if (e < R.cnt) s -= R.val[e] * ring[R.dep[e] & (RING - 1)];
+                for (int k = R.kb + kES; k < R.kb + R.cnt; ++k) s -= val[k] * ring[S.dep[k] & (RING - 1)];
+            } else {
+#pragma unroll
+                for (int e = 0; e < kES; ++e)
+                    if (e < R.cnt) s -= R.val[e] * out[R.dep[e]];
+                for (int k = R.kb + kES; k < R.kb + R.cnt; ++k) s -= val[k] * out[S.dep[k]];
+            }
+            const T r = has_sc ? R.sc * s : s;
+            if (ring_ok) ring[pos & (RING - 1)] = r;
+            out[pos] = r;
         }
         __syncthreads();
     }
 }
 
-template <int NT, class T>
-__device__ void team_upper(const TeamDesc& D, const T* __restrict__ val, const T* sc, const T* b, T* x) {
-    const int n = D.n;
-    for (int l = D.nlev - 1; l >= 0; --l) {
-        const int p0 = D.lvl[l], p1 = D.lvl[l + 1];
-        for (int t = (int)threadIdx.x; t < p1 - p0; t += NT) {
-            const int p = p1 - 1 - t;
-            const int q = n - 1 - p;
-            T s = b[p];
-            const int ke = D.u_ptr[q + 1];
-            for (int k = D.u_ptr[q]; k < ke; ++k) s -= val[k] * x[D.u_dep[k]];
-            x[p] = sc[p] * s;
-        }
-        __syncthreads();
+// x = sweep(b); sc may be absent (unit diagonal), in which case scl is still a valid
+// array of n values (its loads are discarded)
+template <int NT, class T, int RING>
+__device__ void team_sweep(const TeamSweep& Sd, const T* val, const T* sc, const T* scl, const T* b, T* out, int n,
+                           T* ring) {
+    const TeamSweep S = Sd;   // registers: no descriptor reads inside the pipelined loop
+    ProfScope prof(S.upper ? 2 : 1);
+    const bool ring_ok = sizeof(T) == 8 ? S.ring64 != 0 : S.ring32 != 0;
+    const bool has_sc = sc != nullptr;
+    const int G = (n + NT - 1) / NT;
+    SweepRow<T> A, B;
+    int akb, ake, bkb, bke;
+    int ala, alb, bla, blb;
+    sweep_ptr<NT>(S, n, 0, akb, ake);
+    sweep_row<NT, T>(A, S, val, scl, b, n, 0, akb, ake);
+    group_levels<NT>(S, n, 0, ala, alb);
+    sweep_ptr<NT>(S, n, min(1, G - 1), bkb, bke);
+    for (int g = 0; g < G; g += 2) {
+        const int g1 = min(g + 1, G - 1), g2 = min(g + 2, G - 1), g3 = min(g + 3, G - 1);
+        sweep_row<NT, T>(B, S, val, scl, b, n, g1, bkb, bke);
+        group_levels<NT>(S, n, g1, bla, blb);
+        sweep_ptr<NT>(S, n, g2, akb, ake);
+        sweep_compute<NT, T, RING>(A, S, val, has_sc, ring_ok, n, g, ala, alb, out, ring);
+        if (g + 1 >= G) break;
+        sweep_row<NT, T>(A, S, val, scl, b, n, g2, akb, ake);
+        group_levels<NT>(S, n, g2, ala, alb);
+        sweep_ptr<NT>(S, n, g3, bkb, bke);
+        sweep_compute<NT, T, RING>(B, S, val, has_sc, ring_ok, n, g + 1, bla, blb, out, ring);
     }
 }
 
-template <int NT, class T>
-__device__ void team_prec(const TeamDesc& D, const T* r, T* z, T* tmp) {
+// z = M^-1 r (ends with a barrier); tmp holds the intermediate vector
+template <int NT, class T, int RING>
+__device__ void team_prec(const TeamDesc& D, const T* r, T* z, T* tmp, T* ring) {
     const T* sc = pick<T>(D.sc, D.sc32);
     switch (D.kind) {
     case kIdentity:
@@ -151,46 +276,302 @@ __device__ void team_prec(const TeamDesc& D, const T* r, T* z, T* tmp) {
         __syncthreads();
         break;
     case kIC:
-        team_lower<NT, T>(D, pick<T>(D.l_val, D.l_val32), sc, r, tmp);
-        team_upper<NT, T>(D, pick<T>(D.u_val, D.u_val32), sc, tmp, z);
+        team_sweep<NT, T, RING>(D.lo, pick<T>(D.lo.val, D.lo.val32), sc, sc, r, tmp, D.n, ring);
+        team_sweep<NT, T, RING>(D.up, pick<T>(D.up.val, D.up.val32), sc, sc, tmp, z, D.n, ring);
         break;
     default:   // kILU: unit lower factor
-        team_lower<NT, T>(D, pick<T>(D.l_val, D.l_val32), (const T*)nullptr, r, tmp);
-        team_upper<NT, T>(D, pick<T>(D.u_val, D.u_val32), sc, tmp, z);
+        team_sweep<NT, T, RING>(D.lo, pick<T>(D.lo.val, D.lo.val32), (const T*)nullptr, sc, r, tmp, D.n, ring);
+        team_sweep<NT, T, RING>(D.up, pick<T>(D.up.val, D.up.val32), sc, sc, tmp, z, D.n, ring);
         break;
     }
 }
 
-// y_p = sum_k a_k x_{dep_k}, in stored (ascending column) order
+// ---- SpMV: y_p = sum_k a_k x_{dep_k} in stored (ascending column) order ------------------
+// Thread t owns row t of each tile of NT rows.  Row data is fetched one tile ahead (entry
+// ranges two tiles ahead), together with the row's value of one auxiliary vector that the
+// epilogue needs.  When the system's column spread allows (|p - dep| <= w for every
+// entry), x is read from an LDS window that slides with the tiles: the slice the next
+// tile adds is fetched with contiguous loads while the current tile computes.  As in the
+// sweeps every fetch is unconditional; rows past n (up to the padded length) compute
+// zeros.  epi(p, y_p, aux_p) consumes each row.
 template <class T>
-__device__ __forceinline__ T row_product(const TeamDesc& D, const T* __restrict__ val, const T* x, int p) {
-    T s = T(0);
-    const int ke = D.a_ptr[p + 1];
-    for (int k = D.a_ptr[p]; k < ke; ++k) s += val[k] * x[D.a_dep[k]];
-    return s;
+struct SpRow {
+    int kb, cnt;
+    int dep[kEA];
+    T val[kEA];
+    T aux, xs;
+};
+
+struct SpA {   // the matrix arrays, held in registers during a product
+    const int32_t* ptr;
+    const int32_t* dep;
+    int n, nnz1, npad, w;
+};
+
+template <int NT>
+__device__ __forceinline__ void spmv_ptr(const SpA& M, int g, int& kb, int& ke) {
+    const int q = g * NT + (int)threadIdx.x;
+    const int qc = min(q, M.n - 1);
+    kb = M.ptr[qc];
+    ke = M.ptr[qc + 1];
+    if (q >= M.n) ke = kb;
 }
 
+template <int NT, class T>
+__device__ __forceinline__ void spmv_row(SpRow<T>& R, const SpA& M, const T* __restrict__ val,
+                                         const T* __restrict__ aux, const T* __restrict__ x, int g, int kb, int ke) {
+    const int q = g * NT + (int)threadIdx.x;
+    R.kb = kb;
+    R.cnt = ke - kb;
+    const int last = max(R.cnt - 1, 0);
+#pragma unroll
+    for (int e = 0; e < kEA; ++e) {
+        const int k = min(kb + min(e, last), M.nnz1);
+        R.dep[e] = M.dep[k];
+        R.val[e] = val[k];
+    }
+    R.aux = aux[min(q, M.npad - 1)];
+    R.xs = x[min(q + NT + M.w, M.npad - 1)];   // window slice this tile adds for the next one
+}
+
+template <int NT, class T, int RING, class Epi>
+__device__ __forceinline__ void spmv_compute(const SpRow<T>& R, const SpA& M, const T* __restrict__ val,
+                                             const T* __restrict__ x, const T* ring, bool win, int g, Epi& epi) {
+    const int q = g * NT + (int)threadIdx.x;
+    T s = T(0);
+    if (win) {
+#pragma unroll
+        for (int e = 0; e < kEA; ++e)
+            if (e < R.cnt) s += R.val[e] * ring[R.dep[e] & (RING - 1)];
+        for (int k = R.kb + kEA; k < R.kb + R.cnt; ++k) s += val[k] * ring[M.dep[k] & (RING - 1)];
+    } else {
+#pragma unroll
+        for (int e = 0; e < kEA; ++e)
+            if (e < R.cnt) s += R.val[e] * x[R.dep[e]];
+        for (int k = R.kb + kEA; k < R.kb + R.cnt; ++k) s += val[k] * x[M.dep[k]];
+    }
+    epi(q, s, R.aux);
+}
+
+template <int NT, class T, int RING, class Epi>
+__device__ void team_spmv(const TeamDesc& Dd, const T* __restrict__ val, const T* __restrict__ x,
+                          const T* __restrict__ aux, T* ring, Epi epi) {
+    ProfScope prof(0);
+    const SpA M{Dd.a_ptr, Dd.a_dep, Dd.n, Dd.a_nnz1, Dd.npad, Dd.wspan};   // registers from here on
+    const int n = M.n;
+    const bool win = ring != nullptr && (sizeof(T) == 8 ? Dd.win64 != 0 : Dd.win32 != 0);
+    const int G = (n + NT - 1) / NT;
+    if (win) {   // window for tile 0: positions [0, NT + w)
+        const int e0 = min(M.npad, NT + M.w);
+        for (int p = threadIdx.x; p < e0; p += NT) ring[p & (RING - 1)] = x[p];
+    }
+    SpRow<T> A, B;
+    int akb, ake, bkb, bke;
+    spmv_ptr<NT>(M, 0, akb, ake);
+    spmv_row<NT, T>(A, M, val, aux, x, 0, akb, ake);
+    spmv_ptr<NT>(M, min(1, G - 1), bkb, bke);
+    __syncthreads();
+    for (int g = 0; g < G; g += 2) {
+        const int g1 = min(g + 1, G - 1), g2 = min(g + 2, G - 1), g3 = min(g + 3, G - 1);
+        spmv_row<NT, T>(B, M, val, aux, x, g1, bkb, bke);
+        spmv_ptr<NT>(M, g2, akb, ake);
+        spmv_compute<NT, T, RING>(A, M, val, x, ring, win, g, epi);
+        if (g + 1 >= G) break;
+        if (win) {   // tile g+1 needs positions up to (g+2) NT + w: the slice A fetched
+            const int pa = (g + 1) * NT + M.w + (int)threadIdx.x;
+            if (pa < M.npad) ring[pa & (RING - 1)] = A.xs;
+            __syncthreads();
+        }
+        spmv_row<NT, T>(A, M, val, aux, x, g2, akb, ake);
+        spmv_ptr<NT>(M, g3, bkb, bke);
+        spmv_compute<NT, T, RING>(B, M, val, x, ring, win, g + 1, epi);
+        if (win && g + 2 < G) {
+            const int pb = (g + 2) * NT + M.w + (int)threadIdx.x;
+            if (pb < M.npad) ring[pb & (RING - 1)] = B.xs;
+            __syncthreads();
+        }
+    }
+}
+
+// ---- element-wise passes over the padded vectors, four elements per thread step ----------
+template <class T>
+struct alignas(4 * sizeof(T)) V4 {
+    T e[4];
+};
+template <class T>
+__device__ __forceinline__ V4<T> ld4(const T* p, int v) {
+    return *reinterpret_cast<const V4<T>*>(p + 4 * v);
+}
+template <class T>
+__device__ __forceinline__ void st4(T* p, int v, const V4<T>& a) {
+    *reinterpret_cast<V4<T>*>(p + 4 * v) = a;
+}
+
+// p = z; returns partial r.z
+template <int NT, class T>
+__device__ T pass_start(int nv, const T* __restrict__ r, const T* __restrict__ z, T* __restrict__ p) {
+    T q = T(0);
+    for (int v = threadIdx.x; v < nv; v += NT) {
+        const V4<T> R = ld4(r, v), Z = ld4(z, v);
+        st4(p, v, Z);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) q += R.e[e] * Z.e[e];
+    }
+    return q;
+}
+
+template <int NT, class T>
+__device__ T pass_dot(int nv, const T* __restrict__ a, const T* __restrict__ b) {
+    T q = T(0);
+    for (int v = threadIdx.x; v < nv; v += 2 * NT) {
+        const int v2 = v + NT;
+        const V4<T> A = ld4(a, v), B = ld4(b, v);
+        V4<T> A2{}, B2{};
+        if (v2 < nv) {
+            A2 = ld4(a, v2);
+            B2 = ld4(b, v2);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) q += A.e[e] * B.e[e];
+        if (v2 < nv) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) q += A2.e[e] * B2.e[e];
+        }
+    }
+    return q;
+}
+
+// FP64 CG (conjugate_gradient.rs:77-80): x += p*alpha; r -= ap*alpha; returns partial r.r
+template <int NT>
+__device__ double pass_cg_xr(int nv, const double* __restrict__ p, const double* __restrict__ ap,
+                             double* __restrict__ x, double* __restrict__ r, double alpha) {
+    double q = 0.0;
+    for (int v = threadIdx.x; v < nv; v += NT) {
+        const V4<double> P = ld4(p, v), AP = ld4(ap, v);
+        V4<double> X = ld4(x, v), R = ld4(r, v);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            X.e[e] += P.e[e] * alpha;
+            R.e[e] = R.e[e] - AP.e[e] * alpha;
+            q += R.e[e] * R.e[e];
+        }
+        st4(x, v, X);
+        st4(r, v, R);
+    }
+    return q;
+}
+
+// FP64 CG (:98-100): p = z + p*beta
+template <int NT>
+__device__ void pass_cg_p(int nv, const double* __restrict__ z, double* __restrict__ p, double beta) {
+    for (int v = threadIdx.x; v < nv; v += NT) {
+        const V4<double> Z = ld4(z, v);
+        V4<double> P = ld4(p, v);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) P.e[e] = Z.e[e] + P.e[e] * beta;
+        st4(p, v, P);
+    }
+}
+
+// FP64 BiCGStab (:213-216): x = x + y*alpha; s = r - v*alpha; returns partial s.s
+template <int NT>
+__device__ double pass_bi_hs(int nv, const double* __restrict__ y, const double* __restrict__ vv,
+                             const double* __restrict__ r, double* __restrict__ x, double* __restrict__ s,
+                             double alpha) {
+    double q = 0.0;
+    for (int v = threadIdx.x; v < nv; v += NT) {
+        const V4<double> Y = ld4(y, v), VV = ld4(vv, v), R = ld4(r, v);
+        V4<double> X = ld4(x, v), S;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            X.e[e] = X.e[e] + Y.e[e] * alpha;
+            S.e[e] = R.e[e] - VV.e[e] * alpha;
+            q += S.e[e] * S.e[e];
+        }
+        st4(x, v, X);
+        st4(s, v, S);
+    }
+    return q;
+}
+
+// FP64 BiCGStab (:246-249): x = x + z*omega; r = s - t*omega; partials rh.r and r.r
+template <int NT>
+__device__ void pass_bi_xr(int nv, const double* __restrict__ z, const double* __restrict__ t,
+                           const double* __restrict__ s, const double* __restrict__ rh, double* __restrict__ x,
+                           double* __restrict__ r, double omega, double& q0, double& q1) {
+    for (int v = threadIdx.x; v < nv; v += NT) {
+        const V4<double> Z = ld4(z, v), TT = ld4(t, v), S = ld4(s, v), RH = ld4(rh, v);
+        V4<double> X = ld4(x, v), R;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            X.e[e] = X.e[e] + Z.e[e] * omega;
+            R.e[e] = S.e[e] - TT.e[e] * omega;
+            q0 += RH.e[e] * R.e[e];
+            q1 += R.e[e] * R.e[e];
+        }
+        st4(x, v, X);
+        st4(r, v, R);
+    }
+}
+
+// FP64 BiCGStab (:268-282): p = r + (p - v*omega)*beta, or the restart rh = r, p = r
+template <int NT>
+__device__ void pass_bi_p(int nv, const double* __restrict__ r, const double* __restrict__ vv, double* __restrict__ p,
+                          double* __restrict__ rh, double omega, double beta, int mode) {
+    for (int v = threadIdx.x; v < nv; v += NT) {
+        const V4<double> R = ld4(r, v);
+        if (mode == 0) {
+            const V4<double> VV = ld4(vv, v);
+            V4<double> P = ld4(p, v);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) P.e[e] = R.e[e] + (P.e[e] - VV.e[e] * omega) * beta;
+            st4(p, v, P);
+        } else {
+            st4(rh, v, R);
+            st4(p, v, R);
+        }
+    }
+}
+
+// ---- per-system bookkeeping ----------------------------------------------------------------
 __device__ __forceinline__ void team_push(const TeamDesc& D, int64_t& len, double e) {
     if (threadIdx.x == 0 && len < D.hcap) D.hist[len] = e;
     len += 1;
 }
 
-// gather b and x into positions; returns |b|^2
+// gather b and x into positions; returns the partial |b|^2
 template <int NT>
-__device__ double team_gather(const TeamDesc& D, double* b, double* x) {
+__device__ double team_gather(const TeamDesc& D, double* __restrict__ b, double* __restrict__ x) {
     double q = 0.0;
-    for (int p = threadIdx.x; p < D.n; p += NT) {
-        const int row = D.perm[p];
-        const double bi = D.b[row];
-        b[p] = bi;
-        x[p] = D.x[row];
-        q += bi * bi;
+    const int32_t* __restrict__ perm = D.perm;
+    for (int p0 = 0; p0 < D.n; p0 += 2 * NT) {
+        int row[2];
+        double bb[2], xx[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int p = p0 + u * NT + (int)threadIdx.x;
+            row[u] = p < D.n ? perm[p] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            bb[u] = D.b[row[u]];
+            xx[u] = D.x[row[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int p = p0 + u * NT + (int)threadIdx.x;
+            if (p < D.n) {
+                b[p] = bb[u];
+                x[p] = xx[u];
+                q += bb[u] * bb[u];
+            }
+        }
     }
     return q;
 }
 
 template <int NT>
-__device__ void team_scatter(const TeamDesc& D, const double* x) {
+__device__ void team_scatter(const TeamDesc& D, const double* __restrict__ x) {
     for (int p = threadIdx.x; p < D.n; p += NT) D.x[D.perm[p]] = x[p];
 }
 
@@ -221,65 +602,51 @@ __device__ bool team_zero_rhs(const TeamDesc& D, double bnorm, bool mixed) {
 
 // ---- FP64 conjugate_gradient (conjugate_gradient.rs:12-112) ------------------------------
 template <int NT>
-__device__ void team_cg64(const TeamDesc& D, Reducer<NT, double>& R) {
-    const int n = D.n, tid = threadIdx.x;
+__device__ void team_cg64(const TeamDesc& D, Reducer<NT, double>& R, double* ring) {
+    const int n = D.n, np = D.npad, nv = D.npad / 4;
     double* b = D.w64;
-    double* x = b + n;
-    double* r = x + n;
-    double* z = r + n;
-    double* p = z + n;
-    double* ap = p + n;   // also the intermediate vector of the two sweeps
+    double* x = b + np;
+    double* r = x + np;
+    double* z = r + np;
+    double* p = z + np;
+    double* ap = p + np;   // also the intermediate vector of the two sweeps
     const double bnorm = sqrt(R.sum(team_gather<NT>(D, b, x)));
     if (team_zero_rhs<NT>(D, bnorm, false)) return;
+    __syncthreads();
     // r = (A x - b) * -1 and |r| (:47-52)
     double q = 0.0;
-    for (int i = tid; i < n; i += NT) {
-        const double ri = (row_product<double>(D, D.a_val, x, i) - b[i]) * -1.0;
+    team_spmv<NT, double, kRing64>(D, D.a_val, x, b, ring, [&](int i, double s, double bi) {
+        const double ri = (s - bi) * -1.0;
         r[i] = ri;
         q += ri * ri;
-    }
+    });
     const double init_res = sqrt(R.sum(q)) / bnorm;
     double error = 1.0;
     int64_t iter = 0, hlen = 0;
     if ((error > D.atol) && !(iter > D.max_iter)) {
-        team_prec<NT, double>(D, r, z, ap);
-        q = 0.0;
-        for (int i = tid; i < n; i += NT) {
-            p[i] = z[i];
-            q += r[i] * z[i];
-        }
-        double rz = R.sum(q);
+        team_prec<NT, double, kRing64>(D, r, z, ap, ring);
+        double rz = R.sum(pass_start<NT, double>(nv, r, z, p));
         for (;;) {
             q = 0.0;   // ap = A p, p.ap (:72-74)
-            for (int i = tid; i < n; i += NT) {
-                const double s = row_product<double>(D, D.a_val, p, i);
+            team_spmv<NT, double, kRing64>(D, D.a_val, p, p, ring, [&](int i, double s, double pi) {
                 ap[i] = s;
-                q += p[i] * s;
-            }
+                q += pi * s;
+            });
             const double alpha = rz / R.sum(q);
-            q = 0.0;   // (:77-80)
-            for (int i = tid; i < n; i += NT) {
-                x[i] += p[i] * alpha;
-                const double ri = r[i] - ap[i] * alpha;
-                r[i] = ri;
-                q += ri * ri;
-            }
-            error = sqrt(R.sum(q)) / bnorm;   // (:83-89)
+            error = sqrt(R.sum(pass_cg_xr<NT>(nv, p, ap, x, r, alpha))) / bnorm;   // (:77-89)
             if ((error < D.atol) || (error / init_res < D.rtol)) {
                 team_push(D, hlen, error);
                 iter += 1;
                 break;
             }
-            team_prec<NT, double>(D, r, z, ap);
-            q = 0.0;
-            for (int i = tid; i < n; i += NT) q += r[i] * z[i];
-            const double rz_new = R.sum(q);
+            team_prec<NT, double, kRing64>(D, r, z, ap, ring);
+            const double rz_new = R.sum(pass_dot<NT, double>(nv, r, z));
             const double beta = rz_new / rz;   // (:94-105)
             rz = rz_new;
             team_push(D, hlen, error);
             iter += 1;
             if (!(error > D.atol) || iter > D.max_iter) break;
-            for (int i = tid; i < n; i += NT) p[i] = z[i] + p[i] * beta;
+            pass_cg_p<NT>(nv, z, p, beta);
             __syncthreads();
         }
     }
@@ -290,28 +657,29 @@ __device__ void team_cg64(const TeamDesc& D, Reducer<NT, double>& R) {
 
 // ---- FP64 bi_conjugate_gradient_stab (bi_conjugate_gradient_stab.rs:13-179) --------------
 template <int NT>
-__device__ void team_bi64(const TeamDesc& D, Reducer<NT, double>& R) {
-    const int n = D.n, tid = threadIdx.x;
+__device__ void team_bi64(const TeamDesc& D, Reducer<NT, double>& R, double* ring) {
+    const int n = D.n, np = D.npad, nv = D.npad / 4;
     double* b = D.w64;
-    double* x = b + n;
-    double* r = x + n;
-    double* rh = r + n;
-    double* p = rh + n;
-    double* y = p + n;
-    double* v = y + n;
-    double* s = v + n;
-    double* z = s + n;
-    double* t = z + n;   // also the intermediate vector of the sweeps
+    double* x = b + np;
+    double* r = x + np;
+    double* rh = r + np;
+    double* p = rh + np;
+    double* y = p + np;
+    double* v = y + np;
+    double* s = v + np;
+    double* z = s + np;
+    double* t = z + np;   // also the intermediate vector of the sweeps
     const double bnorm = sqrt(R.sum(team_gather<NT>(D, b, x)));
     if (team_zero_rhs<NT>(D, bnorm, false)) return;
+    __syncthreads();
     double q = 0.0;   // r = b - A x, rh = p = r (:165-190)
-    for (int i = tid; i < n; i += NT) {
-        const double ri = (row_product<double>(D, D.a_val, x, i) - b[i]) * -1.0;
+    team_spmv<NT, double, kRing64>(D, D.a_val, x, b, ring, [&](int i, double sv, double bi) {
+        const double ri = (sv - bi) * -1.0;
         r[i] = ri;
         rh[i] = ri;
         p[i] = ri;
         q += ri * ri;
-    }
+    });
     const double rr0 = R.sum(q);
     const double init_res = sqrt(rr0) / bnorm;
     double rho = rr0, error = 1.0, alpha = 0.0, omega = 0.0;
@@ -324,48 +692,34 @@ __device__ void team_bi64(const TeamDesc& D, Reducer<NT, double>& R) {
     } else if (!(error > D.atol) || iter > D.max_iter) {
         run = false;
     }
+    (void)n;
     while (run) {
-        team_prec<NT, double>(D, p, y, t);   // y = M^-1 p
-        q = 0.0;                              // v = A y, rh.v (:208-209)
-        for (int i = tid; i < n; i += NT) {
-            const double vi = row_product<double>(D, D.a_val, y, i);
-            v[i] = vi;
-            q += rh[i] * vi;
-        }
+        team_prec<NT, double, kRing64>(D, p, y, t, ring);   // y = M^-1 p
+        q = 0.0;                                                 // v = A y, rh.v (:208-209)
+        team_spmv<NT, double, kRing64>(D, D.a_val, y, rh, ring, [&](int i, double sv, double rhi) {
+            v[i] = sv;
+            q += rhi * sv;
+        });
         alpha = rho / R.sum(q);
-        q = 0.0;   // h = x + y alpha; s = r - v alpha (:213-216)
-        for (int i = tid; i < n; i += NT) {
-            x[i] = x[i] + y[i] * alpha;
-            const double si = r[i] - v[i] * alpha;
-            s[i] = si;
-            q += si * si;
-        }
-        const double ss = sqrt(R.sum(q)) / bnorm;   // (:219-231)
+        const double ss = sqrt(R.sum(pass_bi_hs<NT>(nv, y, v, r, x, s, alpha))) / bnorm;   // (:213-231)
         if ((ss < D.atol) || (ss / init_res < D.rtol)) {
             error = ss;
             team_push(D, hlen, ss);
             iter += 1;
             break;
         }
-        team_prec<NT, double>(D, s, z, t);   // z = M^-1 s
-        double q0 = 0.0, q1 = 0.0;            // t = A z; t.s, t.t (:239-242)
-        for (int i = tid; i < n; i += NT) {
-            const double ti = row_product<double>(D, D.a_val, z, i);
-            t[i] = ti;
-            q0 += s[i] * ti;
-            q1 += ti * ti;
-        }
+        team_prec<NT, double, kRing64>(D, s, z, t, ring);   // z = M^-1 s
+        double q0 = 0.0, q1 = 0.0;                               // t = A z; t.s, t.t (:239-242)
+        team_spmv<NT, double, kRing64>(D, D.a_val, z, s, ring, [&](int i, double tv, double si) {
+            t[i] = tv;
+            q0 += si * tv;
+            q1 += tv * tv;
+        });
         R.sum2(q0, q1);
         omega = q0 / q1;
         q0 = 0.0;
-        q1 = 0.0;   // x = h + z omega; r = s - t omega; rh.r, r.r (:246-249)
-        for (int i = tid; i < n; i += NT) {
-            x[i] = x[i] + z[i] * omega;
-            const double ri = s[i] - t[i] * omega;
-            r[i] = ri;
-            q0 += rh[i] * ri;
-            q1 += ri * ri;
-        }
+        q1 = 0.0;
+        pass_bi_xr<NT>(nv, z, t, s, rh, x, r, omega, q0, q1);   // (:246-249)
         R.sum2(q0, q1);
         const double rr = sqrt(q1) / bnorm;   // (:252-287)
         if ((rr < D.atol) || (rr / init_res < D.rtol)) {
@@ -386,14 +740,7 @@ __device__ void team_bi64(const TeamDesc& D, Reducer<NT, double>& R) {
         team_push(D, hlen, error);
         iter += 1;
         if (!(error > D.atol) || iter > D.max_iter) break;
-        for (int i = tid; i < n; i += NT) {
-            if (mode == 0) {
-                p[i] = r[i] + (p[i] - v[i] * omega) * beta;
-            } else {
-                rh[i] = r[i];
-                p[i] = r[i];
-            }
-        }
+        pass_bi_p<NT>(nv, r, v, p, rh, omega, beta, mode);
         __syncthreads();
     }
     __syncthreads();
@@ -402,170 +749,205 @@ __device__ void team_bi64(const TeamDesc& D, Reducer<NT, double>& R) {
 }
 
 // ---- mixed precision: FP32 inner solves inside FP64 refinement ---------------------------
-// r64 = b - A x in FP64, |r64|, r32 = (float) r64, d = 0; returns |r64|^2 (every thread)
-template <int NT>
-__device__ double team_outer_residual(const TeamDesc& D, Reducer<NT, double>& R, const double* b, const double* x,
-                                      float* r, float* d) {
-    double q = 0.0;
-    for (int i = threadIdx.x; i < D.n; i += NT) {
-        const double ri = (row_product<double>(D, D.a_val, x, i) - b[i]) * -1.0;
-        r[i] = (float)ri;
-        d[i] = 0.f;
-        q += ri * ri;
-    }
-    return R.sum(q);
+// Reduction asked of the inner solve of a refinement step: the remaining factor to the
+// caller's tolerance, with a margin, not below the floor `lo` and not above 0.1.
+__device__ __forceinline__ double inner_target(const TeamDesc& D, double e, double init_res) {
+    const double goal = fmax(D.rtol * init_res, D.atol);
+    double t = 0.3 * goal / e;
+    t = fmin(t, 0.1);
+    return fmax(t, D.inner_tol);
 }
 
 template <int NT>
-__device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT, float>& F, bool bicg) {
-    const int n = D.n, tid = threadIdx.x;
+__device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT, float>& F, bool bicg,
+                           float* ring) {
+    const int n = D.n, np = D.npad, nv = D.npad / 4;
     double* b = D.w64;
-    double* x = b + n;
+    double* x = b + np;
     float* r = D.w32;
-    float* d = r + n;
-    float* z = d + n;
-    float* p = z + n;
-    float* ap = p + n;   // CG: A p and the sweeps' intermediate
-    float* rh = ap + n;  // BiCGStab only
-    float* y = rh + n;
-    float* v = y + n;
-    float* s = v + n;
-    float* t = s + n;    // BiCGStab: A z and the sweeps' intermediate
+    float* d = r + np;
+    float* z = d + np;
+    float* p = z + np;
+    float* ap = p + np;   // CG: A p and the sweeps' intermediate
+    float* rh = ap + np;  // BiCGStab only
+    float* y = rh + np;
+    float* v = y + np;
+    float* s = v + np;
+    float* t = s + np;    // BiCGStab: A z and the sweeps' intermediate
     const double bnorm = sqrt(R.sum(team_gather<NT>(D, b, x)));
     if (team_zero_rhs<NT>(D, bnorm, true)) return;
-    const double inner_tol = D.inner_tol;
+    __syncthreads();
     const int64_t max_inner = D.max_iter > 0 ? D.max_iter : 1;
     double init_res = 0.0, true_res = 0.0;
     int64_t total = 0, outer = 0, restarts = 0, hlen = 0;
+    (void)n;
     for (int k = 0; k < 64; ++k) {
-        const double e = sqrt(team_outer_residual<NT>(D, R, b, x, r, d)) / bnorm;
+        // r64 = b - A x in FP64 (values, products and sum), r32 = (float) r64, d = 0
+        double q = 0.0;
+        team_spmv<NT, double, kRing64>(D, D.a_val, x, b, (double*)nullptr, [&](int i, double sv, double bi) {
+            const double ri = (sv - bi) * -1.0;
+            r[i] = (float)ri;
+            d[i] = 0.f;
+            q += ri * ri;
+        });
+        const double e = sqrt(R.sum(q)) / bnorm;
         if (outer == 0) init_res = e;
         true_res = e;
         outer += 1;
         team_push(D, hlen, e);
         if ((e < D.atol) || (e / init_res < D.rtol) || total >= max_inner) break;
+        const double tol = inner_target(D, e, init_res);
         if (!bicg) {
-            // z = M r; p = z; rin0 = |r|; rz = r.z
-            team_prec<NT, float>(D, r, z, ap);
+            team_prec<NT, float, kRing32>(D, r, z, ap, ring);   // z = M r; p = z; |r|, r.z
             float q0 = 0.f, q1 = 0.f;
-            for (int i = tid; i < n; i += NT) {
-                p[i] = z[i];
-                q0 += r[i] * r[i];
-                q1 += r[i] * z[i];
+            for (int vi = threadIdx.x; vi < nv; vi += NT) {
+                const V4<float> RR = ld4(r, vi), Z = ld4(z, vi);
+                st4(p, vi, Z);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    q0 += RR.e[j] * RR.e[j];
+                    q1 += RR.e[j] * Z.e[j];
+                }
             }
             F.sum2(q0, q1);
             const double rin0 = sqrt((double)q0);
             double rz = (double)q1;
             if (!(rin0 > 0.0)) break;   // the FP32 residual underflowed: refinement cannot progress
-            {
-                for (;;) {
-                    float qa = 0.f;
-                    for (int i = tid; i < n; i += NT) {
-                        const float si = row_product<float>(D, D.a_val32, p, i);
-                        ap[i] = si;
-                        qa += p[i] * si;
+            for (;;) {
+                float qa = 0.f;
+                team_spmv<NT, float, kRing32>(D, D.a_val32, p, p, ring, [&](int i, float sv, float pi) {
+                    ap[i] = sv;
+                    qa += pi * sv;
+                });
+                const float alpha = (float)(rz / (double)F.sum(qa));
+                float qr = 0.f;
+                for (int vi = threadIdx.x; vi < nv; vi += NT) {
+                    const V4<float> P = ld4(p, vi), AP = ld4(ap, vi);
+                    V4<float> DD = ld4(d, vi), RR = ld4(r, vi);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        DD.e[j] += alpha * P.e[j];
+                        RR.e[j] = RR.e[j] - alpha * AP.e[j];
+                        qr += RR.e[j] * RR.e[j];
                     }
-                    const float alpha = (float)(rz / (double)F.sum(qa));
-                    float qr = 0.f;
-                    for (int i = tid; i < n; i += NT) {
-                        d[i] += alpha * p[i];
-                        const float ri = r[i] - alpha * ap[i];
-                        r[i] = ri;
-                        qr += ri * ri;
-                    }
-                    const double rin = sqrt((double)F.sum(qr));
-                    total += 1;
-                    if (rin <= inner_tol * rin0 || total >= max_inner) break;
-                    team_prec<NT, float>(D, r, z, ap);
-                    float qz = 0.f;
-                    for (int i = tid; i < n; i += NT) qz += r[i] * z[i];
-                    const double rz_new = (double)F.sum(qz);
-                    const float beta = (float)(rz_new / rz);
-                    rz = rz_new;
-                    for (int i = tid; i < n; i += NT) p[i] = z[i] + beta * p[i];
-                    __syncthreads();
+                    st4(d, vi, DD);
+                    st4(r, vi, RR);
                 }
+                const double rin = sqrt((double)F.sum(qr));
+                total += 1;
+                if (rin <= tol * rin0 || total >= max_inner) break;
+                team_prec<NT, float, kRing32>(D, r, z, ap, ring);
+                const double rz_new = (double)F.sum(pass_dot<NT, float>(nv, r, z));
+                const float beta = (float)(rz_new / rz);
+                rz = rz_new;
+                for (int vi = threadIdx.x; vi < nv; vi += NT) {
+                    const V4<float> Z = ld4(z, vi);
+                    V4<float> P = ld4(p, vi);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) P.e[j] = Z.e[j] + beta * P.e[j];
+                    st4(p, vi, P);
+                }
+                __syncthreads();
             }
         } else {
             float q0 = 0.f;
-            for (int i = tid; i < n; i += NT) {
-                rh[i] = r[i];
-                p[i] = r[i];
-                q0 += r[i] * r[i];
+            for (int vi = threadIdx.x; vi < nv; vi += NT) {
+                const V4<float> RR = ld4(r, vi);
+                st4(rh, vi, RR);
+                st4(p, vi, RR);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) q0 += RR.e[j] * RR.e[j];
             }
             q0 = F.sum(q0);
             const double rin0 = sqrt((double)q0);
             double rho = (double)q0, alpha = 0.0, omega = 0.0;
             if (!(rin0 > 0.0)) break;
-            {
-                for (;;) {
-                    team_prec<NT, float>(D, p, y, t);
-                    float qv = 0.f;
-                    for (int i = tid; i < n; i += NT) {
-                        const float vi = row_product<float>(D, D.a_val32, y, i);
-                        v[i] = vi;
-                        qv += rh[i] * vi;
+            for (;;) {
+                team_prec<NT, float, kRing32>(D, p, y, t, ring);
+                float qv = 0.f;
+                team_spmv<NT, float, kRing32>(D, D.a_val32, y, rh, ring, [&](int i, float sv, float rhi) {
+                    v[i] = sv;
+                    qv += rhi * sv;
+                });
+                alpha = rho / (double)F.sum(qv);
+                const float af = (float)alpha;
+                float qs = 0.f;
+                for (int vi = threadIdx.x; vi < nv; vi += NT) {
+                    const V4<float> Y = ld4(y, vi), VV = ld4(v, vi), RR = ld4(r, vi);
+                    V4<float> DD = ld4(d, vi), S;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        DD.e[j] += af * Y.e[j];
+                        S.e[j] = RR.e[j] - af * VV.e[j];
+                        qs += S.e[j] * S.e[j];
                     }
-                    alpha = rho / (double)F.sum(qv);
-                    const float af = (float)alpha;
-                    float qs = 0.f;
-                    for (int i = tid; i < n; i += NT) {
-                        d[i] += af * y[i];
-                        const float si = r[i] - af * v[i];
-                        s[i] = si;
-                        qs += si * si;
-                    }
-                    const double sn = sqrt((double)F.sum(qs));
-                    if (sn <= inner_tol * rin0) {
-                        total += 1;
-                        break;
-                    }
-                    team_prec<NT, float>(D, s, z, t);
-                    float qts = 0.f, qtt = 0.f;
-                    for (int i = tid; i < n; i += NT) {
-                        const float ti = row_product<float>(D, D.a_val32, z, i);
-                        t[i] = ti;
-                        qts += ti * s[i];
-                        qtt += ti * ti;
-                    }
-                    F.sum2(qts, qtt);
-                    omega = (double)qts / (double)qtt;
-                    const float of = (float)omega;
-                    float qhr = 0.f, qrr = 0.f;
-                    for (int i = tid; i < n; i += NT) {
-                        d[i] += of * z[i];
-                        const float ri = s[i] - of * t[i];
-                        r[i] = ri;
-                        qhr += rh[i] * ri;
-                        qrr += ri * ri;
-                    }
-                    F.sum2(qhr, qrr);
-                    const double rin = sqrt((double)qrr);
-                    total += 1;
-                    if (rin <= inner_tol * rin0 || total >= max_inner) break;
-                    const double rho_new = (double)qhr;
-                    const double beta = (rho_new / rho) * (alpha / omega);
-                    rho = rho_new;
-                    int mode = 0;
-                    if (!(fabs(rho) > 1e-30 * rin0 * rin0)) {
-                        mode = 1;
-                        restarts += 1;
-                        rho = (double)qrr;
-                    }
-                    const float bf = (float)beta;
-                    for (int i = tid; i < n; i += NT) {
-                        if (mode == 0) {
-                            p[i] = r[i] + bf * (p[i] - of * v[i]);
-                        } else {
-                            rh[i] = r[i];
-                            p[i] = r[i];
-                        }
-                    }
-                    __syncthreads();
+                    st4(d, vi, DD);
+                    st4(s, vi, S);
                 }
+                const double sn = sqrt((double)F.sum(qs));
+                if (sn <= tol * rin0) {
+                    total += 1;
+                    break;
+                }
+                team_prec<NT, float, kRing32>(D, s, z, t, ring);
+                float qts = 0.f, qtt = 0.f;
+                team_spmv<NT, float, kRing32>(D, D.a_val32, z, s, ring, [&](int i, float tv, float si) {
+                    t[i] = tv;
+                    qts += tv * si;
+                    qtt += tv * tv;
+                });
+                F.sum2(qts, qtt);
+                omega = (double)qts / (double)qtt;
+                const float of = (float)omega;
+                float qhr = 0.f, qrr = 0.f;
+                for (int vi = threadIdx.x; vi < nv; vi += NT) {
+                    const V4<float> Z = ld4(z, vi), TT = ld4(t, vi), S = ld4(s, vi), RH = ld4(rh, vi);
+                    V4<float> DD = ld4(d, vi), RR;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        DD.e[j] += of * Z.e[j];
+                        RR.e[j] = S.e[j] - of * TT.e[j];
+                        qhr += RH.e[j] * RR.e[j];
+                        qrr += RR.e[j] * RR.e[j];
+                    }
+                    st4(d, vi, DD);
+                    st4(r, vi, RR);
+                }
+                F.sum2(qhr, qrr);
+                const double rin = sqrt((double)qrr);
+                total += 1;
+                if (rin <= tol * rin0 || total >= max_inner) break;
+                const double rho_new = (double)qhr;
+                const double beta = (rho_new / rho) * (alpha / omega);
+                rho = rho_new;
+                int mode = 0;
+                if (!(fabs(rho) > 1e-30 * rin0 * rin0)) {
+                    mode = 1;
+                    restarts += 1;
+                    rho = (double)qrr;
+                }
+                const float bf = (float)beta;
+                for (int vi = threadIdx.x; vi < nv; vi += NT) {
+                    const V4<float> RR = ld4(r, vi);
+                    if (mode == 0) {
+                        const V4<float> VV = ld4(v, vi);
+                        V4<float> P = ld4(p, vi);
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) P.e[j] = RR.e[j] + bf * (P.e[j] - of * VV.e[j]);
+                        st4(p, vi, P);
+                    } else {
+                        st4(rh, vi, RR);
+                        st4(p, vi, RR);
+                    }
+                }
+                __syncthreads();
             }
         }
-        for (int i = tid; i < n; i += NT) x[i] += (double)d[i];   // FP64 solution update
+        for (int vi = threadIdx.x; vi < np / 2; vi += NT) {   // FP64 solution update x += d
+            const int i = 2 * vi;
+            x[i] += (double)d[i];
+            x[i + 1] += (double)d[i + 1];
+        }
         __syncthreads();
     }
     __syncthreads();
@@ -575,16 +957,37 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
 
 template <int NT, int VAR>
 __global__ void __launch_bounds__(NT) k_team(const TeamDesc* __restrict__ descs, int n_inst) {
+    using T = typename std::conditional<VAR == kCG32 || VAR == kBI32, float, double>::type;
+    constexpr int RING = sizeof(T) == 8 ? kRing64 : kRing32;
+    __shared__ T ring[RING];
     __shared__ double red64[4 * (NT / 64)];
     __shared__ float red32[4 * (NT / 64)];
+    __shared__ TeamDesc sdesc;
+    static_assert(sizeof(TeamDesc) % 8 == 0 && sizeof(TeamDesc) / 8 <= NT, "descriptor copy");
     Reducer<NT, double> R{red64, 0};
     Reducer<NT, float> F{red32, 0};
     for (int k = blockIdx.x; k < n_inst; k += gridDim.x) {
-        const TeamDesc D = descs[k];
-        if (VAR == kCG64) team_cg64<NT>(D, R);
-        if (VAR == kBI64) team_bi64<NT>(D, R);
-        if (VAR == kCG32) team_mixed<NT>(D, R, F, false);
-        if (VAR == kBI32) team_mixed<NT>(D, R, F, true);
+        // the descriptor lives in LDS for the whole solve: reading it never waits on the
+        // vector-memory counter that the prefetched loads are counted by
+        __syncthreads();
+        if (threadIdx.x < sizeof(TeamDesc) / 8)
+            reinterpret_cast<uint64_t*>(&sdesc)[threadIdx.x] = reinterpret_cast<const uint64_t*>(descs + k)[threadIdx.x];
+        __syncthreads();
+        const TeamDesc& D = sdesc;
+        long long t0 = 0;
+        if (threadIdx.x == 0) {
+            g_prof_on = D.prof != nullptr;
+            for (int j = 0; j < 4; ++j) g_prof[j] = 0;
+            t0 = wall_clock64();
+        }
+        if constexpr (VAR == kCG64) team_cg64<NT>(D, R, ring);
+        if constexpr (VAR == kBI64) team_bi64<NT>(D, R, ring);
+        if constexpr (VAR == kCG32) team_mixed<NT>(D, R, F, false, ring);
+        if constexpr (VAR == kBI32) team_mixed<NT>(D, R, F, true, ring);
+        if (threadIdx.x == 0 && g_prof_on) {
+            g_prof[3] = wall_clock64() - t0;
+            for (int j = 0; j < 4; ++j) D.prof[j] = g_prof[j];
+        }
         __syncthreads();
     }
 }
@@ -621,12 +1024,20 @@ static const void* team_kernel_for(int nt, int var) {
 
 }  // namespace fvg
 
+// one sweep's tables, as offsets into the batch's int arena
+struct TeamSweepLayout {
+    int64_t lvl = 0, ptr = 0, dep = 0, src = 0;
+    int64_t nnz = 0, val = 0;
+    int32_t ring64 = 0, ring32 = 0;
+};
+
 // arrays of one system, as offsets into the batch arenas
 struct TeamLayout {
-    int64_t n = 0, nlev = 0, nnz_a = 0, nnz_l = 0, nnz_u = 0;
-    int64_t perm = 0, lvl = 0, a_ptr = 0, a_dep = 0, l_ptr = 0, l_dep = 0, u_ptr = 0, u_dep = 0;   // int arena
-    int64_t a_src = 0, l_src = 0, u_src = 0;                                                    // int arena
-    int64_t a_val = 0, l_val = 0, u_val = 0, sc = 0;                                            // value arenas
+    int64_t n = 0, npad = 0, nnz_a = 0;
+    int32_t wspan = 0, win64 = 0, win32 = 0;
+    int64_t perm = 0, a_ptr = 0, a_dep = 0, a_src = 0;   // int arena
+    TeamSweepLayout lo, up;
+    int64_t a_val = 0, sc = 0;                           // value arenas
     int64_t w64 = 0, w32 = 0, hist = 0;
 };
 
@@ -635,7 +1046,7 @@ struct fvg_batch {
     int solver = 0;      // 0 CG, 1 BiCGStab
     int precision = 0;
     int variant = 0;
-    int nt = 1024;
+    int nt = 512;
     fvg_solver_options opts{};
     std::vector<const fvg_csr*> A;
     std::vector<const fvg_prec*> P;
@@ -654,6 +1065,7 @@ struct fvg_batch {
     double* hist = nullptr;
     int64_t nhist = 0, hcap = 0;
     fvg_solver_info* d_info = nullptr;
+    int64_t* d_prof = nullptr;        // 4 counters per system when phase timing is on
     fvg::TeamDesc* d_desc = nullptr;
     std::vector<fvg::TeamDesc> desc;  // by launch slot
     hipStream_t stream = nullptr;
@@ -677,20 +1089,42 @@ static void batch_free(fvg_ctx* c, fvg_batch* B) {
     dev_free_n(c, B->w32, B->nw32);
     dev_free_n(c, B->hist, B->nhist);
     dev_free_n(c, B->d_info, B->n_inst);
+    dev_free_n(c, B->d_prof, 4 * B->n_inst);
     dev_free_n(c, B->d_desc, B->n_inst);
     if (B->ready) (void)hipEventDestroy(B->ready);
     if (B->stream) (void)hipStreamDestroy(B->stream);
 }
 
+static int64_t put_ints(std::vector<int32_t>& ints, const std::vector<int32_t>& v) {
+    const int64_t off = (int64_t)ints.size();
+    ints.insert(ints.end(), v.begin(), v.end());
+    return off;
+}
+
+// Ring check of one sweep: rows are solved in groups of up to kMaxGroup processing
+// indices, and a row reads a dependency from the ring slot of its position, so no row
+// written up to the end of the reader's group may share that slot: every dependency
+// must lie less than the ring size before the end of its reader's group.
+static void ring_check(int64_t n, const std::vector<int32_t>& ptr, const std::vector<int32_t>& qdep, int32_t& ring64,
+                       int32_t& ring32) {
+    int64_t span = 0;
+    for (int64_t q = 0; q < n; ++q) {
+        const int64_t qe = std::min<int64_t>((q / kMaxGroup + 1) * kMaxGroup, n);
+        for (int32_t k = ptr[(size_t)q]; k < ptr[(size_t)q + 1]; ++k)
+            span = std::max<int64_t>(span, qe - 1 - qdep[(size_t)k]);
+    }
+    ring64 = span < kRing64 ? 1 : 0;
+    ring32 = span < kRing32 ? 1 : 0;
+}
+
 // Host analysis of one system: level order, permuted A, lower and upper factor
-// entries with their source slots.  Appends index data to `ints`.
+// entries with their source slots, sweep plans.  Appends index data to `ints`.
 static int team_analyse(const fvg_csr* a, const fvg_prec* p, TeamLayout& L, std::vector<int32_t>& ints) {
     const int64_t n = a->nrows;
     const auto& rp = a->h_row_ptr;
     const auto& col = a->h_col;
     const bool fac = p->kind == kIC || p->kind == kILU;
-    // structural symmetry (the upper sweep runs the lower levels in reverse)
-    if (fac) {
+    if (fac) {   // structural symmetry (the upper sweep runs the lower levels in reverse)
         for (int64_t i = 0; i < n; ++i)
             for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
                 const int32_t j = col[(size_t)k];
@@ -725,8 +1159,9 @@ static int team_analyse(const fvg_csr* a, const fvg_prec* p, TeamLayout& L, std:
             pos[(size_t)i] = q;
         }
     }
-    std::vector<int32_t> a_ptr((size_t)n + 1, 0), a_dep, a_src, l_ptr((size_t)n + 1, 0), l_dep, l_src,
-        u_ptr((size_t)n + 1, 0), u_dep, u_src;
+    std::vector<int32_t> a_ptr((size_t)n + 1, 0), a_dep, a_src;
+    std::vector<int32_t> l_ptr((size_t)n + 1, 0), l_dep, l_src, l_qdep;
+    std::vector<int32_t> u_ptr((size_t)n + 1, 0), u_dep, u_src, u_qdep;
     a_dep.reserve(col.size());
     a_src.reserve(col.size());
     for (int64_t q = 0; q < n; ++q) {
@@ -737,6 +1172,7 @@ static int team_analyse(const fvg_csr* a, const fvg_prec* p, TeamLayout& L, std:
             a_src.push_back(k);
             if (fac && j < i) {
                 l_dep.push_back(pos[(size_t)j]);
+                l_qdep.push_back(pos[(size_t)j]);
                 l_src.push_back(k);
             }
         }
@@ -750,6 +1186,7 @@ static int team_analyse(const fvg_csr* a, const fvg_prec* p, TeamLayout& L, std:
                 const int32_t j = col[(size_t)k];
                 if (j > i) {
                     u_dep.push_back(pos[(size_t)j]);
+                    u_qdep.push_back((int32_t)(n - 1 - pos[(size_t)j]));
                     u_src.push_back(p->kind == kIC ? p->h_tpos[(size_t)k] : k);
                 }
             }
@@ -757,26 +1194,39 @@ static int team_analyse(const fvg_csr* a, const fvg_prec* p, TeamLayout& L, std:
         }
     }
     L.n = n;
-    L.nlev = nlev;
+    L.npad = (n + kMaxGroup - 1) / kMaxGroup * kMaxGroup;   // whole tiles: rows past n compute zeros
     L.nnz_a = (int64_t)a_dep.size();
-    L.nnz_l = (int64_t)l_dep.size();
-    L.nnz_u = (int64_t)u_dep.size();
-    auto put = [&](const std::vector<int32_t>& v) {
-        const int64_t off = (int64_t)ints.size();
-        ints.insert(ints.end(), v.begin(), v.end());
-        return off;
-    };
-    L.perm = put(perm);
-    L.lvl = put(lvl);
-    L.a_ptr = put(a_ptr);
-    L.a_dep = put(a_dep);
-    L.a_src = put(a_src);
-    L.l_ptr = put(l_ptr);
-    L.l_dep = put(l_dep);
-    L.l_src = put(l_src);
-    L.u_ptr = put(u_ptr);
-    L.u_dep = put(u_dep);
-    L.u_src = put(u_src);
+    L.perm = put_ints(ints, perm);
+    L.a_ptr = put_ints(ints, a_ptr);
+    L.a_dep = put_ints(ints, a_dep);
+    L.a_src = put_ints(ints, a_src);
+    // SpMV window: largest distance between a row's position and its columns' positions
+    int64_t wspan = 0;
+    for (int64_t q = 0; q < n; ++q)
+        for (int32_t k = a_ptr[(size_t)q]; k < a_ptr[(size_t)q + 1]; ++k)
+            wspan = std::max<int64_t>(wspan, std::abs((int64_t)a_dep[(size_t)k] - q));
+    L.wspan = (int32_t)std::min<int64_t>(wspan, INT32_MAX / 4);
+    L.win64 = 2 * kMaxGroup + 2 * wspan <= kRing64 ? 1 : 0;
+    L.win32 = 2 * kMaxGroup + 2 * wspan <= kRing32 ? 1 : 0;
+    if (fac) {
+        std::vector<int32_t> llev((size_t)n), ulev((size_t)n);   // level of each processing index
+        for (int64_t q = 0; q < n; ++q) {
+            llev[(size_t)q] = lev[(size_t)perm[(size_t)q]];
+            ulev[(size_t)q] = nlev - 1 - lev[(size_t)perm[(size_t)(n - 1 - q)]];
+        }
+        for (int side = 0; side < 2; ++side) {
+            TeamSweepLayout& S = side == 0 ? L.lo : L.up;
+            const auto& sptr = side == 0 ? l_ptr : u_ptr;
+            ring_check(n, sptr, side == 0 ? l_qdep : u_qdep, S.ring64, S.ring32);
+            S.lvl = put_ints(ints, side == 0 ? llev : ulev);
+            S.ptr = put_ints(ints, sptr);
+            std::vector<int32_t> dep = side == 0 ? l_dep : u_dep;
+            if (dep.empty()) dep.push_back(0);   // the sweep reads at least one (unused) entry
+            S.dep = put_ints(ints, dep);
+            S.src = put_ints(ints, side == 0 ? l_src : u_src);
+            S.nnz = side == 0 ? (int64_t)l_dep.size() : (int64_t)u_dep.size();
+        }
+    }
     return FVG_OK;
 }
 
@@ -792,8 +1242,8 @@ static int team_refresh(fvg_ctx* c, fvg_batch* B, int k, hipStream_t s) {
     };
     gather(L.nnz_a, L.a_src, a->val, L.a_val);
     if (p->kind == kIC || p->kind == kILU) {
-        gather(L.nnz_l, L.l_src, p->d_fac, L.l_val);
-        gather(L.nnz_u, L.u_src, p->d_fac, L.u_val);
+        gather(L.lo.nnz, L.lo.src, p->d_fac, L.lo.val);
+        gather(L.up.nnz, L.up.src, p->d_fac, L.up.val);
     }
     if (p->kind != kIdentity) gather(L.n, L.perm, p->d_inv_diag, L.sc);
     FVG_HIP(hipGetLastError());
@@ -829,6 +1279,19 @@ static int batch_capture(fvg_batch* B) {
     return FVG_OK;
 }
 
+static void fill_sweep_desc(const fvg_batch* B, const TeamSweepLayout& S, bool upper, TeamSweep& T) {
+    const int32_t* I = B->iarena;
+    T.lvl = I + S.lvl;
+    T.nnz1 = (int32_t)std::max<int64_t>(S.nnz - 1, 0);
+    T.ptr = I + S.ptr;
+    T.dep = I + S.dep;
+    T.val = B->darena + S.val;
+    T.val32 = B->farena + S.val;
+    T.ring64 = S.ring64;
+    T.ring32 = S.ring32;
+    T.upper = upper ? 1 : 0;
+}
+
 }  // namespace fvg
 
 using namespace fvg;
@@ -844,7 +1307,7 @@ int fvg_batch_create(fvg_ctx* c, int32_t n, const fvg_csr* const* a, const fvg_p
     for (int k = 0; k < n; ++k) {
         if (!a[k] || !p[k]) return fail(FVG_E_INVALID, "null matrix or preconditioner");
         if (a[k]->nrows != a[k]->ncols || a[k]->nrows <= 0) return fail(FVG_E_INVALID, "batched systems must be square");
-        if (a[k]->nrows > INT32_MAX / 2) return fail(FVG_E_INVALID, "system too large for a batch");
+        if (a[k]->nrows > INT32_MAX / 4) return fail(FVG_E_INVALID, "system too large for a batch");
         if (p[k]->n != a[k]->nrows) return fail(FVG_E_INVALID, "preconditioner size does not match its matrix");
         if ((p[k]->kind == kIC || p[k]->kind == kILU) && p[k]->h_col != a[k]->h_col)
             return fail(FVG_E_INVALID, "preconditioner pattern differs from its matrix");
@@ -855,7 +1318,6 @@ int fvg_batch_create(fvg_ctx* c, int32_t n, const fvg_csr* const* a, const fvg_p
     B->precision = o->precision;
     B->opts = *o;
     B->variant = (solver == FVG_SOLVER_CG ? 0 : 1) + (o->precision == FVG_PRECISION_MIXED ? 2 : 0);
-    B->nt = 1024;
     B->A.assign(a, a + n);
     B->P.assign(p, p + n);
     B->a_ver.assign((size_t)n, ~0ull);
@@ -872,16 +1334,16 @@ int fvg_batch_create(fvg_ctx* c, int32_t n, const fvg_csr* const* a, const fvg_p
         TeamLayout& L = B->lay[(size_t)k];
         L.a_val = B->nv;
         B->nv += L.nnz_a;
-        L.l_val = B->nv;
-        B->nv += L.nnz_l;
-        L.u_val = B->nv;
-        B->nv += L.nnz_u;
+        L.lo.val = B->nv;
+        B->nv += std::max<int64_t>(L.lo.nnz, 1);
+        L.up.val = B->nv;
+        B->nv += std::max<int64_t>(L.up.nnz, 1);
         L.sc = B->nv;
         B->nv += L.n;
         L.w64 = B->nw64;
-        B->nw64 += w64_per * L.n;
+        B->nw64 += w64_per * L.npad;
         L.w32 = B->nw32;
-        B->nw32 += w32_per * L.n;
+        B->nw32 += w32_per * L.npad;
         L.hist = B->nhist;
         B->nhist += B->hcap;
     }
@@ -895,6 +1357,11 @@ int fvg_batch_create(fvg_ctx* c, int32_t n, const fvg_csr* const* a, const fvg_p
     if (s == FVG_OK) s = dev_alloc_n(c, n, &B->d_info);
     if (s == FVG_OK) s = dev_alloc_n(c, n, &B->d_desc);
     if (s == FVG_OK) s = upload(c, B->iarena, ints.data(), B->ni);
+    // padding elements of the work vectors stay zero for the whole life of the batch
+    if (s == FVG_OK && B->nw64 > 0 && hipMemsetAsync(B->w64, 0, sizeof(double) * B->nw64, c->stream) != hipSuccess)
+        s = fail(FVG_E_HIP, "workspace clear failed");
+    if (s == FVG_OK && B->nw32 > 0 && hipMemsetAsync(B->w32, 0, sizeof(float) * B->nw32, c->stream) != hipSuccess)
+        s = fail(FVG_E_HIP, "workspace clear failed");
     if (s == FVG_OK && hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess)
         s = fail(FVG_E_HIP, "batch stream creation failed");
     if (s == FVG_OK && hipEventCreateWithFlags(&B->ready, hipEventDisableTiming) != hipSuccess)
@@ -913,23 +1380,20 @@ int fvg_batch_create(fvg_ctx* c, int32_t n, const fvg_csr* const* a, const fvg_p
             TeamDesc& D = B->desc[(size_t)slot];
             std::memset(&D, 0, sizeof(D));
             D.n = (int32_t)L.n;
-            D.nlev = (int32_t)L.nlev;
+            D.npad = (int32_t)L.npad;
+            D.wspan = L.wspan;
+            D.a_nnz1 = (int32_t)(L.nnz_a - 1);
+            D.win64 = L.win64;
+            D.win32 = L.win32;
             D.kind = p[k]->kind;
             int32_t* I = B->iarena;
             D.perm = I + L.perm;
-            D.lvl = I + L.lvl;
             D.a_ptr = I + L.a_ptr;
             D.a_dep = I + L.a_dep;
             D.a_val = B->darena + L.a_val;
             D.a_val32 = B->farena + L.a_val;
-            D.l_ptr = I + L.l_ptr;
-            D.l_dep = I + L.l_dep;
-            D.l_val = B->darena + L.l_val;
-            D.l_val32 = B->farena + L.l_val;
-            D.u_ptr = I + L.u_ptr;
-            D.u_dep = I + L.u_dep;
-            D.u_val = B->darena + L.u_val;
-            D.u_val32 = B->farena + L.u_val;
+            fill_sweep_desc(B, L.lo, false, D.lo);
+            fill_sweep_desc(B, L.up, true, D.up);
             D.sc = B->darena + L.sc;
             D.sc32 = B->farena + L.sc;
             D.w64 = B->w64 + L.w64;
@@ -998,8 +1462,9 @@ int fvg_batch_launch(fvg_ctx* c, fvg_batch* B, const fvg_vec* const* b, fvg_vec*
     for (int k = 0; k < B->n_inst; ++k)
         if (B->a_ver[(size_t)k] != B->A[(size_t)k]->version || B->p_ver[(size_t)k] != B->P[(size_t)k]->fac_version)
             FVG_TRY(team_refresh(c, B, k, c->stream));
-    if (moved) FVG_HIP(hipMemcpyAsync(B->d_desc, B->desc.data(), sizeof(TeamDesc) * B->desc.size(),
-                                      hipMemcpyHostToDevice, c->stream));
+    if (moved)
+        FVG_HIP(hipMemcpyAsync(B->d_desc, B->desc.data(), sizeof(TeamDesc) * B->desc.size(), hipMemcpyHostToDevice,
+                               c->stream));
     // the batch stream starts after everything enqueued on the context stream so far
     FVG_HIP(hipEventRecord(B->ready, c->stream));
     FVG_HIP(hipStreamWaitEvent(B->stream, B->ready, 0));
@@ -1066,6 +1531,33 @@ int fvg_batch_set_threads(fvg_ctx* c, fvg_batch* B, int32_t threads) {
         return fail(FVG_E_INVALID, "workgroup size must be 256, 512 or 1024");
     if (B->launched) return fail(FVG_E_INVALID, "batch in flight");
     B->nt = threads;
+    return FVG_OK;
+}
+
+int fvg_batch_set_phase_timing(fvg_ctx* c, fvg_batch* B, int32_t enable) {
+    if (!c || !B) return fail(FVG_E_INVALID, "null argument");
+    if (B->launched) return fail(FVG_E_INVALID, "batch in flight");
+    if (enable && !B->d_prof) {
+        FVG_TRY(dev_alloc_n(c, 4 * (int64_t)B->n_inst, &B->d_prof));
+        FVG_HIP(hipMemsetAsync(B->d_prof, 0, sizeof(int64_t) * 4 * B->n_inst, c->stream));
+    }
+    for (int slot = 0; slot < B->n_inst; ++slot)
+        B->desc[(size_t)slot].prof = enable ? B->d_prof + 4 * B->order[(size_t)slot] : nullptr;
+    FVG_HIP(hipMemcpyAsync(B->d_desc, B->desc.data(), sizeof(TeamDesc) * B->desc.size(), hipMemcpyHostToDevice,
+                           c->stream));
+    FVG_HIP(hipStreamSynchronize(c->stream));
+    return FVG_OK;
+}
+
+int fvg_batch_phase_times(fvg_ctx* c, fvg_batch* B, double* us) {
+    if (!c || !B || !us) return fail(FVG_E_INVALID, "null argument");
+    if (!B->d_prof) return fail(FVG_E_INVALID, "phase timing is not enabled");
+    std::vector<int64_t> t((size_t)4 * B->n_inst);
+    FVG_TRY(download(c, t.data(), B->d_prof, (int64_t)t.size()));
+    int khz = 0;
+    FVG_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+    if (khz <= 0) return fail(FVG_E_HIP, "wall clock rate unavailable");
+    for (size_t i = 0; i < t.size(); ++i) us[i] = (double)t[i] * 1e3 / (double)khz;
     return FVG_OK;
 }
 

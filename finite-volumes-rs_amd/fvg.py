@@ -143,6 +143,8 @@ def lib():
         "fvg_batch_solve": [P, P, P, P, P, P, C.c_int64],
         "fvg_batch_set_threads": [P, P, C.c_int32],
         "fvg_batch_get_threads": [P, C.POINTER(C.c_int32)],
+        "fvg_batch_set_phase_timing": [P, P, C.c_int32],
+        "fvg_batch_phase_times": [P, P, P],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -744,6 +746,15 @@ class Batch:
     @threads.setter
     def threads(self, value):
         check(lib().fvg_batch_set_threads(self.ctx.h, self.h, int(value)))
+
+    def set_phase_timing(self, enable=True):
+        check(lib().fvg_batch_set_phase_timing(self.ctx.h, self.h, 1 if enable else 0))
+
+    def phase_times(self):
+        """Microseconds per system in SpMV, lower sweeps, upper sweeps and the whole solve (n x 4)."""
+        out = np.zeros((self.n, 4))
+        check(lib().fvg_batch_phase_times(self.ctx.h, self.h, _ptr(out)))
+        return out
 
     def launch(self, rhs, solutions):
         """Enqueue the solves (solutions: initial guesses in, answers out); returns at once."""

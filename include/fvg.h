@@ -258,9 +258,14 @@ int fvg_batch_finish(fvg_ctx* ctx, fvg_batch* batch, fvg_solver_info* infos, dou
 /* launch + finish */
 int fvg_batch_solve(fvg_ctx* ctx, fvg_batch* batch, const fvg_vec* const* b, fvg_vec* const* x,
                     fvg_solver_info* infos, double* const* histories, int64_t history_capacity);
-/* Threads per system (256, 512 or 1024; default 1024). */
+/* Threads per system (256, 512 or 1024; default 512). */
 int fvg_batch_set_threads(fvg_ctx* ctx, fvg_batch* batch, int32_t threads);
 int fvg_batch_get_threads(const fvg_batch* batch, int32_t* threads);
+/* Phase timing inside each system's workgroup (diagnostic; off by default).  After a
+ * finished launch, us[4k + j] holds system k's microseconds in SpMV (j=0), lower sweeps
+ * (1), upper sweeps (2) and its whole solve (3). */
+int fvg_batch_set_phase_timing(fvg_ctx* ctx, fvg_batch* batch, int32_t enable);
+int fvg_batch_phase_times(fvg_ctx* ctx, fvg_batch* batch, double* us);
 
 /* ---- device timing of the most recent solve ------------------------------- */
 typedef struct fvg_kernel_stats {
