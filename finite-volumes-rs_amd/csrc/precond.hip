@@ -369,6 +369,7 @@ static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::ve
 }
 
 static void free_prec(fvg_ctx* c, fvg_prec* p) {
+    team_plan_release(c, p);
     dev_free_n(c, p->d_rp, p->n + 1);
     dev_free_n(c, p->d_col, p->nnz);
     dev_free_n(c, p->d_diag, p->n);

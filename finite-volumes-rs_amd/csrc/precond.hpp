@@ -45,6 +45,7 @@ struct fvg_prec {
     float* d_inv_diag32 = nullptr;
     float* d_tmp32 = nullptr;
     uint64_t fac_version = 0, fac32_version = 0;
+    void* team = nullptr;               // batched-solve plan of this system (team.hip), made on use
 };
 
 namespace fvg {
@@ -57,4 +58,6 @@ int check_sweep_status(fvg_ctx* ctx, const fvg_prec* p);
 int ensure_prec32(fvg_ctx* ctx, fvg_prec* p);
 // solution = M^-1 rhs with FP32 factors, vectors and arithmetic.
 int apply_prec32(fvg_ctx* ctx, const fvg_prec* p, const float* rhs, float* solution, const int32_t* skip);
+// frees the batched-solve plan cached on p (team.hip)
+void team_plan_release(fvg_ctx* ctx, fvg_prec* p);
 }  // namespace fvg

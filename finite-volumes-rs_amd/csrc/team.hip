@@ -85,6 +85,8 @@ struct TeamDesc {
     double atol, rtol, inner_tol;
     int64_t max_iter;
     int64_t* prof;           // optional: wall-clock ticks spent in SpMV, lower sweep, upper sweep, total
+    int32_t* done;           // host-visible completion flag of this system (pinned host memory)
+    int64_t pad2;
 };
 
 // ---- optional phase timing (thread 0's view of its workgroup) ----------------------------
@@ -603,7 +605,7 @@ __device__ bool team_zero_rhs(const TeamDesc& D, double bnorm, bool mixed) {
 // ---- FP64 conjugate_gradient (conjugate_gradient.rs:12-112) ------------------------------
 template <int NT>
 __device__ void team_cg64(const TeamDesc& D, Reducer<NT, double>& R, double* ring) {
-    const int n = D.n, np = D.npad, nv = D.npad / 4;
+    const int np = D.npad, nv = D.npad / 4;
     double* b = D.w64;
     double* x = b + np;
     double* r = x + np;
@@ -988,7 +990,10 @@ __global__ void __launch_bounds__(NT) k_team(const TeamDesc* __restrict__ descs,
             g_prof[3] = wall_clock64() - t0;
             for (int j = 0; j < 4; ++j) D.prof[j] = g_prof[j];
         }
+        // every wave's writes (solution, info) reach memory before the completion flag
+        __threadfence_system();
         __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(D.done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -1024,21 +1029,38 @@ static const void* team_kernel_for(int nt, int var) {
 
 }  // namespace fvg
 
-// one sweep's tables, as offsets into the batch's int arena
+// one sweep's tables, as offsets into its plan's index array
 struct TeamSweepLayout {
     int64_t lvl = 0, ptr = 0, dep = 0, src = 0;
     int64_t nnz = 0, val = 0;
     int32_t ring64 = 0, ring32 = 0;
 };
 
-// arrays of one system, as offsets into the batch arenas
+// arrays of one system, as offsets into its plan's arrays
 struct TeamLayout {
     int64_t n = 0, npad = 0, nnz_a = 0;
     int32_t wspan = 0, win64 = 0, win32 = 0;
     int64_t perm = 0, a_ptr = 0, a_dep = 0, a_src = 0;   // int arena
     TeamSweepLayout lo, up;
-    int64_t a_val = 0, sc = 0;                           // value arenas
-    int64_t w64 = 0, w32 = 0, hist = 0;
+    int64_t a_val = 0, sc = 0;                           // value arrays
+};
+
+// Per-system plan, cached on the preconditioner: the analysis of the matrix pattern,
+// the gathered values and the work vectors.  A batch only points at plans, so building
+// a batch over already-planned systems costs no analysis and no large allocation.
+struct TeamPlan {
+    const fvg_csr* a = nullptr;   // matrix the values are gathered from
+    TeamLayout L;                 // offsets into this plan's arrays
+    int32_t* ints = nullptr;
+    int64_t ni = 0;
+    double* v64 = nullptr;
+    float* v32 = nullptr;
+    int64_t nv = 0;
+    double* w64 = nullptr;        // work vectors: 10 of npad doubles (FP64 BiCGStab is the largest user)
+    float* w32 = nullptr;         // 10 of npad floats
+    int64_t nw = 0;
+    uint64_t a_ver = ~0ull, p_ver = ~0ull;
+    bool busy = false;            // part of a launched batch
 };
 
 struct fvg_batch {
@@ -1049,22 +1071,13 @@ struct fvg_batch {
     int nt = 512;
     fvg_solver_options opts{};
     std::vector<const fvg_csr*> A;
-    std::vector<const fvg_prec*> P;
-    std::vector<uint64_t> a_ver, p_ver;
-    std::vector<TeamLayout> lay;
+    std::vector<fvg_prec*> P;
+    std::vector<TeamPlan*> plan;
     std::vector<int> order;           // launch slot -> system (largest first)
-    int32_t* iarena = nullptr;
-    int64_t ni = 0;
-    double* darena = nullptr;
-    float* farena = nullptr;
-    int64_t nv = 0;
-    double* w64 = nullptr;
-    int64_t nw64 = 0;
-    float* w32 = nullptr;
-    int64_t nw32 = 0;
     double* hist = nullptr;
     int64_t nhist = 0, hcap = 0;
-    fvg_solver_info* d_info = nullptr;
+    fvg_solver_info* h_info = nullptr;  // pinned host memory the kernel writes each system's info to
+    int32_t* h_done = nullptr;          // pinned host completion flags
     int64_t* d_prof = nullptr;        // 4 counters per system when phase timing is on
     fvg::TeamDesc* d_desc = nullptr;
     std::vector<fvg::TeamDesc> desc;  // by launch slot
@@ -1080,19 +1093,34 @@ namespace fvg {
 
 static void batch_free(fvg_ctx* c, fvg_batch* B) {
     if (B->stream) (void)hipStreamSynchronize(B->stream);
+    if (B->launched)
+        for (TeamPlan* t : B->plan) t->busy = false;
     if (B->exec) (void)hipGraphExecDestroy(B->exec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
-    dev_free_n(c, B->iarena, B->ni);
-    dev_free_n(c, B->darena, B->nv);
-    dev_free_n(c, B->farena, B->nv);
-    dev_free_n(c, B->w64, B->nw64);
-    dev_free_n(c, B->w32, B->nw32);
     dev_free_n(c, B->hist, B->nhist);
-    dev_free_n(c, B->d_info, B->n_inst);
+    if (B->h_info) (void)hipHostFree(B->h_info);
+    if (B->h_done) (void)hipHostFree(B->h_done);
     dev_free_n(c, B->d_prof, 4 * B->n_inst);
     dev_free_n(c, B->d_desc, B->n_inst);
     if (B->ready) (void)hipEventDestroy(B->ready);
     if (B->stream) (void)hipStreamDestroy(B->stream);
+}
+
+static void plan_free(fvg_ctx* c, TeamPlan* t) {
+    dev_free_n(c, t->ints, t->ni);
+    dev_free_n(c, t->v64, t->nv);
+    dev_free_n(c, t->v32, t->nv);
+    dev_free_n(c, t->w64, t->nw);
+    dev_free_n(c, t->w32, t->nw);
+}
+
+void team_plan_release(fvg_ctx* c, fvg_prec* p) {
+    if (!p->team) return;
+    auto* t = static_cast<TeamPlan*>(p->team);
+    if (t->busy) (void)hipDeviceSynchronize();   // destroyed while a batch using it runs: let it finish
+    plan_free(c, t);
+    delete t;
+    p->team = nullptr;
 }
 
 static int64_t put_ints(std::vector<int32_t>& ints, const std::vector<int32_t>& v) {
@@ -1230,15 +1258,14 @@ static int team_analyse(const fvg_csr* a, const fvg_prec* p, TeamLayout& L, std:
     return FVG_OK;
 }
 
-// copy matrix and factor values into the batch's permuted arrays (FP64 and FP32)
-static int team_refresh(fvg_ctx* c, fvg_batch* B, int k, hipStream_t s) {
-    const TeamLayout& L = B->lay[(size_t)k];
-    const fvg_csr* a = B->A[(size_t)k];
-    const fvg_prec* p = B->P[(size_t)k];
+// copy matrix and factor values into the plan's permuted arrays (FP64 and FP32)
+static int plan_refresh(fvg_ctx* c, TeamPlan* t, const fvg_prec* p, hipStream_t s) {
+    const TeamLayout& L = t->L;
+    const fvg_csr* a = t->a;
     auto gather = [&](int64_t cnt, int64_t src_off, const double* in, int64_t dst) {
         if (cnt <= 0 || !in) return;
-        hipLaunchKernelGGL(k_team_gather, dim3(grid_for(cnt)), dim3(kBlock), 0, s, cnt, B->iarena + src_off, in,
-                           B->darena + dst, B->farena + dst);
+        hipLaunchKernelGGL(k_team_gather, dim3(grid_for(cnt)), dim3(kBlock), 0, s, cnt, t->ints + src_off, in,
+                           t->v64 + dst, t->v32 + dst);
     };
     gather(L.nnz_a, L.a_src, a->val, L.a_val);
     if (p->kind == kIC || p->kind == kILU) {
@@ -1247,8 +1274,50 @@ static int team_refresh(fvg_ctx* c, fvg_batch* B, int k, hipStream_t s) {
     }
     if (p->kind != kIdentity) gather(L.n, L.perm, p->d_inv_diag, L.sc);
     FVG_HIP(hipGetLastError());
-    B->a_ver[(size_t)k] = a->version;
-    B->p_ver[(size_t)k] = p->fac_version;
+    t->a_ver = a->version;
+    t->p_ver = p->fac_version;
+    return FVG_OK;
+}
+
+// the plan of system (a, p): made on first use, remade when p is used with another matrix
+static int get_plan(fvg_ctx* c, const fvg_csr* a, fvg_prec* p, TeamPlan** out) {
+    if (p->team && static_cast<TeamPlan*>(p->team)->a == a) {
+        *out = static_cast<TeamPlan*>(p->team);
+        return FVG_OK;
+    }
+    if (p->team && static_cast<TeamPlan*>(p->team)->busy)
+        return fail(FVG_E_INVALID, "preconditioner is part of a batch in flight");
+    team_plan_release(c, p);
+    auto* t = new TeamPlan();
+    t->a = a;
+    std::vector<int32_t> ints;
+    int s = team_analyse(a, p, t->L, ints);
+    TeamLayout& L = t->L;
+    if (s == FVG_OK) {
+        L.a_val = 0;
+        L.lo.val = L.a_val + L.nnz_a;
+        L.up.val = L.lo.val + std::max<int64_t>(L.lo.nnz, 1);
+        L.sc = L.up.val + std::max<int64_t>(L.up.nnz, 1);
+        t->nv = L.sc + L.n;
+        t->nw = 10 * L.npad;
+        t->ni = (int64_t)ints.size();
+        if ((s = dev_alloc_n(c, t->ni, &t->ints)) == FVG_OK && (s = dev_alloc_n(c, t->nv, &t->v64)) == FVG_OK &&
+            (s = dev_alloc_n(c, t->nv, &t->v32)) == FVG_OK && (s = dev_alloc_n(c, t->nw, &t->w64)) == FVG_OK &&
+            (s = dev_alloc_n(c, t->nw, &t->w32)) == FVG_OK)
+            s = upload(c, t->ints, ints.data(), t->ni);
+        // padding elements of the work vectors stay zero for the life of the plan
+        if (s == FVG_OK && (hipMemsetAsync(t->w64, 0, sizeof(double) * t->nw, c->stream) != hipSuccess ||
+                            hipMemsetAsync(t->w32, 0, sizeof(float) * t->nw, c->stream) != hipSuccess))
+            s = fail(FVG_E_HIP, "workspace clear failed");
+    }
+    if (s != FVG_OK) {
+        (void)hipStreamSynchronize(c->stream);
+        plan_free(c, t);
+        delete t;
+        return s;
+    }
+    p->team = t;
+    *out = t;
     return FVG_OK;
 }
 
@@ -1279,14 +1348,14 @@ static int batch_capture(fvg_batch* B) {
     return FVG_OK;
 }
 
-static void fill_sweep_desc(const fvg_batch* B, const TeamSweepLayout& S, bool upper, TeamSweep& T) {
-    const int32_t* I = B->iarena;
+static void fill_sweep_desc(const TeamPlan* B, const TeamSweepLayout& S, bool upper, TeamSweep& T) {
+    const int32_t* I = B->ints;
     T.lvl = I + S.lvl;
     T.nnz1 = (int32_t)std::max<int64_t>(S.nnz - 1, 0);
     T.ptr = I + S.ptr;
     T.dep = I + S.dep;
-    T.val = B->darena + S.val;
-    T.val32 = B->farena + S.val;
+    T.val = B->v64 + S.val;
+    T.val32 = B->v32 + S.val;
     T.ring64 = S.ring64;
     T.ring32 = S.ring32;
     T.upper = upper ? 1 : 0;
@@ -1311,6 +1380,8 @@ int fvg_batch_create(fvg_ctx* c, int32_t n, const fvg_csr* const* a, const fvg_p
         if (p[k]->n != a[k]->nrows) return fail(FVG_E_INVALID, "preconditioner size does not match its matrix");
         if ((p[k]->kind == kIC || p[k]->kind == kILU) && p[k]->h_col != a[k]->h_col)
             return fail(FVG_E_INVALID, "preconditioner pattern differs from its matrix");
+        for (int j = 0; j < k; ++j)
+            if (p[j] == p[k]) return fail(FVG_E_INVALID, "a preconditioner appears twice in one batch");
     }
     auto* B = new fvg_batch();
     B->n_inst = n;
@@ -1319,64 +1390,41 @@ int fvg_batch_create(fvg_ctx* c, int32_t n, const fvg_csr* const* a, const fvg_p
     B->opts = *o;
     B->variant = (solver == FVG_SOLVER_CG ? 0 : 1) + (o->precision == FVG_PRECISION_MIXED ? 2 : 0);
     B->A.assign(a, a + n);
-    B->P.assign(p, p + n);
-    B->a_ver.assign((size_t)n, ~0ull);
-    B->p_ver.assign((size_t)n, ~0ull);
-    B->lay.resize((size_t)n);
-    std::vector<int32_t> ints;
-    int s = FVG_OK;
-    for (int k = 0; k < n && s == FVG_OK; ++k) s = team_analyse(a[k], p[k], B->lay[(size_t)k], ints);
+    B->P.resize((size_t)n);
+    B->plan.resize((size_t)n, nullptr);
     const bool mixed = o->precision == FVG_PRECISION_MIXED;
-    const int w64_per = mixed ? 2 : (solver == FVG_SOLVER_CG ? 6 : 10);
-    const int w32_per = mixed ? (solver == FVG_SOLVER_CG ? 5 : 10) : 0;
     B->hcap = mixed ? 64 : o->max_iterations + 2;
+    int s = FVG_OK;
     for (int k = 0; k < n && s == FVG_OK; ++k) {
-        TeamLayout& L = B->lay[(size_t)k];
-        L.a_val = B->nv;
-        B->nv += L.nnz_a;
-        L.lo.val = B->nv;
-        B->nv += std::max<int64_t>(L.lo.nnz, 1);
-        L.up.val = B->nv;
-        B->nv += std::max<int64_t>(L.up.nnz, 1);
-        L.sc = B->nv;
-        B->nv += L.n;
-        L.w64 = B->nw64;
-        B->nw64 += w64_per * L.npad;
-        L.w32 = B->nw32;
-        B->nw32 += w32_per * L.npad;
-        L.hist = B->nhist;
-        B->nhist += B->hcap;
+        B->P[(size_t)k] = const_cast<fvg_prec*>(p[k]);   // the plan is a cache owned by the preconditioner
+        s = get_plan(c, a[k], B->P[(size_t)k], &B->plan[(size_t)k]);
     }
-    B->ni = (int64_t)ints.size();
-    if (s == FVG_OK) s = dev_alloc_n(c, B->ni, &B->iarena);
-    if (s == FVG_OK) s = dev_alloc_n(c, B->nv, &B->darena);
-    if (s == FVG_OK) s = dev_alloc_n(c, B->nv, &B->farena);
-    if (s == FVG_OK) s = dev_alloc_n(c, B->nw64, &B->w64);
-    if (s == FVG_OK) s = dev_alloc_n(c, B->nw32, &B->w32);
+    B->nhist = B->hcap * n;
     if (s == FVG_OK) s = dev_alloc_n(c, B->nhist, &B->hist);
-    if (s == FVG_OK) s = dev_alloc_n(c, n, &B->d_info);
+    if (s == FVG_OK && (hipHostMalloc((void**)&B->h_info, sizeof(fvg_solver_info) * n, hipHostMallocDefault) !=
+                            hipSuccess ||
+                        hipHostMalloc((void**)&B->h_done, sizeof(int32_t) * n, hipHostMallocDefault) != hipSuccess))
+        s = fail(FVG_E_HIP, "pinned batch status allocation failed");
+    if (s == FVG_OK) {
+        std::memset(B->h_info, 0, sizeof(fvg_solver_info) * n);
+        std::memset(B->h_done, 0, sizeof(int32_t) * n);
+    }
     if (s == FVG_OK) s = dev_alloc_n(c, n, &B->d_desc);
-    if (s == FVG_OK) s = upload(c, B->iarena, ints.data(), B->ni);
-    // padding elements of the work vectors stay zero for the whole life of the batch
-    if (s == FVG_OK && B->nw64 > 0 && hipMemsetAsync(B->w64, 0, sizeof(double) * B->nw64, c->stream) != hipSuccess)
-        s = fail(FVG_E_HIP, "workspace clear failed");
-    if (s == FVG_OK && B->nw32 > 0 && hipMemsetAsync(B->w32, 0, sizeof(float) * B->nw32, c->stream) != hipSuccess)
-        s = fail(FVG_E_HIP, "workspace clear failed");
     if (s == FVG_OK && hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess)
         s = fail(FVG_E_HIP, "batch stream creation failed");
     if (s == FVG_OK && hipEventCreateWithFlags(&B->ready, hipEventDisableTiming) != hipSuccess)
         s = fail(FVG_E_HIP, "batch event creation failed");
-    for (int k = 0; k < n && s == FVG_OK; ++k) s = team_refresh(c, B, k, c->stream);
     if (s == FVG_OK) {
         // largest systems first, so they start first and spread over the XCDs
         B->order.resize((size_t)n);
         std::iota(B->order.begin(), B->order.end(), 0);
         std::stable_sort(B->order.begin(), B->order.end(),
-                         [&](int x, int y) { return B->lay[(size_t)x].n > B->lay[(size_t)y].n; });
+                         [&](int x, int y) { return B->plan[(size_t)x]->L.n > B->plan[(size_t)y]->L.n; });
         B->desc.resize((size_t)n);
         for (int slot = 0; slot < n; ++slot) {
             const int k = B->order[(size_t)slot];
-            const TeamLayout& L = B->lay[(size_t)k];
+            const TeamPlan* t = B->plan[(size_t)k];
+            const TeamLayout& L = t->L;
             TeamDesc& D = B->desc[(size_t)slot];
             std::memset(&D, 0, sizeof(D));
             D.n = (int32_t)L.n;
@@ -1386,21 +1434,22 @@ int fvg_batch_create(fvg_ctx* c, int32_t n, const fvg_csr* const* a, const fvg_p
             D.win64 = L.win64;
             D.win32 = L.win32;
             D.kind = p[k]->kind;
-            int32_t* I = B->iarena;
+            const int32_t* I = t->ints;
             D.perm = I + L.perm;
             D.a_ptr = I + L.a_ptr;
             D.a_dep = I + L.a_dep;
-            D.a_val = B->darena + L.a_val;
-            D.a_val32 = B->farena + L.a_val;
-            fill_sweep_desc(B, L.lo, false, D.lo);
-            fill_sweep_desc(B, L.up, true, D.up);
-            D.sc = B->darena + L.sc;
-            D.sc32 = B->farena + L.sc;
-            D.w64 = B->w64 + L.w64;
-            D.w32 = B->w32 + L.w32;
-            D.hist = B->hist + L.hist;
+            D.a_val = t->v64 + L.a_val;
+            D.a_val32 = t->v32 + L.a_val;
+            fill_sweep_desc(t, L.lo, false, D.lo);
+            fill_sweep_desc(t, L.up, true, D.up);
+            D.sc = t->v64 + L.sc;
+            D.sc32 = t->v32 + L.sc;
+            D.w64 = t->w64;
+            D.w32 = t->w32;
+            D.hist = B->hist + B->hcap * k;
             D.hcap = B->hcap;
-            D.info = B->d_info + k;
+            D.info = B->h_info + k;
+            D.done = B->h_done + k;
             D.atol = o->absolute_tolerance;
             D.rtol = o->relative_tolerance;
             D.inner_tol = o->inner_tolerance > 0.0 ? o->inner_tolerance : 1e-4;
@@ -1433,7 +1482,7 @@ int fvg_batch_size(const fvg_batch* B, int32_t* n_systems, int64_t* total_rows) 
     if (n_systems) *n_systems = B->n_inst;
     if (total_rows) {
         int64_t t = 0;
-        for (const auto& L : B->lay) t += L.n;
+        for (const TeamPlan* pl : B->plan) t += pl->L.n;
         *total_rows = t;
     }
     return FVG_OK;
@@ -1442,13 +1491,15 @@ int fvg_batch_size(const fvg_batch* B, int32_t* n_systems, int64_t* total_rows) 
 int fvg_batch_launch(fvg_ctx* c, fvg_batch* B, const fvg_vec* const* b, fvg_vec* const* x) {
     if (!c || !B || !b || !x) return fail(FVG_E_INVALID, "null argument");
     if (B->launched) return fail(FVG_E_INVALID, "batch already in flight: finish it first");
+    for (const TeamPlan* t : B->plan)
+        if (t->busy) return fail(FVG_E_INVALID, "a system of this batch is in another batch in flight");
     bool moved = false;
     for (int slot = 0; slot < B->n_inst; ++slot) {
         const int k = B->order[(size_t)slot];
         const fvg_vec* bk = b[k];
         fvg_vec* xk = x[k];
         if (!bk || !xk) return fail(FVG_E_INVALID, "null vector");
-        if (bk->n != B->lay[(size_t)k].n || xk->n != B->lay[(size_t)k].n)
+        if (bk->n != B->plan[(size_t)k]->L.n || xk->n != B->plan[(size_t)k]->L.n)
             return fail(FVG_E_INVALID, "vector size does not match its system");
         if (bk->d == xk->d) return fail(FVG_E_INVALID, "b and x must not alias");
         TeamDesc& D = B->desc[(size_t)slot];
@@ -1459,18 +1510,22 @@ int fvg_batch_launch(fvg_ctx* c, fvg_batch* B, const fvg_vec* const* b, fvg_vec*
         }
     }
     // refresh values assembled or refactored since the last launch
-    for (int k = 0; k < B->n_inst; ++k)
-        if (B->a_ver[(size_t)k] != B->A[(size_t)k]->version || B->p_ver[(size_t)k] != B->P[(size_t)k]->fac_version)
-            FVG_TRY(team_refresh(c, B, k, c->stream));
+    for (int k = 0; k < B->n_inst; ++k) {
+        TeamPlan* t = B->plan[(size_t)k];
+        if (t->a_ver != t->a->version || t->p_ver != B->P[(size_t)k]->fac_version)
+            FVG_TRY(plan_refresh(c, t, B->P[(size_t)k], c->stream));
+    }
     if (moved)
         FVG_HIP(hipMemcpyAsync(B->d_desc, B->desc.data(), sizeof(TeamDesc) * B->desc.size(), hipMemcpyHostToDevice,
                                c->stream));
+    std::memset(B->h_done, 0, sizeof(int32_t) * (size_t)B->n_inst);
     // the batch stream starts after everything enqueued on the context stream so far
     FVG_HIP(hipEventRecord(B->ready, c->stream));
     FVG_HIP(hipStreamWaitEvent(B->stream, B->ready, 0));
     if (!B->exec || B->graph_nt != B->nt) FVG_TRY(batch_capture(B));
     FVG_HIP(hipGraphLaunch(B->exec, B->stream));
     B->launched = true;
+    for (TeamPlan* t : B->plan) t->busy = true;
     return FVG_OK;
 }
 
@@ -1478,25 +1533,28 @@ int fvg_batch_finish(fvg_ctx* c, fvg_batch* B, fvg_solver_info* infos, double* c
     if (!c || !B) return fail(FVG_E_INVALID, "null argument");
     if (!B->launched) return fail(FVG_E_INVALID, "batch not launched");
     B->launched = false;
+    for (TeamPlan* t : B->plan) t->busy = false;
     FVG_HIP(hipStreamSynchronize(B->stream));
-    std::vector<fvg_solver_info> tmp;
-    if (!infos && histories && cap > 0) {
-        tmp.resize((size_t)B->n_inst);
-        infos = tmp.data();
-    }
-    if (infos)
-        FVG_HIP(hipMemcpyAsync(infos, B->d_info, sizeof(fvg_solver_info) * (size_t)B->n_inst, hipMemcpyDeviceToHost,
-                               B->stream));
-    FVG_HIP(hipStreamSynchronize(B->stream));
+    if (infos) std::memcpy(infos, B->h_info, sizeof(fvg_solver_info) * (size_t)B->n_inst);
+    if (!infos) infos = B->h_info;
     if (histories && cap > 0) {
         for (int k = 0; k < B->n_inst; ++k) {
             if (!histories[k]) continue;
             const int64_t m = std::min(infos[k].history_len, std::min(cap, B->hcap));
             if (m > 0)
-                FVG_HIP(hipMemcpyAsync(histories[k], B->hist + B->lay[(size_t)k].hist, sizeof(double) * (size_t)m,
+                FVG_HIP(hipMemcpyAsync(histories[k], B->hist + B->hcap * k, sizeof(double) * (size_t)m,
                                        hipMemcpyDeviceToHost, B->stream));
         }
         FVG_HIP(hipStreamSynchronize(B->stream));
+    }
+    return FVG_OK;
+}
+
+int fvg_batch_poll(fvg_ctx* c, fvg_batch* B, int32_t* done, fvg_solver_info* infos) {
+    if (!c || !B || !done) return fail(FVG_E_INVALID, "null argument");
+    for (int k = 0; k < B->n_inst; ++k) {
+        done[k] = __atomic_load_n(B->h_done + k, __ATOMIC_ACQUIRE);
+        if (infos && done[k]) infos[k] = B->h_info[k];
     }
     return FVG_OK;
 }

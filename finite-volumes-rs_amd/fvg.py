@@ -139,6 +139,7 @@ def lib():
         "fvg_batch_size": [P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)],
         "fvg_batch_launch": [P, P, P, P],
         "fvg_batch_query": [P, P, C.POINTER(C.c_int32)],
+        "fvg_batch_poll": [P, P, P, P],
         "fvg_batch_finish": [P, P, P, P, C.c_int64],
         "fvg_batch_solve": [P, P, P, P, P, P, C.c_int64],
         "fvg_batch_set_threads": [P, P, C.c_int32],
@@ -764,6 +765,19 @@ class Batch:
         xv = (P * self.n)(*[v.h for v in solutions])
         check(lib().fvg_batch_launch(self.ctx.h, self.h, bv, xv))
         self._inflight = (rhs, solutions)
+
+    def poll(self):
+        """Per-system completion of the launched batch: (done flags, infos of the done ones)."""
+        flags = np.zeros(self.n, np.int32)
+        infos = (SolverInfo * self.n)()
+        check(lib().fvg_batch_poll(self.ctx.h, self.h, _ptr(flags), infos))
+        name = "ConjugateGradient" if self.solver == SOLVER_CG else "BiConjugateGradientStab"
+        out = {}
+        for k in np.nonzero(flags)[0]:
+            i = infos[int(k)]
+            out[int(k)] = LinearSolverInfo(name, i.iterations, i.initial_residual, i.final_residual, np.zeros(0),
+                                           i.restarts, i.outer_iterations, i.true_residual)
+        return flags.astype(bool), out
 
     def done(self):
         d = C.c_int32()

@@ -1,0 +1,364 @@
+"""Serving loop: continuous batching of assemble-and-solve requests on one GPU.
+
+A request names a registered mesh and carries its own fields (diffusivity and source,
+or velocity flux) and solver kind.  The server keeps *slots* resident in HBM: per mesh
+and solver kind, a matrix on the mesh's pattern, its IC(0) or ILU(0) preconditioner
+(with the batched-solve plan cached on it), an assembler whose field buffers are reused,
+and right-hand-side / solution vectors.  Serving a request is: upload its fields,
+assemble on the device, refactor, and solve as a member of a batch.
+
+Scheduling:
+- Admission.  A waiting request is admitted when a free slot of its mesh and kind exists,
+  or a new one fits the HBM budget; idle slots of other meshes are evicted
+  (least recently used first) to make room.  A request that cannot be placed keeps
+  waiting, and the requests behind it are still considered.
+- Lanes.  Up to ``lanes`` batches are in flight at once, each a single kernel launch on a
+  stream of its own, so batches overlap on the device.  Whenever a lane is free and
+  requests are waiting, the admissible ones form a new batch: requests that arrive
+  while others run join at the next free lane instead of waiting for everything in
+  flight to drain.
+- Completion.  Every system of a batch has its own completion flag, so a small request
+  is answered as soon as its own solve ends, not when the batch's largest member does.
+  Its slot returns to the pool when the whole batch is done.
+
+Each result is independent of what it was batched with (one workgroup solves one system,
+with fixed reduction order), so a request gives the same bits alone and in the mix.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import fvg
+
+
+# ---------------------------------------------------------------------------------
+# requests and statistics (pure host logic)
+# ---------------------------------------------------------------------------------
+
+@dataclass
+class Request:
+    """One solve: mesh key, solver kind ('cg' or 'bicgstab') and the equation's fields."""
+    rid: int
+    mesh: str
+    kind: str
+    prog: list
+    face_fields: list
+    cell_fields: list
+    bc_i: np.ndarray
+    bc_e: np.ndarray
+    arrival: float = 0.0          # seconds after the start of the run
+    admitted: float = -1.0
+    done: float = -1.0
+    x: np.ndarray | None = None
+    info: object = None
+    batch_size: int = 0
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+def poisson_arrivals(n, qps, seed=0):
+    """Arrival times (seconds) of n requests from a Poisson process of rate qps."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / qps, n))
+
+
+def latency_summary(requests, wall):
+    """p50 / p99 / mean / max latency in milliseconds and throughput in solves/s."""
+    lat = np.array([r.latency for r in requests if r.done >= 0.0]) * 1e3
+    if lat.size == 0:
+        return dict(count=0)
+    return dict(count=int(lat.size), p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                mean_ms=float(lat.mean()), max_ms=float(lat.max()), throughput=float(lat.size / wall))
+
+
+def form_group(pending, can_place, max_batch, allow_large=True, large_rows=100_000, rows_of=None):
+    """Pick the requests for the next batch, in arrival order.
+
+    The first placeable request fixes the solver kind; later requests of that kind join
+    while ``can_place(req)`` says a slot is available for them, up to max_batch.  With
+    allow_large False, requests on meshes of at least large_rows cells are left waiting.
+    Returns the chosen requests (removed from nothing: the caller removes them).
+    """
+    group, kind = [], None
+    for r in pending:
+        if len(group) >= max_batch:
+            break
+        if kind is not None and r.kind != kind:
+            continue
+        if not allow_large and rows_of is not None and rows_of(r) >= large_rows:
+            continue
+        if not can_place(r):
+            continue
+        group.append(r)
+        kind = r.kind
+    return group
+
+
+# ---------------------------------------------------------------------------------
+# device side
+# ---------------------------------------------------------------------------------
+
+class _MeshEntry:
+    def __init__(self, ctx, key, raw):
+        self.key = key
+        self.raw = raw
+        self.mesh = fvg.Mesh(ctx, raw)
+        self.n = self.mesh.n_cells
+        self.nnz = self.mesh.nnz
+        self.slot_bytes = None    # measured when the first slot is made
+
+
+class _Slot:
+    def __init__(self, server, entry, kind, req):
+        ctx = server.ctx
+        before = ctx.bytes_in_use()
+        self.entry, self.kind = entry, kind
+        self.A = fvg.DistributedMatrix.from_mesh(ctx, entry.mesh)
+        self.b = fvg.DistributedVector(ctx, entry.n)
+        self.x = fvg.DistributedVector(ctx, entry.n)
+        self.asm = fvg.Assembler(entry.mesh, req.prog, req.face_fields, req.cell_fields, req.bc_i, req.bc_e)
+        self.asm.run(self.A, self.b, sync=True)
+        self.M = (fvg.IncompleteCholesky if kind == "cg" else fvg.IncompleteLowerUpper).from_matrix(self.A, 0)
+        # make the batched-solve plan now, so the slot's whole footprint is counted here
+        probe = fvg.Batch(ctx, [self.A], [self.M], kind, server.options)
+        probe.close()
+        self.bytes = ctx.bytes_in_use() - before
+        entry.slot_bytes = self.bytes
+        self.busy = False
+        self.fresh = True          # fields of the creating request are already assembled
+        self.last_used = time.perf_counter()
+
+    def prepare(self, req):
+        """Assemble req's system into this slot and refactor (context stream)."""
+        if not self.fresh:
+            a = self.asm
+            for v, data in zip(a.fvecs, req.face_fields):
+                v.upload(data)
+            for v, data in zip(a.cvecs, req.cell_fields):
+                v.upload(data)
+            if a.bi is not None:
+                a.bi.upload(req.bc_i)
+                a.be.upload(req.bc_e)
+            a.run(self.A, self.b, sync=False)
+            self.M.refactor(self.A)
+        self.fresh = False
+        self.x.fill(0.0)
+
+    def close(self):
+        for o in (self.M, self.A, self.b, self.x):
+            o.close()
+        for v in self.asm.fvecs + self.asm.cvecs + [v for v in (self.asm.bi, self.asm.be) if v is not None]:
+            v.close()
+
+
+@dataclass
+class _Lane:
+    batch: object = None
+    reqs: list = field(default_factory=list)
+    slots: list = field(default_factory=list)
+    reported: np.ndarray | None = None
+    large: bool = False
+
+
+class Server:
+    """Continuous-batching solve server on one device context.
+
+    budget_bytes: HBM the server may hold (matrices, factors, plans, vectors);
+    lanes: batches in flight at once; max_batch: systems per batch;
+    options: fvg.LinearSolverOptions of every solve (mixed precision by default).
+    """
+
+    def __init__(self, budget_bytes=200 * 10**9, lanes=4, max_batch=128, options=None, large_rows=100_000,
+                 device=0):
+        self.ctx = fvg.Context(device, int(budget_bytes))
+        self.io = fvg.Context(device)            # a second stream for result downloads
+        self.budget = int(budget_bytes)
+        self.options = options or fvg.LinearSolverOptions(1e-8, 0.0, 5000, precision=fvg.PRECISION_MIXED)
+        self.lanes = [_Lane() for _ in range(lanes)]
+        self.max_batch = max_batch
+        self.large_rows = large_rows
+        self.meshes = {}
+        self.slots = []
+        self.pending = []
+        self.finished = []
+        self.peak_bytes = 0
+        self.evictions = 0
+        self.t0 = time.perf_counter()
+
+    # -- registration / submission -------------------------------------------------
+    def register_mesh(self, key, raw):
+        self.meshes[key] = _MeshEntry(self.ctx, key, raw)
+        return self.meshes[key]
+
+    def submit(self, req):
+        self.pending.append(req)
+
+    def now(self):
+        return time.perf_counter() - self.t0
+
+    # -- slots and budget ------------------------------------------------------------
+    def _free_slot(self, req):
+        for s in self.slots:
+            if not s.busy and s.entry.key == req.mesh and s.kind == req.kind:
+                return s
+        return None
+
+    def _evict_for(self, need):
+        """Destroy idle slots, least recently used first, until `need` more bytes fit."""
+        idle = sorted((s for s in self.slots if not s.busy), key=lambda s: s.last_used)
+        for s in idle:
+            if self.ctx.bytes_in_use() + need <= self.budget:
+                return True
+            s.close()
+            self.slots.remove(s)
+            self.evictions += 1
+        return self.ctx.bytes_in_use() + need <= self.budget
+
+    def drop_idle_slots(self):
+        """Destroy every slot not in a batch in flight."""
+        for s in [s for s in self.slots if not s.busy]:
+            s.close()
+            self.slots.remove(s)
+
+    def _place(self, req):
+        """A slot for req (existing, new, or after eviction), or None to keep it waiting."""
+        s = self._free_slot(req)
+        if s is not None:
+            return s
+        entry = self.meshes[req.mesh]
+        need = entry.slot_bytes if entry.slot_bytes is not None else 1200 * entry.nnz
+        if not self._evict_for(need):
+            return None
+        try:
+            s = _Slot(self, entry, req.kind, req)
+        except fvg.BudgetError:
+            self.ctx.sync()
+            return None
+        self.slots.append(s)
+        self.peak_bytes = max(self.peak_bytes, self.ctx.bytes_in_use())
+        return s
+
+    # -- the loop --------------------------------------------------------------------
+    def _complete(self, lane):
+        flags, infos = lane.batch.poll()
+        for k in np.nonzero(flags & ~lane.reported)[0]:
+            k = int(k)
+            r = lane.reqs[k]
+            r.info = infos[k]
+            x = np.empty(lane.slots[k].entry.n)
+            fvg.check(fvg.lib().fvg_vec_download(self.io.h, lane.slots[k].x.h, fvg._ptr(x)))
+            r.x = x
+            r.done = self.now()
+            lane.reported[k] = True
+            self.finished.append(r)
+        if lane.reported.all():
+            lane.batch.finish(history=False)
+            lane.batch.close()
+            for s in lane.slots:
+                s.busy = False
+                s.last_used = time.perf_counter()
+            lane.batch, lane.reqs, lane.slots, lane.reported = None, [], [], None
+            return True
+        return bool(flags.any())
+
+    def _launch(self, lane, group, slots, large):
+        for r, s in zip(group, slots):
+            r.admitted = self.now()
+            s.prepare(r)
+        b = fvg.Batch(self.ctx, [s.A for s in slots], [s.M for s in slots], group[0].kind, self.options)
+        self.peak_bytes = max(self.peak_bytes, self.ctx.bytes_in_use())
+        b.launch([s.b for s in slots], [s.x for s in slots])
+        for r in group:
+            r.batch_size = len(group)
+            self.pending.remove(r)
+        lane.batch, lane.reqs, lane.slots = b, list(group), slots
+        lane.reported = np.zeros(len(group), bool)
+        lane.large = large
+
+    def step(self):
+        """Collect finished systems, then admit waiting requests into free lanes.
+        Returns True if anything happened."""
+        progressed = False
+        for lane in self.lanes:
+            if lane.batch is not None:
+                progressed |= self._complete(lane)
+        large_busy = sum(1 for l in self.lanes if l.batch is not None and l.large)
+        for lane in self.lanes:
+            if lane.batch is not None or not self.pending:
+                continue
+            allow_large = large_busy < max(1, len(self.lanes) // 2)
+            reserved = set()
+
+            def can_place(r):
+                s = self._free_slot(r)
+                if s is not None and id(s) not in reserved:
+                    reserved.add(id(s))
+                    return True
+                if s is None:
+                    entry = self.meshes[r.mesh]
+                    need = entry.slot_bytes if entry.slot_bytes is not None else 1200 * entry.nnz
+                    idle = sum(x.bytes for x in self.slots if not x.busy and id(x) not in reserved)
+                    return self.ctx.bytes_in_use() + need <= self.budget + idle
+                return False
+
+            group = form_group(self.pending, can_place, self.max_batch, allow_large, self.large_rows,
+                               lambda r: self.meshes[r.mesh].n)
+            if not group:
+                if all(l.batch is None for l in self.lanes) and not any(s.busy for s in self.slots):
+                    r = self.pending[0]
+                    raise RuntimeError(f"request {r.rid} on mesh {r.mesh} does not fit the HBM budget")
+                break
+            # can_place only estimated; place for real, marking each slot busy at once
+            placed, slots = [], []
+            for r in group:
+                s = self._place(r)
+                if s is None:
+                    break
+                s.busy = True
+                placed.append(r)
+                slots.append(s)
+            if not placed:
+                if all(l.batch is None for l in self.lanes) and not any(s.busy for s in self.slots):
+                    raise RuntimeError(f"request {group[0].rid} on mesh {group[0].mesh} does not fit the HBM budget")
+                break
+            large = any(self.meshes[r.mesh].n >= self.large_rows for r in placed)
+            self._launch(lane, placed, slots, large)
+            if large:
+                large_busy += 1
+            progressed = True
+        return progressed
+
+    def run(self, requests, poll_sleep=2e-5):
+        """Serve requests whose .arrival times (seconds from now) are set; returns when
+        all are answered.  Latency counts from each request's scheduled arrival."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        self.t0 = time.perf_counter()
+        i, n = 0, len(reqs)
+        start_finished = len(self.finished)
+        while len(self.finished) - start_finished < n:
+            now = self.now()
+            while i < n and reqs[i].arrival <= now:
+                self.submit(reqs[i])
+                i += 1
+            if not self.step():
+                time.sleep(poll_sleep)
+        return self.now()
+
+    def close(self):
+        for lane in self.lanes:
+            if lane.batch is not None:
+                lane.batch.finish(history=False)
+                lane.batch.close()
+                lane.batch = None
+        for s in self.slots:
+            s.close()
+        self.slots = []
+        for e in self.meshes.values():
+            e.mesh.close()
+        self.io.close()
+        self.ctx.close()

@@ -18,6 +18,9 @@ Workloads (sizes as in the benchmark definition):
                    upwind, inflow u = 1 / outflow zero gradient, BiCGStab + ILU(0)
   batched_2d       256 independent jittered triangle meshes, cells log-uniform in
                    [1k, 64k], ln k ~ N(0, 1) per cell, f ~ N(0,1), u = 0, CG + IC(0)
+  serving          Poisson stream over a pool of 2D triangle (1k-64k cells, 70%) and 3D
+                   tet (50k-500k cells, 30%) meshes; diffusion (CG + IC(0)) or
+                   convection-diffusion with cell Peclet U[1,50] (BiCGStab + ILU(0))
 """
 from __future__ import annotations
 
@@ -118,6 +121,52 @@ def batched_2d(n_inst=256, seed=1, lo=1000, hi=64000, sigma=1.0):
             return diffusion_fields(raw, geom, kc, f, bc="zero")
 
         out.append((raw, fields))
+    return out
+
+
+def serving_pool(n2d=12, n3d=4, seed=1, tri=(1000, 64000), tet=(50000, 500000)):
+    """Distinct meshes for the serving stream: n2d jittered triangle meshes and n3d Kuhn
+    tetrahedral meshes with cell counts log-uniform in the given ranges.  Returns
+    {key: raw mesh}; keys start with 'tri' or 'tet'."""
+    rng = np.random.default_rng(seed)
+    pool = {}
+    for k, t in enumerate(np.exp(rng.uniform(np.log(tri[0]), np.log(tri[1]), n2d))):
+        nx = max(2, int(round(np.sqrt(t / 2.0))))
+        pool[f"tri{k}_{2 * nx * nx}"] = meshgen.tri2d(nx, jitter=0.25, seed=seed * 7919 + k)
+    for k, t in enumerate(np.exp(rng.uniform(np.log(tet[0]), np.log(tet[1]), n3d))):
+        nx = max(2, int(round((t / 6.0) ** (1.0 / 3.0))))
+        pool[f"tet{k}_{6 * nx ** 3}"] = meshgen.tet3d(nx, jitter=0.2, seed=seed * 104729 + k)
+    return pool
+
+
+def serving_requests(pool, geometry_of, n, qps, frac_3d=0.3, frac_conv=0.3, seed=2, pe=(1.0, 50.0)):
+    """n requests with Poisson arrivals at rate qps over the pool's meshes: a 3D mesh with
+    probability frac_3d (else 2D), convection-diffusion (BiCGStab + ILU(0), Peclet U[pe])
+    with probability frac_conv, else diffusion (CG + IC(0)) with ln k ~ N(0, 1) per cell
+    and f ~ N(0,1).  geometry_of(key) returns the mesh's Mesh::compute geometry.
+    Returns a list of serving.Request."""
+    from serving import Request, poisson_arrivals
+    rng = np.random.default_rng(seed)
+    keys2 = [k for k in pool if k.startswith("tri")]
+    keys3 = [k for k in pool if k.startswith("tet")]
+    arrivals = poisson_arrivals(n, qps, seed + 1)
+    out = []
+    for i in range(n):
+        keys = keys3 if (keys3 and rng.uniform() < frac_3d) else keys2
+        key = keys[int(rng.integers(len(keys)))]
+        raw = pool[key]
+        geom = geometry_of(key)
+        nc = raw["cell_ptr"].shape[0] - 1
+        s = int(rng.integers(1 << 30))
+        if rng.uniform() < frac_conv:
+            f = convdiff_fields(raw, geom, pe=pe, seed=s)
+            kind = "bicgstab"
+        else:
+            kc = meshgen.lognormal_cells(nc, 1.0, s)
+            f = diffusion_fields(raw, geom, kc, np.random.default_rng(s + 1).standard_normal(nc), bc="zero")
+            kind = "cg"
+        out.append(Request(i, key, kind, f["prog"], f["face_fields"], f["cell_fields"], f["bc_i"], f["bc_e"],
+                           arrival=float(arrivals[i])))
     return out
 
 

@@ -252,6 +252,10 @@ int fvg_batch_size(const fvg_batch* batch, int32_t* n_systems, int64_t* total_ro
 int fvg_batch_launch(fvg_ctx* ctx, fvg_batch* batch, const fvg_vec* const* b, fvg_vec* const* x);
 /* 1 in *done once a launched batch has finished (does not block). */
 int fvg_batch_query(fvg_ctx* ctx, fvg_batch* batch, int32_t* done);
+/* Completion of each system of a launched batch, without waiting: done[k] = 1 once system
+ * k's solution is in its x vector (readable from any stream) and, if infos is not NULL,
+ * infos[k] holds its result.  Systems finish independently of the rest of the batch. */
+int fvg_batch_poll(fvg_ctx* ctx, fvg_batch* batch, int32_t* done, fvg_solver_info* infos);
 /* Waits for the launched batch; infos[k] and histories[k] (each may be NULL) per system. */
 int fvg_batch_finish(fvg_ctx* ctx, fvg_batch* batch, fvg_solver_info* infos, double* const* histories,
                      int64_t history_capacity);

@@ -1,0 +1,102 @@
+"""Serving loop on the GPU: continuous batching over a mixed stream.
+
+Bars: every answer reaches a true relative residual <= 1e-8 (mixed precision, FP64
+refinement) and is within 1e-6 relative of the FP64 oracle's solution; each request gives
+the same bits alone and in the mix; the HBM budget is never exceeded.
+"""
+import copy
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import meshgen
+import oracle as O
+import workloads
+from serving import Server, latency_summary
+
+pytestmark = pytest.mark.gpu
+
+
+def _pool():
+    return {"tri0_800": meshgen.tri2d(20, seed=11), "tri1_2450": meshgen.tri2d(35, seed=12),
+            "tri2_288": meshgen.tri2d(12, seed=13), "tet0_1296": meshgen.tet3d(6, seed=14)}
+
+
+def _oracle_system(raw, r):
+    om = O.OracleMesh(raw)
+    return om.assemble(r.prog, r.face_fields, r.cell_fields, r.bc_i, r.bc_e)
+
+
+def _server(budget, lanes, max_batch):
+    pool = _pool()
+    srv = Server(budget_bytes=budget, lanes=lanes, max_batch=max_batch)
+    for k, raw in pool.items():
+        srv.register_mesh(k, raw)
+    return srv, pool
+
+
+def _requests(srv, pool, n, seed):
+    return workloads.serving_requests(pool, lambda k: srv.meshes[k].mesh.geometry(), n, qps=3000.0, frac_3d=0.25,
+                                      frac_conv=0.4, seed=seed)
+
+
+def _serve(budget, lanes, max_batch, n, seed):
+    srv, pool = _server(budget, lanes, max_batch)
+    reqs = _requests(srv, pool, n, seed)
+    wall = srv.run(reqs)
+    return srv, pool, reqs, wall
+
+
+def test_serving_answers_are_accurate_and_the_same_alone_and_in_the_mix():
+    srv, pool, reqs, wall = _serve(2 * 10**9, lanes=2, max_batch=6, n=18, seed=5)
+    try:
+        assert all(r.done >= r.arrival for r in reqs)
+        assert max(r.batch_size for r in reqs) > 1          # requests really were batched together
+        for r in reqs:
+            rp, ci, va, b = _oracle_system(pool[r.mesh], r)
+            n = rp.shape[0] - 1
+            S = sp.csr_matrix((va, ci, rp), shape=(n, n))
+            rel = np.linalg.norm(b - S @ r.x) / np.linalg.norm(b)
+            assert rel <= 1e-8, (r.rid, rel)
+        for r in reqs[:6]:
+            rp, ci, va, b = _oracle_system(pool[r.mesh], r)
+            P = O.OraclePrec("ic" if r.kind == "cg" else "ilu", rp, ci, va, 0)
+            xo, _ = O.solve(r.kind, rp, ci, va, P, b, rtol=1e-8, atol=0.0, max_iter=5000)
+            assert np.linalg.norm(r.x - xo) <= 1e-6 * np.linalg.norm(xo)
+        # the same requests one at a time
+        for r in reqs:
+            alone = copy.copy(r)
+            alone.arrival, alone.done, alone.x = 0.0, -1.0, None
+            srv.run([alone])
+            assert alone.batch_size == 1
+            assert np.array_equal(alone.x, r.x), r.rid
+        s = latency_summary(reqs, wall)
+        assert s["count"] == len(reqs) and s["p99_ms"] >= s["p50_ms"] > 0.0
+    finally:
+        srv.close()
+
+
+def test_serving_stays_within_the_hbm_budget():
+    srv, pool = _server(2 * 10**9, lanes=2, max_batch=4)
+    try:
+        # learn every slot's footprint, then leave room for about two slots only:
+        # slots are evicted and rebuilt, and nothing exceeds the budget
+        srv.run(_requests(srv, pool, 24, seed=3))
+        biggest = max(e.slot_bytes for e in srv.meshes.values() if e.slot_bytes)
+        srv.drop_idle_slots()
+        srv.budget = srv.ctx.bytes_in_use() + int(2.2 * biggest)
+        srv.peak_bytes = 0
+        reqs = _requests(srv, pool, 14, seed=9)
+        srv.run(reqs)
+        assert all(r.done >= 0 for r in reqs)
+        assert srv.peak_bytes <= srv.budget
+        assert srv.ctx.bytes_in_use() <= srv.budget
+        assert srv.evictions > 0
+        for r in reqs:
+            rp, ci, va, b = _oracle_system(pool[r.mesh], r)
+            n = rp.shape[0] - 1
+            S = sp.csr_matrix((va, ci, rp), shape=(n, n))
+            assert np.linalg.norm(b - S @ r.x) / np.linalg.norm(b) <= 1e-8
+    finally:
+        srv.close()
