@@ -24,6 +24,9 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# the serving leg keeps several streams busy at once (lanes, assembly, result copies):
+# give each its own hardware queue (read when the HIP runtime starts)
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 sys.path.insert(0, os.path.join(ROOT, "finite-volumes-rs_amd"))
 
 HBM_PEAK_GBS = 8000.0
@@ -45,7 +48,150 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nx", type=int, default=64, help="hexes per direction (64 = the benchmark workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--legs", default="batched,serving,convdiff",
+                    help="extra legs beside the headline solve: batched, serving, convdiff (comma list, or none)")
+    ap.add_argument("--batch-systems", type=int, default=256)
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-qps", type=float, default=150.0)
     return ap.parse_args()
+
+
+def leg_batched(args, fvg, workloads):
+    """256 independent 2D diffusion systems (cells log-uniform in [1k, 64k]) solved by one
+    graph-launched batch: FP64 and mixed precision, workgroup size autotuned."""
+    inst = workloads.batched_2d(args.batch_systems)
+    ctx = fvg.Context(0)
+    A, M, B, X, ns = [], [], [], [], []
+    for raw, fields in inst:
+        mesh = fvg.Mesh(ctx, raw)
+        pr = fields(mesh.geometry())
+        lhs, rhs = fvg.assemble_program(mesh, pr["prog"], pr["face_fields"], pr["cell_fields"], pr["bc_i"],
+                                        pr["bc_e"])
+        A.append(lhs)
+        B.append(rhs)
+        M.append(fvg.IncompleteCholesky.from_matrix(lhs, 0))
+        X.append(fvg.DistributedVector(ctx, mesh.n_cells))
+        ns.append(mesh.n_cells)
+    ctx.sync()
+    res = {}
+    xs = {}
+    for name, prec in (("fp64", fvg.PRECISION_FP64), ("mixed", fvg.PRECISION_MIXED)):
+        opts = fvg.LinearSolverOptions(1e-8, 0.0, 5000, precision=prec)
+        bt = fvg.Batch(ctx, A, M, "cg", opts)
+        tune = bt.tune(B, X)
+        times = []
+        for _ in range(max(1, args.steps)):
+            for x in X:
+                x.fill(0.0)
+            ctx.sync()
+            t = time.perf_counter()
+            infos = bt.solve(B, X, history=False)
+            times.append(time.perf_counter() - t)
+        xs[name] = [x.numpy() for x in X]
+        worst = 0.0
+        for k in range(len(A)):   # true relative residual of every system, FP64 SpMV on the device
+            y = fvg.DistributedVector(ctx, ns[k])
+            A[k].imul(y, X[k])
+            bh = B[k].numpy()
+            worst = max(worst, float(np.linalg.norm(bh - y.numpy()) / np.linalg.norm(bh)))
+        it = np.array([i.iterations for i in infos])
+        best = min(times)
+        res[name] = {"solves_per_s": len(A) / best, "ms_per_batch": 1e3 * best,
+                     "threads_per_system": bt.threads, "autotune_ms": {str(k): 1e3 * v for k, v in tune.items()},
+                     "iterations_max": int(it.max()), "iterations_mean": float(it.mean()),
+                     "worst_true_relative_residual": worst}
+        bt.close()
+    err = max(float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(xs["mixed"], xs["fp64"]))
+    res["mixed"]["speedup_vs_fp64"] = res["mixed"]["solves_per_s"] / res["fp64"]["solves_per_s"]
+    res["mixed"]["max_rel_error_vs_fp64_solution"] = err
+    res["systems"] = len(A)
+    res["total_cells"] = int(sum(ns))
+    res["largest_system"] = int(max(ns))
+    res["workload"] = ("batched_2d: 256 jittered triangle meshes, cells log-uniform in [1k, 64k], ln k ~ N(0,1) "
+                       "per cell, f ~ N(0,1), u = 0; PCG + IC(0) to 1e-8 rel. residual, x0 = 0; one launch, "
+                       "one workgroup per system")
+    ctx.close()
+    return res
+
+
+def leg_serving(args, fvg, workloads):
+    """A Poisson stream of mixed-size solves through the continuous-batching server."""
+    import serving
+    pool = workloads.serving_pool()
+    srv = serving.Server(budget_bytes=200 * 10**9, lanes=4, max_batch=64)
+    for k, raw in pool.items():
+        srv.register_mesh(k, raw)
+    geo = {k: srv.meshes[k].mesh.geometry() for k in pool}
+    warm = workloads.serving_requests(pool, geo.get, 32, args.serve_qps, seed=11)
+    srv.run(warm)   # first use of every mesh builds its slots (setup, not serving)
+    reqs = workloads.serving_requests(pool, geo.get, args.serve_requests, args.serve_qps, seed=12)
+    wall = srv.run(reqs)
+    s = serving.latency_summary(reqs, wall)
+    by = {}
+    for name, sel in (("tri", lambda r: r.mesh.startswith("tri")), ("tet", lambda r: r.mesh.startswith("tet"))):
+        sub = [r for r in reqs if sel(r)]
+        if sub:
+            lat = np.array([r.latency for r in sub]) * 1e3
+            by[name] = {"count": len(sub), "p50_ms": float(np.percentile(lat, 50)),
+                        "p99_ms": float(np.percentile(lat, 99))}
+    s.update({"offered_qps": args.serve_qps, "requests": len(reqs), "by_mesh": by,
+              "mean_batch": float(np.mean([r.batch_size for r in reqs])),
+              "peak_hbm_bytes": int(srv.peak_bytes), "budget_bytes": srv.budget,
+              "worst_true_relative_residual": float(max(r.info.true_residual for r in reqs)),
+              "workload": "serving: Poisson arrivals; 70% 2D triangle meshes (1k-64k cells), 30% 3D Kuhn tet meshes "
+                          "(50k-500k cells) from a pool of 16; 70% diffusion (ln k ~ N(0,1), CG + IC(0)), 30% "
+                          "convection-diffusion (cell Peclet U[1,50], BiCGStab + ILU(0)); mixed precision to 1e-8; "
+                          "latency = scheduled arrival to solution on the host"})
+    srv.close()
+    return s
+
+
+def leg_convdiff(args, fvg, workloads):
+    """Nonsymmetric convection-diffusion on 128x128x2 triangles x 64 layers of prisms
+    (2,097,152 cells): assemble + ILU(0) + BiCGStab to 1e-8, FP64 and mixed."""
+    raw, fields_of = workloads.convdiff_prism()
+    ctx = fvg.Context(0)
+    mesh = fvg.Mesh(ctx, raw)
+    pr = fields_of(mesh.geometry())
+    asm = fvg.Assembler(mesh, pr["prog"], pr["face_fields"], pr["cell_fields"], pr["bc_i"], pr["bc_e"])
+    lhs = fvg.DistributedMatrix.from_mesh(ctx, mesh)
+    rhs = fvg.DistributedVector.from_size(ctx, mesh.n_cells)
+    x = fvg.DistributedVector.from_size(ctx, mesh.n_cells)
+    asm.run(lhs, rhs)
+    ilu = fvg.IncompleteLowerUpper.from_matrix(lhs, 0)
+    out = {"workload": "convdiff_prism: 128x128x2 jittered triangles x 64 layers = 2,097,152 prisms, velocity "
+                       "(1,0.5,0.25) scaled to cell Peclet U[1,100], k = 1e-2, upwind, inflow u = 1 / outflow zero "
+                       "gradient; assemble + ILU(0) + BiCGStab to 1e-8 rel. residual, x0 = 0",
+           "n_cells": mesh.n_cells, "nnz": mesh.nnz}
+    xs = {}
+    for name, prec in (("fp64", fvg.PRECISION_FP64), ("mixed", fvg.PRECISION_MIXED)):
+        opts = fvg.LinearSolverOptions(1e-8, 0.0, 4000, precision=prec)
+
+        def step():
+            asm.run(lhs, rhs, sync=False)
+            ilu.refactor(lhs)
+            x.fill(0.0)
+            return fvg.bi_conjugate_gradient_stab(x, lhs, rhs, ilu, opts)
+
+        step()
+        ctx.sync()
+        t = time.perf_counter()
+        steps = max(1, min(args.steps, 3))
+        for _ in range(steps):
+            info = step()
+        ctx.sync()
+        el = (time.perf_counter() - t) / steps
+        y = fvg.DistributedVector.from_size(ctx, mesh.n_cells)
+        lhs.imul(y, x)
+        bh = rhs.numpy()
+        xs[name] = x.numpy()
+        out[name] = {"solves_per_s": 1.0 / el, "ms_per_solve": 1e3 * el, "iterations": info.iterations,
+                     "true_relative_residual": float(np.linalg.norm(bh - y.numpy()) / np.linalg.norm(bh))}
+    out["mixed"]["speedup_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    out["mixed"]["rel_error_vs_fp64_solution"] = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) /
+                                                       np.linalg.norm(xs["fp64"]))
+    ctx.close()
+    return out
 
 
 def cpu_baseline(raw, fields_of):
@@ -182,6 +328,11 @@ def main():
                           "total_ms": ks["sweep_ms"], "levels": [lo_lv, up_lv]},
         },
     }
+    legs = [l for l in args.legs.split(",") if l and l != "none"]
+    for name, fn in (("batched", leg_batched), ("serving", leg_serving), ("convdiff", leg_convdiff)):
+        if name in legs:
+            print(f"bench: {name} leg", file=sys.stderr, flush=True)
+            out[name] = fn(args, fvg, workloads)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(raw, fields_of)
     print(json.dumps(out), flush=True)

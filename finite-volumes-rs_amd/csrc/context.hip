@@ -1,5 +1,6 @@
 // context.hip -- context, device memory accounting, vectors, shared vector kernels.
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 
@@ -214,6 +215,10 @@ int fvg_ctx_destroy(fvg_ctx* c) {
     dev_free_n(c, c->partials, c->partials_cap * 4);
     dev_free_n(c, c->scalar, 64);
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->stage) (void)hipHostFree(c->stage);
+    if (c->jobs) dev_free(c, c->jobs, c->jobs_cap);
+    if (c->jobs_host) (void)hipHostFree(c->jobs_host);
+    if (c->jobs_evt) (void)hipEventDestroy(c->jobs_evt);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return FVG_OK;
@@ -332,16 +337,83 @@ int fvg_vec_device_ptr(const fvg_vec* v, double** p) {
     return FVG_OK;
 }
 
+// Host transfers go through a pinned staging buffer, so the copy engines move the data
+// (a copy from or to pageable memory may otherwise need compute units, which long-running
+// kernels can hold).
+static constexpr size_t kStageMax = size_t(32) << 20;
+
+static int stage_reserve(fvg_ctx* c, size_t bytes) {
+    bytes = std::min(bytes, kStageMax);
+    if (c->stage_cap >= bytes) return FVG_OK;
+    if (c->stage) {
+        FVG_HIP(hipStreamSynchronize(c->stream));
+        (void)hipHostFree(c->stage);
+        c->stage = nullptr;
+        c->stage_cap = 0;
+    }
+    size_t cap = size_t(1) << 16;
+    while (cap < bytes) cap <<= 1;
+    FVG_HIP(hipHostMalloc(&c->stage, cap, hipHostMallocDefault));
+    c->stage_cap = cap;
+    return FVG_OK;
+}
+
 int fvg_vec_upload(fvg_ctx* c, fvg_vec* v, const double* host) {
     if (!c || !v || (!host && v->n)) return fail(FVG_E_INVALID, "null argument");
-    FVG_TRY(upload(c, v->d, host, v->n));
+    const size_t total = sizeof(double) * (size_t)v->n;
+    if (total == 0) return FVG_OK;
+    FVG_TRY(stage_reserve(c, total));
+    const size_t chunk = c->stage_cap;
+    for (size_t off = 0; off < total; off += chunk) {
+        const size_t len = std::min(chunk, total - off);
+        FVG_HIP(hipStreamSynchronize(c->stream));   // the previous chunk has left the buffer
+        std::memcpy(c->stage, reinterpret_cast<const char*>(host) + off, len);
+        FVG_HIP(hipMemcpyAsync(reinterpret_cast<char*>(v->d) + off, c->stage, len, hipMemcpyHostToDevice, c->stream));
+    }
     FVG_HIP(hipStreamSynchronize(c->stream));
     return FVG_OK;
 }
 
+int fvg_vec_upload_many(fvg_ctx* c, int32_t count, fvg_vec* const* vecs, const double* const* hosts) {
+    if (!c || count < 0 || (count && (!vecs || !hosts))) return fail(FVG_E_INVALID, "null argument");
+    size_t total = 0;
+    for (int i = 0; i < count; ++i) {
+        if (!vecs[i] || (!hosts[i] && vecs[i]->n)) return fail(FVG_E_INVALID, "null vector or data");
+        total += sizeof(double) * (size_t)vecs[i]->n;
+    }
+    if (total == 0) return FVG_OK;
+    if (total > kStageMax) {   // too large to stage at once: one vector at a time
+        for (int i = 0; i < count; ++i) FVG_TRY(fvg_vec_upload(c, vecs[i], hosts[i]));
+        return FVG_OK;
+    }
+    FVG_TRY(stage_reserve(c, total));
+    FVG_HIP(hipStreamSynchronize(c->stream));   // earlier copies have left the staging buffer
+    size_t off = 0;
+    for (int i = 0; i < count; ++i) {
+        const size_t len = sizeof(double) * (size_t)vecs[i]->n;
+        if (len == 0) continue;
+        std::memcpy(reinterpret_cast<char*>(c->stage) + off, hosts[i], len);
+        FVG_HIP(hipMemcpyAsync(vecs[i]->d, reinterpret_cast<char*>(c->stage) + off, len, hipMemcpyHostToDevice,
+                               c->stream));
+        off += len;
+    }
+    return FVG_OK;   // the copies complete in stream order; the staging buffer is reused only after a sync
+}
+
 int fvg_vec_download(fvg_ctx* c, const fvg_vec* v, double* host) {
     if (!c || !v || (!host && v->n)) return fail(FVG_E_INVALID, "null argument");
-    return download(c, host, v->d, v->n);
+    const size_t total = sizeof(double) * (size_t)v->n;
+    if (total == 0) return FVG_OK;
+    FVG_TRY(stage_reserve(c, total));
+    const size_t chunk = c->stage_cap;
+    for (size_t off = 0; off < total; off += chunk) {
+        const size_t len = std::min(chunk, total - off);
+        FVG_HIP(hipMemcpyAsync(c->stage, reinterpret_cast<const char*>(v->d) + off, len, hipMemcpyDeviceToHost,
+                               c->stream));
+        FVG_HIP(hipStreamSynchronize(c->stream));
+        std::memcpy(reinterpret_cast<char*>(host) + off, c->stage, len);
+    }
+    return FVG_OK;
 }
 
 int fvg_vec_fill(fvg_ctx* c, fvg_vec* v, double value) {

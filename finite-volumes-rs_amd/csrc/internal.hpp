@@ -48,6 +48,12 @@ struct fvg_ctx {
     int64_t partials_cap = 0;
     double* scalar = nullptr;          // small device scalar area
     void* pinned = nullptr;            // pinned host staging (4 KiB)
+    void* stage = nullptr;             // pinned host staging for vector transfers (grown on demand)
+    size_t stage_cap = 0;
+    void* jobs = nullptr;              // device table of multi-system launches (grown on demand)
+    void* jobs_host = nullptr;         // its pinned host twin
+    size_t jobs_cap = 0;
+    hipEvent_t jobs_evt = nullptr;     // recorded after the last copy out of the twin
     bool timing = false;
     struct Span { hipEvent_t a, b; int kind; int64_t launches; };
     std::vector<Span> spans;           // event pairs recorded while timing is enabled

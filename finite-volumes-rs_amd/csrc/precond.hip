@@ -35,14 +35,10 @@ __global__ void k_sym_check(int64_t nnz, const int32_t* __restrict__ tpos, const
     if (!(fabs(a[k] - a[t]) < 1e-14)) atomicAdd(&flags[2], 1);   // ApproximateCmp for f64, linalg/mod.rs:73-77
 }
 
-__global__ void k_ic0_level(int64_t begin, int64_t end, const int32_t* __restrict__ rows,
-                            const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                            const int32_t* __restrict__ diag, const double* __restrict__ a, double* __restrict__ l,
-                            double* __restrict__ inv_diag, int32_t* __restrict__ flags) {
-    const int64_t p = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= end) return;
-    const int i = rows[p];
-    if (i < 0) return;
+// row i of IC(0): strictly lower entries, then the pivot (left-looking)
+__device__ __forceinline__ void ic0_row(int i, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                        const int32_t* __restrict__ diag, const double* __restrict__ a, double* l,
+                                        double* inv_diag, int32_t* flags) {
     const int s = rp[i], d = diag[i];
     for (int k = s; k < d; ++k) {
         const int j = col[k];
@@ -71,14 +67,10 @@ __global__ void k_ic0_level(int64_t begin, int64_t end, const int32_t* __restric
     inv_diag[i] = 1.0 / lkk;
 }
 
-__global__ void k_ilu0_level(int64_t begin, int64_t end, const int32_t* __restrict__ rows,
-                             const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                             const int32_t* __restrict__ diag, const double* __restrict__ a, double* __restrict__ lu,
-                             int32_t* __restrict__ flags) {
-    const int64_t p = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= end) return;
-    const int i = rows[p];
-    if (i < 0) return;
+// row i of ILU(0): IKJ elimination against the rows above, then the drop rule
+__device__ __forceinline__ void ilu0_row(int i, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                         const int32_t* __restrict__ diag, const double* __restrict__ a, double* lu,
+                                         int32_t* flags) {
     const int s = rp[i], e = rp[i + 1], d = diag[i];
     for (int k = s; k < e; ++k) lu[k] = a[k];
     for (int k = s; k < d; ++k) {
@@ -106,6 +98,79 @@ __global__ void k_ilu0_level(int64_t begin, int64_t end, const int32_t* __restri
     const double tol = fabs(lu[d]) * 1e-6;
     for (int k = s; k < e; ++k)
         if (k != d && !(fabs(lu[k]) > tol)) lu[k] = 0.0;
+}
+
+__global__ void k_ic0_level(int64_t begin, int64_t end, const int32_t* __restrict__ rows,
+                            const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                            const int32_t* __restrict__ diag, const double* __restrict__ a, double* __restrict__ l,
+                            double* __restrict__ inv_diag, int32_t* __restrict__ flags) {
+    const int64_t p = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= end) return;
+    const int i = rows[p];
+    if (i < 0) return;
+    ic0_row(i, rp, col, diag, a, l, inv_diag, flags);
+}
+
+__global__ void k_ilu0_level(int64_t begin, int64_t end, const int32_t* __restrict__ rows,
+                             const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                             const int32_t* __restrict__ diag, const double* __restrict__ a, double* __restrict__ lu,
+                             int32_t* __restrict__ flags) {
+    const int64_t p = begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= end) return;
+    const int i = rows[p];
+    if (i < 0) return;
+    ilu0_row(i, rp, col, diag, a, lu, flags);
+}
+
+// One workgroup refactorises one system of a group, level by level (a workgroup barrier
+// between levels instead of a launch per level), then makes the inverse pivots (ILU)
+// and gathers the sweep values: the same per-row arithmetic as the per-level launches.
+struct FactorJob {
+    int32_t kind, nlevels;
+    const int64_t* lvl_ptr;   // positions of each lower level in rows
+    const int32_t* rows;
+    const int32_t* rp;
+    const int32_t* col;
+    const int32_t* diag;
+    const int32_t* tpos;
+    const double* a;
+    double* fac;
+    double* inv_diag;
+    int32_t* flags;
+    const int32_t* lo_src;
+    double* lo_val;
+    const int32_t* up_src;
+    double* up_val;
+    int64_t n, nnz, lo_nnz, up_nnz;
+};
+
+constexpr int kFactorBlock = 512;
+
+__global__ void __launch_bounds__(kFactorBlock) k_factor_team(const FactorJob* __restrict__ jobs) {
+    const FactorJob J = jobs[blockIdx.x];
+    if (J.kind == kIC) {
+        for (int64_t k = threadIdx.x; k < J.nnz; k += kFactorBlock) {   // symmetry (linalg/mod.rs:73-77)
+            const int32_t t = J.tpos[k];
+            if (t >= 0 && !(fabs(J.a[k] - J.a[t]) < 1e-14)) atomicAdd(&J.flags[2], 1);
+        }
+    }
+    for (int l = 0; l < J.nlevels; ++l) {
+        const int64_t b = J.lvl_ptr[l], e = J.lvl_ptr[l + 1];
+        for (int64_t p = b + threadIdx.x; p < e; p += kFactorBlock) {
+            const int i = J.rows[p];
+            if (i < 0) continue;
+            if (J.kind == kIC)
+                ic0_row(i, J.rp, J.col, J.diag, J.a, J.fac, J.inv_diag, J.flags);
+            else
+                ilu0_row(i, J.rp, J.col, J.diag, J.a, J.fac, J.flags);
+        }
+        __syncthreads();
+    }
+    if (J.kind == kILU) {
+        for (int64_t i = threadIdx.x; i < J.n; i += kFactorBlock) J.inv_diag[i] = 1.0 / J.fac[J.diag[i]];
+    }
+    for (int64_t k = threadIdx.x; k < J.lo_nnz; k += kFactorBlock) J.lo_val[k] = J.fac[J.lo_src[k]];
+    for (int64_t k = threadIdx.x; k < J.up_nnz; k += kFactorBlock) J.up_val[k] = J.fac[J.up_src[k]];
 }
 
 __global__ void k_inv_diag(int64_t n, const int32_t* __restrict__ diag, const double* __restrict__ v,
@@ -297,6 +362,7 @@ __global__ void k_scale(int64_t n, const double* __restrict__ d, const double* _
 // ---- host analysis ----------------------------------------------------------------------
 
 static void free_sweep(fvg_ctx* c, Sweep& s) {
+    dev_free_n(c, s.d_lvl_ptr, s.nlevels + 1);
     dev_free_n(c, s.val32, s.nnz);
     dev_free_n(c, s.rows, s.npos);
     dev_free_n(c, s.ptr, s.npos + 1);
@@ -359,6 +425,9 @@ static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::ve
     }
     sw.nnz = (int64_t)ecol.size();
     int s;
+    if ((s = dev_alloc_n(c, sw.nlevels + 1, &sw.d_lvl_ptr)) || (s = upload(c, sw.d_lvl_ptr, sw.lvl_ptr.data(),
+                                                                           sw.nlevels + 1)))
+        return s;
     if ((s = dev_alloc_n(c, sw.npos, &sw.rows)) || (s = dev_alloc_n(c, sw.npos + 1, &sw.ptr)) ||
         (s = dev_alloc_n(c, sw.nnz, &sw.col)) || (s = dev_alloc_n(c, sw.nnz, &sw.src)) ||
         (s = dev_alloc_n(c, sw.nnz, &sw.val)) || (s = upload(c, sw.rows, rows.data(), sw.npos)) ||
@@ -388,6 +457,7 @@ static void free_prec(fvg_ctx* c, fvg_prec* p) {
 static int setup_pattern(fvg_ctx* c, const fvg_csr* a, fvg_prec* p, bool need_transpose) {
     const int64_t n = a->nrows;
     p->n = n;
+    p->pattern_id = a->pattern_id;
     p->nnz = a->nnz;
     p->h_rp = a->h_row_ptr;
     p->h_col = a->h_col;
@@ -434,16 +504,20 @@ static int read_flags(fvg_ctx* c, const fvg_prec* p, int32_t* host4) {
     return FVG_OK;
 }
 
-// numeric factorisation for the current values of a (pattern already analysed)
-static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a) {
+// numeric factorisation for the current values of a (pattern already analysed).
+// check: read the symmetry / pivot flags and fail on them (synchronises); otherwise the
+// whole factorisation is only enqueued and the flags are read by fvg_prec_status.
+static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a, bool check = true) {
     FVG_HIP(hipMemsetAsync(p->d_flags, 0, 4 * sizeof(int32_t), c->stream));
     const Sweep& lo = p->lower;
     if (p->kind == kIC) {
         hipLaunchKernelGGL(k_sym_check, dim3(grid_for(p->nnz)), dim3(kBlock), 0, c->stream, p->nnz, p->d_tpos, a->val,
                            p->d_flags);
-        int32_t fl[4];
-        FVG_TRY(read_flags(c, p, fl));
-        if (fl[2]) return fail(FVG_E_NOT_SYMMETRIC, "IncompleteCholesky requires a symmetric matrix");
+        if (check) {
+            int32_t fl[4];
+            FVG_TRY(read_flags(c, p, fl));
+            if (fl[2]) return fail(FVG_E_NOT_SYMMETRIC, "IncompleteCholesky requires a symmetric matrix");
+        }
         for (int l = 0; l < lo.nlevels; ++l) {
             const int64_t b = lo.lvl_ptr[(size_t)l], e = lo.lvl_ptr[(size_t)l + 1];
             hipLaunchKernelGGL(k_ic0_level, dim3(grid_for(e - b)), dim3(kBlock), 0, c->stream, b, e, lo.rows, p->d_rp,
@@ -459,7 +533,7 @@ static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a) {
                            p->d_inv_diag);
     }
     FVG_HIP(hipGetLastError());
-    if (p->kind == kILU) {
+    if (p->kind == kILU && check) {
         int32_t fl[4];
         FVG_TRY(read_flags(c, p, fl));
         if (fl[0]) return fail(FVG_E_ZERO_PIVOT, "error, lu[k, k] too small");
@@ -469,7 +543,7 @@ static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a) {
     hipLaunchKernelGGL(k_gather, dim3(grid_for(p->upper.nnz)), dim3(kBlock), 0, c->stream, p->upper.nnz, p->upper.src,
                        p->d_fac, p->upper.val);
     FVG_HIP(hipGetLastError());
-    FVG_HIP(hipStreamSynchronize(c->stream));
+    if (check) FVG_HIP(hipStreamSynchronize(c->stream));
     p->fac_version += 1;
     return FVG_OK;
 }
@@ -687,9 +761,11 @@ int fvg_ilu_create(fvg_ctx* c, const fvg_csr* a, int32_t level, fvg_prec** out) 
     return create_factored(c, a, level, kILU, out);
 }
 
-int fvg_prec_refactor(fvg_ctx* c, fvg_prec* p, const fvg_csr* a) {
+static int refactor(fvg_ctx* c, fvg_prec* p, const fvg_csr* a, bool check) {
     if (!c || !p || !a) return fail(FVG_E_INVALID, "null argument");
-    if (a->nrows != p->n || a->nnz != p->nnz || a->h_col != p->h_col)
+    // the same mesh pattern is recognised by its id; other matrices are compared entry by entry
+    const bool same = a->pattern_id != 0 && a->pattern_id == p->pattern_id && a->nrows == p->n && a->nnz == p->nnz;
+    if (!same && (a->nrows != p->n || a->nnz != p->nnz || a->h_col != p->h_col))
         return fail(FVG_E_INVALID, "matrix pattern differs from the analysed one");
     if (p->kind == kIdentity) return FVG_OK;
     if (p->kind == kDiagonal) {
@@ -699,7 +775,88 @@ int fvg_prec_refactor(fvg_ctx* c, fvg_prec* p, const fvg_csr* a) {
         p->fac_version += 1;
         return FVG_OK;
     }
-    return factorize(c, p, a);
+    return factorize(c, p, a, check);
+}
+
+int fvg_prec_refactor(fvg_ctx* c, fvg_prec* p, const fvg_csr* a) { return refactor(c, p, a, true); }
+
+int fvg_prec_refactor_async(fvg_ctx* c, fvg_prec* p, const fvg_csr* a) { return refactor(c, p, a, false); }
+
+int fvg_prec_refactor_many(fvg_ctx* c, int32_t count, fvg_prec* const* ps, const fvg_csr* const* as) {
+    if (!c || count < 0 || (count && (!ps || !as))) return fail(FVG_E_INVALID, "null argument");
+    std::vector<FactorJob> jobs;
+    for (int k = 0; k < count; ++k) {
+        fvg_prec* p = ps[k];
+        const fvg_csr* a = as[k];
+        if (!p || !a) return fail(FVG_E_INVALID, "null preconditioner or matrix");
+        if (p->kind != kIC && p->kind != kILU) {   // Jacobi / identity: nothing to batch
+            FVG_TRY(refactor(c, p, a, false));
+            continue;
+        }
+        const bool same = a->pattern_id != 0 && a->pattern_id == p->pattern_id && a->nrows == p->n && a->nnz == p->nnz;
+        if (!same && (a->nrows != p->n || a->nnz != p->nnz || a->h_col != p->h_col))
+            return fail(FVG_E_INVALID, "matrix pattern differs from the analysed one");
+        FactorJob J{};
+        J.kind = p->kind;
+        J.nlevels = p->lower.nlevels;
+        J.lvl_ptr = p->lower.d_lvl_ptr;
+        J.rows = p->lower.rows;
+        J.rp = p->d_rp;
+        J.col = p->d_col;
+        J.diag = p->d_diag;
+        J.tpos = p->d_tpos;
+        J.a = a->val;
+        J.fac = p->d_fac;
+        J.inv_diag = p->d_inv_diag;
+        J.flags = p->d_flags;
+        J.lo_src = p->lower.src;
+        J.lo_val = p->lower.val;
+        J.up_src = p->upper.src;
+        J.up_val = p->upper.val;
+        J.n = p->n;
+        J.nnz = p->nnz;
+        J.lo_nnz = p->lower.nnz;
+        J.up_nnz = p->upper.nnz;
+        jobs.push_back(J);
+        FVG_HIP(hipMemsetAsync(p->d_flags, 0, 4 * sizeof(int32_t), c->stream));
+        p->fac_version += 1;
+    }
+    if (jobs.empty()) return FVG_OK;
+    // job table: a grow-only device buffer of the context, copied from a pinned host twin;
+    // the twin is rewritten only after the previous copy out of it has run
+    const size_t bytes = sizeof(FactorJob) * jobs.size();
+    if (c->jobs_cap < bytes) {
+        FVG_HIP(hipStreamSynchronize(c->stream));
+        if (c->jobs) dev_free(c, c->jobs, c->jobs_cap);
+        if (c->jobs_host) (void)hipHostFree(c->jobs_host);
+        c->jobs = nullptr;
+        c->jobs_host = nullptr;
+        c->jobs_cap = 0;
+        size_t cap = 4096;
+        while (cap < bytes) cap <<= 1;
+        FVG_TRY(dev_alloc(c, cap, &c->jobs));
+        FVG_HIP(hipHostMalloc(&c->jobs_host, cap, hipHostMallocDefault));
+        c->jobs_cap = cap;
+    }
+    if (!c->jobs_evt) FVG_HIP(hipEventCreateWithFlags(&c->jobs_evt, hipEventDisableTiming));
+    else FVG_HIP(hipEventSynchronize(c->jobs_evt));
+    std::memcpy(c->jobs_host, jobs.data(), bytes);
+    FVG_HIP(hipMemcpyAsync(c->jobs, c->jobs_host, bytes, hipMemcpyHostToDevice, c->stream));
+    FVG_HIP(hipEventRecord(c->jobs_evt, c->stream));
+    hipLaunchKernelGGL(k_factor_team, dim3((unsigned)jobs.size()), dim3(kFactorBlock), 0, c->stream,
+                       static_cast<const FactorJob*>(c->jobs));
+    FVG_HIP(hipGetLastError());
+    return FVG_OK;
+}
+
+int fvg_prec_status(fvg_ctx* c, const fvg_prec* p) {
+    if (!c || !p) return fail(FVG_E_INVALID, "null argument");
+    if (!p->d_flags || p->kind == kDiagonal || p->kind == kIdentity) return FVG_OK;
+    int32_t fl[4];
+    FVG_TRY(read_flags(c, p, fl));
+    if (p->kind == kIC && fl[2]) return fail(FVG_E_NOT_SYMMETRIC, "IncompleteCholesky requires a symmetric matrix");
+    if (p->kind == kILU && fl[0]) return fail(FVG_E_ZERO_PIVOT, "error, lu[k, k] too small");
+    return FVG_OK;
 }
 
 int fvg_prec_destroy(fvg_ctx* c, fvg_prec* p) {

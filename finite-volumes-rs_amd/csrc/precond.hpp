@@ -14,6 +14,7 @@ struct Sweep {
     int64_t n = 0, npos = 0, nnz = 0;
     int nlevels = 0;
     std::vector<int64_t> lvl_ptr;       // host: level l holds positions [lvl_ptr[l], lvl_ptr[l+1])
+    int64_t* d_lvl_ptr = nullptr;       // device copy
     int32_t* rows = nullptr;            // row of each position, -1 for padding (npos)
     int32_t* ptr = nullptr;             // entry range of each position (npos+1)
     int32_t* col = nullptr;             // column of each entry, ascending within a row
@@ -46,6 +47,7 @@ struct fvg_prec {
     float* d_tmp32 = nullptr;
     uint64_t fac_version = 0, fac32_version = 0;
     void* team = nullptr;               // batched-solve plan of this system (team.hip), made on use
+    uint64_t pattern_id = 0;            // mesh pattern the analysis was made for (0: not a mesh pattern)
 };
 
 namespace fvg {

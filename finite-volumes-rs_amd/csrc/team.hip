@@ -86,12 +86,13 @@ struct TeamDesc {
     int64_t max_iter;
     int64_t* prof;           // optional: wall-clock ticks spent in SpMV, lower sweep, upper sweep, total
     int32_t* done;           // host-visible completion flag of this system (pinned host memory)
-    int64_t pad2;
+    const int32_t* pflags;   // the preconditioner's factorisation flags (IC / ILU), checked before solving
 };
 
 // ---- optional phase timing (thread 0's view of its workgroup) ----------------------------
 __shared__ long long g_prof[4];
 __shared__ int g_prof_on;
+__shared__ int g_status;
 
 struct ProfScope {
     int ph;
@@ -588,6 +589,8 @@ __device__ void team_info(const TeamDesc& D, int64_t iters, double init, double 
     I.restarts = restarts;
     I.outer_iterations = outer;
     I.true_residual = true_res;
+    I.status = FVG_OK;
+    I.reserved = 0;
     *D.info = I;
 }
 
@@ -958,7 +961,8 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
 }
 
 template <int NT, int VAR>
-__global__ void __launch_bounds__(NT) k_team(const TeamDesc* __restrict__ descs, int n_inst) {
+__global__ void __launch_bounds__(NT) k_team(const TeamDesc* __restrict__ descs, const int32_t* __restrict__ count) {
+    const int n_inst = *count;   // systems of this launch (the grid is the batch's capacity)
     using T = typename std::conditional<VAR == kCG32 || VAR == kBI32, float, double>::type;
     constexpr int RING = sizeof(T) == 8 ? kRing64 : kRing32;
     __shared__ T ring[RING];
@@ -982,10 +986,27 @@ __global__ void __launch_bounds__(NT) k_team(const TeamDesc* __restrict__ descs,
             for (int j = 0; j < 4; ++j) g_prof[j] = 0;
             t0 = wall_clock64();
         }
-        if constexpr (VAR == kCG64) team_cg64<NT>(D, R, ring);
-        if constexpr (VAR == kBI64) team_bi64<NT>(D, R, ring);
-        if constexpr (VAR == kCG32) team_mixed<NT>(D, R, F, false, ring);
-        if constexpr (VAR == kBI32) team_mixed<NT>(D, R, F, true, ring);
+        // a factorisation that failed its checks (made without waiting for them) is reported,
+        // not used: IC of an unsymmetric matrix, ILU with a zero pivot
+        if (threadIdx.x == 0) {
+            int st = FVG_OK;
+            if (D.pflags && D.kind == kIC && D.pflags[2]) st = FVG_E_NOT_SYMMETRIC;
+            if (D.pflags && D.kind == kILU && D.pflags[0]) st = FVG_E_ZERO_PIVOT;
+            g_status = st;
+        }
+        __syncthreads();
+        if (g_status != FVG_OK) {
+            if (threadIdx.x == 0) {
+                fvg_solver_info I{};
+                I.status = g_status;
+                *D.info = I;
+            }
+        } else {
+            if constexpr (VAR == kCG64) team_cg64<NT>(D, R, ring);
+            if constexpr (VAR == kBI64) team_bi64<NT>(D, R, ring);
+            if constexpr (VAR == kCG32) team_mixed<NT>(D, R, F, false, ring);
+            if constexpr (VAR == kBI32) team_mixed<NT>(D, R, F, true, ring);
+        }
         if (threadIdx.x == 0 && g_prof_on) {
             g_prof[3] = wall_clock64() - t0;
             for (int j = 0; j < 4; ++j) D.prof[j] = g_prof[j];
@@ -1065,6 +1086,7 @@ struct TeamPlan {
 
 struct fvg_batch {
     int n_inst = 0;
+    int capacity = 0;                 // systems the allocations and the captured launch hold
     int solver = 0;      // 0 CG, 1 BiCGStab
     int precision = 0;
     int variant = 0;
@@ -1079,13 +1101,16 @@ struct fvg_batch {
     fvg_solver_info* h_info = nullptr;  // pinned host memory the kernel writes each system's info to
     int32_t* h_done = nullptr;          // pinned host completion flags
     int64_t* d_prof = nullptr;        // 4 counters per system when phase timing is on
-    fvg::TeamDesc* d_desc = nullptr;
+    fvg::TeamDesc* d_desc = nullptr;  // capacity descriptors followed by the system count
+    int32_t* d_count = nullptr;
     std::vector<fvg::TeamDesc> desc;  // by launch slot
+    fvg::TeamDesc* h_desc = nullptr;  // pinned copy the uploads read from (capacity + 1: the count)
+    bool desc_dirty = true;
     hipStream_t stream = nullptr;
     hipEvent_t ready = nullptr;       // recorded on the context stream before each launch
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int graph_nt = 0;
+    int graph_nt = 0, graph_cap = 0;
     bool launched = false;
 };
 
@@ -1100,8 +1125,13 @@ static void batch_free(fvg_ctx* c, fvg_batch* B) {
     dev_free_n(c, B->hist, B->nhist);
     if (B->h_info) (void)hipHostFree(B->h_info);
     if (B->h_done) (void)hipHostFree(B->h_done);
-    dev_free_n(c, B->d_prof, 4 * B->n_inst);
-    dev_free_n(c, B->d_desc, B->n_inst);
+    if (B->h_desc) (void)hipHostFree(B->h_desc);
+    B->h_info = nullptr;
+    B->h_done = nullptr;
+    B->h_desc = nullptr;
+    dev_free_n(c, B->d_prof, 4 * B->capacity);
+    dev_free_n(c, B->d_desc, B->capacity);
+    dev_free_n(c, B->d_count, 1);
     if (B->ready) (void)hipEventDestroy(B->ready);
     if (B->stream) (void)hipStreamDestroy(B->stream);
 }
@@ -1331,10 +1361,9 @@ static int batch_capture(fvg_batch* B) {
         B->graph = nullptr;
     }
     const void* fn = team_kernel_for(B->nt, B->variant);
-    int n_inst = B->n_inst;
-    void* args[] = {(void*)&B->d_desc, (void*)&n_inst};
+    void* args[] = {(void*)&B->d_desc, (void*)&B->d_count};
     FVG_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-    const hipError_t le = hipLaunchKernel(fn, dim3((unsigned)B->n_inst), dim3((unsigned)B->nt), args, 0, B->stream);
+    const hipError_t le = hipLaunchKernel(fn, dim3((unsigned)B->capacity), dim3((unsigned)B->nt), args, 0, B->stream);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(B->stream, &g);
     if (le != hipSuccess) {
@@ -1345,6 +1374,7 @@ static int batch_capture(fvg_batch* B) {
     B->graph = g;
     FVG_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
     B->graph_nt = B->nt;
+    B->graph_cap = B->capacity;
     return FVG_OK;
 }
 
@@ -1363,6 +1393,116 @@ static void fill_sweep_desc(const TeamPlan* B, const TeamSweepLayout& S, bool up
 
 }  // namespace fvg
 
+namespace fvg {
+
+static int check_members(int32_t n, const fvg_csr* const* a, const fvg_prec* const* p) {
+    if (!a || !p || n <= 0) return fail(FVG_E_INVALID, "bad argument");
+    for (int k = 0; k < n; ++k) {
+        if (!a[k] || !p[k]) return fail(FVG_E_INVALID, "null matrix or preconditioner");
+        if (a[k]->nrows != a[k]->ncols || a[k]->nrows <= 0) return fail(FVG_E_INVALID, "batched systems must be square");
+        if (a[k]->nrows > INT32_MAX / 4) return fail(FVG_E_INVALID, "system too large for a batch");
+        if (p[k]->n != a[k]->nrows) return fail(FVG_E_INVALID, "preconditioner size does not match its matrix");
+        const bool same = a[k]->pattern_id != 0 && a[k]->pattern_id == p[k]->pattern_id && a[k]->nnz == p[k]->nnz;
+        if ((p[k]->kind == kIC || p[k]->kind == kILU) && !same && p[k]->h_col != a[k]->h_col)
+            return fail(FVG_E_INVALID, "preconditioner pattern differs from its matrix");
+    }
+    std::vector<const fvg_prec*> seen(p, p + n);
+    std::sort(seen.begin(), seen.end());
+    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+        return fail(FVG_E_INVALID, "a preconditioner appears twice in one batch");
+    return FVG_OK;
+}
+
+// per-system storage for `cap` systems (history, pinned status, descriptors)
+static int batch_reserve(fvg_ctx* c, fvg_batch* B, int cap) {
+    if (cap <= B->capacity) return FVG_OK;
+    if (B->capacity > 0) {   // grow: release the smaller storage first
+        dev_free_n(c, B->hist, B->nhist);
+        (void)hipHostFree(B->h_info);
+        (void)hipHostFree(B->h_done);
+        (void)hipHostFree(B->h_desc);
+        B->h_info = nullptr;
+        B->h_done = nullptr;
+        B->h_desc = nullptr;
+        dev_free_n(c, B->d_prof, 4 * B->capacity);
+        dev_free_n(c, B->d_desc, B->capacity);
+    }
+    B->capacity = cap;
+    B->nhist = B->hcap * cap;
+    FVG_TRY(dev_alloc_n(c, B->nhist, &B->hist));
+    if (hipHostMalloc((void**)&B->h_info, sizeof(fvg_solver_info) * cap, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&B->h_done, sizeof(int32_t) * cap, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&B->h_desc, sizeof(TeamDesc) * (cap + 1), hipHostMallocDefault) != hipSuccess)
+        return fail(FVG_E_HIP, "pinned batch status allocation failed");
+    std::memset(B->h_info, 0, sizeof(fvg_solver_info) * cap);
+    std::memset(B->h_done, 0, sizeof(int32_t) * cap);
+    FVG_TRY(dev_alloc_n(c, cap, &B->d_desc));
+    if (!B->d_count) FVG_TRY(dev_alloc_n(c, 1, &B->d_count));
+    B->desc_dirty = true;
+    return FVG_OK;
+}
+
+// the systems of the next launches: plans, launch order and descriptors (uploaded at launch)
+static int batch_bind(fvg_ctx* c, fvg_batch* B, int32_t n, const fvg_csr* const* a, const fvg_prec* const* p) {
+    FVG_TRY(check_members(n, a, p));
+    FVG_TRY(batch_reserve(c, B, n));
+    B->n_inst = n;
+    B->A.assign(a, a + n);
+    B->P.resize((size_t)n);
+    B->plan.assign((size_t)n, nullptr);
+    for (int k = 0; k < n; ++k) {
+        B->P[(size_t)k] = const_cast<fvg_prec*>(p[k]);   // the plan is a cache owned by the preconditioner
+        FVG_TRY(get_plan(c, a[k], B->P[(size_t)k], &B->plan[(size_t)k]));
+    }
+    // largest systems first, so they start first and spread over the XCDs
+    B->order.resize((size_t)n);
+    std::iota(B->order.begin(), B->order.end(), 0);
+    std::stable_sort(B->order.begin(), B->order.end(),
+                     [&](int x, int y) { return B->plan[(size_t)x]->L.n > B->plan[(size_t)y]->L.n; });
+    const fvg_solver_options* o = &B->opts;
+    B->desc.assign((size_t)n, TeamDesc{});
+    for (int slot = 0; slot < n; ++slot) {
+        const int k = B->order[(size_t)slot];
+        const TeamPlan* t = B->plan[(size_t)k];
+        const TeamLayout& L = t->L;
+        TeamDesc& D = B->desc[(size_t)slot];
+        std::memset(&D, 0, sizeof(D));
+        D.n = (int32_t)L.n;
+        D.npad = (int32_t)L.npad;
+        D.wspan = L.wspan;
+        D.a_nnz1 = (int32_t)(L.nnz_a - 1);
+        D.win64 = L.win64;
+        D.win32 = L.win32;
+        D.kind = p[k]->kind;
+        const int32_t* I = t->ints;
+        D.perm = I + L.perm;
+        D.a_ptr = I + L.a_ptr;
+        D.a_dep = I + L.a_dep;
+        D.a_val = t->v64 + L.a_val;
+        D.a_val32 = t->v32 + L.a_val;
+        fill_sweep_desc(t, L.lo, false, D.lo);
+        fill_sweep_desc(t, L.up, true, D.up);
+        D.sc = t->v64 + L.sc;
+        D.sc32 = t->v32 + L.sc;
+        D.w64 = t->w64;
+        D.w32 = t->w32;
+        D.hist = B->hist + B->hcap * k;
+        D.hcap = B->hcap;
+        D.info = B->h_info + k;
+        D.done = B->h_done + k;
+        D.pflags = (p[k]->kind == kIC || p[k]->kind == kILU) ? p[k]->d_flags : nullptr;
+        D.prof = B->d_prof ? B->d_prof + 4 * k : nullptr;
+        D.atol = o->absolute_tolerance;
+        D.rtol = o->relative_tolerance;
+        D.inner_tol = o->inner_tolerance > 0.0 ? o->inner_tolerance : 1e-4;
+        D.max_iter = o->max_iterations;
+    }
+    B->desc_dirty = true;
+    return FVG_OK;
+}
+
+}  // namespace fvg
+
 using namespace fvg;
 
 extern "C" {
@@ -1373,93 +1513,18 @@ int fvg_batch_create(fvg_ctx* c, int32_t n, const fvg_csr* const* a, const fvg_p
     if (solver != FVG_SOLVER_CG && solver != FVG_SOLVER_BICGSTAB) return fail(FVG_E_INVALID, "unknown solver");
     if (o->max_iterations < 0) return fail(FVG_E_INVALID, "max_iterations must be non-negative");
     *out = nullptr;
-    for (int k = 0; k < n; ++k) {
-        if (!a[k] || !p[k]) return fail(FVG_E_INVALID, "null matrix or preconditioner");
-        if (a[k]->nrows != a[k]->ncols || a[k]->nrows <= 0) return fail(FVG_E_INVALID, "batched systems must be square");
-        if (a[k]->nrows > INT32_MAX / 4) return fail(FVG_E_INVALID, "system too large for a batch");
-        if (p[k]->n != a[k]->nrows) return fail(FVG_E_INVALID, "preconditioner size does not match its matrix");
-        if ((p[k]->kind == kIC || p[k]->kind == kILU) && p[k]->h_col != a[k]->h_col)
-            return fail(FVG_E_INVALID, "preconditioner pattern differs from its matrix");
-        for (int j = 0; j < k; ++j)
-            if (p[j] == p[k]) return fail(FVG_E_INVALID, "a preconditioner appears twice in one batch");
-    }
+    FVG_TRY(check_members(n, a, p));
     auto* B = new fvg_batch();
-    B->n_inst = n;
     B->solver = solver;
     B->precision = o->precision;
     B->opts = *o;
     B->variant = (solver == FVG_SOLVER_CG ? 0 : 1) + (o->precision == FVG_PRECISION_MIXED ? 2 : 0);
-    B->A.assign(a, a + n);
-    B->P.resize((size_t)n);
-    B->plan.resize((size_t)n, nullptr);
-    const bool mixed = o->precision == FVG_PRECISION_MIXED;
-    B->hcap = mixed ? 64 : o->max_iterations + 2;
-    int s = FVG_OK;
-    for (int k = 0; k < n && s == FVG_OK; ++k) {
-        B->P[(size_t)k] = const_cast<fvg_prec*>(p[k]);   // the plan is a cache owned by the preconditioner
-        s = get_plan(c, a[k], B->P[(size_t)k], &B->plan[(size_t)k]);
-    }
-    B->nhist = B->hcap * n;
-    if (s == FVG_OK) s = dev_alloc_n(c, B->nhist, &B->hist);
-    if (s == FVG_OK && (hipHostMalloc((void**)&B->h_info, sizeof(fvg_solver_info) * n, hipHostMallocDefault) !=
-                            hipSuccess ||
-                        hipHostMalloc((void**)&B->h_done, sizeof(int32_t) * n, hipHostMallocDefault) != hipSuccess))
-        s = fail(FVG_E_HIP, "pinned batch status allocation failed");
-    if (s == FVG_OK) {
-        std::memset(B->h_info, 0, sizeof(fvg_solver_info) * n);
-        std::memset(B->h_done, 0, sizeof(int32_t) * n);
-    }
-    if (s == FVG_OK) s = dev_alloc_n(c, n, &B->d_desc);
+    B->hcap = o->precision == FVG_PRECISION_MIXED ? 64 : o->max_iterations + 2;
+    int s = batch_bind(c, B, n, a, p);
     if (s == FVG_OK && hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess)
         s = fail(FVG_E_HIP, "batch stream creation failed");
     if (s == FVG_OK && hipEventCreateWithFlags(&B->ready, hipEventDisableTiming) != hipSuccess)
         s = fail(FVG_E_HIP, "batch event creation failed");
-    if (s == FVG_OK) {
-        // largest systems first, so they start first and spread over the XCDs
-        B->order.resize((size_t)n);
-        std::iota(B->order.begin(), B->order.end(), 0);
-        std::stable_sort(B->order.begin(), B->order.end(),
-                         [&](int x, int y) { return B->plan[(size_t)x]->L.n > B->plan[(size_t)y]->L.n; });
-        B->desc.resize((size_t)n);
-        for (int slot = 0; slot < n; ++slot) {
-            const int k = B->order[(size_t)slot];
-            const TeamPlan* t = B->plan[(size_t)k];
-            const TeamLayout& L = t->L;
-            TeamDesc& D = B->desc[(size_t)slot];
-            std::memset(&D, 0, sizeof(D));
-            D.n = (int32_t)L.n;
-            D.npad = (int32_t)L.npad;
-            D.wspan = L.wspan;
-            D.a_nnz1 = (int32_t)(L.nnz_a - 1);
-            D.win64 = L.win64;
-            D.win32 = L.win32;
-            D.kind = p[k]->kind;
-            const int32_t* I = t->ints;
-            D.perm = I + L.perm;
-            D.a_ptr = I + L.a_ptr;
-            D.a_dep = I + L.a_dep;
-            D.a_val = t->v64 + L.a_val;
-            D.a_val32 = t->v32 + L.a_val;
-            fill_sweep_desc(t, L.lo, false, D.lo);
-            fill_sweep_desc(t, L.up, true, D.up);
-            D.sc = t->v64 + L.sc;
-            D.sc32 = t->v32 + L.sc;
-            D.w64 = t->w64;
-            D.w32 = t->w32;
-            D.hist = B->hist + B->hcap * k;
-            D.hcap = B->hcap;
-            D.info = B->h_info + k;
-            D.done = B->h_done + k;
-            D.atol = o->absolute_tolerance;
-            D.rtol = o->relative_tolerance;
-            D.inner_tol = o->inner_tolerance > 0.0 ? o->inner_tolerance : 1e-4;
-            D.max_iter = o->max_iterations;
-        }
-        if (hipMemcpyAsync(B->d_desc, B->desc.data(), sizeof(TeamDesc) * B->desc.size(), hipMemcpyHostToDevice,
-                           c->stream) != hipSuccess ||
-            hipStreamSynchronize(c->stream) != hipSuccess)
-            s = fail(FVG_E_HIP, "batch descriptor upload failed");
-    }
     if (s != FVG_OK) {
         (void)hipStreamSynchronize(c->stream);
         batch_free(c, B);
@@ -1468,6 +1533,22 @@ int fvg_batch_create(fvg_ctx* c, int32_t n, const fvg_csr* const* a, const fvg_p
     }
     *out = B;
     return FVG_OK;
+}
+
+int fvg_batch_reserve(fvg_ctx* c, fvg_batch* B, int32_t capacity) {
+    if (!c || !B || capacity <= 0) return fail(FVG_E_INVALID, "bad argument");
+    if (B->launched) return fail(FVG_E_INVALID, "batch in flight: finish it first");
+    if (capacity <= B->capacity) return FVG_OK;
+    FVG_TRY(batch_reserve(c, B, capacity));
+    // descriptors point into the per-system storage just moved: rebuild them
+    std::vector<const fvg_prec*> ps(B->P.begin(), B->P.end());
+    return batch_bind(c, B, B->n_inst, B->A.data(), ps.data());
+}
+
+int fvg_batch_rebind(fvg_ctx* c, fvg_batch* B, int32_t n, const fvg_csr* const* a, const fvg_prec* const* p) {
+    if (!c || !B) return fail(FVG_E_INVALID, "null argument");
+    if (B->launched) return fail(FVG_E_INVALID, "batch in flight: finish it first");
+    return batch_bind(c, B, n, a, p);
 }
 
 int fvg_batch_destroy(fvg_ctx* c, fvg_batch* B) {
@@ -1509,20 +1590,29 @@ int fvg_batch_launch(fvg_ctx* c, fvg_batch* B, const fvg_vec* const* b, fvg_vec*
             moved = true;
         }
     }
+    moved = moved || B->desc_dirty;
     // refresh values assembled or refactored since the last launch
     for (int k = 0; k < B->n_inst; ++k) {
         TeamPlan* t = B->plan[(size_t)k];
         if (t->a_ver != t->a->version || t->p_ver != B->P[(size_t)k]->fac_version)
             FVG_TRY(plan_refresh(c, t, B->P[(size_t)k], c->stream));
     }
-    if (moved)
-        FVG_HIP(hipMemcpyAsync(B->d_desc, B->desc.data(), sizeof(TeamDesc) * B->desc.size(), hipMemcpyHostToDevice,
+    if (moved) {
+        // through pinned memory, so the copies never wait for the host; the pinned copy is
+        // rewritten only at a later launch, after this batch has finished
+        std::memcpy(B->h_desc, B->desc.data(), sizeof(TeamDesc) * B->desc.size());
+        int32_t* cnt = reinterpret_cast<int32_t*>(B->h_desc + B->capacity);
+        *cnt = B->n_inst;
+        FVG_HIP(hipMemcpyAsync(B->d_desc, B->h_desc, sizeof(TeamDesc) * B->desc.size(), hipMemcpyHostToDevice,
                                c->stream));
+        FVG_HIP(hipMemcpyAsync(B->d_count, cnt, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        B->desc_dirty = false;
+    }
     std::memset(B->h_done, 0, sizeof(int32_t) * (size_t)B->n_inst);
     // the batch stream starts after everything enqueued on the context stream so far
     FVG_HIP(hipEventRecord(B->ready, c->stream));
     FVG_HIP(hipStreamWaitEvent(B->stream, B->ready, 0));
-    if (!B->exec || B->graph_nt != B->nt) FVG_TRY(batch_capture(B));
+    if (!B->exec || B->graph_nt != B->nt || B->graph_cap != B->capacity) FVG_TRY(batch_capture(B));
     FVG_HIP(hipGraphLaunch(B->exec, B->stream));
     B->launched = true;
     for (TeamPlan* t : B->plan) t->busy = true;
@@ -1596,14 +1686,12 @@ int fvg_batch_set_phase_timing(fvg_ctx* c, fvg_batch* B, int32_t enable) {
     if (!c || !B) return fail(FVG_E_INVALID, "null argument");
     if (B->launched) return fail(FVG_E_INVALID, "batch in flight");
     if (enable && !B->d_prof) {
-        FVG_TRY(dev_alloc_n(c, 4 * (int64_t)B->n_inst, &B->d_prof));
-        FVG_HIP(hipMemsetAsync(B->d_prof, 0, sizeof(int64_t) * 4 * B->n_inst, c->stream));
+        FVG_TRY(dev_alloc_n(c, 4 * (int64_t)B->capacity, &B->d_prof));
+        FVG_HIP(hipMemsetAsync(B->d_prof, 0, sizeof(int64_t) * 4 * B->capacity, c->stream));
     }
     for (int slot = 0; slot < B->n_inst; ++slot)
         B->desc[(size_t)slot].prof = enable ? B->d_prof + 4 * B->order[(size_t)slot] : nullptr;
-    FVG_HIP(hipMemcpyAsync(B->d_desc, B->desc.data(), sizeof(TeamDesc) * B->desc.size(), hipMemcpyHostToDevice,
-                           c->stream));
-    FVG_HIP(hipStreamSynchronize(c->stream));
+    B->desc_dirty = true;
     return FVG_OK;
 }
 

@@ -68,7 +68,7 @@ class SolverOptions(C.Structure):
 class SolverInfo(C.Structure):
     _fields_ = [("iterations", C.c_int64), ("initial_residual", C.c_double), ("final_residual", C.c_double),
                 ("history_len", C.c_int64), ("restarts", C.c_int64), ("outer_iterations", C.c_int64),
-                ("true_residual", C.c_double)]
+                ("true_residual", C.c_double), ("status", C.c_int32), ("reserved", C.c_int32)]
 
 
 class KernelStats(C.Structure):
@@ -127,6 +127,10 @@ def lib():
         "fvg_ic_create": [P, P, C.c_int32, C.POINTER(P)],
         "fvg_ilu_create": [P, P, C.c_int32, C.POINTER(P)],
         "fvg_prec_refactor": [P, P, P],
+        "fvg_prec_refactor_async": [P, P, P],
+        "fvg_prec_status": [P, P],
+        "fvg_vec_upload_many": [P, C.c_int32, P, P],
+        "fvg_prec_refactor_many": [P, C.c_int32, P, P],
         "fvg_prec_destroy": [P, P],
         "fvg_precondition": [P, P, P, P],
         "fvg_prec_factor_download": [P, P, P, P],
@@ -136,6 +140,8 @@ def lib():
         "fvg_bicgstab": [P, P, P, P, P, C.POINTER(SolverOptions), C.POINTER(SolverInfo), P, C.c_int64],
         "fvg_batch_create": [P, C.c_int32, P, P, C.c_int32, C.POINTER(SolverOptions), C.POINTER(P)],
         "fvg_batch_destroy": [P, P],
+        "fvg_batch_rebind": [P, P, C.c_int32, P, P],
+        "fvg_batch_reserve": [P, P, C.c_int32],
         "fvg_batch_size": [P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)],
         "fvg_batch_launch": [P, P, P, P],
         "fvg_batch_query": [P, P, C.POINTER(C.c_int32)],
@@ -209,6 +215,30 @@ class Context:
         check(lib().fvg_ctx_kernel_stats(self.h, C.byref(s), 1 if reset else 0))
         return dict(spmv_ms=s.spmv_ms, sweep_ms=s.sweep_ms, spmv_launches=s.spmv_launches,
                     sweep_launches=s.sweep_launches, spmv_calls=s.spmv_calls, sweep_calls=s.sweep_calls)
+
+
+def refactor_many(ctx, preconditioners, matrices):
+    """Refactorise several preconditioners in one launch (checks deferred to status())."""
+    n = len(preconditioners)
+    if n == 0:
+        return
+    pp = (P * n)(*[p.h for p in preconditioners])
+    mp = (P * n)(*[m.h for m in matrices])
+    check(lib().fvg_prec_refactor_many(ctx.h, n, pp, mp))
+
+
+def upload_many(ctx, vectors, arrays):
+    """Upload arrays into vectors through one staging buffer, without waiting for the copies."""
+    n = len(vectors)
+    if n == 0:
+        return
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    for v, a in zip(vectors, keep):
+        if a.shape != (v.n,):
+            raise ValueError("array length does not match its vector")
+    vp = (P * n)(*[v.h for v in vectors])
+    hp = (P * n)(*[_ptr(a) for a in keep])
+    check(lib().fvg_vec_upload_many(ctx.h, n, vp, hp))
 
 
 class DistributedVector:
@@ -578,6 +608,15 @@ class Preconditioner:
     def precondition(self, solution, rhs):
         check(lib().fvg_precondition(self.ctx.h, self.h, solution.h, rhs.h))
 
+    def refactor_async(self, matrix):
+        """Enqueue the refactorisation; status() reports pivot / symmetry failures later."""
+        check(lib().fvg_prec_refactor_async(self.ctx.h, self.h, matrix.h))
+
+    def status(self, ctx=None):
+        """Raise the factorisation's failure, if any, of the last refactor_async (reads the
+        flags on ctx's stream, the preconditioner's context by default)."""
+        check(lib().fvg_prec_status((ctx or self.ctx).h, self.h))
+
     def refactor(self, matrix):
         check(lib().fvg_prec_refactor(self.ctx.h, self.h, matrix.h))
         self.ctx.sync()
@@ -676,6 +715,7 @@ class LinearSolverInfo:
     restarts: int = 0
     outer_iterations: int = 0
     true_residual: float = 0.0
+    status: int = 0
 
     def __str__(self):
         return (f"(solver={self.solver_identifier}, iterations={self.iterations}, "
@@ -738,6 +778,21 @@ class Batch:
         self.h = h
         self._inflight = None
 
+    def reserve(self, capacity):
+        """Size the per-system storage for up to `capacity` systems now."""
+        check(lib().fvg_batch_reserve(self.ctx.h, self.h, int(capacity)))
+
+    def rebind(self, matrices, preconditioners):
+        """Point this (finished) batch at another set of systems of the same solver."""
+        if len(matrices) != len(preconditioners) or not matrices:
+            raise ValueError("one preconditioner per matrix, at least one system")
+        n = len(matrices)
+        am = (P * n)(*[m.h for m in matrices])
+        pm = (P * n)(*[p.h for p in preconditioners])
+        check(lib().fvg_batch_rebind(self.ctx.h, self.h, n, am, pm))
+        self.n = n
+        self._keep = (list(matrices), list(preconditioners))
+
     @property
     def threads(self):
         t = C.c_int32()
@@ -747,6 +802,27 @@ class Batch:
     @threads.setter
     def threads(self, value):
         check(lib().fvg_batch_set_threads(self.ctx.h, self.h, int(value)))
+
+    def tune(self, rhs, solutions, candidates=(1024, 512, 256), repeats=1):
+        """Autotune the workgroup size per system (rows per tile = threads, one row per
+        thread): time a solve of this batch with each candidate from x = 0 and keep the
+        fastest.  Overwrites the solutions.  Returns {threads: seconds}."""
+        import time as _time
+        times = {}
+        for th in candidates:
+            self.threads = th
+            best = None
+            for _ in range(repeats):
+                for x in solutions:
+                    x.fill(0.0)
+                self.ctx.sync()
+                t = _time.perf_counter()
+                self.solve(rhs, solutions, history=False)
+                el = _time.perf_counter() - t
+                best = el if best is None else min(best, el)
+            times[th] = best
+        self.threads = min(times, key=times.get)
+        return times
 
     def set_phase_timing(self, enable=True):
         check(lib().fvg_batch_set_phase_timing(self.ctx.h, self.h, 1 if enable else 0))
@@ -776,7 +852,7 @@ class Batch:
         for k in np.nonzero(flags)[0]:
             i = infos[int(k)]
             out[int(k)] = LinearSolverInfo(name, i.iterations, i.initial_residual, i.final_residual, np.zeros(0),
-                                           i.restarts, i.outer_iterations, i.true_residual)
+                                           i.restarts, i.outer_iterations, i.true_residual, i.status)
         return flags.astype(bool), out
 
     def done(self):
@@ -799,7 +875,7 @@ class Batch:
             hl = min(i.history_len, cap)
             out.append(LinearSolverInfo(name, i.iterations, i.initial_residual, i.final_residual,
                                         hists[k][:hl].copy() if history else np.zeros(0), i.restarts,
-                                        i.outer_iterations, i.true_residual))
+                                        i.outer_iterations, i.true_residual, i.status))
         return out
 
     def solve(self, rhs, solutions, history=True):

@@ -23,6 +23,10 @@ Scheduling:
 
 Each result is independent of what it was batched with (one workgroup solves one system,
 with fixed reduction order), so a request gives the same bits alone and in the mix.
+
+The server drives ``2 * lanes + 2`` streams (a batch stream per lane and solver kind,
+assembly, result copies); with fewer hardware queues than streams (GPU_MAX_HW_QUEUES,
+read when the HIP runtime starts) some of them share a queue and wait on each other.
 """
 from __future__ import annotations
 
@@ -54,6 +58,7 @@ class Request:
     done: float = -1.0
     x: np.ndarray | None = None
     info: object = None
+    error: object = None
     batch_size: int = 0
 
     @property
@@ -133,21 +138,24 @@ class _Slot:
         self.fresh = True          # fields of the creating request are already assembled
         self.last_used = time.perf_counter()
 
-    def prepare(self, req):
-        """Assemble req's system into this slot and refactor (context stream)."""
-        if not self.fresh:
-            a = self.asm
-            for v, data in zip(a.fvecs, req.face_fields):
-                v.upload(data)
-            for v, data in zip(a.cvecs, req.cell_fields):
-                v.upload(data)
-            if a.bi is not None:
-                a.bi.upload(req.bc_i)
-                a.be.upload(req.bc_e)
-            a.run(self.A, self.b, sync=False)
-            self.M.refactor(self.A)
+    def transfers(self, req):
+        """(vectors, arrays) that carry req's fields into this slot's assembler."""
+        if self.fresh:
+            return [], []
+        a = self.asm
+        vecs = a.fvecs + a.cvecs + ([a.bi, a.be] if a.bi is not None else [])
+        arrs = list(req.face_fields) + list(req.cell_fields) + ([req.bc_i, req.bc_e] if a.bi is not None else [])
+        return vecs, arrs
+
+    def prepare(self):
+        """Assemble on the context stream without waiting (fields uploaded); returns True
+        when the preconditioner needs refactoring."""
+        fresh = self.fresh
+        if not fresh:
+            self.asm.run(self.A, self.b, sync=False)
         self.fresh = False
         self.x.fill(0.0)
+        return not fresh
 
     def close(self):
         for o in (self.M, self.A, self.b, self.x):
@@ -158,7 +166,8 @@ class _Slot:
 
 @dataclass
 class _Lane:
-    batch: object = None
+    batch: object = None                          # the batch in flight, if any
+    objs: dict = field(default_factory=dict)      # solver kind -> this lane's reusable fvg.Batch
     reqs: list = field(default_factory=list)
     slots: list = field(default_factory=list)
     reported: np.ndarray | None = None
@@ -174,7 +183,7 @@ class Server:
     """
 
     def __init__(self, budget_bytes=200 * 10**9, lanes=4, max_batch=128, options=None, large_rows=100_000,
-                 device=0):
+                 device=0, max_inflight=192, threads=1024):
         self.ctx = fvg.Context(device, int(budget_bytes))
         self.io = fvg.Context(device)            # a second stream for result downloads
         self.budget = int(budget_bytes)
@@ -182,12 +191,19 @@ class Server:
         self.lanes = [_Lane() for _ in range(lanes)]
         self.max_batch = max_batch
         self.large_rows = large_rows
+        # systems solving at once: each holds a compute unit, so keep some free for the
+        # assembly and factorisation of the next batch
+        self.max_inflight = max_inflight
+        self.threads = threads
         self.meshes = {}
         self.slots = []
         self.pending = []
         self.finished = []
         self.peak_bytes = 0
         self.evictions = 0
+        # host seconds spent per activity, and counts (diagnostics)
+        self.stats = dict(slot_create_s=0.0, slots_created=0, prepare_s=0.0, batch_setup_s=0.0, batches=0,
+                          complete_s=0.0)
         self.t0 = time.perf_counter()
 
     # -- registration / submission -------------------------------------------------
@@ -219,6 +235,23 @@ class Server:
             self.evictions += 1
         return self.ctx.bytes_in_use() + need <= self.budget
 
+    def reserve(self, template, count):
+        """Provision `count` idle resident slots for the template request's mesh and kind
+        (within the budget), so requests arriving later find them ready."""
+        have = sum(1 for s in self.slots if s.entry.key == template.mesh and s.kind == template.kind)
+        for _ in range(max(0, count - have)):
+            entry = self.meshes[template.mesh]
+            need = entry.slot_bytes if entry.slot_bytes is not None else 1200 * entry.nnz
+            if self.ctx.bytes_in_use() + need > self.budget:
+                break
+            t = time.perf_counter()
+            s = _Slot(self, entry, template.kind, template)
+            self.stats["slot_create_s"] += time.perf_counter() - t
+            self.stats["slots_created"] += 1
+            s.fresh = False          # its fields belong to no waiting request
+            self.slots.append(s)
+        self.peak_bytes = max(self.peak_bytes, self.ctx.bytes_in_use())
+
     def drop_idle_slots(self):
         """Destroy every slot not in a batch in flight."""
         for s in [s for s in self.slots if not s.busy]:
@@ -235,7 +268,10 @@ class Server:
         if not self._evict_for(need):
             return None
         try:
+            t = time.perf_counter()
             s = _Slot(self, entry, req.kind, req)
+            self.stats["slot_create_s"] += time.perf_counter() - t
+            self.stats["slots_created"] += 1
         except fvg.BudgetError:
             self.ctx.sync()
             return None
@@ -245,11 +281,20 @@ class Server:
 
     # -- the loop --------------------------------------------------------------------
     def _complete(self, lane):
+        t = time.perf_counter()
+        try:
+            return self._complete_inner(lane)
+        finally:
+            self.stats["complete_s"] += time.perf_counter() - t
+
+    def _complete_inner(self, lane):
         flags, infos = lane.batch.poll()
         for k in np.nonzero(flags & ~lane.reported)[0]:
             k = int(k)
             r = lane.reqs[k]
             r.info = infos[k]
+            if r.info.status != 0:   # pivot / symmetry failure of its factorisation: not solved
+                r.error = fvg.FvgError(r.info.status, "factorisation failed its checks")
             x = np.empty(lane.slots[k].entry.n)
             fvg.check(fvg.lib().fvg_vec_download(self.io.h, lane.slots[k].x.h, fvg._ptr(x)))
             r.x = x
@@ -258,7 +303,6 @@ class Server:
             self.finished.append(r)
         if lane.reported.all():
             lane.batch.finish(history=False)
-            lane.batch.close()
             for s in lane.slots:
                 s.busy = False
                 s.last_used = time.perf_counter()
@@ -267,12 +311,33 @@ class Server:
         return bool(flags.any())
 
     def _launch(self, lane, group, slots, large):
+        t = time.perf_counter()
+        vecs, arrs = [], []
         for r, s in zip(group, slots):
             r.admitted = self.now()
-            s.prepare(r)
-        b = fvg.Batch(self.ctx, [s.A for s in slots], [s.M for s in slots], group[0].kind, self.options)
+            v, a = s.transfers(r)
+            vecs += v
+            arrs += a
+        fvg.upload_many(self.ctx, vecs, arrs)
+        stale = [s for s in slots if s.prepare()]
+        fvg.refactor_many(self.ctx, [s.M for s in stale], [s.A for s in stale])
+        t1 = time.perf_counter()
+        kind = group[0].kind
+        b = lane.objs.get(kind)
+        t2 = time.perf_counter()
+        if b is None:
+            b = fvg.Batch(self.ctx, [s.A for s in slots], [s.M for s in slots], kind, self.options)
+            b.threads = self.threads
+            b.reserve(self.max_batch)
+            lane.objs[kind] = b
+        else:
+            b.rebind([s.A for s in slots], [s.M for s in slots])
+        self.stats["rebind_s"] = self.stats.get("rebind_s", 0.0) + time.perf_counter() - t2
         self.peak_bytes = max(self.peak_bytes, self.ctx.bytes_in_use())
         b.launch([s.b for s in slots], [s.x for s in slots])
+        self.stats["prepare_s"] += t1 - t
+        self.stats["batch_setup_s"] += time.perf_counter() - t1
+        self.stats["batches"] += 1
         for r in group:
             r.batch_size = len(group)
             self.pending.remove(r)
@@ -288,9 +353,13 @@ class Server:
             if lane.batch is not None:
                 progressed |= self._complete(lane)
         large_busy = sum(1 for l in self.lanes if l.batch is not None and l.large)
+        inflight = sum(len(l.reqs) for l in self.lanes if l.batch is not None)
         for lane in self.lanes:
             if lane.batch is not None or not self.pending:
                 continue
+            room = min(self.max_batch, self.max_inflight - inflight)
+            if room <= 0:
+                break
             allow_large = large_busy < max(1, len(self.lanes) // 2)
             reserved = set()
 
@@ -306,7 +375,7 @@ class Server:
                     return self.ctx.bytes_in_use() + need <= self.budget + idle
                 return False
 
-            group = form_group(self.pending, can_place, self.max_batch, allow_large, self.large_rows,
+            group = form_group(self.pending, can_place, room, allow_large, self.large_rows,
                                lambda r: self.meshes[r.mesh].n)
             if not group:
                 if all(l.batch is None for l in self.lanes) and not any(s.busy for s in self.slots):
@@ -328,6 +397,7 @@ class Server:
                 break
             large = any(self.meshes[r.mesh].n >= self.large_rows for r in placed)
             self._launch(lane, placed, slots, large)
+            inflight += len(placed)
             if large:
                 large_busy += 1
             progressed = True
@@ -353,8 +423,10 @@ class Server:
         for lane in self.lanes:
             if lane.batch is not None:
                 lane.batch.finish(history=False)
-                lane.batch.close()
                 lane.batch = None
+            for b in lane.objs.values():
+                b.close()
+            lane.objs = {}
         for s in self.slots:
             s.close()
         self.slots = []

@@ -81,6 +81,9 @@ int fvg_vec_destroy(fvg_ctx* ctx, fvg_vec* v);
 int fvg_vec_len(const fvg_vec* v, int64_t* n);
 int fvg_vec_device_ptr(const fvg_vec* v, double** device_ptr);
 int fvg_vec_upload(fvg_ctx* ctx, fvg_vec* v, const double* host);     /* from_data (dvector.rs:23-25) */
+/* Uploads several vectors through one staging buffer without waiting for the copies
+ * (they complete in stream order before any later work on the context stream). */
+int fvg_vec_upload_many(fvg_ctx* ctx, int32_t count, fvg_vec* const* vecs, const double* const* hosts);
 int fvg_vec_download(fvg_ctx* ctx, const fvg_vec* v, double* host);
 /* Set every entry to value on the device (asynchronous on the context stream). */
 int fvg_vec_fill(fvg_ctx* ctx, fvg_vec* v, double value);
@@ -184,6 +187,13 @@ int fvg_ic_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out)
 int fvg_ilu_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out);
 /* Re-run the numeric factorisation for new values on the same pattern. */
 int fvg_prec_refactor(fvg_ctx* ctx, fvg_prec* p, const fvg_csr* a);
+/* Enqueues the refactorisation without waiting (no symmetry / pivot checks yet):
+ * fvg_prec_status reports them later (and synchronises). */
+int fvg_prec_refactor_async(fvg_ctx* ctx, fvg_prec* p, const fvg_csr* a);
+int fvg_prec_status(fvg_ctx* ctx, const fvg_prec* p);
+/* Refactorises count preconditioners (each with its matrix) in one launch, one workgroup
+ * per system, without the checks; fvg_prec_status reports them per preconditioner. */
+int fvg_prec_refactor_many(fvg_ctx* ctx, int32_t count, fvg_prec* const* precs, const fvg_csr* const* matrices);
 int fvg_prec_destroy(fvg_ctx* ctx, fvg_prec* p);
 /* Preconditioner::precondition(solution, rhs): solution = M^-1 rhs. */
 int fvg_precondition(fvg_ctx* ctx, const fvg_prec* p, fvg_vec* solution, const fvg_vec* rhs);
@@ -219,6 +229,9 @@ typedef struct fvg_solver_info {
     int64_t restarts;           /* BiCGStab restarts (bi_conjugate_gradient_stab.rs:155-169) */
     int64_t outer_iterations;   /* mixed only: refinement steps */
     double true_residual;       /* mixed only: ||b - A x|| / ||b|| in FP64 at exit */
+    int32_t status;             /* batched solves: FVG_OK, or why the system was not solved
+                                   (FVG_E_ZERO_PIVOT / FVG_E_NOT_SYMMETRIC from its factorisation) */
+    int32_t reserved;
 } fvg_solver_info;
 
 /* conjugate_gradient (solvers/conjugate_gradient.rs:12-112).  x is the initial
@@ -247,6 +260,13 @@ typedef struct fvg_batch fvg_batch;
 int fvg_batch_create(fvg_ctx* ctx, int32_t n, const fvg_csr* const* a, const fvg_prec* const* p, int32_t solver,
                      const fvg_solver_options* opts, fvg_batch** out);
 int fvg_batch_destroy(fvg_ctx* ctx, fvg_batch* batch);
+/* Points a finished (or never launched) batch at another set of n systems, keeping its
+ * stream, captured launch and storage (grown when n exceeds what it held).  A server
+ * reuses one batch per lane this way. */
+int fvg_batch_rebind(fvg_ctx* ctx, fvg_batch* batch, int32_t n, const fvg_csr* const* a, const fvg_prec* const* p);
+/* Sizes a batch's per-system storage for up to capacity systems now, so later rebinds
+ * never reallocate (reallocation synchronises the device). */
+int fvg_batch_reserve(fvg_ctx* ctx, fvg_batch* batch, int32_t capacity);
 int fvg_batch_size(const fvg_batch* batch, int32_t* n_systems, int64_t* total_rows);
 /* Enqueues the solve of every system (x_k: initial guess in, solution out). */
 int fvg_batch_launch(fvg_ctx* ctx, fvg_batch* batch, const fvg_vec* const* b, fvg_vec* const* x);

@@ -204,3 +204,57 @@ def test_batch_zero_rhs_and_errors(ctx):
     M3 = fvg.IncompleteLowerUpper.from_matrix(A3, 0)
     with pytest.raises(fvg.FvgError):
         fvg.Batch(ctx, [A3], [M3], "bicgstab", fvg.LinearSolverOptions(**OPTS))
+
+
+@pytest.mark.parametrize("prec", ["ic", "ilu"])
+def test_refactor_many_gives_the_per_level_factors(ctx, prec):
+    """One launch refactorising several systems gives the bits of the per-level launches
+    (and so the oracle's), and defers the pivot / symmetry checks to status()."""
+    systems = _systems("sym" if prec == "ic" else "nonsym",
+                       _sym_meshes()[:3] if prec == "ic" else _nonsym_meshes(), 100)
+    cls = fvg.IncompleteCholesky if prec == "ic" else fvg.IncompleteLowerUpper
+    A, M, scaled = [], [], []
+    for j, (rp, ci, va, b) in enumerate(systems):
+        a = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+        A.append(a)
+        M.append(cls.from_matrix(a, 0))
+        va2 = va * (2.0 + j)
+        a.set_values(va2)
+        scaled.append((rp, ci, va2))
+    fvg.refactor_many(ctx, M, A)
+    ctx.sync()
+    for m, (rp, ci, va2) in zip(M, scaled):
+        m.status()
+        got = m.factor_values()
+        lrp, lci, lva, linv = O.OraclePrec(prec, rp, ci, va2, 0).factor()
+        ref = cls.from_matrix(fvg.DistributedMatrix.from_csr(ctx, rp, ci, va2), 0).factor_values()
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    if prec == "ic":   # an unsymmetric matrix is reported by status(), not by the launch
+        rp, ci, va, b = systems[0]
+        bad = va.copy()
+        off = np.nonzero(ci != np.repeat(np.arange(rp.shape[0] - 1), np.diff(rp)))[0][0]
+        bad[off] *= 1.5
+        A[0].set_values(bad)
+        fvg.refactor_many(ctx, [M[0]], [A[0]])
+        with pytest.raises(fvg.NotSymmetricError):
+            M[0].status()
+
+
+def test_batch_reports_a_failed_deferred_factorisation(ctx):
+    """A system whose deferred refactorisation failed its check is reported in its info
+    (status) and left unsolved; the other systems of the batch are solved as usual."""
+    systems = _systems("sym", _sym_meshes()[:3], 110)
+    B = _Batch(ctx, systems, "ic", "cg")
+    rp, ci, va, b = systems[1]
+    bad = va.copy()
+    off = np.nonzero(ci != np.repeat(np.arange(rp.shape[0] - 1), np.diff(rp)))[0][0]
+    bad[off] *= 1.5                      # no longer symmetric
+    B.A[1].set_values(bad)
+    B.M[1].refactor_async(B.A[1])
+    for x in B.x:
+        x.fill(0.0)
+    infos = B.batch.solve(B.b, B.x)
+    assert infos[1].status == 5          # FVG_E_NOT_SYMMETRIC
+    assert np.all(B.x[1].numpy() == 0.0)
+    assert infos[0].status == 0 and infos[2].status == 0
+    assert infos[0].final_residual < 1e-8 * infos[0].initial_residual
