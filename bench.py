@@ -51,8 +51,8 @@ def parse():
     ap.add_argument("--legs", default="batched,serving,convdiff",
                     help="extra legs beside the headline solve: batched, serving, convdiff (comma list, or none)")
     ap.add_argument("--batch-systems", type=int, default=256)
-    ap.add_argument("--serve-requests", type=int, default=300)
-    ap.add_argument("--serve-qps", type=float, default=150.0)
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-qps", type=float, default=80.0)
     return ap.parse_args()
 
 
@@ -118,12 +118,18 @@ def leg_serving(args, fvg, workloads):
     """A Poisson stream of mixed-size solves through the continuous-batching server."""
     import serving
     pool = workloads.serving_pool()
-    srv = serving.Server(budget_bytes=200 * 10**9, lanes=4, max_batch=64)
+    srv = serving.Server(budget_bytes=200 * 10**9, lanes=6, max_batch=48)
     for k, raw in pool.items():
         srv.register_mesh(k, raw)
     geo = {k: srv.meshes[k].mesh.geometry() for k in pool}
-    warm = workloads.serving_requests(pool, geo.get, 32, args.serve_qps, seed=11)
-    srv.run(warm)   # first use of every mesh builds its slots (setup, not serving)
+    # resident slots for every mesh and solver kind (setup, not serving), then a warm-up
+    # stream that loads every kernel variant and sizes the staging buffers
+    tmpl = {}
+    for r in workloads.serving_requests(pool, geo.get, 400, args.serve_qps, seed=10):
+        tmpl.setdefault((r.mesh, r.kind), r)
+    for (key, kind), r in tmpl.items():
+        srv.reserve(r, 6 if key.startswith("tet") else 4)
+    srv.run(workloads.serving_requests(pool, geo.get, 96, args.serve_qps, seed=11))
     reqs = workloads.serving_requests(pool, geo.get, args.serve_requests, args.serve_qps, seed=12)
     wall = srv.run(reqs)
     s = serving.latency_summary(reqs, wall)
@@ -134,7 +140,8 @@ def leg_serving(args, fvg, workloads):
             lat = np.array([r.latency for r in sub]) * 1e3
             by[name] = {"count": len(sub), "p50_ms": float(np.percentile(lat, 50)),
                         "p99_ms": float(np.percentile(lat, 99))}
-    s.update({"offered_qps": args.serve_qps, "requests": len(reqs), "by_mesh": by,
+    s.update({"offered_qps": args.serve_qps, "requests": len(reqs), "by_mesh": by, "lanes": len(srv.lanes),
+              "max_batch": srv.max_batch, "resident_slots": len(srv.slots),
               "mean_batch": float(np.mean([r.batch_size for r in reqs])),
               "peak_hbm_bytes": int(srv.peak_bytes), "budget_bytes": srv.budget,
               "worst_true_relative_residual": float(max(r.info.true_residual for r in reqs)),

@@ -130,8 +130,9 @@ __global__ void __launch_bounds__(kScalarBlock) k_mscalar(MState* __restrict__ s
         st->rho = s0;
         if (!(st->rin0 > 0.0)) st->inner_done = 1;
         break;
-    case M_CG_ALPHA:
+    case M_CG_ALPHA:   // an FP32 breakdown ends the inner solve; refinement goes on from d
         st->alpha = st->rz / s0;
+        if (!isfinite(st->alpha)) st->inner_done = 1;
         break;
     case M_CG_ERR:
         st->rin = sqrt(s0);
@@ -145,6 +146,7 @@ __global__ void __launch_bounds__(kScalarBlock) k_mscalar(MState* __restrict__ s
         break;
     case M_BI_ALPHA:
         st->alpha = st->rho / s0;
+        if (!isfinite(st->alpha)) st->inner_done = 1;
         break;
     case M_BI_S: {
         const double ss = sqrt(s0);
@@ -156,8 +158,13 @@ __global__ void __launch_bounds__(kScalarBlock) k_mscalar(MState* __restrict__ s
         }
         break;
     }
-    case M_BI_OMEGA:
+    case M_BI_OMEGA:   // breakdown after h = d + alpha y: keep h
         st->omega = s0 / s1;
+        if (!isfinite(st->omega) || st->omega == 0.0) {
+            st->inner_iter += 1;
+            st->total_inner += 1;
+            st->inner_done = 1;
+        }
         break;
     case M_BI_R: {
         st->rin = sqrt(s1);

@@ -824,7 +824,9 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
                     ap[i] = sv;
                     qa += pi * sv;
                 });
-                const float alpha = (float)(rz / (double)F.sum(qa));
+                const double alpha_d = rz / (double)F.sum(qa);
+                if (!isfinite(alpha_d)) break;   // FP32 breakdown: refine from the correction so far
+                const float alpha = (float)alpha_d;
                 float qr = 0.f;
                 for (int vi = threadIdx.x; vi < nv; vi += NT) {
                     const V4<float> P = ld4(p, vi), AP = ld4(ap, vi);
@@ -875,6 +877,7 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
                     qv += rhi * sv;
                 });
                 alpha = rho / (double)F.sum(qv);
+                if (!isfinite(alpha)) break;     // FP32 breakdown: refine from the correction so far
                 const float af = (float)alpha;
                 float qs = 0.f;
                 for (int vi = threadIdx.x; vi < nv; vi += NT) {
@@ -903,6 +906,10 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
                 });
                 F.sum2(qts, qtt);
                 omega = (double)qts / (double)qtt;
+                if (!isfinite(omega) || omega == 0.0) {   // breakdown after h = d + alpha y: keep h
+                    total += 1;
+                    break;
+                }
                 const float of = (float)omega;
                 float qhr = 0.f, qrr = 0.f;
                 for (int vi = threadIdx.x; vi < nv; vi += NT) {

@@ -55,6 +55,7 @@ class Request:
     bc_e: np.ndarray
     arrival: float = 0.0          # seconds after the start of the run
     admitted: float = -1.0
+    launched: float = -1.0
     done: float = -1.0
     x: np.ndarray | None = None
     info: object = None
@@ -201,6 +202,7 @@ class Server:
         self.finished = []
         self.peak_bytes = 0
         self.evictions = 0
+        self.timeline = []          # one record per batch launch (diagnostics)
         # host seconds spent per activity, and counts (diagnostics)
         self.stats = dict(slot_create_s=0.0, slots_created=0, prepare_s=0.0, batch_setup_s=0.0, batches=0,
                           complete_s=0.0)
@@ -338,9 +340,13 @@ class Server:
         self.stats["prepare_s"] += t1 - t
         self.stats["batch_setup_s"] += time.perf_counter() - t1
         self.stats["batches"] += 1
+        now = self.now()
         for r in group:
             r.batch_size = len(group)
+            r.launched = now
             self.pending.remove(r)
+        self.timeline.append(dict(t=now, n=len(group), kind=kind, lane=self.lanes.index(lane),
+                                  rows=max(s.entry.n for s in slots)))
         lane.batch, lane.reqs, lane.slots = b, list(group), slots
         lane.reported = np.zeros(len(group), bool)
         lane.large = large

@@ -100,3 +100,41 @@ def test_serving_stays_within_the_hbm_budget():
             assert np.linalg.norm(b - S @ r.x) / np.linalg.norm(b) <= 1e-8
     finally:
         srv.close()
+
+
+def test_mixed_bicgstab_recovers_from_an_fp32_breakdown():
+    """A convection-diffusion request of the serving stream whose FP32 inner BiCGStab breaks
+    down (non-finite step) is still answered to a 1e-8 true residual: the inner solve stops
+    before the bad step and the FP64 refinement restarts it, on the batched path and on the
+    single-system path."""
+    import fvg
+    pool = workloads.serving_pool()
+    ctx = fvg.Context(0)
+    meshes = {}
+
+    def geo(k):
+        if k not in meshes:
+            meshes[k] = fvg.Mesh(ctx, pool[k])
+        return meshes[k].geometry()
+
+    reqs = workloads.serving_requests(pool, geo, 600, 80.0, seed=10)
+    r = [q for q in reqs if q.mesh.startswith("tri1_") and q.kind == "bicgstab"][1]
+    rp, ci, va, b = _oracle_system(pool[r.mesh], r)
+    n = rp.shape[0] - 1
+    S = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    mesh = meshes[r.mesh]
+    lhs, rhs = fvg.assemble_program(mesh, r.prog, r.face_fields, r.cell_fields, r.bc_i, r.bc_e)
+    M = fvg.IncompleteLowerUpper.from_matrix(lhs, 0)
+    opts = fvg.LinearSolverOptions(1e-8, 0.0, 5000, precision=fvg.PRECISION_MIXED)
+    x = fvg.DistributedVector(ctx, n)
+    batch = fvg.Batch(ctx, [lhs], [M], "bicgstab", opts)
+    info = batch.solve([rhs], [x])[0]
+    xb = x.numpy()
+    assert np.all(np.isfinite(xb)) and info.iterations < 5000
+    assert np.linalg.norm(b - S @ xb) / np.linalg.norm(b) <= 1e-8
+    x.fill(0.0)
+    info = fvg.bi_conjugate_gradient_stab(x, lhs, rhs, M, opts)
+    xs = x.numpy()
+    assert np.all(np.isfinite(xs))
+    assert np.linalg.norm(b - S @ xs) / np.linalg.norm(b) <= 1e-8
+    batch.close()
