@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-qps", type=float, default=80.0)
+    ap.add_argument("--serve-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
@@ -115,7 +116,25 @@ def leg_batched(args, fvg, workloads):
 
 
 def leg_serving(args, fvg, workloads):
-    """A Poisson stream of mixed-size solves through the continuous-batching server."""
+    """A Poisson stream of mixed-size solves through the continuous-batching server, in a
+    process of its own (its streams then get hardware queues of their own)."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--serve-child", "--serve-requests", str(args.serve_requests),
+           "--serve-qps", str(args.serve_qps)]
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    if res.returncode != 0:
+        raise RuntimeError("serving leg failed: " + res.stderr[-2000:])
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+def serve_child(args):
+    import fvg
+    import workloads
+    out = _serve(args, fvg, workloads)
+    print(json.dumps(out), flush=True)
+
+
+def _serve(args, fvg, workloads):
     import serving
     pool = workloads.serving_pool()
     srv = serving.Server(budget_bytes=200 * 10**9, lanes=6, max_batch=48)
@@ -144,6 +163,7 @@ def leg_serving(args, fvg, workloads):
               "max_batch": srv.max_batch, "resident_slots": len(srv.slots),
               "mean_batch": float(np.mean([r.batch_size for r in reqs])),
               "peak_hbm_bytes": int(srv.peak_bytes), "budget_bytes": srv.budget,
+              "failed": int(sum(1 for r in reqs if r.error is not None or not (r.info.true_residual <= 1e-8))),
               "worst_true_relative_residual": float(max(r.info.true_residual for r in reqs)),
               "workload": "serving: Poisson arrivals; 70% 2D triangle meshes (1k-64k cells), 30% 3D Kuhn tet meshes "
                           "(50k-500k cells) from a pool of 16; 70% diffusion (ln k ~ N(0,1), CG + IC(0)), 30% "
@@ -221,6 +241,9 @@ def cpu_baseline(raw, fields_of):
 
 def main():
     args = parse()
+    if args.serve_child:
+        serve_child(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is not built.",
               file=sys.stderr)

@@ -572,20 +572,40 @@ using namespace fvg;
 
 extern "C" {
 
+// Mixed precision, and when the FP32 iterations cannot reach the tolerance on a system
+// (refinement stalled), the reference FP64 iterations from the refined solution.
+static int solve_mixed(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
+                       const fvg_solver_options* o, fvg_solver_info* info, double* history, int64_t cap, bool bicg) {
+    fvg_solver_info mi{};
+    // the FP32 copies of the matrix values and factors are caches owned by a and p
+    FVG_TRY(run_mixed(c, const_cast<fvg_csr*>(a), const_cast<fvg_prec*>(p), b, x, o, &mi, history, cap, bicg));
+    const bool met = (mi.true_residual < o->absolute_tolerance) ||
+                     (mi.initial_residual > 0.0 && mi.true_residual / mi.initial_residual < o->relative_tolerance) ||
+                     mi.initial_residual == 0.0;
+    if (!met && mi.iterations < o->max_iterations) {
+        fvg_solver_info fi{};
+        FVG_TRY(run_single(c, a, p, b, x, o, &fi, nullptr, 0, bicg));
+        mi.iterations += fi.iterations;
+        mi.final_residual = fi.final_residual;
+        mi.true_residual = fi.final_residual;
+        mi.restarts += fi.restarts;
+        mi.flags |= FVG_INFO_FP64_FALLBACK;
+    }
+    if (info) *info = mi;
+    return FVG_OK;
+}
+
 int fvg_cg(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
            const fvg_solver_options* o, fvg_solver_info* info, double* history, int64_t cap) {
     FVG_TRY(check_solve_args(c, a, p, b, x, o));
-    // the FP32 copies of the matrix values and factors are caches owned by a and p
-    if (o->precision == FVG_PRECISION_MIXED)
-        return run_mixed(c, const_cast<fvg_csr*>(a), const_cast<fvg_prec*>(p), b, x, o, info, history, cap, false);
+    if (o->precision == FVG_PRECISION_MIXED) return solve_mixed(c, a, p, b, x, o, info, history, cap, false);
     return run_single(c, a, p, b, x, o, info, history, cap, false);
 }
 
 int fvg_bicgstab(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
                  const fvg_solver_options* o, fvg_solver_info* info, double* history, int64_t cap) {
     FVG_TRY(check_solve_args(c, a, p, b, x, o));
-    if (o->precision == FVG_PRECISION_MIXED)
-        return run_mixed(c, const_cast<fvg_csr*>(a), const_cast<fvg_prec*>(p), b, x, o, info, history, cap, true);
+    if (o->precision == FVG_PRECISION_MIXED) return solve_mixed(c, a, p, b, x, o, info, history, cap, true);
     return run_single(c, a, p, b, x, o, info, history, cap, true);
 }
 

@@ -443,6 +443,11 @@ int run_mixed(fvg_ctx* c, fvg_csr* a, fvg_prec* p, const fvg_vec* b, fvg_vec* x,
         std::memcpy(&h, c->pinned, sizeof(MState));
         hist.push_back(h.true_res);
         if (h.done) break;
+        // FP32 iterations barely progressing on this system: leave it to the FP64 iterations
+        const size_t m = hist.size();
+        if (m >= 4 && hist[m - 1] > 0.5 * hist[m - 2] && hist[m - 2] > 0.5 * hist[m - 3] &&
+            hist[m - 3] > 0.5 * hist[m - 4])
+            break;
         // inner start: r32 = r64, d = 0, |r32|; CG: z = M r, p = z, r.z; BiCGStab: rh = p = r
         hipLaunchKernelGGL(k_m_start, dim3(nb), dim3(kBlock), 0, c->stream, n, st, r64, r, d, rh, bicgstab ? pp : nullptr,
                            part);

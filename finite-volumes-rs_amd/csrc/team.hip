@@ -241,7 +241,7 @@ __device__ void team_sweep(const TeamSweep& Sd, const T* val, const T* sc, const
                            T* ring) {
     const TeamSweep S = Sd;   // registers: no descriptor reads inside the pipelined loop
     ProfScope prof(S.upper ? 2 : 1);
-    const bool ring_ok = sizeof(T) == 8 ? S.ring64 != 0 : S.ring32 != 0;
+    const bool ring_ok = ring != nullptr && (sizeof(T) == 8 ? S.ring64 != 0 : S.ring32 != 0);
     const bool has_sc = sc != nullptr;
     const int G = (n + NT - 1) / NT;
     SweepRow<T> A, B;
@@ -590,7 +590,7 @@ __device__ void team_info(const TeamDesc& D, int64_t iters, double init, double 
     I.outer_iterations = outer;
     I.true_residual = true_res;
     I.status = FVG_OK;
-    I.reserved = 0;
+    I.flags = 0;
     *D.info = I;
 }
 
@@ -766,9 +766,11 @@ __device__ __forceinline__ double inner_target(const TeamDesc& D, double e, doub
 template <int NT>
 __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT, float>& F, bool bicg,
                            float* ring) {
+    const int tid = threadIdx.x;
     const int n = D.n, np = D.npad, nv = D.npad / 4;
     double* b = D.w64;
     double* x = b + np;
+    double* xa = x + np;  // the last accepted solution
     float* r = D.w32;
     float* d = r + np;
     float* z = d + np;
@@ -783,8 +785,13 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
     if (team_zero_rhs<NT>(D, bnorm, true)) return;
     __syncthreads();
     const int64_t max_inner = D.max_iter > 0 ? D.max_iter : 1;
-    double init_res = 0.0, true_res = 0.0;
+    double init_res = 0.0, true_res = 0.0, e_acc = 0.0;
     int64_t total = 0, outer = 0, restarts = 0, hlen = 0;
+    int64_t cap = 400;     // inner iterations per refinement step; shrinks after a rejected step
+    bool check = false;    // the current x carries a correction not yet accepted
+    int rejected = 0;      // corrections taken back in a row
+    int weak = 0;          // accepted corrections in a row that did not halve the residual
+    bool fallback = false;
     (void)n;
     for (int k = 0; k < 64; ++k) {
         // r64 = b - A x in FP64 (values, products and sum), r32 = (float) r64, d = 0
@@ -797,10 +804,41 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
         });
         const double e = sqrt(R.sum(q)) / bnorm;
         if (outer == 0) init_res = e;
-        true_res = e;
         outer += 1;
+        // refinement keeps only corrections that lower the true residual: an FP32 inner
+        // solve that stagnated or diverged is taken back and retried with a shorter budget
+        if (check && !(e < e_acc)) {
+            for (int i = tid; i < np; i += NT) x[i] = xa[i];
+            __syncthreads();
+            check = false;
+            cap = cap / 4 > 8 ? cap / 4 : 8;
+            if (++rejected >= 2) {   // FP32 iterations make no progress on this system: go on in FP64
+                fallback = true;
+                break;
+            }
+            if (total >= max_inner) break;
+            continue;
+        }
+        if (check) rejected = 0;   // a correction was accepted (not just the restored x re-measured)
+        if (check && e > 0.5 * e_acc) {   // accepted, but the refinement is barely progressing
+            if (++weak >= 3) {
+                e_acc = e;
+                true_res = e;
+                team_push(D, hlen, e);
+                check = false;   // keep this x: it is the best so far
+                fallback = true;
+                break;
+            }
+        } else {
+            weak = 0;
+        }
+        e_acc = e;
+        true_res = e;
+        check = false;
         team_push(D, hlen, e);
         if ((e < D.atol) || (e / init_res < D.rtol) || total >= max_inner) break;
+        for (int i = tid; i < np; i += NT) xa[i] = x[i];
+        const int64_t inner_end = total + cap;
         const double tol = inner_target(D, e, init_res);
         if (!bicg) {
             team_prec<NT, float, kRing32>(D, r, z, ap, ring);   // z = M r; p = z; |r|, r.z
@@ -842,7 +880,8 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
                 }
                 const double rin = sqrt((double)F.sum(qr));
                 total += 1;
-                if (rin <= tol * rin0 || total >= max_inner) break;
+                if (rin <= tol * rin0 || total >= max_inner || total >= inner_end) break;
+                if (!(rin < 1e2 * rin0)) break;   // diverging: let the refinement judge this correction
                 team_prec<NT, float, kRing32>(D, r, z, ap, ring);
                 const double rz_new = (double)F.sum(pass_dot<NT, float>(nv, r, z));
                 const float beta = (float)(rz_new / rz);
@@ -928,7 +967,8 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
                 F.sum2(qhr, qrr);
                 const double rin = sqrt((double)qrr);
                 total += 1;
-                if (rin <= tol * rin0 || total >= max_inner) break;
+                if (rin <= tol * rin0 || total >= max_inner || total >= inner_end) break;
+                if (!(rin < 1e2 * rin0)) break;   // diverging: let the refinement judge this correction
                 const double rho_new = (double)qhr;
                 const double beta = (rho_new / rho) * (alpha / omega);
                 rho = rho_new;
@@ -960,7 +1000,30 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
             x[i] += (double)d[i];
             x[i + 1] += (double)d[i + 1];
         }
+        check = true;
         __syncthreads();
+    }
+    if (check) {   // the loop ended on an unjudged correction (outer-step bound): keep the accepted x
+        for (int i = tid; i < np; i += NT) x[i] = xa[i];
+        __syncthreads();
+    }
+    if (fallback) {
+        // the reference FP64 iterations from the accepted solution (no LDS ring: this kernel's
+        // ring holds FP32 values); its info replaces this one, with the refinement steps kept
+        team_scatter<NT>(D, x);
+        __syncthreads();
+        if (bicg)
+            team_bi64<NT>(D, R, (double*)nullptr);
+        else
+            team_cg64<NT>(D, R, (double*)nullptr);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            D.info->iterations += total;
+            D.info->outer_iterations = outer;
+            D.info->true_residual = D.info->final_residual;
+            D.info->flags |= FVG_INFO_FP64_FALLBACK;
+        }
+        return;
     }
     __syncthreads();
     team_scatter<NT>(D, x);

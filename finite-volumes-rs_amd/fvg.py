@@ -68,7 +68,7 @@ class SolverOptions(C.Structure):
 class SolverInfo(C.Structure):
     _fields_ = [("iterations", C.c_int64), ("initial_residual", C.c_double), ("final_residual", C.c_double),
                 ("history_len", C.c_int64), ("restarts", C.c_int64), ("outer_iterations", C.c_int64),
-                ("true_residual", C.c_double), ("status", C.c_int32), ("reserved", C.c_int32)]
+                ("true_residual", C.c_double), ("status", C.c_int32), ("flags", C.c_int32)]
 
 
 class KernelStats(C.Structure):
@@ -272,8 +272,8 @@ class DistributedVector:
         check(lib().fvg_vec_download(self.ctx.h, self.h, _ptr(out)))
         return out
 
-    def fill(self, value):
-        check(lib().fvg_vec_fill(self.ctx.h, self.h, float(value)))
+    def fill(self, value, ctx=None):
+        check(lib().fvg_vec_fill((ctx or self.ctx).h, self.h, float(value)))
 
     def dot(self, other):
         r = C.c_double()
@@ -574,13 +574,15 @@ class Assembler:
         self._fv = (P * max(1, len(self.fvecs)))(*[v.h for v in self.fvecs])
         self._cv = (P * max(1, len(self.cvecs)))(*[v.h for v in self.cvecs])
 
-    def run(self, lhs, rhs, sync=True):
+    def run(self, lhs, rhs, sync=True, ctx=None):
+        """Assemble on ctx's stream (the mesh's context by default)."""
         m = self.mesh
-        check(lib().fvg_assemble(m.ctx.h, m.h, C.cast(self._prog, P), len(self.ops), C.cast(self._fv, P),
+        c = ctx or m.ctx
+        check(lib().fvg_assemble(c.h, m.h, C.cast(self._prog, P), len(self.ops), C.cast(self._fv, P),
                                  len(self.fvecs), C.cast(self._cv, P), len(self.cvecs),
                                  self.bi.h if self.bi else None, self.be.h if self.be else None, lhs.h, rhs.h))
         if sync:
-            m.ctx.sync()
+            c.sync()
 
 
 def assemble_program(mesh, ops, face_fields, cell_fields, bndi, bnde, lhs=None, rhs=None):
@@ -716,6 +718,7 @@ class LinearSolverInfo:
     outer_iterations: int = 0
     true_residual: float = 0.0
     status: int = 0
+    flags: int = 0
 
     def __str__(self):
         return (f"(solver={self.solver_identifier}, iterations={self.iterations}, "
@@ -730,7 +733,7 @@ def _solve(fn, name, solution, lhs, rhs, preconditioner, options):
     check(fn(lhs.ctx.h, lhs.h, preconditioner.h, rhs.h, solution.h, C.byref(opts), C.byref(info), _ptr(hist), cap))
     hl = min(info.history_len, cap)
     return LinearSolverInfo(name, info.iterations, info.initial_residual, info.final_residual, hist[:hl].copy(),
-                            info.restarts, info.outer_iterations, info.true_residual)
+                            info.restarts, info.outer_iterations, info.true_residual, info.status, info.flags)
 
 
 def conjugate_gradient(solution, lhs, rhs, preconditioner, options=None):
@@ -833,13 +836,15 @@ class Batch:
         check(lib().fvg_batch_phase_times(self.ctx.h, self.h, _ptr(out)))
         return out
 
-    def launch(self, rhs, solutions):
-        """Enqueue the solves (solutions: initial guesses in, answers out); returns at once."""
+    def launch(self, rhs, solutions, ctx=None):
+        """Enqueue the solves (solutions: initial guesses in, answers out); returns at once.
+        The batch starts after the work already enqueued on ctx's stream (the batch's own
+        context by default), where value refreshes are enqueued too."""
         if len(rhs) != self.n or len(solutions) != self.n:
             raise ValueError("one right-hand side and one solution per system")
         bv = (P * self.n)(*[v.h for v in rhs])
         xv = (P * self.n)(*[v.h for v in solutions])
-        check(lib().fvg_batch_launch(self.ctx.h, self.h, bv, xv))
+        check(lib().fvg_batch_launch((ctx or self.ctx).h, self.h, bv, xv))
         self._inflight = (rhs, solutions)
 
     def poll(self):
@@ -852,7 +857,7 @@ class Batch:
         for k in np.nonzero(flags)[0]:
             i = infos[int(k)]
             out[int(k)] = LinearSolverInfo(name, i.iterations, i.initial_residual, i.final_residual, np.zeros(0),
-                                           i.restarts, i.outer_iterations, i.true_residual, i.status)
+                                           i.restarts, i.outer_iterations, i.true_residual, i.status, i.flags)
         return flags.astype(bool), out
 
     def done(self):
@@ -875,7 +880,7 @@ class Batch:
             hl = min(i.history_len, cap)
             out.append(LinearSolverInfo(name, i.iterations, i.initial_residual, i.final_residual,
                                         hists[k][:hl].copy() if history else np.zeros(0), i.restarts,
-                                        i.outer_iterations, i.true_residual, i.status))
+                                        i.outer_iterations, i.true_residual, i.status, i.flags))
         return out
 
     def solve(self, rhs, solutions, history=True):

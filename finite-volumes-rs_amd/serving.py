@@ -24,9 +24,10 @@ Scheduling:
 Each result is independent of what it was batched with (one workgroup solves one system,
 with fixed reduction order), so a request gives the same bits alone and in the mix.
 
-The server drives ``2 * lanes + 2`` streams (a batch stream per lane and solver kind,
-assembly, result copies); with fewer hardware queues than streams (GPU_MAX_HW_QUEUES,
-read when the HIP runtime starts) some of them share a queue and wait on each other.
+The server drives ``3 * lanes + 2`` streams (per lane: preparation and a batch stream per
+solver kind; plus allocation and result copies); with fewer hardware queues than streams
+(GPU_MAX_HW_QUEUES, read when the HIP runtime starts) some of them share a queue and wait
+on each other.
 """
 from __future__ import annotations
 
@@ -148,14 +149,14 @@ class _Slot:
         arrs = list(req.face_fields) + list(req.cell_fields) + ([req.bc_i, req.bc_e] if a.bi is not None else [])
         return vecs, arrs
 
-    def prepare(self):
-        """Assemble on the context stream without waiting (fields uploaded); returns True
-        when the preconditioner needs refactoring."""
+    def prepare(self, ctx):
+        """Assemble on ctx's stream without waiting (fields uploaded); returns True when
+        the preconditioner needs refactoring."""
         fresh = self.fresh
         if not fresh:
-            self.asm.run(self.A, self.b, sync=False)
+            self.asm.run(self.A, self.b, sync=False, ctx=ctx)
         self.fresh = False
-        self.x.fill(0.0)
+        self.x.fill(0.0, ctx=ctx)
         return not fresh
 
     def close(self):
@@ -167,6 +168,7 @@ class _Slot:
 
 @dataclass
 class _Lane:
+    ctx: object = None                            # this lane's stream for request preparation
     batch: object = None                          # the batch in flight, if any
     objs: dict = field(default_factory=dict)      # solver kind -> this lane's reusable fvg.Batch
     reqs: list = field(default_factory=list)
@@ -189,7 +191,9 @@ class Server:
         self.io = fvg.Context(device)            # a second stream for result downloads
         self.budget = int(budget_bytes)
         self.options = options or fvg.LinearSolverOptions(1e-8, 0.0, 5000, precision=fvg.PRECISION_MIXED)
-        self.lanes = [_Lane() for _ in range(lanes)]
+        # each lane prepares its requests (uploads, assembly, refactorisation) on a stream of
+        # its own, so one lane's preparation never waits behind another's
+        self.lanes = [_Lane(ctx=fvg.Context(device)) for _ in range(lanes)]
         self.max_batch = max_batch
         self.large_rows = large_rows
         # systems solving at once: each holds a compute unit, so keep some free for the
@@ -320,9 +324,9 @@ class Server:
             v, a = s.transfers(r)
             vecs += v
             arrs += a
-        fvg.upload_many(self.ctx, vecs, arrs)
-        stale = [s for s in slots if s.prepare()]
-        fvg.refactor_many(self.ctx, [s.M for s in stale], [s.A for s in stale])
+        fvg.upload_many(lane.ctx, vecs, arrs)
+        stale = [s for s in slots if s.prepare(lane.ctx)]
+        fvg.refactor_many(lane.ctx, [s.M for s in stale], [s.A for s in stale])
         t1 = time.perf_counter()
         kind = group[0].kind
         b = lane.objs.get(kind)
@@ -336,7 +340,7 @@ class Server:
             b.rebind([s.A for s in slots], [s.M for s in slots])
         self.stats["rebind_s"] = self.stats.get("rebind_s", 0.0) + time.perf_counter() - t2
         self.peak_bytes = max(self.peak_bytes, self.ctx.bytes_in_use())
-        b.launch([s.b for s in slots], [s.x for s in slots])
+        b.launch([s.b for s in slots], [s.x for s in slots], ctx=lane.ctx)
         self.stats["prepare_s"] += t1 - t
         self.stats["batch_setup_s"] += time.perf_counter() - t1
         self.stats["batches"] += 1
@@ -433,6 +437,7 @@ class Server:
             for b in lane.objs.values():
                 b.close()
             lane.objs = {}
+            lane.ctx.close()
         for s in self.slots:
             s.close()
         self.slots = []

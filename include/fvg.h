@@ -231,8 +231,10 @@ typedef struct fvg_solver_info {
     double true_residual;       /* mixed only: ||b - A x|| / ||b|| in FP64 at exit */
     int32_t status;             /* batched solves: FVG_OK, or why the system was not solved
                                    (FVG_E_ZERO_PIVOT / FVG_E_NOT_SYMMETRIC from its factorisation) */
-    int32_t reserved;
+    int32_t flags;              /* FVG_INFO_FP64_FALLBACK: a mixed solve whose FP32 iterations stalled
+                                   finished with the FP64 iterations from its refined solution */
 } fvg_solver_info;
+enum { FVG_INFO_FP64_FALLBACK = 1 };
 
 /* conjugate_gradient (solvers/conjugate_gradient.rs:12-112).  x is the initial
  * guess on entry and the solution on exit.  history (may be NULL) receives up to
