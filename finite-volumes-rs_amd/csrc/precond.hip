@@ -268,9 +268,19 @@ __device__ __forceinline__ T wait_value(const T* x, int j, int32_t* err) {
     return Bits<T>::value(v);
 }
 
-// POLL: 0 re-poll pending words after a short sleep; 1 re-poll without sleeping;
-// 2 keep two polls in flight, issued a short sleep apart.
-template <class T, int POLL>
+// POLL: 0 re-poll after a short sleep; 1 re-poll without sleeping; 2 re-poll after a
+// longer sleep.  SCALE: the row result is multiplied by scale[i].
+//
+// A row of at most kMaxRowDeps dependencies issues every load it will ever need before
+// its first wait: the column, the coefficient and the dependency word of each entry, and
+// b[i] and scale[i].  Entry indices are clamped into the row (or onto the padding entry
+// that follows the last row, when the row has no entries), so no load is conditional
+// and all of them are in flight together.  Once the last dependency lands, the row is a
+// few multiply-subtracts and one store away from publishing its own value: the chain
+// from one level to the next is one store and one poll, with no load of static data in
+// between.  A re-poll reloads every dependency word; a word that was ready reads the
+// same again, since each row is published once.
+template <class T, int POLL, bool SCALE>
 __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
     k_sweep_persistent(int64_t nchunks, int64_t nwaves, const int32_t* __restrict__ rows,
                        const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
@@ -284,57 +294,46 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
         const int64_t p = c * kChunk + lane;
         const int i = rows[p];
         if (i < 0) continue;
+        const int kb = ptr[p], ke = ptr[p + 1], len = ke - kb;
         T s = b[i];
-        const int kb = ptr[p], ke = ptr[p + 1];
-        if (ke - kb <= kMaxRowDeps) {
-            // load every dependency at once and re-poll all still-pending ones together, so a
-            // row waits one memory round trip after its last dependency lands; then subtract
-            // in ascending column order as the sequential sweep does
-            U w[kMaxRowDeps];
+        const T sc = SCALE ? scale[i] : T(1);
+        if (len <= kMaxRowDeps) {
+            const int last = len > 0 ? ke - 1 : kb;
             int cj[kMaxRowDeps];
+            T a[kMaxRowDeps];
+            U w[kMaxRowDeps];
 #pragma unroll
-            for (int q = 0; q < kMaxRowDeps; ++q)
-                if (kb + q < ke) {
-                    cj[q] = col[kb + q];
-                    w[q] = load_word(x, cj[q]);
-                }
+            for (int q = 0; q < kMaxRowDeps; ++q) {
+                const int k = min(kb + q, last);
+                cj[q] = col[k];
+                a[q] = val[k];
+            }
+#pragma unroll
+            for (int q = 0; q < kMaxRowDeps; ++q) w[q] = load_word(x, cj[q]);
             int spins = 0;
             for (;;) {
                 bool pend = false;
 #pragma unroll
-                for (int q = 0; q < kMaxRowDeps; ++q) pend |= (kb + q < ke) && w[q] == Bits<T>::pending;
+                for (int q = 0; q < kMaxRowDeps; ++q) pend |= (q < len) && w[q] == Bits<T>::pending;
                 if (!pend) break;
                 if (++spins > kSpinLimit ||
                     ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                     __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
-                if (POLL == 2) {
-                    U w2[kMaxRowDeps];
+                if (POLL == 0) __builtin_amdgcn_s_sleep(1);
+                if (POLL == 2) __builtin_amdgcn_s_sleep(4);
 #pragma unroll
-                    for (int q = 0; q < kMaxRowDeps; ++q)
-                        if ((kb + q < ke) && w[q] == Bits<T>::pending) w2[q] = load_word(x, cj[q]);
-                    __builtin_amdgcn_s_sleep(2);
-#pragma unroll
-                    for (int q = 0; q < kMaxRowDeps; ++q)
-                        if ((kb + q < ke) && w[q] == Bits<T>::pending) w[q] = load_word(x, cj[q]);
-#pragma unroll
-                    for (int q = 0; q < kMaxRowDeps; ++q)
-                        if ((kb + q < ke) && w2[q] != Bits<T>::pending && w[q] == Bits<T>::pending) w[q] = w2[q];
-                } else {
-                    if (POLL == 0) __builtin_amdgcn_s_sleep(1);
-#pragma unroll
-                    for (int q = 0; q < kMaxRowDeps; ++q)
-                        if ((kb + q < ke) && w[q] == Bits<T>::pending) w[q] = load_word(x, cj[q]);
-                }
+                for (int q = 0; q < kMaxRowDeps; ++q) w[q] = load_word(x, cj[q]);
             }
+            // subtract in ascending column order, as the sequential sweep does
 #pragma unroll
             for (int q = 0; q < kMaxRowDeps; ++q)
-                if (kb + q < ke) s -= val[kb + q] * Bits<T>::value(w[q]);
+                if (q < len) s -= a[q] * Bits<T>::value(w[q]);
         } else {
             for (int k = kb; k < ke; ++k) s -= val[k] * wait_value(x, col[k], err);
         }
-        const T r = scale ? scale[i] * s : s;
+        const T r = SCALE ? sc * s : s;
         __hip_atomic_store(reinterpret_cast<U*>(x) + i, Bits<T>::bits(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -363,12 +362,12 @@ __global__ void k_scale(int64_t n, const double* __restrict__ d, const double* _
 
 static void free_sweep(fvg_ctx* c, Sweep& s) {
     dev_free_n(c, s.d_lvl_ptr, s.nlevels + 1);
-    dev_free_n(c, s.val32, s.nnz);
+    dev_free_n(c, s.val32, s.nnz + 1);
     dev_free_n(c, s.rows, s.npos);
     dev_free_n(c, s.ptr, s.npos + 1);
-    dev_free_n(c, s.col, s.nnz);
+    dev_free_n(c, s.col, s.nnz + 1);
     dev_free_n(c, s.src, s.nnz);
-    dev_free_n(c, s.val, s.nnz);
+    dev_free_n(c, s.val, s.nnz + 1);
 }
 
 // upper=false: entries j < i; upper=true: entries i < j < n.  src_map (optional)
@@ -424,15 +423,17 @@ static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::ve
         ptr[(size_t)p + 1] = (int32_t)ecol.size();
     }
     sw.nnz = (int64_t)ecol.size();
+    ecol.push_back(0);   // padding entry after the last row: column 0, coefficient 0
+    const double zero = 0.0;
     int s;
     if ((s = dev_alloc_n(c, sw.nlevels + 1, &sw.d_lvl_ptr)) || (s = upload(c, sw.d_lvl_ptr, sw.lvl_ptr.data(),
                                                                            sw.nlevels + 1)))
         return s;
     if ((s = dev_alloc_n(c, sw.npos, &sw.rows)) || (s = dev_alloc_n(c, sw.npos + 1, &sw.ptr)) ||
-        (s = dev_alloc_n(c, sw.nnz, &sw.col)) || (s = dev_alloc_n(c, sw.nnz, &sw.src)) ||
-        (s = dev_alloc_n(c, sw.nnz, &sw.val)) || (s = upload(c, sw.rows, rows.data(), sw.npos)) ||
-        (s = upload(c, sw.ptr, ptr.data(), sw.npos + 1)) || (s = upload(c, sw.col, ecol.data(), sw.nnz)) ||
-        (s = upload(c, sw.src, esrc.data(), sw.nnz)))
+        (s = dev_alloc_n(c, sw.nnz + 1, &sw.col)) || (s = dev_alloc_n(c, sw.nnz, &sw.src)) ||
+        (s = dev_alloc_n(c, sw.nnz + 1, &sw.val)) || (s = upload(c, sw.rows, rows.data(), sw.npos)) ||
+        (s = upload(c, sw.ptr, ptr.data(), sw.npos + 1)) || (s = upload(c, sw.col, ecol.data(), sw.nnz + 1)) ||
+        (s = upload(c, sw.val + sw.nnz, &zero, 1)) || (s = upload(c, sw.src, esrc.data(), sw.nnz)))
         return s;
     return FVG_OK;
 }
@@ -552,7 +553,8 @@ static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a, bool check = tru
 template <class T>
 static int max_coresident_groups(fvg_ctx* c) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent<T, 0>, kChunk * kWavesPerGroup, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persistent<T, 0, true>, kChunk * kWavesPerGroup,
+                                                     0) !=
         hipSuccess)
         return 1;
     const int m = per_cu * c->num_cus;
@@ -572,6 +574,17 @@ static int64_t sweep_waves(const fvg_ctx* c, const Sweep& sw, int max_groups) {
 }
 
 template <class T>
+static const void* sweep_kernel(int64_t poll, bool scaled) {
+    if (scaled)
+        return poll == 1   ? (const void*)k_sweep_persistent<T, 1, true>
+               : poll == 2 ? (const void*)k_sweep_persistent<T, 2, true>
+                           : (const void*)k_sweep_persistent<T, 0, true>;
+    return poll == 1   ? (const void*)k_sweep_persistent<T, 1, false>
+           : poll == 2 ? (const void*)k_sweep_persistent<T, 2, false>
+                       : (const void*)k_sweep_persistent<T, 0, false>;
+}
+
+template <class T>
 static int launch_sweep(fvg_ctx* c, const Sweep& sw, const T* val, const T* scale, const T* b, T* x, int32_t* err,
                         const int32_t* skip) {
     if (sw.n == 0) return FVG_OK;
@@ -582,9 +595,7 @@ static int launch_sweep(fvg_ctx* c, const Sweep& sw, const T* val, const T* scal
     const int64_t groups = nwaves / kWavesPerGroup;
     void* args[] = {&nchunks, &nwaves, (void*)&sw.rows, (void*)&sw.ptr, (void*)&sw.col, (void*)&val, (void*)&scale,
                     (void*)&b, (void*)&x, (void*)&err, (void*)&skip};
-    const void* fn = c->tune_sweep_poll == 1   ? (const void*)k_sweep_persistent<T, 1>
-                     : c->tune_sweep_poll == 2 ? (const void*)k_sweep_persistent<T, 2>
-                                               : (const void*)k_sweep_persistent<T, 0>;
+    const void* fn = sweep_kernel<T>(c->tune_sweep_poll, scale != nullptr);
     FVG_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)groups), dim3(kChunk * kWavesPerGroup), args, 0, c->stream));
     return FVG_OK;
 }
@@ -627,10 +638,9 @@ int ensure_prec32(fvg_ctx* c, fvg_prec* p) {
                        p->d_inv_diag32);
     for (Sweep* sw : {&p->lower, &p->upper}) {
         if (sw->nnz == 0 && sw->n == 0) continue;
-        if (!sw->val32) FVG_TRY(dev_alloc_n(c, sw->nnz, &sw->val32));
-        if (sw->nnz > 0)
-            hipLaunchKernelGGL(k_to_f32, dim3(grid_for(sw->nnz)), dim3(kBlock), 0, c->stream, sw->nnz, sw->val,
-                               sw->val32);
+        if (!sw->val32) FVG_TRY(dev_alloc_n(c, sw->nnz + 1, &sw->val32));
+        hipLaunchKernelGGL(k_to_f32, dim3(grid_for(sw->nnz + 1)), dim3(kBlock), 0, c->stream, sw->nnz + 1, sw->val,
+                           sw->val32);
     }
     FVG_HIP(hipGetLastError());
     p->fac32_version = p->fac_version;

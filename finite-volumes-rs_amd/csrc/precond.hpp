@@ -17,8 +17,10 @@ struct Sweep {
     int64_t* d_lvl_ptr = nullptr;       // device copy
     int32_t* rows = nullptr;            // row of each position, -1 for padding (npos)
     int32_t* ptr = nullptr;             // entry range of each position (npos+1)
+    // col, val and val32 hold nnz + 1 entries: a padding entry (column 0, value 0) follows
+    // the last row, so clamped loads of a row with no entries stay in bounds
     int32_t* col = nullptr;             // column of each entry, ascending within a row
-    int32_t* src = nullptr;             // slot of each entry in the factor value array
+    int32_t* src = nullptr;             // slot of each entry in the factor value array (nnz)
     double* val = nullptr;              // entry values gathered from the factor
     float* val32 = nullptr;             // FP32 copy for the mixed-precision inner solver
 };

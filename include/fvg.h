@@ -65,7 +65,7 @@ int fvg_ctx_stream(const fvg_ctx* ctx, void** stream);
 enum {
     FVG_TUNE_SWEEP_MODE = 1,    /* 0: one persistent launch per triangular sweep; 1: one launch per level */
     FVG_TUNE_SWEEP_WAVES = 2,   /* waves of the persistent sweep (0: about two levels of chunks) */
-    FVG_TUNE_SWEEP_POLL = 3     /* sweep wait loop: 0 sleep between polls, 1 no sleep, 2 two polls in flight */
+    FVG_TUNE_SWEEP_POLL = 3     /* sweep wait loop: 0 short sleep between polls, 1 no sleep, 2 longer sleep */
 };
 int fvg_ctx_set_tuning(fvg_ctx* ctx, int32_t knob, int64_t value);
 int fvg_ctx_get_tuning(const fvg_ctx* ctx, int32_t knob, int64_t* value);
