@@ -256,6 +256,14 @@ int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
         if (value > 2) return fail(FVG_E_INVALID, "poll variant must be 0, 1 or 2");
         c->tune_sweep_poll = value;
         return FVG_OK;
+    case FVG_TUNE_INNER_EXACT:
+        if (value > 1) return fail(FVG_E_INVALID, "inner sweep mode must be 0 or 1");
+        c->tune_inner_exact = value;
+        return FVG_OK;
+    case FVG_TUNE_INNER_SWEEPS:
+        if (value > 64) return fail(FVG_E_INVALID, "at most 64 Jacobi sweeps");
+        c->tune_inner_sweeps = value;
+        return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }
@@ -266,6 +274,8 @@ int fvg_ctx_get_tuning(const fvg_ctx* c, int32_t knob, int64_t* value) {
     case FVG_TUNE_SWEEP_MODE: *value = c->tune_sweep_mode; return FVG_OK;
     case FVG_TUNE_SWEEP_WAVES: *value = c->tune_sweep_waves; return FVG_OK;
     case FVG_TUNE_SWEEP_POLL: *value = c->tune_sweep_poll; return FVG_OK;
+    case FVG_TUNE_INNER_EXACT: *value = c->tune_inner_exact; return FVG_OK;
+    case FVG_TUNE_INNER_SWEEPS: *value = c->tune_inner_sweeps; return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }

@@ -61,6 +61,8 @@ struct fvg_ctx {
     int64_t tune_sweep_mode = 0;       // FVG_TUNE_SWEEP_MODE
     int64_t tune_sweep_waves = 0;      // FVG_TUNE_SWEEP_WAVES
     int64_t tune_sweep_poll = 0;       // FVG_TUNE_SWEEP_POLL
+    int64_t tune_inner_exact = 0;      // FVG_TUNE_INNER_EXACT
+    int64_t tune_inner_sweeps = 0;     // FVG_TUNE_INNER_SWEEPS
     int num_cus = 0;
 };
 

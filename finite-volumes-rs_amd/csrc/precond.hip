@@ -235,6 +235,7 @@ struct Bits<float> {
 };
 
 constexpr int kSpinLimit = 1 << 18;
+constexpr int kInnerSweeps = 2;    // Jacobi sweeps per factor in the FP32 inner solver
 constexpr int kWavesPerGroup = 4;
 constexpr int kMaxRowDeps = 6;     // dependencies loaded together before any wait
 
@@ -336,6 +337,44 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
         const T r = SCALE ? sc * s : s;
         __hip_atomic_store(reinterpret_cast<U*>(x) + i, Bits<T>::bits(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+// ---- FP32 inner preconditioner: Jacobi-iterated triangular solves ----------------------
+// The mixed-precision inner solver may apply the incomplete factor inexactly: the FP64
+// refinement around it corrects whatever the inner iterations leave.  A triangular system
+// T y = v, T = D + N with N strictly triangular, is approximated by m Jacobi sweeps from
+// y0 = D^-1 v, y_{k+1} = D^-1 (v - N y_k): the first m + 1 terms of the Neumann series
+// sum_k (-D^-1 N)^k D^-1 v.  N is nilpotent, so the series is exact after as many terms as
+// the triangular sweep has levels, and a few terms come close when the factor is
+// diagonally dominant.  Each sweep is one fully parallel pass over the factor entries, with
+// no row waiting for another, so the m passes are bound by HBM bandwidth where the exact
+// sweep is bound by the latency of its level-to-level chain.  With the same m for the lower
+// and the upper sweep of IC(0) (whose upper factor is the lower one transposed), the
+// applied operator is P^T P with P the truncated lower series: symmetric positive
+// definite, as CG requires.  (Iterative triangular solves for preconditioning: Chow and
+// Patel, SIAM J. Sci. Comput. 37 (2015); Anzt, Chow and Dongarra, Euro-Par 2015.)
+//
+// One thread per sweep position (rows in level order, padding rows -1); FIRST folds the
+// start y0 = D^-1 v into the first sweep.
+template <bool FIRST, bool SCALE>
+__global__ void __launch_bounds__(256)
+    k_jacobi32(int64_t npos, const int32_t* __restrict__ rows, const int32_t* __restrict__ ptr,
+               const int32_t* __restrict__ col, const float* __restrict__ val, const float* __restrict__ scale,
+               const float* __restrict__ v, const float* __restrict__ yin, float* __restrict__ yout,
+               const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npos) return;
+    const int i = rows[p];
+    if (i < 0) return;
+    float s = v[i];
+    const int ke = ptr[p + 1];
+    for (int k = ptr[p]; k < ke; ++k) {
+        const int j = col[k];
+        const float yj = FIRST ? (SCALE ? scale[j] * v[j] : v[j]) : yin[j];
+        s -= val[k] * yj;
+    }
+    yout[i] = SCALE ? scale[i] * s : s;
 }
 
 __global__ void k_to_f32(int64_t n, const double* __restrict__ s, float* __restrict__ d) {
@@ -450,6 +489,7 @@ static void free_prec(fvg_ctx* c, fvg_prec* p) {
     dev_free_n(c, p->d_flags, 4);
     dev_free_n(c, p->d_inv_diag32, p->n);
     dev_free_n(c, p->d_tmp32, p->n);
+    dev_free_n(c, p->d_tmp32b, p->n);
     free_sweep(c, p->lower);
     free_sweep(c, p->upper);
 }
@@ -634,6 +674,7 @@ int ensure_prec32(fvg_ctx* c, fvg_prec* p) {
     if (p->fac32_version == p->fac_version && p->d_inv_diag32) return FVG_OK;
     if (!p->d_inv_diag32) FVG_TRY(dev_alloc_n(c, p->n, &p->d_inv_diag32));
     if (!p->d_tmp32) FVG_TRY(dev_alloc_n(c, p->n, &p->d_tmp32));
+    if (!p->d_tmp32b) FVG_TRY(dev_alloc_n(c, p->n, &p->d_tmp32b));
     hipLaunchKernelGGL(k_to_f32, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_inv_diag,
                        p->d_inv_diag32);
     for (Sweep* sw : {&p->lower, &p->upper}) {
@@ -647,6 +688,52 @@ int ensure_prec32(fvg_ctx* c, fvg_prec* p) {
     return FVG_OK;
 }
 
+// m Jacobi sweeps of one triangular factor; sweep k writes `odd` for odd k and `even` for
+// even k, so the result lands in `odd` when m is odd and in `even` otherwise
+static void jacobi32(fvg_ctx* c, const Sweep& sw, int m, const float* scale, const float* v, float* odd,
+                     float* even, const int32_t* skip) {
+    if (sw.npos == 0) return;
+    const dim3 grid((unsigned)((sw.npos + 255) / 256)), block(256);
+    const float* prev = nullptr;
+    for (int k = 1; k <= m; ++k) {
+        float* out = (k & 1) ? odd : even;
+        if (k == 1) {
+            if (scale)
+                hipLaunchKernelGGL((k_jacobi32<true, true>), grid, block, 0, c->stream, sw.npos, sw.rows, sw.ptr,
+                                   sw.col, sw.val32, scale, v, nullptr, out, skip);
+            else
+                hipLaunchKernelGGL((k_jacobi32<true, false>), grid, block, 0, c->stream, sw.npos, sw.rows, sw.ptr,
+                                   sw.col, sw.val32, nullptr, v, nullptr, out, skip);
+        } else {
+            if (scale)
+                hipLaunchKernelGGL((k_jacobi32<false, true>), grid, block, 0, c->stream, sw.npos, sw.rows, sw.ptr,
+                                   sw.col, sw.val32, scale, v, prev, out, skip);
+            else
+                hipLaunchKernelGGL((k_jacobi32<false, false>), grid, block, 0, c->stream, sw.npos, sw.rows, sw.ptr,
+                                   sw.col, sw.val32, nullptr, v, prev, out, skip);
+        }
+        prev = out;
+    }
+}
+
+// the FP32 inner solver's factor application: exact level-scheduled sweeps, or m Jacobi
+// sweeps per triangular factor (the default)
+static int apply_factor32(fvg_ctx* c, const fvg_prec* p, const float* lo_scale, const float* r, float* z,
+                          const int32_t* skip) {
+    if (c->tune_inner_exact) {
+        FVG_TRY(sweep32(c, p->lower, lo_scale, r, p->d_tmp32, p->d_flags + 3, skip));
+        return sweep32(c, p->upper, p->d_inv_diag32, p->d_tmp32, z, p->d_flags + 3, skip);
+    }
+    const int m = c->tune_inner_sweeps > 0 ? (int)c->tune_inner_sweeps : kInnerSweeps;
+    // lower: result in tmp32 (m odd) or tmp32b (m even); upper: its last sweep writes z
+    jacobi32(c, p->lower, m, lo_scale, r, p->d_tmp32, p->d_tmp32b, skip);
+    const float* y = (m & 1) ? p->d_tmp32 : p->d_tmp32b;
+    float* other = (m & 1) ? p->d_tmp32b : p->d_tmp32;
+    jacobi32(c, p->upper, m, p->d_inv_diag32, y, (m & 1) ? z : other, (m & 1) ? other : z, skip);
+    FVG_HIP(hipGetLastError());
+    return FVG_OK;
+}
+
 int apply_prec32(fvg_ctx* c, const fvg_prec* p, const float* r, float* z, const int32_t* skip) {
     switch (p->kind) {
     case kIdentity:
@@ -657,11 +744,9 @@ int apply_prec32(fvg_ctx* c, const fvg_prec* p, const float* r, float* z, const 
                            skip);
         break;
     case kIC:
-        FVG_TRY(sweep32(c, p->lower, p->d_inv_diag32, r, p->d_tmp32, p->d_flags + 3, skip));
-        return sweep32(c, p->upper, p->d_inv_diag32, p->d_tmp32, z, p->d_flags + 3, skip);
+        return apply_factor32(c, p, p->d_inv_diag32, r, z, skip);
     case kILU:
-        FVG_TRY(sweep32(c, p->lower, nullptr, r, p->d_tmp32, p->d_flags + 3, skip));
-        return sweep32(c, p->upper, p->d_inv_diag32, p->d_tmp32, z, p->d_flags + 3, skip);
+        return apply_factor32(c, p, nullptr, r, z, skip);
     default:
         return fail(FVG_E_INVALID, "unknown preconditioner");
     }

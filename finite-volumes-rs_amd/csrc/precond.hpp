@@ -47,6 +47,7 @@ struct fvg_prec {
     // FP32 copies used by the mixed-precision inner iterations (made on first use)
     float* d_inv_diag32 = nullptr;
     float* d_tmp32 = nullptr;
+    float* d_tmp32b = nullptr;          // second FP32 temporary (Jacobi-iterated sweeps)
     uint64_t fac_version = 0, fac32_version = 0;
     void* team = nullptr;               // batched-solve plan of this system (team.hip), made on use
     uint64_t pattern_id = 0;            // mesh pattern the analysis was made for (0: not a mesh pattern)

@@ -65,7 +65,13 @@ int fvg_ctx_stream(const fvg_ctx* ctx, void** stream);
 enum {
     FVG_TUNE_SWEEP_MODE = 1,    /* 0: one persistent launch per triangular sweep; 1: one launch per level */
     FVG_TUNE_SWEEP_WAVES = 2,   /* waves of the persistent sweep (0: about two levels of chunks) */
-    FVG_TUNE_SWEEP_POLL = 3     /* sweep wait loop: 0 short sleep between polls, 1 no sleep, 2 longer sleep */
+    FVG_TUNE_SWEEP_POLL = 3,    /* sweep wait loop: 0 short sleep between polls, 1 no sleep, 2 longer sleep */
+    /* FP32 inner iterations of FVG_PRECISION_MIXED (device-wide solves): the incomplete
+     * factor is applied by Jacobi-iterated triangular solves (0, the default) or by the
+     * exact level-scheduled sweeps (1); FVG_TUNE_INNER_SWEEPS Jacobi sweeps per factor
+     * (0: 2).  The FP64 refinement reaches the requested tolerance either way. */
+    FVG_TUNE_INNER_EXACT = 4,
+    FVG_TUNE_INNER_SWEEPS = 5
 };
 int fvg_ctx_set_tuning(fvg_ctx* ctx, int32_t knob, int64_t value);
 int fvg_ctx_get_tuning(const fvg_ctx* ctx, int32_t knob, int64_t* value);
