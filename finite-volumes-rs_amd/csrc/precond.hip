@@ -235,7 +235,6 @@ struct Bits<float> {
 };
 
 constexpr int kSpinLimit = 1 << 18;
-constexpr int kInnerSweeps = 2;    // Jacobi sweeps per factor in the FP32 inner solver
 constexpr int kWavesPerGroup = 4;
 constexpr int kMaxRowDeps = 6;     // dependencies loaded together before any wait
 

@@ -27,6 +27,10 @@ struct Sweep {
 
 enum PrecKind { kIdentity = 0, kDiagonal = 1, kIC = 2, kILU = 3 };
 
+// Jacobi sweeps per triangular factor when FP32 inner iterations apply an incomplete
+// factor inexactly (FVG_TUNE_INNER_SWEEPS = 0)
+constexpr int kInnerSweeps = 2;
+
 }  // namespace fvg
 
 struct fvg_prec {

@@ -1098,6 +1098,8 @@ __global__ void k_team_gather(int64_t n, const int32_t* __restrict__ src, const 
     out32[k] = (float)v;
 }
 
+#include "cluster.hpp"
+
 // ---- host side ---------------------------------------------------------------------------
 
 template <int NT>
@@ -1147,9 +1149,10 @@ struct TeamPlan {
     double* v64 = nullptr;
     float* v32 = nullptr;
     int64_t nv = 0;
-    double* w64 = nullptr;        // work vectors: 10 of npad doubles (FP64 BiCGStab is the largest user)
+    double* w64 = nullptr;        // work vectors: 13 of npad doubles (b, x, x accepted and the ten
+                                  // inner vectors of a multi-workgroup FP64 fallback)
     float* w32 = nullptr;         // 10 of npad floats
-    int64_t nw = 0;
+    int64_t nw = 0, nw32 = 0;
     uint64_t a_ver = ~0ull, p_ver = ~0ull;
     bool busy = false;            // part of a launched batch
 };
@@ -1182,6 +1185,20 @@ struct fvg_batch {
     hipGraphExec_t exec = nullptr;
     int graph_nt = 0, graph_cap = 0;
     bool launched = false;
+    // multi-workgroup solves of mixed-precision batches
+    int cl_rows = 4096;                    // rows per workgroup (0: one workgroup per system)
+    int cl_workers = 0;                    // worker pool (0: as many as can be co-resident)
+    int cl_resident = 1;                   // workgroups of the multi-workgroup kernel that fit the device
+    std::vector<int32_t> cl_groups;        // workgroups of each launch slot
+    std::vector<int32_t> cl_tickets;       // (slot, rank) pairs, largest systems first
+    int32_t* h_cl = nullptr;               // pinned staging: groups, then tickets
+    int32_t* d_cl_groups = nullptr;
+    int2* d_cl_tickets = nullptr;
+    fvg::ClusterHead* d_cl_head = nullptr;
+    fvg::ClusterCtl* d_cl_ctl = nullptr;
+    bool cl_dirty = true;
+    int graph_kind = -1, graph_workers = 0, graph_sweeps = 0;
+    bool cluster() const { return precision == FVG_PRECISION_MIXED && cl_rows > 0; }
 };
 
 namespace fvg {
@@ -1202,6 +1219,12 @@ static void batch_free(fvg_ctx* c, fvg_batch* B) {
     dev_free_n(c, B->d_prof, 4 * B->capacity);
     dev_free_n(c, B->d_desc, B->capacity);
     dev_free_n(c, B->d_count, 1);
+    if (B->h_cl) (void)hipHostFree(B->h_cl);
+    B->h_cl = nullptr;
+    dev_free_n(c, B->d_cl_groups, B->capacity);
+    dev_free_n(c, B->d_cl_tickets, (int64_t)B->capacity * kClMaxGroups);
+    dev_free_n(c, B->d_cl_ctl, B->capacity);
+    dev_free_n(c, B->d_cl_head, 1);
     if (B->ready) (void)hipEventDestroy(B->ready);
     if (B->stream) (void)hipStreamDestroy(B->stream);
 }
@@ -1211,7 +1234,7 @@ static void plan_free(fvg_ctx* c, TeamPlan* t) {
     dev_free_n(c, t->v64, t->nv);
     dev_free_n(c, t->v32, t->nv);
     dev_free_n(c, t->w64, t->nw);
-    dev_free_n(c, t->w32, t->nw);
+    dev_free_n(c, t->w32, t->nw32);
 }
 
 void team_plan_release(fvg_ctx* c, fvg_prec* p) {
@@ -1399,15 +1422,16 @@ static int get_plan(fvg_ctx* c, const fvg_csr* a, fvg_prec* p, TeamPlan** out) {
         L.up.val = L.lo.val + std::max<int64_t>(L.lo.nnz, 1);
         L.sc = L.up.val + std::max<int64_t>(L.up.nnz, 1);
         t->nv = L.sc + L.n;
-        t->nw = 10 * L.npad;
+        t->nw = 13 * L.npad;
+        t->nw32 = 10 * L.npad;
         t->ni = (int64_t)ints.size();
         if ((s = dev_alloc_n(c, t->ni, &t->ints)) == FVG_OK && (s = dev_alloc_n(c, t->nv, &t->v64)) == FVG_OK &&
             (s = dev_alloc_n(c, t->nv, &t->v32)) == FVG_OK && (s = dev_alloc_n(c, t->nw, &t->w64)) == FVG_OK &&
-            (s = dev_alloc_n(c, t->nw, &t->w32)) == FVG_OK)
+            (s = dev_alloc_n(c, t->nw32, &t->w32)) == FVG_OK)
             s = upload(c, t->ints, ints.data(), t->ni);
         // padding elements of the work vectors stay zero for the life of the plan
         if (s == FVG_OK && (hipMemsetAsync(t->w64, 0, sizeof(double) * t->nw, c->stream) != hipSuccess ||
-                            hipMemsetAsync(t->w32, 0, sizeof(float) * t->nw, c->stream) != hipSuccess))
+                            hipMemsetAsync(t->w32, 0, sizeof(float) * t->nw32, c->stream) != hipSuccess))
             s = fail(FVG_E_HIP, "workspace clear failed");
     }
     if (s != FVG_OK) {
@@ -1421,7 +1445,15 @@ static int get_plan(fvg_ctx* c, const fvg_csr* a, fvg_prec* p, TeamPlan** out) {
     return FVG_OK;
 }
 
-static int batch_capture(fvg_batch* B) {
+// worker pool of the multi-workgroup kernel: everything that can be co-resident, or the
+// caller's smaller share
+static int cluster_workers(const fvg_batch* B) {
+    int w = B->cl_resident;
+    if (B->cl_workers > 0) w = std::min(w, B->cl_workers);
+    return std::max(w, 1);
+}
+
+static int batch_capture(fvg_ctx* c, fvg_batch* B) {
     if (B->exec) {
         (void)hipGraphExecDestroy(B->exec);
         B->exec = nullptr;
@@ -1430,10 +1462,32 @@ static int batch_capture(fvg_batch* B) {
         (void)hipGraphDestroy(B->graph);
         B->graph = nullptr;
     }
-    const void* fn = team_kernel_for(B->nt, B->variant);
-    void* args[] = {(void*)&B->d_desc, (void*)&B->d_count};
+    const bool cl = B->cluster();
+    const int workers = cl ? cluster_workers(B) : 0;
+    const int sweeps = c->tune_inner_sweeps > 0 ? (int)c->tune_inner_sweeps : kInnerSweeps;
+    long long tmo = 0;
+    if (cl) {
+        int khz = 0;
+        FVG_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+        tmo = (long long)std::max(khz, 1000) * 1000LL * 10LL;   // a barrier wait gives up after 10 s
+    }
+    const void* fn = cl ? (B->solver == FVG_SOLVER_CG ? (const void*)k_cluster<false> : (const void*)k_cluster<true>)
+                        : team_kernel_for(B->nt, B->variant);
+    int32_t sw = sweeps;
+    void* targs[] = {(void*)&B->d_desc, (void*)&B->d_count};
+    void* cargs[] = {(void*)&B->d_desc, (void*)&B->d_cl_groups, (void*)&B->d_cl_tickets, (void*)&B->d_cl_head,
+                     (void*)&B->d_cl_ctl, (void*)&sw, (void*)&tmo};
     FVG_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-    const hipError_t le = hipLaunchKernel(fn, dim3((unsigned)B->capacity), dim3((unsigned)B->nt), args, 0, B->stream);
+    hipError_t le;
+    if (cl) {
+        hipLaunchKernelGGL(k_cluster_reset, dim3((unsigned)((B->capacity + 255) / 256)), dim3(256), 0, B->stream,
+                           B->d_cl_ctl, B->capacity, B->d_cl_head);
+        le = hipGetLastError();
+        if (le == hipSuccess)
+            le = hipLaunchKernel(fn, dim3((unsigned)workers), dim3(kClThreads), cargs, 0, B->stream);
+    } else {
+        le = hipLaunchKernel(fn, dim3((unsigned)B->capacity), dim3((unsigned)B->nt), targs, 0, B->stream);
+    }
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(B->stream, &g);
     if (le != hipSuccess) {
@@ -1445,6 +1499,9 @@ static int batch_capture(fvg_batch* B) {
     FVG_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
     B->graph_nt = B->nt;
     B->graph_cap = B->capacity;
+    B->graph_kind = cl ? 1 : 0;
+    B->graph_workers = B->cl_workers;
+    B->graph_sweeps = sweeps;
     return FVG_OK;
 }
 
@@ -1496,6 +1553,11 @@ static int batch_reserve(fvg_ctx* c, fvg_batch* B, int cap) {
         B->h_desc = nullptr;
         dev_free_n(c, B->d_prof, 4 * B->capacity);
         dev_free_n(c, B->d_desc, B->capacity);
+        (void)hipHostFree(B->h_cl);
+        B->h_cl = nullptr;
+        dev_free_n(c, B->d_cl_groups, B->capacity);
+        dev_free_n(c, B->d_cl_tickets, (int64_t)B->capacity * kClMaxGroups);
+        dev_free_n(c, B->d_cl_ctl, B->capacity);
     }
     B->capacity = cap;
     B->nhist = B->hcap * cap;
@@ -1508,8 +1570,38 @@ static int batch_reserve(fvg_ctx* c, fvg_batch* B, int cap) {
     std::memset(B->h_done, 0, sizeof(int32_t) * cap);
     FVG_TRY(dev_alloc_n(c, cap, &B->d_desc));
     if (!B->d_count) FVG_TRY(dev_alloc_n(c, 1, &B->d_count));
+    if (hipHostMalloc((void**)&B->h_cl, sizeof(int32_t) * ((size_t)cap * (1 + 2 * kClMaxGroups) + 1),
+                      hipHostMallocDefault) != hipSuccess)
+        return fail(FVG_E_HIP, "pinned batch staging allocation failed");
+    FVG_TRY(dev_alloc_n(c, cap, &B->d_cl_groups));
+    FVG_TRY(dev_alloc_n(c, (int64_t)cap * kClMaxGroups, &B->d_cl_tickets));
+    FVG_TRY(dev_alloc_n(c, cap, &B->d_cl_ctl));
+    if (!B->d_cl_head) FVG_TRY(dev_alloc_n(c, 1, &B->d_cl_head));
     B->desc_dirty = true;
+    B->cl_dirty = true;
     return FVG_OK;
+}
+
+// workgroups and tickets of the multi-workgroup path, by launch slot
+static void cluster_plan(fvg_batch* B) {
+    B->cl_groups.assign((size_t)B->n_inst, 1);
+    B->cl_tickets.clear();
+    if (!B->cluster()) return;
+    for (int slot = 0; slot < B->n_inst; ++slot) {
+        const int64_t n = B->plan[(size_t)B->order[(size_t)slot]]->L.n;
+        const int64_t k = (n + B->cl_rows - 1) / B->cl_rows;
+        // a system's workgroups must all run at once: never more than the launch's pool
+        const int kmax = std::min(kClMaxGroups, std::max(B->cl_workers > 0 ? std::min(B->cl_workers, B->cl_resident)
+                                                                            : B->cl_resident,
+                                                         1));
+        const int K = (int)std::min<int64_t>(std::max<int64_t>(k, 1), kmax);
+        B->cl_groups[(size_t)slot] = K;
+        for (int r = 0; r < K; ++r) {
+            B->cl_tickets.push_back(slot);
+            B->cl_tickets.push_back(r);
+        }
+    }
+    B->cl_dirty = true;
 }
 
 // the systems of the next launches: plans, launch order and descriptors (uploaded at launch)
@@ -1568,6 +1660,7 @@ static int batch_bind(fvg_ctx* c, fvg_batch* B, int32_t n, const fvg_csr* const*
         D.max_iter = o->max_iterations;
     }
     B->desc_dirty = true;
+    cluster_plan(B);
     return FVG_OK;
 }
 
@@ -1590,6 +1683,13 @@ int fvg_batch_create(fvg_ctx* c, int32_t n, const fvg_csr* const* a, const fvg_p
     B->opts = *o;
     B->variant = (solver == FVG_SOLVER_CG ? 0 : 1) + (o->precision == FVG_PRECISION_MIXED ? 2 : 0);
     B->hcap = o->precision == FVG_PRECISION_MIXED ? 64 : o->max_iterations + 2;
+    {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cluster<true>, kClThreads, 0) != hipSuccess ||
+            per_cu <= 0)
+            per_cu = 1;
+        B->cl_resident = per_cu * std::max(c->num_cus, 1);
+    }
     int s = batch_bind(c, B, n, a, p);
     if (s == FVG_OK && hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess)
         s = fail(FVG_E_HIP, "batch stream creation failed");
@@ -1678,11 +1778,31 @@ int fvg_batch_launch(fvg_ctx* c, fvg_batch* B, const fvg_vec* const* b, fvg_vec*
         FVG_HIP(hipMemcpyAsync(B->d_count, cnt, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
         B->desc_dirty = false;
     }
+    if (B->cluster() && B->cl_dirty) {
+        const int nt = (int)(B->cl_tickets.size() / 2);
+        int32_t* g = B->h_cl;
+        int32_t* tk = B->h_cl + B->capacity;
+        std::memcpy(g, B->cl_groups.data(), sizeof(int32_t) * B->cl_groups.size());
+        std::memcpy(tk, B->cl_tickets.data(), sizeof(int32_t) * B->cl_tickets.size());
+        int32_t* cnt = tk + 2 * (size_t)B->capacity * kClMaxGroups;   // last staging word: the ticket count
+        *cnt = nt;
+        FVG_HIP(hipMemcpyAsync(B->d_cl_groups, g, sizeof(int32_t) * B->cl_groups.size(), hipMemcpyHostToDevice,
+                               c->stream));
+        if (nt > 0)
+            FVG_HIP(hipMemcpyAsync(B->d_cl_tickets, tk, sizeof(int32_t) * B->cl_tickets.size(), hipMemcpyHostToDevice,
+                                   c->stream));
+        FVG_HIP(hipMemcpyAsync(&B->d_cl_head->nticket, cnt, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        B->cl_dirty = false;
+    }
     std::memset(B->h_done, 0, sizeof(int32_t) * (size_t)B->n_inst);
     // the batch stream starts after everything enqueued on the context stream so far
     FVG_HIP(hipEventRecord(B->ready, c->stream));
     FVG_HIP(hipStreamWaitEvent(B->stream, B->ready, 0));
-    if (!B->exec || B->graph_nt != B->nt || B->graph_cap != B->capacity) FVG_TRY(batch_capture(B));
+    const int sweeps = c->tune_inner_sweeps > 0 ? (int)c->tune_inner_sweeps : kInnerSweeps;
+    const bool stale = B->cluster() ? (B->graph_kind != 1 || B->graph_workers != B->cl_workers ||
+                                       B->graph_sweeps != sweeps)
+                                    : (B->graph_kind != 0 || B->graph_nt != B->nt);
+    if (!B->exec || stale || B->graph_cap != B->capacity) FVG_TRY(batch_capture(c, B));
     FVG_HIP(hipGraphLaunch(B->exec, B->stream));
     B->launched = true;
     for (TeamPlan* t : B->plan) t->busy = true;
@@ -1774,6 +1894,23 @@ int fvg_batch_phase_times(fvg_ctx* c, fvg_batch* B, double* us) {
     FVG_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
     if (khz <= 0) return fail(FVG_E_HIP, "wall clock rate unavailable");
     for (size_t i = 0; i < t.size(); ++i) us[i] = (double)t[i] * 1e3 / (double)khz;
+    return FVG_OK;
+}
+
+int fvg_batch_set_cluster(fvg_ctx* c, fvg_batch* B, int32_t rows_per_group, int32_t max_workers) {
+    if (!c || !B) return fail(FVG_E_INVALID, "null argument");
+    if (rows_per_group < 0 || max_workers < 0) return fail(FVG_E_INVALID, "negative cluster setting");
+    if (B->launched) return fail(FVG_E_INVALID, "batch in flight");
+    B->cl_rows = rows_per_group;
+    B->cl_workers = max_workers;
+    cluster_plan(B);
+    return FVG_OK;
+}
+
+int fvg_batch_get_cluster(const fvg_batch* B, int32_t* rows_per_group, int32_t* workers) {
+    if (!B) return fail(FVG_E_INVALID, "null argument");
+    if (rows_per_group) *rows_per_group = B->cluster() ? B->cl_rows : 0;
+    if (workers) *workers = B->cluster() ? cluster_workers(B) : 0;
     return FVG_OK;
 }
 

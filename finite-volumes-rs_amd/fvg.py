@@ -150,6 +150,8 @@ def lib():
         "fvg_batch_solve": [P, P, P, P, P, P, C.c_int64],
         "fvg_batch_set_threads": [P, P, C.c_int32],
         "fvg_batch_get_threads": [P, C.POINTER(C.c_int32)],
+        "fvg_batch_set_cluster": [P, P, C.c_int32, C.c_int32],
+        "fvg_batch_get_cluster": [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
         "fvg_batch_set_phase_timing": [P, P, C.c_int32],
         "fvg_batch_phase_times": [P, P, P],
     }
@@ -757,12 +759,13 @@ SOLVER_BICGSTAB = 1
 
 
 class Batch:
-    """Many independent systems solved by one launch, one workgroup per system.
+    """Many independent systems solved by one launch.
 
     ``matrices[k]`` / ``preconditioners[k]`` stay owned by the caller and must outlive
     the batch.  ``solver`` is ``"cg"`` or ``"bicgstab"``; ``options.precision`` selects the
-    FP64 reference algorithm or mixed precision.  Each system's result does not depend on
-    the rest of the batch.
+    FP64 reference algorithm (one workgroup per system) or mixed precision (each system
+    on ceil(n / rows_per_group) workgroups, see set_cluster).  Each system's result does
+    not depend on the rest of the batch.
     """
 
     def __init__(self, ctx, matrices, preconditioners, solver="cg", options=None):
@@ -780,6 +783,22 @@ class Batch:
         check(lib().fvg_batch_create(ctx.h, self.n, am, pm, self.solver, C.byref(opts), C.byref(h)))
         self.h = h
         self._inflight = None
+        self.rows_per_group = 4096 if self.options.precision == PRECISION_MIXED else 0
+        self.max_workers = 0
+
+    def set_cluster(self, rows_per_group=4096, max_workers=0):
+        """Mixed precision: workgroups per system = ceil(n / rows_per_group) (0: one per
+        system); max_workers bounds the resident workgroups of a launch (0: all that fit)."""
+        check(lib().fvg_batch_set_cluster(self.ctx.h, self.h, int(rows_per_group), int(max_workers)))
+        self.rows_per_group = int(rows_per_group) if self.options.precision == PRECISION_MIXED else 0
+        self.max_workers = int(max_workers)
+
+    @property
+    def workers(self):
+        """Workgroups a mixed-precision launch keeps resident (0 for one workgroup per system)."""
+        r, w = C.c_int32(), C.c_int32()
+        check(lib().fvg_batch_get_cluster(self.h, C.byref(r), C.byref(w)))
+        return w.value
 
     def reserve(self, capacity):
         """Size the per-system storage for up to `capacity` systems now."""
@@ -806,24 +825,35 @@ class Batch:
     def threads(self, value):
         check(lib().fvg_batch_set_threads(self.ctx.h, self.h, int(value)))
 
-    def tune(self, rhs, solutions, candidates=(1024, 512, 256), repeats=1):
-        """Autotune the workgroup size per system (rows per tile = threads, one row per
-        thread): time a solve of this batch with each candidate from x = 0 and keep the
-        fastest.  Overwrites the solutions.  Returns {threads: seconds}."""
+    def _time_solve(self, rhs, solutions, repeats):
         import time as _time
+        best = None
+        for _ in range(repeats):
+            for x in solutions:
+                x.fill(0.0)
+            self.ctx.sync()
+            t = _time.perf_counter()
+            self.solve(rhs, solutions, history=False)
+            el = _time.perf_counter() - t
+            best = el if best is None else min(best, el)
+        return best
+
+    def tune(self, rhs, solutions, candidates=None, repeats=1):
+        """Autotune the launch shape: time a solve of this batch from x = 0 with each
+        candidate and keep the fastest.  One workgroup per system (FP64): the workgroup
+        size (rows per tile = threads, one row per thread; candidates 1024, 512, 256).
+        Mixed precision: rows per workgroup of each system's cluster (candidates 2048,
+        4096, 8192, 16384).  Overwrites the solutions.  Returns {candidate: seconds}."""
         times = {}
-        for th in candidates:
+        if self.rows_per_group > 0:
+            for rows in candidates or (2048, 4096, 8192, 16384):
+                self.set_cluster(rows, self.max_workers)
+                times[rows] = self._time_solve(rhs, solutions, repeats)
+            self.set_cluster(min(times, key=times.get), self.max_workers)
+            return times
+        for th in candidates or (1024, 512, 256):
             self.threads = th
-            best = None
-            for _ in range(repeats):
-                for x in solutions:
-                    x.fill(0.0)
-                self.ctx.sync()
-                t = _time.perf_counter()
-                self.solve(rhs, solutions, history=False)
-                el = _time.perf_counter() - t
-                best = el if best is None else min(best, el)
-            times[th] = best
+            times[th] = self._time_solve(rhs, solutions, repeats)
         self.threads = min(times, key=times.get)
         return times
 

@@ -186,7 +186,7 @@ class Server:
     """
 
     def __init__(self, budget_bytes=200 * 10**9, lanes=4, max_batch=128, options=None, large_rows=100_000,
-                 device=0, max_inflight=192, threads=1024):
+                 device=0, max_inflight=192, threads=1024, cluster_rows=8192):
         self.ctx = fvg.Context(device, int(budget_bytes))
         self.io = fvg.Context(device)            # a second stream for result downloads
         self.budget = int(budget_bytes)
@@ -200,6 +200,10 @@ class Server:
         # assembly and factorisation of the next batch
         self.max_inflight = max_inflight
         self.threads = threads
+        # mixed precision: a system runs on ceil(n / cluster_rows) workgroups; each lane's
+        # launches keep at most its share of the device's resident workgroups, so the
+        # lanes' batches can all be resident at once
+        self.cluster_rows = cluster_rows
         self.meshes = {}
         self.slots = []
         self.pending = []
@@ -334,6 +338,8 @@ class Server:
         if b is None:
             b = fvg.Batch(self.ctx, [s.A for s in slots], [s.M for s in slots], kind, self.options)
             b.threads = self.threads
+            if b.rows_per_group > 0:
+                b.set_cluster(self.cluster_rows, max(1, b.workers // len(self.lanes)))
             b.reserve(self.max_batch)
             lane.objs[kind] = b
         else:

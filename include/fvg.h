@@ -290,9 +290,23 @@ int fvg_batch_finish(fvg_ctx* ctx, fvg_batch* batch, fvg_solver_info* infos, dou
 /* launch + finish */
 int fvg_batch_solve(fvg_ctx* ctx, fvg_batch* batch, const fvg_vec* const* b, fvg_vec* const* x,
                     fvg_solver_info* infos, double* const* histories, int64_t history_capacity);
-/* Threads per system (256, 512 or 1024; default 512). */
+/* Threads per system (256, 512 or 1024; default 512) of the one-workgroup-per-system
+ * kernel (FP64 batches, and mixed batches with rows_per_group 0). */
 int fvg_batch_set_threads(fvg_ctx* ctx, fvg_batch* batch, int32_t threads);
 int fvg_batch_get_threads(const fvg_batch* batch, int32_t* threads);
+/* Mixed-precision batches: each system is solved by ceil(n / rows_per_group) workgroups
+ * (at most 64) that meet at device-wide barriers (default 4096 rows per workgroup;
+ * 0 = one workgroup per system, as FP64 batches always are).  The inner iterations then
+ * apply IC(0) / ILU(0) by Jacobi-iterated triangular solves (FVG_TUNE_INNER_SWEEPS of
+ * the context), and an FP32 inner solve that stops progressing is continued in FP64
+ * (FVG_INFO_FP64_FALLBACK).  max_workers bounds the workgroups the launch keeps
+ * resident (0: as many as fit the device; callers running several batches at once give
+ * each a share).  A result depends on its system and its workgroup count only.
+ * No counterpart in the reference. */
+int fvg_batch_set_cluster(fvg_ctx* ctx, fvg_batch* batch, int32_t rows_per_group, int32_t max_workers);
+/* The rows per workgroup in effect (0: one workgroup per system) and the worker pool a
+ * launch uses (a system never gets more workgroups than the pool). */
+int fvg_batch_get_cluster(const fvg_batch* batch, int32_t* rows_per_group, int32_t* workers);
 /* Phase timing inside each system's workgroup (diagnostic; off by default).  After a
  * finished launch, us[4k + j] holds system k's microseconds in SpMV (j=0), lower sweeps
  * (1), upper sweeps (2) and its whole solve (3). */

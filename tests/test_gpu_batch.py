@@ -1,11 +1,15 @@
-"""Batched solves: many independent systems in one launch, one workgroup per system.
+"""Batched solves: many independent systems in one launch.
+
+FP64 batches solve each system with one workgroup; mixed batches spread a system over
+ceil(n / rows_per_group) workgroups (rows_per_group 0: one workgroup per system).
 
 Bars:
 - FP64 batches run the reference algorithm per system. Each system's solution is within
   1e-6 relative of the oracle's at a 1e-8 relative residual, and its iteration count is
   within +-2 of the oracle's.
 - Mixed batches (FP32 inner, FP64 refinement) reach a true relative residual <= 1e-8, and x
-  is within 1e-6 relative of the FP64 oracle solution.
+  is within 1e-6 relative of the FP64 oracle solution, for one and for many workgroups
+  per system.
 - Every system gives the same bits alone and in any mix.
 """
 import numpy as np
@@ -49,7 +53,7 @@ def _nonsym_meshes():
 class _Batch:
     """Device objects for a list of systems plus a batch over them."""
 
-    def __init__(self, ctx, systems, prec, solver, precision=fvg.PRECISION_FP64, order=None):
+    def __init__(self, ctx, systems, prec, solver, precision=fvg.PRECISION_FP64, order=None, rows=None):
         self.systems = systems
         order = list(range(len(systems))) if order is None else order
         self.order = order
@@ -73,6 +77,8 @@ class _Batch:
             self.x.append(fvg.DistributedVector(ctx, rp.shape[0] - 1))
         self.opts = fvg.LinearSolverOptions(**OPTS, precision=precision)
         self.batch = fvg.Batch(ctx, self.A, self.M, solver, self.opts)
+        if rows is not None:
+            self.batch.set_cluster(rows)
 
     def solve(self):
         for x in self.x:
@@ -114,17 +120,19 @@ def test_batch_mixes_preconditioner_kinds(ctx):
     _check_against_oracle(systems, B.solve(), kinds, "cg")
 
 
-@pytest.mark.parametrize("solver,kind,prec,precision", [("cg", "sym", "ic", fvg.PRECISION_FP64),
-                                                        ("cg", "sym", "ic", fvg.PRECISION_MIXED),
-                                                        ("bicgstab", "nonsym", "ilu", fvg.PRECISION_FP64),
-                                                        ("bicgstab", "nonsym", "ilu", fvg.PRECISION_MIXED)])
-def test_batch_result_is_the_same_alone_and_in_the_mix(ctx, solver, kind, prec, precision):
+@pytest.mark.parametrize("solver,kind,prec,precision,rows", [("cg", "sym", "ic", fvg.PRECISION_FP64, None),
+                                                             ("cg", "sym", "ic", fvg.PRECISION_MIXED, 0),
+                                                             ("cg", "sym", "ic", fvg.PRECISION_MIXED, 256),
+                                                             ("bicgstab", "nonsym", "ilu", fvg.PRECISION_FP64, None),
+                                                             ("bicgstab", "nonsym", "ilu", fvg.PRECISION_MIXED, 0),
+                                                             ("bicgstab", "nonsym", "ilu", fvg.PRECISION_MIXED, 256)])
+def test_batch_result_is_the_same_alone_and_in_the_mix(ctx, solver, kind, prec, precision, rows):
     meshes = _sym_meshes() if kind == "sym" else _nonsym_meshes()
     systems = _systems(kind, meshes, 40)
-    full = _Batch(ctx, systems, prec, solver, precision).solve()
-    rev = _Batch(ctx, systems, prec, solver, precision, order=list(reversed(range(len(systems))))).solve()
+    full = _Batch(ctx, systems, prec, solver, precision, rows=rows).solve()
+    rev = _Batch(ctx, systems, prec, solver, precision, order=list(reversed(range(len(systems)))), rows=rows).solve()
     for k in range(len(systems)):
-        alone = _Batch(ctx, [systems[k]], prec, solver, precision).solve()[0]
+        alone = _Batch(ctx, [systems[k]], prec, solver, precision, rows=rows).solve()[0]
         for other in (full[k], rev[k]):
             assert np.array_equal(alone[0], other[0]), k
             assert alone[1].iterations == other[1].iterations
@@ -132,11 +140,13 @@ def test_batch_result_is_the_same_alone_and_in_the_mix(ctx, solver, kind, prec, 
             assert np.array_equal(alone[1].history, other[1].history)
 
 
+@pytest.mark.parametrize("rows", [0, 64, 256, 4096])
 @pytest.mark.parametrize("solver,kind,prec", [("cg", "sym", "ic"), ("bicgstab", "nonsym", "ilu")])
-def test_batch_mixed_precision_reaches_fp64_accuracy(ctx, solver, kind, prec):
+def test_batch_mixed_precision_reaches_fp64_accuracy(ctx, solver, kind, prec, rows):
+    """rows 64 puts the largest systems on 50 workgroups; 4096 gives every system one."""
     meshes = _sym_meshes() if kind == "sym" else _nonsym_meshes()
     systems = _systems(kind, meshes, 50)
-    res = _Batch(ctx, systems, prec, solver, fvg.PRECISION_MIXED).solve()
+    res = _Batch(ctx, systems, prec, solver, fvg.PRECISION_MIXED, rows=rows).solve()
     for k, (rp, ci, va, b) in enumerate(systems):
         n = rp.shape[0] - 1
         xg, ig = res[k]
@@ -206,6 +216,32 @@ def test_batch_zero_rhs_and_errors(ctx):
         fvg.Batch(ctx, [A3], [M3], "bicgstab", fvg.LinearSolverOptions(**OPTS))
 
 
+@pytest.mark.parametrize("rows", [0, 32])
+def test_batch_mixed_zero_rhs_and_preconditioner_kinds(ctx, rows):
+    """Mixed batches: a zero right-hand side gives x = 0 after no iterations, and Jacobi /
+    identity preconditioners solve like IC(0), with one or several workgroups per system."""
+    systems = _systems("sym", [meshgen.tri2d(10, seed=61), meshgen.tri2d(8, seed=62), meshgen.tri2d(9, seed=63),
+                               meshgen.tri2d(7, seed=64)], 60)
+    rp, ci, va, b = systems[1]
+    systems[1] = (rp, ci, va, np.zeros_like(b))
+    kinds = ["ic", "ic", "diagonal", "identity"]
+    B = _Batch(ctx, systems, kinds, "cg", fvg.PRECISION_MIXED, rows=rows)
+    for x in B.x:
+        x.fill(3.0)
+    infos = B.batch.solve(B.b, B.x)
+    assert np.all(B.x[1].numpy() == 0.0)
+    assert infos[1].iterations == 0 and infos[1].status == 0
+    for k in (0, 2, 3):
+        rp, ci, va, b = systems[k]
+        n = rp.shape[0] - 1
+        S = sp.csr_matrix((va, ci, rp), shape=(n, n))
+        xg = B.x[k].numpy()
+        assert infos[k].status == 0 and infos[k].iterations > 0
+        # the tolerance is relative to the initial residual (x0 = 3)
+        r0 = np.linalg.norm(b - S @ np.full(n, 3.0))
+        assert np.linalg.norm(b - S @ xg) <= 1e-8 * r0, k
+
+
 @pytest.mark.parametrize("prec", ["ic", "ilu"])
 def test_refactor_many_gives_the_per_level_factors(ctx, prec):
     """One launch refactorising several systems gives the bits of the per-level launches
@@ -240,11 +276,12 @@ def test_refactor_many_gives_the_per_level_factors(ctx, prec):
             M[0].status()
 
 
-def test_batch_reports_a_failed_deferred_factorisation(ctx):
+@pytest.mark.parametrize("precision,rows", [(fvg.PRECISION_FP64, None), (fvg.PRECISION_MIXED, 256)])
+def test_batch_reports_a_failed_deferred_factorisation(ctx, precision, rows):
     """A system whose deferred refactorisation failed its check is reported in its info
     (status) and left unsolved; the other systems of the batch are solved as usual."""
     systems = _systems("sym", _sym_meshes()[:3], 110)
-    B = _Batch(ctx, systems, "ic", "cg")
+    B = _Batch(ctx, systems, "ic", "cg", precision, rows=rows)
     rp, ci, va, b = systems[1]
     bad = va.copy()
     off = np.nonzero(ci != np.repeat(np.arange(rp.shape[0] - 1), np.diff(rp)))[0][0]
@@ -257,4 +294,7 @@ def test_batch_reports_a_failed_deferred_factorisation(ctx):
     assert infos[1].status == 5          # FVG_E_NOT_SYMMETRIC
     assert np.all(B.x[1].numpy() == 0.0)
     assert infos[0].status == 0 and infos[2].status == 0
-    assert infos[0].final_residual < 1e-8 * infos[0].initial_residual
+    if precision == fvg.PRECISION_FP64:
+        assert infos[0].final_residual < 1e-8 * infos[0].initial_residual
+    else:   # mixed reports residuals relative to |b|
+        assert infos[0].true_residual <= 1e-8
