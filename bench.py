@@ -24,9 +24,6 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-# the serving leg keeps several streams busy at once (lanes, assembly, result copies):
-# give each its own hardware queue (read when the HIP runtime starts)
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 sys.path.insert(0, os.path.join(ROOT, "finite-volumes-rs_amd"))
 
 HBM_PEAK_GBS = 8000.0
@@ -54,12 +51,16 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-qps", type=float, default=80.0)
     ap.add_argument("--serve-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain device copy of known size (scales rocprof FETCH_SIZE / WRITE_SIZE "
+                         "counters of the same run)")
     return ap.parse_args()
 
 
 def leg_batched(args, fvg, workloads):
     """256 independent 2D diffusion systems (cells log-uniform in [1k, 64k]) solved by one
-    graph-launched batch: FP64 and mixed precision, workgroup size autotuned."""
+    graph-launched batch: FP64 (one workgroup per system, workgroup size autotuned) and
+    mixed precision (each system on ceil(n / rows) workgroups, rows per workgroup autotuned)."""
     inst = workloads.batched_2d(args.batch_systems)
     ctx = fvg.Context(0)
     A, M, B, X, ns = [], [], [], [], []
@@ -97,8 +98,10 @@ def leg_batched(args, fvg, workloads):
             worst = max(worst, float(np.linalg.norm(bh - y.numpy()) / np.linalg.norm(bh)))
         it = np.array([i.iterations for i in infos])
         best = min(times)
-        res[name] = {"solves_per_s": len(A) / best, "ms_per_batch": 1e3 * best,
-                     "threads_per_system": bt.threads, "autotune_ms": {str(k): 1e3 * v for k, v in tune.items()},
+        shape = ({"rows_per_workgroup": bt.rows_per_group, "resident_workgroups": bt.workers}
+                 if bt.rows_per_group > 0 else {"threads_per_system": bt.threads})
+        res[name] = {"solves_per_s": len(A) / best, "ms_per_batch": 1e3 * best, "launch_shape": shape,
+                     "autotune_ms": {str(k): 1e3 * v for k, v in tune.items()},
                      "iterations_max": int(it.max()), "iterations_mean": float(it.mean()),
                      "worst_true_relative_residual": worst}
         bt.close()
@@ -109,8 +112,8 @@ def leg_batched(args, fvg, workloads):
     res["total_cells"] = int(sum(ns))
     res["largest_system"] = int(max(ns))
     res["workload"] = ("batched_2d: 256 jittered triangle meshes, cells log-uniform in [1k, 64k], ln k ~ N(0,1) "
-                       "per cell, f ~ N(0,1), u = 0; PCG + IC(0) to 1e-8 rel. residual, x0 = 0; one launch, "
-                       "one workgroup per system")
+                       "per cell, f ~ N(0,1), u = 0; PCG + IC(0) to 1e-8 rel. residual, x0 = 0; one graph "
+                       "launch per batch")
     ctx.close()
     return res
 
@@ -121,7 +124,12 @@ def leg_serving(args, fvg, workloads):
     import subprocess
     cmd = [sys.executable, os.path.abspath(__file__), "--serve-child", "--serve-requests", str(args.serve_requests),
            "--serve-qps", str(args.serve_qps)]
-    res = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    # the server keeps several streams busy at once (per lane: preparation and batches;
+    # allocation, result copies): give each a hardware queue of its own (read when the HIP
+    # runtime starts)
+    env = dict(os.environ)
+    env["GPU_MAX_HW_QUEUES"] = "32"
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=600, env=env)
     if res.returncode != 0:
         raise RuntimeError("serving leg failed: " + res.stderr[-2000:])
     return json.loads(res.stdout.strip().splitlines()[-1])
@@ -152,6 +160,12 @@ def _serve(args, fvg, workloads):
     reqs = workloads.serving_requests(pool, geo.get, args.serve_requests, args.serve_qps, seed=12)
     wall = srv.run(reqs)
     s = serving.latency_summary(reqs, wall)
+    # the same mix offered at three times the rate: latency near capacity
+    hi = workloads.serving_requests(pool, geo.get, args.serve_requests, 3 * args.serve_qps, seed=13)
+    hwall = srv.run(hi)
+    hs = serving.latency_summary(hi, hwall)
+    hs["offered_qps"] = 3 * args.serve_qps
+    hs["failed"] = int(sum(1 for r in hi if r.error is not None or not (r.info.true_residual <= 1e-8)))
     by = {}
     for name, sel in (("tri", lambda r: r.mesh.startswith("tri")), ("tet", lambda r: r.mesh.startswith("tet"))):
         sub = [r for r in reqs if sel(r)]
@@ -159,12 +173,18 @@ def _serve(args, fvg, workloads):
             lat = np.array([r.latency for r in sub]) * 1e3
             by[name] = {"count": len(sub), "p50_ms": float(np.percentile(lat, 50)),
                         "p99_ms": float(np.percentile(lat, 99))}
+    wait = np.array([r.launched - r.arrival for r in reqs]) * 1e3
+    run = np.array([r.done - r.launched for r in reqs]) * 1e3
+    s["queue_ms"] = {"p50": float(np.percentile(wait, 50)), "p99": float(np.percentile(wait, 99))}
+    s["solve_ms"] = {"p50": float(np.percentile(run, 50)), "p99": float(np.percentile(run, 99))}
     s.update({"offered_qps": args.serve_qps, "requests": len(reqs), "by_mesh": by, "lanes": len(srv.lanes),
               "max_batch": srv.max_batch, "resident_slots": len(srv.slots),
               "mean_batch": float(np.mean([r.batch_size for r in reqs])),
               "peak_hbm_bytes": int(srv.peak_bytes), "budget_bytes": srv.budget,
               "failed": int(sum(1 for r in reqs if r.error is not None or not (r.info.true_residual <= 1e-8))),
               "worst_true_relative_residual": float(max(r.info.true_residual for r in reqs)),
+              "fp64_fallbacks": int(sum(1 for r in reqs if r.info.flags & 1)),
+              "high_load": hs,
               "workload": "serving: Poisson arrivals; 70% 2D triangle meshes (1k-64k cells), 30% 3D Kuhn tet meshes "
                           "(50k-500k cells) from a pool of 16; 70% diffusion (ln k ~ N(0,1), CG + IC(0)), 30% "
                           "convection-diffusion (cell Peclet U[1,50], BiCGStab + ILU(0)); mixed precision to 1e-8; "
@@ -221,6 +241,17 @@ def leg_convdiff(args, fvg, workloads):
     return out
 
 
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` on this workload, from the rocprof FETCH_SIZE /
+    WRITE_SIZE passes summarised in profiles/ (scaled by the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_diffusion_3d.json")
+    if args.nx != 64 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f)["kernels"].get(kernel)
+    return None if k is None else k["traffic_bytes"]
+
+
 def cpu_baseline(raw, fields_of):
     """One full solve of the same inputs by the oracle on one host core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -239,15 +270,41 @@ def cpu_baseline(raw, fields_of):
                        f"{info['iterations']} iterations in {t:.2f} s; mesh geometry ({t_mesh:.2f} s) excluded")
 
 
+def calibrate_copy():
+    """Five launches of an elementwise kernel that reads and writes 512 MiB each: the
+    HBM bytes a hardware counter should report per launch are known exactly."""
+    import torch
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print("bench: calibration copy " + json.dumps({"kernel": "torch elementwise mul (copy)", "bytes_read": 4 * n,
+                                                   "bytes_written": 4 * n, "launches": 5}), file=sys.stderr, flush=True)
+    del a, b
+    torch.cuda.empty_cache()
+
+
 def main():
     args = parse()
     if args.serve_child:
         serve_child(args)
         return
+    if args.calibrate_copy:
+        calibrate_copy()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is not built.",
               file=sys.stderr)
         sys.exit(2)
+    legs = [l for l in args.legs.split(",") if l and l != "none"]
+    serving_out = None
+    if "serving" in legs:
+        # first, while this process holds no GPU queues, so that the server's streams get
+        # hardware queues of their own
+        print("bench: serving leg", file=sys.stderr, flush=True)
+        serving_out = leg_serving(args, None, None)
     import fvg
     import workloads
 
@@ -344,7 +401,7 @@ def main():
         "true_relative_residual": true_rel,
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["gbs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(dom["kernel"], args),
                      "algorithmic_bytes": dom["bytes"], "avg_ms": dom["ms"]},
         "mixed": {"solves_per_s": args.steps / m_elapsed, "ms_per_step": 1e3 * m_elapsed / args.steps,
                   "speedup_vs_fp64": (args.steps / m_elapsed) / value, "inner_iterations": minfos[-1].iterations,
@@ -358,11 +415,12 @@ def main():
                           "total_ms": ks["sweep_ms"], "levels": [lo_lv, up_lv]},
         },
     }
-    legs = [l for l in args.legs.split(",") if l and l != "none"]
-    for name, fn in (("batched", leg_batched), ("serving", leg_serving), ("convdiff", leg_convdiff)):
+    for name, fn in (("batched", leg_batched), ("convdiff", leg_convdiff)):
         if name in legs:
             print(f"bench: {name} leg", file=sys.stderr, flush=True)
             out[name] = fn(args, fvg, workloads)
+    if serving_out is not None:
+        out["serving"] = serving_out
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(raw, fields_of)
     print(json.dumps(out), flush=True)

@@ -268,8 +268,8 @@ __device__ __forceinline__ T wait_value(const T* x, int j, int32_t* err) {
     return Bits<T>::value(v);
 }
 
-// POLL: 0 re-poll after a short sleep; 1 re-poll without sleeping; 2 re-poll after a
-// longer sleep.  SCALE: the row result is multiplied by scale[i].
+// POLL: 0 re-poll after a short sleep; 1 re-poll without sleeping; 2 keep two polls in
+// flight.  SCALE: the row result is multiplied by scale[i].
 //
 // A row of at most kMaxRowDeps dependencies issues every load it will ever need before
 // its first wait: the column, the coefficient and the dependency word of each entry, and
@@ -310,21 +310,52 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
             }
 #pragma unroll
             for (int q = 0; q < kMaxRowDeps; ++q) w[q] = load_word(x, cj[q]);
-            int spins = 0;
-            for (;;) {
+            auto pending = [&](const U* v) {
                 bool pend = false;
 #pragma unroll
-                for (int q = 0; q < kMaxRowDeps; ++q) pend |= (q < len) && w[q] == Bits<T>::pending;
-                if (!pend) break;
+                for (int q = 0; q < kMaxRowDeps; ++q) pend |= (q < len) && v[q] == Bits<T>::pending;
+                return pend;
+            };
+            int spins = 0;
+            auto give_up = [&]() {
                 if (++spins > kSpinLimit ||
                     ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                     __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
+                    return true;
                 }
-                if (POLL == 0) __builtin_amdgcn_s_sleep(1);
-                if (POLL == 2) __builtin_amdgcn_s_sleep(4);
+                return false;
+            };
+            if (POLL == 2) {
+                // two polls in flight, issued about half a round trip apart: each check
+                // waits only for the older poll, so a dependency that lands is seen
+                // sooner than with one poll per round trip
+                if (pending(w)) {
+                    U v[kMaxRowDeps];
 #pragma unroll
-                for (int q = 0; q < kMaxRowDeps; ++q) w[q] = load_word(x, cj[q]);
+                    for (int q = 0; q < kMaxRowDeps; ++q) w[q] = load_word(x, cj[q]);
+                    __builtin_amdgcn_s_sleep(6);
+#pragma unroll
+                    for (int q = 0; q < kMaxRowDeps; ++q) v[q] = load_word(x, cj[q]);
+                    for (;;) {
+                        if (!pending(w) || give_up()) break;
+#pragma unroll
+                        for (int q = 0; q < kMaxRowDeps; ++q) w[q] = load_word(x, cj[q]);
+                        if (!pending(v)) {
+#pragma unroll
+                            for (int q = 0; q < kMaxRowDeps; ++q) w[q] = v[q];
+                            break;
+                        }
+#pragma unroll
+                        for (int q = 0; q < kMaxRowDeps; ++q) v[q] = load_word(x, cj[q]);
+                    }
+                }
+            } else {
+                for (;;) {
+                    if (!pending(w) || give_up()) break;
+                    if (POLL == 0) __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                    for (int q = 0; q < kMaxRowDeps; ++q) w[q] = load_word(x, cj[q]);
+                }
             }
             // subtract in ascending column order, as the sequential sweep does
 #pragma unroll
@@ -623,11 +654,11 @@ static const void* sweep_kernel(int64_t poll, bool scaled) {
                        : (const void*)k_sweep_persistent<T, 0, false>;
 }
 
+// the persistent sweep proper; its output must already hold the pending pattern
 template <class T>
 static int launch_sweep(fvg_ctx* c, const Sweep& sw, const T* val, const T* scale, const T* b, T* x, int32_t* err,
                         const int32_t* skip) {
     if (sw.n == 0) return FVG_OK;
-    hipLaunchKernelGGL(k_fill_pending<T>, dim3(grid_for(sw.n)), dim3(kBlock), 0, c->stream, sw.n, x, skip);
     int64_t nchunks = sw.npos / kChunk;
     int64_t nwaves = sweep_waves(c, sw, max_coresident_groups<T>(c));
     nwaves = std::min(nwaves, (nchunks + kWavesPerGroup - 1) / kWavesPerGroup * kWavesPerGroup);
@@ -641,29 +672,34 @@ static int launch_sweep(fvg_ctx* c, const Sweep& sw, const T* val, const T* scal
 
 static int sweep(fvg_ctx* c, const Sweep& sw, const double* scale, const double* b, double* x, int32_t* err,
                  const int32_t* skip) {
-    const int tok = timing_begin(c);
-    int64_t launches = 0;
-    if (c->tune_sweep_mode == 1) {
-        for (int l = 0; l < sw.nlevels; ++l) {
-            const int64_t lb = sw.lvl_ptr[(size_t)l], le = sw.lvl_ptr[(size_t)l + 1];
-            hipLaunchKernelGGL(k_sweep_level, dim3(grid_for(le - lb)), dim3(kBlock), 0, c->stream, lb, le, sw.rows,
-                               sw.ptr, sw.col, sw.val, scale, b, x, skip);
-        }
-        launches = sw.nlevels;
-    } else {
+    if (sw.n == 0) return FVG_OK;
+    if (c->tune_sweep_mode != 1) {
+        // the pending fill, then the timed span holds the persistent sweep kernel alone
+        hipLaunchKernelGGL(k_fill_pending<double>, dim3(grid_for(sw.n)), dim3(kBlock), 0, c->stream, sw.n, x, skip);
+        const int tok = timing_begin(c);
         FVG_TRY(launch_sweep<double>(c, sw, sw.val, scale, b, x, err, skip));
-        launches = 2;
+        timing_end(c, tok, kSpanSweep, 1);
+        FVG_HIP(hipGetLastError());
+        return FVG_OK;
     }
-    timing_end(c, tok, kSpanSweep, launches);
+    const int tok = timing_begin(c);
+    for (int l = 0; l < sw.nlevels; ++l) {
+        const int64_t lb = sw.lvl_ptr[(size_t)l], le = sw.lvl_ptr[(size_t)l + 1];
+        hipLaunchKernelGGL(k_sweep_level, dim3(grid_for(le - lb)), dim3(kBlock), 0, c->stream, lb, le, sw.rows,
+                           sw.ptr, sw.col, sw.val, scale, b, x, skip);
+    }
+    timing_end(c, tok, kSpanSweep, sw.nlevels);
     FVG_HIP(hipGetLastError());
     return FVG_OK;
 }
 
 static int sweep32(fvg_ctx* c, const Sweep& sw, const float* scale, const float* b, float* x, int32_t* err,
                    const int32_t* skip) {
+    if (sw.n == 0) return FVG_OK;
+    hipLaunchKernelGGL(k_fill_pending<float>, dim3(grid_for(sw.n)), dim3(kBlock), 0, c->stream, sw.n, x, skip);
     const int tok = timing_begin(c);
     FVG_TRY(launch_sweep<float>(c, sw, sw.val32, scale, b, x, err, skip));
-    timing_end(c, tok, kSpanSweep, 2);
+    timing_end(c, tok, kSpanSweep, 1);
     FVG_HIP(hipGetLastError());
     return FVG_OK;
 }

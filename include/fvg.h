@@ -65,7 +65,7 @@ int fvg_ctx_stream(const fvg_ctx* ctx, void** stream);
 enum {
     FVG_TUNE_SWEEP_MODE = 1,    /* 0: one persistent launch per triangular sweep; 1: one launch per level */
     FVG_TUNE_SWEEP_WAVES = 2,   /* waves of the persistent sweep (0: about two levels of chunks) */
-    FVG_TUNE_SWEEP_POLL = 3,    /* sweep wait loop: 0 short sleep between polls, 1 no sleep, 2 longer sleep */
+    FVG_TUNE_SWEEP_POLL = 3,    /* sweep wait loop: 0 short sleep between polls, 1 no sleep, 2 two polls in flight */
     /* FP32 inner iterations of FVG_PRECISION_MIXED (device-wide solves): the incomplete
      * factor is applied by Jacobi-iterated triangular solves (0, the default) or by the
      * exact level-scheduled sweeps (1); FVG_TUNE_INNER_SWEEPS Jacobi sweeps per factor
@@ -236,9 +236,11 @@ typedef struct fvg_solver_info {
     int64_t outer_iterations;   /* mixed only: refinement steps */
     double true_residual;       /* mixed only: ||b - A x|| / ||b|| in FP64 at exit */
     int32_t status;             /* batched solves: FVG_OK, or why the system was not solved
-                                   (FVG_E_ZERO_PIVOT / FVG_E_NOT_SYMMETRIC from its factorisation) */
+                                   (FVG_E_ZERO_PIVOT / FVG_E_NOT_SYMMETRIC from its factorisation,
+                                   FVG_E_TIMEOUT when a multi-workgroup barrier wait expired) */
     int32_t flags;              /* FVG_INFO_FP64_FALLBACK: a mixed solve whose FP32 iterations stalled
-                                   finished with the FP64 iterations from its refined solution */
+                                   needed FP64 iterations (the FP64 solver from the refined solution,
+                                   or FP64 inner iterations in multi-workgroup batches) */
 } fvg_solver_info;
 enum { FVG_INFO_FP64_FALLBACK = 1 };
 
