@@ -14,6 +14,7 @@ namespace fvg {
 
 constexpr int kSpmvRows = 256;     // rows (and threads) per workgroup
 constexpr int kSpmvStage = 3072;   // staged products per workgroup (24 KiB of LDS)
+constexpr int kSpmvUnroll = 6;     // products per thread whose loads are in flight together
 
 int spmv_blocks(int64_t nrows) { return grid_for(nrows, kSpmvRows); }
 
@@ -37,19 +38,39 @@ __global__ void __launch_bounds__(kSpmvRows) k_spmv(int n, const int32_t* __rest
                                                     int64_t pstride) {
     __shared__ double prod[kSpmvStage];
     __shared__ double red[kSpmvRows / 64];
-    if (EPI && epi.skip && epi.skip[(epi.blk_inst ? epi.blk_inst[blockIdx.x] : 0) * epi.skip_stride]) return;
-    const int r0 = blockIdx.x * kSpmvRows;
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);
+    const int r0 = blk * kSpmvRows;
     const int r1 = min(r0 + kSpmvRows, n);
+    // the tile's and this thread's row bounds are loaded together with the skip flag
     const int base = rp[r0];
     const int cnt = rp[r1] - base;
     const int r = r0 + (int)threadIdx.x;
+    const int rc = min(r, r1 - 1);
+    const int rb = rp[rc], re = rp[rc + 1];
+    if (EPI && epi.skip && epi.skip[(epi.blk_inst ? epi.blk_inst[blk] : 0) * epi.skip_stride]) return;
     double sum = 0.0;
     if (cnt <= kSpmvStage) {
-        for (int k = threadIdx.x; k < cnt; k += kSpmvRows) prod[k] = v[base + k] * x[ci[base + k]];
+        // every thread issues all its entry loads, then all its gathers, before using any
+        // (indices clamped into the block's range, so no load is conditional)
+        for (int k0 = threadIdx.x; k0 < cnt; k0 += kSpmvRows * kSpmvUnroll) {
+            int c[kSpmvUnroll];
+            double a[kSpmvUnroll], xv[kSpmvUnroll];
+#pragma unroll
+            for (int u = 0; u < kSpmvUnroll; ++u) {
+                const int k = min(k0 + u * kSpmvRows, cnt - 1);
+                c[u] = __builtin_nontemporal_load(ci + base + k);   // streamed once: keep x in L2
+                a[u] = __builtin_nontemporal_load(v + base + k);
+            }
+#pragma unroll
+            for (int u = 0; u < kSpmvUnroll; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < kSpmvUnroll; ++u)
+                if (k0 + u * kSpmvRows < cnt) prod[k0 + u * kSpmvRows] = a[u] * xv[u];
+        }
         __syncthreads();
         if (r < r1) {
-            const int e = rp[r + 1] - base;
-            for (int k = rp[r] - base; k < e; ++k) sum += prod[k];
+            const int e = re - base;
+            for (int k = rb - base; k < e; ++k) sum += prod[k];
         }
     } else if (r < r1) {
         const int e = rp[r + 1];
@@ -71,15 +92,15 @@ __global__ void __launch_bounds__(kSpmvRows) k_spmv(int n, const int32_t* __rest
     }
     if (epi.dotw) {
         d0 = block_sum<kSpmvRows>(d0, red);
-        if (threadIdx.x == 0) partials[blockIdx.x] = d0;
+        if (threadIdx.x == 0) partials[blk] = d0;
     }
     if (epi.dot_self) {
         d1 = block_sum<kSpmvRows>(d1, red);
-        if (threadIdx.x == 0) partials[pstride + blockIdx.x] = d1;
+        if (threadIdx.x == 0) partials[pstride + blk] = d1;
     }
     if (epi.dotw2) {
         d2 = block_sum<kSpmvRows>(d2, red);
-        if (threadIdx.x == 0) partials[2 * pstride + blockIdx.x] = d2;
+        if (threadIdx.x == 0) partials[2 * pstride + blk] = d2;
     }
 }
 

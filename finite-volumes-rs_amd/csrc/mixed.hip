@@ -24,6 +24,7 @@ constexpr int kScalarBlock = 1024;   // threads of the one-workgroup scalar kern
 
 constexpr int kRows32 = 256;
 constexpr int kStage32 = 3072;
+constexpr int kUnroll32 = 6;   // products per thread whose loads are in flight together
 
 struct MState {
     double bnorm, init_res, true_res, atol, rtol, inner_tol;
@@ -46,18 +47,36 @@ __global__ void __launch_bounds__(kRows32) k_spmv32(int n, const int32_t* __rest
     __shared__ float prod[kStage32];
     __shared__ float red[kRows32 / 64];
     if (skip && *skip) return;
-    const int r0 = blockIdx.x * kRows32;
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);
+    const int r0 = blk * kRows32;
     const int r1 = min(r0 + kRows32, n);
     const int base = rp[r0];
     const int cnt = rp[r1] - base;
     const int r = r0 + (int)threadIdx.x;
+    const int rc = min(r, r1 - 1);
+    const int rb = rp[rc], re = rp[rc + 1];   // this thread's row bounds, loaded before the products
     float sum = 0.f;
     if (cnt <= kStage32) {
-        for (int k = threadIdx.x; k < cnt; k += kRows32) prod[k] = v[base + k] * x[ci[base + k]];
+        // all of a thread's entry loads, then all its gathers, before any use (clamped indices)
+        for (int k0 = threadIdx.x; k0 < cnt; k0 += kRows32 * kUnroll32) {
+            int c[kUnroll32];
+            float a[kUnroll32], xv[kUnroll32];
+#pragma unroll
+            for (int u = 0; u < kUnroll32; ++u) {
+                const int k = min(k0 + u * kRows32, cnt - 1);
+                c[u] = __builtin_nontemporal_load(ci + base + k);   // streamed once: keep x in L2
+                a[u] = __builtin_nontemporal_load(v + base + k);
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll32; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < kUnroll32; ++u)
+                if (k0 + u * kRows32 < cnt) prod[k0 + u * kRows32] = a[u] * xv[u];
+        }
         __syncthreads();
         if (r < r1) {
-            const int e = rp[r + 1] - base;
-            for (int k = rp[r] - base; k < e; ++k) sum += prod[k];
+            const int e = re - base;
+            for (int k = rb - base; k < e; ++k) sum += prod[k];
         }
     } else if (r < r1) {
         const int e = rp[r + 1];
@@ -71,11 +90,11 @@ __global__ void __launch_bounds__(kRows32) k_spmv32(int n, const int32_t* __rest
     }
     if (dotw) {
         d0 = block_sum<kRows32>(d0, red);
-        if (threadIdx.x == 0) partials[blockIdx.x] = (double)d0;
+        if (threadIdx.x == 0) partials[blk] = (double)d0;
     }
     if (dot_self) {
         d1 = block_sum<kRows32>(d1, red);
-        if (threadIdx.x == 0) partials[pstride + blockIdx.x] = (double)d1;
+        if (threadIdx.x == 0) partials[pstride + blk] = (double)d1;
     }
 }
 

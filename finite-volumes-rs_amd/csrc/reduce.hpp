@@ -9,6 +9,17 @@
 
 namespace fvg {
 
+// The dispatcher hands workgroup k to XCD k mod 8.  xcd_tile maps workgroup k to the
+// tile it works on so that each XCD gets a contiguous run of tiles: neighbouring tiles
+// (which read overlapping parts of a gathered vector) then share one L2.  A bijection
+// on [0, g); only the schedule changes, never the arithmetic.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int k, int g) {
+    const int x = k % kXcds, local = k / kXcds;
+    const int per = g / kXcds, rem = g % kXcds;
+    return x * per + min(x, rem) + local;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
