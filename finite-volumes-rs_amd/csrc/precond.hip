@@ -384,27 +384,62 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
 // definite, as CG requires.  (Iterative triangular solves for preconditioning: Chow and
 // Patel, SIAM J. Sci. Comput. 37 (2015); Anzt, Chow and Dongarra, Euro-Par 2015.)
 //
-// One thread per sweep position (rows in level order, padding rows -1); FIRST folds the
-// start y0 = D^-1 v into the first sweep.
-template <bool FIRST, bool SCALE>
+// One thread per row, rows in natural order, on the matrix pattern: the lower factor of
+// row i is its slots before the diagonal slot, the upper factor its slots after it, with
+// FP32 values on the pattern (for IC's upper factor, L^T, gathered through the transpose
+// slot map beforehand).  Rows and their neighbours are close in memory, so the gathers
+// and the writes stay local.  FIRST folds the start y0 = D^-1 v into the first sweep.
+constexpr int kJacEnt = 4;   // entries per row whose loads are in flight together
+template <bool FIRST, bool LOWER, bool SCALE>
 __global__ void __launch_bounds__(256)
-    k_jacobi32(int64_t npos, const int32_t* __restrict__ rows, const int32_t* __restrict__ ptr,
-               const int32_t* __restrict__ col, const float* __restrict__ val, const float* __restrict__ scale,
+    k_jacobi32(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+               const int32_t* __restrict__ diag, const float* __restrict__ val, const float* __restrict__ scale,
                const float* __restrict__ v, const float* __restrict__ yin, float* __restrict__ yout,
                const int32_t* __restrict__ skip) {
     if (skip && *skip) return;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npos) return;
-    const int i = rows[p];
-    if (i < 0) return;
-    float s = v[i];
-    const int ke = ptr[p + 1];
-    for (int k = ptr[p]; k < ke; ++k) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int d = diag[i];
+    const int kb = LOWER ? rp[i] : d + 1;
+    const int ke = LOWER ? d : rp[i + 1];
+    const int cnt = ke - kb;
+    const float* __restrict__ src = FIRST ? v : yin;
+    int cj[kJacEnt];
+    float a[kJacEnt], yj[kJacEnt], sj[kJacEnt];
+    // columns >= n (outside the square block, which the factor leaves out) end a row
+#pragma unroll
+    for (int e = 0; e < kJacEnt; ++e) {
+        const int k = cnt > 0 ? kb + min(e, cnt - 1) : d;   // clamped onto the row (or its diagonal)
         const int j = col[k];
-        const float yj = FIRST ? (SCALE ? scale[j] * v[j] : v[j]) : yin[j];
-        s -= val[k] * yj;
+        cj[e] = j < n ? j : (int)i;
+        a[e] = j < n ? val[k] : 0.f;
+    }
+#pragma unroll
+    for (int e = 0; e < kJacEnt; ++e) {
+        yj[e] = src[cj[e]];
+        sj[e] = (FIRST && SCALE) ? scale[cj[e]] : 1.f;
+    }
+    float s = v[i];
+#pragma unroll
+    for (int e = 0; e < kJacEnt; ++e)
+        if (e < cnt) s -= a[e] * ((FIRST && SCALE) ? sj[e] * yj[e] : yj[e]);
+    for (int k = kb + kJacEnt; k < ke; ++k) {
+        const int j = col[k];
+        if (j >= n) break;
+        s -= val[k] * ((FIRST && SCALE) ? scale[j] * src[j] : src[j]);
     }
     yout[i] = SCALE ? scale[i] * s : s;
+}
+
+__global__ void k_fac_f32(int64_t nnz, const double* __restrict__ fac, const int32_t* __restrict__ tpos,
+                          float* __restrict__ f32, float* __restrict__ ft32) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    f32[k] = (float)fac[k];
+    if (ft32) {
+        const int t = tpos[k];
+        ft32[k] = t >= 0 ? (float)fac[t] : 0.f;
+    }
 }
 
 __global__ void k_to_f32(int64_t n, const double* __restrict__ s, float* __restrict__ d) {
@@ -520,6 +555,8 @@ static void free_prec(fvg_ctx* c, fvg_prec* p) {
     dev_free_n(c, p->d_inv_diag32, p->n);
     dev_free_n(c, p->d_tmp32, p->n);
     dev_free_n(c, p->d_tmp32b, p->n);
+    dev_free_n(c, p->d_fac32, p->nnz);
+    dev_free_n(c, p->d_fact32, p->nnz);
     free_sweep(c, p->lower);
     free_sweep(c, p->upper);
 }
@@ -712,6 +749,12 @@ int ensure_prec32(fvg_ctx* c, fvg_prec* p) {
     if (!p->d_tmp32b) FVG_TRY(dev_alloc_n(c, p->n, &p->d_tmp32b));
     hipLaunchKernelGGL(k_to_f32, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_inv_diag,
                        p->d_inv_diag32);
+    if (p->kind == kIC || p->kind == kILU) {   // FP32 factor on the pattern (IC: and L^T)
+        if (!p->d_fac32) FVG_TRY(dev_alloc_n(c, p->nnz, &p->d_fac32));
+        if (p->kind == kIC && !p->d_fact32) FVG_TRY(dev_alloc_n(c, p->nnz, &p->d_fact32));
+        hipLaunchKernelGGL(k_fac_f32, dim3(grid_for(p->nnz)), dim3(kBlock), 0, c->stream, p->nnz, p->d_fac,
+                           p->kind == kIC ? p->d_tpos : nullptr, p->d_fac32, p->kind == kIC ? p->d_fact32 : nullptr);
+    }
     for (Sweep* sw : {&p->lower, &p->upper}) {
         if (sw->nnz == 0 && sw->n == 0) continue;
         if (!sw->val32) FVG_TRY(dev_alloc_n(c, sw->nnz + 1, &sw->val32));
@@ -723,29 +766,31 @@ int ensure_prec32(fvg_ctx* c, fvg_prec* p) {
     return FVG_OK;
 }
 
-// m Jacobi sweeps of one triangular factor; sweep k writes `odd` for odd k and `even` for
-// even k, so the result lands in `odd` when m is odd and in `even` otherwise
-static void jacobi32(fvg_ctx* c, const Sweep& sw, int m, const float* scale, const float* v, float* odd,
-                     float* even, const int32_t* skip) {
-    if (sw.npos == 0) return;
-    const dim3 grid((unsigned)((sw.npos + 255) / 256)), block(256);
+// m Jacobi sweeps of one triangular factor (natural row order); sweep k writes `odd` for
+// odd k and `even` for even k, so the result lands in `odd` when m is odd and in `even`
+// otherwise
+template <bool LOWER>
+static void jacobi32(fvg_ctx* c, const fvg_prec* p, const float* val, int m, const float* scale, const float* v,
+                     float* odd, float* even, const int32_t* skip) {
+    if (p->n == 0) return;
+    const dim3 grid((unsigned)((p->n + 255) / 256)), block(256);
     const float* prev = nullptr;
     for (int k = 1; k <= m; ++k) {
         float* out = (k & 1) ? odd : even;
         if (k == 1) {
             if (scale)
-                hipLaunchKernelGGL((k_jacobi32<true, true>), grid, block, 0, c->stream, sw.npos, sw.rows, sw.ptr,
-                                   sw.col, sw.val32, scale, v, nullptr, out, skip);
+                hipLaunchKernelGGL((k_jacobi32<true, LOWER, true>), grid, block, 0, c->stream, p->n, p->d_rp, p->d_col,
+                                   p->d_diag, val, scale, v, nullptr, out, skip);
             else
-                hipLaunchKernelGGL((k_jacobi32<true, false>), grid, block, 0, c->stream, sw.npos, sw.rows, sw.ptr,
-                                   sw.col, sw.val32, nullptr, v, nullptr, out, skip);
+                hipLaunchKernelGGL((k_jacobi32<true, LOWER, false>), grid, block, 0, c->stream, p->n, p->d_rp,
+                                   p->d_col, p->d_diag, val, nullptr, v, nullptr, out, skip);
         } else {
             if (scale)
-                hipLaunchKernelGGL((k_jacobi32<false, true>), grid, block, 0, c->stream, sw.npos, sw.rows, sw.ptr,
-                                   sw.col, sw.val32, scale, v, prev, out, skip);
+                hipLaunchKernelGGL((k_jacobi32<false, LOWER, true>), grid, block, 0, c->stream, p->n, p->d_rp,
+                                   p->d_col, p->d_diag, val, scale, v, prev, out, skip);
             else
-                hipLaunchKernelGGL((k_jacobi32<false, false>), grid, block, 0, c->stream, sw.npos, sw.rows, sw.ptr,
-                                   sw.col, sw.val32, nullptr, v, prev, out, skip);
+                hipLaunchKernelGGL((k_jacobi32<false, LOWER, false>), grid, block, 0, c->stream, p->n, p->d_rp,
+                                   p->d_col, p->d_diag, val, nullptr, v, prev, out, skip);
         }
         prev = out;
     }
@@ -761,10 +806,11 @@ static int apply_factor32(fvg_ctx* c, const fvg_prec* p, const float* lo_scale, 
     }
     const int m = c->tune_inner_sweeps > 0 ? (int)c->tune_inner_sweeps : kInnerSweeps;
     // lower: result in tmp32 (m odd) or tmp32b (m even); upper: its last sweep writes z
-    jacobi32(c, p->lower, m, lo_scale, r, p->d_tmp32, p->d_tmp32b, skip);
+    const float* upper_val = p->kind == kIC ? p->d_fact32 : p->d_fac32;
+    jacobi32<true>(c, p, p->d_fac32, m, lo_scale, r, p->d_tmp32, p->d_tmp32b, skip);
     const float* y = (m & 1) ? p->d_tmp32 : p->d_tmp32b;
     float* other = (m & 1) ? p->d_tmp32b : p->d_tmp32;
-    jacobi32(c, p->upper, m, p->d_inv_diag32, y, (m & 1) ? z : other, (m & 1) ? other : z, skip);
+    jacobi32<false>(c, p, upper_val, m, p->d_inv_diag32, y, (m & 1) ? z : other, (m & 1) ? other : z, skip);
     FVG_HIP(hipGetLastError());
     return FVG_OK;
 }

@@ -52,6 +52,8 @@ struct fvg_prec {
     float* d_inv_diag32 = nullptr;
     float* d_tmp32 = nullptr;
     float* d_tmp32b = nullptr;          // second FP32 temporary (Jacobi-iterated sweeps)
+    float* d_fac32 = nullptr;           // FP32 factor values on the matrix pattern
+    float* d_fact32 = nullptr;          // IC: FP32 L^T values on the pattern (slot (i,j) holds l_ji)
     uint64_t fac_version = 0, fac32_version = 0;
     void* team = nullptr;               // batched-solve plan of this system (team.hip), made on use
     uint64_t pattern_id = 0;            // mesh pattern the analysis was made for (0: not a mesh pattern)
