@@ -25,6 +25,7 @@ constexpr int kScalarBlock = 1024;   // threads of the one-workgroup scalar kern
 constexpr int kRows32 = 256;
 constexpr int kStage32 = 3072;
 constexpr int kUnroll32 = 6;   // products per thread whose loads are in flight together
+constexpr int kGraphIters = 4; // inner iterations between host checks (one captured graph)
 
 struct MState {
     double bnorm, init_res, true_res, atol, rtol, inner_tol;
@@ -449,6 +450,46 @@ int run_mixed(fvg_ctx* c, fvg_csr* a, fvg_prec* p, const fvg_vec* b, fvg_vec* x,
     const int32_t* inner_skip = &st->inner_done;
     std::vector<double> hist;
 
+    // one inner iteration (kernels only): launched directly, or captured into a graph
+    auto iteration = [&]() -> int {
+        if (!bicgstab) {
+            FVG_TRY(launch_spmv32(c, a, pp, ap, pp, 0, inner_skip));
+            hipLaunchKernelGGL(k_mscalar<M_CG_ALPHA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
+            hipLaunchKernelGGL(k_m_cg_dr, dim3(nb), dim3(kBlock), 0, c->stream, n, st, pp, ap, d, r, part);
+            hipLaunchKernelGGL(k_mscalar<M_CG_ERR>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
+            FVG_TRY(apply_prec32(c, p, r, z, inner_skip));
+            hipLaunchKernelGGL(k_m_dot, dim3(nb), dim3(kBlock), 0, c->stream, n, st, 1, r, z, part);
+            hipLaunchKernelGGL(k_mscalar<M_CG_BETA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
+            hipLaunchKernelGGL(k_m_cg_p, dim3(nb), dim3(kBlock), 0, c->stream, n, st, z, pp);
+        } else {
+            FVG_TRY(apply_prec32(c, p, pp, y, inner_skip));
+            FVG_TRY(launch_spmv32(c, a, y, v, rh, 0, inner_skip));
+            hipLaunchKernelGGL(k_mscalar<M_BI_ALPHA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
+            hipLaunchKernelGGL(k_m_bi_hs, dim3(nb), dim3(kBlock), 0, c->stream, n, st, y, v, r, d, s, part);
+            hipLaunchKernelGGL(k_mscalar<M_BI_S>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
+            FVG_TRY(apply_prec32(c, p, s, z, inner_skip));
+            FVG_TRY(launch_spmv32(c, a, z, t, s, 1, inner_skip));
+            hipLaunchKernelGGL(k_mscalar<M_BI_OMEGA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
+            hipLaunchKernelGGL(k_m_bi_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, st, z, t, s, rh, d, r, part,
+                               stride);
+            hipLaunchKernelGGL(k_mscalar<M_BI_R>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
+            hipLaunchKernelGGL(k_m_bi_p, dim3(nb), dim3(kBlock), 0, c->stream, n, st, r, v, pp, rh);
+        }
+        FVG_HIP(hipGetLastError());
+        return FVG_OK;
+    };
+    // inner iterations from a captured graph unless timing spans (event records) or the
+    // exact FP32 sweeps (cooperative launches) are in use
+    struct Graph {
+        bool usable = false;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        ~Graph() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+        }
+    } graph;
+    graph.usable = !c->timing && !c->tune_inner_exact;
     for (int outer = 0; outer < 64; ++outer) {
         // r = b - A x (FP64) and the refinement test
         SpmvEpilogue e;
@@ -477,36 +518,39 @@ int run_mixed(fvg_ctx* c, fvg_csr* a, fvg_prec* p, const fvg_vec* b, fvg_vec* x,
         }
         hipLaunchKernelGGL(k_mscalar<M_IN_INIT>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
         FVG_HIP(hipGetLastError());
-        for (int64_t it = 0;; ++it) {
-            if (!bicgstab) {
-                FVG_TRY(launch_spmv32(c, a, pp, ap, pp, 0, inner_skip));
-                hipLaunchKernelGGL(k_mscalar<M_CG_ALPHA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
-                hipLaunchKernelGGL(k_m_cg_dr, dim3(nb), dim3(kBlock), 0, c->stream, n, st, pp, ap, d, r, part);
-                hipLaunchKernelGGL(k_mscalar<M_CG_ERR>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
-                FVG_TRY(apply_prec32(c, p, r, z, inner_skip));
-                hipLaunchKernelGGL(k_m_dot, dim3(nb), dim3(kBlock), 0, c->stream, n, st, 1, r, z, part);
-                hipLaunchKernelGGL(k_mscalar<M_CG_BETA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
-                hipLaunchKernelGGL(k_m_cg_p, dim3(nb), dim3(kBlock), 0, c->stream, n, st, z, pp);
-            } else {
-                FVG_TRY(apply_prec32(c, p, pp, y, inner_skip));
-                FVG_TRY(launch_spmv32(c, a, y, v, rh, 0, inner_skip));
-                hipLaunchKernelGGL(k_mscalar<M_BI_ALPHA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
-                hipLaunchKernelGGL(k_m_bi_hs, dim3(nb), dim3(kBlock), 0, c->stream, n, st, y, v, r, d, s, part);
-                hipLaunchKernelGGL(k_mscalar<M_BI_S>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
-                FVG_TRY(apply_prec32(c, p, s, z, inner_skip));
-                FVG_TRY(launch_spmv32(c, a, z, t, s, 1, inner_skip));
-                hipLaunchKernelGGL(k_mscalar<M_BI_OMEGA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
-                hipLaunchKernelGGL(k_m_bi_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, st, z, t, s, rh, d, r, part,
-                                   stride);
-                hipLaunchKernelGGL(k_mscalar<M_BI_R>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
-                hipLaunchKernelGGL(k_m_bi_p, dim3(nb), dim3(kBlock), 0, c->stream, n, st, r, v, pp, rh);
+        if (graph.usable) {
+            // the inner iterations replayed from one captured graph of kGraphIters iterations
+            // (kernels only: the same launches every time, each a no-op once the solve is done)
+            if (!graph.exec) {
+                FVG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                int cs = FVG_OK;
+                for (int k = 0; k < kGraphIters && cs == FVG_OK; ++k) cs = iteration();
+                hipGraph_t g = nullptr;
+                const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+                if (cs != FVG_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return cs;
+                }
+                FVG_HIP(ce);
+                graph.graph = g;
+                FVG_HIP(hipGraphInstantiate(&graph.exec, g, nullptr, nullptr, 0));
             }
-            FVG_HIP(hipGetLastError());
-            if ((it + 1) % 4 == 0) {
+            for (;;) {
+                FVG_HIP(hipGraphLaunch(graph.exec, c->stream));
                 FVG_HIP(hipMemcpyAsync(c->pinned, st, sizeof(MState), hipMemcpyDeviceToHost, c->stream));
                 FVG_HIP(hipStreamSynchronize(c->stream));
                 std::memcpy(&h, c->pinned, sizeof(MState));
                 if (h.inner_done || h.done) break;
+            }
+        } else {
+            for (int64_t it = 0;; ++it) {
+                FVG_TRY(iteration());
+                if ((it + 1) % kGraphIters == 0) {
+                    FVG_HIP(hipMemcpyAsync(c->pinned, st, sizeof(MState), hipMemcpyDeviceToHost, c->stream));
+                    FVG_HIP(hipStreamSynchronize(c->stream));
+                    std::memcpy(&h, c->pinned, sizeof(MState));
+                    if (h.inner_done || h.done) break;
+                }
             }
         }
         hipLaunchKernelGGL(k_m_update, dim3(nb), dim3(kBlock), 0, c->stream, n, st, d, x->d);
