@@ -13,7 +13,7 @@
 namespace fvg {
 
 constexpr int kSpmvRows = 256;     // rows (and threads) per workgroup
-constexpr int kSpmvStage = 3072;   // staged products per workgroup (24 KiB of LDS)
+constexpr int kSpmvStage = 2048;   // staged products per workgroup (16 KiB of LDS)
 constexpr int kSpmvUnroll = 6;     // products per thread whose loads are in flight together
 
 int spmv_blocks(int64_t nrows) { return grid_for(nrows, kSpmvRows); }
@@ -215,6 +215,29 @@ int fvg_spmv(fvg_ctx* c, const fvg_csr* a, const fvg_vec* x, fvg_vec* y) {
     if (x->n < a->ncols || y->n != a->nrows) return fail(FVG_E_INVALID, "vector lengths do not match the matrix");
     if (x->d == y->d) return fail(FVG_E_INVALID, "x and y must not alias");
     return launch_spmv(c, a, x->d, y->d, SpmvEpilogue{});
+}
+
+int fvg_spmv_time(fvg_ctx* c, const fvg_csr* a, const fvg_vec* x, fvg_vec* y, int32_t reps, double* avg_ms) {
+    if (!c || !a || !x || !y || !avg_ms || reps <= 0) return fail(FVG_E_INVALID, "bad argument");
+    if (x->n < a->ncols || y->n < a->nrows) return fail(FVG_E_INVALID, "vector too short for the matrix");
+    hipEvent_t e0, e1;
+    FVG_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(FVG_E_HIP, "event creation failed");
+    }
+    SpmvEpilogue none;
+    int s = FVG_OK;
+    if (hipEventRecord(e0, c->stream) != hipSuccess) s = fail(FVG_E_HIP, "event record failed");
+    for (int k = 0; k < reps && s == FVG_OK; ++k) s = launch_spmv(c, a, x->d, y->d, none);
+    if (s == FVG_OK && hipEventRecord(e1, c->stream) != hipSuccess) s = fail(FVG_E_HIP, "event record failed");
+    float ms = 0.f;
+    if (s == FVG_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        s = fail(FVG_E_HIP, "event timing failed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (s == FVG_OK) *avg_ms = (double)ms / reps;
+    return s;
 }
 
 }  // extern "C"

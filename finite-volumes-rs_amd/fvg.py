@@ -121,6 +121,7 @@ def lib():
         "fvg_csr_download": [P, P, P, P, P],
         "fvg_csr_upload_values": [P, P, P],
         "fvg_spmv": [P, P, P, P],
+        "fvg_spmv_time": [P, P, P, P, C.c_int32, C.POINTER(C.c_double)],
         "fvg_assemble": [P, P, P, C.c_int32, P, C.c_int32, P, C.c_int32, P, P, P, P],
         "fvg_identity_create": [P, C.c_int64, C.POINTER(P)],
         "fvg_diagonal_create": [P, P, C.POINTER(P)],
@@ -395,6 +396,12 @@ class DistributedMatrix:
     def imul(self, result, rhs):
         """result = A rhs (dmatrix.rs:136-146)."""
         check(lib().fvg_spmv(self.ctx.h, self.h, rhs.h, result.h))
+
+    def time_spmv(self, result, rhs, reps=50):
+        """Average milliseconds of `reps` back-to-back result = A rhs launches (HIP events)."""
+        ms = C.c_double()
+        check(lib().fvg_spmv_time(self.ctx.h, self.h, rhs.h, result.h, int(reps), C.byref(ms)))
+        return ms.value
 
     def close(self):
         if getattr(self, "h", None) and self.ctx.h:

@@ -137,6 +137,9 @@ int fvg_csr_upload_values(fvg_ctx* ctx, fvg_csr* a, const double* values);
 /* DistributedMatrix::imul (dmatrix.rs:136-146): y = A x, each row summed in
  * ascending column order, bit-identical to the sequential loop. */
 int fvg_spmv(fvg_ctx* ctx, const fvg_csr* a, const fvg_vec* x, fvg_vec* y);
+/* Measurement: `reps` back-to-back launches of y = A x between two HIP events on the
+ * context stream; *avg_ms = elapsed / reps (no reference counterpart). */
+int fvg_spmv_time(fvg_ctx* ctx, const fvg_csr* a, const fvg_vec* x, fvg_vec* y, int32_t reps, double* avg_ms);
 
 /* ---- assembly: fvm::assembly::assemble (src/fvm/assembly.rs:11-79) -------------
  * The equation (an `impl Term` expression built from laplacian, convection,
