@@ -21,6 +21,7 @@
 #include <cstring>
 
 #include "precond.hpp"
+#include "reduce.hpp"
 
 namespace fvg {
 
@@ -384,62 +385,70 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
 // definite, as CG requires.  (Iterative triangular solves for preconditioning: Chow and
 // Patel, SIAM J. Sci. Comput. 37 (2015); Anzt, Chow and Dongarra, Euro-Par 2015.)
 //
-// One thread per row, rows in natural order, on the matrix pattern: the lower factor of
-// row i is its slots before the diagonal slot, the upper factor its slots after it, with
-// FP32 values on the pattern (for IC's upper factor, L^T, gathered through the transpose
-// slot map beforehand).  Rows and their neighbours are close in memory, so the gathers
-// and the writes stay local.  FIRST folds the start y0 = D^-1 v into the first sweep.
-constexpr int kJacEnt = 4;   // entries per row whose loads are in flight together
-template <bool FIRST, bool LOWER, bool SCALE>
-__global__ void __launch_bounds__(256)
-    k_jacobi32(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-               const int32_t* __restrict__ diag, const float* __restrict__ val, const float* __restrict__ scale,
-               const float* __restrict__ v, const float* __restrict__ yin, float* __restrict__ yout,
-               const int32_t* __restrict__ skip) {
+// Each factor is held as its own CSR in natural row order (strict lower part, or strict
+// upper part with columns inside the square block), with FP32 values, so a sweep is an
+// SpMV with an update: y_new = D^-1 (v - N y_old).  The kernel has the SpMV's shape: a
+// workgroup per tile of kJacRows consecutive rows (tiles in XCD-contiguous order), the
+// tile's (column, value) range streamed with coalesced loads, every thread issuing all its
+// entry loads and then all its gathers before using any, products staged in LDS, and each
+// row summing its own products.  FIRST folds the start y0 = D^-1 v into the first sweep.
+constexpr int kJacRows = 256;
+constexpr int kJacStage = 2048;
+constexpr int kJacUnroll = 4;
+template <bool FIRST, bool SCALE>
+__global__ void __launch_bounds__(kJacRows)
+    k_jacobi32(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col, const float* __restrict__ val,
+               const float* __restrict__ scale, const float* __restrict__ v, const float* __restrict__ yin,
+               float* __restrict__ yout, const int32_t* __restrict__ skip) {
+    __shared__ float prod[kJacStage];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int r0 = tile * kJacRows;
+    const int r1 = min(r0 + kJacRows, n);
+    const int base = ptr[r0];
+    const int cnt = ptr[r1] - base;
+    const int r = r0 + (int)threadIdx.x;
+    const int rc = min(r, r1 - 1);
+    const int rb = ptr[rc], re = ptr[rc + 1];
+    const float vr = v[rc];
+    const float sr = SCALE ? scale[rc] : 1.f;
     if (skip && *skip) return;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int d = diag[i];
-    const int kb = LOWER ? rp[i] : d + 1;
-    const int ke = LOWER ? d : rp[i + 1];
-    const int cnt = ke - kb;
     const float* __restrict__ src = FIRST ? v : yin;
-    int cj[kJacEnt];
-    float a[kJacEnt], yj[kJacEnt], sj[kJacEnt];
-    // columns >= n (outside the square block, which the factor leaves out) end a row
+    float sum = 0.f;
+    if (cnt <= kJacStage) {
+        for (int k0 = threadIdx.x; k0 < cnt; k0 += kJacRows * kJacUnroll) {
+            int c[kJacUnroll];
+            float a[kJacUnroll], yv[kJacUnroll], sv[kJacUnroll];
 #pragma unroll
-    for (int e = 0; e < kJacEnt; ++e) {
-        const int k = cnt > 0 ? kb + min(e, cnt - 1) : d;   // clamped onto the row (or its diagonal)
-        const int j = col[k];
-        cj[e] = j < n ? j : (int)i;
-        a[e] = j < n ? val[k] : 0.f;
-    }
+            for (int u = 0; u < kJacUnroll; ++u) {
+                const int k = min(k0 + u * kJacRows, cnt - 1);
+                c[u] = __builtin_nontemporal_load(col + base + k);
+                a[u] = __builtin_nontemporal_load(val + base + k);
+            }
 #pragma unroll
-    for (int e = 0; e < kJacEnt; ++e) {
-        yj[e] = src[cj[e]];
-        sj[e] = (FIRST && SCALE) ? scale[cj[e]] : 1.f;
-    }
-    float s = v[i];
+            for (int u = 0; u < kJacUnroll; ++u) {
+                yv[u] = src[c[u]];
+                sv[u] = (FIRST && SCALE) ? scale[c[u]] : 1.f;
+            }
 #pragma unroll
-    for (int e = 0; e < kJacEnt; ++e)
-        if (e < cnt) s -= a[e] * ((FIRST && SCALE) ? sj[e] * yj[e] : yj[e]);
-    for (int k = kb + kJacEnt; k < ke; ++k) {
-        const int j = col[k];
-        if (j >= n) break;
-        s -= val[k] * ((FIRST && SCALE) ? scale[j] * src[j] : src[j]);
+            for (int u = 0; u < kJacUnroll; ++u)
+                if (k0 + u * kJacRows < cnt) prod[k0 + u * kJacRows] = a[u] * ((FIRST && SCALE) ? sv[u] * yv[u] : yv[u]);
+        }
+        __syncthreads();
+        if (r < r1)
+            for (int k = rb - base; k < re - base; ++k) sum += prod[k];
+    } else if (r < r1) {
+        for (int k = rb; k < re; ++k) {
+            const int j = col[k];
+            sum += val[k] * ((FIRST && SCALE) ? scale[j] * src[j] : src[j]);
+        }
     }
-    yout[i] = SCALE ? scale[i] * s : s;
+    if (r < r1) yout[r] = SCALE ? sr * (vr - sum) : vr - sum;
 }
 
-__global__ void k_fac_f32(int64_t nnz, const double* __restrict__ fac, const int32_t* __restrict__ tpos,
-                          float* __restrict__ f32, float* __restrict__ ft32) {
+__global__ void k_gather_f32(int64_t nnz, const int32_t* __restrict__ src, const double* __restrict__ fac,
+                             float* __restrict__ out) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nnz) return;
-    f32[k] = (float)fac[k];
-    if (ft32) {
-        const int t = tpos[k];
-        ft32[k] = t >= 0 ? (float)fac[t] : 0.f;
-    }
+    if (k < nnz) out[k] = (float)fac[src[k]];
 }
 
 __global__ void k_to_f32(int64_t n, const double* __restrict__ s, float* __restrict__ d) {
@@ -542,6 +551,14 @@ static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::ve
     return FVG_OK;
 }
 
+static void free_tri32(fvg_ctx* c, Tri32& t) {
+    dev_free_n(c, t.ptr, t.n + 1);
+    dev_free_n(c, t.col, t.nnz);
+    dev_free_n(c, t.src, t.nnz);
+    dev_free_n(c, t.val, t.nnz);
+    t = Tri32{};
+}
+
 static void free_prec(fvg_ctx* c, fvg_prec* p) {
     team_plan_release(c, p);
     dev_free_n(c, p->d_rp, p->n + 1);
@@ -555,8 +572,8 @@ static void free_prec(fvg_ctx* c, fvg_prec* p) {
     dev_free_n(c, p->d_inv_diag32, p->n);
     dev_free_n(c, p->d_tmp32, p->n);
     dev_free_n(c, p->d_tmp32b, p->n);
-    dev_free_n(c, p->d_fac32, p->nnz);
-    dev_free_n(c, p->d_fact32, p->nnz);
+    free_tri32(c, p->jlo);
+    free_tri32(c, p->jup);
     free_sweep(c, p->lower);
     free_sweep(c, p->upper);
 }
@@ -741,6 +758,38 @@ static int sweep32(fvg_ctx* c, const Sweep& sw, const float* scale, const float*
     return FVG_OK;
 }
 
+// one strict triangular factor of p in natural row order (the inexact FP32 apply):
+// lower = slots before the diagonal; upper = slots after it with columns < n (IC: the
+// value of slot (i, j) is l_ji, read through the transpose slot map)
+static int build_tri32(fvg_ctx* c, const fvg_prec* p, bool upper, Tri32& t) {
+    const int64_t n = p->n;
+    std::vector<int32_t> ptr((size_t)n + 1, 0), col, src;
+    col.reserve((size_t)p->nnz / 2 + 1);
+    src.reserve((size_t)p->nnz / 2 + 1);
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t d = p->h_diag[(size_t)i];
+        const int32_t kb = upper ? d + 1 : p->h_rp[(size_t)i], ke = upper ? p->h_rp[(size_t)i + 1] : d;
+        for (int32_t k = kb; k < ke; ++k) {
+            const int32_t j = p->h_col[(size_t)k];
+            if (j >= n) break;
+            col.push_back(j);
+            src.push_back(upper && p->kind == kIC ? p->h_tpos[(size_t)k] : k);
+        }
+        ptr[(size_t)i + 1] = (int32_t)col.size();
+    }
+    t.n = n;
+    t.nnz = (int64_t)col.size();
+    int s;
+    if ((s = dev_alloc_n(c, n + 1, &t.ptr)) || (s = dev_alloc_n(c, t.nnz, &t.col)) ||
+        (s = dev_alloc_n(c, t.nnz, &t.src)) || (s = dev_alloc_n(c, t.nnz, &t.val)) ||
+        (s = upload(c, t.ptr, ptr.data(), n + 1)) || (s = upload(c, t.col, col.data(), t.nnz)) ||
+        (s = upload(c, t.src, src.data(), t.nnz)))
+        return s;
+    FVG_HIP(hipStreamSynchronize(c->stream));   // the host vectors go out of scope
+    return FVG_OK;
+}
+
+
 int ensure_prec32(fvg_ctx* c, fvg_prec* p) {
     if (p->kind == kIdentity) return FVG_OK;
     if (p->fac32_version == p->fac_version && p->d_inv_diag32) return FVG_OK;
@@ -749,11 +798,14 @@ int ensure_prec32(fvg_ctx* c, fvg_prec* p) {
     if (!p->d_tmp32b) FVG_TRY(dev_alloc_n(c, p->n, &p->d_tmp32b));
     hipLaunchKernelGGL(k_to_f32, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_inv_diag,
                        p->d_inv_diag32);
-    if (p->kind == kIC || p->kind == kILU) {   // FP32 factor on the pattern (IC: and L^T)
-        if (!p->d_fac32) FVG_TRY(dev_alloc_n(c, p->nnz, &p->d_fac32));
-        if (p->kind == kIC && !p->d_fact32) FVG_TRY(dev_alloc_n(c, p->nnz, &p->d_fact32));
-        hipLaunchKernelGGL(k_fac_f32, dim3(grid_for(p->nnz)), dim3(kBlock), 0, c->stream, p->nnz, p->d_fac,
-                           p->kind == kIC ? p->d_tpos : nullptr, p->d_fac32, p->kind == kIC ? p->d_fact32 : nullptr);
+    if (p->kind == kIC || p->kind == kILU) {   // FP32 factors as natural-order CSRs
+        for (int u = 0; u < 2; ++u) {
+            Tri32& t = u ? p->jup : p->jlo;
+            if (!t.ptr) FVG_TRY(build_tri32(c, p, u == 1, t));
+            if (t.nnz > 0)
+                hipLaunchKernelGGL(k_gather_f32, dim3(grid_for(t.nnz)), dim3(kBlock), 0, c->stream, t.nnz, t.src,
+                                   p->d_fac, t.val);
+        }
     }
     for (Sweep* sw : {&p->lower, &p->upper}) {
         if (sw->nnz == 0 && sw->n == 0) continue;
@@ -769,28 +821,28 @@ int ensure_prec32(fvg_ctx* c, fvg_prec* p) {
 // m Jacobi sweeps of one triangular factor (natural row order); sweep k writes `odd` for
 // odd k and `even` for even k, so the result lands in `odd` when m is odd and in `even`
 // otherwise
-template <bool LOWER>
-static void jacobi32(fvg_ctx* c, const fvg_prec* p, const float* val, int m, const float* scale, const float* v,
-                     float* odd, float* even, const int32_t* skip) {
-    if (p->n == 0) return;
-    const dim3 grid((unsigned)((p->n + 255) / 256)), block(256);
+static void jacobi32(fvg_ctx* c, const Tri32& t, int m, const float* scale, const float* v, float* odd, float* even,
+                     const int32_t* skip) {
+    if (t.n == 0) return;
+    const dim3 grid((unsigned)((t.n + kJacRows - 1) / kJacRows)), block(kJacRows);
+    const int n = (int)t.n;
     const float* prev = nullptr;
     for (int k = 1; k <= m; ++k) {
         float* out = (k & 1) ? odd : even;
         if (k == 1) {
             if (scale)
-                hipLaunchKernelGGL((k_jacobi32<true, LOWER, true>), grid, block, 0, c->stream, p->n, p->d_rp, p->d_col,
-                                   p->d_diag, val, scale, v, nullptr, out, skip);
+                hipLaunchKernelGGL((k_jacobi32<true, true>), grid, block, 0, c->stream, n, t.ptr, t.col, t.val, scale,
+                                   v, nullptr, out, skip);
             else
-                hipLaunchKernelGGL((k_jacobi32<true, LOWER, false>), grid, block, 0, c->stream, p->n, p->d_rp,
-                                   p->d_col, p->d_diag, val, nullptr, v, nullptr, out, skip);
+                hipLaunchKernelGGL((k_jacobi32<true, false>), grid, block, 0, c->stream, n, t.ptr, t.col, t.val,
+                                   nullptr, v, nullptr, out, skip);
         } else {
             if (scale)
-                hipLaunchKernelGGL((k_jacobi32<false, LOWER, true>), grid, block, 0, c->stream, p->n, p->d_rp,
-                                   p->d_col, p->d_diag, val, scale, v, prev, out, skip);
+                hipLaunchKernelGGL((k_jacobi32<false, true>), grid, block, 0, c->stream, n, t.ptr, t.col, t.val, scale,
+                                   v, prev, out, skip);
             else
-                hipLaunchKernelGGL((k_jacobi32<false, LOWER, false>), grid, block, 0, c->stream, p->n, p->d_rp,
-                                   p->d_col, p->d_diag, val, nullptr, v, prev, out, skip);
+                hipLaunchKernelGGL((k_jacobi32<false, false>), grid, block, 0, c->stream, n, t.ptr, t.col, t.val,
+                                   nullptr, v, prev, out, skip);
         }
         prev = out;
     }
@@ -806,11 +858,10 @@ static int apply_factor32(fvg_ctx* c, const fvg_prec* p, const float* lo_scale, 
     }
     const int m = c->tune_inner_sweeps > 0 ? (int)c->tune_inner_sweeps : kInnerSweeps;
     // lower: result in tmp32 (m odd) or tmp32b (m even); upper: its last sweep writes z
-    const float* upper_val = p->kind == kIC ? p->d_fact32 : p->d_fac32;
-    jacobi32<true>(c, p, p->d_fac32, m, lo_scale, r, p->d_tmp32, p->d_tmp32b, skip);
+    jacobi32(c, p->jlo, m, lo_scale, r, p->d_tmp32, p->d_tmp32b, skip);
     const float* y = (m & 1) ? p->d_tmp32 : p->d_tmp32b;
     float* other = (m & 1) ? p->d_tmp32b : p->d_tmp32;
-    jacobi32<false>(c, p, upper_val, m, p->d_inv_diag32, y, (m & 1) ? z : other, (m & 1) ? other : z, skip);
+    jacobi32(c, p->jup, m, p->d_inv_diag32, y, (m & 1) ? z : other, (m & 1) ? other : z, skip);
     FVG_HIP(hipGetLastError());
     return FVG_OK;
 }

@@ -25,6 +25,16 @@ struct Sweep {
     float* val32 = nullptr;             // FP32 copy for the mixed-precision inner solver
 };
 
+// One strict triangular factor as a CSR in natural row order with FP32 values, for the
+// mixed-precision inner solver's Jacobi-iterated apply.
+struct Tri32 {
+    int64_t n = 0, nnz = 0;
+    int32_t* ptr = nullptr;    // n + 1
+    int32_t* col = nullptr;    // nnz, ascending within a row
+    int32_t* src = nullptr;    // factor slot of each entry
+    float* val = nullptr;      // nnz, gathered from the factor after each refactorisation
+};
+
 enum PrecKind { kIdentity = 0, kDiagonal = 1, kIC = 2, kILU = 3 };
 
 // Jacobi sweeps per triangular factor when FP32 inner iterations apply an incomplete
@@ -52,8 +62,7 @@ struct fvg_prec {
     float* d_inv_diag32 = nullptr;
     float* d_tmp32 = nullptr;
     float* d_tmp32b = nullptr;          // second FP32 temporary (Jacobi-iterated sweeps)
-    float* d_fac32 = nullptr;           // FP32 factor values on the matrix pattern
-    float* d_fact32 = nullptr;          // IC: FP32 L^T values on the pattern (slot (i,j) holds l_ji)
+    fvg::Tri32 jlo, jup;                // FP32 strict factors in natural row order (inexact apply)
     uint64_t fac_version = 0, fac32_version = 0;
     void* team = nullptr;               // batched-solve plan of this system (team.hip), made on use
     uint64_t pattern_id = 0;            // mesh pattern the analysis was made for (0: not a mesh pattern)
