@@ -209,7 +209,7 @@ def leg_convdiff(args, fvg, workloads):
     out = {"workload": "convdiff_prism: 128x128x2 jittered triangles x 64 layers = 2,097,152 prisms, velocity "
                        "(1,0.5,0.25) scaled to cell Peclet U[1,100], k = 1e-2, upwind, inflow u = 1 / outflow zero "
                        "gradient; assemble + ILU(0) + BiCGStab to 1e-8 rel. residual, x0 = 0",
-           "n_cells": mesh.n_cells, "nnz": mesh.nnz}
+           "n_cells": mesh.n_cells, "nnz": mesh.nnz, "mesh_build_ms": time_mesh_build(fvg, ctx, raw)}
     xs = {}
     for name, prec in (("fp64", fvg.PRECISION_FP64), ("mixed", fvg.PRECISION_MIXED)):
         opts = fvg.LinearSolverOptions(1e-8, 0.0, 4000, precision=prec)
@@ -252,6 +252,44 @@ def measured_traffic(kernel, args):
     return None if k is None else k["traffic_bytes"]
 
 
+def assembly_bytes(mesh, pr):
+    """Algorithmic HBM bytes of one assembly: per face its area, Orthogonal factor and
+    face fields (read once), per cell its volume, row and entry offsets, diagonal slot,
+    cell fields and right-hand side, the per-cell entry lists (8 bytes per face side),
+    the boundary closure pair per boundary face, and every CSR value written once."""
+    n, F, nb, nnz = mesh.n_cells, mesh.n_faces, mesh.n_boundary_faces, mesh.nnz
+    sides = 2 * F - nb
+    return (F * (16 + 8 * len(pr["face_fields"])) + n * (8 + 4 + 4 + 4 + 8 + 8 * len(pr["cell_fields"])) +
+            8 * sides + 16 * nb + 8 * nnz)
+
+
+def time_mesh_build(fvg, ctx, raw, reps=3):
+    """Wall time (median of `reps`) of Mesh::compute plus the cell-to-cell CSR pattern
+    on the device, the topology arrays starting in host memory."""
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        m = fvg.Mesh(ctx, raw)
+        a = fvg.DistributedMatrix.from_mesh(ctx, m)
+        ctx.sync()
+        ts.append(time.perf_counter() - t0)
+        a.close()
+        m.close()
+    return 1e3 * float(np.median(ts))
+
+
+def time_assembly(ctx, asm, lhs, rhs, reps=20):
+    """Device time of one assembly (a single row-gather launch), averaged over `reps`
+    back-to-back launches."""
+    asm.run(lhs, rhs)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        asm.run(lhs, rhs, sync=False)
+    ctx.sync()
+    return 1e3 * (time.perf_counter() - t0) / reps
+
+
 def cpu_baseline(raw, fields_of):
     """One full solve of the same inputs by the oracle on one host core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -267,7 +305,8 @@ def cpu_baseline(raw, fields_of):
     t = time.perf_counter() - t0
     return dict(value=1.0 / t, unit="solves/s", cores=1, kind="port",
                 sample="one full solve of the same workload (assemble + IC(0) + CG to 1e-8), "
-                       f"{info['iterations']} iterations in {t:.2f} s; mesh geometry ({t_mesh:.2f} s) excluded")
+                       f"{info['iterations']} iterations in {t:.2f} s; mesh geometry ({t_mesh:.2f} s) excluded",
+                mesh_build_ms=1e3 * t_mesh)
 
 
 def calibrate_copy():
@@ -317,6 +356,8 @@ def main():
     rhs = fvg.DistributedVector.from_size(ctx, mesh.n_cells)
     x = fvg.DistributedVector.from_size(ctx, mesh.n_cells)
     asm.run(lhs, rhs)
+    mesh_build_ms = time_mesh_build(fvg, ctx, raw)
+    asm_ms = time_assembly(ctx, asm, lhs, rhs)
     ic = fvg.IncompleteCholesky.from_matrix(lhs, 0)   # pattern analysis + first factorisation
     opts = fvg.LinearSolverOptions(relative_tolerance=1e-8, absolute_tolerance=0.0, max_iterations=2000)
     n, nnz = mesh.n_cells, mesh.nnz
@@ -410,6 +451,9 @@ def main():
                   "speedup_vs_fp64": (args.steps / m_elapsed) / value, "inner_iterations": minfos[-1].iterations,
                   "outer_iterations": minfos[-1].outer_iterations, "true_relative_residual": m_true,
                   "rel_error_vs_fp64_solution": m_err},
+        "mesh": {"build_ms": mesh_build_ms, "assembly_ms": asm_ms,
+                 "assembly_algorithmic_bytes": assembly_bytes(mesh, pr),
+                 "assembly_GB/s": assembly_bytes(mesh, pr) / (asm_ms * 1e-3) / 1e9},
         "kernels": {
             "spmv": {"avg_ms": spmv_avg_ms, "algorithmic_bytes": spmv_bytes(n, nnz), "GB/s": spmv_gbs,
                      "frac": spmv_gbs / HBM_PEAK_GBS, "calls": ks["spmv_calls"], "total_ms": ks["spmv_ms"],
@@ -429,6 +473,7 @@ def main():
         out["serving"] = serving_out
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(raw, fields_of)
+        out["mesh"]["cpu_oracle_build_ms"] = out["cpu_baseline"].pop("mesh_build_ms")
     print(json.dumps(out), flush=True)
 
 

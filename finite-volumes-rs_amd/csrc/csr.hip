@@ -129,6 +129,42 @@ static void free_csr(fvg_ctx* c, fvg_csr* a) {
     dev_free_n(c, a->val, a->nnz);
 }
 
+// A CSR whose (valid, sorted) pattern already lives on the device: copied on the
+// device, zero values, and a host copy of the pattern for the preconditioner analysis.
+int csr_from_device_pattern(fvg_ctx* c, int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* d_row_ptr,
+                            const int32_t* d_col, fvg_csr** out) {
+    *out = nullptr;
+    auto* a = new fvg_csr();
+    a->nrows = nrows;
+    a->ncols = ncols;
+    a->nnz = nnz;
+    a->h_row_ptr.resize((size_t)nrows + 1);
+    a->h_col.resize((size_t)nnz);
+    int s;
+    if ((s = dev_alloc_n(c, nrows + 1, &a->row_ptr)) || (s = dev_alloc_n(c, nnz, &a->col)) ||
+        (s = dev_alloc_n(c, nnz, &a->val)) || (s = launch_fill(c, nnz, a->val, 0.0))) {
+        (void)hipStreamSynchronize(c->stream);
+        free_csr(c, a);
+        delete a;
+        return s;
+    }
+    if (hipMemcpyAsync(a->row_ptr, d_row_ptr, sizeof(int32_t) * (size_t)(nrows + 1), hipMemcpyDeviceToDevice,
+                       c->stream) != hipSuccess ||
+        (nnz > 0 && hipMemcpyAsync(a->col, d_col, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToDevice,
+                                   c->stream) != hipSuccess))
+        s = fail(FVG_E_HIP, "CSR pattern copy failed");
+    if (s == FVG_OK) s = download(c, a->h_row_ptr.data(), d_row_ptr, nrows + 1);
+    if (s == FVG_OK) s = download(c, a->h_col.data(), d_col, nnz);
+    if (s != FVG_OK) {
+        (void)hipStreamSynchronize(c->stream);
+        free_csr(c, a);
+        delete a;
+        return s;
+    }
+    *out = a;
+    return FVG_OK;
+}
+
 }  // namespace fvg
 
 using namespace fvg;

@@ -1,39 +1,43 @@
 // mesh.hpp -- device-resident mesh used by assembly (internal).
+//
+// Everything Mesh::compute produces lives on the device, indexed by original face and
+// cell ids.  Assembly gathers per cell: `ent` lists, for every cell, the faces it owns
+// or neighbours, in ascending face id, each with its role and the CSR slot (or
+// boundary ordinal) it updates.
 #pragma once
 
 #include "internal.hpp"
 
 struct fvg_mesh {
     int dim = 0;
-    int64_t n_nodes = 0, n_faces = 0, n_cells = 0, n_bnd = 0, nnz = 0;
-    int n_colors = 0;
+    int64_t n_nodes = 0, n_faces = 0, n_cells = 0, n_bnd = 0, nnz = 0, n_ent = 0;
+    int32_t max_cell_faces = 0;
     uint64_t pattern_id = 0;
 
-    // host geometry, exactly as Mesh::compute leaves it
-    std::vector<double> face_area, face_center, face_normal, cell_volume, cell_center;
-    std::vector<int32_t> owner, neighbor, bnd_ordinal;   // neighbor -1 / bnd_ordinal -1 as applicable
-    std::vector<double> face_tinv;                        // 1/|x_j - x_i| or 1/|x_f - x_i| (Orthogonal)
+    // geometry, exactly as Mesh::compute leaves it
+    double* d_area = nullptr;          // [F]
+    double* d_face_center = nullptr;   // [F * dim]
+    double* d_face_normal = nullptr;   // [F * dim], oriented owner -> neighbour / outward
+    double* d_tinv = nullptr;          // [F] 1/|x_j - x_i| or 1/|x_f - x_i| (Orthogonal)
+    double* d_cell_vol = nullptr;      // [N]
+    double* d_cell_center = nullptr;   // [N * dim]
+    int32_t* d_owner = nullptr;        // [F]
+    int32_t* d_neigh = nullptr;        // [F], -1 on boundary faces
+    int32_t* d_bnd = nullptr;          // [F + 1] boundary ordinal (meaningful where d_neigh < 0)
     // cell-to-cell CSR pattern (from_cut_sparsity), int32
-    std::vector<int32_t> row_ptr, col, diagpos;
-    std::vector<int64_t> color_ptr;                       // faces of colour c: [color_ptr[c], color_ptr[c+1])
-
-    // device arrays, faces in colour order
-    int32_t* d_face = nullptr;     // original face id
-    int32_t* d_owner = nullptr;
-    int32_t* d_neigh = nullptr;    // -1 on boundary faces
-    int32_t* d_bnd = nullptr;      // boundary ordinal, -1 interior
-    int32_t* d_slot = nullptr;     // 4 per face: (i,i) (i,j) (j,i) (j,j)
-    double* d_area = nullptr;
-    double* d_tinv = nullptr;
-    double* d_cell_vol = nullptr;
-    int32_t* d_diagpos = nullptr;
-    int32_t* d_row_ptr = nullptr;
+    int32_t* d_row_ptr = nullptr;      // [N + 1]
+    int32_t* d_col = nullptr;          // [nnz]
+    int32_t* d_diagpos = nullptr;      // [N]
+    // per-cell assembly entries: ent[k] = {face | role << 30, slot or boundary ordinal}
+    int32_t* d_ent_ptr = nullptr;      // [N + 1]
+    int2* d_ent = nullptr;             // [n_ent]
 };
 
 namespace fvg {
-// Host part of fvg_mesh_create: geometry, pattern, slots, colouring.  Returns status.
-int build_mesh_host(fvg_mesh* m, int dim, int64_t n_nodes, const double* nodes, int64_t n_faces,
-                    const int64_t* face_ptr, const int32_t* face_nodes, const int32_t* face_patch,
-                    int64_t n_cells, const int64_t* cell_ptr, const int32_t* cell_faces,
-                    std::vector<int32_t>& slots_colour_order, std::vector<int32_t>& face_colour_order);
-}
+// entry roles of the per-cell assembly list
+constexpr int kRoleOwner = 0;       // interior face, this cell owns it: .y = slot (i, j)
+constexpr int kRoleNeighbour = 1;   // interior face, this cell is its neighbour: .y = slot (j, i)
+constexpr int kRoleBoundary = 2;    // boundary face owned by this cell: .y = boundary ordinal
+constexpr int kRoleShift = 30;
+constexpr int32_t kFaceMask = (1 << kRoleShift) - 1;
+}  // namespace fvg

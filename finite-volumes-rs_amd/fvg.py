@@ -51,7 +51,12 @@ class BudgetError(FvgError):
     """An allocation would exceed the context's HBM budget."""
 
 
-_ERRORS = {3: BudgetError, 4: ZeroPivotError, 5: NotSymmetricError}
+class MeshError(FvgError):
+    """Mesh::compute found an invalid topology or a degenerate face (the reference
+    asserts or returns Err, compute.rs:15-31, 308)."""
+
+
+_ERRORS = {3: BudgetError, 4: ZeroPivotError, 5: NotSymmetricError, 7: MeshError}
 
 
 class TermOp(C.Structure):
@@ -317,7 +322,7 @@ class Mesh:
         self.h = h
         nc, nf, nb, nnz, ncol = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
         check(lib().fvg_mesh_sizes(h, C.byref(nc), C.byref(nf), C.byref(nb), C.byref(nnz), C.byref(ncol)))
-        self.n_cells, self.n_faces, self.n_boundary_faces, self.nnz, self.n_colors = (
+        self.n_cells, self.n_faces, self.n_boundary_faces, self.nnz, self.max_cell_faces = (
             nc.value, nf.value, nb.value, nnz.value, ncol.value)
         self.face_patch = fp
         self._geom = None

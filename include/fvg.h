@@ -97,24 +97,30 @@ int fvg_vec_fill(fvg_ctx* ctx, fvg_vec* v, double value);
 int fvg_vec_dot(fvg_ctx* ctx, const fvg_vec* a, const fvg_vec* b, double* out);
 
 /* ---- mesh: Mesh<DIM> built with add_node / add_face / add_cell and
- *      Mesh::compute (src/core/mesh/mod.rs:444-527, src/core/mesh/compute.rs:12-261).
- *      Geometry is computed on the host, bit-for-bit as the reference does,
+ *      Mesh::compute (src/core/mesh/mod.rs:444-527, src/core/mesh/compute.rs:12-152).
+ *      The topology is uploaded once; owners and neighbours, geometry, normal
+ *      orientation, the cell-to-cell CSR pattern and the per-cell assembly lists
+ *      are computed on the device, bit-for-bit as the reference computes them,
  *      including its face-normal scaling in 3D (compute.rs:305) and the cell
- *      volume that follows from it (compute.rs:359-361).  Face data, the CSR
- *      pattern, per-face CSR slots and the face colouring used by assembly are
- *      then resident on the device. ------------------------------------------ */
+ *      volume that follows from it (compute.rs:359-361). ----------------------- */
 typedef struct fvg_mesh fvg_mesh;
 
 /* face_patch[f] >= 0 marks a boundary face of that patch, -1 an interior face.
  * Cells name their faces; the first cell naming a face owns it, the second is
- * its neighbour (mod.rs:511-525). */
+ * its neighbour (mod.rs:511-525).  face_ptr and cell_ptr must start at 0 and not
+ * decrease (FVG_E_INVALID otherwise).  FVG_E_MESH names the first failing face or
+ * cell: a face out of range, a face without owner, an interior face without
+ * neighbour (compute.rs:15-31), a node out of range, a degenerate face
+ * (compute.rs:308), or an interior face whose owner and neighbour are one cell
+ * (the reference would divide by a zero distance there). */
 int fvg_mesh_create(fvg_ctx* ctx, int dim, int64_t n_nodes, const double* nodes, int64_t n_faces,
                     const int64_t* face_ptr, const int32_t* face_nodes, const int32_t* face_patch,
                     int64_t n_cells, const int64_t* cell_ptr, const int32_t* cell_faces, fvg_mesh** out);
 int fvg_mesh_destroy(fvg_ctx* ctx, fvg_mesh* mesh);
-/* Sizes: cells, faces, boundary faces, nnz of the cell-to-cell CSR pattern, colours. */
+/* Sizes: cells, faces, boundary faces, nnz of the cell-to-cell CSR pattern, and the
+ * largest number of faces one cell lists. */
 int fvg_mesh_sizes(const fvg_mesh* mesh, int64_t* n_cells, int64_t* n_faces, int64_t* n_boundary_faces,
-                   int64_t* nnz, int32_t* n_colors);
+                   int64_t* nnz, int32_t* max_cell_faces);
 /* Host copies of Mesh::compute results; any output may be NULL.
  * face_center/face_normal/cell_center are (count x dim) row-major.
  * neighbor[f] is -1 on boundary faces (FaceRef::owner/neighbor mod.rs:339-351). */
@@ -173,7 +179,9 @@ typedef struct fvg_term_op {
 #define FVG_MAX_TERM_OPS 32
 
 /* Writes every value of `lhs` (whose pattern must be fvg_csr_from_mesh(mesh))
- * and every entry of `rhs`.  bc_i / bc_e carry the boundary closure
+ * and every entry of `rhs`, one row per thread: each slot receives its updates in
+ * the reference's face order, so the values are bit-identical to the reference's
+ * sequential loops.  bc_i / bc_e carry the boundary closure
  * (value_outside = bc_i * value_inside + bc_e, assembly.rs:66-72) for each
  * boundary face in face order; they may be NULL when the mesh has none. */
 int fvg_assemble(fvg_ctx* ctx, const fvg_mesh* mesh, const fvg_term_op* program, int32_t n_ops,

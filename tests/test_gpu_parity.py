@@ -3,7 +3,8 @@
 Bars (from the workload definition): CSR row_ptr/col_idx bit-exact, assembled values
 within 1e-12 relative, SpMV / factors / preconditioner sweeps bit-exact (same
 operation order), solutions within 1e-6 relative at a 1e-8 relative residual and
-iteration counts within +-2.
+iteration counts within +-2.  Geometry and assembled values are held to the tighter
+bar of bit-exactness: the device applies every face in the reference's order.
 """
 import numpy as np
 import pytest
@@ -61,8 +62,8 @@ def test_assembly_diffusion(ctx, name):
     mesh, (rp, ci, va, rhs_o), lhs, rhs = _assemble_both(ctx, raw, om, geom, prob)
     grp, gci, gva = lhs.csr()
     assert np.array_equal(grp, rp) and np.array_equal(gci, ci)
-    assert _values_close(gva, va)
-    assert _values_close(rhs.numpy(), rhs_o)
+    assert np.array_equal(gva, va)
+    assert np.array_equal(rhs.numpy(), rhs_o)
 
 
 @pytest.mark.parametrize("name", ["tri64", "prism8"])
@@ -72,8 +73,8 @@ def test_assembly_convection_diffusion(ctx, name):
     mesh, (rp, ci, va, rhs_o), lhs, rhs = _assemble_both(ctx, raw, om, geom, prob)
     grp, gci, gva = lhs.csr()
     assert np.array_equal(grp, rp) and np.array_equal(gci, ci)
-    assert _values_close(gva, va)
-    assert _values_close(rhs.numpy(), rhs_o)
+    assert np.array_equal(gva, va)
+    assert np.array_equal(rhs.numpy(), rhs_o)
 
 
 def test_assembly_term_algebra(ctx):
@@ -91,8 +92,103 @@ def test_assembly_term_algebra(ctx):
     mesh = fvg.Mesh(ctx, raw)
     lhs, rhs = fvg.assemble(eq, (bci, bce), mesh)
     rp, ci, va, rhs_o = om.assemble(ops, ff, cf, bci, bce)
-    assert _values_close(lhs.csr()[2], va)
-    assert _values_close(rhs.numpy(), rhs_o)
+    assert np.array_equal(lhs.csr()[2], va)
+    assert np.array_equal(rhs.numpy(), rhs_o)
+
+
+def _cell_lists(raw):
+    cp, cf = raw["cell_ptr"], raw["cell_faces"]
+    return [list(cf[cp[c]:cp[c + 1]]) for c in range(len(cp) - 1)]
+
+
+def _with_cells(raw, lists, **over):
+    r = dict(raw, **over)
+    r["cell_ptr"] = np.concatenate([[0], np.cumsum([len(x) for x in lists])]).astype(np.int64)
+    r["cell_faces"] = np.concatenate([np.asarray(x, np.int32) for x in lists])
+    return r
+
+
+def test_mesh_listing_edge_cases(ctx):
+    """Face lists in shuffled order, a boundary face listed twice by its cell and an
+    interior face listed by a third cell (add_cell ignores it, FaceRef::other_cell still
+    adds the neighbour to that cell's pattern row): owners, neighbours, geometry, the
+    pattern and the assembled values against the oracle, bit for bit."""
+    raw = meshgen.tri2d(6, seed=4)
+    g0 = O.OracleMesh(raw).geometry()
+    fp = raw["face_patch"]
+    lists = _cell_lists(raw)
+    rng = np.random.default_rng(8)
+    for x in lists:
+        rng.shuffle(x)
+    fb = int(np.nonzero(fp >= 0)[0][3])
+    lists[int(g0["owner"][fb])].append(fb)
+    fi = int(np.nonzero(fp < 0)[0][5])
+    i, j = int(g0["owner"][fi]), int(g0["neighbor"][fi])
+    k = max(c for c in range(len(lists)) if c not in (i, j))
+    assert k > j
+    lists[k].append(fi)
+    edited = _with_cells(raw, lists)
+    om = O.OracleMesh(edited)
+    geom = om.geometry()
+    mesh = fvg.Mesh(ctx, edited)
+    g = mesh.geometry()
+    for key in ("face_area", "face_center", "face_normal", "owner", "neighbor", "cell_volume", "cell_center"):
+        assert np.array_equal(g[key], geom[key]), key
+    prob = problems.diffusion(edited, geom, kc=meshgen.lognormal_cells(om.n_cells, 1.0, seed=2), seed=4, bc="x")
+    _, (rp, ci, va, rhs_o), lhs, rhs = _assemble_both(ctx, edited, om, geom, prob)
+    grp, gci, gva = lhs.csr()
+    assert j in gci[grp[k]:grp[k + 1]]
+    assert np.array_equal(grp, rp) and np.array_equal(gci, ci)
+    assert np.array_equal(gva, va)
+    assert np.array_equal(rhs.numpy(), rhs_o)
+
+
+def test_mesh_errors(ctx):
+    """Every invalid mesh is refused with FVG_E_MESH naming the face or cell a sequential
+    build meets first; the oracle (the reference's asserts and Err returns) refuses the
+    ones it checks too."""
+    raw = meshgen.tri2d(3, seed=2)
+    fp, fptr, fn = raw["face_patch"], raw["face_ptr"], raw["face_nodes"]
+    F, lists = len(fp), _cell_lists(raw)
+
+    def refused(r, text, oracle_too):
+        with pytest.raises(fvg.MeshError, match=text):
+            fvg.Mesh(ctx, r)
+        if oracle_too:
+            with pytest.raises(RuntimeError):
+                O.OracleMesh(r)
+
+    # a face that no cell lists (compute.rs:15-18)
+    refused(dict(raw, face_ptr=np.append(fptr, fptr[-1] + 2), face_nodes=np.append(fn, [0, 1]).astype(np.int32),
+                 face_patch=np.append(fp, 0).astype(np.int32)), f"face {F} has no owner", True)
+    # an interior face with a single cell (compute.rs:21-31)
+    fb = int(np.nonzero(fp >= 0)[0][2])
+    p2 = fp.copy()
+    p2[fb] = -1
+    refused(dict(raw, face_patch=p2), f"interior face {fb} has no neighbour", True)
+    # the same face listed twice by one cell as an interior face: owner == neighbour
+    l2 = [list(x) for x in lists]
+    owner_b = next(c for c, x in enumerate(lists) if fb in x)
+    l2[owner_b].append(fb)
+    refused(_with_cells(raw, l2, face_patch=p2), f"interior face {fb} has one cell as both", False)
+    # a 2D face with three nodes (compute.rs:271-276)
+    fptr3 = fptr.copy()
+    fptr3[1:] += 1
+    fn3 = np.insert(fn, 2, fn[0]).astype(np.int32)
+    refused(dict(raw, face_ptr=fptr3, face_nodes=fn3), "degenerate face 0", True)
+    # a node index out of range
+    fn4 = fn.copy()
+    fn4[7] = raw["nodes"].shape[0]
+    refused(dict(raw, face_nodes=fn4), "face 3 names a node out of range", False)
+    # a cell naming a face out of range
+    l5 = [list(x) for x in lists]
+    l5[2][0] = F
+    refused(_with_cells(raw, l5), "cell 2 names a face out of range", False)
+    # offsets that decrease are refused before anything reaches the device
+    cp = raw["cell_ptr"].copy()
+    cp[1] = cp[2] + 1
+    with pytest.raises(fvg.FvgError, match="FVG_E_INVALID"):
+        fvg.Mesh(ctx, dict(raw, cell_ptr=cp))
 
 
 @pytest.mark.parametrize("name", ["tri64", "tet10"])
