@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nx", type=int, default=64, help="hexes per direction (64 = the benchmark workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--direct-launches", action="store_true",
+                    help="launch the mixed-precision inner iterations kernel by kernel instead of replaying a "
+                         "captured graph (same kernels and work; for kernel tracers)")
     ap.add_argument("--legs", default="batched,serving,convdiff",
                     help="extra legs beside the headline solve: batched, serving, convdiff (comma list, or none)")
     ap.add_argument("--batch-systems", type=int, default=256)
@@ -197,7 +200,7 @@ def leg_convdiff(args, fvg, workloads):
     """Nonsymmetric convection-diffusion on 128x128x2 triangles x 64 layers of prisms
     (2,097,152 cells): assemble + ILU(0) + BiCGStab to 1e-8, FP64 and mixed."""
     raw, fields_of = workloads.convdiff_prism()
-    ctx = fvg.Context(0)
+    ctx = new_context(fvg, args)
     mesh = fvg.Mesh(ctx, raw)
     pr = fields_of(mesh.geometry())
     asm = fvg.Assembler(mesh, pr["prog"], pr["face_fields"], pr["cell_fields"], pr["bc_i"], pr["bc_e"])
@@ -250,6 +253,13 @@ def measured_traffic(kernel, args):
     with open(path) as f:
         k = json.load(f)["kernels"].get(kernel)
     return None if k is None else k["traffic_bytes"]
+
+
+def new_context(fvg, args):
+    ctx = fvg.Context(0)
+    if args.direct_launches:
+        ctx.set_tuning(fvg.TUNE_GRAPHS, 1)
+    return ctx
 
 
 def assembly_bytes(mesh, pr):
@@ -348,7 +358,7 @@ def main():
     import workloads
 
     raw, fields_of = workloads.diffusion_3d(args.nx)
-    ctx = fvg.Context(0)
+    ctx = new_context(fvg, args)
     mesh = fvg.Mesh(ctx, raw)
     pr = fields_of(mesh.geometry())
     asm = fvg.Assembler(mesh, pr["prog"], pr["face_fields"], pr["cell_fields"], pr["bc_i"], pr["bc_e"])

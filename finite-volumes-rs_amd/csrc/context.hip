@@ -264,6 +264,10 @@ int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
         if (value > 64) return fail(FVG_E_INVALID, "at most 64 Jacobi sweeps");
         c->tune_inner_sweeps = value;
         return FVG_OK;
+    case FVG_TUNE_GRAPHS:
+        if (value > 1) return fail(FVG_E_INVALID, "launch mode must be 0 or 1");
+        c->tune_graphs = value;
+        return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }
@@ -276,6 +280,7 @@ int fvg_ctx_get_tuning(const fvg_ctx* c, int32_t knob, int64_t* value) {
     case FVG_TUNE_SWEEP_POLL: *value = c->tune_sweep_poll; return FVG_OK;
     case FVG_TUNE_INNER_EXACT: *value = c->tune_inner_exact; return FVG_OK;
     case FVG_TUNE_INNER_SWEEPS: *value = c->tune_inner_sweeps; return FVG_OK;
+    case FVG_TUNE_GRAPHS: *value = c->tune_graphs; return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }

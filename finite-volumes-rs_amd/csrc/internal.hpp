@@ -63,6 +63,7 @@ struct fvg_ctx {
     int64_t tune_sweep_poll = 0;       // FVG_TUNE_SWEEP_POLL
     int64_t tune_inner_exact = 0;      // FVG_TUNE_INNER_EXACT
     int64_t tune_inner_sweeps = 0;     // FVG_TUNE_INNER_SWEEPS
+    int64_t tune_graphs = 0;           // FVG_TUNE_GRAPHS
     int num_cus = 0;
 };
 

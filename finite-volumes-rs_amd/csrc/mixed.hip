@@ -478,8 +478,8 @@ int run_mixed(fvg_ctx* c, fvg_csr* a, fvg_prec* p, const fvg_vec* b, fvg_vec* x,
         FVG_HIP(hipGetLastError());
         return FVG_OK;
     };
-    // inner iterations from a captured graph unless timing spans (event records) or the
-    // exact FP32 sweeps (cooperative launches) are in use
+    // inner iterations from a captured graph unless timing spans (event records), the
+    // exact FP32 sweeps (cooperative launches) or direct launches are asked for
     struct Graph {
         bool usable = false;
         hipGraph_t graph = nullptr;
@@ -489,7 +489,7 @@ int run_mixed(fvg_ctx* c, fvg_csr* a, fvg_prec* p, const fvg_vec* b, fvg_vec* x,
             if (graph) (void)hipGraphDestroy(graph);
         }
     } graph;
-    graph.usable = !c->timing && !c->tune_inner_exact;
+    graph.usable = !c->timing && !c->tune_inner_exact && !c->tune_graphs;
     for (int outer = 0; outer < 64; ++outer) {
         // r = b - A x (FP64) and the refinement test
         SpmvEpilogue e;

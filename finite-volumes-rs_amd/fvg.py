@@ -30,7 +30,7 @@ STATUS = {1: "FVG_E_INVALID", 2: "FVG_E_HIP", 3: "FVG_E_BUDGET", 4: "FVG_E_ZERO_
 OP_LAPLACIAN_ORTHOGONAL, OP_CONVECTION_UPWIND, OP_SOURCE, OP_LINEAR_SOURCE, OP_TIME_EULER, OP_DIVERGENCE = 1, 2, 3, 4, 5, 6
 OP_ADD, OP_SUB, OP_NEG = 16, 17, 18
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-TUNE_SWEEP_MODE, TUNE_SWEEP_WAVES, TUNE_SWEEP_POLL, TUNE_INNER_EXACT, TUNE_INNER_SWEEPS = 1, 2, 3, 4, 5
+TUNE_SWEEP_MODE, TUNE_SWEEP_WAVES, TUNE_SWEEP_POLL, TUNE_INNER_EXACT, TUNE_INNER_SWEEPS, TUNE_GRAPHS = 1, 2, 3, 4, 5, 6
 
 
 class FvgError(RuntimeError):

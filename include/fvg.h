@@ -71,7 +71,11 @@ enum {
      * exact level-scheduled sweeps (1); FVG_TUNE_INNER_SWEEPS Jacobi sweeps per factor
      * (0: 2).  The FP64 refinement reaches the requested tolerance either way. */
     FVG_TUNE_INNER_EXACT = 4,
-    FVG_TUNE_INNER_SWEEPS = 5
+    FVG_TUNE_INNER_SWEEPS = 5,
+    /* Launch mode of those FP32 inner iterations: 0 (default) replays them from a captured
+     * hipGraph, 1 launches every kernel directly (the same kernels and work; for kernel
+     * tracers that cannot follow graph launches). */
+    FVG_TUNE_GRAPHS = 6
 };
 int fvg_ctx_set_tuning(fvg_ctx* ctx, int32_t knob, int64_t value);
 int fvg_ctx_get_tuning(const fvg_ctx* ctx, int32_t knob, int64_t* value);

@@ -58,6 +58,30 @@ def _check_mixed(ctx, name, kind, solver, prec):
     assert info.iterations <= 4 * io["iterations"] + 20
 
 
+@pytest.mark.parametrize("solver,prec,kind", [("cg", "ic", "sym"), ("bicgstab", "ilu", "nonsym")])
+def test_mixed_graph_replay_and_direct_launches_agree(ctx, solver, prec, kind):
+    """The inner iterations replayed from a captured graph and launched one by one run the
+    same kernels in the same order: identical solutions, bit for bit."""
+    rp, ci, va, b = _oracle_system("tet10", kind)
+    n = rp.shape[0] - 1
+    A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+    M = (fvg.IncompleteCholesky if prec == "ic" else fvg.IncompleteLowerUpper).from_matrix(A, 0)
+    B = fvg.DistributedVector.from_data(ctx, b)
+    opts = fvg.LinearSolverOptions(1e-8, 0.0, 3000, precision=fvg.PRECISION_MIXED)
+    fn = fvg.conjugate_gradient if solver == "cg" else fvg.bi_conjugate_gradient_stab
+    out = []
+    try:
+        for direct in (0, 1, 0):
+            ctx.set_tuning(fvg.TUNE_GRAPHS, direct)
+            x = fvg.DistributedVector(ctx, n)
+            info = fn(x, A, B, M, opts)
+            out.append((x.numpy(), info.iterations, info.outer_iterations))
+    finally:
+        ctx.set_tuning(fvg.TUNE_GRAPHS, 0)
+    for xg, it, outer in out[1:]:
+        assert np.array_equal(xg, out[0][0]) and it == out[0][1] and outer == out[0][2]
+
+
 @pytest.mark.parametrize("mode,waves,poll", [(1, 0, 0), (0, 4, 0), (0, 64, 0), (0, 0, 0), (0, 0, 1), (0, 0, 2)])
 def test_sweep_launch_shapes_give_identical_bits(ctx, mode, waves, poll):
     """One launch per level, and the persistent sweep at several wave counts, all give the
