@@ -22,6 +22,7 @@
 #include <string>
 #include <unordered_map>
 #include <unordered_set>
+#include <type_traits>
 #include <vector>
 
 namespace orc {
@@ -113,6 +114,84 @@ template <int N> Vec<N> normal_of(const Vec<N>& s, const Vec<N>& r) {
 // ---------------------------------------------------------------------------------
 // Mesh<DIM>  (src/core/mesh/mod.rs, src/core/mesh/compute.rs)
 // ---------------------------------------------------------------------------------
+// Matrix<N, N>  (src/core/matrix.rs): rows of Vector<N>
+// ---------------------------------------------------------------------------------
+template <int N>
+struct Mat {
+    Vec<N> r[N];
+    Mat() { for (int i = 0; i < N; ++i) for (int j = 0; j < N; ++j) r[i][j] = 0.0; }
+};
+
+// Vector::outer (matrix.rs:105-115)
+template <int N>
+Mat<N> outer(const Vec<N>& a, const Vec<N>& b) {
+    Mat<N> m;
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) m.r[i][j] = a[i] * b[j];
+    return m;
+}
+
+// Matrix += Matrix (matrix.rs:160-168)
+template <int N>
+void add_to(Mat<N>& a, const Mat<N>& b) {
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) a.r[i][j] += b.r[i][j];
+}
+
+// Matrix * Vector (matrix.rs:221-230): one dot product per row
+template <int N>
+Vec<N> matvec(const Mat<N>& a, const Vec<N>& v) {
+    Vec<N> out;
+    for (int i = 0; i < N; ++i) out[i] = dot(a.r[i], v);
+    return out;
+}
+
+// Matrix::inv (matrix.rs:320-383): Gauss-Jordan without pivoting unless a pivot is
+// below 1e-14, then a swap with row `maxi`, where maxi is the largest entry of ROW k
+// right of the diagonal (the reference's search, kept as written).  False when the
+// reference returns Err(SingularMatrix).
+template <int N>
+bool inverse(Mat<N> a, Mat<N>& out) {
+    out = Mat<N>();
+    for (int i = 0; i < N; ++i) out.r[i][i] = 1.0;
+    for (int k = 0; k < N - 1; ++k) {
+        double piv = a.r[k][k];
+        if (std::fabs(piv) < 1e-14) {
+            int maxi = k;
+            double maxp = std::fabs(piv);
+            for (int j = k + 1; j < N; ++j) {
+                double pj = std::fabs(a.r[k][j]);
+                if (pj > maxp) { maxi = j; maxp = pj; }
+            }
+            if (maxi == k) return false;
+            std::swap(a.r[k], a.r[maxi]);
+            std::swap(out.r[k], out.r[maxi]);
+        }
+        piv = a.r[k][k];
+        for (int i = k + 1; i < N; ++i) {
+            double f = -a.r[i][k] / piv;
+            a.r[i] = a.r[i] + a.r[k] * f;
+            out.r[i] = out.r[i] + out.r[k] * f;
+        }
+    }
+    for (int k = 0; k < N; ++k) {
+        double piv = a.r[k][k];
+        if (std::fabs(piv) < 1e-14) return false;
+        a.r[k] = a.r[k] / piv;
+        out.r[k] = out.r[k] / piv;
+    }
+    for (int k = N - 1; k >= 1; --k) {
+        double piv = a.r[k][k];
+        for (int i = 0; i < k; ++i) {
+            double f = -a.r[i][k] / piv;
+            a.r[i] = a.r[i] + a.r[k] * f;
+            out.r[i] = out.r[i] + out.r[k] * f;
+        }
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------------
 constexpr size_t NONE = (size_t)-1;
 
 template <int DIM>
@@ -135,6 +214,8 @@ struct Mesh {
     Sparsity face_nodes, cell_faces, cell_to_cell;
     std::vector<FaceData<DIM>> face_data;
     std::vector<CellData<DIM>> cell_data;
+    std::vector<Vec<DIM>> cell_face_grad;   // per cell-face listing (compute.rs:156)
+    std::vector<Vec<DIM>> cell_diag_grad;   // per cell (compute.rs:157)
 
     void add_node(const Vec<DIM>& p) { nodes.push_back(p); }   // mod.rs:444-449
 
@@ -266,6 +347,32 @@ struct Mesh {
             for (size_t o : others) cell_to_cell.push_to_major(o);
             cell_to_cell.close_major();
         }
+        // compute.rs:154-200: least-squares gradient coefficients.  For every listing k
+        // of cell c, dgc is the vector to the other cell's centre (or to the face centre
+        // on a boundary); g = sum dgc dgc^T, coef_k = g^-1 dgc, diag_c = -sum coef_k.
+        cell_face_grad.assign(cell_faces.minor_len(), Vec<DIM>());
+        cell_diag_grad.assign(n_cells(), Vec<DIM>());
+        for (size_t c = 0; c < n_cells(); ++c) {
+            Mat<DIM> g;
+            const Vec<DIM> xc = cell_data[c].center;
+            auto dgc_of = [&](size_t k) {
+                const FaceData<DIM>& fd = face_data[cell_faces.minors[k]];
+                if (fd.kind == 2) return fd.center - xc;
+                size_t co = fd.neighbor == c ? fd.owner : fd.neighbor;
+                return cell_data[co].center - xc;
+            };
+            for (size_t k = cell_faces.major_start(c); k < cell_faces.major_end(c); ++k) {
+                Vec<DIM> dgc = dgc_of(k);
+                add_to(g, outer(dgc, dgc));
+            }
+            Mat<DIM> ginv;
+            if (!inverse(g, ginv)) throw std::runtime_error("singular gradient matrix");
+            for (size_t k = cell_faces.major_start(c); k < cell_faces.major_end(c); ++k) {
+                Vec<DIM> gi = matvec(ginv, dgc_of(k));
+                cell_face_grad[k] = gi;
+                cell_diag_grad[c] = cell_diag_grad[c] - gi;
+            }
+        }
     }
 };
 
@@ -363,6 +470,9 @@ enum {
     OP_LINEAR_SOURCE = 4,
     OP_TIME_EULER = 5,
     OP_DIVERGENCE = 6,
+    OP_LAPLACIAN_CORRECTED = 7,
+    OP_CONVECTION_LINEAR = 8,
+    OP_CONVECTION_LIMITED_LINEAR = 9,
     OP_ADD = 16,
     OP_SUB = 17,
     OP_NEG = 18,
@@ -371,6 +481,62 @@ struct TermOp { int32_t op, arg0, arg1, arg2; double scalar; };
 
 struct Triple { double li, lj, r; };
 struct Pair { double l, r; };
+
+// FaceNormalGradientScheme::terms for a value with nc components: (gi, gj, grhs[nc]).
+// Orthogonal (facengrad/orthogonal.rs:26-44) when grads is null, else
+// Corrected(grads, reduction) (facengrad/corrected.rs:34-67) with grads holding nc x DIM
+// entries per cell; for a vector value the gradient is a Matrix<nc, DIM> and every
+// operation acts row by row.
+template <int DIM>
+void fngrad_terms(const Mesh<DIM>& mesh, size_t f, int nc, const double* grads, double reduction, double& gi,
+                  double& gj, double* grhs) {
+    const FaceData<DIM>& fd = mesh.face_data[f];
+    const Vec<DIM> ci = mesh.cell_data[fd.owner].center;
+    if (!grads) {
+        double dx = fd.kind == 1 ? norm(mesh.cell_data[fd.neighbor].center - ci) : norm(fd.center - ci);
+        double t = 1.0 / dx;
+        gi = 1.0 * (-t);
+        gj = 1.0 * t;
+        for (int c = 0; c < nc; ++c) grhs[c] = 0.0;
+        return;
+    }
+    auto grad_row = [&](size_t cell, int c) {
+        Vec<DIM> g;
+        for (int d = 0; d < DIM; ++d) g[d] = grads[(cell * nc + c) * DIM + d];
+        return g;
+    };
+    const Vec<DIM> n = fd.normal;
+    if (fd.kind == 1) {
+        Vec<DIM> d = mesh.cell_data[fd.neighbor].center - ci;
+        double c_corr = 1.0 / dot(n, d);
+        double w = dot(fd.center - ci, n) / dot(d, n);
+        for (int c = 0; c < nc; ++c) {
+            Vec<DIM> gf = grad_row(fd.owner, c) * (1.0 - w) + grad_row(fd.neighbor, c) * w;
+            double expl = dot(gf, n - d * c_corr);
+            grhs[c] = expl * reduction;
+        }
+        gi = 1.0 * (-c_corr);
+        gj = 1.0 * c_corr;
+    } else {
+        Vec<DIM> d = fd.center - ci;
+        double c_corr = 1.0 / dot(n, d);
+        for (int c = 0; c < nc; ++c) {
+            double expl = dot(grad_row(fd.owner, c), n - d * c_corr);
+            grhs[c] = expl * reduction;
+        }
+        gi = 1.0 * (-c_corr);
+        gj = 1.0 * c_corr;
+    }
+}
+
+// Linear interpolation weight (faceinterp/linear.rs:22-44, limitedlinear.rs:46-55)
+template <int DIM>
+double linear_weight(const Mesh<DIM>& mesh, size_t f) {
+    const FaceData<DIM>& fd = mesh.face_data[f];
+    const Vec<DIM> fc = fd.center, cc = mesh.cell_data[fd.owner].center;
+    const Vec<DIM> nc = fd.kind == 1 ? mesh.cell_data[fd.neighbor].center : fd.center;
+    return dot(fd.normal, nc - fc) / std::fmax(dot(fd.normal, nc - cc), 1e-14);
+}
 
 template <int DIM>
 struct Assembler {
@@ -413,6 +579,41 @@ struct Assembler {
                 st[sp++] = {0.0, 0.0, -flux};
                 break;
             }
+            case OP_LAPLACIAN_CORRECTED: {
+                // corrected.rs:34-67 then laplacian.rs:35-45; arg1 = cell gradient field
+                // (n x DIM), scalar = corrector_reduction
+                double l0, l1, r;
+                fngrad_terms(mesh, f, 1, cell_fields[o.arg1], o.scalar, l0, l1, &r);
+                double diff = face_fields[o.arg0][f];
+                st[sp++] = {diff * l0, diff * l1, -(diff * r)};
+                break;
+            }
+            case OP_CONVECTION_LINEAR: {   // linear.rs:22-44 then convection.rs:36-44
+                double w = linear_weight(mesh, f);
+                double l0 = 1.0 * w, l1 = 1.0 * (1.0 - w), r = 0.0;
+                double flux = face_fields[o.arg0][f];
+                st[sp++] = {l0 * flux, l1 * flux, (-r) * flux};
+                break;
+            }
+            case OP_CONVECTION_LIMITED_LINEAR: {
+                // limitedlinear.rs:24-72 then convection.rs:36-44; arg0 = convected flux,
+                // arg1 = the scheme's flux (upwind side), arg2 = face limiters
+                double sflux = face_fields[o.arg1][f];
+                int u = sflux >= 0.0 ? 0 : 1, dn = u == 0 ? 1 : 0;
+                double t[2] = {0.0, 0.0};
+                double lim = face_fields[o.arg2][f];
+                double w = linear_weight(mesh, f);
+                double wu = u == 0 ? w : 1.0 - w;
+                t[u] = wu * lim + (1.0 - lim);
+                t[dn] = (1.0 - wu) * lim;
+                double l0, l1, r = 0.0;
+                if (fd.kind == 1) { l0 = 1.0 * t[0]; l1 = 1.0 * t[1]; }
+                else if (sflux > 0.0) { l0 = 1.0; l1 = 0.0; }
+                else { l0 = 0.0; l1 = 1.0; }
+                double flux = face_fields[o.arg0][f];
+                st[sp++] = {l0 * flux, l1 * flux, (-r) * flux};
+                break;
+            }
             case OP_SOURCE: case OP_LINEAR_SOURCE: case OP_TIME_EULER:
                 st[sp++] = {0.0, 0.0, 0.0};   // Term::face_terms default (terms/mod.rs:32-34)
                 break;
@@ -446,6 +647,7 @@ struct Assembler {
                 break;
             }
             case OP_LAPLACIAN_ORTHOGONAL: case OP_CONVECTION_UPWIND: case OP_DIVERGENCE:
+            case OP_LAPLACIAN_CORRECTED: case OP_CONVECTION_LINEAR: case OP_CONVECTION_LIMITED_LINEAR:
                 st[sp++] = {0.0, 0.0};   // Term::cell_terms default (terms/mod.rs:28-30)
                 break;
             case OP_ADD: { Pair b = st[--sp], a = st[--sp]; st[sp++] = {a.l + b.l, a.r + b.r}; break; }
@@ -491,6 +693,115 @@ struct Assembler {
         }
     }
 };
+
+// ---------------------------------------------------------------------------------
+// fvm tools: compute_gradients (tools/gradients.rs:41-79) and compute_limiters
+// (tools/limiters.rs:47-126) for values with nc components (nc = 1: f64 fields;
+// nc > 1: Vector<nc> fields with Matrix<nc, DIM> gradients).  Boundary closures are
+// (bci per boundary face, bce per boundary face and component), in boundary-face order.
+// ---------------------------------------------------------------------------------
+template <int DIM>
+void compute_gradients(const Mesh<DIM>& mesh, const std::vector<size_t>& bnd_of_face, int nc, const double* values,
+                       const double* bci, const double* bce, double* out) {
+    for (size_t c = 0; c < mesh.n_cells(); ++c) {
+        for (int q = 0; q < nc; ++q) {
+            const double vc = values[c * nc + q];
+            Vec<DIM> grad = mesh.cell_diag_grad[c] * vc;            // mul_to_gradient(own_grad)
+            for (size_t k = mesh.cell_faces.major_start(c); k < mesh.cell_faces.major_end(c); ++k) {
+                const size_t f = mesh.cell_faces.minors[k];
+                const FaceData<DIM>& fd = mesh.face_data[f];
+                const Vec<DIM>& g = mesh.cell_face_grad[k];
+                if (fd.kind == 1) {
+                    size_t o = fd.neighbor == c ? fd.owner : fd.neighbor;   // FaceRef::other_cell
+                    grad = grad + g * values[o * nc + q];
+                } else {
+                    size_t b = bnd_of_face[f];
+                    double fval = bci[b] * vc + bce[b * nc + q];
+                    grad = grad + g * fval;
+                }
+            }
+            for (int d = 0; d < DIM; ++d) out[(c * nc + q) * DIM + d] = grad[d];
+        }
+    }
+}
+
+// limiters/mod.rs:17-82
+inline double limiter_value(int kind, double k, double r) {
+    switch (kind) {
+    case 0: return std::fmin(2.0 * r * k, 1.0);                            // LimitedLinear(k)
+    case 1: return std::fmin(1.0 * r, 1.0);                                // MinMod
+    case 2: return 2.0 * r / (1.0 + r);                                    // VanLeer
+    case 3: return (r * r + r) / (r * r + 1.0);                            // VanAlbada
+    case 4: case 5: {                                                      // Sweby(k), SuperBee = Sweby(2)
+        double beta = kind == 5 ? 2.0 : k;
+        double a = std::fmin(r, beta), b = std::fmin(beta * r, 1.0);
+        return std::fmax(std::fmax(a, b), 0.0);
+    }
+    }
+    throw std::runtime_error("bad limiter kind");
+}
+
+// interp: 0 Linear (linear.rs), 1 Upwind(flux) (upwind.rs:26-33).  fgrads: null for
+// Orthogonal, else the Corrected scheme's gradient field with `reduction`.
+template <int DIM>
+void compute_limiters(const Mesh<DIM>& mesh, const std::vector<size_t>& bnd_of_face, int nc, const double* values,
+                      const double* grads, int limiter, double k, int interp, const double* flux,
+                      const double* fgrads, double reduction, const double* bci, const double* bce, double* out) {
+    std::vector<double> grhs((size_t)nc), fng((size_t)nc), gdd((size_t)nc);
+    for (size_t f = 0; f < mesh.n_faces(); ++f) {
+        const FaceData<DIM>& fd = mesh.face_data[f];
+        const size_t i = fd.owner;
+        const Vec<DIM> ci = mesh.cell_data[i].center;
+        double gi, gj;
+        fngrad_terms(mesh, f, nc, fgrads, reduction, gi, gj, grhs.data());
+        Vec<DIM> delta;
+        if (fd.kind == 1) {
+            const size_t j = fd.neighbor;
+            delta = mesh.cell_data[j].center - ci;
+            double ii, ij;
+            if (interp == 0) { double w = linear_weight(mesh, f); ii = 1.0 * w; ij = 1.0 * (1.0 - w); }
+            else if (flux[f] > 0.0) { ii = 1.0; ij = 0.0; }
+            else { ii = 0.0; ij = 1.0; }
+            for (int q = 0; q < nc; ++q) {
+                double v = 0.0;
+                v += gi * values[i * nc + q];
+                v += gj * values[j * nc + q];
+                v += grhs[q];
+                fng[q] = v;
+                Vec<DIM> gface;
+                for (int d = 0; d < DIM; ++d) {
+                    gface[d] += grads[(i * nc + q) * DIM + d] * ii;
+                    gface[d] += grads[(j * nc + q) * DIM + d] * ij;
+                }
+                gdd[q] = dot(gface, delta);
+            }
+        } else {
+            const size_t b = bnd_of_face[f];
+            delta = fd.center - ci;
+            for (int q = 0; q < nc; ++q) {
+                double fval = bci[b] * values[i * nc + q] + bce[b * nc + q];
+                double v = 0.0;
+                v += gi * values[i * nc + q];
+                v += gj * fval;
+                v += grhs[q];
+                fng[q] = v;
+                Vec<DIM> gface;
+                for (int d = 0; d < DIM; ++d) gface[d] = grads[(i * nc + q) * DIM + d];
+                gdd[q] = dot(gface, delta);
+            }
+        }
+        double r;   // LimiterFrom::to_limiter_r (limiters.rs:17-44)
+        if (nc == 1) {
+            r = std::fabs(fng[0]) > 1e-12 ? std::fmax(2.0 * gdd[0] / (norm(delta) * fng[0]) - 1.0, 0.0) : 1.0;
+        } else {
+            double ff = 0.0, fg = 0.0;
+            for (int q = 0; q < nc; ++q) ff += fng[q] * fng[q];
+            for (int q = 0; q < nc; ++q) fg += fng[q] * gdd[q];
+            r = std::sqrt(ff) > 1e-12 ? std::fmax(2.0 * fg / (norm(delta) * ff) - 1.0, 0.0) : 1.0;
+        }
+        out[f] = limiter_value(limiter, k, r);
+    }
+}
 
 // ---------------------------------------------------------------------------------
 // Preconditioners  (src/linalg/preconditioners/*)
@@ -1023,6 +1334,61 @@ int orc_assemble(void* hv, const TermOp* prog, int nops, const double* const* fa
         g_err = e.what();
         return 1;
     }
+}
+
+// Gradient coefficients of Mesh::compute: per cell-face listing (DIM each) and per cell.
+void orc_mesh_grad_coefficients(void* hv, double* face_coef, double* diag_coef) {
+    auto* h = (MeshHandle*)hv;
+    auto out = [&](auto& m, int dim) {
+        for (size_t k = 0; k < m.cell_face_grad.size(); ++k)
+            for (int d = 0; d < dim; ++d) face_coef[k * dim + d] = m.cell_face_grad[k][d];
+        for (size_t c = 0; c < m.cell_diag_grad.size(); ++c)
+            for (int d = 0; d < dim; ++d) diag_coef[c * dim + d] = m.cell_diag_grad[c][d];
+    };
+    if (h->dim == 2) out(*h->m2, 2);
+    else out(*h->m3, 3);
+}
+
+int orc_gradients(void* hv, int nc, const double* values, const double* bci, const double* bce, double* out) {
+    auto* h = (MeshHandle*)hv;
+    if (h->dim == 2) compute_gradients(*h->m2, h->bnd_of_face, nc, values, bci, bce, out);
+    else compute_gradients(*h->m3, h->bnd_of_face, nc, values, bci, bce, out);
+    return 0;
+}
+
+int orc_limiters(void* hv, int nc, const double* values, const double* grads, int limiter, double k, int interp,
+                 const double* flux, const double* fgrads, double reduction, const double* bci, const double* bce,
+                 double* out) {
+    try {
+        auto* h = (MeshHandle*)hv;
+        if (h->dim == 2)
+            compute_limiters(*h->m2, h->bnd_of_face, nc, values, grads, limiter, k, interp, flux, fgrads, reduction,
+                             bci, bce, out);
+        else
+            compute_limiters(*h->m3, h->bnd_of_face, nc, values, grads, limiter, k, interp, flux, fgrads, reduction,
+                             bci, bce, out);
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// Mat<N>::inverse on one row-major N x N matrix (N = 2 or 3); 0 on success, 1 singular.
+int orc_matrix_inverse(int n, const double* a, double* out) {
+    auto run = [&](auto tag) {
+        constexpr int N = decltype(tag)::value;
+        Mat<N> m, r;
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) m.r[i][j] = a[i * N + j];
+        if (!inverse(m, r)) return 1;
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) out[i * N + j] = r.r[i][j];
+        return 0;
+    };
+    if (n == 2) return run(std::integral_constant<int, 2>());
+    if (n == 3) return run(std::integral_constant<int, 3>());
+    return 1;
 }
 
 void orc_spmv(int64_t n, const int32_t* row_ptr, const int32_t* col, const double* val, const double* x, double* y) {

@@ -16,7 +16,11 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 OP = dict(LAPLACIAN_ORTHOGONAL=1, CONVECTION_UPWIND=2, SOURCE=3, LINEAR_SOURCE=4, TIME_EULER=5,
-          DIVERGENCE=6, ADD=16, SUB=17, NEG=18)
+          DIVERGENCE=6, LAPLACIAN_CORRECTED=7, CONVECTION_LINEAR=8, CONVECTION_LIMITED_LINEAR=9,
+          ADD=16, SUB=17, NEG=18)
+# limiter schemes (schemes/limiters/mod.rs) and gradient interpolation schemes
+LIMITER = dict(limited_linear=0, minmod=1, vanleer=2, vanalbada=3, sweby=4, superbee=5)
+INTERP = dict(linear=0, upwind=1)
 
 
 class TermOp(C.Structure):
@@ -45,6 +49,13 @@ def lib():
         L.orc_assemble.restype = C.c_int
         L.orc_assemble.argtypes = [P, P, C.c_int, P, P, P, P, P, P, P, P]
         L.orc_spmv.argtypes = [C.c_int64, P, P, P, P, P]
+        L.orc_mesh_grad_coefficients.argtypes = [P, P, P]
+        L.orc_gradients.restype = C.c_int
+        L.orc_gradients.argtypes = [P, C.c_int, P, P, P, P]
+        L.orc_limiters.restype = C.c_int
+        L.orc_limiters.argtypes = [P, C.c_int, P, P, C.c_int, C.c_double, C.c_int, P, P, C.c_double, P, P, P]
+        L.orc_matrix_inverse.restype = C.c_int
+        L.orc_matrix_inverse.argtypes = [C.c_int, P, P]
         L.orc_prec_new.restype = P
         L.orc_prec_new.argtypes = [C.c_int, C.c_int64, P, P, P, C.c_int, C.POINTER(C.c_int)]
         L.orc_prec_free.argtypes = [P]
@@ -104,6 +115,45 @@ class OracleMesh:
                                                                "neighbor", "cell_volume", "cell_center")])
         return g
 
+    def grad_coefficients(self):
+        """Mesh::compute's least-squares gradient coefficients (compute.rs:154-200): one
+        DIM-vector per cell-face listing (cell_faces order) and one per cell."""
+        D = self.dim
+        face = np.empty((self._keep[5].shape[0], D))
+        diag = np.empty((self.n_cells, D))
+        lib().orc_mesh_grad_coefficients(self.h, _p(face), _p(diag))
+        return face, diag
+
+    def gradients(self, values, bc_i, bc_e):
+        """compute_gradients (tools/gradients.rs:41-79).  values: (n,) or (n, nc);
+        bc_e: (n_bnd,) or (n_bnd, nc).  Returns (n, DIM) or (n, nc, DIM)."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        nc = 1 if v.ndim == 1 else v.shape[1]
+        out = np.empty((self.n_cells, nc, self.dim))
+        bi = np.ascontiguousarray(bc_i, dtype=np.float64)
+        be = np.ascontiguousarray(bc_e, dtype=np.float64)
+        lib().orc_gradients(self.h, nc, _p(v), _p(bi), _p(be), _p(out))
+        return out[:, 0, :] if v.ndim == 1 else out
+
+    def limiters(self, values, grads, limiter="limited_linear", k=1.0, interp="upwind", flux=None,
+                 corrected_grads=None, reduction=1.0, bc_i=None, bc_e=None):
+        """compute_limiters (tools/limiters.rs:47-126) with the given limiter scheme,
+        gradient interpolation (linear / upwind on `flux`) and face-normal gradient
+        (Orthogonal, or Corrected on `corrected_grads` with `reduction`)."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        nc = 1 if v.ndim == 1 else v.shape[1]
+        g = np.ascontiguousarray(grads, dtype=np.float64)
+        fl = None if flux is None else np.ascontiguousarray(flux, dtype=np.float64)
+        cg = None if corrected_grads is None else np.ascontiguousarray(corrected_grads, dtype=np.float64)
+        bi = np.ascontiguousarray(bc_i, dtype=np.float64)
+        be = np.ascontiguousarray(bc_e, dtype=np.float64)
+        out = np.empty(self.n_faces)
+        rc = lib().orc_limiters(self.h, nc, _p(v), _p(g), LIMITER[limiter], k, INTERP[interp], _p(fl), _p(cg),
+                                reduction, _p(bi), _p(be), _p(out))
+        if rc:
+            raise RuntimeError("oracle limiters: " + _err())
+        return out
+
     def assemble(self, program, face_fields, cell_fields, bc_i, bc_e):
         """program: list of (op, arg0, arg1, arg2, scalar). Returns (row_ptr, col, values, rhs)."""
         ops = (TermOp * max(1, len(program)))(*[TermOp(*t) for t in program])
@@ -122,6 +172,13 @@ class OracleMesh:
         if rc:
             raise RuntimeError("oracle assemble: " + _err())
         return rp, ci, va, rhs
+
+
+def matrix_inverse(a):
+    """Matrix::inv (matrix.rs:320-383) of a 2x2 or 3x3 matrix; None where it returns Err."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    out = np.empty_like(a)
+    return None if lib().orc_matrix_inverse(a.shape[0], _p(a), _p(out)) else out
 
 
 def spmv(row_ptr, col, val, x):

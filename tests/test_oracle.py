@@ -334,3 +334,195 @@ def test_ic_rejects_nonsymmetric():
     S = sp.csr_matrix(A)
     with pytest.raises(O.NotSymmetric):
         O.OraclePrec("ic", S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.copy(), 0)
+
+
+# ---- fvm tools: gradient coefficients, compute_gradients, compute_limiters and the
+#      Corrected / Linear / LimitedLinear schemes.  The reference holds no tests for
+#      them; what pins them is the least-squares identity, exactness on linear fields,
+#      the closed form of the limiter ratio and the schemes' limiting cases. ----------
+
+TOOL_MESHES = {
+    "tri": lambda: meshgen.tri2d(8, seed=3),
+    "tet": lambda: meshgen.tet3d(4, seed=5),
+    "prism": lambda: meshgen.prism3d(4, layers=3, seed=2),
+}
+
+
+def _linear_setup(raw, ncomp=1):
+    om = O.OracleMesh(raw)
+    g = om.geometry()
+    D = om.dim
+    rng = np.random.default_rng(ncomp + 7)
+    a = rng.normal(size=(ncomp, D))
+    b0 = rng.normal(size=ncomp)
+    u = g["cell_center"] @ a.T + b0                      # (n, ncomp)
+    bf = np.nonzero(raw["face_patch"] >= 0)[0]
+    bci = np.zeros(len(bf))
+    bce = g["face_center"][bf] @ a.T + b0
+    return om, g, a, u, bci, bce, bf
+
+
+def test_matrix_inverse_against_numpy_and_its_pivot_rule():
+    rng = np.random.default_rng(3)
+    for n in (2, 3):
+        for _ in range(20):
+            a = rng.normal(size=(n, n)) + n * np.eye(n)
+            assert np.allclose(O.matrix_inverse(a) @ a, np.eye(n), atol=1e-12)
+    # zero leading pivot: rows swap with the one the largest entry of row 0 names
+    p = np.array([[0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]])
+    assert np.array_equal(O.matrix_inverse(p), p)
+    # singular matrices return Err(SingularMatrix)
+    assert O.matrix_inverse(np.array([[1.0, 2.0], [2.0, 4.0]])) is None
+    assert O.matrix_inverse(np.zeros((3, 3))) is None
+
+
+@pytest.mark.parametrize("name", sorted(TOOL_MESHES))
+def test_gradient_coefficients_reconstruct_the_identity(name):
+    """coef_k = G^-1 dgc_k with G = sum dgc dgc^T, so sum_k coef_k dgc_k^T = I and
+    diag = -sum_k coef_k (compute.rs:154-200)."""
+    raw = TOOL_MESHES[name]()
+    om = O.OracleMesh(raw)
+    g = om.geometry()
+    fc, dc = om.grad_coefficients()
+    cp, cf = raw["cell_ptr"], raw["cell_faces"]
+    D = om.dim
+    for c in range(om.n_cells):
+        S = np.zeros((D, D))
+        tot = np.zeros(D)
+        for k in range(cp[c], cp[c + 1]):
+            f = cf[k]
+            if g["neighbor"][f] < 0:
+                dgc = g["face_center"][f] - g["cell_center"][c]
+            else:
+                o = g["owner"][f] if g["neighbor"][f] == c else g["neighbor"][f]
+                dgc = g["cell_center"][o] - g["cell_center"][c]
+            S += np.outer(fc[k], dgc)
+            tot += fc[k]
+        assert np.allclose(S, np.eye(D), atol=1e-10), c
+        assert np.allclose(dc[c], -tot, rtol=1e-12, atol=1e-12 * np.abs(tot).max())
+
+
+@pytest.mark.parametrize("name", sorted(TOOL_MESHES))
+def test_gradients_exact_for_linear_fields_and_componentwise(name):
+    om, g, a, u, bci, bce, bf = _linear_setup(TOOL_MESHES[name](), ncomp=2)
+    gr = om.gradients(u, bci, bce)
+    assert gr.shape == (om.n_cells, 2, om.dim)
+    assert np.abs(gr - a[None]).max() <= 1e-10 * np.abs(a).max()
+    # a Vector<2> field is two f64 fields, bit for bit
+    for q in range(2):
+        assert np.array_equal(om.gradients(u[:, q].copy(), bci, bce[:, q].copy()), gr[:, q, :])
+
+
+@pytest.mark.parametrize("name", sorted(TOOL_MESHES))
+def test_limiters_of_linear_fields(name):
+    """Orthogonal face gradient: r = 1 on every face of a linear field, so each limiter
+    takes its value at 1.  Corrected face gradient with the exact gradient: fngrad = a.n,
+    so r = max(2 (a.d) / (|d| a.n) - 1, 0) in closed form (limiters.rs:17-44)."""
+    om, g, a, u, bci, bce, bf = _linear_setup(TOOL_MESHES[name](), ncomp=1)
+    u, bce, a1 = u[:, 0].copy(), bce[:, 0].copy(), a[0]
+    gr = om.gradients(u, bci, bce)
+    for lim, k, want, tol in (("limited_linear", 1.0, 1.0, 0.0), ("minmod", 1.0, 1.0, 1e-9),
+                              ("vanleer", 1.0, 1.0, 1e-9), ("vanalbada", 1.0, 1.0, 1e-9),
+                              ("sweby", 1.5, 1.0, 1e-9), ("superbee", 1.0, 1.0, 1e-9)):
+        L = om.limiters(u, gr, lim, k, "linear", None, None, 1.0, bci, bce)
+        assert np.abs(L - want).max() <= tol, lim
+    exact = np.tile(a1, (om.n_cells, 1))
+    rng = np.random.default_rng(2)
+    flux = rng.normal(size=om.n_faces)
+    L = om.limiters(u, exact, "limited_linear", 0.5, "upwind", flux, exact, 1.0, bci, bce)
+    inside = g["neighbor"] >= 0
+    xj = np.where(inside[:, None], g["cell_center"][np.maximum(g["neighbor"], 0)], g["face_center"])
+    d = xj - g["cell_center"][g["owner"]]
+    an = g["face_normal"] @ a1
+    r = np.maximum(2.0 * (d @ a1) / (np.linalg.norm(d, axis=1) * an) - 1.0, 0.0)
+    want = np.minimum(r, 1.0)
+    ok = np.abs(an) > 1e-6
+    assert ok.sum() > 0.9 * om.n_faces
+    assert np.allclose(L[ok], want[ok], rtol=1e-9, atol=1e-9)
+
+
+def test_vector_limiter_ratio_closed_form():
+    om, g, a, u, bci, bce, bf = _linear_setup(TOOL_MESHES["tet"](), ncomp=2)
+    exact = np.tile(a, (om.n_cells, 1, 1))
+    L = om.limiters(u, exact, "limited_linear", 0.5, "linear", None, exact, 1.0, bci, bce)
+    inside = g["neighbor"] >= 0
+    xj = np.where(inside[:, None], g["cell_center"][np.maximum(g["neighbor"], 0)], g["face_center"])
+    d = xj - g["cell_center"][g["owner"]]
+    fng = g["face_normal"] @ a.T                         # (F, 2)
+    gdd = d @ a.T
+    ff = (fng * fng).sum(1)
+    r = np.maximum(2.0 * (fng * gdd).sum(1) / (np.linalg.norm(d, axis=1) * ff) - 1.0, 0.0)
+    ok = np.sqrt(ff) > 1e-6
+    assert np.allclose(L[ok], np.minimum(r, 1.0)[ok], rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("name", sorted(TOOL_MESHES))
+def test_corrected_laplacian_is_exact_for_linear_fields(name):
+    """With the exact gradient the Corrected face-normal gradient is a.n on every face,
+    so -laplacian(Corrected) with constant diffusivity and exact Dirichlet values leaves
+    A u - b = -k sum_f A_f a.n_f = 0 in every cell; the Orthogonal scheme does not on
+    these non-orthogonal meshes."""
+    om, g, a, u, bci, bce, bf = _linear_setup(TOOL_MESHES[name](), ncomp=1)
+    u, bce, a1 = u[:, 0].copy(), bce[:, 0].copy(), a[0]
+    kf = np.full(om.n_faces, 1.7)
+    grads = np.tile(a1, (om.n_cells, 1)).ravel()
+    res = {}
+    for scheme, prog in (("corrected", [(O.OP["LAPLACIAN_CORRECTED"], 0, 0, 0, 1.0), (O.OP["NEG"], 0, 0, 0, 0.0)]),
+                         ("orthogonal", [(O.OP["LAPLACIAN_ORTHOGONAL"], 0, 0, 0, 0.0), (O.OP["NEG"], 0, 0, 0, 0.0)])):
+        rp, ci, va, b = om.assemble(prog, [kf], [grads], bci, bce)
+        S = sp.csr_matrix((va, ci, rp), shape=(om.n_cells, om.n_cells))
+        res[scheme] = np.abs(S @ u - b).max() / (np.abs(va).max() * np.abs(u).max())
+    assert res["corrected"] < 1e-12
+    assert res["orthogonal"] > 1e-6
+
+
+@pytest.mark.parametrize("name", sorted(TOOL_MESHES))
+def test_limited_linear_limits_are_upwind_and_linear(name):
+    """LimitedLinear with every limiter 0 is Upwind, bit for bit; with every limiter 1 it
+    is Linear interpolation (limitedlinear.rs:24-72)."""
+    raw = TOOL_MESHES[name]()
+    om = O.OracleMesh(raw)
+    rng = np.random.default_rng(4)
+    flux = rng.normal(size=om.n_faces)
+    nb = om.n_bnd
+    bci, bce = rng.uniform(0, 1, nb), rng.uniform(0, 1, nb)
+    ll = lambda lim: om.assemble([(O.OP["CONVECTION_LIMITED_LINEAR"], 0, 0, 1, 0.0)], [flux, lim], [], bci, bce)
+    up = om.assemble([(O.OP["CONVECTION_UPWIND"], 0, 0, 0, 0.0)], [flux], [], bci, bce)
+    z = ll(np.zeros(om.n_faces))
+    assert np.array_equal(z[2], up[2]) and np.array_equal(z[3], up[3])
+    one = ll(np.ones(om.n_faces))
+    lin = om.assemble([(O.OP["CONVECTION_LINEAR"], 0, 0, 0, 0.0)], [flux], [], bci, bce)
+    inside_only = np.allclose(one[2], lin[2], rtol=1e-12, atol=1e-14)
+    # boundary faces keep the upwind closure in LimitedLinear, Linear interpolates there
+    g = om.geometry()
+    if nb == 0:
+        assert inside_only
+    S1 = sp.csr_matrix((one[2], one[1], one[0]), shape=(om.n_cells, om.n_cells))
+    SL = sp.csr_matrix((lin[2], lin[1], lin[0]), shape=(om.n_cells, om.n_cells))
+    interior_cells = np.setdiff1d(np.arange(om.n_cells), g["owner"][g["neighbor"] < 0])
+    D = (S1 - SL).tocsr()[interior_cells]
+    assert np.abs(D.data).max(initial=0.0) <= 1e-12 * np.abs(lin[2]).max()
+
+
+def test_linear_interpolation_weight_closed_form():
+    """Linear (linear.rs:22-44) puts weight w = n.(x_j - x_f) / n.(x_j - x_i) on the owner:
+    with the flux of a single interior face set, its owner row holds w * flux on the
+    diagonal and (1 - w) * flux at the neighbour."""
+    raw = TOOL_MESHES["tet"]()
+    om = O.OracleMesh(raw)
+    g = om.geometry()
+    inner = np.nonzero(g["neighbor"] >= 0)[0][:12]
+    for f in inner:
+        flux = np.zeros(om.n_faces)
+        flux[f] = 2.5
+        rp, ci, va, b = om.assemble([(O.OP["CONVECTION_LINEAR"], 0, 0, 0, 0.0)], [flux], [], np.zeros(om.n_bnd),
+                                    np.zeros(om.n_bnd))
+        S = sp.csr_matrix((va, ci, rp), shape=(om.n_cells, om.n_cells))
+        i, j = g["owner"][f], g["neighbor"][f]
+        n, xf = g["face_normal"][f], g["face_center"][f]
+        xi, xj = g["cell_center"][i], g["cell_center"][j]
+        w = n @ (xj - xf) / (n @ (xj - xi))
+        A = g["face_area"][f]
+        assert np.isclose(S[i, i], w * 2.5 * A, rtol=1e-12)
+        assert np.isclose(S[i, j], (1.0 - w) * 2.5 * A, rtol=1e-12)
+        assert np.isclose(S[j, i], -w * 2.5 * A, rtol=1e-12)
