@@ -10,7 +10,7 @@
 // per cell in the reference's operation order.
 #include <cstring>
 
-#include "mesh.hpp"
+#include "schemes.hpp"
 
 namespace fvg {
 
@@ -22,6 +22,7 @@ struct TermProgram {
     fvg_term_op ops[FVG_MAX_TERM_OPS];
     const double* face_fields[kMaxFields];
     const double* cell_fields[kMaxFields];
+    MeshView geo;   // geometry for the Corrected, Linear and LimitedLinear schemes
 };
 
 struct Triple {
@@ -29,8 +30,10 @@ struct Triple {
 };
 
 // Term::face_terms of the whole expression for face f (original id), face geometry
-// supplied.  laplacian.rs:35-45 + orthogonal.rs:26-44, convection.rs:36-44 +
-// upwind.rs:26-33, divergence.rs:36-45, algebra operations.rs.
+// supplied.  laplacian.rs:35-45 + orthogonal.rs:26-44 or corrected.rs:34-67,
+// convection.rs:36-44 + upwind.rs:26-33, linear.rs:22-44 or limitedlinear.rs:24-72,
+// divergence.rs:36-45, algebra operations.rs.
+template <int D>
 __device__ __forceinline__ Triple face_terms(const TermProgram& P, int f, double tinv) {
     Triple st[kMaxStack];
     int sp = 0;
@@ -54,6 +57,45 @@ __device__ __forceinline__ Triple face_terms(const TermProgram& P, int f, double
         case FVG_OP_DIVERGENCE:
             st[sp++] = Triple{0.0, 0.0, -P.face_fields[o.arg0][f]};
             break;
+        case FVG_OP_LAPLACIAN_CORRECTED: {
+            double l0, l1, r;
+            corrected_terms<D, 1>(P.geo, f, P.geo.owner[f], P.geo.neigh[f], P.cell_fields[o.arg1], o.scalar, l0, l1,
+                                  &r);
+            const double diff = P.face_fields[o.arg0][f];
+            st[sp++] = Triple{diff * l0, diff * l1, -(diff * r)};
+            break;
+        }
+        case FVG_OP_CONVECTION_LINEAR: {
+            const double w = linear_weight<D>(P.geo, f, P.geo.owner[f], P.geo.neigh[f]);
+            const double l0 = 1.0 * w, l1 = 1.0 * (1.0 - w), r = 0.0;
+            const double flux = P.face_fields[o.arg0][f];
+            st[sp++] = Triple{l0 * flux, l1 * flux, (-r) * flux};
+            break;
+        }
+        case FVG_OP_CONVECTION_LIMITED_LINEAR: {
+            const int j = P.geo.neigh[f];
+            const double sflux = P.face_fields[o.arg1][f];
+            const double lim = P.face_fields[o.arg2][f];
+            const double w = linear_weight<D>(P.geo, f, P.geo.owner[f], j);
+            // t[u] = wu lim + (1 - lim), t[d] = (1 - wu) lim with u the upwind side
+            const bool up0 = sflux >= 0.0;
+            const double wu = up0 ? w : 1.0 - w;
+            const double tu = wu * lim + (1.0 - lim), td = (1.0 - wu) * lim;
+            double l0, l1;
+            if (j >= 0) {
+                l0 = 1.0 * (up0 ? tu : td);
+                l1 = 1.0 * (up0 ? td : tu);
+            } else if (sflux > 0.0) {   // boundary: outlet / inlet
+                l0 = 1.0;
+                l1 = 0.0;
+            } else {
+                l0 = 0.0;
+                l1 = 1.0;
+            }
+            const double r = 0.0, flux = P.face_fields[o.arg0][f];
+            st[sp++] = Triple{l0 * flux, l1 * flux, (-r) * flux};
+            break;
+        }
         case FVG_OP_ADD: {
             Triple b = st[--sp], a = st[--sp];
             st[sp++] = Triple{a.li + b.li, a.lj + b.lj, a.r + b.r};
@@ -133,6 +175,7 @@ __device__ __forceinline__ void cell_terms(const TermProgram& P, int c, double& 
 // Row c of the cell loop (assembly.rs:26-32: A_cc = l*vol, b_c = r*vol, every other
 // slot of the row from zero) followed by the face loop (assembly.rs:35-76) restricted
 // to the faces that update row c, in face order.
+template <int D>
 __global__ void __launch_bounds__(kBlock) k_assemble_rows(TermProgram P, int n, const int32_t* __restrict__ row_ptr,
                                                           const int32_t* __restrict__ diagpos,
                                                           const double* __restrict__ vol,
@@ -155,7 +198,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(TermProgram P, int n, 
     for (int e = ent_ptr[c]; e < ent_ptr[c + 1]; ++e) {
         const int2 en = ent[e];
         const int f = en.x & kFaceMask, role = en.x >> kRoleShift;
-        const Triple t = face_terms(P, f, tinv[f]);
+        const Triple t = face_terms<D>(P, f, tinv[f]);
         const double A = area[f];
         const double li = t.li * A, lj = t.lj * A, rf = t.r * A;
         if (role == kRoleOwner) {          // lhs[i,i] += li; lhs[i,j] += lj; rhs[i] += r
@@ -185,7 +228,18 @@ static int validate_program(const fvg_term_op* prog, int n, int nff, int ncf) {
         switch (o.op) {
         case FVG_OP_LAPLACIAN_ORTHOGONAL:
         case FVG_OP_DIVERGENCE:
+        case FVG_OP_CONVECTION_LINEAR:
             if (o.arg0 < 0 || o.arg0 >= nff) return fail(FVG_E_INVALID, "face field index out of range");
+            ++depth;
+            break;
+        case FVG_OP_LAPLACIAN_CORRECTED:
+            if (o.arg0 < 0 || o.arg0 >= nff) return fail(FVG_E_INVALID, "face field index out of range");
+            if (o.arg1 < 0 || o.arg1 >= ncf) return fail(FVG_E_INVALID, "cell field index out of range");
+            ++depth;
+            break;
+        case FVG_OP_CONVECTION_LIMITED_LINEAR:
+            if (o.arg0 < 0 || o.arg0 >= nff || o.arg1 < 0 || o.arg1 >= nff || o.arg2 < 0 || o.arg2 >= nff)
+                return fail(FVG_E_INVALID, "face field index out of range");
             ++depth;
             break;
         case FVG_OP_CONVECTION_UPWIND:
@@ -246,17 +300,37 @@ int fvg_assemble(fvg_ctx* c, const fvg_mesh* m, const fvg_term_op* program, int3
             return fail(FVG_E_INVALID, "face field length must equal the number of faces");
         P.face_fields[k] = face_fields[k]->d;
     }
+    // a cell field read as a Corrected gradient holds n_cells x dim entries, any other n_cells
+    bool is_grad[kMaxFields] = {}, is_value[kMaxFields] = {};
+    for (int k = 0; k < n_ops; ++k) {
+        const fvg_term_op& o = program[k];
+        if (o.op == FVG_OP_LAPLACIAN_CORRECTED) is_grad[o.arg1] = true;
+        if (o.op == FVG_OP_SOURCE || o.op == FVG_OP_LINEAR_SOURCE) is_value[o.arg0] = true;
+        if (o.op == FVG_OP_TIME_EULER) {
+            is_value[o.arg0] = true;
+            if (o.arg1 >= 0) is_value[o.arg1] = is_value[o.arg2] = true;
+        }
+    }
     for (int k = 0; k < n_cell_fields; ++k) {
-        if (!cell_fields[k] || cell_fields[k]->n != m->n_cells)
-            return fail(FVG_E_INVALID, "cell field length must equal the number of cells");
+        if (is_grad[k] && is_value[k]) return fail(FVG_E_INVALID, "a cell field cannot be a value and a gradient");
+        const int64_t want = is_grad[k] ? m->n_cells * m->dim : m->n_cells;
+        if (!cell_fields[k] || cell_fields[k]->n != want)
+            return fail(FVG_E_INVALID, is_grad[k] ? "a gradient field must hold n_cells x dim entries"
+                                                  : "cell field length must equal the number of cells");
         P.cell_fields[k] = cell_fields[k]->d;
     }
+    P.geo = mesh_view(m);
     const int n = (int)m->n_cells;
     const double* bi = m->n_bnd ? bc_i->d : nullptr;
     const double* be = m->n_bnd ? bc_e->d : nullptr;
-    hipLaunchKernelGGL(k_assemble_rows, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, P, n, m->d_row_ptr,
-                       m->d_diagpos, m->d_cell_vol, m->d_ent_ptr, m->d_ent, m->d_area, m->d_tinv, bi, be, lhs->val,
-                       rhs->d);
+    if (m->dim == 2)
+        hipLaunchKernelGGL(k_assemble_rows<2>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, P, n, m->d_row_ptr,
+                           m->d_diagpos, m->d_cell_vol, m->d_ent_ptr, m->d_ent, m->d_area, m->d_tinv, bi, be,
+                           lhs->val, rhs->d);
+    else
+        hipLaunchKernelGGL(k_assemble_rows<3>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, P, n, m->d_row_ptr,
+                           m->d_diagpos, m->d_cell_vol, m->d_ent_ptr, m->d_ent, m->d_area, m->d_tinv, bi, be,
+                           lhs->val, rhs->d);
     FVG_HIP(hipGetLastError());
     lhs->version += 1;
     return FVG_OK;

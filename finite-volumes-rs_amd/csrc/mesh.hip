@@ -22,6 +22,8 @@
 //                      lengths gives row_ptr and a second pass compacts the columns.
 //   assembly lists     per cell, the faces it owns or neighbours in ascending face
 //                      id, with the CSR slot each one updates.
+//   gradient coeffs    one thread per cell: the least-squares coefficients of every
+//                      listing (compute.rs:154-200), with Matrix::inv restated.
 //
 // The arithmetic repeats the reference's operation for operation (component sums in
 // index order, division by the count, the sub-triangle loop in 3D).  The library is
@@ -49,7 +51,7 @@ enum : int {
     kLevelSlot = 4,        // index = cell
 };
 enum : int { kErrFaceRange = 0, kErrNoOwner = 1, kErrNoNeighbour = 2, kErrSelf = 3, kErrNodeRange = 4,
-             kErrDegenerate = 5, kErrSlot = 6 };
+             kErrDegenerate = 5, kErrSlot = 6, kErrSingular = 7 };
 
 __device__ __forceinline__ void report(unsigned long long* err, int level, int64_t index, int kind) {
     atomicMin(err, ((unsigned long long)level << 48) | ((unsigned long long)index << 3) | (unsigned long long)kind);
@@ -384,6 +386,158 @@ __global__ void __launch_bounds__(kBlock) k_mesh_rows(int64_t n_cells, const int
     }
 }
 
+// ---- least-squares gradient coefficients (compute.rs:154-200) -----------------------
+
+// Matrix::inv (matrix.rs:320-383) on rows a[D][D]: Gauss-Jordan without pivoting unless
+// a pivot is below 1e-14, then a swap with row `maxi`, maxi being the largest entry of
+// row k right of the diagonal (the reference's search as written).  False where the
+// reference returns Err(SingularMatrix).
+template <int D>
+__device__ bool invert(double (&a)[D][D], double (&out)[D][D]) {
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) out[i][j] = i == j ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < D - 1; ++k) {
+        double piv = a[k][k];
+        if (fabs(piv) < 1e-14) {
+            int maxi = k;
+            double maxp = fabs(piv);
+            for (int j = k + 1; j < D; ++j) {
+                const double pj = fabs(a[k][j]);
+                if (pj > maxp) {
+                    maxi = j;
+                    maxp = pj;
+                }
+            }
+            if (maxi == k) return false;
+#pragma unroll
+            for (int j = 0; j < D; ++j) {   // row swap, written without dynamic indexing
+                double t0 = a[k][j], t1 = out[k][j];
+#pragma unroll
+                for (int m = k + 1; m < D; ++m)
+                    if (m == maxi) {
+                        a[k][j] = a[m][j];
+                        a[m][j] = t0;
+                        out[k][j] = out[m][j];
+                        out[m][j] = t1;
+                    }
+            }
+        }
+        piv = a[k][k];
+#pragma unroll
+        for (int i = k + 1; i < D; ++i) {
+            const double f = -a[i][k] / piv;
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                a[i][j] = a[i][j] + a[k][j] * f;
+                out[i][j] = out[i][j] + out[k][j] * f;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double piv = a[k][k];
+        if (fabs(piv) < 1e-14) return false;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            a[k][j] = a[k][j] / piv;
+            out[k][j] = out[k][j] / piv;
+        }
+    }
+#pragma unroll
+    for (int k = D - 1; k >= 1; --k) {
+        const double piv = a[k][k];
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+            const double f = -a[i][k] / piv;
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                a[i][j] = a[i][j] + a[k][j] * f;
+                out[i][j] = out[i][j] + out[k][j] * f;
+            }
+        }
+    }
+    return true;
+}
+
+// One thread per cell: dgc_k to the other cell's centre (or the face centre on a
+// boundary) for every listing k, G = sum dgc dgc^T, coef_k = G^-1 dgc_k, and
+// diag = -sum coef_k.  Also records, per listing, the other cell or the boundary
+// ordinal that compute_gradients reads.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_mesh_grad(int64_t n_cells, const int64_t* __restrict__ cptr,
+                                                      const int32_t* __restrict__ cfaces,
+                                                      const int32_t* __restrict__ owner,
+                                                      const int32_t* __restrict__ neigh,
+                                                      const int32_t* __restrict__ bnd,
+                                                      const double* __restrict__ fcenter,
+                                                      const double* __restrict__ ccenter,
+                                                      int32_t* __restrict__ lptr, int32_t* __restrict__ lother,
+                                                      double* __restrict__ coef, double* __restrict__ diag,
+                                                      unsigned long long* __restrict__ err) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cells) return;
+    const int64_t s = cptr[c], e = cptr[c + 1];
+    lptr[c] = (int32_t)s;
+    if (c == n_cells - 1) lptr[n_cells] = (int32_t)e;
+    double xc[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xc[d] = ccenter[c * D + d];
+    auto dgc_of = [&](int64_t k, double (&v)[D]) {
+        const int32_t f = cfaces[k], j = neigh[f];
+        int32_t code;
+        const double* x;
+        if (j < 0) {
+            code = -bnd[f] - 1;
+            x = fcenter + (int64_t)f * D;
+        } else {
+            const int32_t o = j == (int32_t)c ? owner[f] : j;   // FaceRef::other_cell
+            code = o;
+            x = ccenter + (int64_t)o * D;
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) v[d] = x[d] - xc[d];
+        return code;
+    };
+    double g[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) g[i][j] = 0.0;
+    for (int64_t k = s; k < e; ++k) {
+        double v[D];
+        lother[k] = dgc_of(k, v);
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) g[i][j] += v[i] * v[j];
+    }
+    double ginv[D][D];
+    if (!invert<D>(g, ginv)) {
+        report(err, kLevelSlot + 1, c, kErrSingular);
+        return;
+    }
+    double dg[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) dg[d] = 0.0;
+    for (int64_t k = s; k < e; ++k) {
+        double v[D];
+        (void)dgc_of(k, v);
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            double x = 0.0;   // Matrix * Vector: one dot per row (matrix.rs:221-230)
+#pragma unroll
+            for (int j = 0; j < D; ++j) x += ginv[i][j] * v[j];
+            coef[k * D + i] = x;
+            dg[i] -= x;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) diag[c * D + d] = dg[d];
+}
+
 // ---- exclusive scan of int32 counts (three passes over tiles of 1024) ---------------
 
 constexpr int kScanThreads = 256, kScanItems = 4, kScanTile = kScanThreads * kScanItems;
@@ -506,6 +660,7 @@ std::string mesh_error(unsigned long long key) {
     case kErrNodeRange: return "face " + std::to_string(idx) + " names a node out of range";
     case kErrDegenerate: return "degenerate face " + std::to_string(idx);
     case kErrSlot: return "cell " + std::to_string(idx) + ": face slot missing from the CSR pattern";
+    case kErrSingular: return "cell " + std::to_string(idx) + ": singular gradient matrix (compute.rs:180)";
     default: return "mesh check " + std::to_string(level) + " failed";
     }
 }
@@ -522,11 +677,16 @@ void free_mesh(fvg_ctx* c, fvg_mesh* m) {
     dev_free_n(c, m->d_cell_center, N * D);
     dev_free_n(c, m->d_owner, F);
     dev_free_n(c, m->d_neigh, F);
+    dev_free_n(c, m->d_bnd, F + 1);
     dev_free_n(c, m->d_row_ptr, N + 1);
     dev_free_n(c, m->d_col, m->nnz);
     dev_free_n(c, m->d_diagpos, N);
     dev_free_n(c, m->d_ent_ptr, N + 1);
     dev_free_n(c, m->d_ent, m->n_ent);
+    dev_free_n(c, m->d_lptr, N + 1);
+    dev_free_n(c, m->d_lother, m->n_list);
+    dev_free_n(c, m->d_gcoef, m->n_list * D);
+    dev_free_n(c, m->d_gdiag, N * D);
 }
 
 // offsets must start at 0 and never decrease (then every range lies inside the list)
@@ -551,7 +711,7 @@ int build_mesh(fvg_ctx* c, fvg_mesh* m, int64_t n_nodes, const double* nodes, co
     Scratch t(c);
     double* d_nodes;
     int64_t *d_fptr, *d_cptr, *d_meta, *d_toff_b, *d_toff_r, *d_toff_e;
-    int32_t *d_fnodes, *d_patch, *d_cfaces, *d_isb, *d_bnd, *d_cand, *d_rowcnt, *d_entf, *d_entcnt;
+    int32_t *d_fnodes, *d_patch, *d_cfaces, *d_isb, *d_cand, *d_rowcnt, *d_entf, *d_entcnt;
     unsigned long long *d_own, *d_nb;
     FVG_TRY(t.get(n_nodes * D, &d_nodes));
     FVG_TRY(t.get(F + 1, &d_fptr));
@@ -580,7 +740,8 @@ int build_mesh(fvg_ctx* c, fvg_mesh* m, int64_t n_nodes, const double* nodes, co
     FVG_TRY(dev_alloc_n(c, F * D, &m->d_face_normal));
     FVG_TRY(dev_alloc_n(c, F, &m->d_tinv));
     FVG_TRY(t.get(F, &d_isb));
-    FVG_TRY(t.get(F + 1, &d_bnd));
+    FVG_TRY(dev_alloc_n(c, F + 1, &m->d_bnd));
+    int32_t* d_bnd = m->d_bnd;
     FVG_TRY(t.get(scan_tiles(F), &d_toff_b));
     hipLaunchKernelGGL(k_mesh_owner, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, N, F, d_cptr, d_cfaces, d_own,
                        err);
@@ -628,9 +789,20 @@ int build_mesh(fvg_ctx* c, fvg_mesh* m, int64_t n_nodes, const double* nodes, co
     hipLaunchKernelGGL(k_mesh_rows, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, N, d_cptr, d_cand,
                        m->d_row_ptr, m->d_col, m->d_diagpos, d_entf, m->d_ent_ptr, m->d_owner, m->d_neigh, d_bnd,
                        m->d_ent, err);
+    m->n_list = Lc;
+    FVG_TRY(dev_alloc_n(c, N + 1, &m->d_lptr));
+    FVG_TRY(dev_alloc_n(c, Lc, &m->d_lother));
+    FVG_TRY(dev_alloc_n(c, Lc * D, &m->d_gcoef));
+    FVG_TRY(dev_alloc_n(c, N * D, &m->d_gdiag));
+    hipLaunchKernelGGL(k_mesh_grad<D>, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, N, d_cptr, d_cfaces,
+                       m->d_owner, m->d_neigh, d_bnd, m->d_face_center, m->d_cell_center, m->d_lptr, m->d_lother,
+                       m->d_gcoef, m->d_gdiag, err);
     FVG_HIP(hipGetLastError());
     FVG_TRY(download(c, meta, d_meta, 1));
-    if ((unsigned long long)meta[0] != kUnsetKey) return fail(FVG_E_MISSING_ENTRY, mesh_error((unsigned long long)meta[0]));
+    if ((unsigned long long)meta[0] != kUnsetKey) {
+        const unsigned long long key = (unsigned long long)meta[0];
+        return fail((key & 7) == kErrSlot ? FVG_E_MISSING_ENTRY : FVG_E_MESH, mesh_error(key));
+    }
     return FVG_OK;
 }
 

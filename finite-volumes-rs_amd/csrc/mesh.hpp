@@ -31,6 +31,13 @@ struct fvg_mesh {
     // per-cell assembly entries: ent[k] = {face | role << 30, slot or boundary ordinal}
     int32_t* d_ent_ptr = nullptr;      // [N + 1]
     int2* d_ent = nullptr;             // [n_ent]
+    // least-squares gradient coefficients (compute.rs:154-200), per cell-face listing in
+    // the order the cells list their faces
+    int64_t n_list = 0;
+    int32_t* d_lptr = nullptr;         // [N + 1] listing offsets
+    int32_t* d_lother = nullptr;       // [n_list] other cell, or -(boundary ordinal + 1)
+    double* d_gcoef = nullptr;         // [n_list * dim]
+    double* d_gdiag = nullptr;         // [N * dim]
 };
 
 namespace fvg {

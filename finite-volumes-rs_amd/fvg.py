@@ -28,6 +28,10 @@ STATUS = {1: "FVG_E_INVALID", 2: "FVG_E_HIP", 3: "FVG_E_BUDGET", 4: "FVG_E_ZERO_
           6: "FVG_E_MISSING_ENTRY", 7: "FVG_E_MESH", 8: "FVG_E_TIMEOUT", 9: "FVG_E_UNSUPPORTED"}
 
 OP_LAPLACIAN_ORTHOGONAL, OP_CONVECTION_UPWIND, OP_SOURCE, OP_LINEAR_SOURCE, OP_TIME_EULER, OP_DIVERGENCE = 1, 2, 3, 4, 5, 6
+OP_LAPLACIAN_CORRECTED, OP_CONVECTION_LINEAR, OP_CONVECTION_LIMITED_LINEAR = 7, 8, 9
+LIMITER_LIMITED_LINEAR, LIMITER_MINMOD, LIMITER_VANLEER, LIMITER_VANALBADA, LIMITER_SWEBY, LIMITER_SUPERBEE = range(6)
+INTERP_LINEAR, INTERP_UPWIND = 0, 1
+FNGRAD_ORTHOGONAL, FNGRAD_CORRECTED = 0, 1
 OP_ADD, OP_SUB, OP_NEG = 16, 17, 18
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 TUNE_SWEEP_MODE, TUNE_SWEEP_WAVES, TUNE_SWEEP_POLL, TUNE_INNER_EXACT, TUNE_INNER_SWEEPS, TUNE_GRAPHS = 1, 2, 3, 4, 5, 6
@@ -62,6 +66,11 @@ _ERRORS = {3: BudgetError, 4: ZeroPivotError, 5: NotSymmetricError, 7: MeshError
 class TermOp(C.Structure):
     _fields_ = [("op", C.c_int32), ("arg0", C.c_int32), ("arg1", C.c_int32), ("arg2", C.c_int32),
                 ("scalar", C.c_double)]
+
+
+class LimiterSpec(C.Structure):
+    _fields_ = [("limiter", C.c_int32), ("interp", C.c_int32), ("fngrad", C.c_int32), ("reserved", C.c_int32),
+                ("k", C.c_double), ("reduction", C.c_double), ("flux", C.c_void_p), ("fn_grads", C.c_void_p)]
 
 
 class SolverOptions(C.Structure):
@@ -128,6 +137,8 @@ def lib():
         "fvg_spmv": [P, P, P, P],
         "fvg_spmv_time": [P, P, P, P, C.c_int32, C.POINTER(C.c_double)],
         "fvg_assemble": [P, P, P, C.c_int32, P, C.c_int32, P, C.c_int32, P, P, P, P],
+        "fvg_compute_gradients": [P, P, C.c_int32, P, P, P, P],
+        "fvg_compute_limiters": [P, P, C.c_int32, P, P, P, P, P, P],
         "fvg_identity_create": [P, C.c_int64, C.POINTER(P)],
         "fvg_diagonal_create": [P, P, C.POINTER(P)],
         "fvg_ic_create": [P, P, C.c_int32, C.POINTER(P)],
@@ -421,15 +432,68 @@ class DistributedMatrix:
 # terms (src/fvm/terms) -> postfix program
 # ---------------------------------------------------------------------------------
 
+def _field(a):
+    """A field argument: a DistributedVector stays on the device, anything else becomes a
+    contiguous float64 host array (uploaded when the equation is assembled)."""
+    if isinstance(a, DistributedVector):
+        return a
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if a.ndim == 1 else a.ravel()
+
+
 class Orthogonal:
     """facengrad::Orthogonal (src/fvm/schemes/facengrad/orthogonal.rs)."""
+
+
+class Corrected:
+    """facengrad::Corrected(&gradients, corrector_reduction)
+    (src/fvm/schemes/facengrad/corrected.rs:10-30); gradients is n_cells x dim."""
+
+    def __init__(self, gradients, corrector_reduction):
+        self.gradients = _field(gradients)
+        self.reduction = float(corrector_reduction)
 
 
 class Upwind:
     """faceinterp::Upwind(&flux) (src/fvm/schemes/faceinterp/upwind.rs)."""
 
     def __init__(self, flux):
-        self.flux = np.ascontiguousarray(flux, dtype=np.float64)
+        self.flux = _field(flux)
+
+
+class Linear:
+    """faceinterp::Linear (src/fvm/schemes/faceinterp/linear.rs)."""
+
+
+class LimitedLinear:
+    """faceinterp::LimitedLinear(&flux, &limiters) (src/fvm/schemes/faceinterp/limitedlinear.rs)."""
+
+    def __init__(self, flux, limiters):
+        self.flux = _field(flux)
+        self.limiters = _field(limiters)
+
+
+class _Limiter:
+    def __init__(self, kind, k=1.0):
+        self.kind, self.k = kind, float(k)
+
+
+class limiters:
+    """schemes::limiters (src/fvm/schemes/limiters/mod.rs)."""
+
+    @staticmethod
+    def LimitedLinear(k):
+        return _Limiter(LIMITER_LIMITED_LINEAR, k)
+
+    MinMod = staticmethod(lambda: _Limiter(LIMITER_MINMOD))
+    VanLeer = staticmethod(lambda: _Limiter(LIMITER_VANLEER))
+    VanAlbada = staticmethod(lambda: _Limiter(LIMITER_VANALBADA))
+
+    @staticmethod
+    def Sweby(beta):
+        return _Limiter(LIMITER_SWEBY, beta)
+
+    SuperBee = staticmethod(lambda: _Limiter(LIMITER_SUPERBEE))
 
 
 class Euler:
@@ -479,9 +543,21 @@ class Term:
                 walk(t.kw["a"])
                 ops.append((OP_NEG, 0, 0, 0, 0.0))
             elif k == "laplacian":
-                ops.append((OP_LAPLACIAN_ORTHOGONAL, fidx(ff, t.kw["k"]), 0, 0, 0.0))
+                sc = t.kw["scheme"]
+                if isinstance(sc, Corrected):
+                    ops.append((OP_LAPLACIAN_CORRECTED, fidx(ff, t.kw["k"]), fidx(cf, sc.gradients), 0,
+                                sc.reduction))
+                else:
+                    ops.append((OP_LAPLACIAN_ORTHOGONAL, fidx(ff, t.kw["k"]), 0, 0, 0.0))
             elif k == "convection":
-                ops.append((OP_CONVECTION_UPWIND, fidx(ff, t.kw["flux"]), fidx(ff, t.kw["upwind"]), 0, 0.0))
+                sc = t.kw["scheme"]
+                if isinstance(sc, Upwind):
+                    ops.append((OP_CONVECTION_UPWIND, fidx(ff, t.kw["flux"]), fidx(ff, sc.flux), 0, 0.0))
+                elif isinstance(sc, Linear):
+                    ops.append((OP_CONVECTION_LINEAR, fidx(ff, t.kw["flux"]), 0, 0, 0.0))
+                else:
+                    ops.append((OP_CONVECTION_LIMITED_LINEAR, fidx(ff, t.kw["flux"]), fidx(ff, sc.flux),
+                                fidx(ff, sc.limiters), 0.0))
             elif k == "divergence":
                 ops.append((OP_DIVERGENCE, fidx(ff, t.kw["flux"]), 0, 0, 0.0))
             elif k == "source":
@@ -503,17 +579,18 @@ class Term:
 
 
 def laplacian(scheme, diffusivity):
-    """laplacian(scheme, &k_f) (src/fvm/terms/laplacian.rs:31-46); Orthogonal scheme."""
-    if not isinstance(scheme, Orthogonal):
-        raise FvgError(9, "only the Orthogonal face-normal-gradient scheme is implemented")
-    return Term("laplacian", k=np.ascontiguousarray(diffusivity, dtype=np.float64))
+    """laplacian(scheme, &k_f) (src/fvm/terms/laplacian.rs:31-46); Orthogonal or Corrected."""
+    if not isinstance(scheme, (Orthogonal, Corrected)):
+        raise FvgError(9, "face-normal-gradient schemes: Orthogonal, Corrected")
+    return Term("laplacian", k=_field(diffusivity), scheme=scheme)
 
 
 def convection(scheme, flux):
-    """convection(Upwind(&w), &flux) (src/fvm/terms/convection.rs:32-45)."""
-    if not isinstance(scheme, Upwind):
-        raise FvgError(9, "only the Upwind face-interpolation scheme is implemented")
-    return Term("convection", flux=np.ascontiguousarray(flux, dtype=np.float64), upwind=scheme.flux)
+    """convection(scheme, &flux) (src/fvm/terms/convection.rs:32-45); Upwind, Linear or
+    LimitedLinear."""
+    if not isinstance(scheme, (Upwind, Linear, LimitedLinear)):
+        raise FvgError(9, "face-interpolation schemes: Upwind, Linear, LimitedLinear")
+    return Term("convection", flux=_field(flux), scheme=scheme)
 
 
 def divergence(flux):
@@ -579,8 +656,9 @@ class Assembler:
         bnde = np.ascontiguousarray(bnde, dtype=np.float64)
         if bndi.shape != (nb,) or bnde.shape != (nb,):
             raise FvgError(1, "boundary closure arrays must have one entry per boundary face")
-        self.fvecs = [DistributedVector.from_data(ctx, a) for a in face_fields]
-        self.cvecs = [DistributedVector.from_data(ctx, a) for a in cell_fields]
+        dev = lambda a: a if isinstance(a, DistributedVector) else DistributedVector.from_data(ctx, a)
+        self.fvecs = [dev(a) for a in face_fields]
+        self.cvecs = [dev(a) for a in cell_fields]
         self.bi = DistributedVector.from_data(ctx, bndi) if nb else None
         self.be = DistributedVector.from_data(ctx, bnde) if nb else None
         self.ops = list(ops)
@@ -609,6 +687,74 @@ def assemble_program(mesh, ops, face_fields, cell_fields, bndi, bnde, lhs=None, 
         rhs = DistributedVector.from_size(mesh.ctx, mesh.n_cells)
     asm.run(lhs, rhs)
     return lhs, rhs
+
+
+# ---------------------------------------------------------------------------------
+# fvm tools (src/fvm/tools)
+# ---------------------------------------------------------------------------------
+
+def _closure_vecs(mesh, boundary_condition, ncomp):
+    """The boundary closure as device vectors: bc_i per boundary face, bc_e per boundary
+    face and component (cached on the FaceConstraints object)."""
+    bc = boundary_condition
+    if not isinstance(bc, FaceConstraints):
+        bc = FaceConstraints(*bc)
+    nb = mesh.n_boundary_faces
+    if nb == 0:
+        return None, None
+    key = ("_dev", id(mesh.ctx), ncomp)
+    cache = bc.__dict__.setdefault("_dev", {})
+    if key not in cache:
+        if bc.bndi.shape != (nb,) or bc.bnde.size != nb * ncomp:
+            raise FvgError(1, "boundary closure: bc_i per boundary face, bc_e per boundary face and component")
+        cache[key] = (DistributedVector.from_data(mesh.ctx, bc.bndi),
+                      DistributedVector.from_data(mesh.ctx, bc.bnde.ravel()))
+    return cache[key]
+
+
+def compute_gradients(gradients, values, boundary_condition, mesh, ncomp=1):
+    """compute_gradients(&mut gradients, &values, bc, &mesh) (tools/gradients.rs:41-79) on
+    the device: values is n_cells x ncomp, gradients n_cells x ncomp x dim."""
+    bi, be = _closure_vecs(mesh, boundary_condition, ncomp)
+    check(lib().fvg_compute_gradients(mesh.ctx.h, mesh.h, int(ncomp), values.h, bi.h if bi else None,
+                                      be.h if be else None, gradients.h))
+
+
+def compute_limiters(limiters_out, values, gradients, limiter_scheme, fgradinterp_scheme, fngrad_scheme,
+                     boundary_condition, mesh, ncomp=1):
+    """compute_limiters(&mut limiters, &values, &gradients, limiter, fgradinterp, fngrad,
+    bc, &mesh) (tools/limiters.rs:47-126) on the device.  The gradient interpolation is
+    Linear or Upwind(&flux); the face-normal gradient Orthogonal or Corrected."""
+    ctx = mesh.ctx
+    keep = []
+
+    def dev(a):
+        if a is None or isinstance(a, DistributedVector):
+            return a
+        v = DistributedVector.from_data(ctx, np.ascontiguousarray(a, dtype=np.float64).ravel())
+        keep.append(v)
+        return v
+
+    spec = LimiterSpec()
+    spec.limiter, spec.k = limiter_scheme.kind, limiter_scheme.k
+    if isinstance(fgradinterp_scheme, Upwind):
+        spec.interp, spec.flux = INTERP_UPWIND, dev(fgradinterp_scheme.flux).h
+    elif isinstance(fgradinterp_scheme, Linear):
+        spec.interp = INTERP_LINEAR
+    else:
+        raise FvgError(9, "gradient interpolation schemes: Linear, Upwind")
+    if isinstance(fngrad_scheme, Corrected):
+        spec.fngrad, spec.reduction, spec.fn_grads = FNGRAD_CORRECTED, fngrad_scheme.reduction, \
+            dev(fngrad_scheme.gradients).h
+    elif isinstance(fngrad_scheme, Orthogonal):
+        spec.fngrad = FNGRAD_ORTHOGONAL
+    else:
+        raise FvgError(9, "face-normal-gradient schemes: Orthogonal, Corrected")
+    bi, be = _closure_vecs(mesh, boundary_condition, ncomp)
+    check(lib().fvg_compute_limiters(ctx.h, mesh.h, int(ncomp), values.h, gradients.h, C.byref(spec),
+                                     bi.h if bi else None, be.h if be else None, limiters_out.h))
+    if keep:
+        ctx.sync()
 
 
 # ---------------------------------------------------------------------------------

@@ -167,6 +167,15 @@ enum {
                                          arg0 = cell field previous, scalar = dt, arg1/arg2 = cell
                                          fields density/density_previous or -1 */
     FVG_OP_DIVERGENCE = 6,            /* divergence(&flux): terms/divergence.rs:36-45.  arg0 = face field */
+    FVG_OP_LAPLACIAN_CORRECTED = 7,   /* laplacian(Corrected(&grad, reduction), k_f): terms/laplacian.rs:35-45,
+                                         schemes/facengrad/corrected.rs:34-67.  arg0 = face field k,
+                                         arg1 = cell field holding the gradient (n_cells x dim, row-major),
+                                         scalar = corrector_reduction */
+    FVG_OP_CONVECTION_LINEAR = 8,     /* convection(Linear, &flux): schemes/faceinterp/linear.rs:22-44.
+                                         arg0 = face field flux */
+    FVG_OP_CONVECTION_LIMITED_LINEAR = 9, /* convection(LimitedLinear(&w, &lim), &flux):
+                                         schemes/faceinterp/limitedlinear.rs:24-72.  arg0 = face field
+                                         flux, arg1 = face field w (upwind side), arg2 = face limiters */
     FVG_OP_ADD = 16,                  /* a + b (operations.rs:86-120) */
     FVG_OP_SUB = 17,                  /* a - b (operations.rs:123-157) */
     FVG_OP_NEG = 18                   /* -a    (operations.rs:190-199) */
@@ -191,6 +200,46 @@ typedef struct fvg_term_op {
 int fvg_assemble(fvg_ctx* ctx, const fvg_mesh* mesh, const fvg_term_op* program, int32_t n_ops,
                  const fvg_vec* const* face_fields, int32_t n_face_fields, const fvg_vec* const* cell_fields,
                  int32_t n_cell_fields, const fvg_vec* bc_i, const fvg_vec* bc_e, fvg_csr* lhs, fvg_vec* rhs);
+
+/* ---- fvm tools (src/fvm/tools) ---------------------------------------------------
+ * Fields with ncomp components (1: Field<f64>; 2 or 3: Field<Vector<ncomp>>) are
+ * n_cells x ncomp row-major; their gradients (Vector<dim> or Matrix<ncomp, dim>) are
+ * n_cells x ncomp x dim.  The boundary closure is bc_i (one per boundary face) and
+ * bc_e (one per boundary face and component), in boundary-face order; both may be
+ * NULL on a mesh without boundary faces.  Results are bit-identical to the reference's
+ * sequential loops. */
+
+/* compute_gradients (tools/gradients.rs:41-79): least-squares cell gradients from the
+ * coefficients Mesh::compute made (compute.rs:154-200). */
+int fvg_compute_gradients(fvg_ctx* ctx, const fvg_mesh* mesh, int32_t ncomp, const fvg_vec* values,
+                          const fvg_vec* bc_i, const fvg_vec* bc_e, fvg_vec* gradients);
+
+enum {   /* schemes/limiters/mod.rs */
+    FVG_LIMITER_LIMITED_LINEAR = 0,   /* LimitedLinear(k) */
+    FVG_LIMITER_MINMOD = 1,
+    FVG_LIMITER_VANLEER = 2,
+    FVG_LIMITER_VANALBADA = 3,
+    FVG_LIMITER_SWEBY = 4,            /* Sweby(k) */
+    FVG_LIMITER_SUPERBEE = 5
+};
+enum { FVG_INTERP_LINEAR = 0, FVG_INTERP_UPWIND = 1 };          /* gradient interpolation scheme */
+enum { FVG_FNGRAD_ORTHOGONAL = 0, FVG_FNGRAD_CORRECTED = 1 };  /* face-normal gradient scheme */
+
+typedef struct fvg_limiter_spec {
+    int32_t limiter;          /* FVG_LIMITER_* */
+    int32_t interp;           /* FVG_INTERP_*: how the cell gradients are taken to the face */
+    int32_t fngrad;           /* FVG_FNGRAD_* */
+    int32_t reserved;
+    double k;                 /* the limiter's parameter (LimitedLinear, Sweby) */
+    double reduction;         /* Corrected: corrector_reduction */
+    const fvg_vec* flux;      /* Upwind: face flux deciding the upwind side */
+    const fvg_vec* fn_grads;  /* Corrected: the gradient field it corrects with */
+} fvg_limiter_spec;
+
+/* compute_limiters (tools/limiters.rs:47-126): one limiter per face. */
+int fvg_compute_limiters(fvg_ctx* ctx, const fvg_mesh* mesh, int32_t ncomp, const fvg_vec* values,
+                         const fvg_vec* gradients, const fvg_limiter_spec* spec, const fvg_vec* bc_i,
+                         const fvg_vec* bc_e, fvg_vec* limiters);
 
 /* ---- preconditioners: Preconditioner<T> (src/linalg/preconditioners/mod.rs:13-17) ---- */
 typedef struct fvg_prec fvg_prec;
