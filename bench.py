@@ -48,8 +48,9 @@ def parse():
     ap.add_argument("--direct-launches", action="store_true",
                     help="launch the mixed-precision inner iterations kernel by kernel instead of replaying a "
                          "captured graph (same kernels and work; for kernel tracers)")
-    ap.add_argument("--legs", default="batched,serving,convdiff",
-                    help="extra legs beside the headline solve: batched, serving, convdiff (comma list, or none)")
+    ap.add_argument("--legs", default="batched,serving,convdiff,transient",
+                    help="extra legs beside the headline solve: batched, serving, convdiff, transient "
+                         "(comma list, or none)")
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-qps", type=float, default=80.0)
@@ -240,6 +241,104 @@ def leg_convdiff(args, fvg, workloads):
     out["mixed"]["speedup_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
     out["mixed"]["rel_error_vs_fp64_solution"] = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) /
                                                        np.linalg.norm(xs["fp64"]))
+    ctx.close()
+    return out
+
+
+def leg_transient(args, fvg, workloads):
+    """One time step of the crate's ex4 on the 2,097,152-prism mesh of the convection-
+    diffusion leg, everything on the device: compute_gradients, compute_limiters
+    (LimitedLinear(1), Upwind on the flux, Corrected), assembly of time(Euler) +
+    convection(LimitedLinear) - laplacian(Corrected), then ILU(0) + BiCGStab to 1e-8.
+    The oracle's gradients + limiters + assembly of the same step run beside it on one
+    host core."""
+    raw, fields_of = workloads.convdiff_prism()
+    ctx = new_context(fvg, args)
+    mesh = fvg.Mesh(ctx, raw)
+    geom = mesh.geometry()
+    pr = fields_of(geom)
+    flux, kf = pr["face_fields"]
+    bci, bce = pr["bc_i"], pr["bc_e"]
+    bc = fvg.FaceConstraints(bci, bce)
+    n, F, nb, D = mesh.n_cells, mesh.n_faces, mesh.n_boundary_faces, 3
+    u0 = np.exp(-8.0 * ((geom["cell_center"] - 0.5) ** 2).sum(1))
+    u = fvg.DistributedVector.from_data(ctx, u0)
+    g = fvg.DistributedVector(ctx, n * D)
+    lim = fvg.DistributedVector(ctx, F)
+    fl = fvg.DistributedVector.from_data(ctx, flux)
+    up, corr = fvg.Upwind(fl), fvg.Corrected(g, 1.0)
+    llim = fvg.limiters.LimitedLinear(1.0)
+    eq = (fvg.time(fvg.Euler(u, 1e-3)) + fvg.convection(fvg.LimitedLinear(fl, lim), fl)
+          - fvg.laplacian(fvg.Corrected(g, 1.0), kf))
+    ops, ff, cf = eq.compile()
+    asm = fvg.Assembler(mesh, ops, ff, cf, bci, bce)
+    lhs = fvg.DistributedMatrix.from_mesh(ctx, mesh)
+    rhs = fvg.DistributedVector.from_size(ctx, n)
+    x = fvg.DistributedVector.from_size(ctx, n)
+    fvg.compute_gradients(g, u, bc, mesh)
+    fvg.compute_limiters(lim, u, g, llim, up, corr, bc, mesh)
+    asm.run(lhs, rhs)
+    ilu = fvg.IncompleteLowerUpper.from_matrix(lhs, 0)
+    opts = fvg.LinearSolverOptions(1e-8, 0.0, 1000)
+
+    def timed(fn, reps=20):
+        fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.sync()
+        return 1e3 * (time.perf_counter() - t0) / reps
+
+    grad_ms = timed(lambda: fvg.compute_gradients(g, u, bc, mesh))
+    lim_ms = timed(lambda: fvg.compute_limiters(lim, u, g, llim, up, corr, bc, mesh))
+    asm_ms = timed(lambda: asm.run(lhs, rhs, sync=False))
+
+    def step():
+        fvg.compute_gradients(g, u, bc, mesh)
+        fvg.compute_limiters(lim, u, g, llim, up, corr, bc, mesh)
+        asm.run(lhs, rhs, sync=False)
+        ilu.refactor(lhs)
+        x.fill(0.0)
+        return fvg.bi_conjugate_gradient_stab(x, lhs, rhs, ilu, opts)
+
+    step()
+    ctx.sync()
+    steps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        info = step()
+    ctx.sync()
+    step_ms = 1e3 * (time.perf_counter() - t0) / steps
+    L = int(raw["cell_ptr"][-1])
+    # algorithmic bytes: every input read once, every output written once
+    grad_bytes = n * (4 + 8 * D + 8 + 8 * D) + L * (4 + 8 * D) + nb * 16
+    lim_bytes = F * (4 + 4 + 8 * D + 8 * D + 8 + 8) + n * (8 * D + 8 + 8 * D) + nb * (4 + 16)
+    out = {"workload": "ex4 step on convdiff_prism: 2,097,152 prisms, LimitedLinear(1) limiters (Upwind, "
+                       "Corrected), time(Euler, dt=1e-3) + convection(LimitedLinear) - laplacian(Corrected), "
+                       "ILU(0) + BiCGStab to 1e-8",
+           "n_cells": n, "n_faces": F, "step_ms": step_ms, "steps_per_s": 1e3 / step_ms,
+           "iterations": info.iterations,
+           "gradients": {"avg_ms": grad_ms, "algorithmic_bytes": grad_bytes,
+                         "GB/s": grad_bytes / (grad_ms * 1e-3) / 1e9,
+                         "frac": grad_bytes / (grad_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+           "limiters": {"avg_ms": lim_ms, "algorithmic_bytes": lim_bytes,
+                        "GB/s": lim_bytes / (lim_ms * 1e-3) / 1e9,
+                        "frac": lim_bytes / (lim_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+           "assembly_ms": asm_ms}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        om = O.OracleMesh(raw)
+        t0 = time.perf_counter()
+        go = om.gradients(u0, bci, bce)
+        lo = om.limiters(u0, go, "limited_linear", 1.0, "upwind", flux, go, 1.0, bci, bce)
+        prog = [(O.OP["TIME_EULER"], 0, -1, -1, 1e-3), (O.OP["CONVECTION_LIMITED_LINEAR"], 0, 0, 1, 0.0),
+                (O.OP["ADD"], 0, 0, 0, 0.0), (O.OP["LAPLACIAN_CORRECTED"], 2, 1, 0, 1.0), (O.OP["SUB"], 0, 0, 0, 0.0)]
+        om.assemble(prog, [flux, lo, kf], [u0, go.ravel()], bci, bce)
+        t_cpu = time.perf_counter() - t0
+        out["cpu_oracle_tools_ms"] = 1e3 * t_cpu
+        out["tools_speedup_vs_cpu"] = 1e3 * t_cpu / (grad_ms + lim_ms + asm_ms)
     ctx.close()
     return out
 
@@ -475,7 +574,7 @@ def main():
                           "total_ms": ks["sweep_ms"], "levels": [lo_lv, up_lv]},
         },
     }
-    for name, fn in (("batched", leg_batched), ("convdiff", leg_convdiff)):
+    for name, fn in (("batched", leg_batched), ("convdiff", leg_convdiff), ("transient", leg_transient)):
         if name in legs:
             print(f"bench: {name} leg", file=sys.stderr, flush=True)
             out[name] = fn(args, fvg, workloads)

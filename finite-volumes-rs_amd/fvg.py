@@ -500,7 +500,7 @@ class Euler:
     """time::Euler (src/fvm/schemes/time/euler.rs)."""
 
     def __init__(self, previous, dt, density=None, density_previous=None):
-        self.previous = np.ascontiguousarray(previous, dtype=np.float64)
+        self.previous = _field(previous)
         self.dt = float(dt)
         self.density = density
         self.density_previous = density_previous
