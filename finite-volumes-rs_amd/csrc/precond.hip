@@ -269,8 +269,8 @@ __device__ __forceinline__ T wait_value(const T* x, int j, int32_t* err) {
     return Bits<T>::value(v);
 }
 
-// POLL: 0 re-poll after a short sleep; 1 re-poll without sleeping; 2 keep two polls in
-// flight.  SCALE: the row result is multiplied by scale[i].
+// POLL: 0 re-poll at once; 1 re-poll after a short sleep; 2 keep two polls in flight.
+// SCALE: the row result is multiplied by scale[i].
 //
 // A row of at most kMaxRowDeps dependencies issues every load it will ever need before
 // its first wait: the column, the coefficient and the dependency word of each entry, and
@@ -353,9 +353,12 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
             } else {
                 for (;;) {
                     if (!pending(w) || give_up()) break;
-                    if (POLL == 0) __builtin_amdgcn_s_sleep(1);
+                    if (POLL == 1) __builtin_amdgcn_s_sleep(1);
+                    // re-poll only the words still pending: lanes whose dependencies have
+                    // landed issue no requests
 #pragma unroll
-                    for (int q = 0; q < kMaxRowDeps; ++q) w[q] = load_word(x, cj[q]);
+                    for (int q = 0; q < kMaxRowDeps; ++q)
+                        if (q < len && w[q] == Bits<T>::pending) w[q] = load_word(x, cj[q]);
                 }
             }
             // subtract in ascending column order, as the sequential sweep does
@@ -685,12 +688,13 @@ static int max_coresident_groups(fvg_ctx* c) {
     return m > 0 ? m : 1;
 }
 
-// waves of the persistent sweep: about two levels' worth of chunks in flight
+// waves of the persistent sweep: about two and a half levels' worth of chunks in flight
+// (fewer expose each chunk's start, more add polling traffic that slows every hop)
 static int64_t sweep_waves(const fvg_ctx* c, const Sweep& sw, int max_groups) {
     int64_t w = c->tune_sweep_waves;
     if (w <= 0) {
         const int64_t chunks = sw.npos / kChunk;
-        w = 2 * ((chunks + sw.nlevels - 1) / std::max(1, sw.nlevels));
+        w = 5 * ((chunks + sw.nlevels - 1) / std::max(1, sw.nlevels)) / 2;
     }
     w = std::max<int64_t>(w, kWavesPerGroup);
     w = (w + kWavesPerGroup - 1) / kWavesPerGroup * kWavesPerGroup;

@@ -64,8 +64,8 @@ int fvg_ctx_stream(const fvg_ctx* ctx, void** stream);
  * Values of 0 restore the built-in choice. */
 enum {
     FVG_TUNE_SWEEP_MODE = 1,    /* 0: one persistent launch per triangular sweep; 1: one launch per level */
-    FVG_TUNE_SWEEP_WAVES = 2,   /* waves of the persistent sweep (0: about two levels of chunks) */
-    FVG_TUNE_SWEEP_POLL = 3,    /* sweep wait loop: 0 short sleep between polls, 1 no sleep, 2 two polls in flight */
+    FVG_TUNE_SWEEP_WAVES = 2,   /* waves of the persistent sweep (0: about two and a half levels of chunks) */
+    FVG_TUNE_SWEEP_POLL = 3,    /* sweep wait loop: 0 re-poll at once, 1 short sleep between polls, 2 two polls in flight */
     /* FP32 inner iterations of FVG_PRECISION_MIXED (device-wide solves): the incomplete
      * factor is applied by Jacobi-iterated triangular solves (0, the default) or by the
      * exact level-scheduled sweeps (1); FVG_TUNE_INNER_SWEEPS Jacobi sweeps per factor
