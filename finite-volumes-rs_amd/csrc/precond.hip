@@ -309,8 +309,9 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
                 cj[q] = col[k];
                 a[q] = val[k];
             }
+            // only the row's own dependencies are polled (padding slots read as ready)
 #pragma unroll
-            for (int q = 0; q < kMaxRowDeps; ++q) w[q] = load_word(x, cj[q]);
+            for (int q = 0; q < kMaxRowDeps; ++q) w[q] = q < len ? load_word(x, cj[q]) : U(0);
             auto pending = [&](const U* v) {
                 bool pend = false;
 #pragma unroll
@@ -333,21 +334,32 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
                 if (pending(w)) {
                     U v[kMaxRowDeps];
 #pragma unroll
-                    for (int q = 0; q < kMaxRowDeps; ++q) w[q] = load_word(x, cj[q]);
+                    for (int q = 0; q < kMaxRowDeps; ++q) v[q] = w[q];
+#pragma unroll
+                    for (int q = 0; q < kMaxRowDeps; ++q)
+                        if (q < len && w[q] == Bits<T>::pending) w[q] = load_word(x, cj[q]);
                     __builtin_amdgcn_s_sleep(6);
 #pragma unroll
-                    for (int q = 0; q < kMaxRowDeps; ++q) v[q] = load_word(x, cj[q]);
+                    for (int q = 0; q < kMaxRowDeps; ++q)
+                        if (q < len && v[q] == Bits<T>::pending) v[q] = load_word(x, cj[q]);
                     for (;;) {
                         if (!pending(w) || give_up()) break;
+                        // merge what the younger poll already saw, then re-poll the rest
 #pragma unroll
-                        for (int q = 0; q < kMaxRowDeps; ++q) w[q] = load_word(x, cj[q]);
+                        for (int q = 0; q < kMaxRowDeps; ++q)
+                            if (w[q] == Bits<T>::pending) w[q] = v[q];
+                        if (!pending(w)) break;
+#pragma unroll
+                        for (int q = 0; q < kMaxRowDeps; ++q)
+                            if (q < len && w[q] == Bits<T>::pending) w[q] = load_word(x, cj[q]);
                         if (!pending(v)) {
 #pragma unroll
                             for (int q = 0; q < kMaxRowDeps; ++q) w[q] = v[q];
                             break;
                         }
 #pragma unroll
-                        for (int q = 0; q < kMaxRowDeps; ++q) v[q] = load_word(x, cj[q]);
+                        for (int q = 0; q < kMaxRowDeps; ++q)
+                            if (q < len && v[q] == Bits<T>::pending) v[q] = load_word(x, cj[q]);
                     }
                 }
             } else {
