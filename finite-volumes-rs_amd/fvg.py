@@ -128,6 +128,22 @@ def lib():
         "fvg_mesh_sizes": [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                            C.POINTER(C.c_int64), C.POINTER(C.c_int32)],
         "fvg_mesh_geometry": [P, P, P, P, P, P, P, P],
+        "fvg_topology_read_su2": [C.c_char_p, C.POINTER(P)],
+        "fvg_topology_read_native": [C.c_char_p, C.POINTER(P)],
+        "fvg_topology_create": [C.c_int, C.c_int64, P, C.c_int64, P, P, P, C.c_int64, P, P, C.POINTER(P)],
+        "fvg_topology_destroy": [P],
+        "fvg_topology_sizes": [P, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                               C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                               C.POINTER(C.c_int32)],
+        "fvg_topology_arrays": [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P),
+                                C.POINTER(P)],
+        "fvg_topology_patch_name": [P, C.c_int32, C.POINTER(C.c_char_p)],
+        "fvg_mesh_create_from_topology": [P, P, C.POINTER(P)],
+        "fvg_pvtu_writer_create": [P, C.POINTER(P)],
+        "fvg_pvtu_write": [P, C.c_char_p, C.c_int32, P, P, P],
+        "fvg_pvtu_write_async": [P, P, C.c_char_p, C.c_int32, P, P, P],
+        "fvg_pvtu_wait": [P],
+        "fvg_pvtu_writer_destroy": [P],
         "fvg_csr_from_mesh": [P, P, C.POINTER(P)],
         "fvg_csr_create": [P, C.c_int64, C.c_int64, C.c_int64, P, P, P, C.POINTER(P)],
         "fvg_csr_destroy": [P, P],
@@ -318,18 +334,24 @@ class DistributedVector:
 class Mesh:
     """Mesh<DIM> (src/core/mesh/mod.rs:200-234) built from raw topology and computed."""
 
-    def __init__(self, ctx, raw):
+    def __init__(self, ctx, raw, _topology=None):
         self.ctx = ctx
-        self.dim = int(raw["dim"])
-        nodes = np.ascontiguousarray(raw["nodes"], dtype=np.float64)
-        fptr = np.ascontiguousarray(raw["face_ptr"], dtype=np.int64)
-        fn = np.ascontiguousarray(raw["face_nodes"], dtype=np.int32)
-        fp = np.ascontiguousarray(raw["face_patch"], dtype=np.int32)
-        cptr = np.ascontiguousarray(raw["cell_ptr"], dtype=np.int64)
-        cf = np.ascontiguousarray(raw["cell_faces"], dtype=np.int32)
         h = P()
-        check(lib().fvg_mesh_create(ctx.h, self.dim, nodes.shape[0], _ptr(nodes), fptr.shape[0] - 1, _ptr(fptr),
-                                    _ptr(fn), _ptr(fp), cptr.shape[0] - 1, _ptr(cptr), _ptr(cf), C.byref(h)))
+        if _topology is not None:
+            self.dim = _topology.dim
+            check(lib().fvg_mesh_create_from_topology(ctx.h, _topology.h, C.byref(h)))
+            fp = _topology.raw()["face_patch"]
+        else:
+            self.dim = int(raw["dim"])
+            nodes = np.ascontiguousarray(raw["nodes"], dtype=np.float64)
+            fptr = np.ascontiguousarray(raw["face_ptr"], dtype=np.int64)
+            fn = np.ascontiguousarray(raw["face_nodes"], dtype=np.int32)
+            fp = np.ascontiguousarray(raw["face_patch"], dtype=np.int32)
+            cptr = np.ascontiguousarray(raw["cell_ptr"], dtype=np.int64)
+            cf = np.ascontiguousarray(raw["cell_faces"], dtype=np.int32)
+            check(lib().fvg_mesh_create(ctx.h, self.dim, nodes.shape[0], _ptr(nodes), fptr.shape[0] - 1,
+                                        _ptr(fptr), _ptr(fn), _ptr(fp), cptr.shape[0] - 1, _ptr(cptr), _ptr(cf),
+                                        C.byref(h)))
         self.h = h
         nc, nf, nb, nnz, ncol = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
         check(lib().fvg_mesh_sizes(h, C.byref(nc), C.byref(nf), C.byref(nb), C.byref(nnz), C.byref(ncol)))
@@ -341,6 +363,10 @@ class Mesh:
     @classmethod
     def from_arrays(cls, ctx, raw):
         return cls(ctx, raw)
+
+    @classmethod
+    def from_topology(cls, ctx, topology):
+        return cls(ctx, None, _topology=topology)
 
     def geometry(self):
         """face_area, face_center, face_normal, owner, neighbor, cell_volume, cell_center."""
@@ -369,6 +395,140 @@ class Mesh:
 # ---------------------------------------------------------------------------------
 # matrix
 # ---------------------------------------------------------------------------------
+
+# ---------------------------------------------------------------------------------
+# mesh files and results (src/core/mesh/io.rs, src/post/write_pvtu.rs)
+# ---------------------------------------------------------------------------------
+
+class Topology:
+    """A host mesh topology: what Mesh::add_node / add_face / add_cell receive."""
+
+    def __init__(self, handle):
+        self.h = handle
+        d, nn, nf, nfn, nc, ncf, npat = (C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64(),
+                                         C.c_int64(), C.c_int32())
+        check(lib().fvg_topology_sizes(handle, C.byref(d), C.byref(nn), C.byref(nf), C.byref(nfn), C.byref(nc),
+                                       C.byref(ncf), C.byref(npat)))
+        self.dim, self.n_nodes, self.n_faces, self.n_cells, self.n_patches = d.value, nn.value, nf.value, \
+            nc.value, npat.value
+        self._sizes = (nfn.value, ncf.value)
+
+    @classmethod
+    def read_su2(cls, path):
+        """Mesh::read_su2 (io.rs:489-533)."""
+        h = P()
+        check(lib().fvg_topology_read_su2(os.fsencode(path), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def read(cls, path):
+        """Mesh::read (io.rs:68-211): the crate's own mesh format."""
+        h = P()
+        check(lib().fvg_topology_read_native(os.fsencode(path), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_raw(cls, raw):
+        nodes = np.ascontiguousarray(raw["nodes"], dtype=np.float64)
+        fptr = np.ascontiguousarray(raw["face_ptr"], dtype=np.int64)
+        fn = np.ascontiguousarray(raw["face_nodes"], dtype=np.int32)
+        fp = np.ascontiguousarray(raw["face_patch"], dtype=np.int32)
+        cptr = np.ascontiguousarray(raw["cell_ptr"], dtype=np.int64)
+        cf = np.ascontiguousarray(raw["cell_faces"], dtype=np.int32)
+        h = P()
+        check(lib().fvg_topology_create(int(raw["dim"]), nodes.shape[0], _ptr(nodes), fptr.shape[0] - 1, _ptr(fptr),
+                                        _ptr(fn), _ptr(fp), cptr.shape[0] - 1, _ptr(cptr), _ptr(cf), C.byref(h)))
+        return cls(h)
+
+    def raw(self):
+        """Copies of the arrays, in the raw-mesh layout fvg.Mesh takes."""
+        ptrs = [P() for _ in range(6)]
+        check(lib().fvg_topology_arrays(self.h, *[C.byref(q) for q in ptrs]))
+        nfn, ncf = self._sizes
+
+        def arr(p, ctype, n, dtype):
+            if n == 0:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(C.cast(p, C.POINTER(ctype)), shape=(n,)).astype(dtype, copy=True)
+
+        return dict(dim=self.dim,
+                    nodes=arr(ptrs[0], C.c_double, self.n_nodes * self.dim, np.float64).reshape(-1, self.dim),
+                    face_ptr=arr(ptrs[1], C.c_int64, self.n_faces + 1, np.int64),
+                    face_nodes=arr(ptrs[2], C.c_int32, nfn, np.int32),
+                    face_patch=arr(ptrs[3], C.c_int32, self.n_faces, np.int32),
+                    cell_ptr=arr(ptrs[4], C.c_int64, self.n_cells + 1, np.int64),
+                    cell_faces=arr(ptrs[5], C.c_int32, ncf, np.int32))
+
+    def patch_name(self, patch):
+        v = C.c_char_p()
+        check(lib().fvg_topology_patch_name(self.h, int(patch), C.byref(v)))
+        return v.value.decode()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fvg_topology_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class PvtuWriter:
+    """PvtuWriter::new(&mesh).with(name, &field)...write(path) (src/post/write_pvtu.rs:30-58).
+    Fields are host arrays (n_cells,) or (n_cells, ncomp) for write(), device vectors
+    (with their component count) for write_async()."""
+
+    def __init__(self, topology):
+        h = P()
+        check(lib().fvg_pvtu_writer_create(topology.h, C.byref(h)))
+        self.h = h
+        self.n_cells = topology.n_cells
+        self.fields = []
+
+    def with_(self, name, values):
+        v = values if isinstance(values, DistributedVector) else np.ascontiguousarray(values, dtype=np.float64)
+        nc = 1 if isinstance(v, DistributedVector) and v.n == self.n_cells else \
+            (v.n // self.n_cells if isinstance(v, DistributedVector) else (1 if v.ndim == 1 else v.shape[1]))
+        self.fields.append((name, nc, v))
+        return self
+
+    def _args(self):
+        names = (C.c_char_p * max(1, len(self.fields)))(*[n.encode() for n, _, _ in self.fields])
+        nc = np.array([c for _, c, _ in self.fields] or [1], np.int32)
+        return names, nc
+
+    def write(self, path):
+        names, nc = self._args()
+        vals = [np.ascontiguousarray(v.numpy() if isinstance(v, DistributedVector) else v, dtype=np.float64).ravel()
+                for _, _, v in self.fields]
+        vp = (P * max(1, len(vals)))(*[a.ctypes.data for a in vals])
+        check(lib().fvg_pvtu_write(self.h, os.fsencode(path), len(self.fields), C.cast(names, P), _ptr(nc),
+                                   C.cast(vp, P)))
+
+    def write_async(self, ctx, path):
+        """Copies the device fields out on ctx's stream and writes the files from a host
+        thread; returns at once (wait() joins it)."""
+        names, nc = self._args()
+        for _, _, v in self.fields:
+            if not isinstance(v, DistributedVector):
+                raise FvgError(1, "write_async takes device vectors")
+        self._keep = (names, nc)
+        vp = (P * max(1, len(self.fields)))(*[v.h for _, _, v in self.fields])
+        self._keep_vp = vp
+        check(lib().fvg_pvtu_write_async(self.h, ctx.h, os.fsencode(path), len(self.fields), C.cast(names, P),
+                                         _ptr(nc), C.cast(vp, P)))
+
+    def wait(self):
+        check(lib().fvg_pvtu_wait(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fvg_pvtu_writer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
 
 class DistributedMatrix:
     """DistributedMatrix<f64> in CSR (src/linalg/dmatrix.rs:14-301), device resident."""

@@ -131,6 +131,56 @@ int fvg_mesh_sizes(const fvg_mesh* mesh, int64_t* n_cells, int64_t* n_faces, int
 int fvg_mesh_geometry(const fvg_mesh* mesh, double* face_area, double* face_center, double* face_normal,
                       int32_t* owner, int32_t* neighbor, double* cell_volume, double* cell_center);
 
+/* ---- mesh files (src/core/mesh/io.rs) and results (src/post/write_pvtu.rs) ----------
+ * A topology is what Mesh::add_node / add_face / add_cell receive: coordinates, face
+ * node lists with their boundary patch (-1 interior), cell face lists.  It lives on the
+ * host; fvg_mesh_create_from_topology hands it to the device build. */
+typedef struct fvg_topology fvg_topology;
+
+/* Mesh::read_su2 (io.rs:489-533): NDIME, NPOIN, NELEM (VTK kinds 3, 5, 9, 10, 12),
+ * markers numbered 0.. in file order; faces numbered as compute_vtk_faces does
+ * (io.rs:364-486): interior faces first, then boundary faces by marker, each in order of
+ * first appearance.  Tokens may be separated by any whitespace. */
+int fvg_topology_read_su2(const char* path, fvg_topology** out);
+/* Mesh::read (io.rs:68-211): the crate's own format written by Mesh::write
+ * (io.rs:10-64).  A partition of a distributed mesh (remote-owned faces or cells) is
+ * refused with FVG_E_UNSUPPORTED. */
+int fvg_topology_read_native(const char* path, fvg_topology** out);
+/* A topology from arrays (copied), laid out as for fvg_mesh_create. */
+int fvg_topology_create(int dim, int64_t n_nodes, const double* nodes, int64_t n_faces, const int64_t* face_ptr,
+                        const int32_t* face_nodes, const int32_t* face_patch, int64_t n_cells,
+                        const int64_t* cell_ptr, const int32_t* cell_faces, fvg_topology** out);
+int fvg_topology_destroy(fvg_topology* topology);
+int fvg_topology_sizes(const fvg_topology* topology, int32_t* dim, int64_t* n_nodes, int64_t* n_faces,
+                       int64_t* n_face_nodes, int64_t* n_cells, int64_t* n_cell_faces, int32_t* n_patches);
+/* Pointers into the topology's own arrays (valid until it is destroyed); any may be NULL. */
+int fvg_topology_arrays(const fvg_topology* topology, const double** nodes, const int64_t** face_ptr,
+                        const int32_t** face_nodes, const int32_t** face_patch, const int64_t** cell_ptr,
+                        const int32_t** cell_faces);
+/* The marker / patch name of a patch id ("" when the file names none). */
+int fvg_topology_patch_name(const fvg_topology* topology, int32_t patch, const char** name);
+int fvg_mesh_create_from_topology(fvg_ctx* ctx, const fvg_topology* topology, fvg_mesh** out);
+
+/* PvtuWriter (write_pvtu.rs:30-466) for one rank: <prefix>.pvtu and <prefix>_0.vtu, the
+ * prefix being the path without its extension (file_without_extension, :416-424);
+ * polygons in 2D, polyhedra in 3D; cell fields of 1 to 9 components (n_cells x ncomp,
+ * row-major); numbers printed as Rust prints f64.  The mesh part is built once per
+ * writer.  The polyhedra connectivity lists a cell's nodes in order of first
+ * appearance, where the reference's HashSet gives an arbitrary order. */
+typedef struct fvg_pvtu_writer fvg_pvtu_writer;
+int fvg_pvtu_writer_create(const fvg_topology* topology, fvg_pvtu_writer** out);
+int fvg_pvtu_write(fvg_pvtu_writer* writer, const char* path, int32_t n_fields, const char* const* names,
+                   const int32_t* ncomps, const double* const* values);
+/* The same from device vectors, off the critical path (no counterpart in the reference):
+ * the fields are copied to pinned host memory on the context stream and the files are
+ * written by a host thread once the copies land; the call returns at once.  One write is
+ * in flight per writer: the next call, fvg_pvtu_wait or destroy waits for it. */
+int fvg_pvtu_write_async(fvg_pvtu_writer* writer, fvg_ctx* ctx, const char* path, int32_t n_fields,
+                         const char* const* names, const int32_t* ncomps, const fvg_vec* const* values);
+/* Waits for the writer's asynchronous write and returns its status. */
+int fvg_pvtu_wait(fvg_pvtu_writer* writer);
+int fvg_pvtu_writer_destroy(fvg_pvtu_writer* writer);
+
 /* ---- sparse matrix: DistributedMatrix<f64> in CSR (src/linalg/dmatrix.rs:14-301) ---- */
 typedef struct fvg_csr fvg_csr;
 

@@ -8,10 +8,13 @@ Element face lists follow the VTK conventions the crate uses (src/core/mesh/vtk.
 import numpy as np
 
 _FACES = {
+    3: ((0,), (1,)),                                   # line
     5: ((0, 1), (1, 2), (2, 0)),                       # triangle
     9: ((0, 1), (1, 2), (2, 3), (3, 0)),               # quad
     10: ((0, 2, 1), (0, 1, 3), (1, 2, 3), (0, 3, 2)),  # tetrahedron
+    12: ((0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)),   # hexahedron
 }
+_NODES = {3: 2, 5: 3, 9: 4, 10: 4, 12: 8}
 
 
 def read(path):
@@ -27,7 +30,7 @@ def read(path):
             ne = int(l.split("=")[1])
             for k in range(ne):
                 v = [int(t) for t in lines[i + 1 + k].split()]
-                elems.append((v[0], v[1:1 + {5: 3, 9: 4, 10: 4}[v[0]]]))
+                elems.append((v[0], v[1:1 + _NODES[v[0]]]))
             i += ne
         elif l.startswith("NPOIN="):
             npt = int(l.split("=")[1].split()[0])
