@@ -246,7 +246,7 @@ int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
     if (!c || value < 0) return fail(FVG_E_INVALID, "bad tuning argument");
     switch (knob) {
     case FVG_TUNE_SWEEP_MODE:
-        if (value > 1) return fail(FVG_E_INVALID, "sweep mode must be 0 or 1");
+        if (value > 2) return fail(FVG_E_INVALID, "sweep mode must be 0, 1 or 2");
         c->tune_sweep_mode = value;
         return FVG_OK;
     case FVG_TUNE_SWEEP_WAVES:
@@ -268,6 +268,10 @@ int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
         if (value > 1) return fail(FVG_E_INVALID, "launch mode must be 0 or 1");
         c->tune_graphs = value;
         return FVG_OK;
+    case FVG_TUNE_SWEEP_BLOCKS:
+        if (value > 4096) return fail(FVG_E_INVALID, "at most 4096 sweep blocks");
+        c->tune_sweep_blocks = value;
+        return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }
@@ -281,6 +285,7 @@ int fvg_ctx_get_tuning(const fvg_ctx* c, int32_t knob, int64_t* value) {
     case FVG_TUNE_INNER_EXACT: *value = c->tune_inner_exact; return FVG_OK;
     case FVG_TUNE_INNER_SWEEPS: *value = c->tune_inner_sweeps; return FVG_OK;
     case FVG_TUNE_GRAPHS: *value = c->tune_graphs; return FVG_OK;
+    case FVG_TUNE_SWEEP_BLOCKS: *value = c->tune_sweep_blocks; return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }

@@ -64,6 +64,7 @@ struct fvg_ctx {
     int64_t tune_inner_exact = 0;      // FVG_TUNE_INNER_EXACT
     int64_t tune_inner_sweeps = 0;     // FVG_TUNE_INNER_SWEEPS
     int64_t tune_graphs = 0;           // FVG_TUNE_GRAPHS
+    int64_t tune_sweep_blocks = 0;     // FVG_TUNE_SWEEP_BLOCKS
     int num_cus = 0;
 };
 

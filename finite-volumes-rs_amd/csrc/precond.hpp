@@ -23,7 +23,32 @@ struct Sweep {
     int32_t* src = nullptr;             // slot of each entry in the factor value array (nnz)
     double* val = nullptr;              // entry values gathered from the factor
     float* val32 = nullptr;             // FP32 copy for the mixed-precision inner solver
+    // Blocked layout (k_sweep_blocked; nblk == 0 when not built).  Rows are grouped into
+    // blocks of whole gap-1 components (rows joined by dependencies one level apart), so
+    // every dependency between blocks spans at least two levels.  Positions run block by
+    // block, level by level; a block's rows of one level are consecutive.
+    int32_t nblk = 0;
+    int32_t bW = 0;                     // most rows of one block in one level
+    int32_t bX = 0;                     // most values one block fetches from outside for one level
+    int32_t bK = 0;                     // most dependencies of one row
+    int32_t bF = 0;                     // most fetched rows of one block (all levels)
+    int32_t bOK = -1;                   // blocked launch fits co-resident: -1 unknown, 0 no, 1 yes
+    int32_t* brange = nullptr;          // per block: first level, one past the last (2 x nblk)
+    int32_t* blvl = nullptr;            // nblk x (nlevels + 1): first position of each level
+    int32_t* bxptr = nullptr;           // nblk x (nlevels + 1): offsets into bxidx
+    int32_t* bxidx = nullptr;           // rows fetched from outside the block's value ring
+    int64_t nbx = 0;                    // length of bxidx
+    int32_t* brow = nullptr;            // row of each position (n)
+    int32_t* bcnt = nullptr;            // dependencies of each position (n)
+    int32_t* bdep = nullptr;            // bK x n: LDS slot of each dependency, ascending columns
+    int32_t* bsrc = nullptr;            // bK x n: factor slot of each dependency (-1: none)
+    double* bval = nullptr;             // bK x n: its coefficient
+    double* spos = nullptr;             // scale (inverse diagonal) in position order, per refactor
+    double* bpos = nullptr;             // right-hand side in position order, per call
+    int32_t* bfill = nullptr;           // rows some block fetches from outside, ascending (nfill)
+    int64_t nfill = 0;                  // only they need the pending fill before a call
 };
+constexpr int kBlockRing = 3;           // levels of values a block keeps in LDS
 
 // One strict triangular factor as a CSR in natural row order with FP32 values, for the
 // mixed-precision inner solver's Jacobi-iterated apply.

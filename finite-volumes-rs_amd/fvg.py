@@ -34,7 +34,8 @@ INTERP_LINEAR, INTERP_UPWIND = 0, 1
 FNGRAD_ORTHOGONAL, FNGRAD_CORRECTED = 0, 1
 OP_ADD, OP_SUB, OP_NEG = 16, 17, 18
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-TUNE_SWEEP_MODE, TUNE_SWEEP_WAVES, TUNE_SWEEP_POLL, TUNE_INNER_EXACT, TUNE_INNER_SWEEPS, TUNE_GRAPHS = 1, 2, 3, 4, 5, 6
+(TUNE_SWEEP_MODE, TUNE_SWEEP_WAVES, TUNE_SWEEP_POLL, TUNE_INNER_EXACT, TUNE_INNER_SWEEPS, TUNE_GRAPHS,
+ TUNE_SWEEP_BLOCKS) = 1, 2, 3, 4, 5, 6, 7
 
 
 class FvgError(RuntimeError):
@@ -144,6 +145,7 @@ def lib():
         "fvg_pvtu_write_async": [P, P, C.c_char_p, C.c_int32, P, P, P],
         "fvg_pvtu_wait": [P],
         "fvg_pvtu_writer_destroy": [P],
+        "fvg_prec_sweep_layout": [P, P, C.c_int32],
         "fvg_csr_from_mesh": [P, P, C.POINTER(P)],
         "fvg_csr_create": [P, C.c_int64, C.c_int64, C.c_int64, P, P, P, C.POINTER(P)],
         "fvg_csr_destroy": [P, P],
@@ -929,6 +931,16 @@ class Preconditioner:
 
     def precondition(self, solution, rhs):
         check(lib().fvg_precondition(self.ctx.h, self.h, solution.h, rhs.h))
+
+    def sweep_layout(self):
+        """Shape of the blocked sweep layouts: lower_blocks, upper_blocks (0: not built),
+        levels, lower_rows / lower_fetch (most rows of a block in one level, most values it
+        fetches from outside for one level), lower_fetch_total, upper_rows, upper_fetch."""
+        info = np.zeros(8, dtype=np.int32)
+        check(lib().fvg_prec_sweep_layout(self.h, info.ctypes.data, 8))
+        keys = ("lower_blocks", "upper_blocks", "levels", "lower_rows", "lower_fetch", "lower_fetch_total",
+                "upper_rows", "upper_fetch")
+        return dict(zip(keys, (int(v) for v in info)))
 
     def refactor_async(self, matrix):
         """Enqueue the refactorisation; status() reports pivot / symmetry failures later."""

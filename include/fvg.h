@@ -63,7 +63,8 @@ int fvg_ctx_stream(const fvg_ctx* ctx, void** stream);
 /* Launch-shape knobs (no counterpart in the reference; used by the autotuner).
  * Values of 0 restore the built-in choice. */
 enum {
-    FVG_TUNE_SWEEP_MODE = 1,    /* 0: one persistent launch per triangular sweep; 1: one launch per level */
+    FVG_TUNE_SWEEP_MODE = 1,    /* 0: the blocked sweep where its layout was built, else the persistent
+                                   one; 1: one launch per level; 2: the persistent sweep */
     FVG_TUNE_SWEEP_WAVES = 2,   /* waves of the persistent sweep (0: about two and a half levels of chunks) */
     FVG_TUNE_SWEEP_POLL = 3,    /* sweep wait loop: 0 re-poll at once, 1 short sleep between polls, 2 two polls in flight */
     /* FP32 inner iterations of FVG_PRECISION_MIXED (device-wide solves): the incomplete
@@ -75,7 +76,10 @@ enum {
     /* Launch mode of those FP32 inner iterations: 0 (default) replays them from a captured
      * hipGraph, 1 launches every kernel directly (the same kernels and work; for kernel
      * tracers that cannot follow graph launches). */
-    FVG_TUNE_GRAPHS = 6
+    FVG_TUNE_GRAPHS = 6,
+    /* About how many blocks the blocked sweep layout of a preconditioner created
+     * afterwards is cut into (0: one per compute unit; at most 4096). */
+    FVG_TUNE_SWEEP_BLOCKS = 7
 };
 int fvg_ctx_set_tuning(fvg_ctx* ctx, int32_t knob, int64_t value);
 int fvg_ctx_get_tuning(const fvg_ctx* ctx, int32_t knob, int64_t* value);
@@ -305,6 +309,13 @@ int fvg_ic_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out)
 /* IncompleteLowerUpper::from_matrix(&A, level) (incomplete_lower_upper.rs:266-282),
  * with its drop rule |v| <= 1e-6 |u_ii| (:255-258).  level must be 0. */
 int fvg_ilu_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out);
+/* Diagnostics (no reference counterpart): the shape of the blocked sweep layouts, the
+ * first `count` (at most 8) of: blocks of the lower and of the upper factor (0: not built,
+ * the persistent sweep serves), dependency levels of the lower sweep, the lower layout's
+ * most rows of one block in one level and most values one block fetches from outside for
+ * one level, its total number of such fetches per sweep, then the upper layout's most rows
+ * and most fetches. */
+int fvg_prec_sweep_layout(const fvg_prec* prec, int32_t* info, int32_t count);
 /* Re-run the numeric factorisation for new values on the same pattern. */
 int fvg_prec_refactor(fvg_ctx* ctx, fvg_prec* p, const fvg_csr* a);
 /* Enqueues the refactorisation without waiting (no symmetry / pivot checks yet):

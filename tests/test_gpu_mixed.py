@@ -82,10 +82,12 @@ def test_mixed_graph_replay_and_direct_launches_agree(ctx, solver, prec, kind):
         assert np.array_equal(xg, out[0][0]) and it == out[0][1] and outer == out[0][2]
 
 
-@pytest.mark.parametrize("mode,waves,poll", [(1, 0, 0), (0, 4, 0), (0, 64, 0), (0, 0, 0), (0, 0, 1), (0, 0, 2)])
+@pytest.mark.parametrize("mode,waves,poll", [(1, 0, 0), (2, 4, 0), (2, 64, 0), (2, 0, 0), (2, 0, 1), (2, 0, 2),
+                                             (0, 0, 0)])
 def test_sweep_launch_shapes_give_identical_bits(ctx, mode, waves, poll):
-    """One launch per level, and the persistent sweep at several wave counts, all give the
-    oracle's bits (the schedule never changes the arithmetic)."""
+    """One launch per level, the persistent sweep at several wave counts and poll modes,
+    and the blocked sweep all give the oracle's bits (the schedule never changes the
+    arithmetic)."""
     rp, ci, va, b = _oracle_system("tet10", "sym")
     n = rp.shape[0] - 1
     A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
@@ -95,6 +97,8 @@ def test_sweep_launch_shapes_give_identical_bits(ctx, mode, waves, poll):
         ctx.set_tuning(fvg.TUNE_SWEEP_MODE, mode)
         ctx.set_tuning(fvg.TUNE_SWEEP_WAVES, waves)
         ctx.set_tuning(fvg.TUNE_SWEEP_POLL, poll)
+        if mode == 0:
+            assert ic.sweep_layout()["lower_blocks"] > 0
         z = fvg.DistributedVector(ctx, n)
         ic.precondition(z, fvg.DistributedVector.from_data(ctx, np.random.default_rng(1).normal(size=n)))
         assert np.array_equal(z.numpy(), ref)
@@ -102,3 +106,31 @@ def test_sweep_launch_shapes_give_identical_bits(ctx, mode, waves, poll):
         ctx.set_tuning(fvg.TUNE_SWEEP_MODE, 0)
         ctx.set_tuning(fvg.TUNE_SWEEP_WAVES, 0)
         ctx.set_tuning(fvg.TUNE_SWEEP_POLL, 0)
+
+
+@pytest.mark.parametrize("name,kind,prec,blocked", [("tet10", "sym", "ic", True), ("tet10", "nonsym", "ilu", True),
+                                                    ("tri64", "sym", "ic", False)])
+def test_blocked_sweep_is_built_and_exact_after_refactor(ctx, name, kind, prec, blocked):
+    """The default sweep mode runs the blocked sweep on both factors of the tet mesh; on
+    the triangle mesh every dependency is one level deep, the whole mesh is one
+    component and the persistent sweep serves.  Either way the results are the oracle's
+    bits, also after a refactorisation with new values (the blocked layout's
+    coefficients are gathered again by the factor kernel)."""
+    rp, ci, va, b = _oracle_system(name, kind)
+    n = rp.shape[0] - 1
+    A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+    P = (fvg.IncompleteCholesky if prec == "ic" else fvg.IncompleteLowerUpper).from_matrix(A, 0)
+    lay = P.sweep_layout()
+    assert (lay["lower_blocks"] > 0 and lay["upper_blocks"] > 0) == blocked
+    assert blocked or (lay["lower_blocks"] == 0 and lay["upper_blocks"] == 0)
+    r = np.random.default_rng(4).normal(size=n)
+    z = fvg.DistributedVector(ctx, n)
+    P.precondition(z, fvg.DistributedVector.from_data(ctx, r))
+    assert np.array_equal(z.numpy(), O.OraclePrec(prec, rp, ci, va, 0).apply(r))
+    # new values on the same pattern: a diagonal shift and a rescaled off-diagonal part
+    diag = np.repeat(np.arange(n), np.diff(rp)) == ci
+    va2 = np.where(diag, va * 1.25 + 0.5, va * 0.75)
+    A2 = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va2)
+    P.refactor(A2)
+    P.precondition(z, fvg.DistributedVector.from_data(ctx, r))
+    assert np.array_equal(z.numpy(), O.OraclePrec(prec, rp, ci, va2, 0).apply(r))
