@@ -443,7 +443,8 @@ int fvg_batch_phase_times(fvg_ctx* ctx, fvg_batch* batch, double* us);
 /* ---- device timing of the most recent solve ------------------------------- */
 typedef struct fvg_kernel_stats {
     double spmv_ms;             /* total time of the SpMV kernels */
-    double sweep_ms;            /* total time of the triangular sweeps (all levels of each) */
+    double sweep_ms;            /* total time of the triangular solves: each one's set-up kernel (pending
+                                   fill, right-hand side in sweep order) and its sweep (all levels) */
     int64_t spmv_launches;
     int64_t sweep_launches;     /* kernel launches spent in sweeps */
     int64_t spmv_calls;         /* SpMV products timed */
