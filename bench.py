@@ -570,7 +570,7 @@ def main():
                                       "GB/s": spmv_bytes(n, nnz) / (spmv_stream_ms * 1e-3) / 1e9,
                                       "frac": spmv_bytes(n, nnz) / (spmv_stream_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}},
             "ic0_sweep": {"span": "one triangular solve: k_sweep_prepare (b in position order, pending fill of "
-                                  "polled rows) + k_sweep_blocked",
+                                  "polled positions) + k_sweep_blocked + k_sweep_finish (values to their rows)",
                           "avg_ms": sweep_avg_ms, "algorithmic_bytes": sweep_bytes(n, nnz_s), "GB/s": sweep_gbs,
                           "frac": sweep_gbs / HBM_PEAK_GBS, "calls": ks["sweep_calls"], "launches": ks["sweep_launches"],
                           "total_ms": ks["sweep_ms"], "levels": [lo_lv, up_lv]},

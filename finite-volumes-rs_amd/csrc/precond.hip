@@ -424,16 +424,20 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
 // loads, the outside values level l + 1 will read and puts them in LDS.  The other
 // waves only read LDS for their operands; their own loads are the static data of each
 // row (count, LDS slots, coefficients, right-hand side, scale: all in position order),
-// fetched kBlockAhead levels before use, so no load of theirs waits on a poll.  Each
-// row subtracts its products in ascending column order from b, as the sequential sweep
-// does: the same bits.  Cross-block waits only ever target lower levels, so the block at
-// the lowest level always progresses; every wait is bounded and raises the error flag.
+// fetched kBlockAhead levels before use, so no load of theirs waits on a poll.  Solved
+// values are published in position order (xpos), so a level's stores are contiguous
+// and polls address positions; k_sweep_finish scatters them to their rows afterwards.
+// (Scattered per-row stores inside this kernel measured 40 us slower per sweep: they
+// load the memory system that the polls' latency depends on.)  Each row subtracts its
+// products in ascending column order from b, as the sequential sweep does: the same
+// bits.  Cross-block waits only ever target lower levels, so the block at the lowest
+// level always progresses; every wait is bounded and raises the error flag.
 constexpr int kBlockAhead = 4;
 constexpr int kFetchBatch = 4;   // polls in flight per loader lane
 
 // before each blocked sweep: the right-hand side in position order (threads [0, n)) and
-// the pending fill of the rows that some block fetches from outside (threads [n, n + m));
-// no other row is ever polled
+// the pending fill of the positions that some block fetches from outside (threads
+// [n, n + m)); no other position is ever polled
 template <class T>
 __global__ void k_sweep_prepare(int64_t n, const int32_t* __restrict__ brow, const T* __restrict__ b,
                                 T* __restrict__ bpos, int64_t m, const int32_t* __restrict__ rows, T* __restrict__ x,
@@ -446,6 +450,15 @@ __global__ void k_sweep_prepare(int64_t n, const int32_t* __restrict__ brow, con
         reinterpret_cast<typename Bits<T>::U*>(x)[rows[k - n]] = Bits<T>::pending;
 }
 
+// after each blocked sweep: its values, in position order, to their rows
+template <class T>
+__global__ void k_sweep_finish(int64_t n, const int32_t* __restrict__ brow, const T* __restrict__ xpos,
+                               T* __restrict__ x, const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) x[brow[p]] = xpos[p];
+}
+
 // scale in position order (after each refactorisation)
 __global__ void k_gather_rows(int64_t n, const int32_t* __restrict__ rows, const double* __restrict__ v,
                               double* __restrict__ out) {
@@ -455,7 +468,7 @@ __global__ void k_gather_rows(int64_t n, const int32_t* __restrict__ rows, const
 
 template <class T, int K>
 struct BlockRow {
-    int i, cnt;
+    int p, cnt;
     bool ok;
     int d[K];
     T a[K];
@@ -466,9 +479,9 @@ template <class T, bool SCALE, int K>
 __global__ void __launch_bounds__(1024)
     k_sweep_blocked(int nlevels, int64_t n, int W, int X, const int32_t* __restrict__ brange,
                     const int32_t* __restrict__ blvl, const int32_t* __restrict__ bxptr,
-                    const int32_t* __restrict__ bxidx, const int32_t* __restrict__ brow,
-                    const int32_t* __restrict__ bcnt, const int32_t* __restrict__ bdep, const T* __restrict__ bval,
-                    const T* __restrict__ bpos, const T* __restrict__ spos, T* x, int32_t* __restrict__ err,
+                    const int32_t* __restrict__ bxidx, const int32_t* __restrict__ bcnt,
+                    const int32_t* __restrict__ bdep, const T* __restrict__ bval, const T* __restrict__ bpos,
+                    const T* __restrict__ spos, T* xpos, int32_t* __restrict__ err,
                     const int32_t* __restrict__ skip) {
     using U = typename Bits<T>::U;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -516,7 +529,7 @@ __global__ void __launch_bounds__(1024)
                     jj[q] = k < k1 ? s_xi[k] : -1;
                 }
 #pragma unroll
-                for (int q = 0; q < kFetchBatch; ++q) w[q] = jj[q] >= 0 ? load_word(x, jj[q]) : U(0);
+                for (int q = 0; q < kFetchBatch; ++q) w[q] = jj[q] >= 0 ? load_word(xpos, jj[q]) : U(0);
                 for (int spins = 0;; ++spins) {
                     bool pend = false;
 #pragma unroll
@@ -529,7 +542,7 @@ __global__ void __launch_bounds__(1024)
                     }
 #pragma unroll
                     for (int q = 0; q < kFetchBatch; ++q)
-                        if (w[q] == Bits<T>::pending) w[q] = load_word(x, jj[q]);
+                        if (w[q] == Bits<T>::pending) w[q] = load_word(xpos, jj[q]);
                 }
 #pragma unroll
                 for (int q = 0; q < kFetchBatch; ++q)
@@ -556,7 +569,7 @@ __global__ void __launch_bounds__(1024)
         const bool ok = l < Le && p0 + ct < p1;
         int64_t p = ok ? p0 + ct : p0;
         p = p < n ? p : n - 1;
-        r.i = brow[p];
+        r.p = (int)p;
         r.cnt = bcnt[p];
 #pragma unroll
         for (int q = 0; q < K; ++q) {
@@ -577,7 +590,7 @@ __global__ void __launch_bounds__(1024)
             if (q < r.cnt) s -= r.a[q] * ring[r.d[q]];
         const T res = SCALE ? r.sc * s : s;
         ring[(l % kBlockRing) * W + ct] = res;
-        __hip_atomic_store(reinterpret_cast<U*>(x) + r.i, Bits<T>::bits(res), __ATOMIC_RELAXED,
+        __hip_atomic_store(reinterpret_cast<U*>(xpos) + r.p, Bits<T>::bits(res), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     };
     BlockRow<T, K> r0, r1, r2, r3;
@@ -727,6 +740,7 @@ static void free_sweep(fvg_ctx* c, Sweep& s) {
     dev_free_n(c, s.spos, s.n);
     dev_free_n(c, s.bpos, s.n);
     dev_free_n(c, s.bfill, s.nfill);
+    dev_free_n(c, s.xpos, s.n);
     s.nblk = 0;
     s.nfill = 0;
 }
@@ -933,7 +947,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
             std::sort(v.begin(), v.end());
             v.erase(std::unique(v.begin(), v.end()), v.end());
             bxptr[(size_t)(b * L1 + l)] = (int32_t)bxidx.size();
-            bxidx.insert(bxidx.end(), v.begin(), v.end());
+            for (int32_t j : v) bxidx.push_back(pos_of[(size_t)j]);   // polled by position
             X = std::max<int32_t>(X, (int32_t)v.size());
         }
         bxptr[(size_t)(b * L1 + nlevels)] = (int32_t)bxidx.size();
@@ -982,6 +996,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
         (s = dev_alloc_n(c, n, &sw.brow)) || (s = dev_alloc_n(c, n, &sw.bcnt)) ||
         (s = dev_alloc_n(c, kmax * n, &sw.bdep)) || (s = dev_alloc_n(c, kmax * n, &sw.bsrc)) ||
         (s = dev_alloc_n(c, kmax * n, &sw.bval)) || (s = dev_alloc_n(c, n, &sw.spos)) ||
+        (s = dev_alloc_n(c, n, &sw.xpos)) ||
         (s = dev_alloc_n(c, n, &sw.bpos)) ||
         (s = dev_alloc_n(c, sw.nfill, &sw.bfill)) || (s = upload(c, sw.bfill, fill.data(), sw.nfill)) ||
         (s = upload(c, sw.brange, brange.data(), 2 * nb)) ||
@@ -1278,19 +1293,21 @@ static bool blocked_fits(fvg_ctx* c, Sweep& sw) {
 
 static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, const double* b, double* x, int32_t* err,
                           const int32_t* skip) {
-    // the timed span holds one whole triangular solve: the prepare kernel and the sweep
+    // the timed span holds one whole triangular solve: set-up, sweep and scatter kernels
     const int tok = timing_begin(c);
     hipLaunchKernelGGL(k_sweep_prepare<double>, dim3(grid_for(sw.n + sw.nfill)), dim3(kBlock), 0, c->stream, sw.n,
-                       sw.brow, b, sw.bpos, sw.nfill, sw.bfill, x, skip);
+                       sw.brow, b, sw.bpos, sw.nfill, sw.bfill, sw.xpos, skip);
     int nlev = sw.nlevels, W = sw.bW, X = sw.bX;
     int64_t n = sw.n;
     const void* fn = scale ? blocked_kernel<true>(sw.bK) : blocked_kernel<false>(sw.bK);
     void* args[] = {&nlev, &n, &W, &X, (void*)&sw.brange, (void*)&sw.blvl, (void*)&sw.bxptr, (void*)&sw.bxidx,
-                    (void*)&sw.brow, (void*)&sw.bcnt, (void*)&sw.bdep, (void*)&sw.bval, (void*)&sw.bpos,
-                    (void*)&sw.spos, (void*)&x, (void*)&err, (void*)&skip};
+                    (void*)&sw.bcnt, (void*)&sw.bdep, (void*)&sw.bval, (void*)&sw.bpos,
+                    (void*)&sw.spos, (void*)&sw.xpos, (void*)&err, (void*)&skip};
     const int threads = ((sw.bW + 63) / 64 + 1) * 64;
     FVG_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)sw.nblk), dim3(threads), args, blocked_lds(sw), c->stream));
-    timing_end(c, tok, kSpanSweep, 2);
+    hipLaunchKernelGGL(k_sweep_finish<double>, dim3(grid_for(sw.n)), dim3(kBlock), 0, c->stream, sw.n, sw.brow,
+                       (const double*)sw.xpos, x, skip);
+    timing_end(c, tok, kSpanSweep, 3);
     FVG_HIP(hipGetLastError());
     return FVG_OK;
 }

@@ -36,7 +36,7 @@ struct Sweep {
     int32_t* brange = nullptr;          // per block: first level, one past the last (2 x nblk)
     int32_t* blvl = nullptr;            // nblk x (nlevels + 1): first position of each level
     int32_t* bxptr = nullptr;           // nblk x (nlevels + 1): offsets into bxidx
-    int32_t* bxidx = nullptr;           // rows fetched from outside the block's value ring
+    int32_t* bxidx = nullptr;           // positions fetched from outside the block's value ring
     int64_t nbx = 0;                    // length of bxidx
     int32_t* brow = nullptr;            // row of each position (n)
     int32_t* bcnt = nullptr;            // dependencies of each position (n)
@@ -45,8 +45,9 @@ struct Sweep {
     double* bval = nullptr;             // bK x n: its coefficient
     double* spos = nullptr;             // scale (inverse diagonal) in position order, per refactor
     double* bpos = nullptr;             // right-hand side in position order, per call
-    int32_t* bfill = nullptr;           // rows some block fetches from outside, ascending (nfill)
+    int32_t* bfill = nullptr;           // positions some block fetches from outside, ascending (nfill)
     int64_t nfill = 0;                  // only they need the pending fill before a call
+    double* xpos = nullptr;             // the sweep's values in position order (n), scattered after
 };
 constexpr int kBlockRing = 3;           // levels of values a block keeps in LDS
 

@@ -444,7 +444,8 @@ int fvg_batch_phase_times(fvg_ctx* ctx, fvg_batch* batch, double* us);
 typedef struct fvg_kernel_stats {
     double spmv_ms;             /* total time of the SpMV kernels */
     double sweep_ms;            /* total time of the triangular solves: each one's set-up kernel (pending
-                                   fill, right-hand side in sweep order) and its sweep (all levels) */
+                                   fill, right-hand side in sweep order), its sweep (all levels) and
+                                   the scatter of its values to their rows */
     int64_t spmv_launches;
     int64_t sweep_launches;     /* kernel launches spent in sweeps */
     int64_t spmv_calls;         /* SpMV products timed */
