@@ -741,6 +741,8 @@ static void free_sweep(fvg_ctx* c, Sweep& s) {
     dev_free_n(c, s.bpos, s.n);
     dev_free_n(c, s.bfill, s.nfill);
     dev_free_n(c, s.xpos, s.n);
+    dev_free_n(c, s.bfrom, s.n);
+    s.h_brow.clear();
     s.nblk = 0;
     s.nfill = 0;
 }
@@ -985,6 +987,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
     std::vector<int32_t> fill(bxidx);
     std::sort(fill.begin(), fill.end());
     fill.erase(std::unique(fill.begin(), fill.end()), fill.end());
+    sw.h_brow = order;
     sw.nblk = nb;
     sw.bX = X;
     sw.bK = kmax;
@@ -1075,6 +1078,24 @@ static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::ve
         return s;
     return build_blocked(c, rp, col, n, upper, src_map, lev, sw.nlevels,
                          c->tune_sweep_blocks ? (int)c->tune_sweep_blocks : std::max(2, c->num_cus), sw);
+}
+
+// When both layouts of a factor are blocked, the upper set-up reads the lower sweep's
+// values in the lower layout's position order: up.bfrom[q] = lower position of row
+// up.brow[q].
+static int pair_layouts(fvg_ctx* c, Sweep& lo, Sweep& up) {
+    int s = FVG_OK;
+    if (lo.nblk > 0 && up.nblk > 0 && lo.n == up.n) {
+        std::vector<int32_t> pos_lo((size_t)lo.n), from((size_t)up.n);
+        for (int64_t p = 0; p < lo.n; ++p) pos_lo[(size_t)lo.h_brow[(size_t)p]] = (int32_t)p;
+        for (int64_t q = 0; q < up.n; ++q) from[(size_t)q] = pos_lo[(size_t)up.h_brow[(size_t)q]];
+        if ((s = dev_alloc_n(c, up.n, &up.bfrom)) == FVG_OK) s = upload(c, up.bfrom, from.data(), up.n);
+    }
+    for (Sweep* sw : {&lo, &up}) {
+        sw->h_brow.clear();
+        sw->h_brow.shrink_to_fit();
+    }
+    return s;
 }
 
 static void free_tri32(fvg_ctx* c, Tri32& t) {
@@ -1291,12 +1312,15 @@ static bool blocked_fits(fvg_ctx* c, Sweep& sw) {
     return ok;
 }
 
+// One blocked triangular solve.  The right-hand side is b[brow[p]] (natural order), or,
+// with `from`, b[from[p]] (b in another layout's position order).  `scatter` = false
+// leaves the values in sw.xpos only (a paired upper sweep reads them there).
 static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, const double* b, double* x, int32_t* err,
-                          const int32_t* skip) {
+                          const int32_t* skip, const int32_t* from = nullptr, bool scatter = true) {
     // the timed span holds one whole triangular solve: set-up, sweep and scatter kernels
     const int tok = timing_begin(c);
     hipLaunchKernelGGL(k_sweep_prepare<double>, dim3(grid_for(sw.n + sw.nfill)), dim3(kBlock), 0, c->stream, sw.n,
-                       sw.brow, b, sw.bpos, sw.nfill, sw.bfill, sw.xpos, skip);
+                       from ? from : sw.brow, b, sw.bpos, sw.nfill, sw.bfill, sw.xpos, skip);
     int nlev = sw.nlevels, W = sw.bW, X = sw.bX;
     int64_t n = sw.n;
     const void* fn = scale ? blocked_kernel<true>(sw.bK) : blocked_kernel<false>(sw.bK);
@@ -1305,9 +1329,10 @@ static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, cons
                     (void*)&sw.spos, (void*)&sw.xpos, (void*)&err, (void*)&skip};
     const int threads = ((sw.bW + 63) / 64 + 1) * 64;
     FVG_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)sw.nblk), dim3(threads), args, blocked_lds(sw), c->stream));
-    hipLaunchKernelGGL(k_sweep_finish<double>, dim3(grid_for(sw.n)), dim3(kBlock), 0, c->stream, sw.n, sw.brow,
-                       (const double*)sw.xpos, x, skip);
-    timing_end(c, tok, kSpanSweep, 3);
+    if (scatter)
+        hipLaunchKernelGGL(k_sweep_finish<double>, dim3(grid_for(sw.n)), dim3(kBlock), 0, c->stream, sw.n, sw.brow,
+                           (const double*)sw.xpos, x, skip);
+    timing_end(c, tok, kSpanSweep, scatter ? 3 : 2);
     FVG_HIP(hipGetLastError());
     return FVG_OK;
 }
@@ -1488,6 +1513,21 @@ int check_sweep_status(fvg_ctx* c, const fvg_prec* p) {
     return FVG_OK;
 }
 
+// z = U^-1 L^-1 r: the lower sweep into d_tmp, then the upper sweep.  When both layouts
+// are blocked, the upper set-up reads the lower sweep's values in its position order
+// and d_tmp is not written.
+static int apply_factor(fvg_ctx* c, const fvg_prec* p, const double* lo_scale, const double* r, double* z,
+                        const int32_t* skip) {
+    Sweep& lo = const_cast<Sweep&>(p->lower);
+    Sweep& up = const_cast<Sweep&>(p->upper);
+    if (c->tune_sweep_mode == 0 && up.bfrom && blocked_fits(c, lo) && blocked_fits(c, up)) {
+        FVG_TRY(launch_blocked(c, lo, lo_scale, r, p->d_tmp, p->d_flags + 3, skip, nullptr, false));
+        return launch_blocked(c, up, p->d_inv_diag, lo.xpos, z, p->d_flags + 3, skip, up.bfrom, true);
+    }
+    FVG_TRY(sweep(c, p->lower, lo_scale, r, p->d_tmp, p->d_flags + 3, skip));
+    return sweep(c, p->upper, p->d_inv_diag, p->d_tmp, z, p->d_flags + 3, skip);
+}
+
 int apply_prec(fvg_ctx* c, const fvg_prec* p, const double* r, double* z, const int32_t* skip) {
     switch (p->kind) {
     case kIdentity:
@@ -1499,11 +1539,9 @@ int apply_prec(fvg_ctx* c, const fvg_prec* p, const double* r, double* z, const 
         FVG_HIP(hipGetLastError());
         return FVG_OK;
     case kIC:
-        FVG_TRY(sweep(c, p->lower, p->d_inv_diag, r, p->d_tmp, p->d_flags + 3, skip));
-        return sweep(c, p->upper, p->d_inv_diag, p->d_tmp, z, p->d_flags + 3, skip);
+        return apply_factor(c, p, p->d_inv_diag, r, z, skip);
     case kILU:
-        FVG_TRY(sweep(c, p->lower, nullptr, r, p->d_tmp, p->d_flags + 3, skip));
-        return sweep(c, p->upper, p->d_inv_diag, p->d_tmp, z, p->d_flags + 3, skip);
+        return apply_factor(c, p, nullptr, r, z, skip);
     }
     return fail(FVG_E_INVALID, "unknown preconditioner");
 }
@@ -1522,6 +1560,7 @@ static int create_factored(fvg_ctx* c, const fvg_csr* a, int32_t level, int kind
     if (s == FVG_OK) s = dev_alloc_n(c, n, &p->d_tmp);
     if (s == FVG_OK) s = build_sweep(c, p->h_rp, p->h_col, n, false, nullptr, p->lower);
     if (s == FVG_OK) s = build_sweep(c, p->h_rp, p->h_col, n, true, kind == kIC ? &p->h_tpos : nullptr, p->upper);
+    if (s == FVG_OK) s = pair_layouts(c, p->lower, p->upper);
     if (s == FVG_OK) s = launch_fill(c, p->nnz, p->d_fac, 0.0);
     if (s == FVG_OK) s = factorize(c, p, a);
     if (s != FVG_OK) {

@@ -1,6 +1,8 @@
 // precond.hpp -- preconditioner objects (internal).
 #pragma once
 
+#include <vector>
+
 #include "internal.hpp"
 
 namespace fvg {
@@ -48,6 +50,11 @@ struct Sweep {
     int32_t* bfill = nullptr;           // positions some block fetches from outside, ascending (nfill)
     int64_t nfill = 0;                  // only they need the pending fill before a call
     double* xpos = nullptr;             // the sweep's values in position order (n), scattered after
+    // Upper layout of a factor whose lower layout is blocked too: the lower layout's
+    // position of the row at each upper position, so the upper set-up gathers its
+    // right-hand side straight from the lower sweep's xpos (no scatter in between).
+    int32_t* bfrom = nullptr;           // n, or null
+    std::vector<int32_t> h_brow;        // row of each position (host copy, freed once paired)
 };
 constexpr int kBlockRing = 3;           // levels of values a block keeps in LDS
 
