@@ -468,10 +468,10 @@ __global__ void k_gather_rows(int64_t n, const int32_t* __restrict__ rows, const
 
 template <class T, int K>
 struct BlockRow {
-    int p, cnt;
+    int p;
     bool ok;
-    int d[K];
-    T a[K];
+    uint64_t dw[K / 4];   // K 16-bit LDS slots, unpacked when the row is solved
+    double2 a[K / 2];     // K coefficients
     T bv, sc;
 };
 
@@ -479,8 +479,8 @@ template <class T, bool SCALE, int K>
 __global__ void __launch_bounds__(1024)
     k_sweep_blocked(int nlevels, int64_t n, int W, int X, const int32_t* __restrict__ brange,
                     const int32_t* __restrict__ blvl, const int32_t* __restrict__ bxptr,
-                    const int32_t* __restrict__ bxidx, const int32_t* __restrict__ bcnt,
-                    const int32_t* __restrict__ bdep, const T* __restrict__ bval, const T* __restrict__ bpos,
+                    const int32_t* __restrict__ bxidx, const uint16_t* __restrict__ bdep,
+                    const T* __restrict__ bval, const T* __restrict__ bpos,
                     const T* __restrict__ spos, T* xpos, int32_t* __restrict__ err,
                     const int32_t* __restrict__ skip) {
     using U = typename Bits<T>::U;
@@ -570,24 +570,27 @@ __global__ void __launch_bounds__(1024)
         int64_t p = ok ? p0 + ct : p0;
         p = p < n ? p : n - 1;
         r.p = (int)p;
-        r.cnt = bcnt[p];
+        const uint64_t* dw = reinterpret_cast<const uint64_t*>(bdep) + p * (K / 4);
+        const double2* a2 = reinterpret_cast<const double2*>(bval) + p * (K / 2);
 #pragma unroll
-        for (int q = 0; q < K; ++q) {
-            r.d[q] = bdep[(int64_t)q * n + p];
-            r.a[q] = bval[(int64_t)q * n + p];
-        }
+        for (int g = 0; g < K / 4; ++g) r.dw[g] = dw[g];
+#pragma unroll
+        for (int h = 0; h < K / 2; ++h) r.a[h] = a2[h];
         r.bv = bpos[p];
         r.sc = SCALE ? spos[p] : T(1);
         r.ok = ok;
     };
     auto solve = [&](int l, const BlockRow<T, K>& r) {
         if (!r.ok) return;
-        // (reading only the row's cnt slots measured faster than issuing all K reads at
+        // (reading only the row's used slots measured faster than issuing all K reads at
         // once with the unused ones pointed at slot 0)
         T s = r.bv;
 #pragma unroll
-        for (int q = 0; q < K; ++q)
-            if (q < r.cnt) s -= r.a[q] * ring[r.d[q]];
+        for (int q = 0; q < K; ++q) {
+            const unsigned d = (unsigned)(r.dw[q / 4] >> (16 * (q % 4))) & 0xFFFFu;
+            const T aq = (q & 1) ? r.a[q / 2].y : r.a[q / 2].x;
+            if (d != kNoSlot) s -= aq * ring[d];
+        }
         const T res = SCALE ? r.sc * s : s;
         ring[(l % kBlockRing) * W + ct] = res;
         __hip_atomic_store(reinterpret_cast<U*>(xpos) + r.p, Bits<T>::bits(res), __ATOMIC_RELAXED,
@@ -733,7 +736,6 @@ static void free_sweep(fvg_ctx* c, Sweep& s) {
     dev_free_n(c, s.bxptr, s.nblk * L1);
     dev_free_n(c, s.bxidx, s.nbx);
     dev_free_n(c, s.brow, s.n);
-    dev_free_n(c, s.bcnt, s.n);
     dev_free_n(c, s.bdep, K * s.n);
     dev_free_n(c, s.bsrc, K * s.n);
     dev_free_n(c, s.bval, K * s.n);
@@ -927,7 +929,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
     auto blk_of_row = [&](int32_t i) { return blk_of_comp[(size_t)comp_of[(size_t)i]]; };
     // dependencies: a value of the same block less than kBlockRing levels back is read from
     // the block's LDS ring; any other is fetched into the level's staging slots
-    std::vector<int32_t> bcnt((size_t)n, 0), bdep((size_t)(kmax * n), 0), bsrc((size_t)(kmax * n), -1);
+    std::vector<int32_t> bdep((size_t)(kmax * n), -1 - (1 << 30)), bsrc((size_t)(kmax * n), -1);
     std::vector<int32_t> bxptr((size_t)(nb * L1), 0), bxidx;
     std::vector<std::vector<int32_t>> need;   // per level of the block being built: fetched rows
     int32_t X = 0;
@@ -963,7 +965,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
                 for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
                     const int32_t j = col[(size_t)k];
                     if (!take(i, j)) continue;
-                    const size_t e = (size_t)((int64_t)q * n + p);
+                    const size_t e = (size_t)((int64_t)p * kmax + q);
                     bsrc[e] = src_map ? (*src_map)[(size_t)k] : k;
                     if (blk_of_row(j) == b && lev[(size_t)i] - lev[(size_t)j] < kBlockRing) {
                         const int lj = lev[(size_t)j];
@@ -975,15 +977,23 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
                     }
                     ++q;
                 }
-                bcnt[(size_t)p] = q;
             }
     }
-    if (X >= (1 << 20)) return FVG_OK;
-    for (auto& d : bdep)
+    // LDS slots must fit 16 bits (kNoSlot marks the unused ones)
+    if (X >= (1 << 20) || (int64_t)kBlockRing * W + 2 * (int64_t)X >= kNoSlot) return FVG_OK;
+    std::vector<uint16_t> bdep16(bdep.size());
+    for (size_t e = 0; e < bdep.size(); ++e) {
+        int32_t d = bdep[e];
+        if (d == -1 - (1 << 30)) {
+            bdep16[e] = kNoSlot;
+            continue;
+        }
         if (d < 0) {
             const int32_t v = -1 - d;
             d = kBlockRing * W + (v >> 20) * X + (v & ((1 << 20) - 1));
         }
+        bdep16[e] = (uint16_t)d;
+    }
     std::vector<int32_t> fill(bxidx);
     std::sort(fill.begin(), fill.end());
     fill.erase(std::unique(fill.begin(), fill.end()), fill.end());
@@ -996,7 +1006,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
     int s;
     if ((s = dev_alloc_n(c, 2 * nb, &sw.brange)) || (s = dev_alloc_n(c, nb * L1, &sw.blvl)) ||
         (s = dev_alloc_n(c, nb * L1, &sw.bxptr)) || (s = dev_alloc_n(c, sw.nbx, &sw.bxidx)) ||
-        (s = dev_alloc_n(c, n, &sw.brow)) || (s = dev_alloc_n(c, n, &sw.bcnt)) ||
+        (s = dev_alloc_n(c, n, &sw.brow)) ||
         (s = dev_alloc_n(c, kmax * n, &sw.bdep)) || (s = dev_alloc_n(c, kmax * n, &sw.bsrc)) ||
         (s = dev_alloc_n(c, kmax * n, &sw.bval)) || (s = dev_alloc_n(c, n, &sw.spos)) ||
         (s = dev_alloc_n(c, n, &sw.xpos)) ||
@@ -1005,7 +1015,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
         (s = upload(c, sw.brange, brange.data(), 2 * nb)) ||
         (s = upload(c, sw.blvl, blvl.data(), nb * L1)) || (s = upload(c, sw.bxptr, bxptr.data(), nb * L1)) ||
         (s = upload(c, sw.bxidx, bxidx.data(), sw.nbx)) || (s = upload(c, sw.brow, order.data(), n)) ||
-        (s = upload(c, sw.bcnt, bcnt.data(), n)) || (s = upload(c, sw.bdep, bdep.data(), kmax * n)) ||
+        (s = upload(c, sw.bdep, bdep16.data(), kmax * n)) ||
         (s = upload(c, sw.bsrc, bsrc.data(), kmax * n)))
         return s;
     return FVG_OK;
@@ -1325,7 +1335,7 @@ static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, cons
     int64_t n = sw.n;
     const void* fn = scale ? blocked_kernel<true>(sw.bK) : blocked_kernel<false>(sw.bK);
     void* args[] = {&nlev, &n, &W, &X, (void*)&sw.brange, (void*)&sw.blvl, (void*)&sw.bxptr, (void*)&sw.bxidx,
-                    (void*)&sw.bcnt, (void*)&sw.bdep, (void*)&sw.bval, (void*)&sw.bpos,
+                    (void*)&sw.bdep, (void*)&sw.bval, (void*)&sw.bpos,
                     (void*)&sw.spos, (void*)&sw.xpos, (void*)&err, (void*)&skip};
     const int threads = ((sw.bW + 63) / 64 + 1) * 64;
     FVG_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)sw.nblk), dim3(threads), args, blocked_lds(sw), c->stream));

@@ -41,10 +41,10 @@ struct Sweep {
     int32_t* bxidx = nullptr;           // positions fetched from outside the block's value ring
     int64_t nbx = 0;                    // length of bxidx
     int32_t* brow = nullptr;            // row of each position (n)
-    int32_t* bcnt = nullptr;            // dependencies of each position (n)
-    int32_t* bdep = nullptr;            // bK x n: LDS slot of each dependency, ascending columns
-    int32_t* bsrc = nullptr;            // bK x n: factor slot of each dependency (-1: none)
-    double* bval = nullptr;             // bK x n: its coefficient
+    uint16_t* bdep = nullptr;           // n x bK: LDS slot of each dependency, ascending columns, then
+                                        // kNoSlot (a row's bK slots are one or two 64-bit words)
+    int32_t* bsrc = nullptr;            // n x bK: factor slot of each dependency (-1: none)
+    double* bval = nullptr;             // n x bK: its coefficient (0 for none)
     double* spos = nullptr;             // scale (inverse diagonal) in position order, per refactor
     double* bpos = nullptr;             // right-hand side in position order, per call
     int32_t* bfill = nullptr;           // positions some block fetches from outside, ascending (nfill)
@@ -57,6 +57,7 @@ struct Sweep {
     std::vector<int32_t> h_brow;        // row of each position (host copy, freed once paired)
 };
 constexpr int kBlockRing = 3;           // levels of values a block keeps in LDS
+constexpr uint16_t kNoSlot = 0xFFFF;    // an unused dependency slot of the blocked layout
 
 // One strict triangular factor as a CSR in natural row order with FP32 values, for the
 // mixed-precision inner solver's Jacobi-iterated apply.
