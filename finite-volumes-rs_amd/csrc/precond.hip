@@ -1646,8 +1646,8 @@ static int refactor(fvg_ctx* c, fvg_prec* p, const fvg_csr* a, bool check) {
 
 int fvg_prec_sweep_layout(const fvg_prec* p, int32_t* info, int32_t count) {
     if (!p || (count > 0 && !info)) return fail(FVG_E_INVALID, "null argument");
-    const int64_t v[] = {p->lower.nblk, p->upper.nblk, p->lower.nlevels, p->lower.bW,
-                         p->lower.bX,   p->lower.nbx,  p->upper.bW,       p->upper.bX};
+    const int64_t v[] = {p->lower.nblk, p->upper.nblk, p->lower.nlevels, p->lower.bW, p->lower.bX,
+                         p->lower.nbx,  p->upper.bW,   p->upper.bX,       p->lower.bK, p->upper.bK};
     for (int k = 0; k < count && k < (int)(sizeof(v) / sizeof(v[0])); ++k)
         info[k] = (int32_t)std::min<int64_t>(v[k], INT32_MAX);
     return FVG_OK;

@@ -935,11 +935,12 @@ class Preconditioner:
     def sweep_layout(self):
         """Shape of the blocked sweep layouts: lower_blocks, upper_blocks (0: not built),
         levels, lower_rows / lower_fetch (most rows of a block in one level, most values it
-        fetches from outside for one level), lower_fetch_total, upper_rows, upper_fetch."""
-        info = np.zeros(8, dtype=np.int32)
-        check(lib().fvg_prec_sweep_layout(self.h, info.ctypes.data, 8))
+        fetches from outside for one level), lower_fetch_total, upper_rows, upper_fetch,
+        lower_slots / upper_slots (dependency slots per row: 4 or 8)."""
+        info = np.zeros(10, dtype=np.int32)
+        check(lib().fvg_prec_sweep_layout(self.h, info.ctypes.data, 10))
         keys = ("lower_blocks", "upper_blocks", "levels", "lower_rows", "lower_fetch", "lower_fetch_total",
-                "upper_rows", "upper_fetch")
+                "upper_rows", "upper_fetch", "lower_slots", "upper_slots")
         return dict(zip(keys, (int(v) for v in info)))
 
     def refactor_async(self, matrix):

@@ -310,11 +310,12 @@ int fvg_ic_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out)
  * with its drop rule |v| <= 1e-6 |u_ii| (:255-258).  level must be 0. */
 int fvg_ilu_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out);
 /* Diagnostics (no reference counterpart): the shape of the blocked sweep layouts, the
- * first `count` (at most 8) of: blocks of the lower and of the upper factor (0: not built,
- * the persistent sweep serves), dependency levels of the lower sweep, the lower layout's
- * most rows of one block in one level and most values one block fetches from outside for
- * one level, its total number of such fetches per sweep, then the upper layout's most rows
- * and most fetches. */
+ * first `count` (at most 10) of: blocks of the lower and of the upper factor (0: not
+ * built, the persistent sweep serves), dependency levels of the lower sweep, the lower
+ * layout's most rows of one block in one level and most values one block fetches from
+ * outside for one level, its total number of such fetches per sweep, the upper layout's
+ * most rows and most fetches, then the dependency slots per row of the lower and of the
+ * upper layout (4 or 8). */
 int fvg_prec_sweep_layout(const fvg_prec* prec, int32_t* info, int32_t count);
 /* Re-run the numeric factorisation for new values on the same pattern. */
 int fvg_prec_refactor(fvg_ctx* ctx, fvg_prec* p, const fvg_csr* a);

@@ -134,3 +134,35 @@ def test_blocked_sweep_is_built_and_exact_after_refactor(ctx, name, kind, prec, 
     P.refactor(A2)
     P.precondition(z, fvg.DistributedVector.from_data(ctx, r))
     assert np.array_equal(z.numpy(), O.OraclePrec(prec, rp, ci, va2, 0).apply(r))
+
+
+@pytest.mark.parametrize("prec,kind", [("ic", "sym"), ("ilu", "nonsym")])
+def test_blocked_sweep_with_eight_dependency_slots(ctx, prec, kind):
+    """Rows with 5 to 8 lower (and upper) dependencies take the blocked kernel's 8-slot
+    form: the tet mesh's matrix plus long-range couplings of every 7th row to the rows
+    1500, 2500, 3500 and 4500 before and after it (many levels apart, so the layout's
+    components stay small), the diagonal raised to keep the factorisation stable.
+    Oracle bits."""
+    rp, ci, va, b = _oracle_system("tet10", kind)
+    n = rp.shape[0] - 1
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n)).tolil()
+    rng = np.random.default_rng(6)
+    for i in range(0, n, 7):
+        for off in (-4500, -3500, -2500, -1500, 1500, 2500, 3500, 4500):
+            j = i + off
+            if 0 <= j < n:
+                A[i, j] = -0.05
+                A[j, i] = -0.05 if kind == "sym" else -0.05 * rng.uniform(0.5, 1.5)
+    A = A.tocsr()
+    A.setdiag(A.diagonal() + 0.5)
+    A.sort_indices()
+    rp2, ci2, va2 = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+    M = fvg.DistributedMatrix.from_csr(ctx, rp2, ci2, va2)
+    P = (fvg.IncompleteCholesky if prec == "ic" else fvg.IncompleteLowerUpper).from_matrix(M, 0)
+    lay = P.sweep_layout()
+    assert lay["lower_blocks"] > 0 and lay["upper_blocks"] > 0
+    assert lay["lower_slots"] == 8 and lay["upper_slots"] == 8
+    r = np.random.default_rng(8).normal(size=n)
+    z = fvg.DistributedVector(ctx, n)
+    P.precondition(z, fvg.DistributedVector.from_data(ctx, r))
+    assert np.array_equal(z.numpy(), O.OraclePrec(prec, rp2, ci2, va2, 0).apply(r))
