@@ -272,6 +272,10 @@ int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
         if (value > 4096) return fail(FVG_E_INVALID, "at most 4096 sweep blocks");
         c->tune_sweep_blocks = value;
         return FVG_OK;
+    case FVG_TUNE_SPIN_LIMIT:
+        if (value > (1 << 30)) return fail(FVG_E_INVALID, "spin limit above 2^30");
+        c->tune_spin_limit = value;
+        return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }
@@ -286,6 +290,7 @@ int fvg_ctx_get_tuning(const fvg_ctx* c, int32_t knob, int64_t* value) {
     case FVG_TUNE_INNER_SWEEPS: *value = c->tune_inner_sweeps; return FVG_OK;
     case FVG_TUNE_GRAPHS: *value = c->tune_graphs; return FVG_OK;
     case FVG_TUNE_SWEEP_BLOCKS: *value = c->tune_sweep_blocks; return FVG_OK;
+    case FVG_TUNE_SPIN_LIMIT: *value = c->tune_spin_limit; return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }

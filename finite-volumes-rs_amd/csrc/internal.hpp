@@ -65,6 +65,7 @@ struct fvg_ctx {
     int64_t tune_inner_sweeps = 0;     // FVG_TUNE_INNER_SWEEPS
     int64_t tune_graphs = 0;           // FVG_TUNE_GRAPHS
     int64_t tune_sweep_blocks = 0;     // FVG_TUNE_SWEEP_BLOCKS
+    int64_t tune_spin_limit = 0;       // FVG_TUNE_SPIN_LIMIT
     int num_cus = 0;
 };
 

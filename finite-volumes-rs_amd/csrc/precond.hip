@@ -264,7 +264,9 @@ struct Bits<float> {
     __device__ static U bits(float v) { return __float_as_uint(v); }
 };
 
-constexpr int kSpinLimit = 1 << 18;
+constexpr int kSpinLimit = 1 << 18;   // polls before a wait gives up (FVG_TUNE_SPIN_LIMIT overrides)
+
+static int spin_limit(const fvg_ctx* c) { return c->tune_spin_limit > 0 ? (int)c->tune_spin_limit : kSpinLimit; }
 constexpr int kWavesPerGroup = 4;
 constexpr int kMaxRowDeps = 6;     // dependencies loaded together before any wait
 
@@ -282,12 +284,12 @@ __device__ __forceinline__ typename Bits<T>::U load_word(const T* x, int j) {
 }
 
 template <class T>
-__device__ __forceinline__ T wait_value(const T* x, int j, int32_t* err) {
+__device__ __forceinline__ T wait_value(const T* x, int j, int32_t* err, int spin_limit) {
     using U = typename Bits<T>::U;
     U v = load_word(x, j);
     int spins = 0;
     while (v == Bits<T>::pending) {
-        if (++spins > kSpinLimit ||
+        if (++spins > spin_limit ||
             ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
@@ -315,7 +317,7 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
     k_sweep_persistent(int64_t nchunks, int64_t nwaves, const int32_t* __restrict__ rows,
                        const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                        const T* __restrict__ val, const T* __restrict__ scale, const T* __restrict__ b, T* x,
-                       int32_t* __restrict__ err, const int32_t* __restrict__ skip) {
+                       int32_t* __restrict__ err, const int32_t* __restrict__ skip, int spin_limit) {
     using U = typename Bits<T>::U;
     if (skip && *skip) return;
     const int64_t w0 = (int64_t)blockIdx.x * kWavesPerGroup + (threadIdx.x >> 6);
@@ -349,7 +351,7 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
             };
             int spins = 0;
             auto give_up = [&]() {
-                if (++spins > kSpinLimit ||
+                if (++spins > spin_limit ||
                     ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                     __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     return true;
@@ -407,7 +409,7 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
             for (int q = 0; q < kMaxRowDeps; ++q)
                 if (q < len) s -= a[q] * Bits<T>::value(w[q]);
         } else {
-            for (int k = kb; k < ke; ++k) s -= val[k] * wait_value(x, col[k], err);
+            for (int k = kb; k < ke; ++k) s -= val[k] * wait_value(x, col[k], err, spin_limit);
         }
         const T r = SCALE ? sc * s : s;
         __hip_atomic_store(reinterpret_cast<U*>(x) + i, Bits<T>::bits(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -482,7 +484,7 @@ __global__ void __launch_bounds__(1024)
                     const int32_t* __restrict__ bxidx, const uint16_t* __restrict__ bdep,
                     const T* __restrict__ bval, const T* __restrict__ bpos,
                     const T* __restrict__ spos, T* xpos, int32_t* __restrict__ err,
-                    const int32_t* __restrict__ skip) {
+                    const int32_t* __restrict__ skip, int spin_limit) {
     using U = typename Bits<T>::U;
     extern __shared__ __align__(16) unsigned char smem[];
     if (skip && *skip) return;
@@ -535,7 +537,7 @@ __global__ void __launch_bounds__(1024)
 #pragma unroll
                     for (int q = 0; q < kFetchBatch; ++q) pend |= w[q] == Bits<T>::pending;
                     if (!pend) break;
-                    if (spins > kSpinLimit ||
+                    if (spins > spin_limit ||
                         ((spins & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         break;
@@ -1284,8 +1286,9 @@ static int launch_sweep(fvg_ctx* c, const Sweep& sw, const T* val, const T* scal
     int64_t nwaves = sweep_waves(c, sw, max_coresident_groups<T>(c));
     nwaves = std::min(nwaves, (nchunks + kWavesPerGroup - 1) / kWavesPerGroup * kWavesPerGroup);
     const int64_t groups = nwaves / kWavesPerGroup;
+    int spin = spin_limit(c);
     void* args[] = {&nchunks, &nwaves, (void*)&sw.rows, (void*)&sw.ptr, (void*)&sw.col, (void*)&val, (void*)&scale,
-                    (void*)&b, (void*)&x, (void*)&err, (void*)&skip};
+                    (void*)&b, (void*)&x, (void*)&err, (void*)&skip, (void*)&spin};
     const void* fn = sweep_kernel<T>(c->tune_sweep_poll, scale != nullptr);
     FVG_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)groups), dim3(kChunk * kWavesPerGroup), args, 0, c->stream));
     return FVG_OK;
@@ -1331,12 +1334,12 @@ static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, cons
     const int tok = timing_begin(c);
     hipLaunchKernelGGL(k_sweep_prepare<double>, dim3(grid_for(sw.n + sw.nfill)), dim3(kBlock), 0, c->stream, sw.n,
                        from ? from : sw.brow, b, sw.bpos, sw.nfill, sw.bfill, sw.xpos, skip);
-    int nlev = sw.nlevels, W = sw.bW, X = sw.bX;
+    int nlev = sw.nlevels, W = sw.bW, X = sw.bX, spin = spin_limit(c);
     int64_t n = sw.n;
     const void* fn = scale ? blocked_kernel<true>(sw.bK) : blocked_kernel<false>(sw.bK);
     void* args[] = {&nlev, &n, &W, &X, (void*)&sw.brange, (void*)&sw.blvl, (void*)&sw.bxptr, (void*)&sw.bxidx,
                     (void*)&sw.bdep, (void*)&sw.bval, (void*)&sw.bpos,
-                    (void*)&sw.spos, (void*)&sw.xpos, (void*)&err, (void*)&skip};
+                    (void*)&sw.spos, (void*)&sw.xpos, (void*)&err, (void*)&skip, (void*)&spin};
     const int threads = ((sw.bW + 63) / 64 + 1) * 64;
     FVG_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)sw.nblk), dim3(threads), args, blocked_lds(sw), c->stream));
     if (scatter)

@@ -79,7 +79,10 @@ enum {
     FVG_TUNE_GRAPHS = 6,
     /* About how many blocks the blocked sweep layout of a preconditioner created
      * afterwards is cut into (0: one per compute unit; at most 4096). */
-    FVG_TUNE_SWEEP_BLOCKS = 7
+    FVG_TUNE_SWEEP_BLOCKS = 7,
+    /* Polls a triangular-sweep wait makes before it gives up and the call returns
+     * FVG_E_TIMEOUT (0: 2^18).  A tiny value forces the time-out path, for tests. */
+    FVG_TUNE_SPIN_LIMIT = 8
 };
 int fvg_ctx_set_tuning(fvg_ctx* ctx, int32_t knob, int64_t value);
 int fvg_ctx_get_tuning(const fvg_ctx* ctx, int32_t knob, int64_t* value);

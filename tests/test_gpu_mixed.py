@@ -166,3 +166,35 @@ def test_blocked_sweep_with_eight_dependency_slots(ctx, prec, kind):
     z = fvg.DistributedVector(ctx, n)
     P.precondition(z, fvg.DistributedVector.from_data(ctx, r))
     assert np.array_equal(z.numpy(), O.OraclePrec(prec, rp2, ci2, va2, 0).apply(r))
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_sweep_wait_time_out_is_reported_and_recovers(ctx, mode):
+    """Every wait of a triangular sweep is bounded.  With the spin limit at its minimum
+    (a re-poll or two), a wait that finds its value still pending gives up, sets the error
+    flag that ends all other waits, and the kernel exits normally: the call returns
+    FVG_E_TIMEOUT.  With the limit restored, the next call gives the oracle's bits
+    again (blocked sweep: mode 0; persistent sweep: mode 2)."""
+    rp, ci, va, b = _oracle_system("tet10", "sym")
+    n = rp.shape[0] - 1
+    A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+    ic = fvg.IncompleteCholesky.from_matrix(A, 0)
+    r = fvg.DistributedVector.from_data(ctx, np.random.default_rng(2).normal(size=n))
+    ref = O.OraclePrec("ic", rp, ci, va, 0).apply(r.numpy())
+    z = fvg.DistributedVector(ctx, n)
+    timed_out = 0
+    try:
+        ctx.set_tuning(fvg.TUNE_SWEEP_MODE, mode)
+        ctx.set_tuning(fvg.TUNE_SPIN_LIMIT, 1)
+        for _ in range(20):
+            try:
+                ic.precondition(z, r)
+            except fvg.FvgError as e:
+                assert fvg.STATUS[e.status] == "FVG_E_TIMEOUT"
+                timed_out += 1
+    finally:
+        ctx.set_tuning(fvg.TUNE_SPIN_LIMIT, 0)
+        ctx.set_tuning(fvg.TUNE_SWEEP_MODE, 0)
+    assert timed_out > 0
+    ic.precondition(z, r)
+    assert np.array_equal(z.numpy(), ref)
