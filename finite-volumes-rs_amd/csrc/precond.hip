@@ -732,7 +732,7 @@ static void free_sweep(fvg_ctx* c, Sweep& s) {
     dev_free_n(c, s.col, s.nnz + 1);
     dev_free_n(c, s.src, s.nnz);
     dev_free_n(c, s.val, s.nnz + 1);
-    const int64_t L1 = s.nlevels + 1, K = s.bK;
+    const int64_t L1 = s.bL + 1, K = s.bK;
     dev_free_n(c, s.brange, 2 * (int64_t)s.nblk);
     dev_free_n(c, s.blvl, s.nblk * L1);
     dev_free_n(c, s.bxptr, s.nblk * L1);
@@ -760,8 +760,8 @@ static void free_sweep(fvg_ctx* c, Sweep& s) {
 constexpr int kBlockMaxDeps = 8;
 constexpr int kBlockMaxRows = 960;   // 15 compute waves + the loader wave
 static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
-                         bool upper, const std::vector<int32_t>* src_map, const std::vector<int32_t>& lev,
-                         int nlevels, int target, Sweep& sw) {
+                         bool upper, const std::vector<int32_t>* src_map, const std::vector<int32_t>& lev_in,
+                         int nlevels_in, int target, Sweep& sw) {
     auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
     if (n < 4096 || target < 2) return FVG_OK;
     std::vector<int32_t> par((size_t)n);
@@ -783,7 +783,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
             const int32_t j = col[(size_t)k];
             if (!take(i, j)) continue;
             ++cnt;
-            if (lev[(size_t)i] - lev[(size_t)j] == 1) {
+            if (lev_in[(size_t)i] - lev_in[(size_t)j] == 1) {
                 const int32_t a = find((int32_t)i), b = find(j);
                 if (a != b) par[(size_t)a] = b;
             }
@@ -805,8 +805,47 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
         comp_of[(size_t)i] = id_of_root[(size_t)r];
         comp_size[(size_t)comp_of[(size_t)i]]++;
     }
+    bool contiguous = false;
     for (int32_t sz : comp_size)
-        if ((int64_t)sz * 8 > n) return FVG_OK;
+        if ((int64_t)sz * 8 > n) contiguous = true;
+    // One component holding much of the matrix (every dependency one level deep, as on
+    // extruded and 2D meshes): blocks of consecutive rows instead, with delayed levels.
+    // A row's level is then one past its deepest dependency in its own block and two
+    // past one in another block, so every dependency between blocks spans at least two
+    // levels, as the kernel requires.  Built only when that costs at most half again the
+    // levels and keeps at least 8 blocks busy per level on average (on a 2D mesh the
+    // consecutive blocks would run one after another).
+    std::vector<int32_t> dlev;
+    int nlevels = nlevels_in;
+    if (contiguous) {
+        const int32_t nbc = (int32_t)std::min<int64_t>(target, n / 64);
+        comp_size.assign((size_t)nbc, 0);
+        for (int64_t i = 0; i < n; ++i) {
+            comp_of[(size_t)i] = (int32_t)(i * nbc / n);
+            comp_size[(size_t)comp_of[(size_t)i]]++;
+        }
+        dlev.assign((size_t)n, 0);
+        int maxl = 0;
+        for (int64_t t = 0; t < n; ++t) {
+            const int64_t i = upper ? n - 1 - t : t;
+            int l = 0;
+            for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
+                const int32_t j = col[(size_t)k];
+                if (take(i, j))
+                    l = std::max(l, dlev[(size_t)j] + (comp_of[(size_t)j] == comp_of[(size_t)i] ? 1 : 2));
+            }
+            dlev[(size_t)i] = l;
+            maxl = std::max(maxl, l);
+        }
+        nlevels = maxl + 1;
+        if ((int64_t)nlevels * 2 > (int64_t)nlevels_in * 3) return FVG_OK;
+        std::vector<int64_t> pairs((size_t)n);
+        for (int64_t i = 0; i < n; ++i) pairs[(size_t)i] = (int64_t)comp_of[(size_t)i] * nlevels + dlev[(size_t)i];
+        std::sort(pairs.begin(), pairs.end());
+        const int64_t busy = (int64_t)(std::unique(pairs.begin(), pairs.end()) - pairs.begin());
+        if (busy < 8 * (int64_t)nlevels) return FVG_OK;
+    }
+    const std::vector<int32_t>& lev = contiguous ? dlev : lev_in;
     // Packing.  A block's rows of one level are processed together and the fullest
     // block-level sets the pace of its level, while every value a block reads from another
     // block costs a device-scope poll.  So the components, in order of their first row, go
@@ -816,7 +855,10 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
     const size_t C = comp_size.size();
     int32_t nb = (int32_t)std::min<size_t>((size_t)target, C);
     std::vector<int32_t> blk_of_comp(C);
-    {
+    if (contiguous) {
+        for (size_t q = 0; q < C; ++q) blk_of_comp[q] = (int32_t)q;
+        nb = (int32_t)C;
+    } else {
         // rows per (component, level), and the components joined to each by a dependency
         // two or more levels apart
         std::vector<int64_t> cl_key;
@@ -1000,6 +1042,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
     std::sort(fill.begin(), fill.end());
     fill.erase(std::unique(fill.begin(), fill.end()), fill.end());
     sw.h_brow = order;
+    sw.bL = nlevels;
     sw.nblk = nb;
     sw.bX = X;
     sw.bK = kmax;
@@ -1301,7 +1344,7 @@ static const void* blocked_kernel(int K) {
 
 static size_t blocked_lds(const Sweep& sw) {
     return (size_t)(kBlockRing * sw.bW + 2 * sw.bX) * sizeof(double) +
-           (size_t)(2 * (sw.nlevels + 1) + sw.bF) * sizeof(int32_t);
+           (size_t)(2 * (sw.bL + 1) + sw.bF) * sizeof(int32_t);
 }
 
 // whether every block of the blocked layout can be resident at once (decided on first use)
@@ -1334,7 +1377,7 @@ static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, cons
     const int tok = timing_begin(c);
     hipLaunchKernelGGL(k_sweep_prepare<double>, dim3(grid_for(sw.n + sw.nfill)), dim3(kBlock), 0, c->stream, sw.n,
                        from ? from : sw.brow, b, sw.bpos, sw.nfill, sw.bfill, sw.xpos, skip);
-    int nlev = sw.nlevels, W = sw.bW, X = sw.bX, spin = spin_limit(c);
+    int nlev = sw.bL, W = sw.bW, X = sw.bX, spin = spin_limit(c);
     int64_t n = sw.n;
     const void* fn = scale ? blocked_kernel<true>(sw.bK) : blocked_kernel<false>(sw.bK);
     void* args[] = {&nlev, &n, &W, &X, (void*)&sw.brange, (void*)&sw.blvl, (void*)&sw.bxptr, (void*)&sw.bxidx,

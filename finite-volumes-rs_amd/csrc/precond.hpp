@@ -30,6 +30,7 @@ struct Sweep {
     // every dependency between blocks spans at least two levels.  Positions run block by
     // block, level by level; a block's rows of one level are consecutive.
     int32_t nblk = 0;
+    int32_t bL = 0;                     // levels of the layout (more than nlevels when delayed)
     int32_t bW = 0;                     // most rows of one block in one level
     int32_t bX = 0;                     // most values one block fetches from outside for one level
     int32_t bK = 0;                     // most dependencies of one row

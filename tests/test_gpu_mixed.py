@@ -109,13 +109,16 @@ def test_sweep_launch_shapes_give_identical_bits(ctx, mode, waves, poll):
 
 
 @pytest.mark.parametrize("name,kind,prec,blocked", [("tet10", "sym", "ic", True), ("tet10", "nonsym", "ilu", True),
+                                                    ("prism24", "nonsym", "ilu", True), ("prism24", "sym", "ic", True),
                                                     ("tri64", "sym", "ic", False)])
 def test_blocked_sweep_is_built_and_exact_after_refactor(ctx, name, kind, prec, blocked):
-    """The default sweep mode runs the blocked sweep on both factors of the tet mesh; on
-    the triangle mesh every dependency is one level deep, the whole mesh is one
-    component and the persistent sweep serves.  Either way the results are the oracle's
-    bits, also after a refactorisation with new values (the blocked layout's
-    coefficients are gathered again by the factor kernel)."""
+    """The default sweep mode runs the blocked sweep on both factors of the tet mesh
+    (blocks of dependency-one-level-apart components) and of the prism mesh (one
+    component: blocks of consecutive rows with delayed levels).  On the triangle mesh
+    consecutive blocks would run one after another, so no layout is built and the
+    persistent sweep serves.  Either way the results are the oracle's bits, also after a
+    refactorisation with new values (the blocked layout's coefficients are gathered
+    again by the factor kernel)."""
     rp, ci, va, b = _oracle_system(name, kind)
     n = rp.shape[0] - 1
     A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)

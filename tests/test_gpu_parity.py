@@ -22,6 +22,7 @@ MESHES = {
     "tet4": lambda: meshgen.tet3d(4, seed=5),
     "tet10": lambda: meshgen.tet3d(10, seed=7),
     "prism8": lambda: meshgen.prism3d(8, layers=4, seed=2),
+    "prism24": lambda: meshgen.prism3d(24, layers=12, seed=2),
 }
 
 
