@@ -418,14 +418,15 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
 
 // ---- the blocked sweep ----------------------------------------------------------------
 // One workgroup per block of the blocked layout, all co-resident (cooperative launch).
-// A block walks its levels in order with one barrier per level.  Because the rows of a
-// block are whole gap-1 components, a row's dependencies are either values its own
+// A block walks its levels in order with one barrier per level.  The layout guarantees
+// that a dependency between blocks spans at least two levels (blocks of whole gap-1
+// components, or delayed levels), so a row's dependencies are either values its own
 // block made one or two levels before (kept in an LDS ring of kBlockRing levels) or
 // values from other blocks (or older ones of its own) at least two levels before.  The
 // last wave of the workgroup is the loader: during level l it polls, with device-scope
 // loads, the outside values level l + 1 will read and puts them in LDS.  The other
 // waves only read LDS for their operands; their own loads are the static data of each
-// row (count, LDS slots, coefficients, right-hand side, scale: all in position order),
+// row (packed LDS slots, coefficients, right-hand side, scale: all in position order),
 // fetched kBlockAhead levels before use, so no load of theirs waits on a poll.  Solved
 // values are published in position order (xpos), so a level's stores are contiguous
 // and polls address positions; k_sweep_finish scatters them to their rows afterwards.
