@@ -478,7 +478,10 @@ struct BlockRow {
     T bv, sc;
 };
 
-template <class T, bool SCALE, int K>
+// SPIN: the wait limit is the spin_limit argument (a tuning override) instead of the
+// compile-time kSpinLimit; the default launch keeps the constant (a runtime limit in the
+// polling loop measured 12 us slower per sweep)
+template <class T, bool SCALE, int K, bool SPIN>
 __global__ void __launch_bounds__(1024)
     k_sweep_blocked(int nlevels, int64_t n, int W, int X, const int32_t* __restrict__ brange,
                     const int32_t* __restrict__ blvl, const int32_t* __restrict__ bxptr,
@@ -538,7 +541,7 @@ __global__ void __launch_bounds__(1024)
 #pragma unroll
                     for (int q = 0; q < kFetchBatch; ++q) pend |= w[q] == Bits<T>::pending;
                     if (!pend) break;
-                    if (spins > spin_limit ||
+                    if (spins > (SPIN ? spin_limit : kSpinLimit) ||
                         ((spins & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         break;
@@ -1339,8 +1342,12 @@ static int launch_sweep(fvg_ctx* c, const Sweep& sw, const T* val, const T* scal
 }
 
 template <bool SCALE>
-static const void* blocked_kernel(int K) {
-    return K <= 4 ? (const void*)k_sweep_blocked<double, SCALE, 4> : (const void*)k_sweep_blocked<double, SCALE, 8>;
+static const void* blocked_kernel(int K, bool spin) {
+    if (spin)
+        return K <= 4 ? (const void*)k_sweep_blocked<double, SCALE, 4, true>
+                      : (const void*)k_sweep_blocked<double, SCALE, 8, true>;
+    return K <= 4 ? (const void*)k_sweep_blocked<double, SCALE, 4, false>
+                  : (const void*)k_sweep_blocked<double, SCALE, 8, false>;
 }
 
 static size_t blocked_lds(const Sweep& sw) {
@@ -1355,8 +1362,9 @@ static bool blocked_fits(fvg_ctx* c, Sweep& sw) {
     const int threads = ((sw.bW + 63) / 64 + 1) * 64;
     const size_t lds = blocked_lds(sw);
     bool ok = threads <= 1024 && lds <= 160 * 1024;
-    for (int scaled = 0; ok && scaled < 2; ++scaled) {
-        const void* fn = scaled ? blocked_kernel<true>(sw.bK) : blocked_kernel<false>(sw.bK);
+    for (int v = 0; ok && v < 4; ++v) {
+        const bool scaled = v & 1, spin = v >> 1;
+        const void* fn = scaled ? blocked_kernel<true>(sw.bK, spin) : blocked_kernel<false>(sw.bK, spin);
         if (lds > 64 * 1024 &&
             hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
             ok = false;
@@ -1380,7 +1388,8 @@ static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, cons
                        from ? from : sw.brow, b, sw.bpos, sw.nfill, sw.bfill, sw.xpos, skip);
     int nlev = sw.bL, W = sw.bW, X = sw.bX, spin = spin_limit(c);
     int64_t n = sw.n;
-    const void* fn = scale ? blocked_kernel<true>(sw.bK) : blocked_kernel<false>(sw.bK);
+    const bool custom = c->tune_spin_limit > 0;
+    const void* fn = scale ? blocked_kernel<true>(sw.bK, custom) : blocked_kernel<false>(sw.bK, custom);
     void* args[] = {&nlev, &n, &W, &X, (void*)&sw.brange, (void*)&sw.blvl, (void*)&sw.bxptr, (void*)&sw.bxidx,
                     (void*)&sw.bdep, (void*)&sw.bval, (void*)&sw.bpos,
                     (void*)&sw.spos, (void*)&sw.xpos, (void*)&err, (void*)&skip, (void*)&spin};
