@@ -797,14 +797,13 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
     if (kmax > kBlockMaxDeps) return FVG_OK;
     kmax = kmax <= 4 ? 4 : kBlockMaxDeps;   // the kernel reads exactly 4 or 8 slots per row
     // components, ordered by their first row, packed into blocks
-    std::vector<int32_t> comp_of((size_t)n), comp_size, comp_first;
+    std::vector<int32_t> comp_of((size_t)n), comp_size;
     std::vector<int32_t> id_of_root((size_t)n, -1);
     for (int64_t i = 0; i < n; ++i) {
         const int32_t r = find((int32_t)i);
         if (id_of_root[(size_t)r] < 0) {
             id_of_root[(size_t)r] = (int32_t)comp_size.size();
             comp_size.push_back(0);
-            comp_first.push_back((int32_t)i);
         }
         comp_of[(size_t)i] = id_of_root[(size_t)r];
         comp_size[(size_t)comp_of[(size_t)i]]++;
