@@ -15,6 +15,8 @@ namespace fvg {
 
 namespace {
 
+constexpr int kGradUnroll = 8;   // listings up to this long take the unrolled path
+
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_gradients(int n, int nc, const int32_t* __restrict__ lptr,
                                                       const int32_t* __restrict__ lother,
@@ -25,7 +27,46 @@ __global__ void __launch_bounds__(kBlock) k_gradients(int n, int nc, const int32
                                                       const double* __restrict__ bce, double* __restrict__ out) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    const int s = lptr[c], e = lptr[c + 1];
+    const int s = lptr[c], e = lptr[c + 1], m = e - s;
+    if (m >= 1 && m <= kGradUnroll) {
+        // all of the listing's entries and coefficients in flight at once (indices clamped
+        // into the listing), then all the neighbour values; the products are added in
+        // listing order, as in the loop below
+        int32_t o[kGradUnroll];
+        double cf[kGradUnroll][D];
+#pragma unroll
+        for (int u = 0; u < kGradUnroll; ++u) {
+            const int k = s + min(u, m - 1);
+            o[u] = lother[k];
+#pragma unroll
+            for (int d = 0; d < D; ++d) cf[u][d] = coef[(int64_t)k * D + d];
+        }
+        for (int q = 0; q < nc; ++q) {
+            const double vc = values[(int64_t)c * nc + q];
+            double v[kGradUnroll];
+#pragma unroll
+            for (int u = 0; u < kGradUnroll; ++u) {
+                if (o[u] >= 0) {
+                    v[u] = values[(int64_t)o[u] * nc + q];
+                } else {   // boundary: value_j = bci * value_i + bce
+                    const int b = -o[u] - 1;
+                    v[u] = bci[b] * vc + bce[(int64_t)b * nc + q];
+                }
+            }
+            double gr[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) gr[d] = diag[(int64_t)c * D + d] * vc;
+#pragma unroll
+            for (int u = 0; u < kGradUnroll; ++u)
+                if (u < m) {
+#pragma unroll
+                    for (int d = 0; d < D; ++d) gr[d] += cf[u][d] * v[u];
+                }
+#pragma unroll
+            for (int d = 0; d < D; ++d) out[((int64_t)c * nc + q) * D + d] = gr[d];
+        }
+        return;
+    }
     for (int q = 0; q < nc; ++q) {
         const double vc = values[(int64_t)c * nc + q];
         double gr[D];
