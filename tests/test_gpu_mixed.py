@@ -204,3 +204,31 @@ def test_sweep_wait_time_out_is_reported_and_recovers(ctx, mode):
     assert timed_out > 0
     ic.precondition(z, r)
     assert np.array_equal(z.numpy(), ref)
+
+
+def test_blocked_sweeps_on_an_unsymmetric_pattern(ctx):
+    """ILU(0) on a pattern that is not symmetric (the tet mesh's matrix plus couplings of
+    every 5th row to the rows 1500 and 3000 before it, below the diagonal only): the lower
+    layout has 8 slots per row and the upper 4, their blocks differ, and the upper
+    sweep's set-up reads the lower sweep's values through the pairing index.  Oracle
+    bits."""
+    rp, ci, va, b = _oracle_system("tet10", "nonsym")
+    n = rp.shape[0] - 1
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n)).tolil()
+    for i in range(0, n, 5):
+        for off in (1500, 3000):
+            if i - off >= 0:
+                A[i, i - off] = -0.04
+    A = A.tocsr()
+    A.setdiag(A.diagonal() + 0.3)
+    A.sort_indices()
+    rp2, ci2, va2 = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+    M = fvg.DistributedMatrix.from_csr(ctx, rp2, ci2, va2)
+    P = fvg.IncompleteLowerUpper.from_matrix(M, 0)
+    lay = P.sweep_layout()
+    assert lay["lower_blocks"] > 0 and lay["upper_blocks"] > 0
+    assert lay["lower_slots"] == 8 and lay["upper_slots"] == 4
+    r = np.random.default_rng(9).normal(size=n)
+    z = fvg.DistributedVector(ctx, n)
+    P.precondition(z, fvg.DistributedVector.from_data(ctx, r))
+    assert np.array_equal(z.numpy(), O.OraclePrec("ilu", rp2, ci2, va2, 0).apply(r))
