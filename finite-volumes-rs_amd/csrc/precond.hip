@@ -1355,7 +1355,7 @@ static size_t blocked_lds(const Sweep& sw) {
 }
 
 // whether every block of the blocked layout can be resident at once (decided on first use)
-static bool blocked_fits(fvg_ctx* c, Sweep& sw) {
+static bool blocked_fits(fvg_ctx* c, const Sweep& sw) {
     if (sw.nblk == 0) return false;
     if (sw.bOK >= 0) return sw.bOK == 1;
     const int threads = ((sw.bW + 63) / 64 + 1) * 64;
@@ -1405,7 +1405,7 @@ static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, cons
 static int sweep(fvg_ctx* c, const Sweep& sw, const double* scale, const double* b, double* x, int32_t* err,
                  const int32_t* skip) {
     if (sw.n == 0) return FVG_OK;
-    if (c->tune_sweep_mode == 0 && blocked_fits(c, const_cast<Sweep&>(sw)))
+    if (c->tune_sweep_mode == 0 && blocked_fits(c, sw))
         return launch_blocked(c, sw, scale, b, x, err, skip);
     if (c->tune_sweep_mode != 1) {   // the persistent sweep (mode 2, or no blocked layout)
         // the timed span holds one whole triangular solve: the pending fill and the sweep
@@ -1583,8 +1583,8 @@ int check_sweep_status(fvg_ctx* c, const fvg_prec* p) {
 // and d_tmp is not written.
 static int apply_factor(fvg_ctx* c, const fvg_prec* p, const double* lo_scale, const double* r, double* z,
                         const int32_t* skip) {
-    Sweep& lo = const_cast<Sweep&>(p->lower);
-    Sweep& up = const_cast<Sweep&>(p->upper);
+    const Sweep& lo = p->lower;
+    const Sweep& up = p->upper;
     if (c->tune_sweep_mode == 0 && up.bfrom && blocked_fits(c, lo) && blocked_fits(c, up)) {
         FVG_TRY(launch_blocked(c, lo, lo_scale, r, p->d_tmp, p->d_flags + 3, skip, nullptr, false));
         return launch_blocked(c, up, p->d_inv_diag, lo.xpos, z, p->d_flags + 3, skip, up.bfrom, true);

@@ -35,7 +35,8 @@ struct Sweep {
     int32_t bX = 0;                     // most values one block fetches from outside for one level
     int32_t bK = 0;                     // most dependencies of one row
     int32_t bF = 0;                     // most fetched rows of one block (all levels)
-    int32_t bOK = -1;                   // blocked launch fits co-resident: -1 unknown, 0 no, 1 yes
+    mutable int32_t bOK = -1;           // blocked launch fits co-resident: -1 unknown, 0 no, 1 yes
+                                        // (decided on first use, cached)
     int32_t* brange = nullptr;          // per block: first level, one past the last (2 x nblk)
     int32_t* blvl = nullptr;            // nblk x (nlevels + 1): first position of each level
     int32_t* bxptr = nullptr;           // nblk x (nlevels + 1): offsets into bxidx
