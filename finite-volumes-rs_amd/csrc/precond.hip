@@ -882,30 +882,37 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
         int32_t widest = 0;
         for (int32_t v : cl_cnt) widest = std::max(widest, v);
         for (size_t q = 0; q < C; ++q) cl_ptr[q + 1] += cl_ptr[q];
-        std::vector<int64_t> pairs;
-        for (int64_t i = 0; i < n; ++i)
-            for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
-                const int32_t j = col[(size_t)k];
-                if (!take(i, j)) continue;
-                const int32_t a = comp_of[(size_t)i], b = comp_of[(size_t)j];
-                if (a != b) {
-                    pairs.push_back((int64_t)a * (int64_t)C + b);
-                    pairs.push_back((int64_t)b * (int64_t)C + a);
+        // component adjacency (each dependency between two components, both ways, with
+        // multiplicity), by a two-pass counting sort
+        std::vector<int64_t> nb_ptr(C + 1, 0);
+        auto each_link = [&](auto&& f) {
+            for (int64_t i = 0; i < n; ++i)
+                for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
+                    const int32_t j = col[(size_t)k];
+                    if (!take(i, j)) continue;
+                    const int32_t a = comp_of[(size_t)i], b = comp_of[(size_t)j];
+                    if (a != b) f(a, b);
                 }
-            }
-        std::sort(pairs.begin(), pairs.end());
-        std::vector<int32_t> nb_ptr(C + 1, 0), nb_of(pairs.size());
-        for (size_t k = 0; k < pairs.size(); ++k) {
-            nb_ptr[(size_t)(pairs[k] / (int64_t)C) + 1]++;
-            nb_of[k] = (int32_t)(pairs[k] % (int64_t)C);
-        }
+        };
+        each_link([&](int32_t a, int32_t b) {
+            nb_ptr[(size_t)a + 1]++;
+            nb_ptr[(size_t)b + 1]++;
+        });
         for (size_t q = 0; q < C; ++q) nb_ptr[q + 1] += nb_ptr[q];
+        std::vector<int32_t> nb_of((size_t)nb_ptr[C]);
+        {
+            std::vector<int64_t> next(nb_ptr.begin(), nb_ptr.end() - 1);
+            each_link([&](int32_t a, int32_t b) {
+                nb_of[(size_t)next[(size_t)a]++] = b;
+                nb_of[(size_t)next[(size_t)b]++] = a;
+            });
+        }
         const int32_t cap = (widest + 63) / 64 * 64;
         std::vector<int32_t> load((size_t)nb * nlevels, 0), aff((size_t)nb, 0), peak((size_t)nb);
         std::fill(blk_of_comp.begin(), blk_of_comp.end(), -1);
         int32_t used = 0;
         for (size_t q = 0; q < C; ++q) {
-            for (int32_t k = nb_ptr[q]; k < nb_ptr[q + 1]; ++k) {
+            for (int64_t k = nb_ptr[q]; k < nb_ptr[q + 1]; ++k) {
                 const int32_t o = blk_of_comp[(size_t)nb_of[(size_t)k]];
                 if (o >= 0) aff[(size_t)o]++;
             }
@@ -930,7 +937,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
             used = std::max(used, best + 1);
             for (int32_t k = cl_ptr[q]; k < cl_ptr[q + 1]; ++k)
                 load[(size_t)best * nlevels + cl_lev[(size_t)k]] += cl_cnt[(size_t)k];
-            for (int32_t k = nb_ptr[q]; k < nb_ptr[q + 1]; ++k) {
+            for (int64_t k = nb_ptr[q]; k < nb_ptr[q + 1]; ++k) {
                 const int32_t o = blk_of_comp[(size_t)nb_of[(size_t)k]];
                 if (o >= 0) aff[(size_t)o] = 0;
             }
