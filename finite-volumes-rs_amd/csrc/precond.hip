@@ -763,6 +763,126 @@ static void free_sweep(fvg_ctx* c, Sweep& s) {
 // level: the persistent sweep then serves.
 constexpr int kBlockMaxDeps = 8;
 constexpr int kBlockMaxRows = 960;   // 15 compute waves + the loader wave
+// Rows packed by component into at most `target` blocks (see build_blocked): the block
+// of each component; returns the number of blocks used.  A block's rows of one level are
+// processed together and the fullest block-level sets the pace of its level, while every
+// value a block reads from another block costs a device-scope poll.  So the components,
+// in order of their first row, go greedily to the block that they share the most
+// dependencies with among those that stay within `cap` rows on every level (cap: the
+// widest component's rows of one level, rounded up to a whole wave), else to the block
+// whose fullest level grows least.
+static int32_t pack_components(int64_t n, const std::vector<int32_t>& rp, const std::vector<int32_t>& col,
+                               bool upper, const std::vector<int32_t>& comp_of, size_t C,
+                               const std::vector<int32_t>& lev, int nlevels, int target,
+                               std::vector<int32_t>& blk_of_comp) {
+    auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
+    const int32_t nb = (int32_t)std::min<size_t>((size_t)target, C);
+    // rows per (component, level), and the components joined to each by a dependency
+    // two or more levels apart
+    std::vector<int64_t> cl_key;
+    cl_key.reserve((size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+        cl_key.push_back((int64_t)comp_of[(size_t)i] * nlevels + lev[(size_t)i]);
+    std::sort(cl_key.begin(), cl_key.end());
+    std::vector<int32_t> cl_ptr(C + 1, 0), cl_lev, cl_cnt;
+    for (size_t k = 0; k < cl_key.size();) {
+        size_t e = k;
+        while (e < cl_key.size() && cl_key[e] == cl_key[k]) ++e;
+        const int32_t q = (int32_t)(cl_key[k] / nlevels);
+        cl_lev.push_back((int32_t)(cl_key[k] % nlevels));
+        cl_cnt.push_back((int32_t)(e - k));
+        cl_ptr[(size_t)q + 1]++;
+        k = e;
+    }
+    int32_t widest = 0;
+    for (int32_t v : cl_cnt) widest = std::max(widest, v);
+    for (size_t q = 0; q < C; ++q) cl_ptr[q + 1] += cl_ptr[q];
+    // component adjacency (each dependency between two components, both ways, with
+    // multiplicity), by a two-pass counting sort
+    std::vector<int64_t> nb_ptr(C + 1, 0);
+    auto each_link = [&](auto&& f) {
+        for (int64_t i = 0; i < n; ++i)
+            for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
+                const int32_t j = col[(size_t)k];
+                if (!take(i, j)) continue;
+                const int32_t a = comp_of[(size_t)i], b = comp_of[(size_t)j];
+                if (a != b) f(a, b);
+            }
+    };
+    each_link([&](int32_t a, int32_t b) {
+        nb_ptr[(size_t)a + 1]++;
+        nb_ptr[(size_t)b + 1]++;
+    });
+    for (size_t q = 0; q < C; ++q) nb_ptr[q + 1] += nb_ptr[q];
+    std::vector<int32_t> nb_of((size_t)nb_ptr[C]);
+    {
+        std::vector<int64_t> next(nb_ptr.begin(), nb_ptr.end() - 1);
+        each_link([&](int32_t a, int32_t b) {
+            nb_of[(size_t)next[(size_t)a]++] = b;
+            nb_of[(size_t)next[(size_t)b]++] = a;
+        });
+    }
+    const int32_t cap = (widest + 63) / 64 * 64;
+    std::vector<int32_t> load((size_t)nb * nlevels, 0), aff((size_t)nb, 0), peak((size_t)nb);
+    std::fill(blk_of_comp.begin(), blk_of_comp.end(), -1);
+    int32_t used = 0;
+    for (size_t q = 0; q < C; ++q) {
+        for (int64_t k = nb_ptr[q]; k < nb_ptr[q + 1]; ++k) {
+            const int32_t o = blk_of_comp[(size_t)nb_of[(size_t)k]];
+            if (o >= 0) aff[(size_t)o]++;
+        }
+        int32_t best = -1;
+        for (int32_t b = 0; b < nb; ++b) {
+            int32_t m = 0;
+            for (int32_t k = cl_ptr[q]; k < cl_ptr[q + 1]; ++k)
+                m = std::max(m, load[(size_t)b * nlevels + cl_lev[(size_t)k]] + cl_cnt[(size_t)k]);
+            peak[(size_t)b] = m;
+            if (best < 0) {
+                best = b;
+                continue;
+            }
+            const bool fb = peak[(size_t)best] <= cap, fx = m <= cap;
+            if (fx != fb ? fx
+                         : (fx ? (aff[(size_t)b] > aff[(size_t)best] ||
+                                  (aff[(size_t)b] == aff[(size_t)best] && m < peak[(size_t)best]))
+                               : m < peak[(size_t)best]))
+                best = b;
+        }
+        blk_of_comp[q] = best;
+        used = std::max(used, best + 1);
+        for (int32_t k = cl_ptr[q]; k < cl_ptr[q + 1]; ++k)
+            load[(size_t)best * nlevels + cl_lev[(size_t)k]] += cl_cnt[(size_t)k];
+        for (int64_t k = nb_ptr[q]; k < nb_ptr[q + 1]; ++k) {
+            const int32_t o = blk_of_comp[(size_t)nb_of[(size_t)k]];
+            if (o >= 0) aff[(size_t)o] = 0;
+        }
+    }
+    // blocks are filled in increasing order, so the used ones are 0 .. used - 1
+    return used;
+}
+
+// Delayed levels of a sweep whose rows are grouped by comp_of: a row's level is one past
+// its deepest dependency in its own group and two past one in another group; returns
+// the number of levels.
+static int delayed_levels(int64_t n, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, bool upper,
+                          const std::vector<int32_t>& comp_of, std::vector<int32_t>& dlev) {
+    auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
+    dlev.assign((size_t)n, 0);
+    int maxl = 0;
+    for (int64_t t = 0; t < n; ++t) {
+        const int64_t i = upper ? n - 1 - t : t;
+        int l = 0;
+        for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
+            const int32_t j = col[(size_t)k];
+            if (take(i, j))
+                l = std::max(l, dlev[(size_t)j] + (comp_of[(size_t)j] == comp_of[(size_t)i] ? 1 : 2));
+        }
+        dlev[(size_t)i] = l;
+        maxl = std::max(maxl, l);
+    }
+    return maxl + 1;
+}
+
 static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
                          bool upper, const std::vector<int32_t>* src_map, const std::vector<int32_t>& lev_in,
                          int nlevels_in, int target, Sweep& sw) {
@@ -827,20 +947,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
             comp_of[(size_t)i] = (int32_t)(i * nbc / n);
             comp_size[(size_t)comp_of[(size_t)i]]++;
         }
-        dlev.assign((size_t)n, 0);
-        int maxl = 0;
-        for (int64_t t = 0; t < n; ++t) {
-            const int64_t i = upper ? n - 1 - t : t;
-            int l = 0;
-            for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
-                const int32_t j = col[(size_t)k];
-                if (take(i, j))
-                    l = std::max(l, dlev[(size_t)j] + (comp_of[(size_t)j] == comp_of[(size_t)i] ? 1 : 2));
-            }
-            dlev[(size_t)i] = l;
-            maxl = std::max(maxl, l);
-        }
-        nlevels = maxl + 1;
+        nlevels = delayed_levels(n, rp, col, upper, comp_of, dlev);
         if ((int64_t)nlevels * 2 > (int64_t)nlevels_in * 3) return FVG_OK;
         std::vector<int64_t> pairs((size_t)n);
         for (int64_t i = 0; i < n; ++i) pairs[(size_t)i] = (int64_t)comp_of[(size_t)i] * nlevels + dlev[(size_t)i];
@@ -849,101 +956,15 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
         if (busy < 8 * (int64_t)nlevels) return FVG_OK;
     }
     const std::vector<int32_t>& lev = contiguous ? dlev : lev_in;
-    // Packing.  A block's rows of one level are processed together and the fullest
-    // block-level sets the pace of its level, while every value a block reads from another
-    // block costs a device-scope poll.  So the components, in order of their first row, go
-    // greedily to the block that they share the most dependencies with among those that
-    // stay within `cap` rows on every level (cap: the widest component's rows of one level,
-    // rounded up to a whole wave), else to the block whose fullest level grows least.
+    // blocks: the consecutive row ranges as they are, or the components packed
     const size_t C = comp_size.size();
-    int32_t nb = (int32_t)std::min<size_t>((size_t)target, C);
+    int32_t nb = 0;
     std::vector<int32_t> blk_of_comp(C);
     if (contiguous) {
         for (size_t q = 0; q < C; ++q) blk_of_comp[q] = (int32_t)q;
         nb = (int32_t)C;
     } else {
-        // rows per (component, level), and the components joined to each by a dependency
-        // two or more levels apart
-        std::vector<int64_t> cl_key;
-        cl_key.reserve((size_t)n);
-        for (int64_t i = 0; i < n; ++i)
-            cl_key.push_back((int64_t)comp_of[(size_t)i] * nlevels + lev[(size_t)i]);
-        std::sort(cl_key.begin(), cl_key.end());
-        std::vector<int32_t> cl_ptr(C + 1, 0), cl_lev, cl_cnt;
-        for (size_t k = 0; k < cl_key.size();) {
-            size_t e = k;
-            while (e < cl_key.size() && cl_key[e] == cl_key[k]) ++e;
-            const int32_t q = (int32_t)(cl_key[k] / nlevels);
-            cl_lev.push_back((int32_t)(cl_key[k] % nlevels));
-            cl_cnt.push_back((int32_t)(e - k));
-            cl_ptr[(size_t)q + 1]++;
-            k = e;
-        }
-        int32_t widest = 0;
-        for (int32_t v : cl_cnt) widest = std::max(widest, v);
-        for (size_t q = 0; q < C; ++q) cl_ptr[q + 1] += cl_ptr[q];
-        // component adjacency (each dependency between two components, both ways, with
-        // multiplicity), by a two-pass counting sort
-        std::vector<int64_t> nb_ptr(C + 1, 0);
-        auto each_link = [&](auto&& f) {
-            for (int64_t i = 0; i < n; ++i)
-                for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
-                    const int32_t j = col[(size_t)k];
-                    if (!take(i, j)) continue;
-                    const int32_t a = comp_of[(size_t)i], b = comp_of[(size_t)j];
-                    if (a != b) f(a, b);
-                }
-        };
-        each_link([&](int32_t a, int32_t b) {
-            nb_ptr[(size_t)a + 1]++;
-            nb_ptr[(size_t)b + 1]++;
-        });
-        for (size_t q = 0; q < C; ++q) nb_ptr[q + 1] += nb_ptr[q];
-        std::vector<int32_t> nb_of((size_t)nb_ptr[C]);
-        {
-            std::vector<int64_t> next(nb_ptr.begin(), nb_ptr.end() - 1);
-            each_link([&](int32_t a, int32_t b) {
-                nb_of[(size_t)next[(size_t)a]++] = b;
-                nb_of[(size_t)next[(size_t)b]++] = a;
-            });
-        }
-        const int32_t cap = (widest + 63) / 64 * 64;
-        std::vector<int32_t> load((size_t)nb * nlevels, 0), aff((size_t)nb, 0), peak((size_t)nb);
-        std::fill(blk_of_comp.begin(), blk_of_comp.end(), -1);
-        int32_t used = 0;
-        for (size_t q = 0; q < C; ++q) {
-            for (int64_t k = nb_ptr[q]; k < nb_ptr[q + 1]; ++k) {
-                const int32_t o = blk_of_comp[(size_t)nb_of[(size_t)k]];
-                if (o >= 0) aff[(size_t)o]++;
-            }
-            int32_t best = -1;
-            for (int32_t b = 0; b < nb; ++b) {
-                int32_t m = 0;
-                for (int32_t k = cl_ptr[q]; k < cl_ptr[q + 1]; ++k)
-                    m = std::max(m, load[(size_t)b * nlevels + cl_lev[(size_t)k]] + cl_cnt[(size_t)k]);
-                peak[(size_t)b] = m;
-                if (best < 0) {
-                    best = b;
-                    continue;
-                }
-                const bool fb = peak[(size_t)best] <= cap, fx = m <= cap;
-                if (fx != fb ? fx
-                             : (fx ? (aff[(size_t)b] > aff[(size_t)best] ||
-                                      (aff[(size_t)b] == aff[(size_t)best] && m < peak[(size_t)best]))
-                                   : m < peak[(size_t)best]))
-                    best = b;
-            }
-            blk_of_comp[q] = best;
-            used = std::max(used, best + 1);
-            for (int32_t k = cl_ptr[q]; k < cl_ptr[q + 1]; ++k)
-                load[(size_t)best * nlevels + cl_lev[(size_t)k]] += cl_cnt[(size_t)k];
-            for (int64_t k = nb_ptr[q]; k < nb_ptr[q + 1]; ++k) {
-                const int32_t o = blk_of_comp[(size_t)nb_of[(size_t)k]];
-                if (o >= 0) aff[(size_t)o] = 0;
-            }
-        }
-        // blocks are filled in increasing order, so the used ones are 0 .. used - 1
-        nb = used;
+        nb = pack_components(n, rp, col, upper, comp_of, C, lev, nlevels, target, blk_of_comp);
     }
     // positions: block by block, level by level, ascending rows
     std::vector<int32_t> order((size_t)n);
