@@ -78,7 +78,8 @@ enum {
      * tracers that cannot follow graph launches). */
     FVG_TUNE_GRAPHS = 6,
     /* About how many blocks the blocked sweep layout of a preconditioner created
-     * afterwards is cut into (0: one per compute unit; at most 4096). */
+     * afterwards is cut into (0: one per compute unit for packed components, two for
+     * blocks of consecutive rows; at most 4096). */
     FVG_TUNE_SWEEP_BLOCKS = 7,
     /* Polls a triangular-sweep wait makes before it gives up and the call returns
      * FVG_E_TIMEOUT (0: 2^18).  A tiny value forces the time-out path, for tests. */
