@@ -576,6 +576,11 @@ def main():
                           "total_ms": ks["sweep_ms"], "levels": [lo_lv, up_lv]},
         },
     }
+    # each leg below runs as the only live context on the device, as the headline did (the
+    # sweep launch mode follows the number of live contexts)
+    for obj in (ic, y_spmv, x, rhs, lhs, mesh):
+        obj.close()
+    ctx.close()
     for name, fn in (("batched", leg_batched), ("convdiff", leg_convdiff), ("transient", leg_transient)):
         if name in legs:
             print(f"bench: {name} leg", file=sys.stderr, flush=True)

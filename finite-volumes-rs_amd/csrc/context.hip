@@ -2,6 +2,7 @@
 #include <cmath>
 #include <algorithm>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 
 #include "internal.hpp"
@@ -152,6 +153,24 @@ static int collect_spans(fvg_ctx* ctx) {
     return FVG_OK;
 }
 
+// contexts alive per device in this process
+constexpr int kMaxDevices = 64;
+static std::atomic<int> g_live[kMaxDevices];
+
+hipError_t launch_coresident(fvg_ctx* ctx, const void* fn, unsigned groups, unsigned threads, void** args,
+                             size_t lds) {
+    // Workgroups that wait on one another deadlock only if some of them never become
+    // resident.  On a device where the only other kernels are ones that do not wait on
+    // this one (no other context of this process), the caller's sizing makes them all
+    // resident as soon as the preceding kernel of the stream drains, so the ordinary
+    // launch, without the cooperative launch's cross-queue hand-off, is enough.
+    bool coop = ctx->tune_sweep_launch == 2;
+    if (ctx->tune_sweep_launch == 0)
+        coop = ctx->device >= kMaxDevices || g_live[ctx->device].load(std::memory_order_relaxed) != 1;
+    return coop ? hipLaunchCooperativeKernel(fn, dim3(groups), dim3(threads), args, lds, ctx->stream)
+                : hipLaunchKernel(fn, dim3(groups), dim3(threads), args, lds, ctx->stream);
+}
+
 }  // namespace fvg
 
 using namespace fvg;
@@ -200,12 +219,15 @@ int fvg_ctx_create(int device, uint64_t budget, fvg_ctx** out) {
         fvg_ctx_destroy(c);
         return s;
     }
+    if (device < fvg::kMaxDevices) fvg::g_live[device].fetch_add(1);
+    c->counted = true;
     *out = c;
     return FVG_OK;
 }
 
 int fvg_ctx_destroy(fvg_ctx* c) {
     if (!c) return FVG_OK;
+    if (c->counted && c->device < fvg::kMaxDevices) fvg::g_live[c->device].fetch_sub(1);
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& s : c->spans) {
@@ -276,6 +298,10 @@ int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
         if (value > (1 << 30)) return fail(FVG_E_INVALID, "spin limit above 2^30");
         c->tune_spin_limit = value;
         return FVG_OK;
+    case FVG_TUNE_SWEEP_LAUNCH:
+        if (value > 2) return fail(FVG_E_INVALID, "sweep launch must be 0, 1 or 2");
+        c->tune_sweep_launch = value;
+        return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }
@@ -291,6 +317,7 @@ int fvg_ctx_get_tuning(const fvg_ctx* c, int32_t knob, int64_t* value) {
     case FVG_TUNE_GRAPHS: *value = c->tune_graphs; return FVG_OK;
     case FVG_TUNE_SWEEP_BLOCKS: *value = c->tune_sweep_blocks; return FVG_OK;
     case FVG_TUNE_SPIN_LIMIT: *value = c->tune_spin_limit; return FVG_OK;
+    case FVG_TUNE_SWEEP_LAUNCH: *value = c->tune_sweep_launch; return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }

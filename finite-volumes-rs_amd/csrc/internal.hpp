@@ -66,6 +66,8 @@ struct fvg_ctx {
     int64_t tune_graphs = 0;           // FVG_TUNE_GRAPHS
     int64_t tune_sweep_blocks = 0;     // FVG_TUNE_SWEEP_BLOCKS
     int64_t tune_spin_limit = 0;       // FVG_TUNE_SPIN_LIMIT
+    int64_t tune_sweep_launch = 0;     // FVG_TUNE_SWEEP_LAUNCH
+    bool counted = false;              // counted among the device's live contexts
     int num_cus = 0;
 };
 
@@ -127,5 +129,10 @@ int read_scalars(fvg_ctx* ctx, const double* dev, double* host, int count);
 enum { kSpanSpmv = 0, kSpanSweep = 1 };
 int timing_begin(fvg_ctx* ctx);
 void timing_end(fvg_ctx* ctx, int token, int kind, int64_t launches);
+
+// Launch of a kernel whose workgroups wait on one another and that the caller sized to
+// be resident together: ordinary or cooperative as FVG_TUNE_SWEEP_LAUNCH decides.
+hipError_t launch_coresident(fvg_ctx* ctx, const void* fn, unsigned groups, unsigned threads, void** args,
+                             size_t lds);
 
 }  // namespace fvg

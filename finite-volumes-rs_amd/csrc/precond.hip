@@ -1369,7 +1369,7 @@ static int launch_sweep(fvg_ctx* c, const Sweep& sw, const T* val, const T* scal
     void* args[] = {&nchunks, &nwaves, (void*)&sw.rows, (void*)&sw.ptr, (void*)&sw.col, (void*)&val, (void*)&scale,
                     (void*)&b, (void*)&x, (void*)&err, (void*)&skip, (void*)&spin};
     const void* fn = sweep_kernel<T>(c->tune_sweep_poll, scale != nullptr);
-    FVG_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)groups), dim3(kChunk * kWavesPerGroup), args, 0, c->stream));
+    FVG_HIP(launch_coresident(c, fn, (unsigned)groups, (unsigned)(kChunk * kWavesPerGroup), args, 0));
     return FVG_OK;
 }
 
@@ -1426,7 +1426,7 @@ static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, cons
                     (void*)&sw.bdep, (void*)&sw.bval, (void*)&sw.bpos,
                     (void*)&sw.spos, (void*)&sw.xpos, (void*)&err, (void*)&skip, (void*)&spin};
     const int threads = ((sw.bW + 63) / 64 + 1) * 64;
-    FVG_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)sw.nblk), dim3(threads), args, blocked_lds(sw), c->stream));
+    FVG_HIP(launch_coresident(c, fn, (unsigned)sw.nblk, (unsigned)threads, args, blocked_lds(sw)));
     if (scatter)
         hipLaunchKernelGGL(k_sweep_finish<double>, dim3(grid_for(sw.n)), dim3(kBlock), 0, c->stream, sw.n, sw.brow,
                            (const double*)sw.xpos, x, skip);

@@ -171,6 +171,41 @@ def test_blocked_sweep_with_eight_dependency_slots(ctx, prec, kind):
     assert np.array_equal(z.numpy(), O.OraclePrec(prec, rp2, ci2, va2, 0).apply(r))
 
 
+@pytest.mark.parametrize("sweep_mode", [0, 2])
+def test_sweep_launch_modes_give_the_same_bits(ctx, sweep_mode):
+    """The triangular sweeps whose workgroups wait on one another run from an ordinary
+    launch (1), a cooperative one (2) or the automatic choice (0: ordinary while one
+    context is alive on the device, which a second context here turns to cooperative).
+    Each gives the oracle's bits, for the blocked (sweep mode 0) and the persistent
+    (sweep mode 2) sweep; a launch mode outside 0-2 is rejected."""
+    rp, ci, va, b = _oracle_system("tet10", "sym")
+    n = rp.shape[0] - 1
+    want = O.OraclePrec("ic", rp, ci, va, 0).apply(b)
+    with pytest.raises(fvg.FvgError):
+        ctx.set_tuning(fvg.TUNE_SWEEP_LAUNCH, 3)
+    ctx.set_tuning(fvg.TUNE_SWEEP_MODE, sweep_mode)
+    try:
+        A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+        P = fvg.IncompleteCholesky.from_matrix(A, 0)
+        assert P.sweep_layout()["lower_blocks"] > 0
+        rhs = fvg.DistributedVector.from_data(ctx, b)
+        z = fvg.DistributedVector(ctx, n)
+        for launch in (1, 2, 0):
+            ctx.set_tuning(fvg.TUNE_SWEEP_LAUNCH, launch)
+            assert ctx.get_tuning(fvg.TUNE_SWEEP_LAUNCH) == launch
+            z.fill(0.0)
+            P.precondition(z, rhs)
+            assert np.array_equal(z.numpy(), want), launch
+        other = fvg.Context(0)
+        z.fill(0.0)
+        P.precondition(z, rhs)
+        assert np.array_equal(z.numpy(), want)
+        other.close()
+    finally:
+        ctx.set_tuning(fvg.TUNE_SWEEP_LAUNCH, 0)
+        ctx.set_tuning(fvg.TUNE_SWEEP_MODE, 0)
+
+
 @pytest.mark.parametrize("mode", [0, 2])
 def test_sweep_wait_time_out_is_reported_and_recovers(ctx, mode):
     """Every wait of a triangular sweep is bounded.  With the spin limit at its minimum
