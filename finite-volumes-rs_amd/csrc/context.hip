@@ -157,18 +157,29 @@ static int collect_spans(fvg_ctx* ctx) {
 constexpr int kMaxDevices = 64;
 static std::atomic<int> g_live[kMaxDevices];
 
+// Launch mode of a kernel whose workgroups wait on one another (1 ordinary, 2 cooperative).
+// Such workgroups deadlock only if some of them never become resident.  The caller sizes
+// the grid to fit the device, so once the stream's previous kernel drains they all become
+// resident, unless other kernels hold compute units for as long as the waits last: kernels
+// of another context of this process (which may themselves wait on one another), or a
+// batch of this context in flight on its own stream (a persistent kernel running up to
+// hundreds of milliseconds).  Only with neither is the ordinary launch, without the
+// cooperative launch's hand-off to another queue and back, chosen by default.  After a
+// wait of this context gave up, launches stay cooperative.  Kernels of other processes are
+// not visible here: a wait they delay past its bound ends the call with FVG_E_TIMEOUT,
+// after which this context launches cooperatively.
+int sweep_launch_rule(int64_t tune, int live_contexts, int batches_in_flight, bool timed_out_before) {
+    if (tune == 1 || tune == 2) return (int)tune;
+    return live_contexts == 1 && batches_in_flight == 0 && !timed_out_before ? 1 : 2;
+}
+
 hipError_t launch_coresident(fvg_ctx* ctx, const void* fn, unsigned groups, unsigned threads, void** args,
                              size_t lds) {
-    // Workgroups that wait on one another deadlock only if some of them never become
-    // resident.  On a device where the only other kernels are ones that do not wait on
-    // this one (no other context of this process), the caller's sizing makes them all
-    // resident as soon as the preceding kernel of the stream drains, so the ordinary
-    // launch, without the cooperative launch's cross-queue hand-off, is enough.
-    bool coop = ctx->tune_sweep_launch == 2;
-    if (ctx->tune_sweep_launch == 0)
-        coop = ctx->device >= kMaxDevices || g_live[ctx->device].load(std::memory_order_relaxed) != 1;
-    return coop ? hipLaunchCooperativeKernel(fn, dim3(groups), dim3(threads), args, lds, ctx->stream)
-                : hipLaunchKernel(fn, dim3(groups), dim3(threads), args, lds, ctx->stream);
+    const int live = ctx->device < kMaxDevices ? g_live[ctx->device].load(std::memory_order_relaxed) : 2;
+    const int mode = sweep_launch_rule(ctx->tune_sweep_launch, live, ctx->batches_in_flight, ctx->sweep_timed_out);
+    ctx->last_sweep_launch = mode;
+    return mode == 2 ? hipLaunchCooperativeKernel(fn, dim3(groups), dim3(threads), args, lds, ctx->stream)
+                     : hipLaunchKernel(fn, dim3(groups), dim3(threads), args, lds, ctx->stream);
 }
 
 }  // namespace fvg
@@ -302,6 +313,10 @@ int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
         if (value > 2) return fail(FVG_E_INVALID, "sweep launch must be 0, 1 or 2");
         c->tune_sweep_launch = value;
         return FVG_OK;
+    case FVG_TUNE_SWEEP_WITHHOLD:
+        if (value > 1) return fail(FVG_E_INVALID, "withhold must be 0 or 1");
+        c->tune_withhold = value;
+        return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }
@@ -318,8 +333,21 @@ int fvg_ctx_get_tuning(const fvg_ctx* c, int32_t knob, int64_t* value) {
     case FVG_TUNE_SWEEP_BLOCKS: *value = c->tune_sweep_blocks; return FVG_OK;
     case FVG_TUNE_SPIN_LIMIT: *value = c->tune_spin_limit; return FVG_OK;
     case FVG_TUNE_SWEEP_LAUNCH: *value = c->tune_sweep_launch; return FVG_OK;
+    case FVG_TUNE_SWEEP_WITHHOLD: *value = c->tune_withhold; return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
+}
+
+int fvg_ctx_sweep_launch_state(const fvg_ctx* c, int32_t* last_launch, int32_t* batches_in_flight) {
+    if (!c) return fail(FVG_E_INVALID, "null context");
+    if (last_launch) *last_launch = c->last_sweep_launch;
+    if (batches_in_flight) *batches_in_flight = c->batches_in_flight;
+    return FVG_OK;
+}
+
+int fvg_sweep_launch_rule(int64_t tune, int32_t live_contexts, int32_t batches_in_flight, int32_t timed_out_before) {
+    if (tune < 0 || tune > 2 || live_contexts < 0 || batches_in_flight < 0) return 0;
+    return fvg::sweep_launch_rule(tune, live_contexts, batches_in_flight, timed_out_before != 0);
 }
 
 int fvg_ctx_enable_timing(fvg_ctx* c, int enable) {

@@ -67,6 +67,10 @@ struct fvg_ctx {
     int64_t tune_sweep_blocks = 0;     // FVG_TUNE_SWEEP_BLOCKS
     int64_t tune_spin_limit = 0;       // FVG_TUNE_SPIN_LIMIT
     int64_t tune_sweep_launch = 0;     // FVG_TUNE_SWEEP_LAUNCH
+    int64_t tune_withhold = 0;         // FVG_TUNE_SWEEP_WITHHOLD
+    int batches_in_flight = 0;         // batches of this context launched and not yet finished
+    bool sweep_timed_out = false;      // a sweep wait gave up here: co-resident launches stay cooperative
+    int last_sweep_launch = 0;         // 1 ordinary, 2 cooperative (0: none yet)
     bool counted = false;              // counted among the device's live contexts
     int num_cus = 0;
 };
@@ -131,8 +135,12 @@ int timing_begin(fvg_ctx* ctx);
 void timing_end(fvg_ctx* ctx, int token, int kind, int64_t launches);
 
 // Launch of a kernel whose workgroups wait on one another and that the caller sized to
-// be resident together: ordinary or cooperative as FVG_TUNE_SWEEP_LAUNCH decides.
+// be resident together: ordinary or cooperative as sweep_launch_rule decides.
 hipError_t launch_coresident(fvg_ctx* ctx, const void* fn, unsigned groups, unsigned threads, void** args,
                              size_t lds);
+// 1 (ordinary) or 2 (cooperative) for FVG_TUNE_SWEEP_LAUNCH = tune, the live contexts of the
+// device in this process, this context's batches in flight and whether a wait of this
+// context gave up before
+int sweep_launch_rule(int64_t tune, int live_contexts, int batches_in_flight, bool timed_out_before);
 
 }  // namespace fvg

@@ -317,7 +317,8 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
     k_sweep_persistent(int64_t nchunks, int64_t nwaves, const int32_t* __restrict__ rows,
                        const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                        const T* __restrict__ val, const T* __restrict__ scale, const T* __restrict__ b, T* x,
-                       int32_t* __restrict__ err, const int32_t* __restrict__ skip, int spin_limit) {
+                       int32_t* __restrict__ err, const int32_t* __restrict__ skip, int spin_limit,
+                       int withhold) {
     using U = typename Bits<T>::U;
     if (skip && *skip) return;
     const int64_t w0 = (int64_t)blockIdx.x * kWavesPerGroup + (threadIdx.x >> 6);
@@ -412,7 +413,9 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
             for (int k = kb; k < ke; ++k) s -= val[k] * wait_value(x, col[k], err, spin_limit);
         }
         const T r = SCALE ? sc * s : s;
-        __hip_atomic_store(reinterpret_cast<U*>(x) + i, Bits<T>::bits(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!withhold)
+            __hip_atomic_store(reinterpret_cast<U*>(x) + i, Bits<T>::bits(r), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -479,8 +482,11 @@ struct BlockRow {
 };
 
 // SPIN: the wait limit is the spin_limit argument (a tuning override) instead of the
-// compile-time kSpinLimit; the default launch keeps the constant (a runtime limit in the
-// polling loop measured 12 us slower per sweep)
+// compile-time kSpinLimit, and `withhold` != 0 keeps every solved value out of xpos (the
+// block still uses its own values from LDS), so each fetch from outside waits until its
+// limit: the forced time-out of the tests.  The default launch keeps the constant limit
+// and always publishes (a runtime limit in the polling loop measured 12 us slower per
+// sweep).
 template <class T, bool SCALE, int K, bool SPIN>
 __global__ void __launch_bounds__(1024)
     k_sweep_blocked(int nlevels, int64_t n, int W, int X, const int32_t* __restrict__ brange,
@@ -488,7 +494,7 @@ __global__ void __launch_bounds__(1024)
                     const int32_t* __restrict__ bxidx, const uint16_t* __restrict__ bdep,
                     const T* __restrict__ bval, const T* __restrict__ bpos,
                     const T* __restrict__ spos, T* xpos, int32_t* __restrict__ err,
-                    const int32_t* __restrict__ skip, int spin_limit) {
+                    const int32_t* __restrict__ skip, int spin_limit, int withhold) {
     using U = typename Bits<T>::U;
     extern __shared__ __align__(16) unsigned char smem[];
     if (skip && *skip) return;
@@ -599,8 +605,9 @@ __global__ void __launch_bounds__(1024)
         }
         const T res = SCALE ? r.sc * s : s;
         ring[(l % kBlockRing) * W + ct] = res;
-        __hip_atomic_store(reinterpret_cast<U*>(xpos) + r.p, Bits<T>::bits(res), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        if (!SPIN || !withhold)
+            __hip_atomic_store(reinterpret_cast<U*>(xpos) + r.p, Bits<T>::bits(res), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     };
     BlockRow<T, K> r0, r1, r2, r3;
     prefetch(Lb, r0);
@@ -1365,9 +1372,9 @@ static int launch_sweep(fvg_ctx* c, const Sweep& sw, const T* val, const T* scal
     int64_t nwaves = sweep_waves(c, sw, max_coresident_groups<T>(c));
     nwaves = std::min(nwaves, (nchunks + kWavesPerGroup - 1) / kWavesPerGroup * kWavesPerGroup);
     const int64_t groups = nwaves / kWavesPerGroup;
-    int spin = spin_limit(c);
+    int spin = spin_limit(c), withhold = c->tune_withhold ? 1 : 0;
     void* args[] = {&nchunks, &nwaves, (void*)&sw.rows, (void*)&sw.ptr, (void*)&sw.col, (void*)&val, (void*)&scale,
-                    (void*)&b, (void*)&x, (void*)&err, (void*)&skip, (void*)&spin};
+                    (void*)&b, (void*)&x, (void*)&err, (void*)&skip, (void*)&spin, (void*)&withhold};
     const void* fn = sweep_kernel<T>(c->tune_sweep_poll, scale != nullptr);
     FVG_HIP(launch_coresident(c, fn, (unsigned)groups, (unsigned)(kChunk * kWavesPerGroup), args, 0));
     return FVG_OK;
@@ -1418,13 +1425,13 @@ static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, cons
     const int tok = timing_begin(c);
     hipLaunchKernelGGL(k_sweep_prepare<double>, dim3(grid_for(sw.n + sw.nfill)), dim3(kBlock), 0, c->stream, sw.n,
                        from ? from : sw.brow, b, sw.bpos, sw.nfill, sw.bfill, sw.xpos, skip);
-    int nlev = sw.bL, W = sw.bW, X = sw.bX, spin = spin_limit(c);
+    int nlev = sw.bL, W = sw.bW, X = sw.bX, spin = spin_limit(c), withhold = c->tune_withhold ? 1 : 0;
     int64_t n = sw.n;
-    const bool custom = c->tune_spin_limit > 0;
+    const bool custom = c->tune_spin_limit > 0 || withhold;
     const void* fn = scale ? blocked_kernel<true>(sw.bK, custom) : blocked_kernel<false>(sw.bK, custom);
     void* args[] = {&nlev, &n, &W, &X, (void*)&sw.brange, (void*)&sw.blvl, (void*)&sw.bxptr, (void*)&sw.bxidx,
                     (void*)&sw.bdep, (void*)&sw.bval, (void*)&sw.bpos,
-                    (void*)&sw.spos, (void*)&sw.xpos, (void*)&err, (void*)&skip, (void*)&spin};
+                    (void*)&sw.spos, (void*)&sw.xpos, (void*)&err, (void*)&skip, (void*)&spin, (void*)&withhold};
     const int threads = ((sw.bW + 63) / 64 + 1) * 64;
     FVG_HIP(launch_coresident(c, fn, (unsigned)sw.nblk, (unsigned)threads, args, blocked_lds(sw)));
     if (scatter)
@@ -1606,6 +1613,9 @@ int check_sweep_status(fvg_ctx* c, const fvg_prec* p) {
     if (fl[3]) {
         FVG_HIP(hipMemsetAsync(p->d_flags + 3, 0, sizeof(int32_t), c->stream));
         FVG_HIP(hipStreamSynchronize(c->stream));
+        // a time-out that no tuning override forced: later co-resident launches of this
+        // context are cooperative (some workgroup may have been kept from residency)
+        if (c->tune_spin_limit == 0 && c->tune_withhold == 0) c->sweep_timed_out = true;
         return fail(FVG_E_TIMEOUT, "a triangular-sweep wait exceeded its spin limit");
     }
     return FVG_OK;

@@ -1185,6 +1185,7 @@ struct fvg_batch {
     hipGraphExec_t exec = nullptr;
     int graph_nt = 0, graph_cap = 0;
     bool launched = false;
+    fvg_ctx* flight_ctx = nullptr;    // the context whose stream a launch followed (counts it in flight)
     // multi-workgroup solves of mixed-precision batches
     int cl_rows = 4096;                    // rows per workgroup (0: one workgroup per system)
     int cl_workers = 0;                    // worker pool (0: as many as can be co-resident)
@@ -1205,8 +1206,12 @@ namespace fvg {
 
 static void batch_free(fvg_ctx* c, fvg_batch* B) {
     if (B->stream) (void)hipStreamSynchronize(B->stream);
-    if (B->launched)
+    if (B->launched) {
         for (TeamPlan* t : B->plan) t->busy = false;
+        B->launched = false;
+        if (B->flight_ctx) B->flight_ctx->batches_in_flight -= 1;
+        B->flight_ctx = nullptr;
+    }
     if (B->exec) (void)hipGraphExecDestroy(B->exec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
     dev_free_n(c, B->hist, B->nhist);
@@ -1805,6 +1810,8 @@ int fvg_batch_launch(fvg_ctx* c, fvg_batch* B, const fvg_vec* const* b, fvg_vec*
     if (!B->exec || stale || B->graph_cap != B->capacity) FVG_TRY(batch_capture(c, B));
     FVG_HIP(hipGraphLaunch(B->exec, B->stream));
     B->launched = true;
+    B->flight_ctx = c;
+    c->batches_in_flight += 1;   // until finished: co-resident sweeps of c launch cooperatively
     for (TeamPlan* t : B->plan) t->busy = true;
     return FVG_OK;
 }
@@ -1813,6 +1820,8 @@ int fvg_batch_finish(fvg_ctx* c, fvg_batch* B, fvg_solver_info* infos, double* c
     if (!c || !B) return fail(FVG_E_INVALID, "null argument");
     if (!B->launched) return fail(FVG_E_INVALID, "batch not launched");
     B->launched = false;
+    if (B->flight_ctx) B->flight_ctx->batches_in_flight -= 1;
+    B->flight_ctx = nullptr;
     for (TeamPlan* t : B->plan) t->busy = false;
     FVG_HIP(hipStreamSynchronize(B->stream));
     if (infos) std::memcpy(infos, B->h_info, sizeof(fvg_solver_info) * (size_t)B->n_inst);

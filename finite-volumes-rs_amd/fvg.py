@@ -35,7 +35,7 @@ FNGRAD_ORTHOGONAL, FNGRAD_CORRECTED = 0, 1
 OP_ADD, OP_SUB, OP_NEG = 16, 17, 18
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 (TUNE_SWEEP_MODE, TUNE_SWEEP_WAVES, TUNE_SWEEP_POLL, TUNE_INNER_EXACT, TUNE_INNER_SWEEPS, TUNE_GRAPHS,
- TUNE_SWEEP_BLOCKS, TUNE_SPIN_LIMIT, TUNE_SWEEP_LAUNCH) = 1, 2, 3, 4, 5, 6, 7, 8, 9
+ TUNE_SWEEP_BLOCKS, TUNE_SPIN_LIMIT, TUNE_SWEEP_LAUNCH, TUNE_SWEEP_WITHHOLD) = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10
 
 
 class FvgError(RuntimeError):
@@ -114,6 +114,8 @@ def lib():
         "fvg_ctx_enable_timing": [P, C.c_int],
         "fvg_ctx_set_tuning": [P, C.c_int32, C.c_int64],
         "fvg_ctx_get_tuning": [P, C.c_int32, C.POINTER(C.c_int64)],
+        "fvg_ctx_sweep_launch_state": [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+        "fvg_sweep_launch_rule": [C.c_int64, C.c_int32, C.c_int32, C.c_int32],
         "fvg_ctx_kernel_stats": [P, C.POINTER(KernelStats), C.c_int],
         "fvg_vec_create": [P, C.c_int64, C.POINTER(P)],
         "fvg_vec_wrap": [P, C.c_int64, P, C.POINTER(P)],
@@ -208,6 +210,13 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(P)
 
 
+def sweep_launch_rule(tune, live_contexts, batches_in_flight, timed_out_before):
+    """Launch mode of the co-resident triangular sweeps (1 ordinary, 2 cooperative; 0 for
+    arguments out of range): the rule the library applies, evaluated on the host."""
+    return lib().fvg_sweep_launch_rule(int(tune), int(live_contexts), int(batches_in_flight),
+                                       int(bool(timed_out_before)))
+
+
 # ---------------------------------------------------------------------------------
 # context, vectors
 # ---------------------------------------------------------------------------------
@@ -243,6 +252,13 @@ class Context:
         v = C.c_int64()
         check(lib().fvg_ctx_get_tuning(self.h, int(knob), C.byref(v)))
         return v.value
+
+    def sweep_launch_state(self):
+        """(last co-resident sweep launch: 1 ordinary, 2 cooperative, 0 none yet; batches of
+        this context in flight)"""
+        mode, busy = C.c_int32(), C.c_int32()
+        check(lib().fvg_ctx_sweep_launch_state(self.h, C.byref(mode), C.byref(busy)))
+        return mode.value, busy.value
 
     def enable_timing(self, on=True):
         check(lib().fvg_ctx_enable_timing(self.h, 1 if on else 0))

@@ -87,15 +87,30 @@ enum {
     /* Launch of the triangular sweeps whose workgroups wait on one another (sized so
      * that all of them fit on the device at once): 1 an ordinary launch, 2 a cooperative
      * launch, which also guarantees that they are resident together while other streams
-     * run kernels that wait on each other; 0 (default) ordinary while this context is
-     * the only one alive on its device in the process, else cooperative.  The ordinary
-     * launch saves about 19 us of launch latency per sweep.  Should a wait still find a
-     * workgroup that never became resident, it gives up after the spin limit and the
-     * call returns FVG_E_TIMEOUT. */
-    FVG_TUNE_SWEEP_LAUNCH = 9
+     * run kernels that hold compute units; 0 (default) as fvg_sweep_launch_rule decides:
+     * ordinary while this context is the only one alive on its device in the process, no
+     * batch of it is in flight and no sweep wait of it has timed out, else cooperative.
+     * The ordinary launch saves about 19 us of launch latency per sweep.  Should a wait
+     * still find a workgroup that never became resident (kernels of another process can
+     * hold compute units), it gives up after the spin limit, the call returns
+     * FVG_E_TIMEOUT, and the context launches cooperatively from then on. */
+    FVG_TUNE_SWEEP_LAUNCH = 9,
+    /* Test-only: 1 makes the triangular sweeps keep every solved value from the other
+     * workgroups and waves that wait on it, so every such wait runs to its spin limit and
+     * the call returns FVG_E_TIMEOUT, while all kernels still exit normally. */
+    FVG_TUNE_SWEEP_WITHHOLD = 10
 };
 int fvg_ctx_set_tuning(fvg_ctx* ctx, int32_t knob, int64_t value);
 int fvg_ctx_get_tuning(const fvg_ctx* ctx, int32_t knob, int64_t* value);
+/* The launch mode the last co-resident sweep of this context used (1 ordinary,
+ * 2 cooperative, 0 none yet) and how many of its batches are launched and not yet
+ * finished (no counterpart in the reference). */
+int fvg_ctx_sweep_launch_state(const fvg_ctx* ctx, int32_t* last_launch, int32_t* batches_in_flight);
+/* The rule of FVG_TUNE_SWEEP_LAUNCH: the mode (1 ordinary, 2 cooperative) for the knob's
+ * value, the live contexts of the device in the process, the context's batches in flight
+ * and whether a sweep wait of the context timed out before; 0 for arguments out of range.
+ * Pure host function (no device needed; no counterpart in the reference). */
+int fvg_sweep_launch_rule(int64_t tune, int32_t live_contexts, int32_t batches_in_flight, int32_t timed_out_before);
 
 /* ---- dense vectors: DistributedVector<f64> (src/linalg/dvector.rs:10-179) ---- */
 typedef struct fvg_vec fvg_vec;

@@ -68,3 +68,19 @@ def test_term_program_compiles_like_the_reference_algebra():
     eq = fvg.convection(fvg.Upwind(flux), flux) - fvg.laplacian(fvg.Orthogonal(), k)
     ops, ff, cf = eq.compile()
     assert ops[0][:3] == (fvg.OP_CONVECTION_UPWIND, 0, 0) and len(ff) == 2
+
+
+def test_sweep_launch_rule():
+    """The co-resident sweeps launch ordinarily only while their context is the only one
+    alive on the device in the process, none of its batches is in flight and none of its
+    sweep waits has timed out; otherwise cooperatively.  Knob values 1 and 2 force the
+    mode; arguments out of range give 0."""
+    R = fvg.sweep_launch_rule
+    assert R(0, 1, 0, False) == 1
+    assert R(0, 2, 0, False) == 2          # another context of the process
+    assert R(0, 1, 1, False) == 2          # a batch of this context in flight
+    assert R(0, 1, 0, True) == 2           # a wait of this context timed out before
+    assert R(0, 3, 2, True) == 2
+    assert R(1, 4, 3, True) == 1 and R(2, 1, 0, False) == 2
+    assert R(3, 1, 0, False) == 0 and R(-1, 1, 0, False) == 0
+    assert R(0, -1, 0, False) == 0 and R(0, 1, -1, False) == 0

@@ -177,68 +177,129 @@ def test_sweep_launch_modes_give_the_same_bits(ctx, sweep_mode):
     launch (1), a cooperative one (2) or the automatic choice (0: ordinary while one
     context is alive on the device, which a second context here turns to cooperative).
     Each gives the oracle's bits, for the blocked (sweep mode 0) and the persistent
-    (sweep mode 2) sweep; a launch mode outside 0-2 is rejected."""
+    (sweep mode 2) sweep, and the mode each call used is read back; a launch mode
+    outside 0-2 is rejected."""
     rp, ci, va, b = _oracle_system("tet10", "sym")
     n = rp.shape[0] - 1
     want = O.OraclePrec("ic", rp, ci, va, 0).apply(b)
     with pytest.raises(fvg.FvgError):
         ctx.set_tuning(fvg.TUNE_SWEEP_LAUNCH, 3)
     ctx.set_tuning(fvg.TUNE_SWEEP_MODE, sweep_mode)
+    other = None
     try:
         A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
         P = fvg.IncompleteCholesky.from_matrix(A, 0)
         assert P.sweep_layout()["lower_blocks"] > 0
         rhs = fvg.DistributedVector.from_data(ctx, b)
         z = fvg.DistributedVector(ctx, n)
-        for launch in (1, 2, 0):
+        for launch, used in ((1, 1), (2, 2), (0, 1)):
             ctx.set_tuning(fvg.TUNE_SWEEP_LAUNCH, launch)
             assert ctx.get_tuning(fvg.TUNE_SWEEP_LAUNCH) == launch
             z.fill(0.0)
             P.precondition(z, rhs)
             assert np.array_equal(z.numpy(), want), launch
+            assert ctx.sweep_launch_state() == (used, 0), launch
         other = fvg.Context(0)
         z.fill(0.0)
         P.precondition(z, rhs)
         assert np.array_equal(z.numpy(), want)
-        other.close()
+        assert ctx.sweep_launch_state() == (2, 0)
     finally:
+        if other is not None:
+            other.close()
         ctx.set_tuning(fvg.TUNE_SWEEP_LAUNCH, 0)
         ctx.set_tuning(fvg.TUNE_SWEEP_MODE, 0)
 
 
 @pytest.mark.parametrize("mode", [0, 2])
 def test_sweep_wait_time_out_is_reported_and_recovers(ctx, mode):
-    """Every wait of a triangular sweep is bounded.  With the spin limit at its minimum
-    (a re-poll or two), a wait that finds its value still pending gives up, sets the error
-    flag that ends all other waits, and the kernel exits normally: the call returns
-    FVG_E_TIMEOUT.  With the limit restored, the next call gives the oracle's bits
-    again (blocked sweep: mode 0; persistent sweep: mode 2)."""
+    """Every wait of a triangular sweep is bounded.  With the sweeps withholding their
+    values from the workgroups and waves that wait on them (a test-only knob) and a small
+    spin limit, every such wait runs to its limit, sets the error flag that ends all
+    other waits, and the kernel exits normally: each call returns FVG_E_TIMEOUT.  With the
+    knobs cleared, the next call gives the oracle's bits again (blocked sweep: mode 0;
+    persistent sweep: mode 2)."""
     rp, ci, va, b = _oracle_system("tet10", "sym")
     n = rp.shape[0] - 1
     A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
     ic = fvg.IncompleteCholesky.from_matrix(A, 0)
+    lay = ic.sweep_layout()
+    if mode == 0:
+        # values cross blocks, so the blocked sweep has waits that can be starved
+        assert lay["lower_blocks"] > 1 and lay["lower_fetch_total"] > 0
+    assert lay["levels"] > 1
     r = fvg.DistributedVector.from_data(ctx, np.random.default_rng(2).normal(size=n))
     ref = O.OraclePrec("ic", rp, ci, va, 0).apply(r.numpy())
     z = fvg.DistributedVector(ctx, n)
-    timed_out = 0
+    statuses = []
     try:
         ctx.set_tuning(fvg.TUNE_SWEEP_MODE, mode)
-        ctx.set_tuning(fvg.TUNE_SPIN_LIMIT, 1)
-        # measured: about half of the blocked sweep's applies and all of the persistent
-        # sweep's time out at this limit; stop at the first
-        for _ in range(50):
+        ctx.set_tuning(fvg.TUNE_SPIN_LIMIT, 16)
+        ctx.set_tuning(fvg.TUNE_SWEEP_WITHHOLD, 1)
+        for _ in range(3):
             try:
                 ic.precondition(z, r)
+                statuses.append("FVG_OK")
             except fvg.FvgError as e:
-                assert fvg.STATUS[e.status] == "FVG_E_TIMEOUT"
-                timed_out += 1
-                break
+                statuses.append(fvg.STATUS[e.status])
     finally:
+        ctx.set_tuning(fvg.TUNE_SWEEP_WITHHOLD, 0)
         ctx.set_tuning(fvg.TUNE_SPIN_LIMIT, 0)
         ctx.set_tuning(fvg.TUNE_SWEEP_MODE, 0)
-    assert timed_out > 0
+    assert statuses == ["FVG_E_TIMEOUT"] * 3, statuses
     ic.precondition(z, r)
     assert np.array_equal(z.numpy(), ref)
+    # a forced time-out does not make the context's later launches cooperative
+    assert ctx.sweep_launch_state()[0] in (1, 2)
+
+
+def test_sweep_while_a_batch_is_in_flight(ctx):
+    """A batch of this context runs on a stream of its own (persistent kernels holding
+    compute units).  While it is in flight, the co-resident IC(0) sweep launches
+    cooperatively, and gives the oracle's bits; once the batch is finished, the
+    automatic choice is the ordinary launch again."""
+    import meshgen
+    rp, ci, va, b = _oracle_system("tet10", "sym")
+    n = rp.shape[0] - 1
+    want = O.OraclePrec("ic", rp, ci, va, 0).apply(b)
+    A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+    P = fvg.IncompleteCholesky.from_matrix(A, 0)
+    rhs = fvg.DistributedVector.from_data(ctx, b)
+    z = fvg.DistributedVector(ctx, n)
+    # a batch of a few large systems, so it is still running when the sweep is enqueued
+    mats, precs, bs, xs = [], [], [], []
+    for j in range(4):
+        raw = meshgen.tri2d(120, seed=50 + j)
+        om = O.OracleMesh(raw)
+        brp, bci, bva, bb = om.assemble(*_diffusion_program(om))
+        Aj = fvg.DistributedMatrix.from_csr(ctx, brp, bci, bva)
+        mats.append(Aj)
+        precs.append(fvg.IncompleteCholesky.from_matrix(Aj, 0))
+        bs.append(fvg.DistributedVector.from_data(ctx, bb))
+        xs.append(fvg.DistributedVector(ctx, brp.shape[0] - 1))
+    batch = fvg.Batch(ctx, mats, precs, "cg", fvg.LinearSolverOptions(1e-8, 0.0, 3000))
+    try:
+        batch.launch(bs, xs)
+        assert ctx.sweep_launch_state()[1] == 1
+        P.precondition(z, rhs)
+        assert ctx.sweep_launch_state() == (2, 1)
+        assert np.array_equal(z.numpy(), want)
+        infos = batch.finish()
+        assert all(i.final_residual <= 1e-8 * i.initial_residual for i in infos)
+        assert ctx.sweep_launch_state()[1] == 0
+        z.fill(0.0)
+        P.precondition(z, rhs)
+        assert ctx.sweep_launch_state() == (1, 0)
+        assert np.array_equal(z.numpy(), want)
+    finally:
+        batch.close()
+
+
+def _diffusion_program(om):
+    """-laplacian(Orthogonal, k = 1) - source(1) with u = 0 on the boundary."""
+    eq = -fvg.laplacian(fvg.Orthogonal(), np.ones(om.n_faces)) - fvg.source(np.ones(om.n_cells))
+    ops, ff, cf = eq.compile()
+    return ops, ff, cf, np.zeros(om.n_bnd), np.zeros(om.n_bnd)
 
 
 def test_blocked_sweeps_on_an_unsymmetric_pattern(ctx):
