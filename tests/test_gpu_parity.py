@@ -352,3 +352,35 @@ def test_error_paths(ctx):
     with pytest.raises(fvg.BudgetError):
         fvg.DistributedVector(small, 1 << 20)
     small.close()
+
+
+def test_bicgstab_restart_branch(ctx):
+    """rho collapses to 0 after the first iteration of A x = e1 with A = [[2,1,-1],[1,4,1],
+    [1,0,2]] and no preconditioner (exact in binary; the oracle's pin of the restart branch,
+    bi_conjugate_gradient_stab.rs:155-169): device-wide and batched FP64 BiCGStab restart
+    once, as the oracle does, and reach its solution."""
+    A = np.array([[2.0, 1.0, -1.0], [1.0, 4.0, 1.0], [1.0, 0.0, 2.0]])
+    rp = np.array([0, 3, 6, 8], np.int32)
+    ci = np.array([0, 1, 2, 0, 1, 2, 0, 2], np.int32)
+    va = np.array([2.0, 1.0, -1.0, 1.0, 4.0, 1.0, 1.0, 2.0])
+    b = np.array([1.0, 0.0, 0.0])
+    P = O.OraclePrec("identity", rp, ci, va, 0)
+    xo, io = O.solve("bicgstab", rp, ci, va, P, b, rtol=1e-12, atol=0.0, max_iter=50)
+    assert io["restarts"] == 1
+    M = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+    I = fvg.IdentityPreconditioner.new(ctx, 3)
+    B = fvg.DistributedVector.from_data(ctx, b)
+    opts = fvg.LinearSolverOptions(1e-12, 0.0, 50)
+    x = fvg.DistributedVector(ctx, 3)
+    info = fvg.bi_conjugate_gradient_stab(x, M, B, I, opts)
+    assert info.restarts == io["restarts"]
+    assert abs(info.iterations - io["iterations"]) <= 2
+    assert np.abs(x.numpy() - np.linalg.solve(A, b)).max() < 1e-12
+    batch = fvg.Batch(ctx, [M], [I], "bicgstab", opts)
+    try:
+        xb = fvg.DistributedVector(ctx, 3)
+        (bi,) = batch.solve([B], [xb])
+        assert bi.restarts == io["restarts"]
+        assert np.abs(xb.numpy() - np.linalg.solve(A, b)).max() < 1e-12
+    finally:
+        batch.close()

@@ -526,3 +526,118 @@ def test_linear_interpolation_weight_closed_form():
         assert np.isclose(S[i, i], w * 2.5 * A, rtol=1e-12)
         assert np.isclose(S[i, j], (1.0 - w) * 2.5 * A, rtol=1e-12)
         assert np.isclose(S[j, i], -w * 2.5 * A, rtol=1e-12)
+
+
+# ---- time(Euler), linear_source, divergence: terms the assembly tests above do not reach ----
+
+def _assemble_terms(m, prog, face_fields, cell_fields, bnde=None):
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    nb = om.n_bnd
+    be = np.zeros(nb) if bnde is None else bnde
+    rp, ci, va, b = om.assemble(prog, face_fields, cell_fields, np.ones(nb), be)
+    n = om.n_cells
+    return om, g, sp.csr_matrix((va, ci, rp), shape=(n, n)), b
+
+
+@pytest.mark.parametrize("mesh,total_volume", [("tri", 1.0), ("tet", 1.0 / 3.0)])
+def test_time_euler_is_mass_over_dt(mesh, total_volume):
+    """time(Euler) (schemes/time/euler.rs:27-37) as a cell term assembled as l*vol, r*vol
+    (assembly.rs:26-32): a diagonal matrix rho*vol/dt and rhs rho_prev*u_prev*vol/dt; no
+    face terms, so a boundary value changes nothing.  Summed over the cells the diagonal
+    times dt is the mesh volume: 1 on the unit square, 1/3 for tets (computed tet volume is
+    a third of the true one, compute.rs:359-361)."""
+    m = meshgen.tri2d(12, seed=3) if mesh == "tri" else meshgen.tet3d(4, seed=3)
+    n = m["cell_ptr"].shape[0] - 1
+    rng = np.random.default_rng(4)
+    prev, rho, rho0 = rng.normal(size=n), rng.uniform(0.5, 2.0, n), rng.uniform(0.5, 2.0, n)
+    dt = 0.0125
+    TE = O.OP["TIME_EULER"]
+    nb = int((m["face_patch"] >= 0).sum())
+    om, g, A, b = _assemble_terms(m, [(TE, 0, -1, -1, dt)], [], [prev], bnde=rng.normal(size=nb))
+    vol = g["cell_volume"]
+    assert A.nnz == n + 2 * int((g["neighbor"] >= 0).sum())
+    off = A - sp.diags(A.diagonal())
+    assert abs(off).max() == 0.0
+    assert np.allclose(A.diagonal(), vol / dt, rtol=4 * EPS, atol=0)
+    assert np.allclose(b, prev * vol / dt, rtol=4 * EPS, atol=0)
+    assert abs(A.diagonal().sum() * dt - total_volume) < 1e-12
+    # with density: rho on the diagonal, rho_prev on the right-hand side
+    om, g, A, b = _assemble_terms(m, [(TE, 0, 1, 2, dt)], [], [prev, rho, rho0])
+    assert np.allclose(A.diagonal(), rho * vol / dt, rtol=4 * EPS, atol=0)
+    assert np.allclose(b, prev * rho0 * vol / dt, rtol=4 * EPS, atol=0)
+    assert not np.allclose(b, prev * rho * vol / dt)   # the two densities are not swapped
+
+
+def test_linear_source_adds_c_vol_to_the_diagonal():
+    """linear_source(c) (source.rs:69-78) contributes (c, 0) per cell: diagonal c*vol, no
+    right-hand side.  Added to a Laplacian it changes only the diagonal, by exactly that
+    amount (term algebra, operations.rs:34-41); with c = 1 the diagonal sums to the area 1."""
+    m = meshgen.tri2d(10, seed=5)
+    n = m["cell_ptr"].shape[0] - 1
+    F = m["face_ptr"].shape[0] - 1
+    c = np.random.default_rng(6).uniform(0.1, 3.0, n)
+    LS, LAP, ADD = O.OP["LINEAR_SOURCE"], O.OP["LAPLACIAN_ORTHOGONAL"], O.OP["ADD"]
+    om, g, A, b = _assemble_terms(m, [(LS, 0, 0, 0, 0.0)], [], [c])
+    vol = g["cell_volume"]
+    assert abs(A - sp.diags(A.diagonal())).max() == 0.0 and not b.any()
+    assert np.allclose(A.diagonal(), c * vol, rtol=4 * EPS, atol=0)
+    om, g, A1, _ = _assemble_terms(m, [(LS, 0, 0, 0, 0.0)], [], [np.ones(n)])
+    assert abs(A1.diagonal().sum() - 1.0) < 1e-12
+    k = np.random.default_rng(7).uniform(0.5, 2.0, F)
+    _, _, L, bl = _assemble_terms(m, [(LAP, 0, 0, 0, 0.0)], [k], [])
+    _, _, LC, blc = _assemble_terms(m, [(LAP, 0, 0, 0, 0.0), (LS, 0, 0, 0, 0.0), (ADD, 0, 0, 0, 0.0)], [k], [c])
+    d = (LC - L).toarray()
+    assert np.allclose(np.diag(d), c * vol, rtol=1e-12, atol=1e-15)
+    assert np.abs(d - np.diag(np.diag(d))).max() == 0.0
+    assert np.array_equal(blc, bl)
+
+
+@pytest.mark.parametrize("mesh", ["tri", "tet"])
+def test_divergence_rhs_is_minus_the_net_outflow(mesh):
+    """divergence(flux) (divergence.rs:36-45) gives face terms (0, 0, -flux), times the face
+    area in assembly: no matrix entries, and rhs_c = -(net outflow of c) = -sum over c's
+    faces of flux_f*area_f, + for faces c owns and - for faces it neighbours.  For a
+    constant velocity u, flux_f = u . n_f, the outflow of every closed cell is zero
+    (divergence theorem; the 3D normals of triangles are all scaled by 1/3, which keeps
+    the sum zero on tets)."""
+    m = meshgen.tri2d(9, seed=8) if mesh == "tri" else meshgen.tet3d(4, seed=8)
+    F = m["face_ptr"].shape[0] - 1
+    DIV = O.OP["DIVERGENCE"]
+    flux = np.random.default_rng(9).normal(size=F)
+    om, g, A, b = _assemble_terms(m, [(DIV, 0, 0, 0, 0.0)], [flux], [])
+    n = om.n_cells
+    assert not A.data.any()
+    fa = flux * g["face_area"]
+    net = np.bincount(g["owner"], weights=fa, minlength=n)
+    inner = g["neighbor"] >= 0
+    net -= np.bincount(g["neighbor"][inner], weights=fa[inner], minlength=n)
+    assert np.allclose(b, -net, rtol=0, atol=1e-13)
+    assert abs(b.sum() + fa[~inner].sum()) < 1e-12   # interior faces cancel pairwise
+    u = np.array([0.7, -1.3, 0.4])[: om.dim]
+    flux_u = g["face_normal"] @ u
+    _, _, _, b0 = _assemble_terms(m, [(DIV, 0, 0, 0, 0.0)], [flux_u], [])
+    assert np.abs(b0).max() < 1e-13
+    assert np.abs(flux_u * g["face_area"]).max() > 1e-3
+
+
+def test_bicgstab_restart_when_rho_collapses():
+    """The restart branch (bi_conjugate_gradient_stab.rs:155-169).  A = [[2,1,-1],[1,4,1],
+    [1,0,2]], b = e1, x0 = 0, no preconditioner: alpha = 1/2 and s = (0, -1/2, -1/2), so
+    t = A s has t_0 = 0 and r_1 = s - omega t keeps r_1[0] = 0: rho_1 = r^.r_1 = 0 exactly,
+    every number so far exact in binary.  The solver restarts (rh = p = r_1) once and then
+    converges to A^-1 b.  Without the restart rho would stay 0 and the iteration stall."""
+    A = np.array([[2.0, 1.0, -1.0], [1.0, 4.0, 1.0], [1.0, 0.0, 2.0]])
+    S = sp.csr_matrix(A)
+    S.sort_indices()
+    rp, ci, va = S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.astype(np.float64)
+    b = np.array([1.0, 0.0, 0.0])
+    P = O.OraclePrec("identity", rp, ci, va, 0)
+    x, info = O.solve("bicgstab", rp, ci, va, P, b, rtol=1e-12, atol=0.0, max_iter=50)
+    assert info["restarts"] == 1
+    assert np.abs(x - np.linalg.solve(A, b)).max() < 1e-12
+    assert 2 <= info["iterations"] <= 5
+    # the same system with b = e2 never meets rho = 0: no restart
+    x2, info2 = O.solve("bicgstab", rp, ci, va, P, np.array([0.0, 1.0, 0.0]), rtol=1e-12, atol=0.0, max_iter=50)
+    assert info2["restarts"] == 0
+    assert np.abs(x2 - np.linalg.solve(A, [0.0, 1.0, 0.0])).max() < 1e-12
