@@ -109,14 +109,16 @@ int launch_spmv(fvg_ctx* ctx, const fvg_csr* a, const double* x, double* y, cons
     if (n <= 0) return FVG_OK;
     const int g = spmv_blocks(n);
     const bool has_epi = epi.b || epi.copy1 || epi.copy2 || epi.dotw || epi.dotw2 || epi.dot_self || epi.skip;
-    if (has_epi) FVG_TRY(ensure_partials(ctx, g));
+    if (has_epi && !epi.partials) FVG_TRY(ensure_partials(ctx, g));
+    double* part = epi.partials ? epi.partials : ctx->partials;
+    const int64_t pstride = epi.partials ? epi.pstride : ctx->partials_cap;
     const int tok = timing_begin(ctx);
     if (has_epi)
         hipLaunchKernelGGL(k_spmv<true>, dim3(g), dim3(kSpmvRows), 0, ctx->stream, (int)n, a->row_ptr, a->col, a->val,
-                           x, y, epi, ctx->partials, ctx->partials_cap);
+                           x, y, epi, part, pstride);
     else
         hipLaunchKernelGGL(k_spmv<false>, dim3(g), dim3(kSpmvRows), 0, ctx->stream, (int)n, a->row_ptr, a->col,
-                           a->val, x, y, epi, ctx->partials, ctx->partials_cap);
+                           a->val, x, y, epi, part, pstride);
     timing_end(ctx, tok, kSpanSpmv, 1);
     FVG_HIP(hipGetLastError());
     return FVG_OK;

@@ -343,6 +343,39 @@ struct Workspace {
     }
 };
 
+// zeroed states carrying the options, and the flags {members iterating, all done}
+__global__ void k_kstate_init(int n_inst, KState* __restrict__ st, double atol, double rtol, int64_t max_iter,
+                              int64_t hist_cap, int32_t* __restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_inst) {
+        KState s{};
+        s.atol = atol;
+        s.rtol = rtol;
+        s.max_iter = max_iter;
+        s.hist_cap = hist_cap;
+        st[i] = s;
+    }
+    if (i == 0) {
+        flags[0] = n_inst;
+        flags[1] = 0;
+    }
+}
+
+int KRun::init_at(fvg_ctx* ctx, int ninst, const fvg_solver_options* o, KState* st_, double* hist_,
+                  int32_t* flags_) {
+    c = ctx;
+    n_inst = ninst;
+    hist_cap = o->max_iterations + 2;
+    st = st_;
+    hist = hist_;
+    flags = flags_;
+    owned = false;
+    hipLaunchKernelGGL(k_kstate_init, dim3(grid_for(n_inst)), dim3(kBlock), 0, c->stream, n_inst, st,
+                       o->absolute_tolerance, o->relative_tolerance, (int64_t)o->max_iterations, hist_cap, flags);
+    FVG_HIP(hipGetLastError());
+    return FVG_OK;
+}
+
 int KRun::init(fvg_ctx* ctx, int ninst, const fvg_solver_options* o) {
     c = ctx;
     n_inst = ninst;
@@ -350,22 +383,15 @@ int KRun::init(fvg_ctx* ctx, int ninst, const fvg_solver_options* o) {
     FVG_TRY(dev_alloc_n(c, n_inst, &st));
     FVG_TRY(dev_alloc_n(c, hist_cap * n_inst, &hist));
     FVG_TRY(dev_alloc_n(c, 2, &flags));
-    std::vector<KState> h((size_t)n_inst);
-    for (auto& s : h) {
-        std::memset(&s, 0, sizeof(s));
-        s.atol = o->absolute_tolerance;
-        s.rtol = o->relative_tolerance;
-        s.max_iter = o->max_iterations;
-        s.hist_cap = hist_cap;
-    }
-    FVG_TRY(upload(c, st, h.data(), n_inst));
-    int32_t f[2] = {n_inst, 0};
-    FVG_TRY(upload(c, flags, f, 2));
+    owned = true;
+    hipLaunchKernelGGL(k_kstate_init, dim3(grid_for(n_inst)), dim3(kBlock), 0, c->stream, n_inst, st,
+                       o->absolute_tolerance, o->relative_tolerance, (int64_t)o->max_iterations, hist_cap, flags);
+    FVG_HIP(hipGetLastError());
     return FVG_OK;
 }
 
 KRun::~KRun() {
-    if (c) {
+    if (c && owned) {
         (void)hipStreamSynchronize(c->stream);
         dev_free_n(c, st, n_inst);
         dev_free_n(c, hist, hist_cap * n_inst);
@@ -383,10 +409,20 @@ int KRun::all_done(bool* out) {
 }
 
 int KRun::finish(fvg_solver_info* infos, double* const* histories, int64_t cap) {
-    std::vector<KState> h((size_t)n_inst);
-    FVG_TRY(download(c, h.data(), st, n_inst));
+    // one state through the context's pinned buffer (nothing allocated); several through a host array
+    std::vector<KState> h;
+    const KState* hs;
+    if (sizeof(KState) * (size_t)n_inst <= 4096) {
+        FVG_HIP(hipMemcpyAsync(c->pinned, st, sizeof(KState) * (size_t)n_inst, hipMemcpyDeviceToHost, c->stream));
+        FVG_HIP(hipStreamSynchronize(c->stream));
+        hs = static_cast<const KState*>(c->pinned);
+    } else {
+        h.resize((size_t)n_inst);
+        FVG_TRY(download(c, h.data(), st, n_inst));
+        hs = h.data();
+    }
     for (int i = 0; i < n_inst; ++i) {
-        const KState& s = h[(size_t)i];
+        const KState& s = hs[i];
         if (infos) {
             fvg_solver_info& info = infos[i];
             std::memset(&info, 0, sizeof(info));
@@ -412,19 +448,33 @@ int KRun::finish(fvg_solver_info* infos, double* const* histories, int64_t cap) 
 // iterations enqueued between two polls of the "all systems done" flag
 constexpr int kPollBatch = 4;
 
+// work vectors and partials: the caller's (kb), or allocated here and the context's
+static int solve_buffers(fvg_ctx* c, int64_t n, int count, const KBuffers* kb, Workspace& ws, double** v,
+                         double** part, int64_t* stride) {
+    if (kb) {
+        for (int k = 0; k < count; ++k) v[k] = kb->vec[k];
+        *part = kb->partials;
+        *stride = kb->pstride;
+        return FVG_OK;
+    }
+    for (int k = 0; k < count; ++k) FVG_TRY(ws.get(n, &v[k]));
+    FVG_TRY(ensure_partials(c, spmv_blocks(n)));
+    *part = c->partials;
+    *stride = c->partials_cap;
+    return FVG_OK;
+}
+
 int run_cg_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* b, double* x, const Layout& L,
-                 KRun& K) {
+                 KRun& K, const KBuffers* kb) {
     const int64_t n = a->nrows;
     Workspace ws{c};
-    double *r, *z, *pp, *ap;
-    FVG_TRY(ws.get(n, &r));
-    FVG_TRY(ws.get(n, &z));
-    FVG_TRY(ws.get(n, &pp));
-    FVG_TRY(ws.get(n, &ap));
+    double* v[4];
+    double* part;
+    int64_t stride;
+    FVG_TRY(solve_buffers(c, n, 4, kb, ws, v, &part, &stride));
+    double *r = v[0], *z = v[1], *pp = v[2], *ap = v[3];
     const int nb = spmv_blocks(n);
-    FVG_TRY(ensure_partials(c, nb));
-    const int64_t stride = c->partials_cap;
-    double* part = c->partials;
+    double* const epart = kb ? part : nullptr;   // the SpMV epilogues' partials (null: the context's)
     const int32_t* all_done = K.flags + 1;
     const int32_t sstride = (int32_t)(sizeof(KState) / sizeof(int32_t));
 
@@ -439,6 +489,8 @@ int run_cg_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* 
     e0.skip = &K.st[0].halt;
     e0.skip_stride = sstride;
     e0.blk_inst = L.blk_inst;
+    e0.partials = epart;
+    e0.pstride = stride;
     FVG_TRY(launch_spmv(c, a, x, r, e0));
     SCALAR(CG_INIT, part);
     // z = M^-1 r; p = z; r.z
@@ -453,6 +505,8 @@ int run_cg_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* 
         e1.skip = &K.st[0].halt;
         e1.skip_stride = sstride;
         e1.blk_inst = L.blk_inst;
+        e1.partials = epart;
+        e1.pstride = stride;
         FVG_TRY(launch_spmv(c, a, pp, ap, e1));
         SCALAR(CG_ALPHA, part);
         hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, pp, ap, x, r, part);
@@ -472,22 +526,16 @@ int run_cg_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* 
 }
 
 int run_bicgstab_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* b, double* x, const Layout& L,
-                       KRun& K) {
+                       KRun& K, const KBuffers* kb) {
     const int64_t n = a->nrows;
     Workspace ws{c};
-    double *r, *rh, *pp, *y, *v, *s, *z, *t;
-    FVG_TRY(ws.get(n, &r));
-    FVG_TRY(ws.get(n, &rh));
-    FVG_TRY(ws.get(n, &pp));
-    FVG_TRY(ws.get(n, &y));
-    FVG_TRY(ws.get(n, &v));
-    FVG_TRY(ws.get(n, &s));
-    FVG_TRY(ws.get(n, &z));
-    FVG_TRY(ws.get(n, &t));
+    double* w[8];
+    double* part;
+    int64_t stride;
+    FVG_TRY(solve_buffers(c, n, 8, kb, ws, w, &part, &stride));
+    double *r = w[0], *rh = w[1], *pp = w[2], *y = w[3], *v = w[4], *s = w[5], *z = w[6], *t = w[7];
     const int nb = spmv_blocks(n);
-    FVG_TRY(ensure_partials(c, nb));
-    const int64_t stride = c->partials_cap;
-    double* part = c->partials;
+    double* const epart = kb ? part : nullptr;   // the SpMV epilogues' partials (null: the context's)
     const int32_t* all_done = K.flags + 1;
     const int32_t sstride = (int32_t)(sizeof(KState) / sizeof(int32_t));
 
@@ -503,6 +551,8 @@ int run_bicgstab_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const do
     e0.skip = &K.st[0].halt;
     e0.skip_stride = sstride;
     e0.blk_inst = L.blk_inst;
+    e0.partials = epart;
+    e0.pstride = stride;
     FVG_TRY(launch_spmv(c, a, x, r, e0));
     SCALAR(BI_INIT, part);
     FVG_HIP(hipGetLastError());
@@ -514,6 +564,8 @@ int run_bicgstab_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const do
         e1.skip = &K.st[0].halt;
         e1.skip_stride = sstride;
         e1.blk_inst = L.blk_inst;
+        e1.partials = epart;
+        e1.pstride = stride;
         FVG_TRY(launch_spmv(c, a, y, v, e1));
         SCALAR(BI_ALPHA, part);
         hipLaunchKernelGGL(k_bi_hs, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, y, v, r, x, s, part);
@@ -525,6 +577,8 @@ int run_bicgstab_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const do
         e2.skip = &K.st[0].done;
         e2.skip_stride = sstride;
         e2.blk_inst = L.blk_inst;
+        e2.partials = epart;
+        e2.pstride = stride;
         FVG_TRY(launch_spmv(c, a, z, t, e2));
         SCALAR(BI_OMEGA, part);
         hipLaunchKernelGGL(k_bi_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, L.blk_inst, z, t, s, rh, x, r, part,
@@ -541,7 +595,7 @@ int run_bicgstab_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const do
     return check_sweep_status(c, p);
 }
 
-static int check_solve_args(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, const fvg_vec* x,
+int check_solve_args(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, const fvg_vec* x,
                             const fvg_solver_options* o) {
     if (!c || !a || !p || !b || !x || !o) return fail(FVG_E_INVALID, "null argument");
     if (a->nrows != a->ncols) return fail(FVG_E_INVALID, "solver needs a square matrix");

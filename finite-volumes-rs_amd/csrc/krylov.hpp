@@ -35,15 +35,31 @@ struct KRun {
     double* hist = nullptr;
     int64_t hist_cap = 0;
     int32_t* flags = nullptr;   // [0] members still iterating, [1] all done
+    bool owned = true;          // st, hist and flags were allocated by init()
     int init(fvg_ctx* ctx, int ninst, const fvg_solver_options* o);
+    // the same state in caller-provided device memory (nothing allocated, nothing freed):
+    // ninst states, ninst * (max_iterations + 2) history slots, 2 flags
+    int init_at(fvg_ctx* ctx, int ninst, const fvg_solver_options* o, KState* st, double* hist, int32_t* flags);
     int all_done(bool* out);
     int finish(fvg_solver_info* infos, double* const* histories, int64_t cap);
     ~KRun();
 };
 
+// Work vectors and partial sums of one solve, when the caller provides them (no
+// allocation): vec[0..3] for CG, vec[0..7] for BiCGStab, each of n doubles; partials
+// holds 4 slots of `pstride` >= spmv_blocks(n) doubles.
+struct KBuffers {
+    double* vec[8] = {};
+    double* partials = nullptr;
+    int64_t pstride = 0;
+};
+
 int run_cg_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* b, double* x, const Layout& L,
-                 KRun& K);
+                 KRun& K, const KBuffers* kb = nullptr);
 int run_bicgstab_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* b, double* x, const Layout& L,
-                       KRun& K);
+                       KRun& K, const KBuffers* kb = nullptr);
+// checks of the solver entry points' arguments
+int check_solve_args(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, const fvg_vec* x,
+                     const fvg_solver_options* o);
 
 }  // namespace fvg

@@ -23,6 +23,10 @@ struct SpmvEpilogue {
     const int32_t* skip = nullptr;
     int32_t skip_stride = 0;
     const int32_t* blk_inst = nullptr;
+    // partial sums into partials[slot * pstride + block] instead of the context's
+    // (caller-provided storage of at least 3 slots of spmv_blocks(n); nothing is allocated)
+    double* partials = nullptr;
+    int64_t pstride = 0;
 };
 
 // Number of partial sums one fused SpMV writes per dot (one per workgroup).

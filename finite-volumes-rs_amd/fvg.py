@@ -114,6 +114,7 @@ def lib():
         "fvg_ctx_enable_timing": [P, C.c_int],
         "fvg_ctx_set_tuning": [P, C.c_int32, C.c_int64],
         "fvg_ctx_get_tuning": [P, C.c_int32, C.POINTER(C.c_int64)],
+        "fvg_noalloc_scratch_bytes": [C.POINTER(C.c_uint64)],
         "fvg_ctx_sweep_launch_state": [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
         "fvg_sweep_launch_rule": [C.c_int64, C.c_int32, C.c_int32, C.c_int32],
         "fvg_ctx_kernel_stats": [P, C.POINTER(KernelStats), C.c_int],
@@ -175,6 +176,8 @@ def lib():
         "fvg_prec_nonpositive_pivots": [P, P, C.POINTER(C.c_int64)],
         "fvg_cg": [P, P, P, P, P, C.POINTER(SolverOptions), C.POINTER(SolverInfo), P, C.c_int64],
         "fvg_bicgstab": [P, P, P, P, P, C.POINTER(SolverOptions), C.POINTER(SolverInfo), P, C.c_int64],
+        "fvg_cg_noalloc": [P, P, P, P, P, C.POINTER(SolverOptions), C.POINTER(SolverInfo), P, C.c_int64],
+        "fvg_bicgstab_noalloc": [P, P, P, P, P, C.POINTER(SolverOptions), C.POINTER(SolverInfo), P, C.c_int64],
         "fvg_batch_create": [P, C.c_int32, P, P, C.c_int32, C.POINTER(SolverOptions), C.POINTER(P)],
         "fvg_batch_destroy": [P, P],
         "fvg_batch_rebind": [P, P, C.c_int32, P, P],
@@ -1089,6 +1092,26 @@ def conjugate_gradient(solution, lhs, rhs, preconditioner, options=None):
     """conjugate_gradient (src/linalg/solvers/conjugate_gradient.rs:12-112)."""
     return _solve(lib().fvg_cg, "ConjugateGradient", solution, lhs, rhs, preconditioner,
                   options or LinearSolverOptions())
+
+
+def conjugate_gradient_noalloc(solution, lhs, rhs, preconditioner, options=None):
+    """conjugate_gradient in FP64 with every device buffer in the library's static scratch
+    array (fvg_cg_noalloc): nothing is allocated."""
+    return _solve(lib().fvg_cg_noalloc, "ConjugateGradient", solution, lhs, rhs, preconditioner,
+                  options or LinearSolverOptions())
+
+
+def bi_conjugate_gradient_stab_noalloc(solution, lhs, rhs, preconditioner, options=None):
+    """bi_conjugate_gradient_stab in FP64 with every device buffer in the library's static
+    scratch array (fvg_bicgstab_noalloc): nothing is allocated."""
+    return _solve(lib().fvg_bicgstab_noalloc, "BiConjugateGradientStab", solution, lhs, rhs, preconditioner,
+                  options or LinearSolverOptions())
+
+
+def noalloc_scratch_bytes():
+    v = C.c_uint64()
+    check(lib().fvg_noalloc_scratch_bytes(C.byref(v)))
+    return v.value
 
 
 def bi_conjugate_gradient_stab(solution, lhs, rhs, preconditioner, options=None):

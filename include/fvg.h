@@ -409,6 +409,24 @@ int fvg_bicgstab(fvg_ctx* ctx, const fvg_csr* a, const fvg_prec* p, const fvg_ve
                  const fvg_solver_options* opts, fvg_solver_info* info, double* history,
                  int64_t history_capacity);
 
+/* FP64 fvg_cg / fvg_bicgstab that allocate nothing: no counterpart in the reference,
+ * for callers that cannot allocate device memory (code under stream capture, tight
+ * serving loops).  Same arguments, kernels, iterations and results as fvg_cg /
+ * fvg_bicgstab with precision FVG_PRECISION_FP64 (other precisions give
+ * FVG_E_UNSUPPORTED).  Work vectors, partial sums and solver state live in one static
+ * device array of fvg_noalloc_scratch_bytes() bytes (160 MiB), which each device the
+ * library runs them on holds from the first call there until the process ends: a CG on
+ * up to about 5.2M rows, a BiCGStab on up to about 2.6M rows; a larger system (or a
+ * history beyond the array) gives FVG_E_INVALID.  Every call on a device shares that
+ * device's array: calls that overlap on one device race.  The preconditioner, matrix and
+ * vectors are the caller's, set up beforehand; kernel timing is off during the call. */
+int fvg_cg_noalloc(fvg_ctx* ctx, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
+                   const fvg_solver_options* opts, fvg_solver_info* info, double* history, int64_t history_capacity);
+int fvg_bicgstab_noalloc(fvg_ctx* ctx, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
+                         const fvg_solver_options* opts, fvg_solver_info* info, double* history,
+                         int64_t history_capacity);
+int fvg_noalloc_scratch_bytes(uint64_t* bytes);
+
 /* ---- batched solves (no reference counterpart: many independent systems) ----- */
 /* A batch holds n independent systems A_k x_k = b_k, each with its own matrix and
  * preconditioner (identity, Jacobi, IC(0) or ILU(0)), solved by one kernel launch in
