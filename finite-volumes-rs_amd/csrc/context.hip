@@ -221,7 +221,7 @@ int fvg_ctx_create(int device, uint64_t budget, fvg_ctx** out) {
         delete c;
         return fail(FVG_E_HIP, "hipStreamCreate failed");
     }
-    int s = dev_alloc_n(c, (int64_t)kMaxReduceBlocks * 4, &c->partials);
+    int s = dev_alloc_n(c, (int64_t)kMaxReduceBlocks * kPartialSlots, &c->partials);
     c->partials_cap = kMaxReduceBlocks;
     if (s == FVG_OK) s = dev_alloc_n(c, 64, &c->scalar);
     if (s == FVG_OK && hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault) != hipSuccess)
@@ -245,7 +245,7 @@ int fvg_ctx_destroy(fvg_ctx* c) {
         (void)hipEventDestroy(s.a);
         (void)hipEventDestroy(s.b);
     }
-    dev_free_n(c, c->partials, c->partials_cap * 4);
+    dev_free_n(c, c->partials, c->partials_cap * kPartialSlots);
     dev_free_n(c, c->scalar, 64);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stage) (void)hipHostFree(c->stage);

@@ -23,9 +23,9 @@ int64_t partial_stride(const fvg_ctx* ctx) { return ctx->partials_cap; }
 int ensure_partials(fvg_ctx* ctx, int64_t count) {
     if (count <= ctx->partials_cap) return FVG_OK;
     FVG_HIP(hipStreamSynchronize(ctx->stream));
-    dev_free_n(ctx, ctx->partials, ctx->partials_cap * 4);
+    dev_free_n(ctx, ctx->partials, ctx->partials_cap * kPartialSlots);
     ctx->partials_cap = 0;
-    FVG_TRY(dev_alloc_n(ctx, count * 4, &ctx->partials));
+    FVG_TRY(dev_alloc_n(ctx, count * kPartialSlots, &ctx->partials));
     ctx->partials_cap = count;
     return FVG_OK;
 }
@@ -37,7 +37,7 @@ __global__ void __launch_bounds__(kSpmvRows) k_spmv(int n, const int32_t* __rest
                                                     SpmvEpilogue epi, double* __restrict__ partials,
                                                     int64_t pstride) {
     __shared__ double prod[kSpmvStage];
-    __shared__ double red[kSpmvRows / 64];
+    __shared__ double red[2 * kSpmvRows / 64];
     const int blk = xcd_tile(blockIdx.x, gridDim.x);
     const int r0 = blk * kSpmvRows;
     const int r1 = min(r0 + kSpmvRows, n);
@@ -90,17 +90,32 @@ __global__ void __launch_bounds__(kSpmvRows) k_spmv(int n, const int32_t* __rest
         if (epi.dot_self) d1 = sum * sum;
         if (epi.dotw2) d2 = epi.dotw2[r] * sum;
     }
-    if (epi.dotw) {
-        d0 = block_sum<kSpmvRows>(d0, red);
-        if (threadIdx.x == 0) partials[blk] = d0;
-    }
-    if (epi.dot_self) {
-        d1 = block_sum<kSpmvRows>(d1, red);
-        if (threadIdx.x == 0) partials[pstride + blk] = d1;
-    }
-    if (epi.dotw2) {
-        d2 = block_sum<kSpmvRows>(d2, red);
-        if (threadIdx.x == 0) partials[2 * pstride + blk] = d2;
+    if (epi.dd) {
+        if (epi.dotw) {
+            const DD v = block_sum<kSpmvRows>(DD{d0, 0.0}, red);
+            if (threadIdx.x == 0) put_dd(partials, pstride, 0, blk, v);
+        }
+        if (epi.dot_self) {
+            const DD v = block_sum<kSpmvRows>(DD{d1, 0.0}, red);
+            if (threadIdx.x == 0) put_dd(partials, pstride, 1, blk, v);
+        }
+        if (epi.dotw2) {
+            const DD v = block_sum<kSpmvRows>(DD{d2, 0.0}, red);
+            if (threadIdx.x == 0) put_dd(partials, pstride, 2, blk, v);
+        }
+    } else {
+        if (epi.dotw) {
+            d0 = block_sum<kSpmvRows>(d0, red);
+            if (threadIdx.x == 0) partials[blk] = d0;
+        }
+        if (epi.dot_self) {
+            d1 = block_sum<kSpmvRows>(d1, red);
+            if (threadIdx.x == 0) partials[pstride + blk] = d1;
+        }
+        if (epi.dotw2) {
+            d2 = block_sum<kSpmvRows>(d2, red);
+            if (threadIdx.x == 0) partials[2 * pstride + blk] = d2;
+        }
     }
 }
 

@@ -31,6 +31,7 @@ int fail(int status, const std::string& msg);
 // ---- launch geometry ------------------------------------------------------------
 constexpr int kBlock = 256;                 // threads per workgroup (4 wave64)
 constexpr int kMaxReduceBlocks = 2048;      // partial sums per reduction
+constexpr int kPartialSlots = 8;            // slots of partials (a double-double slot takes two)
 inline int grid_for(int64_t n, int per_block = kBlock) {
     int64_t g = (n + per_block - 1) / per_block;
     return (int)(g < 1 ? 1 : g);
@@ -44,7 +45,7 @@ struct fvg_ctx {
     hipStream_t stream = nullptr;
     uint64_t budget = 0;       // 0 = unlimited
     uint64_t in_use = 0;
-    double* partials = nullptr;        // 4 slots of partials_cap doubles of reduction scratch
+    double* partials = nullptr;        // kPartialSlots slots of partials_cap doubles of reduction scratch
     int64_t partials_cap = 0;
     double* scalar = nullptr;          // small device scalar area
     void* pinned = nullptr;            // pinned host staging (4 KiB)

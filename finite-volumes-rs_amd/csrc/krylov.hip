@@ -24,45 +24,12 @@ namespace fvg {
 
 constexpr int kScalarBlock = 1024;   // threads per scalar workgroup
 
-enum ScalarStep {
-    ST_BNORM = 0,   // |b|, zero right-hand side exit (conjugate_gradient.rs:29-36)
-    CG_INIT,        // initial residual, loop test
-    CG_RZ0,         // r.z of the first iteration
-    CG_ALPHA,       // alpha = rz / p.Ap
-    CG_ERROR,       // error = |r| / |b| and the convergence test
-    CG_BETA,        // beta = rz_new / rz, history, loop test
-    BI_INIT,        // initial residual, rho = rh.r, early exit (bi_conjugate_gradient_stab.rs:174-190)
-    BI_ALPHA,       // alpha = rho / rh.v
-    BI_S,           // |s| test
-    BI_OMEGA,       // omega = t.s / t.t
-    BI_R,           // |r| test, rho, beta, restart decision, history, loop test
-};
-
-__device__ __forceinline__ void push_history(KState* st, double* hist, double e) {
-    if (st->hist_len < st->hist_cap) hist[st->hist_len] = e;
-    st->hist_len += 1;
-}
-
-// first time a system stops iterating: retire it from the batch
-__device__ __forceinline__ void retire(KState* st, int32_t* active, int32_t* all_done) {
-    st->halt = 1;
-    if (!st->counted) {
-        st->counted = 1;
-        if (atomicSub(active, 1) == 1) *all_done = 1;
-    }
-}
-
-__device__ __forceinline__ void loop_test(KState* st, int32_t* active, int32_t* all_done) {
-    // while error > tolerance { if iter > max_iter { break } ... }
-    if (!(st->error > st->atol) || st->iter > st->max_iter) retire(st, active, all_done);
-}
-
 template <int STEP>
 __global__ void __launch_bounds__(kScalarBlock) k_scalar(KState* __restrict__ states, const double* __restrict__ partials,
                                                          int64_t stride, int count, const int32_t* __restrict__ blk_ptr,
                                                          double* __restrict__ hist_all, int64_t hcap,
                                                          int32_t* __restrict__ active, int32_t* __restrict__ all_done) {
-    __shared__ double lds[kScalarBlock / 64];
+    __shared__ double lds[2 * kScalarBlock / 64];
     __shared__ int skip;
     KState* st = states + blockIdx.x;
     double* hist = hist_all + (int64_t)blockIdx.x * hcap;
@@ -83,108 +50,10 @@ __global__ void __launch_bounds__(kScalarBlock) k_scalar(KState* __restrict__ st
     const int nb = blk_ptr ? blk_ptr[blockIdx.x + 1] - b0 : count;
     constexpr bool need0 = !(STEP == CG_INIT || STEP == BI_INIT);
     constexpr bool need1 = STEP == CG_INIT || STEP == BI_INIT || STEP == BI_OMEGA || STEP == BI_R;
-    const double s0 = need0 ? sum_partials<kScalarBlock>(partials + b0, nb, lds) : 0.0;
-    const double s1 = need1 ? sum_partials<kScalarBlock>(partials + stride + b0, nb, lds) : 0.0;
+    const double s0 = need0 ? sum_partials_dd<kScalarBlock>(partials + 2 * b0, nb, lds).hi : 0.0;
+    const double s1 = need1 ? sum_partials_dd<kScalarBlock>(partials + 2 * stride + 2 * b0, nb, lds).hi : 0.0;
     if (threadIdx.x != 0) return;
-    switch (STEP) {
-    case ST_BNORM:
-        st->bnorm = sqrt(s0);
-        if (st->bnorm < 1e-14) {
-            st->zero_rhs = 1;
-            st->init_res = st->bnorm;
-            st->error = st->bnorm;
-            push_history(st, hist, st->bnorm);
-            st->done = 1;
-            retire(st, active, all_done);
-        }
-        break;
-    case CG_INIT:   // s1 = r.r   (conjugate_gradient.rs:52, 62-66)
-        st->init_res = sqrt(s1) / st->bnorm;
-        st->error = 1.0;
-        st->iter = 0;
-        loop_test(st, active, all_done);
-        break;
-    case CG_RZ0:    // s0 = r.z (conjugate_gradient.rs:71)
-        st->rz = s0;
-        break;
-    case CG_ALPHA:  // s0 = p.Ap (:72-74)
-        st->alpha = st->rz / s0;
-        break;
-    case CG_ERROR: {   // s0 = r.r (:83-89)
-        st->error = sqrt(s0) / st->bnorm;
-        if ((st->error < st->atol) || (st->error / st->init_res < st->rtol)) {
-            push_history(st, hist, st->error);
-            st->iter += 1;
-            st->done = 1;
-            retire(st, active, all_done);
-        }
-        break;
-    }
-    case CG_BETA:   // s0 = r.z_new (:94-105)
-        st->beta = s0 / st->rz;
-        st->rz = s0;
-        push_history(st, hist, st->error);
-        st->iter += 1;
-        loop_test(st, active, all_done);
-        break;
-    case BI_INIT:   // s1 = r.r; rho = rh.r with rh == r (bi_conjugate_gradient_stab.rs:174-190)
-        st->init_res = sqrt(s1) / st->bnorm;
-        st->rho = s1;
-        st->error = 1.0;
-        st->iter = 0;
-        if (st->init_res < st->atol) {   // :176-178
-            st->error = st->init_res;
-            push_history(st, hist, st->init_res);
-            st->done = 1;
-            retire(st, active, all_done);
-            break;
-        }
-        loop_test(st, active, all_done);
-        break;
-    case BI_ALPHA:  // s0 = rh.v (:208-209)
-        st->alpha = st->rho / s0;
-        break;
-    case BI_S: {    // s0 = s.s (:219-231)
-        const double ss = sqrt(s0) / st->bnorm;
-        if ((ss < st->atol) || (ss / st->init_res < st->rtol)) {
-            st->error = ss;
-            push_history(st, hist, ss);
-            st->iter += 1;
-            st->done = 1;
-            retire(st, active, all_done);
-        }
-        break;
-    }
-    case BI_OMEGA:  // s0 = t.s, s1 = t.t (:239-242)
-        st->omega = s0 / s1;
-        break;
-    case BI_R: {    // s1 = r.r, s0 = rh.r (:252-287)
-        const double rr = sqrt(s1) / st->bnorm;
-        if ((rr < st->atol) || (rr / st->init_res < st->rtol)) {
-            st->error = rr;
-            push_history(st, hist, rr);
-            st->iter += 1;
-            st->done = 1;
-            retire(st, active, all_done);
-            break;
-        }
-        const double rho_new = s0;
-        st->beta = (rho_new / st->rho) * (st->alpha / st->omega);
-        st->rho = rho_new;
-        if (fabs(st->rho) > (1e-15 * st->bnorm)) {
-            st->mode = 0;
-        } else {   // restart: rh = r, p = r, rho = rh.r == r.r
-            st->mode = 1;
-            st->restarts += 1;
-            st->rho = s1;
-        }
-        st->error = rr / st->bnorm;
-        push_history(st, hist, st->error);
-        st->iter += 1;
-        loop_test(st, active, all_done);
-        break;
-    }
-    }
+    scalar_step(STEP, st, hist, s0, s1, active, all_done);
 }
 
 // ---- fused vector kernels (one element per thread, kBlock rows per workgroup) -----------
@@ -195,13 +64,13 @@ __global__ void __launch_bounds__(kBlock) k_dot_g(int64_t n, const KState* __res
                                                   const int32_t* __restrict__ blk_inst, int skip_done,
                                                   const double* __restrict__ a, const double* __restrict__ b,
                                                   double* __restrict__ partials) {
-    __shared__ double lds[kBlock / 64];
+    __shared__ double lds[2 * kBlock / 64];
     K_STATE;
     if (skip_done && S.done) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     double d = i < n ? a[i] * b[i] : 0.0;
-    d = block_sum<kBlock>(d, lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = d;
+    const DD dd = block_sum<kBlock>(DD{d, 0.0}, lds);
+    if (threadIdx.x == 0) put_dd(partials, 0, 0, blockIdx.x, dd);
 }
 
 // x = 0 where the right-hand side was zero (conjugate_gradient.rs:31-36)
@@ -218,7 +87,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_xr(int64_t n, const KState* __res
                                                   const int32_t* __restrict__ blk_inst, const double* __restrict__ p,
                                                   const double* __restrict__ ap, double* __restrict__ x,
                                                   double* __restrict__ r, double* __restrict__ partials) {
-    __shared__ double lds[kBlock / 64];
+    __shared__ double lds[2 * kBlock / 64];
     K_STATE;
     if (S.done) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -230,8 +99,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_xr(int64_t n, const KState* __res
         r[i] = ri;
         d = ri * ri;
     }
-    d = block_sum<kBlock>(d, lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = d;
+    const DD dd = block_sum<kBlock>(DD{d, 0.0}, lds);
+    if (threadIdx.x == 0) put_dd(partials, 0, 0, blockIdx.x, dd);
 }
 
 // CG (conjugate_gradient.rs:98-100): p = z + p*beta
@@ -249,7 +118,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_start(int64_t n, const KState* __
                                                      const int32_t* __restrict__ blk_inst, const double* __restrict__ r,
                                                      const double* __restrict__ z, double* __restrict__ p,
                                                      double* __restrict__ partials) {
-    __shared__ double lds[kBlock / 64];
+    __shared__ double lds[2 * kBlock / 64];
     K_STATE;
     if (S.halt) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -258,8 +127,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_start(int64_t n, const KState* __
         p[i] = z[i];
         d = r[i] * z[i];
     }
-    d = block_sum<kBlock>(d, lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = d;
+    const DD dd = block_sum<kBlock>(DD{d, 0.0}, lds);
+    if (threadIdx.x == 0) put_dd(partials, 0, 0, blockIdx.x, dd);
 }
 
 // BiCGStab (bi_conjugate_gradient_stab.rs:213-216): h = x + y*alpha (kept in x); s = r - v*alpha; partial s.s
@@ -268,7 +137,7 @@ __global__ void __launch_bounds__(kBlock) k_bi_hs(int64_t n, const KState* __res
                                                   const double* __restrict__ v, const double* __restrict__ r,
                                                   double* __restrict__ x, double* __restrict__ s,
                                                   double* __restrict__ partials) {
-    __shared__ double lds[kBlock / 64];
+    __shared__ double lds[2 * kBlock / 64];
     K_STATE;
     if (S.done) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -280,8 +149,8 @@ __global__ void __launch_bounds__(kBlock) k_bi_hs(int64_t n, const KState* __res
         s[i] = si;
         d = si * si;
     }
-    d = block_sum<kBlock>(d, lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = d;
+    const DD dd = block_sum<kBlock>(DD{d, 0.0}, lds);
+    if (threadIdx.x == 0) put_dd(partials, 0, 0, blockIdx.x, dd);
 }
 
 // BiCGStab (:246-249): x = h + z*omega; r = s - t*omega; partials rh.r (slot 0) and r.r (slot 1)
@@ -291,7 +160,7 @@ __global__ void __launch_bounds__(kBlock) k_bi_xr(int64_t n, const KState* __res
                                                   const double* __restrict__ rh, double* __restrict__ x,
                                                   double* __restrict__ r, double* __restrict__ partials,
                                                   int64_t stride) {
-    __shared__ double lds[kBlock / 64];
+    __shared__ double lds[2 * kBlock / 64];
     K_STATE;
     if (S.done) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -304,10 +173,10 @@ __global__ void __launch_bounds__(kBlock) k_bi_xr(int64_t n, const KState* __res
         d0 = rh[i] * ri;
         d1 = ri * ri;
     }
-    d0 = block_sum<kBlock>(d0, lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = d0;
-    d1 = block_sum<kBlock>(d1, lds);
-    if (threadIdx.x == 0) partials[stride + blockIdx.x] = d1;
+    const DD e0 = block_sum<kBlock>(DD{d0, 0.0}, lds);
+    if (threadIdx.x == 0) put_dd(partials, stride, 0, blockIdx.x, e0);
+    const DD e1 = block_sum<kBlock>(DD{d1, 0.0}, lds);
+    if (threadIdx.x == 0) put_dd(partials, stride, 1, blockIdx.x, e1);
 }
 
 // BiCGStab (:268-282): p = r + (p - v*omega)*beta, or restart rh = r, p = r
@@ -358,7 +227,16 @@ __global__ void k_kstate_init(int n_inst, KState* __restrict__ st, double atol, 
     if (i == 0) {
         flags[0] = n_inst;
         flags[1] = 0;
+        flags[2] = 0;   // unused
+        flags[3] = 0;
     }
+}
+
+static int kstate_init(KRun& K, const fvg_solver_options* o) {
+    hipLaunchKernelGGL(k_kstate_init, dim3(grid_for(K.n_inst)), dim3(kBlock), 0, K.c->stream, K.n_inst, K.st,
+                       o->absolute_tolerance, o->relative_tolerance, (int64_t)o->max_iterations, K.hist_cap, K.flags);
+    FVG_HIP(hipGetLastError());
+    return FVG_OK;
 }
 
 int KRun::init_at(fvg_ctx* ctx, int ninst, const fvg_solver_options* o, KState* st_, double* hist_,
@@ -370,24 +248,18 @@ int KRun::init_at(fvg_ctx* ctx, int ninst, const fvg_solver_options* o, KState* 
     hist = hist_;
     flags = flags_;
     owned = false;
-    hipLaunchKernelGGL(k_kstate_init, dim3(grid_for(n_inst)), dim3(kBlock), 0, c->stream, n_inst, st,
-                       o->absolute_tolerance, o->relative_tolerance, (int64_t)o->max_iterations, hist_cap, flags);
-    FVG_HIP(hipGetLastError());
-    return FVG_OK;
+    return kstate_init(*this, o);
 }
 
 int KRun::init(fvg_ctx* ctx, int ninst, const fvg_solver_options* o) {
     c = ctx;
     n_inst = ninst;
     hist_cap = o->max_iterations + 2;
+    owned = true;
     FVG_TRY(dev_alloc_n(c, n_inst, &st));
     FVG_TRY(dev_alloc_n(c, hist_cap * n_inst, &hist));
-    FVG_TRY(dev_alloc_n(c, 2, &flags));
-    owned = true;
-    hipLaunchKernelGGL(k_kstate_init, dim3(grid_for(n_inst)), dim3(kBlock), 0, c->stream, n_inst, st,
-                       o->absolute_tolerance, o->relative_tolerance, (int64_t)o->max_iterations, hist_cap, flags);
-    FVG_HIP(hipGetLastError());
-    return FVG_OK;
+    FVG_TRY(dev_alloc_n(c, 4, &flags));
+    return kstate_init(*this, o);
 }
 
 KRun::~KRun() {
@@ -395,7 +267,7 @@ KRun::~KRun() {
         (void)hipStreamSynchronize(c->stream);
         dev_free_n(c, st, n_inst);
         dev_free_n(c, hist, hist_cap * n_inst);
-        dev_free_n(c, flags, 2);
+        dev_free_n(c, flags, 4);
     }
 }
 
@@ -490,6 +362,7 @@ int run_cg_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* 
     e0.skip_stride = sstride;
     e0.blk_inst = L.blk_inst;
     e0.partials = epart;
+    e0.dd = true;
     e0.pstride = stride;
     FVG_TRY(launch_spmv(c, a, x, r, e0));
     SCALAR(CG_INIT, part);
@@ -506,6 +379,7 @@ int run_cg_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* 
         e1.skip_stride = sstride;
         e1.blk_inst = L.blk_inst;
         e1.partials = epart;
+        e1.dd = true;
         e1.pstride = stride;
         FVG_TRY(launch_spmv(c, a, pp, ap, e1));
         SCALAR(CG_ALPHA, part);
@@ -552,6 +426,7 @@ int run_bicgstab_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const do
     e0.skip_stride = sstride;
     e0.blk_inst = L.blk_inst;
     e0.partials = epart;
+    e0.dd = true;
     e0.pstride = stride;
     FVG_TRY(launch_spmv(c, a, x, r, e0));
     SCALAR(BI_INIT, part);
@@ -565,6 +440,7 @@ int run_bicgstab_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const do
         e1.skip_stride = sstride;
         e1.blk_inst = L.blk_inst;
         e1.partials = epart;
+        e1.dd = true;
         e1.pstride = stride;
         FVG_TRY(launch_spmv(c, a, y, v, e1));
         SCALAR(BI_ALPHA, part);
@@ -578,6 +454,7 @@ int run_bicgstab_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const do
         e2.skip_stride = sstride;
         e2.blk_inst = L.blk_inst;
         e2.partials = epart;
+        e2.dd = true;
         e2.pstride = stride;
         FVG_TRY(launch_spmv(c, a, z, t, e2));
         SCALAR(BI_OMEGA, part);

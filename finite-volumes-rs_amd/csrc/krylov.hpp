@@ -2,23 +2,11 @@
 #pragma once
 
 #include "internal.hpp"
+#include "kstate.hpp"
 
 struct fvg_prec;
 
 namespace fvg {
-
-// Per-system solver state, resident on the device.
-struct KState {
-    double bnorm, init_res, error, atol, rtol;
-    double rz, alpha, beta, rho, omega;
-    int64_t iter, max_iter, hist_len, hist_cap, restarts;
-    int32_t done;      // converged or stopped inside an iteration: the rest of it is skipped
-    int32_t halt;      // the system no longer iterates
-    int32_t mode;      // BiCGStab direction update: 0 update, 1 restart
-    int32_t counted;   // already retired from the batch's active count
-    int32_t zero_rhs;  // |b| < 1e-14: x = 0 answered without iterating
-    int32_t pad;
-};
 
 // How the rows of the system split into members.  Default: one member owning all rows.
 struct Layout {
@@ -38,7 +26,7 @@ struct KRun {
     bool owned = true;          // st, hist and flags were allocated by init()
     int init(fvg_ctx* ctx, int ninst, const fvg_solver_options* o);
     // the same state in caller-provided device memory (nothing allocated, nothing freed):
-    // ninst states, ninst * (max_iterations + 2) history slots, 2 flags
+    // ninst states, ninst * (max_iterations + 2) history slots, 4 flags
     int init_at(fvg_ctx* ctx, int ninst, const fvg_solver_options* o, KState* st, double* hist, int32_t* flags);
     int all_done(bool* out);
     int finish(fvg_solver_info* infos, double* const* histories, int64_t cap);
@@ -47,7 +35,7 @@ struct KRun {
 
 // Work vectors and partial sums of one solve, when the caller provides them (no
 // allocation): vec[0..3] for CG, vec[0..7] for BiCGStab, each of n doubles; partials
-// holds 4 slots of `pstride` >= spmv_blocks(n) doubles.
+// holds kPartialSlots slots of `pstride` >= spmv_blocks(n) doubles.
 struct KBuffers {
     double* vec[8] = {};
     double* partials = nullptr;

@@ -59,9 +59,9 @@ static int solve_noalloc(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const 
     const int nvec = bicg ? 8 : 4;
     if (o->max_iterations > (int64_t)(kScratchBytes / sizeof(double)))
         return fail(FVG_E_INVALID, "max_iterations exceeds the no-allocation scratch's history");
-    // layout: partials (4 slots), solver state, flags, history, work vectors
+    // layout: partials (kPartialSlots slots), solver state, flags, history, work vectors
     const int64_t pstride = lines(spmv_blocks(n));
-    const int64_t o_state = 4 * pstride;
+    const int64_t o_state = kPartialSlots * pstride;
     const int64_t o_flags = o_state + lines((int64_t)((sizeof(KState) + 7) / 8));
     const int64_t o_hist = o_flags + 32;
     const int64_t hist = o->max_iterations + 2;   // KRun's history slots

@@ -252,7 +252,7 @@ struct Bits;
 template <>
 struct Bits<double> {
     using U = uint64_t;
-    static constexpr U pending = 0x7FF4DEADBEEF0F64ull;
+    static constexpr U pending = kPendingBits;
     __device__ static double value(U u) { return __longlong_as_double((long long)u); }
     __device__ static U bits(double v) { return (U)__double_as_longlong(v); }
 };
@@ -757,6 +757,7 @@ static void free_sweep(fvg_ctx* c, Sweep& s) {
     dev_free_n(c, s.bfill, s.nfill);
     dev_free_n(c, s.xpos, s.n);
     dev_free_n(c, s.bfrom, s.n);
+    dev_free_n(c, s.bposof, s.n);
     s.h_brow.clear();
     s.nblk = 0;
     s.nfill = 0;
@@ -1188,6 +1189,9 @@ static int pair_layouts(fvg_ctx* c, Sweep& lo, Sweep& up) {
         for (int64_t p = 0; p < lo.n; ++p) pos_lo[(size_t)lo.h_brow[(size_t)p]] = (int32_t)p;
         for (int64_t q = 0; q < up.n; ++q) from[(size_t)q] = pos_lo[(size_t)up.h_brow[(size_t)q]];
         if ((s = dev_alloc_n(c, up.n, &up.bfrom)) == FVG_OK) s = upload(c, up.bfrom, from.data(), up.n);
+        if (s == FVG_OK && (s = dev_alloc_n(c, lo.n, &lo.bposof)) == FVG_OK)
+            s = upload(c, lo.bposof, pos_lo.data(), lo.n);
+        if (s == FVG_OK) s = hipStreamSynchronize(c->stream) == hipSuccess ? FVG_OK : FVG_E_HIP;   // host arrays
     }
     for (Sweep* sw : {&lo, &up}) {
         sw->h_brow.clear();
@@ -1420,11 +1424,14 @@ static bool blocked_fits(fvg_ctx* c, const Sweep& sw) {
 // with `from`, b[from[p]] (b in another layout's position order).  `scatter` = false
 // leaves the values in sw.xpos only (a paired upper sweep reads them there).
 static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, const double* b, double* x, int32_t* err,
-                          const int32_t* skip, const int32_t* from = nullptr, bool scatter = true) {
+                          const int32_t* skip, const int32_t* from = nullptr, bool scatter = true,
+                          bool prepare = true) {
     // the timed span holds one whole triangular solve: set-up, sweep and scatter kernels
+    // (without `prepare`, the caller has written sw.bpos and the pending fill)
     const int tok = timing_begin(c);
-    hipLaunchKernelGGL(k_sweep_prepare<double>, dim3(grid_for(sw.n + sw.nfill)), dim3(kBlock), 0, c->stream, sw.n,
-                       from ? from : sw.brow, b, sw.bpos, sw.nfill, sw.bfill, sw.xpos, skip);
+    if (prepare)
+        hipLaunchKernelGGL(k_sweep_prepare<double>, dim3(grid_for(sw.n + sw.nfill)), dim3(kBlock), 0, c->stream, sw.n,
+                           from ? from : sw.brow, b, sw.bpos, sw.nfill, sw.bfill, sw.xpos, skip);
     int nlev = sw.bL, W = sw.bW, X = sw.bX, spin = spin_limit(c), withhold = c->tune_withhold ? 1 : 0;
     int64_t n = sw.n;
     const bool custom = c->tune_spin_limit > 0 || withhold;
@@ -1437,7 +1444,7 @@ static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, cons
     if (scatter)
         hipLaunchKernelGGL(k_sweep_finish<double>, dim3(grid_for(sw.n)), dim3(kBlock), 0, c->stream, sw.n, sw.brow,
                            (const double*)sw.xpos, x, skip);
-    timing_end(c, tok, kSpanSweep, scatter ? 3 : 2);
+    timing_end(c, tok, kSpanSweep, (prepare ? 2 : 1) + (scatter ? 1 : 0));
     FVG_HIP(hipGetLastError());
     return FVG_OK;
 }

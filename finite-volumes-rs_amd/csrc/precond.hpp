@@ -56,6 +56,9 @@ struct Sweep {
     // position of the row at each upper position, so the upper set-up gathers its
     // right-hand side straight from the lower sweep's xpos (no scatter in between).
     int32_t* bfrom = nullptr;           // n, or null
+    // Lower layout paired with a blocked upper one: the position of each row (n), so a
+    // solver can write the sweep's right-hand side in position order itself.
+    int32_t* bposof = nullptr;
     std::vector<int32_t> h_brow;        // row of each position (host copy, freed once paired)
 };
 constexpr int kBlockRing = 3;           // levels of values a block keeps in LDS
@@ -116,4 +119,7 @@ int ensure_prec32(fvg_ctx* ctx, fvg_prec* p);
 int apply_prec32(fvg_ctx* ctx, const fvg_prec* p, const float* rhs, float* solution, const int32_t* skip);
 // frees the batched-solve plan cached on p (team.hip)
 void team_plan_release(fvg_ctx* ctx, fvg_prec* p);
+
+// pending value bits of an FP64 sweep output
+constexpr uint64_t kPendingBits = 0x7FF4DEADBEEF0F64ull;
 }  // namespace fvg

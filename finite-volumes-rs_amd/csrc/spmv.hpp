@@ -2,6 +2,7 @@
 #pragma once
 
 #include "internal.hpp"
+#include "kstate.hpp"
 
 namespace fvg {
 
@@ -11,6 +12,7 @@ namespace fvg {
 //   partials slot 0: sum of dotw[r] * y_r     (if dotw)
 //   partials slot 1: sum of y_r * y_r         (if dot_self)
 //   partials slot 2: sum of dotw2[r] * y_r    (if dotw2)
+// (slot k at partials[k * stride + block], or with `dd` the pair of put_dd)
 struct SpmvEpilogue {
     const double* b = nullptr;
     double* copy1 = nullptr;
@@ -27,11 +29,13 @@ struct SpmvEpilogue {
     // (caller-provided storage of at least 3 slots of spmv_blocks(n); nothing is allocated)
     double* partials = nullptr;
     int64_t pstride = 0;
+    // the dots as double-double pairs (put_dd / sum_partials_dd layout) rather than doubles
+    bool dd = false;
 };
 
 // Number of partial sums one fused SpMV writes per dot (one per workgroup).
 int spmv_blocks(int64_t nrows);
-// Ensures ctx->partials holds 4 slots of `count` partials each.
+// Ensures ctx->partials holds kPartialSlots slots of `count` partials each.
 int ensure_partials(fvg_ctx* ctx, int64_t count);
 // y = A x with the epilogue; partials go to ctx->partials[slot * stride + block].
 int launch_spmv(fvg_ctx* ctx, const fvg_csr* a, const double* x, double* y, const SpmvEpilogue& epi);
