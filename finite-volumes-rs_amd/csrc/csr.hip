@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(kSpmvRows) k_spmv(int n, const int32_t* __rest
                 a[u] = __builtin_nontemporal_load(v + base + k);
             }
 #pragma unroll
-            for (int u = 0; u < kSpmvUnroll; ++u) xv[u] = x[c[u]];
+            for (int u = 0; u < kSpmvUnroll; ++u) xv[u] = x[EPI && epi.xrev ? epi.xrev - 1 - c[u] : c[u]];
 #pragma unroll
             for (int u = 0; u < kSpmvUnroll; ++u)
                 if (k0 + u * kSpmvRows < cnt) prod[k0 + u * kSpmvRows] = a[u] * xv[u];
@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(kSpmvRows) k_spmv(int n, const int32_t* __rest
         }
     } else if (r < r1) {
         const int e = rp[r + 1];
-        for (int k = rp[r]; k < e; ++k) sum += v[k] * x[ci[k]];
+        for (int k = rp[r]; k < e; ++k) sum += v[k] * x[EPI && epi.xrev ? epi.xrev - 1 - ci[k] : ci[k]];
     }
     if (!EPI) {
         if (r < r1) y[r] = sum;

@@ -196,6 +196,158 @@ __global__ void __launch_bounds__(kBlock) k_bi_p(int64_t n, const KState* __rest
     }
 }
 
+// ---- kernels of the position-order solve (one system; see PosSolve) ------------------------
+// Every vector is indexed by the lower sweep's position; z (and BiCGStab's y) is read
+// reversed from the upper sweep's output.  The vector kernels that precede an apply also
+// write the pending pattern at the positions the two sweeps poll; where such a kernel reads
+// the upper output, each thread reads its element before overwriting it, so no element is
+// overwritten while another thread may still read it.
+#define K_ONE const KState& S = *states
+
+__device__ __forceinline__ void pending_at(const PosSolve& P, int64_t q, int64_t qu) {
+    if (P.lo_poll[q]) reinterpret_cast<uint64_t*>(P.lo_x)[q] = kPendingBits;
+    if (P.up_poll[qu]) reinterpret_cast<uint64_t*>(P.up_x)[qu] = kPendingBits;
+}
+
+// b and x into position order
+__global__ void k_pos_in(int64_t n, const int32_t* __restrict__ row, const double* __restrict__ b,
+                         const double* __restrict__ x, double* __restrict__ bp, double* __restrict__ xp) {
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q < n) {
+        const int32_t i = row[q];
+        bp[q] = b[i];
+        xp[q] = x[i];
+    }
+}
+
+// the solution back to its rows
+__global__ void k_pos_out(int64_t n, const int32_t* __restrict__ row, const double* __restrict__ xp,
+                          double* __restrict__ x) {
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q < n) x[row[q]] = xp[q];
+}
+
+// the pending pattern for the first apply
+__global__ void k_pos_pending(int64_t n, const KState* __restrict__ states, PosSolve P) {
+    K_ONE;
+    if (S.halt) return;
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q < n) pending_at(P, q, q);
+}
+
+// CG start (conjugate_gradient.rs:59): p = z, partial r.z; the next apply's pending pattern
+__global__ void __launch_bounds__(kBlock) k_cg_start_pos(int64_t n, const KState* __restrict__ states,
+                                                         const double* __restrict__ r, double* __restrict__ p,
+                                                         double* __restrict__ partials, PosSolve P) {
+    __shared__ double lds[2 * kBlock / 64];
+    K_ONE;
+    if (S.halt) return;
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double d = 0.0;
+    if (q < n) {
+        const int64_t qu = n - 1 - q;
+        const double zq = P.up_x[qu];
+        p[q] = zq;
+        d = r[q] * zq;
+        pending_at(P, q, qu);
+    }
+    const DD dd = block_sum<kBlock>(DD{d, 0.0}, lds);
+    if (threadIdx.x == 0) put_dd(partials, 0, 0, blockIdx.x, dd);
+}
+
+// CG (conjugate_gradient.rs:94): partial r.z_new
+__global__ void __launch_bounds__(kBlock) k_cg_rz_pos(int64_t n, const KState* __restrict__ states,
+                                                      const double* __restrict__ r, const double* __restrict__ up_x,
+                                                      double* __restrict__ partials) {
+    __shared__ double lds[2 * kBlock / 64];
+    K_ONE;
+    if (S.done) return;
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const double d = q < n ? r[q] * up_x[n - 1 - q] : 0.0;
+    const DD dd = block_sum<kBlock>(DD{d, 0.0}, lds);
+    if (threadIdx.x == 0) put_dd(partials, 0, 0, blockIdx.x, dd);
+}
+
+// CG (conjugate_gradient.rs:98-100): p = z + p*beta; the next apply's pending pattern
+__global__ void __launch_bounds__(kBlock) k_cg_p_pos(int64_t n, const KState* __restrict__ states,
+                                                     double* __restrict__ p, PosSolve P) {
+    K_ONE;
+    if (S.done) return;
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q < n) {
+        const int64_t qu = n - 1 - q;
+        p[q] = P.up_x[qu] + p[q] * S.beta;
+        pending_at(P, q, qu);
+    }
+}
+
+// BiCGStab (bi_conjugate_gradient_stab.rs:213-216): h = x + y*alpha (kept in x); s = r - v*alpha;
+// partial s.s; the second apply's pending pattern (y is read from the upper output first)
+__global__ void __launch_bounds__(kBlock) k_bi_hs_pos(int64_t n, const KState* __restrict__ states,
+                                                      const double* __restrict__ v, const double* __restrict__ r,
+                                                      double* __restrict__ x, double* __restrict__ s,
+                                                      double* __restrict__ partials, PosSolve P) {
+    __shared__ double lds[2 * kBlock / 64];
+    K_ONE;
+    if (S.done) return;
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const double alpha = S.alpha;
+    double d = 0.0;
+    if (q < n) {
+        const int64_t qu = n - 1 - q;
+        x[q] = x[q] + P.up_x[qu] * alpha;
+        const double sq = r[q] - v[q] * alpha;
+        s[q] = sq;
+        d = sq * sq;
+        pending_at(P, q, qu);
+    }
+    const DD dd = block_sum<kBlock>(DD{d, 0.0}, lds);
+    if (threadIdx.x == 0) put_dd(partials, 0, 0, blockIdx.x, dd);
+}
+
+// BiCGStab (:246-249): x = h + z*omega; r = s - t*omega; partials rh.r (slot 0) and r.r (slot 1)
+__global__ void __launch_bounds__(kBlock) k_bi_xr_pos(int64_t n, const KState* __restrict__ states,
+                                                      const double* __restrict__ up_x, const double* __restrict__ t,
+                                                      const double* __restrict__ s, const double* __restrict__ rh,
+                                                      double* __restrict__ x, double* __restrict__ r,
+                                                      double* __restrict__ partials, int64_t stride) {
+    __shared__ double lds[2 * kBlock / 64];
+    K_ONE;
+    if (S.done) return;
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const double omega = S.omega;
+    double d0 = 0.0, d1 = 0.0;
+    if (q < n) {
+        x[q] = x[q] + up_x[n - 1 - q] * omega;
+        const double rq = s[q] - t[q] * omega;
+        r[q] = rq;
+        d0 = rh[q] * rq;
+        d1 = rq * rq;
+    }
+    const DD e0 = block_sum<kBlock>(DD{d0, 0.0}, lds);
+    if (threadIdx.x == 0) put_dd(partials, stride, 0, blockIdx.x, e0);
+    const DD e1 = block_sum<kBlock>(DD{d1, 0.0}, lds);
+    if (threadIdx.x == 0) put_dd(partials, stride, 1, blockIdx.x, e1);
+}
+
+// BiCGStab (:268-282): p = r + (p - v*omega)*beta, or the restart rh = r, p = r; the next
+// iteration's first apply's pending pattern
+__global__ void __launch_bounds__(kBlock) k_bi_p_pos(int64_t n, const KState* __restrict__ states,
+                                                     const double* __restrict__ r, const double* __restrict__ v,
+                                                     double* __restrict__ p, double* __restrict__ rh, PosSolve P) {
+    K_ONE;
+    if (S.done) return;
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= n) return;
+    if (S.mode == 0) {
+        p[q] = r[q] + (p[q] - v[q] * S.omega) * S.beta;
+    } else {
+        rh[q] = r[q];
+        p[q] = r[q];
+    }
+    pending_at(P, q, q);
+}
+
 // ---- host drivers ----------------------------------------------------------------------
 
 struct Workspace {
@@ -336,8 +488,146 @@ static int solve_buffers(fvg_ctx* c, int64_t n, int count, const KBuffers* kb, W
     return FVG_OK;
 }
 
+// CG in the lower sweep's position order (PosSolve): 9 kernels per iteration, none of
+// which reorders a vector.
+static int run_cg_pos(fvg_ctx* c, const fvg_prec* p, const double* b, double* x, const Layout& L, KRun& K,
+                      const KBuffers* kb, const PosSolve& P) {
+    const int64_t n = P.n;
+    Workspace ws{c};
+    double* v[4];
+    double* part;
+    int64_t stride;
+    FVG_TRY(solve_buffers(c, n, 4, kb, ws, v, &part, &stride));
+    double *xp = v[0], *r = v[1], *pp = v[2], *ap = v[3];
+    double* bp = v[3];   // b in position order, dead once r is formed (then ap)
+    const int nb = spmv_blocks(n);
+    double* const epart = kb ? part : nullptr;
+    const int32_t* all_done = K.flags + 1;
+    hipLaunchKernelGGL(k_pos_in, dim3(nb), dim3(kBlock), 0, c->stream, n, P.row, b, (const double*)x, bp, xp);
+    hipLaunchKernelGGL(k_dot_g, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, nullptr, 0, bp, bp, part);
+    SCALAR(ST_BNORM, part);
+    hipLaunchKernelGGL(k_zero_rhs_x, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, nullptr, xp);
+    SpmvEpilogue e0;   // r = (A x - b) * -1 and r.r
+    e0.b = bp;
+    e0.dot_self = true;
+    e0.skip = &K.st[0].halt;
+    e0.partials = epart;
+    e0.pstride = stride;
+    e0.dd = true;
+    FVG_TRY(launch_spmv(c, P.pa, xp, r, e0));
+    SCALAR(CG_INIT, part);
+    hipLaunchKernelGGL(k_pos_pending, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, P);
+    FVG_TRY(apply_pos(c, p, r, all_done));
+    hipLaunchKernelGGL(k_cg_start_pos, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, r, pp, part, P);
+    SCALAR(CG_RZ0, part);
+    FVG_HIP(hipGetLastError());
+    for (int64_t it = 0;; ++it) {
+        SpmvEpilogue e1;   // ap = A p with p.ap
+        e1.dotw = pp;
+        e1.skip = &K.st[0].halt;
+        e1.partials = epart;
+        e1.pstride = stride;
+        e1.dd = true;
+        FVG_TRY(launch_spmv(c, P.pa, pp, ap, e1));
+        SCALAR(CG_ALPHA, part);
+        hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, nullptr, pp, ap, xp, r, part);
+        SCALAR(CG_ERROR, part);
+        FVG_TRY(apply_pos(c, p, r, all_done));
+        hipLaunchKernelGGL(k_cg_rz_pos, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, r, (const double*)P.up_x, part);
+        SCALAR(CG_BETA, part);
+        hipLaunchKernelGGL(k_cg_p_pos, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, pp, P);
+        FVG_HIP(hipGetLastError());
+        if ((it + 1) % kPollBatch == 0) {
+            bool d;
+            FVG_TRY(K.all_done(&d));
+            if (d) break;
+        }
+    }
+    hipLaunchKernelGGL(k_pos_out, dim3(nb), dim3(kBlock), 0, c->stream, n, P.row, (const double*)xp, x);
+    FVG_HIP(hipGetLastError());
+    return check_sweep_status(c, p);
+}
+
+// BiCGStab in the lower sweep's position order (PosSolve); y and z are read reversed from
+// the upper sweep's output.
+static int run_bicgstab_pos(fvg_ctx* c, const fvg_prec* p, const double* b, double* x, const Layout& L, KRun& K,
+                            const KBuffers* kb, const PosSolve& P) {
+    const int64_t n = P.n;
+    Workspace ws{c};
+    double* w[7];
+    double* part;
+    int64_t stride;
+    FVG_TRY(solve_buffers(c, n, 7, kb, ws, w, &part, &stride));
+    double *xp = w[0], *r = w[1], *rh = w[2], *pp = w[3], *v = w[4], *s = w[5], *t = w[6];
+    double* bp = w[6];   // b in position order, dead once r is formed (then t)
+    const int nb = spmv_blocks(n);
+    double* const epart = kb ? part : nullptr;
+    const int32_t* all_done = K.flags + 1;
+    hipLaunchKernelGGL(k_pos_in, dim3(nb), dim3(kBlock), 0, c->stream, n, P.row, b, (const double*)x, bp, xp);
+    hipLaunchKernelGGL(k_dot_g, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, nullptr, 0, bp, bp, part);
+    SCALAR(ST_BNORM, part);
+    hipLaunchKernelGGL(k_zero_rhs_x, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, nullptr, xp);
+    SpmvEpilogue e0;   // r = b - A x, rh = r, p = r, r.r
+    e0.b = bp;
+    e0.copy1 = rh;
+    e0.copy2 = pp;
+    e0.dot_self = true;
+    e0.skip = &K.st[0].halt;
+    e0.partials = epart;
+    e0.pstride = stride;
+    e0.dd = true;
+    FVG_TRY(launch_spmv(c, P.pa, xp, r, e0));
+    SCALAR(BI_INIT, part);
+    hipLaunchKernelGGL(k_pos_pending, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, P);
+    FVG_HIP(hipGetLastError());
+    for (int64_t it = 0;; ++it) {
+        FVG_TRY(apply_pos(c, p, pp, all_done));   // y = M^-1 p (reversed in up_x)
+        SpmvEpilogue e1;                            // v = A y, rh.v
+        e1.dotw = rh;
+        e1.skip = &K.st[0].halt;
+        e1.partials = epart;
+        e1.pstride = stride;
+        e1.dd = true;
+        e1.xrev = n;
+        FVG_TRY(launch_spmv(c, P.pa, P.up_x, v, e1));
+        SCALAR(BI_ALPHA, part);
+        hipLaunchKernelGGL(k_bi_hs_pos, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, v, r, xp, s, part, P);
+        SCALAR(BI_S, part);
+        FVG_TRY(apply_pos(c, p, s, all_done));    // z = M^-1 s (reversed in up_x)
+        SpmvEpilogue e2;                            // t = A z, t.s, t.t
+        e2.dotw = s;
+        e2.dot_self = true;
+        e2.skip = &K.st[0].done;
+        e2.partials = epart;
+        e2.pstride = stride;
+        e2.dd = true;
+        e2.xrev = n;
+        FVG_TRY(launch_spmv(c, P.pa, P.up_x, t, e2));
+        SCALAR(BI_OMEGA, part);
+        hipLaunchKernelGGL(k_bi_xr_pos, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, (const double*)P.up_x, t, s, rh,
+                           xp, r, part, stride);
+        SCALAR(BI_R, part);
+        hipLaunchKernelGGL(k_bi_p_pos, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, r, v, pp, rh, P);
+        FVG_HIP(hipGetLastError());
+        if ((it + 1) % kPollBatch == 0) {
+            bool d;
+            FVG_TRY(K.all_done(&d));
+            if (d) break;
+        }
+    }
+    hipLaunchKernelGGL(k_pos_out, dim3(nb), dim3(kBlock), 0, c->stream, n, P.row, (const double*)xp, x);
+    FVG_HIP(hipGetLastError());
+    return check_sweep_status(c, p);
+}
+
 int run_cg_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* b, double* x, const Layout& L,
                  KRun& K, const KBuffers* kb) {
+    if (L.n_inst == 1) {
+        PosSolve P;
+        bool ok;
+        FVG_TRY(pos_solve(c, p, a, kb == nullptr, &P, &ok));
+        if (ok) return run_cg_pos(c, p, b, x, L, K, kb, P);
+    }
     const int64_t n = a->nrows;
     Workspace ws{c};
     double* v[4];
@@ -401,6 +691,12 @@ int run_cg_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* 
 
 int run_bicgstab_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* b, double* x, const Layout& L,
                        KRun& K, const KBuffers* kb) {
+    if (L.n_inst == 1) {
+        PosSolve P;
+        bool ok;
+        FVG_TRY(pos_solve(c, p, a, kb == nullptr, &P, &ok));
+        if (ok) return run_bicgstab_pos(c, p, b, x, L, K, kb, P);
+    }
     const int64_t n = a->nrows;
     Workspace ws{c};
     double* w[8];

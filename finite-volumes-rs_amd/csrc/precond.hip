@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(1024)
                     const int32_t* __restrict__ bxidx, const uint16_t* __restrict__ bdep,
                     const T* __restrict__ bval, const T* __restrict__ bpos,
                     const T* __restrict__ spos, T* xpos, int32_t* __restrict__ err,
-                    const int32_t* __restrict__ skip, int spin_limit, int withhold) {
+                    const int32_t* __restrict__ skip, int spin_limit, int withhold, int brev) {
     using U = typename Bits<T>::U;
     extern __shared__ __align__(16) unsigned char smem[];
     if (skip && *skip) return;
@@ -588,7 +588,7 @@ __global__ void __launch_bounds__(1024)
         for (int g = 0; g < K / 4; ++g) r.dw[g] = dw[g];
 #pragma unroll
         for (int h = 0; h < K / 2; ++h) r.a[h] = a2[h];
-        r.bv = bpos[p];
+        r.bv = bpos[brev ? n - 1 - p : p];   // brev: the right-hand side in reverse order
         r.sc = SCALE ? spos[p] : T(1);
         r.ok = ok;
     };
@@ -758,7 +758,10 @@ static void free_sweep(fvg_ctx* c, Sweep& s) {
     dev_free_n(c, s.xpos, s.n);
     dev_free_n(c, s.bfrom, s.n);
     dev_free_n(c, s.bposof, s.n);
+    dev_free_n(c, s.bpolled, s.n);
     s.h_brow.clear();
+    s.h_assign = BlockAssign{};
+    s.mirrored = false;
     s.nblk = 0;
     s.nfill = 0;
 }
@@ -885,19 +888,19 @@ static int delayed_levels(int64_t n, const std::vector<int32_t>& rp, const std::
     return maxl + 1;
 }
 
-// The blocked layout of a sweep (see k_sweep_blocked).  Rows joined by a dependency one
-// level apart are kept in one block (union-find over those edges) and the components are
-// packed into at most `target` blocks (pack_components); when one component holds more
-// than an eighth of the rows, the blocks are instead target_consecutive ranges of
-// consecutive rows with delayed levels (delayed_levels).  Not built (nblk stays 0) when
-// that would cost more than half again the levels or keep fewer than 8 blocks busy per
-// level, when a row has more than kBlockMaxDeps dependencies, or when a block holds more
-// than kBlockMaxRows rows of one level: the persistent sweep then serves.
-static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
-                         bool upper, const std::vector<int32_t>* src_map, const std::vector<int32_t>& lev_in,
-                         int nlevels_in, int target, int target_consecutive, Sweep& sw) {
+// The block and level assignment of a blocked sweep layout (see k_sweep_blocked).  Rows
+// joined by a dependency one level apart are kept in one block (union-find over those
+// edges) and the components are packed into at most `target` blocks (pack_components);
+// when one component holds more than an eighth of the rows, the blocks are instead
+// target_consecutive ranges of consecutive rows with delayed levels (delayed_levels).
+// False (no blocked layout) when that would cost more than half again the levels or keep
+// fewer than 8 blocks busy per level, or when a row has more than kBlockMaxDeps
+// dependencies: the persistent sweep then serves.
+static bool assign_blocked(const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n, bool upper,
+                           const std::vector<int32_t>& lev_in, int nlevels_in, int target, int target_consecutive,
+                           BlockAssign& A) {
     auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
-    if (n < 4096 || target < 2) return FVG_OK;
+    if (n < 4096 || target < 2) return false;
     std::vector<int32_t> par((size_t)n);
     for (int64_t i = 0; i < n; ++i) par[(size_t)i] = (int32_t)i;
     auto find = [&](int32_t a) {
@@ -924,7 +927,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
         }
         kmax = std::max(kmax, cnt);
     }
-    if (kmax > kBlockMaxDeps) return FVG_OK;
+    if (kmax > kBlockMaxDeps) return false;
     kmax = kmax <= 4 ? 4 : kBlockMaxDeps;   // the kernel reads exactly 4 or 8 slots per row
     // components, ordered by their first row, packed into blocks
     std::vector<int32_t> comp_of((size_t)n), comp_size;
@@ -958,12 +961,12 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
             comp_size[(size_t)comp_of[(size_t)i]]++;
         }
         nlevels = delayed_levels(n, rp, col, upper, comp_of, dlev);
-        if ((int64_t)nlevels * 2 > (int64_t)nlevels_in * 3) return FVG_OK;
+        if ((int64_t)nlevels * 2 > (int64_t)nlevels_in * 3) return false;
         std::vector<int64_t> pairs((size_t)n);
         for (int64_t i = 0; i < n; ++i) pairs[(size_t)i] = (int64_t)comp_of[(size_t)i] * nlevels + dlev[(size_t)i];
         std::sort(pairs.begin(), pairs.end());
         const int64_t busy = (int64_t)(std::unique(pairs.begin(), pairs.end()) - pairs.begin());
-        if (busy < 8 * (int64_t)nlevels) return FVG_OK;
+        if (busy < 8 * (int64_t)nlevels) return false;
     }
     const std::vector<int32_t>& lev = contiguous ? dlev : lev_in;
     // blocks: the consecutive row ranges as they are, or the components packed
@@ -976,14 +979,37 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
     } else {
         nb = pack_components(n, rp, col, upper, comp_of, C, lev, nlevels, target, blk_of_comp);
     }
-    // positions: block by block, level by level, ascending rows
+    A.nb = nb;
+    A.nlevels = nlevels;
+    A.kmax = kmax;
+    A.lev = lev;
+    A.blk.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i) A.blk[(size_t)i] = blk_of_comp[(size_t)comp_of[(size_t)i]];
+    return true;
+}
+
+// The blocked layout of an assignment (block and level of every row).  Positions run block
+// by block and level by level; within a block-level rows ascend, or descend with
+// `descending` (a mirrored layout).  Returns FVG_OK with sw.nblk = 0 when the layout does not
+// fit the kernel (a block-level beyond kBlockMaxRows rows, LDS slots beyond 16 bits).
+static int materialize_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
+                               bool upper, const std::vector<int32_t>* src_map, const BlockAssign& A, bool descending,
+                               Sweep& sw) {
+    auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
+    const int32_t nb = A.nb;
+    const int nlevels = A.nlevels, kmax = A.kmax;
+    const std::vector<int32_t>& lev = A.lev;
+    const std::vector<int32_t>& blk = A.blk;
+    sw.bW = 0;
+    sw.bF = 0;
+    // positions: block by block, level by level, rows ascending (or descending)
     std::vector<int32_t> order((size_t)n);
     for (int64_t i = 0; i < n; ++i) order[(size_t)i] = (int32_t)i;
     std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-        const int32_t ba = blk_of_comp[(size_t)comp_of[(size_t)a]], bb = blk_of_comp[(size_t)comp_of[(size_t)b]];
+        const int32_t ba = blk[(size_t)a], bb = blk[(size_t)b];
         if (ba != bb) return ba < bb;
         if (lev[(size_t)a] != lev[(size_t)b]) return lev[(size_t)a] < lev[(size_t)b];
-        return a < b;
+        return descending ? a > b : a < b;
     });
     const int64_t L1 = nlevels + 1;
     std::vector<int32_t> blvl((size_t)(nb * L1), 0), brange((size_t)(2 * nb), 0), pos_of((size_t)n);
@@ -995,8 +1021,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
             for (int l = 0; l < nlevels; ++l) {
                 blvl[(size_t)(b * L1 + l)] = (int32_t)p;
                 const int64_t p0 = p;
-                while (p < n && blk_of_comp[(size_t)comp_of[(size_t)order[(size_t)p]]] == b &&
-                       lev[(size_t)order[(size_t)p]] == l)
+                while (p < n && blk[(size_t)order[(size_t)p]] == b && lev[(size_t)order[(size_t)p]] == l)
                     ++p;
                 if (p > p0) {
                     if (first < 0) first = l;
@@ -1011,7 +1036,7 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
         }
     }
     const int32_t W = sw.bW;
-    auto blk_of_row = [&](int32_t i) { return blk_of_comp[(size_t)comp_of[(size_t)i]]; };
+    auto blk_of_row = [&](int32_t i) { return blk[(size_t)i]; };
     // dependencies: a value of the same block less than kBlockRing levels back is read from
     // the block's LDS ring; any other is fetched into the level's staging slots
     std::vector<int32_t> bdep((size_t)(kmax * n), -1 - (1 << 30)), bsrc((size_t)(kmax * n), -1);
@@ -1107,10 +1132,68 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
     return FVG_OK;
 }
 
+// The blocked layout of a sweep (see k_sweep_blocked): assign_blocked, then the layout.
+// The assignment is kept (sw.h_assign) so that a mirrored layout can be made from it.
+static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
+                         bool upper, const std::vector<int32_t>* src_map, const std::vector<int32_t>& lev_in,
+                         int nlevels_in, int target, int target_consecutive, Sweep& sw) {
+    BlockAssign A;
+    if (!assign_blocked(rp, col, n, upper, lev_in, nlevels_in, target, target_consecutive, A)) return FVG_OK;
+    FVG_TRY(materialize_blocked(c, rp, col, n, upper, src_map, A, false, sw));
+    if (sw.nblk > 0) sw.h_assign = std::move(A);
+    return FVG_OK;
+}
+
+// The upper layout mirrored from the blocked lower layout `lo`: block nb-1-b, level
+// L-1-l and rows descending where the lower has block b, level l and rows ascending, so
+// upper position q holds the row of lower position n-1-q.  A valid upper schedule when the
+// pattern is structurally symmetric: an upper dependency i -> j (j > i) is then the lower
+// dependency j -> i, one or more levels deep (two or more between blocks) in the lower
+// layout and so in its mirror.  Leaves up.nblk = 0 when the pattern is not symmetric or
+// the layout does not fit the kernel.
+static int build_mirrored(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
+                          const std::vector<int32_t>* src_map, const Sweep& lo, Sweep& up) {
+    const BlockAssign& L = lo.h_assign;
+    if (lo.nblk == 0 || (int64_t)L.blk.size() != n) return FVG_OK;
+    int kmax = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int cnt = 0;
+        for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
+            const int32_t j = col[(size_t)k];
+            if (j >= n) continue;
+            if (j > i) ++cnt;
+            // structural symmetry: (j, i) is in the pattern too
+            auto b = col.begin() + rp[(size_t)j], e = col.begin() + rp[(size_t)j + 1];
+            auto it = std::lower_bound(b, e, (int32_t)i);
+            if (it == e || *it != (int32_t)i) return FVG_OK;
+        }
+        kmax = std::max(kmax, cnt);
+    }
+    if (kmax > kBlockMaxDeps) return FVG_OK;
+    BlockAssign U;
+    U.nb = L.nb;
+    U.nlevels = L.nlevels;
+    U.kmax = kmax <= 4 ? 4 : kBlockMaxDeps;
+    U.blk.resize((size_t)n);
+    U.lev.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        U.blk[(size_t)i] = L.nb - 1 - L.blk[(size_t)i];
+        U.lev[(size_t)i] = L.nlevels - 1 - L.lev[(size_t)i];
+    }
+    FVG_TRY(materialize_blocked(c, rp, col, n, true, src_map, U, true, up));
+    if (up.nblk == 0) return FVG_OK;
+    bool ok = up.h_brow.size() == (size_t)n && lo.h_brow.size() == (size_t)n;
+    for (int64_t q = 0; ok && q < n; ++q) ok = up.h_brow[(size_t)q] == lo.h_brow[(size_t)(n - 1 - q)];
+    if (!ok) return fail(FVG_E_INVALID, "mirrored sweep layout does not reverse the lower one");
+    up.mirrored = true;
+    return FVG_OK;
+}
+
 // upper=false: entries j < i; upper=true: entries i < j < n.  src_map (optional)
-// maps a matrix slot to the factor slot holding that entry's value.
+// maps a matrix slot to the factor slot holding that entry's value.  An upper sweep with
+// `mirror` (the factor's blocked lower layout) takes the mirrored layout where it can.
 static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
-                       bool upper, const std::vector<int32_t>* src_map, Sweep& sw) {
+                       bool upper, const std::vector<int32_t>* src_map, Sweep& sw, const Sweep* mirror = nullptr) {
     std::vector<int32_t> lev((size_t)n, 0);
     int maxlev = 0;
     auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
@@ -1176,6 +1259,10 @@ static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::ve
     // 2,097,152 prisms: 597 us per ILU(0) apply with 512 blocks, 621 with 256, 608 with 1024)
     const int packed = c->tune_sweep_blocks ? (int)c->tune_sweep_blocks : std::max(2, c->num_cus);
     const int consecutive = c->tune_sweep_blocks ? (int)c->tune_sweep_blocks : std::max(2, 2 * c->num_cus);
+    if (upper && mirror && mirror->nblk > 0) {
+        FVG_TRY(build_mirrored(c, rp, col, n, src_map, *mirror, sw));
+        if (sw.nblk > 0) return FVG_OK;
+    }
     return build_blocked(c, rp, col, n, upper, src_map, lev, sw.nlevels, packed, consecutive, sw);
 }
 
@@ -1191,11 +1278,28 @@ static int pair_layouts(fvg_ctx* c, Sweep& lo, Sweep& up) {
         if ((s = dev_alloc_n(c, up.n, &up.bfrom)) == FVG_OK) s = upload(c, up.bfrom, from.data(), up.n);
         if (s == FVG_OK && (s = dev_alloc_n(c, lo.n, &lo.bposof)) == FVG_OK)
             s = upload(c, lo.bposof, pos_lo.data(), lo.n);
+        // a mirrored pair: which positions each sweep polls
+        std::vector<uint8_t> pl, pu;
+        if (s == FVG_OK && up.mirrored) {
+            pl.assign((size_t)lo.n, 0);
+            pu.assign((size_t)up.n, 0);
+            for (Sweep* sw : {&lo, &up}) {
+                std::vector<int32_t> fill((size_t)sw->nfill);
+                if ((s = download(c, fill.data(), sw->bfill, sw->nfill)) != FVG_OK) break;
+                std::vector<uint8_t>& f = sw == &lo ? pl : pu;
+                for (int32_t q : fill) f[(size_t)q] = 1;
+            }
+            if (s == FVG_OK && (s = dev_alloc_n(c, lo.n, &lo.bpolled)) == FVG_OK &&
+                (s = dev_alloc_n(c, up.n, &up.bpolled)) == FVG_OK && (s = upload(c, lo.bpolled, pl.data(), lo.n)) ==
+                    FVG_OK)
+                s = upload(c, up.bpolled, pu.data(), up.n);
+        }
         if (s == FVG_OK) s = hipStreamSynchronize(c->stream) == hipSuccess ? FVG_OK : FVG_E_HIP;   // host arrays
     }
     for (Sweep* sw : {&lo, &up}) {
         sw->h_brow.clear();
         sw->h_brow.shrink_to_fit();
+        sw->h_assign = BlockAssign{};
     }
     return s;
 }
@@ -1206,6 +1310,24 @@ static void free_tri32(fvg_ctx* c, Tri32& t) {
     dev_free_n(c, t.src, t.nnz);
     dev_free_n(c, t.val, t.nnz);
     t = Tri32{};
+}
+
+// pa->val[k] = a->val[pa_src[k]]
+__global__ void k_gather_values(int64_t nnz, const int32_t* __restrict__ src, const double* __restrict__ v,
+                                double* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnz) out[k] = v[src[k]];
+}
+
+static void free_pos_matrix(fvg_ctx* c, const fvg_prec* p) {
+    if (!p->pa) return;
+    dev_free_n(c, p->pa->row_ptr, p->n + 1);
+    dev_free_n(c, p->pa->col, p->nnz);
+    dev_free_n(c, p->pa->val, p->nnz);
+    dev_free_n(c, p->pa_src, p->nnz);
+    delete p->pa;
+    p->pa = nullptr;
+    p->pa_of = nullptr;
 }
 
 static void free_prec(fvg_ctx* c, fvg_prec* p) {
@@ -1225,6 +1347,7 @@ static void free_prec(fvg_ctx* c, fvg_prec* p) {
     free_tri32(c, p->jup);
     free_sweep(c, p->lower);
     free_sweep(c, p->upper);
+    free_pos_matrix(c, p);
 }
 
 // pattern, diagonal slots and device copies shared by every kind
@@ -1425,20 +1548,23 @@ static bool blocked_fits(fvg_ctx* c, const Sweep& sw) {
 // leaves the values in sw.xpos only (a paired upper sweep reads them there).
 static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, const double* b, double* x, int32_t* err,
                           const int32_t* skip, const int32_t* from = nullptr, bool scatter = true,
-                          bool prepare = true) {
+                          bool prepare = true, const double* bsrc = nullptr, int brev = 0) {
     // the timed span holds one whole triangular solve: set-up, sweep and scatter kernels
-    // (without `prepare`, the caller has written sw.bpos and the pending fill)
+    // (without `prepare`, the caller has written the pending fill, and the right-hand side
+    // in position order is sw.bpos, or bsrc, read in reverse with brev)
     const int tok = timing_begin(c);
     if (prepare)
         hipLaunchKernelGGL(k_sweep_prepare<double>, dim3(grid_for(sw.n + sw.nfill)), dim3(kBlock), 0, c->stream, sw.n,
                            from ? from : sw.brow, b, sw.bpos, sw.nfill, sw.bfill, sw.xpos, skip);
     int nlev = sw.bL, W = sw.bW, X = sw.bX, spin = spin_limit(c), withhold = c->tune_withhold ? 1 : 0;
     int64_t n = sw.n;
+    const double* bp = bsrc ? bsrc : sw.bpos;
     const bool custom = c->tune_spin_limit > 0 || withhold;
     const void* fn = scale ? blocked_kernel<true>(sw.bK, custom) : blocked_kernel<false>(sw.bK, custom);
     void* args[] = {&nlev, &n, &W, &X, (void*)&sw.brange, (void*)&sw.blvl, (void*)&sw.bxptr, (void*)&sw.bxidx,
-                    (void*)&sw.bdep, (void*)&sw.bval, (void*)&sw.bpos,
-                    (void*)&sw.spos, (void*)&sw.xpos, (void*)&err, (void*)&skip, (void*)&spin, (void*)&withhold};
+                    (void*)&sw.bdep, (void*)&sw.bval, (void*)&bp,
+                    (void*)&sw.spos, (void*)&sw.xpos, (void*)&err, (void*)&skip, (void*)&spin, (void*)&withhold,
+                    (void*)&brev};
     const int threads = ((sw.bW + 63) / 64 + 1) * 64;
     FVG_HIP(launch_coresident(c, fn, (unsigned)sw.nblk, (unsigned)threads, args, blocked_lds(sw)));
     if (scatter)
@@ -1643,6 +1769,85 @@ static int apply_factor(fvg_ctx* c, const fvg_prec* p, const double* lo_scale, c
     return sweep(c, p->upper, p->d_inv_diag, p->d_tmp, z, p->d_flags + 3, skip);
 }
 
+// The matrix with rows and columns in the lower layout's position order, each row's entries
+// in the original (ascending original column) order, so a product row sums the same terms
+// in the same order as the original matrix's row.
+static int build_pos_matrix(fvg_ctx* c, const fvg_prec* p) {
+    const int64_t n = p->n;
+    std::vector<int32_t> row((size_t)n), pos((size_t)n);
+    FVG_TRY(download(c, row.data(), p->lower.brow, n));
+    for (int64_t q = 0; q < n; ++q) pos[(size_t)row[(size_t)q]] = (int32_t)q;
+    std::vector<int32_t> prp((size_t)n + 1, 0), pcol((size_t)p->nnz), psrc((size_t)p->nnz);
+    for (int64_t q = 0; q < n; ++q) {
+        const int32_t i = row[(size_t)q];
+        int32_t o = prp[(size_t)q];
+        for (int32_t k = p->h_rp[(size_t)i]; k < p->h_rp[(size_t)i + 1]; ++k, ++o) {
+            pcol[(size_t)o] = pos[(size_t)p->h_col[(size_t)k]];
+            psrc[(size_t)o] = k;
+        }
+        prp[(size_t)q + 1] = o;
+    }
+    auto* pa = new fvg_csr();
+    pa->nrows = pa->ncols = n;
+    pa->nnz = p->nnz;
+    p->pa = pa;
+    int s;
+    if ((s = dev_alloc_n(c, n + 1, &pa->row_ptr)) || (s = dev_alloc_n(c, p->nnz, &pa->col)) ||
+        (s = dev_alloc_n(c, p->nnz, &pa->val)) || (s = dev_alloc_n(c, p->nnz, &p->pa_src)) ||
+        (s = upload(c, pa->row_ptr, prp.data(), n + 1)) || (s = upload(c, pa->col, pcol.data(), p->nnz)) ||
+        (s = upload(c, p->pa_src, psrc.data(), p->nnz))) {
+        (void)hipStreamSynchronize(c->stream);
+        free_pos_matrix(c, p);
+        return s;
+    }
+    FVG_HIP(hipStreamSynchronize(c->stream));   // the host arrays
+    return FVG_OK;
+}
+
+int pos_solve(fvg_ctx* c, const fvg_prec* p, const fvg_csr* a, bool allow_alloc, PosSolve* out, bool* ok) {
+    *ok = false;
+    if (c->tune_solve_order != 0 || (p->kind != kIC && p->kind != kILU)) return FVG_OK;
+    const Sweep& lo = p->lower;
+    const Sweep& up = p->upper;
+    if (!up.mirrored || !lo.bpolled || !up.bpolled || c->tune_sweep_mode != 0 || !blocked_fits(c, lo) ||
+        !blocked_fits(c, up))
+        return FVG_OK;
+    if (a->nrows != p->n || a->ncols != p->n || a->nnz != p->nnz) return FVG_OK;
+    if (p->pa_of != a) {   // a matrix met for the first time: the pattern must be p's
+        if (a->h_row_ptr != p->h_rp || a->h_col != p->h_col) return FVG_OK;
+        if (!p->pa) {
+            if (!allow_alloc) return FVG_OK;
+            FVG_TRY(build_pos_matrix(c, p));
+        }
+        p->pa_of = a;
+        p->pa_ver = ~0ull;
+    }
+    if (p->pa_ver != a->version) {
+        hipLaunchKernelGGL(k_gather_values, dim3(grid_for(p->nnz)), dim3(kBlock), 0, c->stream, p->nnz, p->pa_src,
+                           (const double*)a->val, p->pa->val);
+        FVG_HIP(hipGetLastError());
+        p->pa_ver = a->version;
+    }
+    out->n = p->n;
+    out->row = lo.brow;
+    out->lo_x = lo.xpos;
+    out->up_x = up.xpos;
+    out->lo_poll = lo.bpolled;
+    out->up_poll = up.bpolled;
+    out->pa = p->pa;
+    *ok = true;
+    return FVG_OK;
+}
+
+int apply_pos(fvg_ctx* c, const fvg_prec* p, const double* r_pos, const int32_t* skip) {
+    const Sweep& lo = p->lower;
+    const Sweep& up = p->upper;
+    const double* lo_scale = p->kind == kIC ? p->d_inv_diag : nullptr;
+    FVG_TRY(launch_blocked(c, lo, lo_scale, nullptr, nullptr, p->d_flags + 3, skip, nullptr, false, false, r_pos, 0));
+    return launch_blocked(c, up, p->d_inv_diag, nullptr, nullptr, p->d_flags + 3, skip, nullptr, false, false, lo.xpos,
+                          1);
+}
+
 int apply_prec(fvg_ctx* c, const fvg_prec* p, const double* r, double* z, const int32_t* skip) {
     switch (p->kind) {
     case kIdentity:
@@ -1674,7 +1879,8 @@ static int create_factored(fvg_ctx* c, const fvg_csr* a, int32_t level, int kind
     if (s == FVG_OK) s = dev_alloc_n(c, p->nnz, &p->d_fac);
     if (s == FVG_OK) s = dev_alloc_n(c, n, &p->d_tmp);
     if (s == FVG_OK) s = build_sweep(c, p->h_rp, p->h_col, n, false, nullptr, p->lower);
-    if (s == FVG_OK) s = build_sweep(c, p->h_rp, p->h_col, n, true, kind == kIC ? &p->h_tpos : nullptr, p->upper);
+    if (s == FVG_OK)
+        s = build_sweep(c, p->h_rp, p->h_col, n, true, kind == kIC ? &p->h_tpos : nullptr, p->upper, &p->lower);
     if (s == FVG_OK) s = pair_layouts(c, p->lower, p->upper);
     if (s == FVG_OK) s = launch_fill(c, p->nnz, p->d_fac, 0.0);
     if (s == FVG_OK) s = factorize(c, p, a);
@@ -1752,7 +1958,8 @@ static int refactor(fvg_ctx* c, fvg_prec* p, const fvg_csr* a, bool check) {
 int fvg_prec_sweep_layout(const fvg_prec* p, int32_t* info, int32_t count) {
     if (!p || (count > 0 && !info)) return fail(FVG_E_INVALID, "null argument");
     const int64_t v[] = {p->lower.nblk, p->upper.nblk, p->lower.nlevels, p->lower.bW, p->lower.bX,
-                         p->lower.nbx,  p->upper.bW,   p->upper.bX,       p->lower.bK, p->upper.bK};
+                         p->lower.nbx,  p->upper.bW,   p->upper.bX,       p->lower.bK, p->upper.bK,
+                         p->upper.mirrored ? 1 : 0};
     for (int k = 0; k < count && k < (int)(sizeof(v) / sizeof(v[0])); ++k)
         info[k] = (int32_t)std::min<int64_t>(v[k], INT32_MAX);
     return FVG_OK;

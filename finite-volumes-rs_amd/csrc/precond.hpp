@@ -12,6 +12,15 @@ namespace fvg {
 // level padded to a whole number of kChunk-row chunks (padding rows are -1), so a
 // level is a contiguous range of chunks and its rows' entries are contiguous in HBM.
 constexpr int kChunk = 64;              // rows per wave64 chunk
+
+// Block and level of every row of a blocked sweep layout (host).
+struct BlockAssign {
+    int32_t nb = 0;                     // blocks
+    int nlevels = 0;                    // levels (more than the sweep's when delayed)
+    int kmax = 0;                       // dependency slots per row (4 or 8)
+    std::vector<int32_t> blk, lev;      // per row
+};
+
 struct Sweep {
     int64_t n = 0, npos = 0, nnz = 0;
     int nlevels = 0;
@@ -60,6 +69,13 @@ struct Sweep {
     // solver can write the sweep's right-hand side in position order itself.
     int32_t* bposof = nullptr;
     std::vector<int32_t> h_brow;        // row of each position (host copy, freed once paired)
+    BlockAssign h_assign;               // the blocked layout's assignment (host, freed once paired)
+    // Mirrored upper layout: the lower layout's blocks, levels and rows reversed, so upper
+    // position q holds the row of lower position n - 1 - q (structurally symmetric patterns).
+    bool mirrored = false;
+    // per position: 1 if some block polls it (it must hold the pending pattern before a
+    // sweep); lower layouts of a mirrored pair only
+    uint8_t* bpolled = nullptr;
 };
 constexpr int kBlockRing = 3;           // levels of values a block keeps in LDS
 constexpr uint16_t kNoSlot = 0xFFFF;    // an unused dependency slot of the blocked layout
@@ -104,6 +120,12 @@ struct fvg_prec {
     fvg::Tri32 jlo, jup;                // FP32 strict factors in natural row order (inexact apply)
     uint64_t fac_version = 0, fac32_version = 0;
     void* team = nullptr;               // batched-solve plan of this system (team.hip), made on use
+    // the matrix in the lower layout's position order (position-order solves; made on use)
+    // (a cache: made and refreshed by solves that take the preconditioner as const)
+    mutable fvg_csr* pa = nullptr;
+    mutable int32_t* pa_src = nullptr;          // slot of the original matrix of each of its entries
+    mutable const fvg_csr* pa_of = nullptr;     // the matrix whose pattern was checked and whose values it holds
+    mutable uint64_t pa_ver = ~0ull;            // version of those values
     uint64_t pattern_id = 0;            // mesh pattern the analysis was made for (0: not a mesh pattern)
 };
 
@@ -120,6 +142,27 @@ int apply_prec32(fvg_ctx* ctx, const fvg_prec* p, const float* rhs, float* solut
 // frees the batched-solve plan cached on p (team.hip)
 void team_plan_release(fvg_ctx* ctx, fvg_prec* p);
 
+// The FP64 solve in the lower sweep's position order (IC / ILU with a mirrored pair of
+// blocked layouts): the solver's vectors are indexed by lower position; the lower sweep
+// reads its right-hand side from a position-order vector, the upper sweep reads the lower
+// one's values in reverse, and the result sits reversed in up_x (z at lower position q is
+// up_x[n - 1 - q]).  Before each apply the caller writes the pending pattern where polled:
+// lo_x[q] where lo_poll[q], up_x[q] where up_poll[q].
+struct PosSolve {
+    int64_t n = 0;
+    const int32_t* row = nullptr;       // row of each lower position
+    double* lo_x = nullptr;
+    double* up_x = nullptr;
+    const uint8_t* lo_poll = nullptr;
+    const uint8_t* up_poll = nullptr;
+    const fvg_csr* pa = nullptr;        // the matrix in position order
+};
+// *ok: whether p preconditions a this way (same pattern, layouts mirrored, blocked sweeps
+// selected).  Makes the position-order matrix if needed (when allow_alloc) and refreshes its
+// values when a's have changed.
+int pos_solve(fvg_ctx* ctx, const fvg_prec* p, const fvg_csr* a, bool allow_alloc, PosSolve* out, bool* ok);
+// z = M^-1 r, r in position order: both sweeps (no host synchronisation)
+int apply_pos(fvg_ctx* ctx, const fvg_prec* p, const double* r_pos, const int32_t* skip);
 // pending value bits of an FP64 sweep output
 constexpr uint64_t kPendingBits = 0x7FF4DEADBEEF0F64ull;
 }  // namespace fvg

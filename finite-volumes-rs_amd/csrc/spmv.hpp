@@ -31,6 +31,8 @@ struct SpmvEpilogue {
     int64_t pstride = 0;
     // the dots as double-double pairs (put_dd / sum_partials_dd layout) rather than doubles
     bool dd = false;
+    // x read in reverse: x[xrev - 1 - column] (a vector kept reversed; 0: as stored)
+    int64_t xrev = 0;
 };
 
 // Number of partial sums one fused SpMV writes per dot (one per workgroup).

@@ -98,7 +98,12 @@ enum {
     /* Test-only: 1 makes the triangular sweeps keep every solved value from the other
      * workgroups and waves that wait on it, so every such wait runs to its spin limit and
      * the call returns FVG_E_TIMEOUT, while all kernels still exit normally. */
-    FVG_TUNE_SWEEP_WITHHOLD = 10
+    FVG_TUNE_SWEEP_WITHHOLD = 10,
+    /* FP64 fvg_cg / fvg_bicgstab with IC(0) / ILU(0) whose two sweeps are blocked and
+     * mirrored (structurally symmetric patterns): 0 (default) iterates with every vector in
+     * the lower sweep's position order, so no kernel reorders a vector for the sweeps; 1
+     * iterates in natural row order.  The same iteration arithmetic element by element. */
+    FVG_TUNE_SOLVE_ORDER = 11
 };
 int fvg_ctx_set_tuning(fvg_ctx* ctx, int32_t knob, int64_t value);
 int fvg_ctx_get_tuning(const fvg_ctx* ctx, int32_t knob, int64_t* value);
@@ -338,12 +343,13 @@ int fvg_ic_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out)
  * with its drop rule |v| <= 1e-6 |u_ii| (:255-258).  level must be 0. */
 int fvg_ilu_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out);
 /* Diagnostics (no reference counterpart): the shape of the blocked sweep layouts, the
- * first `count` (at most 10) of: blocks of the lower and of the upper factor (0: not
+ * first `count` (at most 11) of: blocks of the lower and of the upper factor (0: not
  * built, the persistent sweep serves), dependency levels of the lower sweep, the lower
  * layout's most rows of one block in one level and most values one block fetches from
  * outside for one level, its total number of such fetches per sweep, the upper layout's
- * most rows and most fetches, then the dependency slots per row of the lower and of the
- * upper layout (4 or 8). */
+ * most rows and most fetches, the dependency slots per row of the lower and of the
+ * upper layout (4 or 8), and 1 if the upper layout is the lower one mirrored (then FP64
+ * solves iterate in the lower layout's position order, FVG_TUNE_SOLVE_ORDER). */
 int fvg_prec_sweep_layout(const fvg_prec* prec, int32_t* info, int32_t count);
 /* Re-run the numeric factorisation for new values on the same pattern. */
 int fvg_prec_refactor(fvg_ctx* ctx, fvg_prec* p, const fvg_csr* a);

@@ -384,3 +384,48 @@ def test_bicgstab_restart_branch(ctx):
         assert np.abs(xb.numpy() - np.linalg.solve(A, b)).max() < 1e-12
     finally:
         batch.close()
+
+
+@pytest.mark.parametrize("name,kind,solver,prec", [("tet10", "sym", "cg", "ic"), ("prism24", "nonsym", "bicgstab", "ilu")])
+def test_position_order_solve_matches_natural_order(ctx, name, kind, solver, prec):
+    """With both sweep layouts blocked and the upper one the lower one mirrored (symmetric
+    patterns), FP64 solves iterate with every vector in the lower sweep's position order
+    (FVG_TUNE_SOLVE_ORDER 0, the default) and launch one kernel per triangular solve; in
+    natural order (1) each solve also gathers its right-hand side (and the last scatters).
+    Element by element the iteration arithmetic is the same and the dot products are
+    accumulated in double-double, so both give the same iterations and x to 1e-12, and
+    both meet the oracle's bars."""
+    rp, ci, va, b = _oracle_system(name, kind)
+    n = rp.shape[0] - 1
+    P = O.OraclePrec(prec, rp, ci, va, 0)
+    A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+    M = (fvg.IncompleteCholesky if prec == "ic" else fvg.IncompleteLowerUpper).from_matrix(A, 0)
+    lay = M.sweep_layout()
+    assert lay["lower_blocks"] > 0 and lay["upper_blocks"] > 0 and lay["mirrored"] == 1
+    B = fvg.DistributedVector.from_data(ctx, b)
+    opts = fvg.LinearSolverOptions(1e-8, 0.0, 2000)
+    fn = fvg.conjugate_gradient if solver == "cg" else fvg.bi_conjugate_gradient_stab
+    out = {}
+    try:
+        for order in (1, 0):
+            ctx.set_tuning(fvg.TUNE_SOLVE_ORDER, order)
+            x = fvg.DistributedVector.from_data(ctx, np.full(n, 0.5))
+            ctx.enable_timing(True)
+            ctx.kernel_stats(reset=True)
+            info = fn(x, A, B, M, opts)
+            st = ctx.kernel_stats(reset=True)
+            ctx.enable_timing(False)
+            out[order] = (x.numpy(), info, st)
+    finally:
+        ctx.set_tuning(fvg.TUNE_SOLVE_ORDER, 0)
+        ctx.enable_timing(False)
+    (xn, inat, snat), (xp, ipos, spos) = out[1], out[0]
+    xo, io = O.solve(solver, rp, ci, va, P, b, x0=np.full(n, 0.5), rtol=1e-8, atol=0.0, max_iter=2000)
+    for xg, ig in ((xn, inat), (xp, ipos)):
+        assert abs(ig.iterations - io["iterations"]) <= 2, (ig.iterations, io["iterations"])
+        assert np.linalg.norm(xg - xo) <= 1e-6 * np.linalg.norm(xo)
+        assert ig.final_residual <= 1e-8 * ig.initial_residual
+    assert ipos.iterations == inat.iterations and ipos.restarts == inat.restarts
+    assert np.linalg.norm(xp - xn) <= 1e-12 * np.linalg.norm(xn)
+    assert spos["sweep_launches"] == spos["sweep_calls"]          # one kernel per triangular solve
+    assert snat["sweep_launches"] > snat["sweep_calls"]
