@@ -113,10 +113,24 @@ int read_scalars(fvg_ctx* ctx, const double* dev, double* host, int count) {
     return FVG_OK;
 }
 
+// Timing records an event pair around one launch group in every `timing_every` groups
+// (events reused from a pool), so that the markers do not slow the stream they measure.
 int timing_begin(fvg_ctx* ctx) {
     if (!ctx->timing) return -1;
+    if (ctx->timing_skip > 0) {
+        ctx->timing_skip -= 1;
+        return -1;
+    }
+    ctx->timing_skip = ctx->timing_every - 1;
     fvg_ctx::Span s{};
-    if (hipEventCreate(&s.a) != hipSuccess || hipEventCreate(&s.b) != hipSuccess) return -1;
+    for (hipEvent_t* e : {&s.a, &s.b}) {
+        if (!ctx->event_pool.empty()) {
+            *e = ctx->event_pool.back();
+            ctx->event_pool.pop_back();
+        } else if (hipEventCreate(e) != hipSuccess) {
+            return -1;
+        }
+    }
     (void)hipEventRecord(s.a, ctx->stream);
     ctx->spans.push_back(s);
     return (int)ctx->spans.size() - 1;
@@ -146,8 +160,8 @@ static int collect_spans(fvg_ctx* ctx) {
                 ctx->stats.sweep_calls += 1;
             }
         }
-        (void)hipEventDestroy(s.a);
-        (void)hipEventDestroy(s.b);
+        ctx->event_pool.push_back(s.a);
+        ctx->event_pool.push_back(s.b);
     }
     ctx->spans.clear();
     return FVG_OK;
@@ -226,6 +240,11 @@ int fvg_ctx_create(int device, uint64_t budget, fvg_ctx** out) {
     if (s == FVG_OK) s = dev_alloc_n(c, 64, &c->scalar);
     if (s == FVG_OK && hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault) != hipSuccess)
         s = fail(FVG_E_HIP, "hipHostMalloc failed");
+    if (s == FVG_OK && hipHostMalloc((void**)&c->poll_host, 64, hipHostMallocDefault) != hipSuccess)
+        s = fail(FVG_E_HIP, "hipHostMalloc failed");
+    for (int k = 0; s == FVG_OK && k < 2; ++k)
+        if (hipEventCreateWithFlags(&c->poll_evt[k], hipEventDisableTiming) != hipSuccess)
+            s = fail(FVG_E_HIP, "hipEventCreate failed");
     if (s != FVG_OK) {
         fvg_ctx_destroy(c);
         return s;
@@ -245,9 +264,13 @@ int fvg_ctx_destroy(fvg_ctx* c) {
         (void)hipEventDestroy(s.a);
         (void)hipEventDestroy(s.b);
     }
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     dev_free_n(c, c->partials, c->partials_cap * kPartialSlots);
     dev_free_n(c, c->scalar, 64);
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->poll_host) (void)hipHostFree(c->poll_host);
+    for (hipEvent_t e : c->poll_evt)
+        if (e) (void)hipEventDestroy(e);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->jobs) dev_free(c, c->jobs, c->jobs_cap);
     if (c->jobs_host) (void)hipHostFree(c->jobs_host);
@@ -356,8 +379,10 @@ int fvg_sweep_launch_rule(int64_t tune, int32_t live_contexts, int32_t batches_i
 }
 
 int fvg_ctx_enable_timing(fvg_ctx* c, int enable) {
-    if (!c) return fail(FVG_E_INVALID, "null context");
+    if (!c || enable < 0) return fail(FVG_E_INVALID, "bad timing argument");
     c->timing = enable != 0;
+    c->timing_every = enable > 0 ? enable : 1;
+    c->timing_skip = 0;
     return FVG_OK;
 }
 

@@ -49,6 +49,8 @@ struct fvg_ctx {
     int64_t partials_cap = 0;
     double* scalar = nullptr;          // small device scalar area
     void* pinned = nullptr;            // pinned host staging (4 KiB)
+    int32_t* poll_host = nullptr;      // pinned: the two slots of the solvers' done-flag polls
+    hipEvent_t poll_evt[2] = {nullptr, nullptr};   // recorded after each slot's copy
     void* stage = nullptr;             // pinned host staging for vector transfers (grown on demand)
     size_t stage_cap = 0;
     void* jobs = nullptr;              // device table of multi-system launches (grown on demand)
@@ -58,6 +60,9 @@ struct fvg_ctx {
     bool timing = false;
     struct Span { hipEvent_t a, b; int kind; int64_t launches; };
     std::vector<Span> spans;           // event pairs recorded while timing is enabled
+    std::vector<hipEvent_t> event_pool;   // events of collected spans, reused
+    int timing_every = 1;              // time one launch group in this many
+    int timing_skip = 0;               // groups to skip before the next timed one
     fvg_kernel_stats stats{};
     int64_t tune_sweep_mode = 0;       // FVG_TUNE_SWEEP_MODE
     int64_t tune_sweep_waves = 0;      // FVG_TUNE_SWEEP_WAVES

@@ -472,6 +472,22 @@ int KRun::finish(fvg_solver_info* infos, double* const* histories, int64_t cap) 
 // iterations enqueued between two polls of the "all systems done" flag
 constexpr int kPollBatch = 4;
 
+// Poll number `batch` (1, 2, ...) of a solve: enqueues the copy of the done flag into the
+// context's pinned slot batch & 1 and records the slot's event, then reads the flag of
+// the poll before (the host waits for that one only).  So one batch of iterations is
+// always queued behind the one the host waits for, and the device never idles on the
+// host; a solve that converges runs at most one batch of iterations that skip at once.
+int KRun::poll(int64_t batch, bool* done) {
+    const int slot = (int)(batch & 1);
+    FVG_HIP(hipMemcpyAsync(c->poll_host + slot, flags + 1, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    FVG_HIP(hipEventRecord(c->poll_evt[slot], c->stream));
+    *done = false;
+    if (batch < 2) return FVG_OK;
+    FVG_HIP(hipEventSynchronize(c->poll_evt[slot ^ 1]));
+    *done = __atomic_load_n(c->poll_host + (slot ^ 1), __ATOMIC_ACQUIRE) != 0;
+    return FVG_OK;
+}
+
 // work vectors and partials: the caller's (kb), or allocated here and the context's
 static int solve_buffers(fvg_ctx* c, int64_t n, int count, const KBuffers* kb, Workspace& ws, double** v,
                          double** part, int64_t* stride) {
@@ -539,7 +555,7 @@ static int run_cg_pos(fvg_ctx* c, const fvg_prec* p, const double* b, double* x,
         FVG_HIP(hipGetLastError());
         if ((it + 1) % kPollBatch == 0) {
             bool d;
-            FVG_TRY(K.all_done(&d));
+            FVG_TRY(K.poll((it + 1) / kPollBatch, &d));
             if (d) break;
         }
     }
@@ -611,7 +627,7 @@ static int run_bicgstab_pos(fvg_ctx* c, const fvg_prec* p, const double* b, doub
         FVG_HIP(hipGetLastError());
         if ((it + 1) % kPollBatch == 0) {
             bool d;
-            FVG_TRY(K.all_done(&d));
+            FVG_TRY(K.poll((it + 1) / kPollBatch, &d));
             if (d) break;
         }
     }
@@ -682,7 +698,7 @@ int run_cg_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const double* 
         FVG_HIP(hipGetLastError());
         if ((it + 1) % kPollBatch == 0) {
             bool d;
-            FVG_TRY(K.all_done(&d));
+            FVG_TRY(K.poll((it + 1) / kPollBatch, &d));
             if (d) break;
         }
     }
@@ -761,7 +777,7 @@ int run_bicgstab_multi(fvg_ctx* c, const fvg_csr* a, const fvg_prec* p, const do
         FVG_HIP(hipGetLastError());
         if ((it + 1) % kPollBatch == 0) {
             bool d;
-            FVG_TRY(K.all_done(&d));
+            FVG_TRY(K.poll((it + 1) / kPollBatch, &d));
             if (d) break;
         }
     }

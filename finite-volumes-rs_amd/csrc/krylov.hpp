@@ -29,6 +29,8 @@ struct KRun {
     // ninst states, ninst * (max_iterations + 2) history slots, 4 flags
     int init_at(fvg_ctx* ctx, int ninst, const fvg_solver_options* o, KState* st, double* hist, int32_t* flags);
     int all_done(bool* out);
+    // poll number `batch` of the loop (see krylov.hip): *done is the flag of the poll before
+    int poll(int64_t batch, bool* done);
     int finish(fvg_solver_info* infos, double* const* histories, int64_t cap);
     ~KRun();
 };

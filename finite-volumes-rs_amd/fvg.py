@@ -264,8 +264,9 @@ class Context:
         check(lib().fvg_ctx_sweep_launch_state(self.h, C.byref(mode), C.byref(busy)))
         return mode.value, busy.value
 
-    def enable_timing(self, on=True):
-        check(lib().fvg_ctx_enable_timing(self.h, 1 if on else 0))
+    def enable_timing(self, on=True, every=1):
+        """HIP-event timing of SpMV / sweep launch groups; `every`: time one group in this many."""
+        check(lib().fvg_ctx_enable_timing(self.h, int(every) if on else 0))
 
     def kernel_stats(self, reset=True):
         s = KernelStats()

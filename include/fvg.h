@@ -504,7 +504,9 @@ typedef struct fvg_kernel_stats {
     int64_t spmv_calls;         /* SpMV products timed */
     int64_t sweep_calls;        /* whole triangular sweeps timed */
 } fvg_kernel_stats;
-/* Enable HIP-event timing of SpMV / sweep kernels on this context (adds events). */
+/* HIP-event timing of SpMV / sweep kernels on this context: 0 off; N > 0 times one of
+ * every N launch groups (an SpMV, a triangular solve) with an event pair on the context
+ * stream, so that sampled timing barely slows the stream (1: every group). */
 int fvg_ctx_enable_timing(fvg_ctx* ctx, int enable);
 int fvg_ctx_kernel_stats(fvg_ctx* ctx, fvg_kernel_stats* out, int reset);
 
