@@ -1147,10 +1147,10 @@ static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::
 // The upper layout mirrored from the blocked lower layout `lo`: block nb-1-b, level
 // L-1-l and rows descending where the lower has block b, level l and rows ascending, so
 // upper position q holds the row of lower position n-1-q.  A valid upper schedule when the
-// pattern is structurally symmetric: an upper dependency i -> j (j > i) is then the lower
-// dependency j -> i, one or more levels deep (two or more between blocks) in the lower
-// layout and so in its mirror.  Leaves up.nblk = 0 when the pattern is not symmetric or
-// the layout does not fit the kernel.
+// upper dependencies are the lower ones transposed (the caller checks): an upper
+// dependency i -> j (j > i) is then the lower dependency j -> i, one or more levels deep
+// (two or more between blocks) in the lower layout and so in its mirror.  Leaves
+// up.nblk = 0 when the layout does not fit the kernel.
 static int build_mirrored(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
                           const std::vector<int32_t>* src_map, const Sweep& lo, Sweep& up) {
     const BlockAssign& L = lo.h_assign;
@@ -1160,12 +1160,7 @@ static int build_mirrored(fvg_ctx* c, const std::vector<int32_t>& rp, const std:
         int cnt = 0;
         for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
             const int32_t j = col[(size_t)k];
-            if (j >= n) continue;
-            if (j > i) ++cnt;
-            // structural symmetry: (j, i) is in the pattern too
-            auto b = col.begin() + rp[(size_t)j], e = col.begin() + rp[(size_t)j + 1];
-            auto it = std::lower_bound(b, e, (int32_t)i);
-            if (it == e || *it != (int32_t)i) return FVG_OK;
+            if (j > i && j < n) ++cnt;
         }
         kmax = std::max(kmax, cnt);
     }
@@ -1189,9 +1184,11 @@ static int build_mirrored(fvg_ctx* c, const std::vector<int32_t>& rp, const std:
     return FVG_OK;
 }
 
-// upper=false: entries j < i; upper=true: entries i < j < n.  src_map (optional)
-// maps a matrix slot to the factor slot holding that entry's value.  An upper sweep with
-// `mirror` (the factor's blocked lower layout) takes the mirrored layout where it can.
+// upper=false: entries j < i; upper=true: entries i < j < n, each row's entries taken in
+// their stored order.  src_map (optional) maps a pattern slot to the factor slot holding
+// that entry's value.  An upper sweep with `mirror` (the factor's blocked lower layout,
+// whose dependencies the caller has checked are these transposed) takes the mirrored
+// layout where it can.
 static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
                        bool upper, const std::vector<int32_t>* src_map, Sweep& sw, const Sweep* mirror = nullptr) {
     std::vector<int32_t> lev((size_t)n, 0);
@@ -1322,9 +1319,9 @@ __global__ void k_gather_values(int64_t nnz, const int32_t* __restrict__ src, co
 static void free_pos_matrix(fvg_ctx* c, const fvg_prec* p) {
     if (!p->pa) return;
     dev_free_n(c, p->pa->row_ptr, p->n + 1);
-    dev_free_n(c, p->pa->col, p->nnz);
-    dev_free_n(c, p->pa->val, p->nnz);
-    dev_free_n(c, p->pa_src, p->nnz);
+    dev_free_n(c, p->pa->col, p->a_nnz);
+    dev_free_n(c, p->pa->val, p->a_nnz);
+    dev_free_n(c, p->pa_src, p->a_nnz);
     delete p->pa;
     p->pa = nullptr;
     p->pa_of = nullptr;
@@ -1336,7 +1333,9 @@ static void free_prec(fvg_ctx* c, fvg_prec* p) {
     dev_free_n(c, p->d_col, p->nnz);
     dev_free_n(c, p->d_diag, p->n);
     dev_free_n(c, p->d_tpos, p->nnz);
-    dev_free_n(c, p->d_fac, p->nnz);
+    dev_free_n(c, p->d_fac, p->nnz + 1);
+    dev_free_n(c, p->d_amap, p->a_nnz);
+    dev_free_n(c, p->d_afill, p->nnz);
     dev_free_n(c, p->d_inv_diag, p->n);
     dev_free_n(c, p->d_tmp, p->n);
     dev_free_n(c, p->d_flags, 4);
@@ -1351,13 +1350,12 @@ static void free_prec(fvg_ctx* c, fvg_prec* p) {
 }
 
 // pattern, diagonal slots and device copies shared by every kind
-static int setup_pattern(fvg_ctx* c, const fvg_csr* a, fvg_prec* p, bool need_transpose) {
-    const int64_t n = a->nrows;
+static int setup_pattern(fvg_ctx* c, int64_t n, std::vector<int32_t> rp, std::vector<int32_t> col, fvg_prec* p,
+                         bool need_transpose) {
     p->n = n;
-    p->pattern_id = a->pattern_id;
-    p->nnz = a->nnz;
-    p->h_rp = a->h_row_ptr;
-    p->h_col = a->h_col;
+    p->nnz = (int64_t)col.size();
+    p->h_rp = std::move(rp);
+    p->h_col = std::move(col);
     p->h_diag.assign((size_t)n, -1);
     for (int64_t i = 0; i < n; ++i) {
         auto b = p->h_col.begin() + p->h_rp[(size_t)i], e = p->h_col.begin() + p->h_rp[(size_t)i + 1];
@@ -1404,11 +1402,14 @@ static int read_flags(fvg_ctx* c, const fvg_prec* p, int32_t* host4) {
 // numeric factorisation for the current values of a (pattern already analysed).
 // check: read the symmetry / pivot flags and fail on them (synchronises); otherwise the
 // whole factorisation is only enqueued and the flags are read by fvg_prec_status.
+static const double* factor_input(fvg_ctx* c, const fvg_prec* p, const fvg_csr* a);
+
 static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a, bool check = true) {
     FVG_HIP(hipMemsetAsync(p->d_flags, 0, 4 * sizeof(int32_t), c->stream));
     const Sweep& lo = p->lower;
+    const double* av = factor_input(c, p, a);   // values on the factor pattern
     if (p->kind == kIC) {
-        hipLaunchKernelGGL(k_sym_check, dim3(grid_for(p->nnz)), dim3(kBlock), 0, c->stream, p->nnz, p->d_tpos, a->val,
+        hipLaunchKernelGGL(k_sym_check, dim3(grid_for(p->nnz)), dim3(kBlock), 0, c->stream, p->nnz, p->d_tpos, av,
                            p->d_flags);
         if (check) {
             int32_t fl[4];
@@ -1418,13 +1419,13 @@ static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a, bool check = tru
         for (int l = 0; l < lo.nlevels; ++l) {
             const int64_t b = lo.lvl_ptr[(size_t)l], e = lo.lvl_ptr[(size_t)l + 1];
             hipLaunchKernelGGL(k_ic0_level, dim3(grid_for(e - b)), dim3(kBlock), 0, c->stream, b, e, lo.rows, p->d_rp,
-                               p->d_col, p->d_diag, a->val, p->d_fac, p->d_inv_diag, p->d_flags);
+                               p->d_col, p->d_diag, av, p->d_fac, p->d_inv_diag, p->d_flags);
         }
     } else {
         for (int l = 0; l < lo.nlevels; ++l) {
             const int64_t b = lo.lvl_ptr[(size_t)l], e = lo.lvl_ptr[(size_t)l + 1];
             hipLaunchKernelGGL(k_ilu0_level, dim3(grid_for(e - b)), dim3(kBlock), 0, c->stream, b, e, lo.rows,
-                               p->d_rp, p->d_col, p->d_diag, a->val, p->d_fac, p->d_flags);
+                               p->d_rp, p->d_col, p->d_diag, av, p->d_fac, p->d_flags);
         }
         hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_diag, p->d_fac,
                            p->d_inv_diag);
@@ -1773,29 +1774,31 @@ static int apply_factor(fvg_ctx* c, const fvg_prec* p, const double* lo_scale, c
 // in the original (ascending original column) order, so a product row sums the same terms
 // in the same order as the original matrix's row.
 static int build_pos_matrix(fvg_ctx* c, const fvg_prec* p) {
-    const int64_t n = p->n;
+    const int64_t n = p->n, nnz = p->a_nnz;
+    const auto& arp = p->level ? p->h_arp : p->h_rp;
+    const auto& acol = p->level ? p->h_acol : p->h_col;
     std::vector<int32_t> row((size_t)n), pos((size_t)n);
     FVG_TRY(download(c, row.data(), p->lower.brow, n));
     for (int64_t q = 0; q < n; ++q) pos[(size_t)row[(size_t)q]] = (int32_t)q;
-    std::vector<int32_t> prp((size_t)n + 1, 0), pcol((size_t)p->nnz), psrc((size_t)p->nnz);
+    std::vector<int32_t> prp((size_t)n + 1, 0), pcol((size_t)nnz), psrc((size_t)nnz);
     for (int64_t q = 0; q < n; ++q) {
         const int32_t i = row[(size_t)q];
         int32_t o = prp[(size_t)q];
-        for (int32_t k = p->h_rp[(size_t)i]; k < p->h_rp[(size_t)i + 1]; ++k, ++o) {
-            pcol[(size_t)o] = pos[(size_t)p->h_col[(size_t)k]];
+        for (int32_t k = arp[(size_t)i]; k < arp[(size_t)i + 1]; ++k, ++o) {
+            pcol[(size_t)o] = pos[(size_t)acol[(size_t)k]];
             psrc[(size_t)o] = k;
         }
         prp[(size_t)q + 1] = o;
     }
     auto* pa = new fvg_csr();
     pa->nrows = pa->ncols = n;
-    pa->nnz = p->nnz;
+    pa->nnz = nnz;
     p->pa = pa;
     int s;
-    if ((s = dev_alloc_n(c, n + 1, &pa->row_ptr)) || (s = dev_alloc_n(c, p->nnz, &pa->col)) ||
-        (s = dev_alloc_n(c, p->nnz, &pa->val)) || (s = dev_alloc_n(c, p->nnz, &p->pa_src)) ||
-        (s = upload(c, pa->row_ptr, prp.data(), n + 1)) || (s = upload(c, pa->col, pcol.data(), p->nnz)) ||
-        (s = upload(c, p->pa_src, psrc.data(), p->nnz))) {
+    if ((s = dev_alloc_n(c, n + 1, &pa->row_ptr)) || (s = dev_alloc_n(c, nnz, &pa->col)) ||
+        (s = dev_alloc_n(c, nnz, &pa->val)) || (s = dev_alloc_n(c, nnz, &p->pa_src)) ||
+        (s = upload(c, pa->row_ptr, prp.data(), n + 1)) || (s = upload(c, pa->col, pcol.data(), nnz)) ||
+        (s = upload(c, p->pa_src, psrc.data(), nnz))) {
         (void)hipStreamSynchronize(c->stream);
         free_pos_matrix(c, p);
         return s;
@@ -1812,9 +1815,10 @@ int pos_solve(fvg_ctx* c, const fvg_prec* p, const fvg_csr* a, bool allow_alloc,
     if (!up.mirrored || !lo.bpolled || !up.bpolled || c->tune_sweep_mode != 0 || !blocked_fits(c, lo) ||
         !blocked_fits(c, up))
         return FVG_OK;
-    if (a->nrows != p->n || a->ncols != p->n || a->nnz != p->nnz) return FVG_OK;
+    if (a->nrows != p->n || a->ncols != p->n || a->nnz != p->a_nnz) return FVG_OK;
     if (p->pa_of != a) {   // a matrix met for the first time: the pattern must be p's
-        if (a->h_row_ptr != p->h_rp || a->h_col != p->h_col) return FVG_OK;
+        if (a->h_row_ptr != (p->level ? p->h_arp : p->h_rp) || a->h_col != (p->level ? p->h_acol : p->h_col))
+            return FVG_OK;
         if (!p->pa) {
             if (!allow_alloc) return FVG_OK;
             FVG_TRY(build_pos_matrix(c, p));
@@ -1823,8 +1827,8 @@ int pos_solve(fvg_ctx* c, const fvg_prec* p, const fvg_csr* a, bool allow_alloc,
         p->pa_ver = ~0ull;
     }
     if (p->pa_ver != a->version) {
-        hipLaunchKernelGGL(k_gather_values, dim3(grid_for(p->nnz)), dim3(kBlock), 0, c->stream, p->nnz, p->pa_src,
-                           (const double*)a->val, p->pa->val);
+        hipLaunchKernelGGL(k_gather_values, dim3(grid_for(p->a_nnz)), dim3(kBlock), 0, c->stream, p->a_nnz,
+                           p->pa_src, (const double*)a->val, p->pa->val);
         FVG_HIP(hipGetLastError());
         p->pa_ver = a->version;
     }
@@ -1866,23 +1870,250 @@ int apply_prec(fvg_ctx* c, const fvg_prec* p, const double* r, double* z, const 
     return fail(FVG_E_INVALID, "unknown preconditioner");
 }
 
+// ---- incomplete factorisations with fill (level > 0) --------------------------------------
+// Row i's fill follows the reference's pattern walks (incomplete_cholesky.rs:35-85,
+// incomplete_lower_upper.rs:30-137): a depth-first walk from i over the matrix graph, a
+// stack of vertices, each vertex's columns taken in their stored order, every vertex met
+// at most once per row (marked on first meeting, even when not explored), and explored
+// only while the walk's length to it is below the level.
+struct Walk {
+    std::vector<int32_t> mark, length, stack;   // mark[j] == i: met in the walk of row i
+    explicit Walk(int64_t n) : mark((size_t)n, -1), length((size_t)n, 0) {}
+    // kind 0 (IC): every vertex met from a vertex whose length is below the level joins;
+    // 1 (ILU upper): vertices above i join, those below are explored while within the level;
+    // 2 (ILU lower): vertices below i join, those above are explored while within the level
+    template <int KIND>
+    void run(const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n, int32_t i, int level,
+             std::vector<int32_t>& out) {
+        out.clear();
+        stack.assign(1, i);
+        mark[(size_t)i] = i;
+        length[(size_t)i] = 0;
+        while (!stack.empty()) {
+            const int32_t k = stack.back();
+            stack.pop_back();
+            const int lk = length[(size_t)k];
+            for (int32_t kj = rp[(size_t)k]; kj < rp[(size_t)k + 1]; ++kj) {
+                const int32_t j = col[(size_t)kj];
+                if (j == k || j >= n) continue;   // the square block
+                if (mark[(size_t)j] == i) continue;
+                mark[(size_t)j] = i;
+                if (KIND == 0) {
+                    if (lk < level) {
+                        stack.push_back(j);
+                        length[(size_t)j] = lk + 1;
+                        out.push_back(j);
+                    }
+                } else if (KIND == 1) {
+                    if (j < i && lk < level) {
+                        stack.push_back(j);
+                        length[(size_t)j] = lk + 1;
+                    } else if (j > i) {
+                        out.push_back(j);
+                    }
+                } else {
+                    if (j > i && lk < level) {
+                        stack.push_back(j);
+                        length[(size_t)j] = lk + 1;
+                    } else if (j < i) {
+                        out.push_back(j);
+                    }
+                }
+            }
+        }
+        std::sort(out.begin(), out.end());
+    }
+};
+
+// The factor pattern of IC(level) / ILU(level) of a square matrix pattern (rp, col):
+// frp/fcol (rows sorted), amap (matrix slot -> factor slot, -1 for a column >= n), and for
+// IC the upper sweep's entries in the order of the reference's transposed pattern sp_t
+// (each row: the matrix's columns above the diagonal, then the fill's, each ascending) with
+// the factor slot of each (-1: an sp_t entry without a factor entry, coefficient 0).
+// FVG_E_MISSING_ENTRY where the reference's l_vt construction would fail (a factor entry
+// whose transpose is not in sp_t).
+struct FillPattern {
+    std::vector<int32_t> frp, fcol, amap, urp, ucol, usrc;
+    bool transposed = true;   // IC: the upper entries are exactly the lower ones transposed
+};
+
+static int fill_pattern(const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n, int kind,
+                        int level, FillPattern& F) {
+    std::vector<std::vector<int32_t>> rows((size_t)n);
+    std::vector<int32_t> s1, s2, arow;
+    if (kind == kIC) {
+        Walk w(n);
+        std::vector<std::vector<int32_t>> lrows((size_t)n), trows((size_t)n);
+        for (int64_t i = 0; i < n; ++i) {
+            w.run<0>(rp, col, n, (int32_t)i, level, s1);
+            arow.clear();
+            for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k)
+                if (col[(size_t)k] < n) arow.push_back(col[(size_t)k]);
+            auto& L = lrows[(size_t)i];
+            auto& T = trows[(size_t)i];
+            for (int32_t j : arow) {
+                if (j <= i) L.push_back(j);
+                if (j >= i) T.push_back(j);
+            }
+            for (int32_t j : s1) {
+                if (std::binary_search(arow.begin(), arow.end(), j)) continue;   // arow is sorted
+                if (j <= i) L.push_back(j);
+                if (j >= i) T.push_back(j);
+            }
+            std::sort(L.begin(), L.end());   // from_sparsity_and_values sorts the factor's rows
+        }
+        // the factor pattern: L and its transpose, rows sorted
+        for (int64_t i = 0; i < n; ++i)
+            for (int32_t j : lrows[(size_t)i]) {
+                rows[(size_t)i].push_back(j);
+                if (j != i) rows[(size_t)j].push_back((int32_t)i);
+            }
+        for (auto& r : rows) std::sort(r.begin(), r.end());
+        F.frp.assign((size_t)n + 1, 0);
+        for (int64_t i = 0; i < n; ++i) F.frp[(size_t)i + 1] = F.frp[(size_t)i] + (int32_t)rows[(size_t)i].size();
+        F.fcol.reserve((size_t)F.frp[(size_t)n]);
+        for (auto& r : rows) F.fcol.insert(F.fcol.end(), r.begin(), r.end());
+        auto slot = [&](int32_t i, int32_t j) -> int32_t {   // factor slot of (i, j), or -1
+            auto b = F.fcol.begin() + F.frp[(size_t)i], e = F.fcol.begin() + F.frp[(size_t)i + 1];
+            auto it = std::lower_bound(b, e, j);
+            return (it != e && *it == j) ? (int32_t)(it - F.fcol.begin()) : -1;
+        };
+        // every factor entry (r, c), c < r, needs r in sp_t row c
+        for (int64_t r = 0; r < n; ++r)
+            for (int32_t c : lrows[(size_t)r]) {
+                if (c == r) continue;
+                const auto& T = trows[(size_t)c];
+                if (std::find(T.begin(), T.end(), (int32_t)r) == T.end())
+                    return fail(FVG_E_MISSING_ENTRY, "IncompleteCholesky: transpose entry (" + std::to_string(c) +
+                                                         ", " + std::to_string(r) + ") missing from the fill pattern");
+            }
+        // the upper sweep: sp_t order, coefficient l_ji (the factor slot of (j, i)) or 0
+        F.urp.assign((size_t)n + 1, 0);
+        for (int64_t i = 0; i < n; ++i) {
+            for (int32_t j : trows[(size_t)i]) {
+                if (j <= i) continue;
+                const int32_t sl = std::binary_search(lrows[(size_t)j].begin(), lrows[(size_t)j].end(), (int32_t)i)
+                                       ? slot(j, (int32_t)i) : -1;
+                if (sl < 0) F.transposed = false;
+                F.ucol.push_back(j);
+                F.usrc.push_back(sl);
+            }
+            F.urp[(size_t)i + 1] = (int32_t)F.ucol.size();
+        }
+        // every factor entry above the diagonal is an sp_t entry (checked above), so the
+        // upper entries are the lower ones transposed unless sp_t has extra entries
+        int64_t lower_strict = 0;
+        for (const auto& L : lrows) lower_strict += (int64_t)L.size() - 1;
+        if ((int64_t)F.ucol.size() != lower_strict) F.transposed = false;
+    } else {
+        Walk wu(n), wl(n);
+        for (int64_t i = 0; i < n; ++i) {
+            wl.run<2>(rp, col, n, (int32_t)i, level, s1);
+            wu.run<1>(rp, col, n, (int32_t)i, level, s2);
+            auto& r = rows[(size_t)i];
+            r = s1;
+            r.push_back((int32_t)i);
+            r.insert(r.end(), s2.begin(), s2.end());
+        }
+        F.frp.assign((size_t)n + 1, 0);
+        for (int64_t i = 0; i < n; ++i) F.frp[(size_t)i + 1] = F.frp[(size_t)i] + (int32_t)rows[(size_t)i].size();
+        for (auto& r : rows) F.fcol.insert(F.fcol.end(), r.begin(), r.end());
+    }
+    // the matrix's slots in the factor pattern
+    F.amap.assign(col.size(), -1);
+    for (int64_t i = 0; i < n; ++i)
+        for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
+            const int32_t j = col[(size_t)k];
+            if (j >= n) continue;
+            auto b = F.fcol.begin() + F.frp[(size_t)i], e = F.fcol.begin() + F.frp[(size_t)i + 1];
+            auto it = std::lower_bound(b, e, j);
+            if (it != e && *it == j) F.amap[(size_t)k] = (int32_t)(it - F.fcol.begin());
+        }
+    return FVG_OK;
+}
+
+// whether every entry (i, j), j < n, of the pattern has (j, i) too
+static bool structurally_symmetric(const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n) {
+    for (int64_t i = 0; i < n; ++i)
+        for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
+            const int32_t j = col[(size_t)k];
+            if (j >= n) continue;
+            auto b = col.begin() + rp[(size_t)j], e = col.begin() + rp[(size_t)j + 1];
+            if (!std::binary_search(b, e, (int32_t)i)) return false;
+        }
+    return true;
+}
+
+// the factorisation's input: the matrix values, or (with fill) scattered into the factor
+// pattern with zeros elsewhere
+__global__ void k_scatter_values(int64_t nnz, const int32_t* __restrict__ amap, const double* __restrict__ a,
+                                 double* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnz) {
+        const int32_t t = amap[k];
+        if (t >= 0) out[t] = a[k];
+    }
+}
+
+static const double* factor_input(fvg_ctx* c, const fvg_prec* p, const fvg_csr* a) {
+    if (p->level == 0) return a->val;
+    (void)launch_fill(c, p->nnz, p->d_afill, 0.0);
+    hipLaunchKernelGGL(k_scatter_values, dim3(grid_for(p->a_nnz)), dim3(kBlock), 0, c->stream, p->a_nnz, p->d_amap,
+                       (const double*)a->val, p->d_afill);
+    return p->d_afill;
+}
+
+// the matrix a has the pattern p was analysed for
+static bool same_pattern(const fvg_prec* p, const fvg_csr* a) {
+    if (a->nrows != p->n || a->nnz != p->a_nnz) return false;
+    if (a->pattern_id != 0 && a->pattern_id == p->pattern_id) return true;
+    return a->h_col == (p->level == 0 ? p->h_col : p->h_acol);
+}
+
 static int create_factored(fvg_ctx* c, const fvg_csr* a, int32_t level, int kind, fvg_prec** out) {
     if (!c || !a || !out) return fail(FVG_E_INVALID, "null argument");
     *out = nullptr;
-    if (level != 0)
-        return fail(FVG_E_UNSUPPORTED, "only level 0 incomplete factorisations are implemented on the device");
+    if (level < 0) return fail(FVG_E_INVALID, "negative fill level");
     if (a->nrows <= 0) return fail(FVG_E_INVALID, "empty matrix");
+    const int64_t n = a->nrows;
     auto* p = new fvg_prec();
     p->kind = kind;
-    int s = setup_pattern(c, a, p, kind == kIC);
-    const int64_t n = a->nrows;
-    if (s == FVG_OK) s = dev_alloc_n(c, p->nnz, &p->d_fac);
+    p->level = level;
+    p->pattern_id = a->pattern_id;
+    p->a_nnz = a->nnz;
+    FillPattern F;
+    int s = FVG_OK;
+    if (level > 0) {
+        s = fill_pattern(a->h_row_ptr, a->h_col, n, kind, level, F);
+        p->h_arp = a->h_row_ptr;
+        p->h_acol = a->h_col;
+    }
+    if (s == FVG_OK)
+        s = level > 0 ? setup_pattern(c, n, F.frp, F.fcol, p, kind == kIC)
+                      : setup_pattern(c, n, a->h_row_ptr, a->h_col, p, kind == kIC);
+    if (s == FVG_OK && level > 0) {
+        if ((s = dev_alloc_n(c, p->a_nnz, &p->d_amap)) == FVG_OK && (s = dev_alloc_n(c, p->nnz, &p->d_afill)) == FVG_OK)
+            s = upload(c, p->d_amap, F.amap.data(), p->a_nnz);
+    }
+    // one slot past the factor values holds 0: the coefficient of sweep entries without a
+    // factor entry
+    if (s == FVG_OK) s = dev_alloc_n(c, p->nnz + 1, &p->d_fac);
     if (s == FVG_OK) s = dev_alloc_n(c, n, &p->d_tmp);
     if (s == FVG_OK) s = build_sweep(c, p->h_rp, p->h_col, n, false, nullptr, p->lower);
-    if (s == FVG_OK)
-        s = build_sweep(c, p->h_rp, p->h_col, n, true, kind == kIC ? &p->h_tpos : nullptr, p->upper, &p->lower);
+    if (s == FVG_OK) {
+        if (kind == kIC && level > 0) {
+            for (int32_t& v : F.usrc)
+                if (v < 0) v = (int32_t)p->nnz;
+            s = build_sweep(c, F.urp, F.ucol, n, true, &F.usrc, p->upper, F.transposed ? &p->lower : nullptr);
+        } else {
+            // IC(0): the upper entries are the lower ones transposed; ILU: when the pattern is symmetric
+            const bool tr = kind == kIC || structurally_symmetric(p->h_rp, p->h_col, n);
+            s = build_sweep(c, p->h_rp, p->h_col, n, true, kind == kIC ? &p->h_tpos : nullptr, p->upper,
+                            tr ? &p->lower : nullptr);
+        }
+    }
     if (s == FVG_OK) s = pair_layouts(c, p->lower, p->upper);
-    if (s == FVG_OK) s = launch_fill(c, p->nnz, p->d_fac, 0.0);
+    if (s == FVG_OK) s = launch_fill(c, p->nnz + 1, p->d_fac, 0.0);
     if (s == FVG_OK) s = factorize(c, p, a);
     if (s != FVG_OK) {
         (void)hipStreamSynchronize(c->stream);
@@ -1914,7 +2145,9 @@ int fvg_diagonal_create(fvg_ctx* c, const fvg_csr* a, fvg_prec** out) {
     *out = nullptr;
     auto* p = new fvg_prec();
     p->kind = kDiagonal;
-    int s = setup_pattern(c, a, p, false);
+    p->pattern_id = a->pattern_id;
+    p->a_nnz = a->nnz;
+    int s = setup_pattern(c, a->nrows, a->h_row_ptr, a->h_col, p, false);
     if (s == FVG_OK) {
         hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_diag, a->val,
                            p->d_inv_diag);
@@ -1940,11 +2173,9 @@ int fvg_ilu_create(fvg_ctx* c, const fvg_csr* a, int32_t level, fvg_prec** out) 
 
 static int refactor(fvg_ctx* c, fvg_prec* p, const fvg_csr* a, bool check) {
     if (!c || !p || !a) return fail(FVG_E_INVALID, "null argument");
+    if (p->kind == kIdentity) return a->nrows == p->n ? FVG_OK : fail(FVG_E_INVALID, "matrix size differs");
     // the same mesh pattern is recognised by its id; other matrices are compared entry by entry
-    const bool same = a->pattern_id != 0 && a->pattern_id == p->pattern_id && a->nrows == p->n && a->nnz == p->nnz;
-    if (!same && (a->nrows != p->n || a->nnz != p->nnz || a->h_col != p->h_col))
-        return fail(FVG_E_INVALID, "matrix pattern differs from the analysed one");
-    if (p->kind == kIdentity) return FVG_OK;
+    if (!same_pattern(p, a)) return fail(FVG_E_INVALID, "matrix pattern differs from the analysed one");
     if (p->kind == kDiagonal) {
         hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_diag, a->val,
                            p->d_inv_diag);
@@ -1980,9 +2211,7 @@ int fvg_prec_refactor_many(fvg_ctx* c, int32_t count, fvg_prec* const* ps, const
             FVG_TRY(refactor(c, p, a, false));
             continue;
         }
-        const bool same = a->pattern_id != 0 && a->pattern_id == p->pattern_id && a->nrows == p->n && a->nnz == p->nnz;
-        if (!same && (a->nrows != p->n || a->nnz != p->nnz || a->h_col != p->h_col))
-            return fail(FVG_E_INVALID, "matrix pattern differs from the analysed one");
+        if (!same_pattern(p, a)) return fail(FVG_E_INVALID, "matrix pattern differs from the analysed one");
         FactorJob J{};
         J.kind = p->kind;
         J.nlevels = p->lower.nlevels;
@@ -1992,7 +2221,7 @@ int fvg_prec_refactor_many(fvg_ctx* c, int32_t count, fvg_prec* const* ps, const
         J.col = p->d_col;
         J.diag = p->d_diag;
         J.tpos = p->d_tpos;
-        J.a = a->val;
+        J.a = factor_input(c, p, a);
         J.fac = p->d_fac;
         J.inv_diag = p->d_inv_diag;
         J.flags = p->d_flags;
@@ -2075,6 +2304,15 @@ int fvg_precondition(fvg_ctx* c, const fvg_prec* p, fvg_vec* solution, const fvg
     FVG_TRY(apply_prec(c, p, rhs->d, solution->d, nullptr));
     FVG_HIP(hipStreamSynchronize(c->stream));
     return check_sweep_status(c, p);
+}
+
+int fvg_prec_factor_pattern(const fvg_prec* p, int64_t* n, int64_t* nnz, int32_t* row_ptr, int32_t* col) {
+    if (!p) return fail(FVG_E_INVALID, "null argument");
+    if (n) *n = p->n;
+    if (nnz) *nnz = p->nnz;
+    if (row_ptr && !p->h_rp.empty()) std::memcpy(row_ptr, p->h_rp.data(), sizeof(int32_t) * p->h_rp.size());
+    if (col && !p->h_col.empty()) std::memcpy(col, p->h_col.data(), sizeof(int32_t) * p->h_col.size());
+    return FVG_OK;
 }
 
 int fvg_prec_factor_download(fvg_ctx* c, const fvg_prec* p, double* values, double* inv_diag) {

@@ -100,7 +100,12 @@ constexpr int kInnerSweeps = 2;
 
 struct fvg_prec {
     int kind = 0;
-    int64_t n = 0, nnz = 0;
+    int level = 0;                      // fill level of IC / ILU
+    int64_t n = 0, nnz = 0;             // nnz: entries of the factor pattern (the matrix's at level 0)
+    int64_t a_nnz = 0;                  // entries of the matrix pattern it was built from
+    std::vector<int32_t> h_arp, h_acol; // level > 0: that matrix pattern (the factor's is h_rp / h_col)
+    int32_t* d_amap = nullptr;          // level > 0: factor slot of each matrix slot (-1: none)
+    double* d_afill = nullptr;          // level > 0: the matrix values on the factor pattern
     // pattern of the matrix the preconditioner was built from
     std::vector<int32_t> h_rp, h_col, h_diag, h_tpos;
     int32_t* d_rp = nullptr;

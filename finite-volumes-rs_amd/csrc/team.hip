@@ -1534,6 +1534,8 @@ static int check_members(int32_t n, const fvg_csr* const* a, const fvg_prec* con
         if (a[k]->nrows != a[k]->ncols || a[k]->nrows <= 0) return fail(FVG_E_INVALID, "batched systems must be square");
         if (a[k]->nrows > INT32_MAX / 4) return fail(FVG_E_INVALID, "system too large for a batch");
         if (p[k]->n != a[k]->nrows) return fail(FVG_E_INVALID, "preconditioner size does not match its matrix");
+        if (p[k]->level > 0)
+            return fail(FVG_E_UNSUPPORTED, "batched solves take level-0 incomplete factorisations");
         const bool same = a[k]->pattern_id != 0 && a[k]->pattern_id == p[k]->pattern_id && a[k]->nnz == p[k]->nnz;
         if ((p[k]->kind == kIC || p[k]->kind == kILU) && !same && p[k]->h_col != a[k]->h_col)
             return fail(FVG_E_INVALID, "preconditioner pattern differs from its matrix");

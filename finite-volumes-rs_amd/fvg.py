@@ -150,6 +150,7 @@ def lib():
         "fvg_pvtu_wait": [P],
         "fvg_pvtu_writer_destroy": [P],
         "fvg_prec_sweep_layout": [P, P, C.c_int32],
+        "fvg_prec_factor_pattern": [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), P, P],
         "fvg_csr_from_mesh": [P, P, C.POINTER(P)],
         "fvg_csr_create": [P, C.c_int64, C.c_int64, C.c_int64, P, P, P, C.POINTER(P)],
         "fvg_csr_destroy": [P, P],
@@ -978,8 +979,19 @@ class Preconditioner:
         check(lib().fvg_prec_refactor(self.ctx.h, self.h, matrix.h))
         self.ctx.sync()
 
+    def factor_pattern(self):
+        """(row_ptr, col) of the factor pattern: the matrix's at level 0, with the fill above."""
+        n, nnz = C.c_int64(), C.c_int64()
+        check(lib().fvg_prec_factor_pattern(self.h, C.byref(n), C.byref(nnz), None, None))
+        rp = np.empty(n.value + 1, np.int32)
+        ci = np.empty(nnz.value, np.int32)
+        check(lib().fvg_prec_factor_pattern(self.h, None, None, _ptr(rp), _ptr(ci)))
+        return rp, ci
+
     def factor_values(self):
-        v = np.empty(self._nnz)
+        nnz = C.c_int64()
+        check(lib().fvg_prec_factor_pattern(self.h, None, C.byref(nnz), None, None))
+        v = np.empty(nnz.value)
         d = np.empty(self.n)
         check(lib().fvg_prec_factor_download(self.ctx.h, self.h, _ptr(v), _ptr(d)))
         return v, d

@@ -336,11 +336,15 @@ int fvg_identity_create(fvg_ctx* ctx, int64_t n, fvg_prec** out);
 /* DiagonalPreconditioner::from_matrix (mod.rs:38-68) */
 int fvg_diagonal_create(fvg_ctx* ctx, const fvg_csr* a, fvg_prec** out);
 /* IncompleteCholesky::from_matrix(&A, level) (incomplete_cholesky.rs:205-211).
- * Factorised on the device.  level must be 0 in this build (FVG_E_UNSUPPORTED
- * otherwise).  FVG_E_NOT_SYMMETRIC mirrors the reference's assert. */
+ * Factorised on the device; level > 0 adds the reference's fill (incomplete_cholesky.rs:
+ * 35-85, 148-202: a depth-first walk of the matrix graph per row).  FVG_E_NOT_SYMMETRIC
+ * mirrors the reference's assert; FVG_E_MISSING_ENTRY where its fill pattern breaks the
+ * transpose lookup (:138-143) that panics in the reference. */
 int fvg_ic_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out);
 /* IncompleteLowerUpper::from_matrix(&A, level) (incomplete_lower_upper.rs:266-282),
- * with its drop rule |v| <= 1e-6 |u_ii| (:255-258).  level must be 0. */
+ * with its drop rule |v| <= 1e-6 |u_ii| (:255-258); level > 0 adds the reference's fill
+ * (:30-137).  Preconditioners with level > 0 serve fvg_cg / fvg_bicgstab /
+ * fvg_precondition, not batches (FVG_E_UNSUPPORTED there). */
 int fvg_ilu_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out);
 /* Diagnostics (no reference counterpart): the shape of the blocked sweep layouts, the
  * first `count` (at most 11) of: blocks of the lower and of the upper factor (0: not
@@ -363,8 +367,11 @@ int fvg_prec_refactor_many(fvg_ctx* ctx, int32_t count, fvg_prec* const* precs, 
 int fvg_prec_destroy(fvg_ctx* ctx, fvg_prec* p);
 /* Preconditioner::precondition(solution, rhs): solution = M^-1 rhs. */
 int fvg_precondition(fvg_ctx* ctx, const fvg_prec* p, fvg_vec* solution, const fvg_vec* rhs);
-/* Factor download (for checks): IC -> L (lower with diagonal, A's lower pattern) and
- * inv_diag; ILU -> LU values on A's pattern (dropped entries read 0). */
+/* The factor pattern (A's at level 0; A's plus the fill above): n, its entries, and (each
+ * may be NULL) row pointers (n + 1) and columns (nnz), rows sorted. */
+int fvg_prec_factor_pattern(const fvg_prec* p, int64_t* n, int64_t* nnz, int32_t* row_ptr, int32_t* col);
+/* Factor download (for checks): IC -> L on the lower part of the factor pattern (with the
+ * diagonal) and inv_diag; ILU -> LU values on the factor pattern (dropped entries read 0). */
 int fvg_prec_factor_download(fvg_ctx* ctx, const fvg_prec* p, double* values, double* inv_diag);
 /* Number of levels of the lower and upper triangular dependency graphs. */
 int fvg_prec_levels(const fvg_prec* p, int32_t* lower_levels, int32_t* upper_levels);

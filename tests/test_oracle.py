@@ -641,3 +641,99 @@ def test_bicgstab_restart_when_rho_collapses():
     x2, info2 = O.solve("bicgstab", rp, ci, va, P, np.array([0.0, 1.0, 0.0]), rtol=1e-12, atol=0.0, max_iter=50)
     assert info2["restarts"] == 0
     assert np.abs(x2 - np.linalg.solve(A, [0.0, 1.0, 0.0])).max() < 1e-12
+
+
+# ---- IC(level) / ILU(level) with fill (incomplete_cholesky.rs:35-85, incomplete_lower_upper.rs:30-137) ----
+
+def _mesh_system(kind, nx=8, seed=5):
+    m = meshgen.tri2d(nx, seed=seed)
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    if kind == "sym":
+        prob = problems.diffusion(m, g, kc=meshgen.lognormal_cells(om.n_cells, 1.0, 1))
+    else:
+        prob = problems.convection_diffusion(m, g, pe=(1, 20), seed=3)
+    rp, ci, va, b = om.assemble(prob["prog"], prob["face_fields"], prob["cell_fields"], prob["bc_i"], prob["bc_e"])
+    return rp, ci, va, om.n_cells
+
+
+def _neighbours(rp, ci, n):
+    return [set(int(j) for j in ci[rp[i]:rp[i + 1]] if j != i and j < n) for i in range(n)]
+
+
+def test_ic_level_one_walk_adds_no_fill():
+    """The reference's IC walk inserts a vertex only when met from a vertex whose walk length is
+    below the level (incomplete_cholesky.rs:72-76): at level 1 only row i's own neighbours,
+    which are already in A's pattern, so IC(1) is IC(0) bit for bit."""
+    rp, ci, va, n = _mesh_system("sym")
+    f0 = O.OraclePrec("ic", rp, ci, va, 0).factor()
+    f1 = O.OraclePrec("ic", rp, ci, va, 1).factor()
+    for a, b in zip(f0, f1):
+        assert np.array_equal(a, b)
+
+
+def test_ic_level_two_defining_property_and_fill():
+    """IC(2) fills distance-two couplings; on whatever pattern it fills, the incomplete factor
+    satisfies (L L^T)_ij = A_ij on that pattern (A_ij = 0 at fill positions), closer to A than
+    IC(0) off the pattern, and every fill entry couples two vertices at graph distance 2."""
+    rp, ci, va, n = _mesh_system("sym")
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n)).toarray()
+    nb = _neighbours(rp, ci, n)
+    lrp, lci, lva, _ = O.OraclePrec("ic", rp, ci, va, 2).factor()
+    L = sp.csr_matrix((lva, lci, lrp), shape=(n, n)).toarray()
+    mask = np.zeros((n, n), bool)
+    for i in range(n):
+        for j in lci[lrp[i]:lrp[i + 1]]:
+            mask[i, j] = mask[j, i] = True
+    fill = mask & (A == 0)
+    assert fill.sum() > 0
+    for i, j in zip(*np.nonzero(np.tril(fill))):
+        assert j not in nb[i] and nb[i] & nb[j], (i, j)    # distance 2
+    LLt = L @ L.T
+    assert np.allclose(LLt[mask], A[mask], rtol=1e-12, atol=1e-12 * np.abs(A).max())
+    l0rp, l0ci, l0va, _ = O.OraclePrec("ic", rp, ci, va, 0).factor()
+    L0 = sp.csr_matrix((l0va, l0ci, l0rp), shape=(n, n)).toarray()
+    assert np.linalg.norm(LLt - A) < np.linalg.norm(L0 @ L0.T - A)
+
+
+def test_ic_high_level_is_the_exact_cholesky_factor():
+    """A level beyond the graph's diameter fills every row completely (every vertex met is
+    explored and inserted): IC(level) is the exact Cholesky factor and PCG converges at once."""
+    rp, ci, va, n = _mesh_system("sym", nx=4)
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n)).toarray()
+    P = O.OraclePrec("ic", rp, ci, va, n)
+    lrp, lci, lva, inv = P.factor()
+    L = sp.csr_matrix((lva, lci, lrp), shape=(n, n)).toarray()
+    assert np.allclose(L, np.linalg.cholesky(A), rtol=1e-12, atol=1e-12)
+    b = np.random.default_rng(1).normal(size=n)
+    x, info = O.solve("cg", rp, ci, va, P, b, rtol=1e-10, atol=0.0)
+    assert info["iterations"] == 1 and np.allclose(x, np.linalg.solve(A, b), rtol=1e-10)
+
+
+def test_ilu_level_one_pattern_and_defining_property():
+    """ILU(1) (incomplete_lower_upper.rs:30-137): row i's upper part holds the j > i adjacent to
+    i or to a lower neighbour of i, its lower part the j < i adjacent to i or to a higher
+    neighbour of i (brute force below).  On that pattern, without drops, (L U)_ij = A_ij."""
+    rp, ci, va, n = _mesh_system("nonsym")
+    nb = _neighbours(rp, ci, n)
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n)).toarray()
+    P = O.OraclePrec("ilu", rp, ci, va, 1)
+    frp, fci, fva, _ = P.factor()
+    for i in range(n):
+        up = {j for j in nb[i] if j > i} | {j for k in nb[i] if k < i for j in nb[k] if j > i}
+        lo = {j for j in nb[i] if j < i} | {j for k in nb[i] if k > i for j in nb[k] if j < i}
+        got = set(int(j) for j in fci[frp[i]:frp[i + 1]])
+        # entries may be dropped (|v| <= 1e-6 |u_ii|), never added
+        assert got <= up | lo | {i} and i in got
+    LU = sp.csr_matrix((fva, fci, frp), shape=(n, n)).toarray()
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    mask = np.zeros((n, n), bool)
+    for i in range(n):
+        mask[i, list(fci[frp[i]:frp[i + 1]])] = True
+    prod = L @ U
+    assert np.allclose(prod[mask], A[mask], rtol=1e-10, atol=1e-10 * np.abs(A).max())
+    # the fill makes it a closer factorisation than ILU(0)
+    f0 = O.OraclePrec("ilu", rp, ci, va, 0).factor()
+    LU0 = sp.csr_matrix((f0[2], f0[1], f0[0]), shape=(n, n)).toarray()
+    assert np.linalg.norm(prod - A) < np.linalg.norm((np.tril(LU0, -1) + np.eye(n)) @ np.triu(LU0) - A)
