@@ -249,8 +249,9 @@ def leg_convdiff(args, fvg, workloads):
 def leg_transient(args, fvg, workloads):
     """One time step of the crate's ex4 on the 2,097,152-prism mesh of the convection-
     diffusion leg, everything on the device: compute_gradients, compute_limiters
-    (LimitedLinear(1), Upwind on the flux, Corrected), assembly of time(Euler) +
-    convection(LimitedLinear) - laplacian(Corrected), then ILU(0) + BiCGStab to 1e-8.
+    (LimitedLinear(1), Upwind on the flux, Corrected), assembly of time(Euler, dt = 0.01) +
+    convection(LimitedLinear) - laplacian(Corrected) - source (a unit source in a ball of
+    radius 0.1), then ILU(1) + BiCGStab to 1e-8, as examples/ex4/ex4.rs:137-155 does.
     The oracle's gradients + limiters + assembly of the same step run beside it on one
     host core."""
     raw, fields_of = workloads.convdiff_prism()
@@ -263,14 +264,16 @@ def leg_transient(args, fvg, workloads):
     bc = fvg.FaceConstraints(bci, bce)
     n, F, nb, D = mesh.n_cells, mesh.n_faces, mesh.n_boundary_faces, 3
     u0 = np.exp(-8.0 * ((geom["cell_center"] - 0.5) ** 2).sum(1))
+    src = (np.linalg.norm(geom["cell_center"] - np.array([0.25, 0.5, 0.5]), axis=1) < 0.1).astype(np.float64)
+    dt = 0.01
     u = fvg.DistributedVector.from_data(ctx, u0)
     g = fvg.DistributedVector(ctx, n * D)
     lim = fvg.DistributedVector(ctx, F)
     fl = fvg.DistributedVector.from_data(ctx, flux)
     up, corr = fvg.Upwind(fl), fvg.Corrected(g, 1.0)
     llim = fvg.limiters.LimitedLinear(1.0)
-    eq = (fvg.time(fvg.Euler(u, 1e-3)) + fvg.convection(fvg.LimitedLinear(fl, lim), fl)
-          - fvg.laplacian(fvg.Corrected(g, 1.0), kf))
+    eq = (fvg.time(fvg.Euler(u, dt)) + fvg.convection(fvg.LimitedLinear(fl, lim), fl)
+          - fvg.laplacian(fvg.Corrected(g, 1.0), kf) - fvg.source(src))
     ops, ff, cf = eq.compile()
     asm = fvg.Assembler(mesh, ops, ff, cf, bci, bce)
     lhs = fvg.DistributedMatrix.from_mesh(ctx, mesh)
@@ -279,7 +282,7 @@ def leg_transient(args, fvg, workloads):
     fvg.compute_gradients(g, u, bc, mesh)
     fvg.compute_limiters(lim, u, g, llim, up, corr, bc, mesh)
     asm.run(lhs, rhs)
-    ilu = fvg.IncompleteLowerUpper.from_matrix(lhs, 0)
+    ilu = fvg.IncompleteLowerUpper.from_matrix(lhs, 1)
     opts = fvg.LinearSolverOptions(1e-8, 0.0, 1000)
 
     def timed(fn, reps=20):
@@ -316,8 +319,9 @@ def leg_transient(args, fvg, workloads):
     grad_bytes = n * (4 + 8 * D + 8 + 8 * D) + L * (4 + 8 * D) + nb * 16
     lim_bytes = F * (4 + 4 + 8 * D + 8 * D + 8 + 8) + n * (8 * D + 8 + 8 * D) + nb * (4 + 16)
     out = {"workload": "ex4 step on convdiff_prism: 2,097,152 prisms, LimitedLinear(1) limiters (Upwind, "
-                       "Corrected), time(Euler, dt=1e-3) + convection(LimitedLinear) - laplacian(Corrected), "
-                       "ILU(0) + BiCGStab to 1e-8",
+                       "Corrected), time(Euler, dt=0.01) + convection(LimitedLinear) - laplacian(Corrected) "
+                       "- source (unit source in a ball of radius 0.1), ILU(1) + BiCGStab to 1e-8",
+           "ilu_level": 1, "factor_nnz": int(ilu.factor_pattern()[0][-1]),
            "n_cells": n, "n_faces": F, "step_ms": step_ms, "steps_per_s": 1e3 / step_ms,
            "iterations": info.iterations,
            "gradients": {"avg_ms": grad_ms, "algorithmic_bytes": grad_bytes,
@@ -334,9 +338,10 @@ def leg_transient(args, fvg, workloads):
         t0 = time.perf_counter()
         go = om.gradients(u0, bci, bce)
         lo = om.limiters(u0, go, "limited_linear", 1.0, "upwind", flux, go, 1.0, bci, bce)
-        prog = [(O.OP["TIME_EULER"], 0, -1, -1, 1e-3), (O.OP["CONVECTION_LIMITED_LINEAR"], 0, 0, 1, 0.0),
-                (O.OP["ADD"], 0, 0, 0, 0.0), (O.OP["LAPLACIAN_CORRECTED"], 2, 1, 0, 1.0), (O.OP["SUB"], 0, 0, 0, 0.0)]
-        om.assemble(prog, [flux, lo, kf], [u0, go.ravel()], bci, bce)
+        prog = [(O.OP["TIME_EULER"], 0, -1, -1, dt), (O.OP["CONVECTION_LIMITED_LINEAR"], 0, 0, 1, 0.0),
+                (O.OP["ADD"], 0, 0, 0, 0.0), (O.OP["LAPLACIAN_CORRECTED"], 2, 1, 0, 1.0), (O.OP["SUB"], 0, 0, 0, 0.0),
+                (O.OP["SOURCE"], 2, 0, 0, 0.0), (O.OP["SUB"], 0, 0, 0, 0.0)]
+        om.assemble(prog, [flux, lo, kf], [u0, go.ravel(), src], bci, bce)
         t_cpu = time.perf_counter() - t0
         out["cpu_oracle_tools_ms"] = 1e3 * t_cpu
         out["tools_speedup_vs_cpu"] = 1e3 * t_cpu / (grad_ms + lim_ms + asm_ms)

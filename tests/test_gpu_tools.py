@@ -127,7 +127,8 @@ def test_new_schemes_assemble_bit_exact(ctx, name):
 def test_transient_step_like_ex4(ctx):
     """One time step of the crate's ex4 on the device against the oracle: gradients,
     limiters (LimitedLinear(1), Upwind, Corrected), assembly of time + convection
-    (LimitedLinear) - laplacian(Corrected) - source, then BiCGStab + ILU(0)."""
+    (LimitedLinear) - laplacian(Corrected) - source, then BiCGStab + ILU(1), as
+    examples/ex4/ex4.rs:137-155 does."""
     raw = meshgen.prism3d(12, layers=6, seed=4)
     om = O.OracleMesh(raw)
     geom = om.geometry()
@@ -153,7 +154,7 @@ def test_transient_step_like_ex4(ctx):
     eq = (fvg.time(fvg.Euler(u0, 0.01)) + fvg.convection(fvg.LimitedLinear(flux, lim), flux)
           - fvg.laplacian(fvg.Corrected(g, 1.0), kf) - fvg.source(src))
     lhs, rhs = fvg.assemble(eq, bc, mesh)
-    ilu = fvg.IncompleteLowerUpper.from_matrix(lhs, 0)
+    ilu = fvg.IncompleteLowerUpper.from_matrix(lhs, 1)
     x = fvg.DistributedVector(ctx, n)
     info = fvg.bi_conjugate_gradient_stab(x, lhs, rhs, ilu, fvg.LinearSolverOptions(1e-8, 0.0, 1000))
     # the same step on the oracle
@@ -166,7 +167,7 @@ def test_transient_step_like_ex4(ctx):
             (OPc["SOURCE"], 2, 0, 0, 0.0), (OPc["SUB"], 0, 0, 0, 0.0)]
     rp, ci, va, b = om.assemble(prog, [flux, lims, kf], [u0, grads.ravel(), src], bci, bce)
     assert np.array_equal(lhs.csr()[2], va) and np.array_equal(rhs.numpy(), b)
-    P = O.OraclePrec("ilu", rp, ci, va, 0)
+    P = O.OraclePrec("ilu", rp, ci, va, 1)
     xo, io = O.solve("bicgstab", rp, ci, va, P, b, rtol=1e-8, atol=0.0, max_iter=1000)
     assert abs(info.iterations - io["iterations"]) <= 2
     assert np.linalg.norm(x.numpy() - xo) <= 1e-6 * np.linalg.norm(xo)
