@@ -15,7 +15,51 @@
 namespace fvg {
 
 constexpr int kMaxFields = 8;
-constexpr int kMaxStack = 16;
+constexpr int kMaxStack = 8;
+
+struct Triple {
+    double li, lj, r;
+};
+struct Pair {
+    double l, r;
+};
+__device__ __forceinline__ Triple pick(bool c, const Triple& a, const Triple& b) {
+    return Triple{c ? a.li : b.li, c ? a.lj : b.lj, c ? a.r : b.r};
+}
+__device__ __forceinline__ Pair pick(bool c, const Pair& a, const Pair& b) {
+    return Pair{c ? a.l : b.l, c ? a.r : b.r};
+}
+
+// The evaluation stack of a term program.  Every thread runs the same program, so the
+// stack depth is the same in every lane.  Each slot is read and written only as a
+// member-wise select on the depth (no branch per slot, no computed address), so the
+// stack stays in registers; a runtime-indexed private array would live in scratch.
+template <class T, int S>
+struct RegStack {
+    T s[S];
+    int sp = 0;
+    __device__ __forceinline__ T at(int i) const {
+        T v = s[0];
+#pragma unroll
+        for (int k = 1; k < S; ++k) v = pick(i == k, s[k], v);
+        return v;
+    }
+    __device__ __forceinline__ void put(int i, const T& v) {
+#pragma unroll
+        for (int k = 0; k < S; ++k) s[k] = pick(i == k, v, s[k]);
+    }
+    __device__ __forceinline__ void push(const T& v) { put(sp++, v); }
+    __device__ __forceinline__ T pop() { return at(--sp); }
+};
+
+// Backward differencing coefficients (schemes/time/backward.rs:24-29), powi(2) as x * x
+struct BackwardCoef {
+    double a_next, a_last, a_last_2;
+};
+__host__ __device__ inline BackwardCoef backward_coef(double dt, double dt_2) {
+    const double wn = dt / dt_2;
+    return BackwardCoef{(1.0 + 2.0 * wn) / (1.0 + wn), ((1.0 + wn) * (1.0 + wn)) / (1.0 + wn), (wn * wn) / (1.0 + wn)};
+}
 
 struct TermProgram {
     int n_ops;
@@ -25,51 +69,54 @@ struct TermProgram {
     MeshView geo;   // geometry for the Corrected, Linear and LimitedLinear schemes
 };
 
-struct Triple {
-    double li, lj, r;
-};
-
 // Term::face_terms of the whole expression for face f (original id), face geometry
 // supplied.  laplacian.rs:35-45 + orthogonal.rs:26-44 or corrected.rs:34-67,
 // convection.rs:36-44 + upwind.rs:26-33, linear.rs:22-44 or limitedlinear.rs:24-72,
 // divergence.rs:36-45, algebra operations.rs.
-template <int D>
+template <int D, int S>
 __device__ __forceinline__ Triple face_terms(const TermProgram& P, int f, double tinv) {
-    Triple st[kMaxStack];
-    int sp = 0;
+    RegStack<Triple, S> st;
     for (int k = 0; k < P.n_ops; ++k) {
         const fvg_term_op o = P.ops[k];
         switch (o.op) {
+        case FVG_OP_PARAM:   // a parameter of the word before
+            break;
+        case FVG_OP_USER_TERM:
+            if (o.arg0 >= 0)
+                st.push(Triple{P.face_fields[o.arg0][f], P.face_fields[o.arg0 + 1][f], P.face_fields[o.arg0 + 2][f]});
+            else
+                st.push(Triple{0.0, 0.0, 0.0});
+            break;
         case FVG_OP_LAPLACIAN_ORTHOGONAL: {
             const double t = tinv;
             const double l0 = 1.0 * (-t), l1 = 1.0 * t, r = 0.0;
             const double diff = P.face_fields[o.arg0][f];
-            st[sp++] = Triple{diff * l0, diff * l1, -(diff * r)};
+            st.push(Triple{diff * l0, diff * l1, -(diff * r)});
             break;
         }
         case FVG_OP_CONVECTION_UPWIND: {
             const double w = P.face_fields[o.arg1][f];
             const double l0 = w > 0.0 ? 1.0 : 0.0, l1 = w > 0.0 ? 0.0 : 1.0, r = 0.0;
             const double flux = P.face_fields[o.arg0][f];
-            st[sp++] = Triple{l0 * flux, l1 * flux, (-r) * flux};
+            st.push(Triple{l0 * flux, l1 * flux, (-r) * flux});
             break;
         }
         case FVG_OP_DIVERGENCE:
-            st[sp++] = Triple{0.0, 0.0, -P.face_fields[o.arg0][f]};
+            st.push(Triple{0.0, 0.0, -P.face_fields[o.arg0][f]});
             break;
         case FVG_OP_LAPLACIAN_CORRECTED: {
             double l0, l1, r;
             corrected_terms<D, 1>(P.geo, f, P.geo.owner[f], P.geo.neigh[f], P.cell_fields[o.arg1], o.scalar, l0, l1,
                                   &r);
             const double diff = P.face_fields[o.arg0][f];
-            st[sp++] = Triple{diff * l0, diff * l1, -(diff * r)};
+            st.push(Triple{diff * l0, diff * l1, -(diff * r)});
             break;
         }
         case FVG_OP_CONVECTION_LINEAR: {
             const double w = linear_weight<D>(P.geo, f, P.geo.owner[f], P.geo.neigh[f]);
             const double l0 = 1.0 * w, l1 = 1.0 * (1.0 - w), r = 0.0;
             const double flux = P.face_fields[o.arg0][f];
-            st[sp++] = Triple{l0 * flux, l1 * flux, (-r) * flux};
+            st.push(Triple{l0 * flux, l1 * flux, (-r) * flux});
             break;
         }
         case FVG_OP_CONVECTION_LIMITED_LINEAR: {
@@ -93,47 +140,48 @@ __device__ __forceinline__ Triple face_terms(const TermProgram& P, int f, double
                 l1 = 1.0;
             }
             const double r = 0.0, flux = P.face_fields[o.arg0][f];
-            st[sp++] = Triple{l0 * flux, l1 * flux, (-r) * flux};
+            st.push(Triple{l0 * flux, l1 * flux, (-r) * flux});
             break;
         }
         case FVG_OP_ADD: {
-            Triple b = st[--sp], a = st[--sp];
-            st[sp++] = Triple{a.li + b.li, a.lj + b.lj, a.r + b.r};
+            const Triple b = st.pop(), a = st.pop();
+            st.push(Triple{a.li + b.li, a.lj + b.lj, a.r + b.r});
             break;
         }
         case FVG_OP_SUB: {
-            Triple b = st[--sp], a = st[--sp];
-            st[sp++] = Triple{a.li - b.li, a.lj - b.lj, a.r - b.r};
+            const Triple b = st.pop(), a = st.pop();
+            st.push(Triple{a.li - b.li, a.lj - b.lj, a.r - b.r});
             break;
         }
         case FVG_OP_NEG: {
-            Triple a = st[--sp];
-            st[sp++] = Triple{-a.li, -a.lj, -a.r};
+            const Triple a = st.pop();
+            st.push(Triple{-a.li, -a.lj, -a.r});
             break;
         }
         default:   // cell-only terms contribute the Term::face_terms default
-            st[sp++] = Triple{0.0, 0.0, 0.0};
+            st.push(Triple{0.0, 0.0, 0.0});
             break;
         }
     }
-    return st[0];
+    return st.at(0);
 }
 
 // Term::cell_terms of the whole expression for cell c.  source.rs:34-39, 72-77,
-// terms/time.rs:31-34 + schemes/time/euler.rs:27-37.
+// terms/time.rs:31-34 + schemes/time/euler.rs:27-37, schemes/time/backward.rs:21-62.
+
+template <int S>
 __device__ __forceinline__ void cell_terms(const TermProgram& P, int c, double& l_out, double& r_out) {
-    double sl[kMaxStack], sr[kMaxStack];
-    int sp = 0;
+    RegStack<Pair, S> st;
     for (int k = 0; k < P.n_ops; ++k) {
         const fvg_term_op o = P.ops[k];
         switch (o.op) {
+        case FVG_OP_PARAM:   // a parameter of the word before
+            break;
         case FVG_OP_SOURCE:
-            sl[sp] = 0.0;
-            sr[sp++] = -P.cell_fields[o.arg0][c];
+            st.push(Pair{0.0, -P.cell_fields[o.arg0][c]});
             break;
         case FVG_OP_LINEAR_SOURCE:
-            sl[sp] = P.cell_fields[o.arg0][c];
-            sr[sp++] = 0.0;
+            st.push(Pair{P.cell_fields[o.arg0][c], 0.0});
             break;
         case FVG_OP_TIME_EULER: {
             double r0 = 1.0, r1 = 1.0;
@@ -142,40 +190,57 @@ __device__ __forceinline__ void cell_terms(const TermProgram& P, int c, double& 
                 r1 = P.cell_fields[o.arg2][c];
             }
             const double dt = o.scalar;
-            sl[sp] = 1.0 * (1.0 / dt) * r0;
-            sr[sp++] = P.cell_fields[o.arg0][c] / dt * r1;
+            st.push(Pair{1.0 * (1.0 / dt) * r0, P.cell_fields[o.arg0][c] / dt * r1});
             break;
         }
+        case FVG_OP_TIME_BACKWARD: {   // schemes/time/backward.rs:45-61
+            const double dt = o.scalar;
+            const BackwardCoef a = backward_coef(dt, P.ops[k + 1].scalar);
+            double r0 = 1.0, r1 = 1.0, r2 = 1.0;
+            if (o.arg2 >= 0) {
+                r0 = P.cell_fields[o.arg2][c];
+                r1 = P.cell_fields[o.arg2 + 1][c];
+                r2 = P.cell_fields[o.arg2 + 2][c];
+            }
+            st.push(Pair{1.0 * (a.a_next / dt) * r0,
+                         (P.cell_fields[o.arg0][c] * a.a_last * r1 - P.cell_fields[o.arg1][c] * a.a_last_2 * r2) / dt});
+            break;
+        }
+        case FVG_OP_USER_TERM:
+            if (o.arg1 >= 0)
+                st.push(Pair{P.cell_fields[o.arg1][c], P.cell_fields[o.arg1 + 1][c]});
+            else
+                st.push(Pair{0.0, 0.0});
+            break;
         case FVG_OP_ADD: {
-            --sp;
-            sl[sp - 1] = sl[sp - 1] + sl[sp];
-            sr[sp - 1] = sr[sp - 1] + sr[sp];
+            const Pair b = st.pop(), a = st.pop();
+            st.push(Pair{a.l + b.l, a.r + b.r});
             break;
         }
         case FVG_OP_SUB: {
-            --sp;
-            sl[sp - 1] = sl[sp - 1] - sl[sp];
-            sr[sp - 1] = sr[sp - 1] - sr[sp];
+            const Pair b = st.pop(), a = st.pop();
+            st.push(Pair{a.l - b.l, a.r - b.r});
             break;
         }
-        case FVG_OP_NEG:
-            sl[sp - 1] = -sl[sp - 1];
-            sr[sp - 1] = -sr[sp - 1];
+        case FVG_OP_NEG: {
+            const Pair a = st.pop();
+            st.push(Pair{-a.l, -a.r});
             break;
+        }
         default:   // face-only terms contribute the Term::cell_terms default
-            sl[sp] = 0.0;
-            sr[sp++] = 0.0;
+            st.push(Pair{0.0, 0.0});
             break;
         }
     }
-    l_out = sl[0];
-    r_out = sr[0];
+    const Pair t = st.at(0);
+    l_out = t.l;
+    r_out = t.r;
 }
 
 // Row c of the cell loop (assembly.rs:26-32: A_cc = l*vol, b_c = r*vol, every other
 // slot of the row from zero) followed by the face loop (assembly.rs:35-76) restricted
 // to the faces that update row c, in face order.
-template <int D>
+template <int D, int S>
 __global__ void __launch_bounds__(kBlock) k_assemble_rows(TermProgram P, int n, const int32_t* __restrict__ row_ptr,
                                                           const int32_t* __restrict__ diagpos,
                                                           const double* __restrict__ vol,
@@ -189,7 +254,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(TermProgram P, int n, 
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     double l, r;
-    cell_terms(P, c, l, r);
+    cell_terms<S>(P, c, l, r);
     const double v = vol[c];
     const int d = diagpos[c];
     for (int k = row_ptr[c]; k < row_ptr[c + 1]; ++k)
@@ -198,7 +263,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(TermProgram P, int n, 
     for (int e = ent_ptr[c]; e < ent_ptr[c + 1]; ++e) {
         const int2 en = ent[e];
         const int f = en.x & kFaceMask, role = en.x >> kRoleShift;
-        const Triple t = face_terms<D>(P, f, tinv[f]);
+        const Triple t = face_terms<D, S>(P, f, tinv[f]);
         const double A = area[f];
         const double li = t.li * A, lj = t.lj * A, rf = t.r * A;
         if (role == kRoleOwner) {          // lhs[i,i] += li; lhs[i,j] += lj; rhs[i] += r
@@ -219,10 +284,102 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(TermProgram P, int n, 
     rhs[c] = b_c;
 }
 
+// The same assembly for meshes whose off-diagonal slots each belong to one face, with one
+// lane per cell-face entry instead of one per row, so that every entry's gathers are in
+// flight at once.  A block owns `cpb` consecutive cells and walks their entries (which
+// are consecutive in `ent`) a block-width at a time: each lane evaluates one entry's face
+// terms, stores its off-diagonal slot (0 + lj or 0 - li, the reference's update of a
+// slot that starts at zero), and leaves its diagonal and right-hand-side contributions in
+// LDS; then lane c of the block adds its cell's contributions to the cell terms, in
+// ascending face id, as the row kernel does.  x - y is x + (-y) in IEEE arithmetic, so
+// the neighbour side's subtractions are stored negated.
+constexpr int kAsmBlock = 256;
+
+template <int D, int S>
+__global__ void __launch_bounds__(kAsmBlock) k_assemble_entries(TermProgram P, int n, int cpb,
+                                                                const int32_t* __restrict__ diagpos,
+                                                                const double* __restrict__ vol,
+                                                                const int32_t* __restrict__ ent_ptr,
+                                                                const int2* __restrict__ ent,
+                                                                const double* __restrict__ area,
+                                                                const double* __restrict__ tinv,
+                                                                const double* __restrict__ bc_i,
+                                                                const double* __restrict__ bc_e,
+                                                                double* __restrict__ val, double* __restrict__ rhs) {
+    __shared__ double s_a[kAsmBlock], s_a2[kAsmBlock], s_b[kAsmBlock];
+    __shared__ unsigned char s_bnd[kAsmBlock];
+    const int tid = threadIdx.x;
+    const int c0 = blockIdx.x * cpb;
+    const int c1 = min(c0 + cpb, n);
+    const int e0 = ent_ptr[c0], e1 = ent_ptr[c1];
+    const int c = c0 + tid;
+    const bool mine = c < c1;   // this lane sums cell c
+    double a = 0.0, b = 0.0;
+    int ecur = 0, eend = 0;
+    if (mine) {
+        double l, r;
+        cell_terms<S>(P, c, l, r);
+        const double v = vol[c];
+        a = l * v;
+        b = r * v;
+        ecur = ent_ptr[c];
+        eend = ent_ptr[c + 1];
+    }
+    for (int base = e0; base < e1; base += kAsmBlock) {
+        const int e = base + tid;
+        if (e < e1) {
+            const int2 en = ent[e];
+            const int f = en.x & kFaceMask, role = en.x >> kRoleShift;
+            const Triple t = face_terms<D, S>(P, f, tinv[f]);
+            const double A = area[f];
+            const double li = t.li * A, lj = t.lj * A, rf = t.r * A;
+            if (role == kRoleOwner) {          // lhs[i,i] += li; lhs[i,j] += lj; rhs[i] += r
+                val[en.y] = 0.0 + lj;
+                s_a[tid] = li;
+                s_b[tid] = rf;
+                s_bnd[tid] = 0;
+            } else if (role == kRoleNeighbour) {   // lhs[j,i] -= li; lhs[j,j] -= lj; rhs[j] -= r
+                val[en.y] = 0.0 - li;
+                s_a[tid] = -lj;
+                s_b[tid] = -rf;
+                s_bnd[tid] = 0;
+            } else {                           // boundary: value_j = bndi * value_i + bnde
+                s_a[tid] = li;
+                s_a2[tid] = lj * bc_i[en.y];
+                s_b[tid] = rf - lj * bc_e[en.y];
+                s_bnd[tid] = 1;
+            }
+        }
+        __syncthreads();
+        if (mine) {
+            const int stop = min(eend, base + kAsmBlock);
+            for (; ecur < stop; ++ecur) {
+                const int k = ecur - base;
+                a += s_a[k];
+                if (s_bnd[k]) a += s_a2[k];
+                b += s_b[k];
+            }
+        }
+        __syncthreads();
+    }
+    if (mine) {
+        val[diagpos[c]] = a;
+        rhs[c] = b;
+    }
+}
+
+// cells per block of the entry kernel: about two block-widths of entries per block, at
+// most one cell per lane
+inline int assemble_cells_per_block(int64_t n_cells, int64_t n_ent) {
+    const int64_t per = n_ent > 0 ? (2 * kAsmBlock * n_cells) / n_ent : kAsmBlock;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(kAsmBlock, per));
+}
+
 // stack-depth check so the device evaluator never over- or under-runs
-static int validate_program(const fvg_term_op* prog, int n, int nff, int ncf) {
+static int validate_program(const fvg_term_op* prog, int n, int nff, int ncf, int* max_depth) {
     if (n <= 0 || n > FVG_MAX_TERM_OPS) return fail(FVG_E_INVALID, "term program length out of range");
     int depth = 0;
+    *max_depth = 0;
     for (int k = 0; k < n; ++k) {
         const fvg_term_op& o = prog[k];
         switch (o.op) {
@@ -258,6 +415,23 @@ static int validate_program(const fvg_term_op* prog, int n, int nff, int ncf) {
                 return fail(FVG_E_INVALID, "density field index out of range");
             ++depth;
             break;
+        case FVG_OP_TIME_BACKWARD:
+            if (o.arg0 < 0 || o.arg0 >= ncf || o.arg1 < 0 || o.arg1 >= ncf)
+                return fail(FVG_E_INVALID, "cell field index out of range");
+            if (o.arg2 != -1 && (o.arg2 < 0 || o.arg2 + 2 >= ncf))
+                return fail(FVG_E_INVALID, "density field index out of range");
+            if (k + 1 >= n || prog[k + 1].op != FVG_OP_PARAM)
+                return fail(FVG_E_INVALID, "time(Backward) needs an FVG_OP_PARAM word with dt_2 after it");
+            ++depth;
+            ++k;   // its parameter word
+            break;
+        case FVG_OP_USER_TERM:
+            if (o.arg0 != -1 && (o.arg0 < 0 || o.arg0 + 2 >= nff))
+                return fail(FVG_E_INVALID, "user term face field index out of range");
+            if (o.arg1 != -1 && (o.arg1 < 0 || o.arg1 + 1 >= ncf))
+                return fail(FVG_E_INVALID, "user term cell field index out of range");
+            ++depth;
+            break;
         case FVG_OP_ADD:
         case FVG_OP_SUB:
             if (depth < 2) return fail(FVG_E_INVALID, "binary term operator needs two operands");
@@ -270,9 +444,37 @@ static int validate_program(const fvg_term_op* prog, int n, int nff, int ncf) {
             return fail(FVG_E_INVALID, "unknown term opcode " + std::to_string(o.op));
         }
         if (depth > kMaxStack) return fail(FVG_E_INVALID, "term program too deep");
+        *max_depth = std::max(*max_depth, depth);
     }
     if (depth != 1) return fail(FVG_E_INVALID, "term program must leave exactly one term");
     return FVG_OK;
+}
+
+struct LaunchArgs {
+    const TermProgram& P;
+    int n;
+    const fvg_mesh* m;
+    const double *bi, *be;
+    double *val, *rhs;
+    bool entries;
+    hipStream_t stream;
+};
+
+// the kernels for a program whose evaluation stack needs at most S slots
+template <int S>
+void launch_assemble(const LaunchArgs& L) {
+    const fvg_mesh* m = L.m;
+    if (L.entries) {
+        const int cpb = assemble_cells_per_block(m->n_cells, m->n_ent);
+        const dim3 grid((unsigned)((L.n + cpb - 1) / cpb));
+        auto k = m->dim == 2 ? k_assemble_entries<2, S> : k_assemble_entries<3, S>;
+        hipLaunchKernelGGL(k, grid, dim3(kAsmBlock), 0, L.stream, L.P, L.n, cpb, m->d_diagpos, m->d_cell_vol,
+                           m->d_ent_ptr, m->d_ent, m->d_area, m->d_tinv, L.bi, L.be, L.val, L.rhs);
+    } else {
+        auto k = m->dim == 2 ? k_assemble_rows<2, S> : k_assemble_rows<3, S>;
+        hipLaunchKernelGGL(k, dim3(grid_for(L.n)), dim3(kBlock), 0, L.stream, L.P, L.n, m->d_row_ptr, m->d_diagpos,
+                           m->d_cell_vol, m->d_ent_ptr, m->d_ent, m->d_area, m->d_tinv, L.bi, L.be, L.val, L.rhs);
+    }
 }
 
 }  // namespace fvg
@@ -287,7 +489,8 @@ int fvg_assemble(fvg_ctx* c, const fvg_mesh* m, const fvg_term_op* program, int3
     if (!c || !m || !program || !lhs || !rhs) return fail(FVG_E_INVALID, "null argument");
     if (n_face_fields < 0 || n_face_fields > kMaxFields || n_cell_fields < 0 || n_cell_fields > kMaxFields)
         return fail(FVG_E_INVALID, "too many fields");
-    FVG_TRY(validate_program(program, n_ops, n_face_fields, n_cell_fields));
+    int depth = 0;
+    FVG_TRY(validate_program(program, n_ops, n_face_fields, n_cell_fields, &depth));
     if (lhs->pattern_id != m->pattern_id) return fail(FVG_E_INVALID, "lhs was not created from this mesh");
     if (rhs->n != m->n_cells) return fail(FVG_E_INVALID, "rhs length must equal the number of cells");
     if (m->n_bnd > 0 && (!bc_i || !bc_e || bc_i->n != m->n_bnd || bc_e->n != m->n_bnd))
@@ -310,6 +513,12 @@ int fvg_assemble(fvg_ctx* c, const fvg_mesh* m, const fvg_term_op* program, int3
             is_value[o.arg0] = true;
             if (o.arg1 >= 0) is_value[o.arg1] = is_value[o.arg2] = true;
         }
+        if (o.op == FVG_OP_TIME_BACKWARD) {
+            is_value[o.arg0] = is_value[o.arg1] = true;
+            if (o.arg2 >= 0) is_value[o.arg2] = is_value[o.arg2 + 1] = is_value[o.arg2 + 2] = true;
+            ++k;
+        }
+        if (o.op == FVG_OP_USER_TERM && o.arg1 >= 0) is_value[o.arg1] = is_value[o.arg1 + 1] = true;
     }
     for (int k = 0; k < n_cell_fields; ++k) {
         if (is_grad[k] && is_value[k]) return fail(FVG_E_INVALID, "a cell field cannot be a value and a gradient");
@@ -323,14 +532,10 @@ int fvg_assemble(fvg_ctx* c, const fvg_mesh* m, const fvg_term_op* program, int3
     const int n = (int)m->n_cells;
     const double* bi = m->n_bnd ? bc_i->d : nullptr;
     const double* be = m->n_bnd ? bc_e->d : nullptr;
-    if (m->dim == 2)
-        hipLaunchKernelGGL(k_assemble_rows<2>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, P, n, m->d_row_ptr,
-                           m->d_diagpos, m->d_cell_vol, m->d_ent_ptr, m->d_ent, m->d_area, m->d_tinv, bi, be,
-                           lhs->val, rhs->d);
-    else
-        hipLaunchKernelGGL(k_assemble_rows<3>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, P, n, m->d_row_ptr,
-                           m->d_diagpos, m->d_cell_vol, m->d_ent_ptr, m->d_ent, m->d_area, m->d_tinv, bi, be,
-                           lhs->val, rhs->d);
+    const LaunchArgs L{P, n, m, bi, be, lhs->val, rhs->d, m->unique_slots && c->tune_assemble != 1, c->stream};
+    if (depth <= 2) launch_assemble<2>(L);
+    else if (depth <= 4) launch_assemble<4>(L);
+    else launch_assemble<kMaxStack>(L);
     FVG_HIP(hipGetLastError());
     lhs->version += 1;
     return FVG_OK;

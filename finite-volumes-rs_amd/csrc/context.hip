@@ -344,6 +344,10 @@ int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
         if (value > 1) return fail(FVG_E_INVALID, "solve order must be 0 or 1");
         c->tune_solve_order = value;
         return FVG_OK;
+    case FVG_TUNE_ASSEMBLE:
+        if (value > 1) return fail(FVG_E_INVALID, "assemble kernel must be 0 or 1");
+        c->tune_assemble = value;
+        return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }
@@ -362,6 +366,7 @@ int fvg_ctx_get_tuning(const fvg_ctx* c, int32_t knob, int64_t* value) {
     case FVG_TUNE_SWEEP_LAUNCH: *value = c->tune_sweep_launch; return FVG_OK;
     case FVG_TUNE_SWEEP_WITHHOLD: *value = c->tune_withhold; return FVG_OK;
     case FVG_TUNE_SOLVE_ORDER: *value = c->tune_solve_order; return FVG_OK;
+    case FVG_TUNE_ASSEMBLE: *value = c->tune_assemble; return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }

@@ -779,6 +779,9 @@ int build_mesh(fvg_ctx* c, fvg_mesh* m, int64_t n_nodes, const double* nodes, co
     if (meta[2] >= INT32_MAX) return fail(FVG_E_INVALID, "cell-to-cell pattern too large for int32 indices");
     m->nnz = meta[2];
     m->n_ent = meta[3];
+    // each off-diagonal slot is met by at least one interior entry, so equal counts mean
+    // exactly one each
+    m->unique_slots = m->n_ent - m->n_bnd == m->nnz - N;
     FVG_TRY(dev_alloc_n(c, N + 1, &m->d_row_ptr));
     FVG_TRY(dev_alloc_n(c, m->nnz, &m->d_col));
     FVG_TRY(dev_alloc_n(c, N, &m->d_diagpos));

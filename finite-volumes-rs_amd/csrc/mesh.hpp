@@ -13,6 +13,9 @@ struct fvg_mesh {
     int64_t n_nodes = 0, n_faces = 0, n_cells = 0, n_bnd = 0, nnz = 0, n_ent = 0;
     int32_t max_cell_faces = 0;
     uint64_t pattern_id = 0;
+    // every off-diagonal CSR slot is updated by exactly one cell-face entry (no two faces
+    // join the same pair of cells), so assembly may store those slots instead of adding
+    bool unique_slots = false;
 
     // geometry, exactly as Mesh::compute leaves it
     double* d_area = nullptr;          // [F]

@@ -29,14 +29,15 @@ STATUS = {1: "FVG_E_INVALID", 2: "FVG_E_HIP", 3: "FVG_E_BUDGET", 4: "FVG_E_ZERO_
 
 OP_LAPLACIAN_ORTHOGONAL, OP_CONVECTION_UPWIND, OP_SOURCE, OP_LINEAR_SOURCE, OP_TIME_EULER, OP_DIVERGENCE = 1, 2, 3, 4, 5, 6
 OP_LAPLACIAN_CORRECTED, OP_CONVECTION_LINEAR, OP_CONVECTION_LIMITED_LINEAR = 7, 8, 9
+OP_TIME_BACKWARD, OP_USER_TERM, OP_PARAM = 10, 11, 15
 LIMITER_LIMITED_LINEAR, LIMITER_MINMOD, LIMITER_VANLEER, LIMITER_VANALBADA, LIMITER_SWEBY, LIMITER_SUPERBEE = range(6)
 INTERP_LINEAR, INTERP_UPWIND = 0, 1
 FNGRAD_ORTHOGONAL, FNGRAD_CORRECTED = 0, 1
 OP_ADD, OP_SUB, OP_NEG = 16, 17, 18
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 (TUNE_SWEEP_MODE, TUNE_SWEEP_WAVES, TUNE_SWEEP_POLL, TUNE_INNER_EXACT, TUNE_INNER_SWEEPS, TUNE_GRAPHS,
- TUNE_SWEEP_BLOCKS, TUNE_SPIN_LIMIT, TUNE_SWEEP_LAUNCH, TUNE_SWEEP_WITHHOLD, TUNE_SOLVE_ORDER) = (
-    1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11)
+ TUNE_SWEEP_BLOCKS, TUNE_SPIN_LIMIT, TUNE_SWEEP_LAUNCH, TUNE_SWEEP_WITHHOLD, TUNE_SOLVE_ORDER, TUNE_ASSEMBLE) = (
+    1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12)
 
 
 class FvgError(RuntimeError):
@@ -690,6 +691,22 @@ class Euler:
         self.density_previous = density_previous
 
 
+class Backward:
+    """time::Backward (src/fvm/schemes/time/backward.rs:21-35): second-order backward
+    differencing from the two previous levels, dt the step now and dt_2 the one before.
+    With densities it is Backward::new_with_density (density, density_last,
+    density_last2 per cell)."""
+
+    def __init__(self, previous, previous_2, dt, dt_2, density=None, density_last=None, density_last2=None):
+        self.previous = _field(previous)
+        self.previous_2 = _field(previous_2)
+        self.dt, self.dt_2 = float(dt), float(dt_2)
+        dens = (density, density_last, density_last2)
+        if any(d is not None for d in dens) and any(d is None for d in dens):
+            raise FvgError(1, "Backward needs all three densities or none")
+        self.densities = None if density is None else tuple(_field(d) for d in dens)
+
+
 class Term:
     """An expression tree of reference terms; +, - and unary - build TermsBinaryOp / OpNeg."""
 
@@ -716,6 +733,10 @@ class Term:
                     return i
             lst.append(arr)
             return len(lst) - 1
+
+        def consecutive(lst, arrs):   # fields an opcode reads as a run of slots
+            lst.extend(arrs)
+            return len(lst) - len(arrs)
 
         def walk(t):
             k = t.kind
@@ -748,6 +769,16 @@ class Term:
                 ops.append((OP_SOURCE, fidx(cf, t.kw["f"]), 0, 0, 0.0))
             elif k == "linear_source":
                 ops.append((OP_LINEAR_SOURCE, fidx(cf, t.kw["c"]), 0, 0, 0.0))
+            elif k == "time" and isinstance(t.kw["scheme"], Backward):
+                e = t.kw["scheme"]
+                a0, a1 = fidx(cf, e.previous), fidx(cf, e.previous_2)
+                d = -1 if e.densities is None else consecutive(cf, e.densities)
+                ops.append((OP_TIME_BACKWARD, a0, a1, d, e.dt))
+                ops.append((OP_PARAM, 0, 0, 0, e.dt_2))
+            elif k == "user":
+                a0 = -1 if t.kw["face_terms"] is None else consecutive(ff, t.kw["face_terms"])
+                a1 = -1 if t.kw["cell_terms"] is None else consecutive(cf, t.kw["cell_terms"])
+                ops.append((OP_USER_TERM, a0, a1, 0, 0.0))
             elif k == "time":
                 e = t.kw["scheme"]
                 a1 = a2 = -1
@@ -791,7 +822,28 @@ def linear_source(c):
 
 
 def time(scheme):
+    """time(scheme) (src/fvm/terms/time.rs:19-23); Euler or Backward."""
+    if not isinstance(scheme, (Euler, Backward)):
+        raise FvgError(9, "time schemes: Euler, Backward")
     return Term("time", scheme=scheme)
+
+
+def user_term(face_terms=None, cell_terms=None):
+    """Any other `impl Term` (src/fvm/terms/mod.rs:24-40) whose values the caller has
+    evaluated: face_terms = (li, lj, r) with one entry per face in face order (what
+    Term::face_terms returns for that face, before the area scaling of assembly.rs),
+    cell_terms = (l, r) with one entry per cell (Term::cell_terms, before the volume
+    scaling).  Either may be None: the trait's zero default.  Each array may be a
+    DistributedVector, which stays on the device."""
+    if face_terms is not None:
+        if len(face_terms) != 3:
+            raise FvgError(1, "face_terms is (li, lj, r)")
+        face_terms = tuple(_field(a) for a in face_terms)
+    if cell_terms is not None:
+        if len(cell_terms) != 2:
+            raise FvgError(1, "cell_terms is (l, r)")
+        cell_terms = tuple(_field(a) for a in cell_terms)
+    return Term("user", face_terms=face_terms, cell_terms=cell_terms)
 
 
 class FaceConstraints:

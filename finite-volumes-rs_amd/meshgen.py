@@ -106,6 +106,40 @@ def tri2d(nx, ny=None, jitter=0.25, seed=0):
     return mesh
 
 
+def split_faces_2d(mesh, faces):
+    """A copy of a 2D mesh with each listed interior face split at its midpoint into two
+    faces between the same two cells (those cells gain a vertex on a straight edge), so
+    that two faces update the same off-diagonal matrix slot."""
+    nodes = mesh["nodes"]
+    fp, fn = mesh["face_ptr"], mesh["face_nodes"]
+    cp, cf = mesh["cell_ptr"], mesh["cell_faces"]
+    face_list = [list(fn[fp[f]:fp[f + 1]]) for f in range(len(fp) - 1)]
+    patches = list(mesh["face_patch"])
+    cells = [list(cf[cp[c]:cp[c + 1]]) for c in range(len(cp) - 1)]
+    extra = []
+    for f in faces:
+        if patches[f] >= 0 or len(face_list[f]) != 2:
+            raise ValueError("only interior 2D faces can be split")
+        a, b = face_list[f]
+        mid = len(nodes) + len(extra)
+        extra.append(0.5 * (nodes[a] + nodes[b]))
+        face_list[f] = [a, mid]
+        g = len(face_list)
+        face_list.append([mid, b])
+        patches.append(-1)
+        for cl in cells:
+            if f in cl:
+                cl.append(g)
+    out = dict(mesh)
+    out.update(nodes=np.vstack([nodes] + [e[None] for e in extra]),
+               face_ptr=np.arange(0, 2 * len(face_list) + 1, 2, dtype=np.int64),
+               face_nodes=np.array(sum(face_list, []), dtype=np.int32),
+               face_patch=np.array(patches, dtype=np.int32),
+               cell_ptr=np.cumsum([0] + [len(c) for c in cells]).astype(np.int64),
+               cell_faces=np.array(sum(cells, []), dtype=np.int32), name=mesh["name"] + "_split")
+    return out
+
+
 def tet3d(nx, ny=None, nz=None, jitter=0.2, seed=0):
     """Unit cube, nx*ny*nz hexes each split into 6 Kuhn tetrahedra.
 

@@ -103,7 +103,11 @@ enum {
      * mirrored (structurally symmetric patterns): 0 (default) iterates with every vector in
      * the lower sweep's position order, so no kernel reorders a vector for the sweeps; 1
      * iterates in natural row order.  The same iteration arithmetic element by element. */
-    FVG_TUNE_SOLVE_ORDER = 11
+    FVG_TUNE_SOLVE_ORDER = 11,
+    /* fvg_assemble's kernel: 0 (default) one lane per cell-face entry where each
+     * off-diagonal slot belongs to one face, else one lane per row; 1 always one lane per
+     * row.  The same values either way. */
+    FVG_TUNE_ASSEMBLE = 12
 };
 int fvg_ctx_set_tuning(fvg_ctx* ctx, int32_t knob, int64_t value);
 int fvg_ctx_get_tuning(const fvg_ctx* ctx, int32_t knob, int64_t* value);
@@ -263,6 +267,17 @@ enum {
     FVG_OP_CONVECTION_LIMITED_LINEAR = 9, /* convection(LimitedLinear(&w, &lim), &flux):
                                          schemes/faceinterp/limitedlinear.rs:24-72.  arg0 = face field
                                          flux, arg1 = face field w (upwind side), arg2 = face limiters */
+    FVG_OP_TIME_BACKWARD = 10,        /* time(Backward): terms/time.rs:31-34, schemes/time/backward.rs:21-62.
+                                         arg0 / arg1 = cell fields previous / previous_2, arg2 = the first of
+                                         three consecutive cell fields density, density_last,
+                                         density_last2 or -1, scalar = dt; the next word must be
+                                         FVG_OP_PARAM with scalar = dt_2 */
+    FVG_OP_USER_TERM = 11,            /* any other `impl Term` (terms/mod.rs:24-40) with its values given:
+                                         arg0 = the first of three consecutive face fields li, lj, r
+                                         (its face_terms) or -1 for the trait's zero default, arg1 =
+                                         the first of two consecutive cell fields l, r (its
+                                         cell_terms) or -1 likewise */
+    FVG_OP_PARAM = 15,                /* a parameter of the word before it (scalar); no term */
     FVG_OP_ADD = 16,                  /* a + b (operations.rs:86-120) */
     FVG_OP_SUB = 17,                  /* a - b (operations.rs:123-157) */
     FVG_OP_NEG = 18                   /* -a    (operations.rs:190-199) */

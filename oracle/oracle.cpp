@@ -473,6 +473,9 @@ enum {
     OP_LAPLACIAN_CORRECTED = 7,
     OP_CONVECTION_LINEAR = 8,
     OP_CONVECTION_LIMITED_LINEAR = 9,
+    OP_TIME_BACKWARD = 10,   // arg0 previous, arg1 previous_2, arg2 first density or -1, scalar dt
+    OP_USER_TERM = 11,       // another impl Term: arg0 = face fields li, lj, r (or -1), arg1 = cell fields l, r (or -1)
+    OP_PARAM = 15,           // dt_2 of the OP_TIME_BACKWARD before it
     OP_ADD = 16,
     OP_SUB = 17,
     OP_NEG = 18,
@@ -614,7 +617,13 @@ struct Assembler {
                 st[sp++] = {l0 * flux, l1 * flux, (-r) * flux};
                 break;
             }
-            case OP_SOURCE: case OP_LINEAR_SOURCE: case OP_TIME_EULER:
+            case OP_USER_TERM:   // any other impl Term (terms/mod.rs:24-40): its face_terms as given
+                if (o.arg0 >= 0) st[sp++] = {face_fields[o.arg0][f], face_fields[o.arg0 + 1][f], face_fields[o.arg0 + 2][f]};
+                else st[sp++] = {0.0, 0.0, 0.0};
+                break;
+            case OP_PARAM:
+                break;
+            case OP_SOURCE: case OP_LINEAR_SOURCE: case OP_TIME_EULER: case OP_TIME_BACKWARD:
                 st[sp++] = {0.0, 0.0, 0.0};   // Term::face_terms default (terms/mod.rs:32-34)
                 break;
             case OP_ADD: { Triple b = st[--sp], a = st[--sp]; st[sp++] = {a.li + b.li, a.lj + b.lj, a.r + b.r}; break; }   // operations.rs:34-41, 56-62
@@ -646,6 +655,25 @@ struct Assembler {
                 st[sp++] = {1.0 * (1.0 / dt) * r0, cell_fields[o.arg0][c] / dt * r1};
                 break;
             }
+            case OP_TIME_BACKWARD: {   // terms/time.rs:31-34 -> schemes/time/backward.rs:21-62
+                // Backward::new: wn = dt / dt_2, powi(2) is one rounded product
+                double dt = o.scalar, dt_2 = prog[k + 1].scalar;
+                double wn = dt / dt_2;
+                double a_next = (1.0 + 2.0 * wn) / (1.0 + wn);
+                double a_last = ((1.0 + wn) * (1.0 + wn)) / (1.0 + wn);
+                double a_last_2 = (wn * wn) / (1.0 + wn);
+                double r0 = 1.0, r1 = 1.0, r2 = 1.0;
+                if (o.arg2 >= 0) { r0 = cell_fields[o.arg2][c]; r1 = cell_fields[o.arg2 + 1][c]; r2 = cell_fields[o.arg2 + 2][c]; }
+                double prev = cell_fields[o.arg0][c], prev2 = cell_fields[o.arg1][c];
+                st[sp++] = {1.0 * (a_next / dt) * r0, (prev * a_last * r1 - prev2 * a_last_2 * r2) / dt};
+                break;
+            }
+            case OP_USER_TERM:   // any other impl Term: its cell_terms as given
+                if (o.arg1 >= 0) st[sp++] = {cell_fields[o.arg1][c], cell_fields[o.arg1 + 1][c]};
+                else st[sp++] = {0.0, 0.0};
+                break;
+            case OP_PARAM:
+                break;
             case OP_LAPLACIAN_ORTHOGONAL: case OP_CONVECTION_UPWIND: case OP_DIVERGENCE:
             case OP_LAPLACIAN_CORRECTED: case OP_CONVECTION_LINEAR: case OP_CONVECTION_LIMITED_LINEAR:
                 st[sp++] = {0.0, 0.0};   // Term::cell_terms default (terms/mod.rs:28-30)

@@ -17,7 +17,7 @@ _lib = None
 
 OP = dict(LAPLACIAN_ORTHOGONAL=1, CONVECTION_UPWIND=2, SOURCE=3, LINEAR_SOURCE=4, TIME_EULER=5,
           DIVERGENCE=6, LAPLACIAN_CORRECTED=7, CONVECTION_LINEAR=8, CONVECTION_LIMITED_LINEAR=9,
-          ADD=16, SUB=17, NEG=18)
+          TIME_BACKWARD=10, USER_TERM=11, PARAM=15, ADD=16, SUB=17, NEG=18)
 # limiter schemes (schemes/limiters/mod.rs) and gradient interpolation schemes
 LIMITER = dict(limited_linear=0, minmod=1, vanleer=2, vanalbada=3, sweby=4, superbee=5)
 INTERP = dict(linear=0, upwind=1)

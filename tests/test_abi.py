@@ -5,6 +5,7 @@ import re
 import subprocess
 
 import numpy as np
+import pytest
 
 import fvg
 
@@ -68,6 +69,31 @@ def test_term_program_compiles_like_the_reference_algebra():
     eq = fvg.convection(fvg.Upwind(flux), flux) - fvg.laplacian(fvg.Orthogonal(), k)
     ops, ff, cf = eq.compile()
     assert ops[0][:3] == (fvg.OP_CONVECTION_UPWIND, 0, 0) and len(ff) == 2
+
+
+def test_backward_and_user_term_compile():
+    """time(Backward) compiles to its opcode plus a parameter word carrying dt_2, with the
+    three densities in consecutive cell slots; a user term puts its face arrays (li, lj, r)
+    and cell arrays (l, r) in consecutive slots, -1 for a part left at the zero default."""
+    p1, p2, r = np.zeros(4), np.ones(4), np.full(4, 2.0)
+    eq = fvg.time(fvg.Backward(p1, p2, 0.1, 0.2, r, r, r)) - fvg.source(p1)
+    ops, ff, cf = eq.compile()
+    assert ops == [(fvg.OP_TIME_BACKWARD, 0, 1, 2, 0.1), (fvg.OP_PARAM, 0, 0, 0, 0.2),
+                   (fvg.OP_SOURCE, 0, 0, 0, 0.0), (fvg.OP_SUB, 0, 0, 0, 0.0)]
+    assert len(cf) == 5 and all(cf[k] is r for k in (2, 3, 4))
+    with pytest.raises(fvg.FvgError):
+        fvg.Backward(p1, p2, 0.1, 0.2, r)
+    with pytest.raises(fvg.FvgError):
+        fvg.time(object())
+    fa = (np.zeros(6), np.ones(6), np.zeros(6))
+    eq = fvg.user_term(face_terms=fa) + fvg.user_term(face_terms=fa, cell_terms=(p1, p2)) + fvg.user_term(
+        cell_terms=(p2, p1))
+    ops, ff, cf = eq.compile()
+    assert ops == [(fvg.OP_USER_TERM, 0, -1, 0, 0.0), (fvg.OP_USER_TERM, 3, 0, 0, 0.0), (fvg.OP_ADD, 0, 0, 0, 0.0),
+                   (fvg.OP_USER_TERM, -1, 2, 0, 0.0), (fvg.OP_ADD, 0, 0, 0, 0.0)]
+    assert len(ff) == 6 and len(cf) == 4
+    with pytest.raises(fvg.FvgError):
+        fvg.user_term(face_terms=fa[:2])
 
 
 def test_sweep_launch_rule():

@@ -737,3 +737,88 @@ def test_ilu_level_one_pattern_and_defining_property():
     f0 = O.OraclePrec("ilu", rp, ci, va, 0).factor()
     LU0 = sp.csr_matrix((f0[2], f0[1], f0[0]), shape=(n, n)).toarray()
     assert np.linalg.norm(prod - A) < np.linalg.norm((np.tril(LU0, -1) + np.eye(n)) @ np.triu(LU0) - A)
+
+
+# ---- time(Backward) and user terms ----
+
+@pytest.mark.parametrize("omega", [1.0, 0.4, 2.5])
+def test_time_backward_is_exact_for_quadratics_in_time(omega):
+    """time(Backward) (schemes/time/backward.rs:21-62) is variable-step BDF2: with
+    w = dt/dt_2 the cell terms are ((1+2w)/(1+w)/dt, ((1+w) u_n - w^2/(1+w) u_{n-1})/dt),
+    times the cell volume (assembly.rs:26-32).  BDF2 differentiates quadratics in time
+    exactly, so time(Backward) - source(u'(t_{n+1})) returns u(t_{n+1}) for
+    u(t) = a + b t + c t^2 sampled at t_{n-1} = 0, t_n = dt_2; a swapped level, a wrong
+    sign or a coefficient for the other step ratio misses it.  For w = 1 the
+    coefficients are the textbook 3/2, 2, 1/2."""
+    m = meshgen.tet3d(3, seed=6)
+    n = m["cell_ptr"].shape[0] - 1
+    rng = np.random.default_rng(12)
+    a, bb, c = rng.normal(size=n), rng.normal(size=n), rng.normal(size=n)
+    dt_2 = 0.03
+    dt = omega * dt_2
+    u = lambda t: a + bb * t + c * t * t
+    t1 = dt_2 + dt
+    TB, PAR, SRC, SUB = O.OP["TIME_BACKWARD"], O.OP["PARAM"], O.OP["SOURCE"], O.OP["SUB"]
+    prog = [(TB, 0, 1, -1, dt), (PAR, 0, 0, 0, dt_2), (SRC, 2, 0, 0, 0.0), (SUB, 0, 0, 0, 0.0)]
+    om, g, A, b = _assemble_terms(m, prog, [], [u(dt_2), u(0.0), bb + 2.0 * c * t1])
+    vol = g["cell_volume"]
+    assert abs(A - sp.diags(A.diagonal())).max() == 0.0
+    assert np.allclose(A.diagonal(), (1 + 2 * omega) / (1 + omega) / dt * vol, rtol=8 * EPS, atol=0)
+    x = b / A.diagonal()
+    assert np.allclose(x, u(t1), rtol=1e-11, atol=1e-12)
+    if omega == 1.0:
+        _, _, A1, b1 = _assemble_terms(m, prog[:2], [], [u(dt_2), u(0.0)])
+        assert np.allclose(A1.diagonal(), 1.5 / dt * vol, rtol=4 * EPS, atol=0)
+        assert np.allclose(b1, (2.0 * u(dt_2) - 0.5 * u(0.0)) / dt * vol, rtol=1e-12, atol=1e-13)
+
+
+def test_time_backward_densities():
+    """Backward::new_with_density: density scales the unknown's coefficient, density_last
+    the previous level and density_last2 the one before (backward.rs:45-60); all three at
+    one give the bits of Backward::new."""
+    m = meshgen.tri2d(10, seed=2)
+    n = m["cell_ptr"].shape[0] - 1
+    rng = np.random.default_rng(13)
+    p1, p2 = rng.normal(size=n), rng.normal(size=n)
+    r0, r1, r2 = (rng.uniform(0.5, 2.0, n) for _ in range(3))
+    dt, dt_2 = 0.02, 0.025
+    TB, PAR = O.OP["TIME_BACKWARD"], O.OP["PARAM"]
+    om, g, A, b = _assemble_terms(m, [(TB, 0, 1, 2, dt), (PAR, 0, 0, 0, dt_2)], [], [p1, p2, r0, r1, r2])
+    vol = g["cell_volume"]
+    w = dt / dt_2
+    assert np.allclose(A.diagonal(), (1 + 2 * w) / (1 + w) / dt * r0 * vol, rtol=8 * EPS, atol=0)
+    want = ((1 + w) * p1 * r1 - w * w / (1 + w) * p2 * r2) / dt * vol
+    assert np.allclose(b, want, rtol=1e-12, atol=1e-12)
+    ones = np.ones(n)
+    _, _, A1, b1 = _assemble_terms(m, [(TB, 0, 1, 2, dt), (PAR, 0, 0, 0, dt_2)], [], [p1, p2, ones, ones, ones])
+    _, _, A0, b0 = _assemble_terms(m, [(TB, 0, 1, -1, dt), (PAR, 0, 0, 0, dt_2)], [], [p1, p2])
+    assert np.array_equal(A1.data, A0.data) and np.array_equal(b1, b0)
+
+
+@pytest.mark.parametrize("mesh", ["tri", "tet"])
+def test_user_term_reproduces_builtin_terms(mesh):
+    """A user `impl Term` (terms/mod.rs:24-40) supplying the same face and cell terms as a
+    built-in one assembles to the same bits: convection(Upwind) has face terms
+    (flux if w > 0 else 0*flux, 0*flux if w > 0 else flux, -0*flux) (upwind.rs:26-33,
+    convection.rs:36-44), source(f) has cell terms (0, -f) (source.rs:34-39).  Swapping
+    li and lj, or l and r, breaks the match."""
+    m = meshgen.tri2d(8, seed=4) if mesh == "tri" else meshgen.tet3d(3, seed=4)
+    om = O.OracleMesh(m)
+    n, F, nb = om.n_cells, om.n_faces, om.n_bnd
+    rng = np.random.default_rng(14)
+    flux, w, f = rng.normal(size=F), rng.normal(size=F), rng.normal(size=n)
+    bci, bce = rng.uniform(0, 1, nb), rng.normal(size=nb)
+    OP = O.OP
+    ref = om.assemble([(OP["CONVECTION_UPWIND"], 0, 1, 0, 0.0), (OP["SOURCE"], 0, 0, 0, 0.0), (OP["SUB"], 0, 0, 0, 0.0)],
+                      [flux, w], [f], bci, bce)
+    li = np.where(w > 0.0, 1.0, 0.0) * flux
+    lj = np.where(w > 0.0, 0.0, 1.0) * flux
+    r = (-0.0) * flux
+    prog = [(OP["USER_TERM"], 0, -1, 0, 0.0), (OP["USER_TERM"], -1, 0, 0, 0.0), (OP["SUB"], 0, 0, 0, 0.0)]
+    got = om.assemble(prog, [li, lj, r], [np.zeros(n), -f], bci, bce)
+    for k in range(4):
+        assert np.array_equal(got[k], ref[k])
+    swapped = om.assemble(prog, [lj, li, r], [np.zeros(n), -f], bci, bce)
+    assert not np.array_equal(swapped[2], ref[2])
+    swapped = om.assemble(prog, [li, lj, r], [-f, np.zeros(n)], bci, bce)
+    assert not np.array_equal(swapped[3], ref[3])
