@@ -10,6 +10,7 @@
 // per cell in the reference's operation order.
 #include <cstring>
 
+#include "reduce.hpp"
 #include "schemes.hpp"
 
 namespace fvg {
@@ -69,101 +70,137 @@ struct TermProgram {
     MeshView geo;   // geometry for the Corrected, Linear and LimitedLinear schemes
 };
 
-// Term::face_terms of the whole expression for face f (original id), face geometry
-// supplied.  laplacian.rs:35-45 + orthogonal.rs:26-44 or corrected.rs:34-67,
-// convection.rs:36-44 + upwind.rs:26-33, linear.rs:22-44 or limitedlinear.rs:24-72,
-// divergence.rs:36-45, algebra operations.rs.
-template <int D, int S>
-__device__ __forceinline__ Triple face_terms(const TermProgram& P, int f, double tinv) {
-    RegStack<Triple, S> st;
+// Term::face_terms of the whole expression for E faces f[j] (original ids) at once, so
+// that each term's loads for all E faces are in flight together.  laplacian.rs:35-45 +
+// orthogonal.rs:26-44 or corrected.rs:34-67, convection.rs:36-44 + upwind.rs:26-33,
+// linear.rs:22-44 or limitedlinear.rs:24-72, divergence.rs:36-45, algebra operations.rs.
+template <int D, int S, int E, bool G = true>
+__device__ __forceinline__ void face_terms(const TermProgram& P, const int (&f)[E], const double (&tinv)[E],
+                                           Triple (&out)[E]) {
+    RegStack<Triple, S> st[E];
     for (int k = 0; k < P.n_ops; ++k) {
         const fvg_term_op o = P.ops[k];
         switch (o.op) {
         case FVG_OP_PARAM:   // a parameter of the word before
             break;
         case FVG_OP_USER_TERM:
-            if (o.arg0 >= 0)
-                st.push(Triple{P.face_fields[o.arg0][f], P.face_fields[o.arg0 + 1][f], P.face_fields[o.arg0 + 2][f]});
-            else
-                st.push(Triple{0.0, 0.0, 0.0});
-            break;
-        case FVG_OP_LAPLACIAN_ORTHOGONAL: {
-            const double t = tinv;
-            const double l0 = 1.0 * (-t), l1 = 1.0 * t, r = 0.0;
-            const double diff = P.face_fields[o.arg0][f];
-            st.push(Triple{diff * l0, diff * l1, -(diff * r)});
-            break;
-        }
-        case FVG_OP_CONVECTION_UPWIND: {
-            const double w = P.face_fields[o.arg1][f];
-            const double l0 = w > 0.0 ? 1.0 : 0.0, l1 = w > 0.0 ? 0.0 : 1.0, r = 0.0;
-            const double flux = P.face_fields[o.arg0][f];
-            st.push(Triple{l0 * flux, l1 * flux, (-r) * flux});
-            break;
-        }
-        case FVG_OP_DIVERGENCE:
-            st.push(Triple{0.0, 0.0, -P.face_fields[o.arg0][f]});
-            break;
-        case FVG_OP_LAPLACIAN_CORRECTED: {
-            double l0, l1, r;
-            corrected_terms<D, 1>(P.geo, f, P.geo.owner[f], P.geo.neigh[f], P.cell_fields[o.arg1], o.scalar, l0, l1,
-                                  &r);
-            const double diff = P.face_fields[o.arg0][f];
-            st.push(Triple{diff * l0, diff * l1, -(diff * r)});
-            break;
-        }
-        case FVG_OP_CONVECTION_LINEAR: {
-            const double w = linear_weight<D>(P.geo, f, P.geo.owner[f], P.geo.neigh[f]);
-            const double l0 = 1.0 * w, l1 = 1.0 * (1.0 - w), r = 0.0;
-            const double flux = P.face_fields[o.arg0][f];
-            st.push(Triple{l0 * flux, l1 * flux, (-r) * flux});
-            break;
-        }
-        case FVG_OP_CONVECTION_LIMITED_LINEAR: {
-            const int j = P.geo.neigh[f];
-            const double sflux = P.face_fields[o.arg1][f];
-            const double lim = P.face_fields[o.arg2][f];
-            const double w = linear_weight<D>(P.geo, f, P.geo.owner[f], j);
-            // t[u] = wu lim + (1 - lim), t[d] = (1 - wu) lim with u the upwind side
-            const bool up0 = sflux >= 0.0;
-            const double wu = up0 ? w : 1.0 - w;
-            const double tu = wu * lim + (1.0 - lim), td = (1.0 - wu) * lim;
-            double l0, l1;
-            if (j >= 0) {
-                l0 = 1.0 * (up0 ? tu : td);
-                l1 = 1.0 * (up0 ? td : tu);
-            } else if (sflux > 0.0) {   // boundary: outlet / inlet
-                l0 = 1.0;
-                l1 = 0.0;
-            } else {
-                l0 = 0.0;
-                l1 = 1.0;
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                if (o.arg0 >= 0)
+                    st[j].push(Triple{P.face_fields[o.arg0][f[j]], P.face_fields[o.arg0 + 1][f[j]],
+                                      P.face_fields[o.arg0 + 2][f[j]]});
+                else
+                    st[j].push(Triple{0.0, 0.0, 0.0});
             }
-            const double r = 0.0, flux = P.face_fields[o.arg0][f];
-            st.push(Triple{l0 * flux, l1 * flux, (-r) * flux});
             break;
-        }
-        case FVG_OP_ADD: {
-            const Triple b = st.pop(), a = st.pop();
-            st.push(Triple{a.li + b.li, a.lj + b.lj, a.r + b.r});
+        case FVG_OP_LAPLACIAN_ORTHOGONAL:
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const double t = tinv[j];
+                const double l0 = 1.0 * (-t), l1 = 1.0 * t, r = 0.0;
+                const double diff = P.face_fields[o.arg0][f[j]];
+                st[j].push(Triple{diff * l0, diff * l1, -(diff * r)});
+            }
             break;
-        }
-        case FVG_OP_SUB: {
-            const Triple b = st.pop(), a = st.pop();
-            st.push(Triple{a.li - b.li, a.lj - b.lj, a.r - b.r});
+        case FVG_OP_CONVECTION_UPWIND:
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const double w = P.face_fields[o.arg1][f[j]];
+                const double l0 = w > 0.0 ? 1.0 : 0.0, l1 = w > 0.0 ? 0.0 : 1.0, r = 0.0;
+                const double flux = P.face_fields[o.arg0][f[j]];
+                st[j].push(Triple{l0 * flux, l1 * flux, (-r) * flux});
+            }
             break;
-        }
-        case FVG_OP_NEG: {
-            const Triple a = st.pop();
-            st.push(Triple{-a.li, -a.lj, -a.r});
+        case FVG_OP_DIVERGENCE:
+#pragma unroll
+            for (int j = 0; j < E; ++j) st[j].push(Triple{0.0, 0.0, -P.face_fields[o.arg0][f[j]]});
             break;
-        }
+        case FVG_OP_LAPLACIAN_CORRECTED:
+            if constexpr (!G) break;   // G = false: the program uses no scheme that reads geometry
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                double l0, l1, r;
+                corrected_terms<D, 1>(P.geo, f[j], P.geo.owner[f[j]], P.geo.neigh[f[j]], P.cell_fields[o.arg1],
+                                      o.scalar, l0, l1, &r);
+                const double diff = P.face_fields[o.arg0][f[j]];
+                st[j].push(Triple{diff * l0, diff * l1, -(diff * r)});
+            }
+            break;
+        case FVG_OP_CONVECTION_LINEAR:
+            if constexpr (!G) break;
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const double w = linear_weight<D>(P.geo, f[j], P.geo.owner[f[j]], P.geo.neigh[f[j]]);
+                const double l0 = 1.0 * w, l1 = 1.0 * (1.0 - w), r = 0.0;
+                const double flux = P.face_fields[o.arg0][f[j]];
+                st[j].push(Triple{l0 * flux, l1 * flux, (-r) * flux});
+            }
+            break;
+        case FVG_OP_CONVECTION_LIMITED_LINEAR:
+            if constexpr (!G) break;
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const int nb = P.geo.neigh[f[j]];
+                const double sflux = P.face_fields[o.arg1][f[j]];
+                const double lim = P.face_fields[o.arg2][f[j]];
+                const double w = linear_weight<D>(P.geo, f[j], P.geo.owner[f[j]], nb);
+                // t[u] = wu lim + (1 - lim), t[d] = (1 - wu) lim with u the upwind side
+                const bool up0 = sflux >= 0.0;
+                const double wu = up0 ? w : 1.0 - w;
+                const double tu = wu * lim + (1.0 - lim), td = (1.0 - wu) * lim;
+                double l0, l1;
+                if (nb >= 0) {
+                    l0 = 1.0 * (up0 ? tu : td);
+                    l1 = 1.0 * (up0 ? td : tu);
+                } else if (sflux > 0.0) {   // boundary: outlet / inlet
+                    l0 = 1.0;
+                    l1 = 0.0;
+                } else {
+                    l0 = 0.0;
+                    l1 = 1.0;
+                }
+                const double r = 0.0, flux = P.face_fields[o.arg0][f[j]];
+                st[j].push(Triple{l0 * flux, l1 * flux, (-r) * flux});
+            }
+            break;
+        case FVG_OP_ADD:
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const Triple b = st[j].pop(), a = st[j].pop();
+                st[j].push(Triple{a.li + b.li, a.lj + b.lj, a.r + b.r});
+            }
+            break;
+        case FVG_OP_SUB:
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const Triple b = st[j].pop(), a = st[j].pop();
+                st[j].push(Triple{a.li - b.li, a.lj - b.lj, a.r - b.r});
+            }
+            break;
+        case FVG_OP_NEG:
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const Triple a = st[j].pop();
+                st[j].push(Triple{-a.li, -a.lj, -a.r});
+            }
+            break;
         default:   // cell-only terms contribute the Term::face_terms default
-            st.push(Triple{0.0, 0.0, 0.0});
+#pragma unroll
+            for (int j = 0; j < E; ++j) st[j].push(Triple{0.0, 0.0, 0.0});
             break;
         }
     }
-    return st.at(0);
+#pragma unroll
+    for (int j = 0; j < E; ++j) out[j] = st[j].at(0);
+}
+
+template <int D, int S>
+__device__ __forceinline__ Triple face_terms(const TermProgram& P, int f, double tinv) {
+    const int fs[1] = {f};
+    const double ts[1] = {tinv};
+    Triple t[1];
+    face_terms<D, S, 1>(P, fs, ts, t);
+    return t[0];
 }
 
 // Term::cell_terms of the whole expression for cell c.  source.rs:34-39, 72-77,
@@ -284,34 +321,48 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(TermProgram P, int n, 
     rhs[c] = b_c;
 }
 
-// The same assembly for meshes whose off-diagonal slots each belong to one face, with one
-// lane per cell-face entry instead of one per row, so that every entry's gathers are in
+// The same assembly for meshes whose off-diagonal slots each belong to one face, with
+// lanes over cell-face entries instead of rows, so that the entries' gathers are in
 // flight at once.  A block owns `cpb` consecutive cells and walks their entries (which
-// are consecutive in `ent`) a block-width at a time: each lane evaluates one entry's face
-// terms, stores its off-diagonal slot (0 + lj or 0 - li, the reference's update of a
-// slot that starts at zero), and leaves its diagonal and right-hand-side contributions in
-// LDS; then lane c of the block adds its cell's contributions to the cell terms, in
-// ascending face id, as the row kernel does.  x - y is x + (-y) in IEEE arithmetic, so
-// the neighbour side's subtractions are stored negated.
+// are consecutive in `ent`) kAsmEntriesPerLane block-widths at a time: each lane
+// evaluates the face terms of its entries, sets their off-diagonal slots (0 + lj or
+// 0 - li, the reference's update of a slot that starts at zero), and leaves their
+// diagonal and right-hand-side contributions in LDS; then lane c of the block adds its
+// cell's contributions to the cell terms, in ascending face id, as the row kernel does.
+// x - y is x + (-y) in IEEE arithmetic, so the neighbour side's subtractions are stored
+// negated.
 constexpr int kAsmBlock = 256;
+constexpr int kAsmSlots = 1024;   // CSR values one block stages in LDS (8 KiB)
 
-template <int D, int S>
-__global__ void __launch_bounds__(kAsmBlock) k_assemble_entries(TermProgram P, int n, int cpb,
+template <int D, int S, int E, bool G>
+__global__ void __launch_bounds__(kAsmBlock) k_assemble_entries(TermProgram P, int n, int cpb, int stage_cap,
+                                                                const int32_t* __restrict__ row_ptr,
                                                                 const int32_t* __restrict__ diagpos,
                                                                 const double* __restrict__ vol,
                                                                 const int32_t* __restrict__ ent_ptr,
                                                                 const int2* __restrict__ ent,
-                                                                const double* __restrict__ area,
-                                                                const double* __restrict__ tinv,
+                                                                const double2* __restrict__ tarea,
                                                                 const double* __restrict__ bc_i,
                                                                 const double* __restrict__ bc_e,
                                                                 double* __restrict__ val, double* __restrict__ rhs) {
-    __shared__ double s_a[kAsmBlock], s_a2[kAsmBlock], s_b[kAsmBlock];
-    __shared__ unsigned char s_bnd[kAsmBlock];
+    constexpr int CH = E * kAsmBlock;
+    __shared__ double s_a[CH], s_a2[CH], s_b[CH];
+    __shared__ double s_val[kAsmSlots];
+    __shared__ unsigned char s_bnd[CH];
     const int tid = threadIdx.x;
-    const int c0 = blockIdx.x * cpb;
+    // consecutive cell ranges on one XCD, so that neighbouring cells' face lines meet in
+    // one L2
+    const int c0 = xcd_tile(blockIdx.x, gridDim.x) * cpb;
     const int c1 = min(c0 + cpb, n);
     const int e0 = ent_ptr[c0], e1 = ent_ptr[c1];
+    // the block's rows are one range of CSR values: built in LDS and written out in full
+    // lines when it fits, else stored slot by slot
+    const int r0 = row_ptr[c0], nslot = row_ptr[c1] - r0;
+    const bool staged = nslot <= stage_cap;   // the same in every lane of the block
+    auto put = [&](int slot, double v) {
+        if (staged) s_val[slot - r0] = v;
+        else val[slot] = v;
+    };
     const int c = c0 + tid;
     const bool mine = c < c1;   // this lane sums cell c
     double a = 0.0, b = 0.0;
@@ -325,34 +376,51 @@ __global__ void __launch_bounds__(kAsmBlock) k_assemble_entries(TermProgram P, i
         ecur = ent_ptr[c];
         eend = ent_ptr[c + 1];
     }
-    for (int base = e0; base < e1; base += kAsmBlock) {
-        const int e = base + tid;
-        if (e < e1) {
-            const int2 en = ent[e];
-            const int f = en.x & kFaceMask, role = en.x >> kRoleShift;
-            const Triple t = face_terms<D, S>(P, f, tinv[f]);
-            const double A = area[f];
-            const double li = t.li * A, lj = t.lj * A, rf = t.r * A;
+    for (int base = e0; base < e1; base += CH) {
+        // E entries per lane, strided by the block width; lanes past the end evaluate
+        // entry e0 again and discard it
+        int2 en[E];
+        int f[E];
+        double tv[E], A[E];
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            const int e = base + tid + j * kAsmBlock;
+            en[j] = ent[e < e1 ? e : e0];
+            f[j] = en[j].x & kFaceMask;
+        }
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            const double2 ta = tarea[f[j]];
+            tv[j] = ta.x;
+            A[j] = ta.y;
+        }
+        Triple t[E];
+        face_terms<D, S, E, G>(P, f, tv, t);
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            if (base + tid + j * kAsmBlock >= e1) continue;
+            const int k = tid + j * kAsmBlock, role = en[j].x >> kRoleShift;
+            const double li = t[j].li * A[j], lj = t[j].lj * A[j], rf = t[j].r * A[j];
             if (role == kRoleOwner) {          // lhs[i,i] += li; lhs[i,j] += lj; rhs[i] += r
-                val[en.y] = 0.0 + lj;
-                s_a[tid] = li;
-                s_b[tid] = rf;
-                s_bnd[tid] = 0;
+                put(en[j].y, 0.0 + lj);
+                s_a[k] = li;
+                s_b[k] = rf;
+                s_bnd[k] = 0;
             } else if (role == kRoleNeighbour) {   // lhs[j,i] -= li; lhs[j,j] -= lj; rhs[j] -= r
-                val[en.y] = 0.0 - li;
-                s_a[tid] = -lj;
-                s_b[tid] = -rf;
-                s_bnd[tid] = 0;
+                put(en[j].y, 0.0 - li);
+                s_a[k] = -lj;
+                s_b[k] = -rf;
+                s_bnd[k] = 0;
             } else {                           // boundary: value_j = bndi * value_i + bnde
-                s_a[tid] = li;
-                s_a2[tid] = lj * bc_i[en.y];
-                s_b[tid] = rf - lj * bc_e[en.y];
-                s_bnd[tid] = 1;
+                s_a[k] = li;
+                s_a2[k] = lj * bc_i[en[j].y];
+                s_b[k] = rf - lj * bc_e[en[j].y];
+                s_bnd[k] = 1;
             }
         }
         __syncthreads();
         if (mine) {
-            const int stop = min(eend, base + kAsmBlock);
+            const int stop = min(eend, base + CH);
             for (; ecur < stop; ++ecur) {
                 const int k = ecur - base;
                 a += s_a[k];
@@ -363,15 +431,19 @@ __global__ void __launch_bounds__(kAsmBlock) k_assemble_entries(TermProgram P, i
         __syncthreads();
     }
     if (mine) {
-        val[diagpos[c]] = a;
+        put(diagpos[c], a);
         rhs[c] = b;
+    }
+    if (staged) {
+        __syncthreads();
+        for (int k = tid; k < nslot; k += kAsmBlock) val[r0 + k] = s_val[k];
     }
 }
 
-// cells per block of the entry kernel: about two block-widths of entries per block, at
-// most one cell per lane
-inline int assemble_cells_per_block(int64_t n_cells, int64_t n_ent) {
-    const int64_t per = n_ent > 0 ? (2 * kAsmBlock * n_cells) / n_ent : kAsmBlock;
+// cells per block of the entry kernel: on average one pass of entries per block (E per
+// lane), at most one cell per lane
+inline int assemble_cells_per_block(int64_t n_cells, int64_t n_ent, int E) {
+    const int64_t per = n_ent > 0 ? (E * kAsmBlock * n_cells) / n_ent : kAsmBlock;
     return (int)std::max<int64_t>(1, std::min<int64_t>(kAsmBlock, per));
 }
 
@@ -457,24 +529,35 @@ struct LaunchArgs {
     const double *bi, *be;
     double *val, *rhs;
     bool entries;
+    int stage_cap;   // CSR values a block may stage in LDS (0: store slot by slot)
     hipStream_t stream;
 };
 
-// the kernels for a program whose evaluation stack needs at most S slots
-template <int S>
+// the kernels for a program whose evaluation stack needs at most S slots; G: it uses a
+// scheme that reads face or cell geometry.  The entry kernel takes two entries per lane
+// where the stack is shallow enough for the registers.
+template <int S, bool G>
 void launch_assemble(const LaunchArgs& L) {
     const fvg_mesh* m = L.m;
     if (L.entries) {
-        const int cpb = assemble_cells_per_block(m->n_cells, m->n_ent);
+        constexpr int E = S <= 4 ? 2 : 1;
+        const int cpb = assemble_cells_per_block(m->n_cells, m->n_ent, E);
         const dim3 grid((unsigned)((L.n + cpb - 1) / cpb));
-        auto k = m->dim == 2 ? k_assemble_entries<2, S> : k_assemble_entries<3, S>;
-        hipLaunchKernelGGL(k, grid, dim3(kAsmBlock), 0, L.stream, L.P, L.n, cpb, m->d_diagpos, m->d_cell_vol,
-                           m->d_ent_ptr, m->d_ent, m->d_area, m->d_tinv, L.bi, L.be, L.val, L.rhs);
+        auto k = m->dim == 2 ? k_assemble_entries<2, S, E, G> : k_assemble_entries<3, S, E, G>;
+        hipLaunchKernelGGL(k, grid, dim3(kAsmBlock), 0, L.stream, L.P, L.n, cpb, L.stage_cap, m->d_row_ptr,
+                           m->d_diagpos, m->d_cell_vol, m->d_ent_ptr, m->d_ent, m->d_tarea, L.bi, L.be, L.val,
+                           L.rhs);
     } else {
         auto k = m->dim == 2 ? k_assemble_rows<2, S> : k_assemble_rows<3, S>;
         hipLaunchKernelGGL(k, dim3(grid_for(L.n)), dim3(kBlock), 0, L.stream, L.P, L.n, m->d_row_ptr, m->d_diagpos,
                            m->d_cell_vol, m->d_ent_ptr, m->d_ent, m->d_area, m->d_tinv, L.bi, L.be, L.val, L.rhs);
     }
+}
+
+template <int S>
+void launch_assemble(const LaunchArgs& L, bool geometry) {
+    if (geometry) launch_assemble<S, true>(L);
+    else launch_assemble<S, false>(L);
 }
 
 }  // namespace fvg
@@ -532,10 +615,15 @@ int fvg_assemble(fvg_ctx* c, const fvg_mesh* m, const fvg_term_op* program, int3
     const int n = (int)m->n_cells;
     const double* bi = m->n_bnd ? bc_i->d : nullptr;
     const double* be = m->n_bnd ? bc_e->d : nullptr;
-    const LaunchArgs L{P, n, m, bi, be, lhs->val, rhs->d, m->unique_slots && c->tune_assemble != 1, c->stream};
-    if (depth <= 2) launch_assemble<2>(L);
-    else if (depth <= 4) launch_assemble<4>(L);
-    else launch_assemble<kMaxStack>(L);
+    const LaunchArgs L{P, n, m, bi, be, lhs->val, rhs->d, m->unique_slots && c->tune_assemble != 1,
+                       c->tune_assemble == 2 ? 0 : kAsmSlots, c->stream};
+    bool geometry = false;
+    for (int k = 0; k < n_ops; ++k)
+        geometry |= program[k].op == FVG_OP_LAPLACIAN_CORRECTED || program[k].op == FVG_OP_CONVECTION_LINEAR ||
+                    program[k].op == FVG_OP_CONVECTION_LIMITED_LINEAR;
+    if (depth <= 2) launch_assemble<2>(L, geometry);
+    else if (depth <= 4) launch_assemble<4>(L, geometry);
+    else launch_assemble<kMaxStack>(L, geometry);
     FVG_HIP(hipGetLastError());
     lhs->version += 1;
     return FVG_OK;

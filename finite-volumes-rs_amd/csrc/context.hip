@@ -345,7 +345,7 @@ int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
         c->tune_solve_order = value;
         return FVG_OK;
     case FVG_TUNE_ASSEMBLE:
-        if (value > 1) return fail(FVG_E_INVALID, "assemble kernel must be 0 or 1");
+        if (value > 2) return fail(FVG_E_INVALID, "assemble kernel must be 0, 1 or 2");
         c->tune_assemble = value;
         return FVG_OK;
     }

@@ -314,7 +314,9 @@ __global__ void __launch_bounds__(kBlock) k_mesh_orient(int64_t n_faces, const i
                                                         const int32_t* __restrict__ neigh,
                                                         const double* __restrict__ fcenter,
                                                         const double* __restrict__ ccenter,
-                                                        double* __restrict__ normal, double* __restrict__ tinv) {
+                                                        double* __restrict__ normal, double* __restrict__ tinv,
+                                                        const double* __restrict__ area,
+                                                        double2* __restrict__ tarea) {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= n_faces) return;
     const int64_t i = owner[f], j = neigh[f];
@@ -334,6 +336,7 @@ __global__ void __launch_bounds__(kBlock) k_mesh_orient(int64_t n_faces, const i
     for (int k = 0; k < D; ++k) d[k] = (j >= 0 ? ccenter[j * D + k] : fc[k]) - ci[k];
     const double dx = sqrt(vdot<D>(d, d));
     tinv[f] = 1.0 / dx;
+    tarea[f] = make_double2(1.0 / dx, area[f]);
 }
 
 // compacts the sorted rows into the CSR pattern and turns each cell's face list into
@@ -673,6 +676,7 @@ void free_mesh(fvg_ctx* c, fvg_mesh* m) {
     dev_free_n(c, m->d_face_center, F * D);
     dev_free_n(c, m->d_face_normal, F * D);
     dev_free_n(c, m->d_tinv, F);
+    dev_free_n(c, m->d_tarea, F);
     dev_free_n(c, m->d_cell_vol, N);
     dev_free_n(c, m->d_cell_center, N * D);
     dev_free_n(c, m->d_owner, F);
@@ -739,6 +743,7 @@ int build_mesh(fvg_ctx* c, fvg_mesh* m, int64_t n_nodes, const double* nodes, co
     FVG_TRY(dev_alloc_n(c, F * D, &m->d_face_center));
     FVG_TRY(dev_alloc_n(c, F * D, &m->d_face_normal));
     FVG_TRY(dev_alloc_n(c, F, &m->d_tinv));
+    FVG_TRY(dev_alloc_n(c, F, &m->d_tarea));
     FVG_TRY(t.get(F, &d_isb));
     FVG_TRY(dev_alloc_n(c, F + 1, &m->d_bnd));
     int32_t* d_bnd = m->d_bnd;
@@ -771,7 +776,7 @@ int build_mesh(fvg_ctx* c, fvg_mesh* m, int64_t n_nodes, const double* nodes, co
                        m->d_area, m->d_face_center, m->d_face_normal, m->d_owner, m->d_neigh, m->d_cell_vol,
                        m->d_cell_center, d_cand, d_rowcnt, d_entf, d_entcnt);
     hipLaunchKernelGGL(k_mesh_orient<D>, dim3(grid_for(F)), dim3(kBlock), 0, c->stream, F, m->d_owner, m->d_neigh,
-                       m->d_face_center, m->d_cell_center, m->d_face_normal, m->d_tinv);
+                       m->d_face_center, m->d_cell_center, m->d_face_normal, m->d_tinv, m->d_area, m->d_tarea);
     scan_prepare(c, N, d_rowcnt, d_toff_r, d_meta + 2);
     scan_prepare(c, N, d_entcnt, d_toff_e, d_meta + 3);
     FVG_HIP(hipGetLastError());

@@ -22,6 +22,7 @@ struct fvg_mesh {
     double* d_face_center = nullptr;   // [F * dim]
     double* d_face_normal = nullptr;   // [F * dim], oriented owner -> neighbour / outward
     double* d_tinv = nullptr;          // [F] 1/|x_j - x_i| or 1/|x_f - x_i| (Orthogonal)
+    double2* d_tarea = nullptr;        // [F] {tinv, area}: one 16-byte gather per face in assembly
     double* d_cell_vol = nullptr;      // [N]
     double* d_cell_center = nullptr;   // [N * dim]
     int32_t* d_owner = nullptr;        // [F]

@@ -105,8 +105,9 @@ enum {
      * iterates in natural row order.  The same iteration arithmetic element by element. */
     FVG_TUNE_SOLVE_ORDER = 11,
     /* fvg_assemble's kernel: 0 (default) one lane per cell-face entry where each
-     * off-diagonal slot belongs to one face, else one lane per row; 1 always one lane per
-     * row.  The same values either way. */
+     * off-diagonal slot belongs to one face (a workgroup's rows built in LDS and written
+     * out whole), else one lane per row; 1 always one lane per row; 2 the entry kernel
+     * storing each value straight to its slot.  The same values every way. */
     FVG_TUNE_ASSEMBLE = 12
 };
 int fvg_ctx_set_tuning(fvg_ctx* ctx, int32_t knob, int64_t value);

@@ -159,14 +159,15 @@ def _program_fields(om, seed):
 @pytest.mark.parametrize("prog_name", sorted(PROGRAMS))
 def test_entry_kernel_matches_row_kernel(ctx, name, prog_name):
     """One lane per cell-face entry (the default where every off-diagonal slot belongs to
-    one face) and one lane per row give the same bits, and both the oracle's."""
+    one face; its rows staged in LDS, or stored slot by slot) and one lane per row give
+    the same bits, and all the oracle's."""
     raw = MESHES[name]()
     om, rng, n, F, bci, bce = _fields(raw, 25)
     mesh = fvg.Mesh(ctx, raw)
     prog, ff, cf = PROGRAMS[prog_name](O.OP, _program_fields(om, 26))
     ref = om.assemble(prog, ff, cf, bci, bce)
     try:
-        for mode in (0, 1):
+        for mode in (0, 1, 2):
             ctx.set_tuning(fvg.TUNE_ASSEMBLE, mode)
             lhs, rhs = fvg.assemble_program(mesh, prog, ff, cf, bci, bce)
             _check(lhs, rhs, ref)
