@@ -31,7 +31,7 @@ OP_LAPLACIAN_ORTHOGONAL, OP_CONVECTION_UPWIND, OP_SOURCE, OP_LINEAR_SOURCE, OP_T
 OP_LAPLACIAN_CORRECTED, OP_CONVECTION_LINEAR, OP_CONVECTION_LIMITED_LINEAR = 7, 8, 9
 OP_TIME_BACKWARD, OP_USER_TERM, OP_PARAM = 10, 11, 15
 LIMITER_LIMITED_LINEAR, LIMITER_MINMOD, LIMITER_VANLEER, LIMITER_VANALBADA, LIMITER_SWEBY, LIMITER_SUPERBEE = range(6)
-INTERP_LINEAR, INTERP_UPWIND = 0, 1
+INTERP_LINEAR, INTERP_UPWIND, INTERP_LIMITED_LINEAR = 0, 1, 2
 FNGRAD_ORTHOGONAL, FNGRAD_CORRECTED = 0, 1
 OP_ADD, OP_SUB, OP_NEG = 16, 17, 18
 PRECISION_FP64, PRECISION_MIXED = 0, 1
@@ -74,6 +74,10 @@ class TermOp(C.Structure):
 class LimiterSpec(C.Structure):
     _fields_ = [("limiter", C.c_int32), ("interp", C.c_int32), ("fngrad", C.c_int32), ("reserved", C.c_int32),
                 ("k", C.c_double), ("reduction", C.c_double), ("flux", C.c_void_p), ("fn_grads", C.c_void_p)]
+
+
+class InterpSpec(C.Structure):
+    _fields_ = [("interp", C.c_int32), ("reserved", C.c_int32), ("flux", C.c_void_p), ("limiters", C.c_void_p)]
 
 
 class SolverOptions(C.Structure):
@@ -163,6 +167,11 @@ def lib():
         "fvg_assemble": [P, P, P, C.c_int32, P, C.c_int32, P, C.c_int32, P, P, P, P],
         "fvg_compute_gradients": [P, P, C.c_int32, P, P, P, P],
         "fvg_compute_limiters": [P, P, C.c_int32, P, P, P, P, P, P],
+        "fvg_compute_hbya_ainv": [P, P, P, P, P, P, P],
+        "fvg_interpolate_hbya_ainv_faces": [P, P, P, P, P, P, P, P, P, P, P, P],
+        "fvg_correct_phi": [P, P, P, P, P, C.c_int32, P, C.c_double, P, P, P],
+        "fvg_correct_velocity": [P, P, P, P, P, P],
+        "fvg_compute_wall_distance": [P, P, P, C.c_int32, C.c_double, P, P, P, P],
         "fvg_identity_create": [P, C.c_int64, C.POINTER(P)],
         "fvg_diagonal_create": [P, P, C.POINTER(P)],
         "fvg_ic_create": [P, P, C.c_int32, C.POINTER(P)],
@@ -383,7 +392,18 @@ class Mesh:
         self.n_cells, self.n_faces, self.n_boundary_faces, self.nnz, self.max_cell_faces = (
             nc.value, nf.value, nb.value, nnz.value, ncol.value)
         self.face_patch = fp
+        self.topology = _topology
+        self.names = None if raw is None else raw.get("patch_names")
         self._geom = None
+
+    def patch_name(self, patch):
+        """The patch's name from the mesh file (its topology) or the raw arrays'
+        "patch_names"; None when the mesh carries no names."""
+        if self.topology is not None:
+            return self.topology.patch_name(patch)
+        if self.names is not None:
+            return self.names[patch] if isinstance(self.names, (list, tuple)) else self.names.get(patch)
+        return None
 
     @classmethod
     def from_arrays(cls, ctx, raw):
@@ -991,6 +1011,120 @@ def compute_limiters(limiters_out, values, gradients, limiter_scheme, fgradinter
                                      bi.h if bi else None, be.h if be else None, limiters_out.h))
     if keep:
         ctx.sync()
+
+
+# ---- SIMPLE helpers (src/fvm/tools/hbya.rs) ----
+
+def _interp_spec(scheme, dev):
+    spec = InterpSpec()
+    if isinstance(scheme, Linear):
+        spec.interp = INTERP_LINEAR
+    elif isinstance(scheme, Upwind):
+        spec.interp, spec.flux = INTERP_UPWIND, dev(scheme.flux).h
+    elif isinstance(scheme, LimitedLinear):
+        spec.interp, spec.flux, spec.limiters = INTERP_LIMITED_LINEAR, dev(scheme.flux).h, dev(scheme.limiters).h
+    else:
+        raise FvgError(9, "face interpolation schemes: Linear, Upwind, LimitedLinear")
+    return spec
+
+
+def _devs(ctx):
+    keep = []
+
+    def dev(a):
+        if a is None or isinstance(a, DistributedVector):
+            return a
+        v = DistributedVector.from_data(ctx, np.ascontiguousarray(a, dtype=np.float64).ravel())
+        keep.append(v)
+        return v
+
+    return dev, keep
+
+
+def compute_hbya_ainv(hbya, ainv, velocity, mlhs, mrhs, mesh):
+    """compute_hbya_ainv(&mut hbya, &mut ainv, &velocity, &mlhs, &mrhs, &mesh)
+    (tools/hbya.rs:15-50) on the device.  hbya, velocity and mrhs are Vector<dim> cell
+    fields (n_cells x dim); mlhs is the momentum matrix, one row per cell."""
+    ctx = mesh.ctx
+    dev, keep = _devs(ctx)
+    check(lib().fvg_compute_hbya_ainv(ctx.h, mesh.h, mlhs.h, dev(mrhs).h, dev(velocity).h, hbya.h, ainv.h))
+    if keep:
+        ctx.sync()
+
+
+def intepolate_hbya_ainv_faces(hbyan_face, ainv_face, hbya, ainv, velocity_bc, pressure_bc, hbya_interp_scheme,
+                               ainv_interp_scheme, mesh):
+    """intepolate_hbya_ainv_faces (tools/hbya.rs:56-102, the reference's spelling) on the
+    device.  velocity_bc: FaceConstraints with bndi per boundary face and bnde per boundary
+    face and component; pressure_bc: FaceConstraints of the pressure."""
+    ctx = mesh.ctx
+    dev, keep = _devs(ctx)
+    hs = _interp_spec(hbya_interp_scheme, dev)
+    a_s = _interp_spec(ainv_interp_scheme, dev)
+    ubi, ube = _closure_vecs(mesh, velocity_bc, mesh.dim)
+    pbi, pbe = _closure_vecs(mesh, pressure_bc, 1)
+    h = lambda v: v.h if v is not None else None
+    check(lib().fvg_interpolate_hbya_ainv_faces(ctx.h, mesh.h, dev(hbya).h, dev(ainv).h, C.byref(hs), C.byref(a_s),
+                                                h(ubi), h(ube), h(pbi), h(pbe), hbyan_face.h, ainv_face.h))
+    if keep:
+        ctx.sync()
+
+
+interpolate_hbya_ainv_faces = intepolate_hbya_ainv_faces
+
+
+def correct_phi(phi, hbyan_face, ainv_face, pressure, fngrad_scheme, pressure_bc, mesh):
+    """correct_phi (tools/hbya.rs:108-157) on the device; fngrad_scheme Orthogonal or
+    Corrected(&pressure_grad, reduction)."""
+    ctx = mesh.ctx
+    dev, keep = _devs(ctx)
+    if isinstance(fngrad_scheme, Corrected):
+        kind, grads, red = FNGRAD_CORRECTED, dev(fngrad_scheme.gradients).h, fngrad_scheme.reduction
+    elif isinstance(fngrad_scheme, Orthogonal):
+        kind, grads, red = FNGRAD_ORTHOGONAL, None, 0.0
+    else:
+        raise FvgError(9, "face-normal-gradient schemes: Orthogonal, Corrected")
+    pbi, pbe = _closure_vecs(mesh, pressure_bc, 1)
+    h = lambda v: v.h if v is not None else None
+    check(lib().fvg_correct_phi(ctx.h, mesh.h, dev(hbyan_face).h, dev(ainv_face).h, dev(pressure).h, kind, grads,
+                                red, h(pbi), h(pbe), phi.h))
+    if keep:
+        ctx.sync()
+
+
+def correct_velocity(velocity, hbya, ainv, pressure_gradient, mesh):
+    """correct_velocity (tools/hbya.rs:161-179) on the device."""
+    ctx = mesh.ctx
+    dev, keep = _devs(ctx)
+    check(lib().fvg_correct_velocity(ctx.h, mesh.h, dev(hbya).h, dev(ainv).h, dev(pressure_gradient).h,
+                                     velocity.h))
+    if keep:
+        ctx.sync()
+
+
+def compute_wall_distance(wall_distance, mesh, wall_patches, max_iterations, tolerance, solver_options=None):
+    """compute_wall_distance(&mut wall_distance, &mesh, wall_patches, max_iterations,
+    tolerance, solver_options) (tools/wall_distance.rs:8-119) on the device.
+    wall_patches: patch ids, or None for every patch whose name contains "wall" and not
+    "slip" (wall_distance.rs:17-27; the mesh must carry patch names).  wall_distance is a
+    DistributedVector holding the initial guess and, on return, the distance.  Returns
+    (number of solves, LinearSolverInfo of the last one)."""
+    if wall_patches is None:
+        ids = np.unique(mesh.face_patch[mesh.face_patch >= 0])
+        names = {int(p): mesh.patch_name(int(p)) for p in ids}
+        if any(v is None for v in names.values()):
+            raise FvgError(1, "the mesh carries no patch names: pass wall_patches")
+        wall_patches = [p for p, name in names.items() if "wall" in name and "slip" not in name]
+    bpatch = mesh.face_patch[mesh.face_patch >= 0]
+    wall = np.ascontiguousarray(np.isin(bpatch, np.asarray(list(wall_patches), dtype=np.int64)), dtype=np.uint8)
+    opts = (solver_options or LinearSolverOptions()).c()
+    info = SolverInfo()
+    n = C.c_int32()
+    check(lib().fvg_compute_wall_distance(mesh.ctx.h, mesh.h, _ptr(wall) if wall.size else None,
+                                          int(max_iterations), float(tolerance), C.byref(opts), wall_distance.h,
+                                          C.byref(info), C.byref(n)))
+    return n.value, LinearSolverInfo("ConjugateGradient", info.iterations, info.initial_residual,
+                                     info.final_residual)
 
 
 # ---------------------------------------------------------------------------------

@@ -344,6 +344,44 @@ int fvg_compute_limiters(fvg_ctx* ctx, const fvg_mesh* mesh, int32_t ncomp, cons
                          const fvg_vec* gradients, const fvg_limiter_spec* spec, const fvg_vec* bc_i,
                          const fvg_vec* bc_e, fvg_vec* limiters);
 
+/* ---- SIMPLE helpers (tools/hbya.rs) -----------------------------------------------
+ * Vector<dim> cell fields (velocity, hbya, pressure gradient) are n_cells x dim; the
+ * velocity closure is u_bc_i (one per boundary face) and u_bc_e (boundary face x dim),
+ * the pressure closure p_bc_i and p_bc_e (one each per boundary face).  Results are
+ * bit-identical to the reference's sequential loops. */
+
+/* The face interpolation schemes the helpers take (schemes/faceinterp): Linear,
+ * Upwind(flux) or LimitedLinear(flux, limiters). */
+enum { FVG_INTERP_LIMITED_LINEAR = 2 };
+typedef struct fvg_interp_spec {
+    int32_t interp;            /* FVG_INTERP_LINEAR, FVG_INTERP_UPWIND or FVG_INTERP_LIMITED_LINEAR */
+    int32_t reserved;
+    const fvg_vec* flux;       /* Upwind, LimitedLinear: one entry per face */
+    const fvg_vec* limiters;   /* LimitedLinear: one entry per face */
+} fvg_interp_spec;
+
+/* compute_hbya_ainv (hbya.rs:15-50): ainv_i = vol_i / a_ii and hbya_i = (mrhs_i -
+ * sum_{j != i} u_j a_ij) / a_ii over row i in column order.  mlhs has one row per cell;
+ * FVG_E_MISSING_ENTRY where a row has no diagonal entry (the reference's index panics). */
+int fvg_compute_hbya_ainv(fvg_ctx* ctx, const fvg_mesh* mesh, const fvg_csr* mlhs, const fvg_vec* mrhs,
+                          const fvg_vec* velocity, fvg_vec* hbya, fvg_vec* ainv);
+/* intepolate_hbya_ainv_faces (hbya.rs:56-102): hbya interpolated to the faces and dotted
+ * with the face normal, ainv interpolated; on a boundary face the owner's values, with
+ * hbyan = 0 where both closures describe a wall. */
+int fvg_interpolate_hbya_ainv_faces(fvg_ctx* ctx, const fvg_mesh* mesh, const fvg_vec* hbya, const fvg_vec* ainv,
+                                    const fvg_interp_spec* hbya_interp, const fvg_interp_spec* ainv_interp,
+                                    const fvg_vec* u_bc_i, const fvg_vec* u_bc_e, const fvg_vec* p_bc_i,
+                                    const fvg_vec* p_bc_e, fvg_vec* hbyan_face, fvg_vec* ainv_face);
+/* correct_phi (hbya.rs:108-157): phi = hbyan - ainv * (the pressure's face-normal gradient)
+ * by FVG_FNGRAD_ORTHOGONAL, or FVG_FNGRAD_CORRECTED with fn_grads (n_cells x dim) and
+ * reduction. */
+int fvg_correct_phi(fvg_ctx* ctx, const fvg_mesh* mesh, const fvg_vec* hbyan_face, const fvg_vec* ainv_face,
+                    const fvg_vec* pressure, int32_t fngrad, const fvg_vec* fn_grads, double reduction,
+                    const fvg_vec* p_bc_i, const fvg_vec* p_bc_e, fvg_vec* phi);
+/* correct_velocity (hbya.rs:161-179): velocity = hbya - ainv * pressure_gradient. */
+int fvg_correct_velocity(fvg_ctx* ctx, const fvg_mesh* mesh, const fvg_vec* hbya, const fvg_vec* ainv,
+                         const fvg_vec* pressure_gradient, fvg_vec* velocity);
+
 /* ---- preconditioners: Preconditioner<T> (src/linalg/preconditioners/mod.rs:13-17) ---- */
 typedef struct fvg_prec fvg_prec;
 
@@ -437,6 +475,20 @@ int fvg_cg(fvg_ctx* ctx, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, 
 int fvg_bicgstab(fvg_ctx* ctx, const fvg_csr* a, const fvg_prec* p, const fvg_vec* b, fvg_vec* x,
                  const fvg_solver_options* opts, fvg_solver_info* info, double* history,
                  int64_t history_capacity);
+
+/* compute_wall_distance (tools/wall_distance.rs:8-119): the p-Poisson (p = 3) wall
+ * distance.  wall (host, one byte per boundary face in boundary-face order, may be NULL
+ * when the mesh has none) marks the wall faces: closure (0, 0); every other boundary face
+ * is zero-gradient, (1, 0).  distance holds the first solve's initial guess on entry and
+ * the normalised distance on exit.  Each of up to max_iterations rounds assembles
+ * -laplacian(Corrected(grad, 1), visc) + source(-1), factorises IC(1) and runs fvg_cg with
+ * opts (NULL: the reference's defaults); it stops after a solve whose initial residual is
+ * below tolerance, else takes the new gradients and the face diffusivity |grad_f|^(p-2).
+ * info (may be NULL) receives the last solve's record, solves (may be NULL) their count;
+ * a solver or factorisation error ends the loop with that status. */
+int fvg_compute_wall_distance(fvg_ctx* ctx, const fvg_mesh* mesh, const uint8_t* wall, int32_t max_iterations,
+                              double tolerance, const fvg_solver_options* opts, fvg_vec* distance,
+                              fvg_solver_info* info, int32_t* solves);
 
 /* FP64 fvg_cg / fvg_bicgstab that allocate nothing: no counterpart in the reference,
  * for callers that cannot allocate device memory (code under stream capture, tight

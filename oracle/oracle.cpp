@@ -420,6 +420,11 @@ struct Matrix {
         if (p < 0) throw std::runtime_error("Sparse matrix index not found");
         return values[(size_t)p];
     }
+    double index(size_t i, size_t j) const {   // Index<[usize; 2]> (dmatrix.rs:262-276): panics if absent
+        long p = pos(i, j);
+        if (p < 0) throw std::runtime_error("Sparse matrix index not found");
+        return values[(size_t)p];
+    }
     double get(size_t i, size_t j) const {   // dmatrix.rs:105-117
         long p = pos(i, j);
         return p < 0 ? 0.0 : values[(size_t)p];
@@ -1245,6 +1250,212 @@ using namespace orc;
 namespace {
 thread_local std::string g_err;
 
+// ---------------------------------------------------------------------------------
+// SIMPLE helpers (tools/hbya.rs) and the p-Poisson wall distance (tools/wall_distance.rs)
+// Vector<DIM> cell fields are n x DIM, face closures as in assembly (bci per boundary
+// face, bce per boundary face and component), in boundary-face order.
+// ---------------------------------------------------------------------------------
+// Face interpolation scheme terms (t0, t1) with a zero right-hand side: 0 Linear
+// (linear.rs:25-43), 1 Upwind(flux) (upwind.rs:26-33), 2 LimitedLinear(flux, limiters)
+// (limitedlinear.rs:30-76).
+template <int DIM>
+void interp_terms(const Mesh<DIM>& mesh, size_t f, int kind, const double* flux, const double* lim, double& t0,
+                  double& t1) {
+    const FaceData<DIM>& fd = mesh.face_data[f];
+    if (kind == 0) {
+        double w = linear_weight(mesh, f);
+        t0 = 1.0 * w;
+        t1 = 1.0 * (1.0 - w);
+    } else if (kind == 1) {
+        if (flux[f] > 0.0) { t0 = 1.0; t1 = 0.0; }
+        else { t0 = 0.0; t1 = 1.0; }
+    } else if (kind == 2) {
+        double fl = flux[f];
+        int u = fl >= 0.0 ? 0 : 1, dn = u == 0 ? 1 : 0;
+        double t[2] = {0.0, 0.0};
+        double l = lim[f];
+        double w = linear_weight(mesh, f);
+        double wu = u == 0 ? w : 1.0 - w;
+        t[u] = wu * l + (1.0 - l);
+        t[dn] = (1.0 - wu) * l;
+        if (fd.kind == 1) { t0 = 1.0 * t[0]; t1 = 1.0 * t[1]; }
+        else if (fl > 0.0) { t0 = 1.0; t1 = 0.0; }
+        else { t0 = 0.0; t1 = 1.0; }
+    } else {
+        throw std::runtime_error("bad interpolation scheme");
+    }
+}
+
+// compute_hbya_ainv (hbya.rs:15-50): ainv_i = vol_i / a_ii; hbya_i = (mrhs_i -
+// sum_{j != i} u_j a_ij) / a_ii over row i in column order.
+template <int DIM>
+void compute_hbya_ainv(const Mesh<DIM>& mesh, const Matrix& a, const double* mrhs, const double* vel, double* hbya,
+                       double* ainv) {
+    for (size_t i = 0; i < a.nrows; ++i) ainv[i] = mesh.cell_data[i].volume / a.index(i, i);
+    for (size_t i = 0; i < a.nrows; ++i) {
+        Vec<DIM> hi;
+        for (int d = 0; d < DIM; ++d) hi[d] = mrhs[i * DIM + d];
+        for (size_t k = a.sp.major_start(i); k < a.sp.major_end(i); ++k) {
+            const size_t j = a.sp.minors[k];
+            if (i != j) {
+                Vec<DIM> uj;
+                for (int d = 0; d < DIM; ++d) uj[d] = vel[j * DIM + d];
+                hi = hi - uj * a.values[k];
+            }
+        }
+        hi = hi / a.index(i, i);
+        for (int d = 0; d < DIM; ++d) hbya[i * DIM + d] = hi[d];
+    }
+}
+
+// intepolate_hbya_ainv_faces (hbya.rs:56-102)
+template <int DIM>
+void interpolate_hbya_ainv_faces(const Mesh<DIM>& mesh, const std::vector<size_t>& bnd_of_face, const double* hbya,
+                                 const double* ainv, int hkind, const double* hflux, const double* hlim, int akind,
+                                 const double* aflux, const double* alim, const double* ubci, const double* ubce,
+                                 const double* pbci, const double* pbce, double* hbyan_face, double* ainv_face) {
+    auto cellv = [&](const double* v, size_t c) {
+        Vec<DIM> x;
+        for (int d = 0; d < DIM; ++d) x[d] = v[c * DIM + d];
+        return x;
+    };
+    for (size_t f = 0; f < mesh.n_faces(); ++f) {
+        const FaceData<DIM>& fd = mesh.face_data[f];
+        const size_t i = fd.owner;
+        double hi, hj, ai, aj;
+        interp_terms(mesh, f, hkind, hflux, hlim, hi, hj);
+        interp_terms(mesh, f, akind, aflux, alim, ai, aj);
+        if (fd.kind == 1) {
+            const size_t j = fd.neighbor;
+            Vec<DIM> zero;
+            Vec<DIM> hf = cellv(hbya, i) * hi + cellv(hbya, j) * hj + zero;   // the schemes' zero rhs
+            hbyan_face[f] = dot(hf, fd.normal);
+            ainv_face[f] = ai * ainv[i] + aj * ainv[j] + 0.0;
+        } else {
+            const size_t b = bnd_of_face[f];
+            const double blhsu = ubci[b], blhsp = pbci[b], bvp = pbce[b];
+            Vec<DIM> bvu;
+            for (int d = 0; d < DIM; ++d) bvu[d] = ubce[b * DIM + d];
+            const bool p_wall = (std::fabs(blhsp - 1.0) < 1e-10) && (std::fabs(bvp) < 1e-10);
+            const bool u_wall = (std::fabs(blhsu) < 1e-10) && (std::fabs(dot(bvu, fd.normal)) < 1e-10);
+            hbyan_face[f] = p_wall && u_wall ? 0.0 : dot(cellv(hbya, i), fd.normal);
+            ainv_face[f] = ainv[i];
+        }
+    }
+}
+
+// correct_phi (hbya.rs:108-157): fgrads null for Orthogonal, else Corrected(fgrads, reduction)
+template <int DIM>
+void correct_phi(const Mesh<DIM>& mesh, const std::vector<size_t>& bnd_of_face, const double* hbyan_face,
+                 const double* ainv_face, const double* pressure, const double* fgrads, double reduction,
+                 const double* pbci, const double* pbce, double* phi) {
+    for (size_t f = 0; f < mesh.n_faces(); ++f) {
+        const FaceData<DIM>& fd = mesh.face_data[f];
+        const size_t i = fd.owner;
+        const double ainv = ainv_face[f], hbyan = hbyan_face[f];
+        double gi, gj, grhs;
+        fngrad_terms(mesh, f, 1, fgrads, reduction, gi, gj, &grhs);
+        double pgrad_n;
+        if (fd.kind == 1) {
+            pgrad_n = gi * pressure[i] + gj * pressure[fd.neighbor] + grhs;
+        } else {
+            const size_t b = bnd_of_face[f];
+            const double p_face = pbci[b] * pressure[i] + pbce[b];
+            pgrad_n = gi * pressure[i] + gj * p_face + grhs;
+        }
+        phi[f] = hbyan - ainv * pgrad_n;
+    }
+}
+
+// correct_velocity (hbya.rs:161-179)
+template <int DIM>
+void correct_velocity(const Mesh<DIM>& mesh, const double* hbya, const double* ainv, const double* pgrad,
+                      double* vel) {
+    for (size_t c = 0; c < mesh.n_cells(); ++c)
+        for (int d = 0; d < DIM; ++d) vel[c * DIM + d] = hbya[c * DIM + d] - pgrad[c * DIM + d] * ainv[c];
+}
+
+// f64::powi for the small positive exponents used here, as binary exponentiation
+// (the expansion the Rust compiler emits; the products commute, so the order of the
+// two factors of each product does not change a bit)
+inline double powi_pos(double x, int n) {
+    double r = 1.0;
+    bool first = true;
+    while (n) {
+        if (n & 1) { r = first ? x : r * x; first = false; }
+        n >>= 1;
+        if (n) x = x * x;
+    }
+    return r;
+}
+
+// wall_distance.rs:92-103: face diffusivity |grad_f|^(p-2), grad_f the Linear
+// interpolation of the cell gradients (the owner's on a boundary face)
+template <int DIM>
+void wall_face_viscosity(const Mesh<DIM>& mesh, const double* grads, int p, double* visc) {
+    for (size_t f = 0; f < mesh.n_faces(); ++f) {
+        const FaceData<DIM>& fd = mesh.face_data[f];
+        Vec<DIM> g0, gf;
+        for (int d = 0; d < DIM; ++d) g0[d] = grads[fd.owner * DIM + d];
+        if (fd.kind == 1) {
+            Vec<DIM> g1;
+            for (int d = 0; d < DIM; ++d) g1[d] = grads[fd.neighbor * DIM + d];
+            double t0, t1;
+            interp_terms(mesh, f, 0, nullptr, nullptr, t0, t1);
+            gf = g0 * t0 + g1 * t1;
+        } else {
+            gf = g0;
+        }
+        visc[f] = powi_pos(norm(gf), p - 2);
+    }
+}
+
+// wall_distance.rs:107-116
+template <int DIM>
+void wall_normalize(const Mesh<DIM>& mesh, const double* grads, int p, double* dist) {
+    for (size_t c = 0; c < mesh.n_cells(); ++c) {
+        Vec<DIM> g;
+        for (int d = 0; d < DIM; ++d) g[d] = grads[c * DIM + d];
+        const double gnorm = norm(g), u = dist[c], pf = (double)p;
+        dist[c] = -powi_pos(gnorm, p - 1) + std::pow((pf / (pf - 1.)) * u + powi_pos(gnorm, p), (pf - 1.) / pf);
+    }
+}
+
+// compute_wall_distance (wall_distance.rs:8-119) with p = 3: wall[b] marks boundary face
+// b (boundary order) as a wall (closure (0, 0)), the others are (1, 0).  Returns the
+// number of solves; info receives the last solve's record.
+template <int DIM>
+int compute_wall_distance(const Mesh<DIM>& mesh, const std::vector<size_t>& bnd_of_face, size_t n_bnd,
+                          const uint8_t* wall, int max_iterations, double tolerance, double rtol, double atol,
+                          int64_t max_iter, double* dist, Info* last) {
+    const int p = 3;
+    const size_t n = mesh.n_cells(), F = mesh.n_faces();
+    std::vector<double> grad(n * DIM, 0.0), visc(F, 1.0), src(n, -1.0), bci(n_bnd), bce(n_bnd, 0.0);
+    for (size_t b = 0; b < n_bnd; ++b) bci[b] = wall[b] ? 0.0 : 1.0;
+    // - laplacian(Corrected(&grad, 1.0), &visc) + source(&src)
+    const TermOp prog[4] = {{OP_LAPLACIAN_CORRECTED, 0, 1, 0, 1.0}, {OP_NEG, 0, 0, 0, 0.0},
+                            {OP_SOURCE, 0, 0, 0, 0.0}, {OP_ADD, 0, 0, 0, 0.0}};
+    int solves = 0;
+    for (int iter = 1; iter <= max_iterations; ++iter) {
+        const double* ff[1] = {visc.data()};
+        const double* cf[2] = {src.data(), grad.data()};
+        Matrix lhs;
+        std::vector<double> rhs;
+        Assembler<DIM>{mesh, prog, 4, ff, cf}.assemble(lhs, rhs, bci.data(), bce.data());
+        std::vector<double> sol(dist, dist + n);
+        ICPrec prec(lhs, 1);
+        Info info = conjugate_gradient(sol, lhs, rhs, prec, rtol, atol, max_iter);
+        ++solves;
+        for (size_t c = 0; c < n; ++c) dist[c] = sol[c];
+        if (last) *last = info;
+        if (info.initial_residual < tolerance) break;
+        compute_gradients(mesh, bnd_of_face, 1, dist, bci.data(), bce.data(), grad.data());
+        wall_face_viscosity(mesh, grad.data(), p, visc.data());
+    }
+    wall_normalize(mesh, grad.data(), p, dist);
+    return solves;
+}
+
 struct MeshHandle {
     int dim;
     Mesh<2>* m2 = nullptr;
@@ -1499,6 +1710,92 @@ int orc_solve(int solver, int64_t n, const int32_t* row_ptr, const int32_t* col,
         out_info[4] = (double)info.history.size();
         if (history)
             for (size_t k = 0; k < info.history.size() && (int64_t)k < history_cap; ++k) history[k] = info.history[k];
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// SIMPLE helpers (tools/hbya.rs).  Vector fields are n x DIM (DIM = the mesh's).
+int orc_hbya_ainv(void* hv, int64_t n, const int32_t* row_ptr, const int32_t* col, const double* val,
+                  const double* mrhs, const double* vel, double* hbya, double* ainv) {
+    try {
+        auto* h = (MeshHandle*)hv;
+        Matrix a = csr_to_matrix(n, row_ptr, col, val);
+        if (h->dim == 2) compute_hbya_ainv(*h->m2, a, mrhs, vel, hbya, ainv);
+        else compute_hbya_ainv(*h->m3, a, mrhs, vel, hbya, ainv);
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+int orc_interpolate_hbya_ainv_faces(void* hv, const double* hbya, const double* ainv, int hkind, const double* hflux,
+                                    const double* hlim, int akind, const double* aflux, const double* alim,
+                                    const double* ubci, const double* ubce, const double* pbci, const double* pbce,
+                                    double* hbyan_face, double* ainv_face) {
+    try {
+        auto* h = (MeshHandle*)hv;
+        if (h->dim == 2)
+            interpolate_hbya_ainv_faces(*h->m2, h->bnd_of_face, hbya, ainv, hkind, hflux, hlim, akind, aflux, alim,
+                                        ubci, ubce, pbci, pbce, hbyan_face, ainv_face);
+        else
+            interpolate_hbya_ainv_faces(*h->m3, h->bnd_of_face, hbya, ainv, hkind, hflux, hlim, akind, aflux, alim,
+                                        ubci, ubce, pbci, pbce, hbyan_face, ainv_face);
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+int orc_correct_phi(void* hv, const double* hbyan_face, const double* ainv_face, const double* pressure,
+                    const double* fgrads, double reduction, const double* pbci, const double* pbce, double* phi) {
+    auto* h = (MeshHandle*)hv;
+    if (h->dim == 2) correct_phi(*h->m2, h->bnd_of_face, hbyan_face, ainv_face, pressure, fgrads, reduction, pbci, pbce, phi);
+    else correct_phi(*h->m3, h->bnd_of_face, hbyan_face, ainv_face, pressure, fgrads, reduction, pbci, pbce, phi);
+    return 0;
+}
+
+int orc_correct_velocity(void* hv, const double* hbya, const double* ainv, const double* pgrad, double* vel) {
+    auto* h = (MeshHandle*)hv;
+    if (h->dim == 2) correct_velocity(*h->m2, hbya, ainv, pgrad, vel);
+    else correct_velocity(*h->m3, hbya, ainv, pgrad, vel);
+    return 0;
+}
+
+// wall distance pieces: the face diffusivity update and the final normalisation (p = 3)
+int orc_wall_face_viscosity(void* hv, const double* grads, double* visc) {
+    auto* h = (MeshHandle*)hv;
+    if (h->dim == 2) wall_face_viscosity(*h->m2, grads, 3, visc);
+    else wall_face_viscosity(*h->m3, grads, 3, visc);
+    return 0;
+}
+
+int orc_wall_normalize(void* hv, const double* grads, double* dist) {
+    auto* h = (MeshHandle*)hv;
+    if (h->dim == 2) wall_normalize(*h->m2, grads, 3, dist);
+    else wall_normalize(*h->m3, grads, 3, dist);
+    return 0;
+}
+
+// compute_wall_distance; out_info: [solves, iterations, initial_residual, final_residual]
+// of the last solve
+int orc_wall_distance(void* hv, const uint8_t* wall, int max_iterations, double tolerance, double rtol, double atol,
+                      int64_t max_iter, double* dist, double* out_info) {
+    try {
+        auto* h = (MeshHandle*)hv;
+        Info last;
+        int solves = h->dim == 2 ? compute_wall_distance(*h->m2, h->bnd_of_face, h->n_bnd, wall, max_iterations,
+                                                         tolerance, rtol, atol, max_iter, dist, &last)
+                                 : compute_wall_distance(*h->m3, h->bnd_of_face, h->n_bnd, wall, max_iterations,
+                                                         tolerance, rtol, atol, max_iter, dist, &last);
+        out_info[0] = solves;
+        out_info[1] = (double)last.iterations;
+        out_info[2] = last.initial_residual;
+        out_info[3] = last.final_residual;
         return 0;
     } catch (std::exception& e) {
         g_err = e.what();

@@ -67,6 +67,17 @@ def lib():
         L.orc_solve.restype = C.c_int
         L.orc_solve.argtypes = [C.c_int, C.c_int64, P, P, P, P, P, P, C.c_double, C.c_double, C.c_int64, P, P,
                                 C.c_int64]
+        for name, args in (("orc_hbya_ainv", [P, C.c_int64, P, P, P, P, P, P, P]),
+                           ("orc_interpolate_hbya_ainv_faces", [P, P, P, C.c_int, P, P, C.c_int, P, P, P, P, P, P,
+                                                                P, P]),
+                           ("orc_correct_phi", [P, P, P, P, P, C.c_double, P, P, P]),
+                           ("orc_correct_velocity", [P, P, P, P, P]),
+                           ("orc_wall_face_viscosity", [P, P, P]),
+                           ("orc_wall_normalize", [P, P, P]),
+                           ("orc_wall_distance", [P, P, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int64, P,
+                                                  P])):
+            getattr(L, name).restype = C.c_int
+            getattr(L, name).argtypes = args
         _lib = L
     return _lib
 
@@ -153,6 +164,88 @@ class OracleMesh:
         if rc:
             raise RuntimeError("oracle limiters: " + _err())
         return out
+
+    # ---- SIMPLE helpers (tools/hbya.rs) and wall distance (tools/wall_distance.rs) ----
+    # An interpolation scheme is "linear", ("upwind", flux) or ("limited_linear", flux,
+    # limiters); Vector fields are (n, dim).
+
+    @staticmethod
+    def _interp(spec):
+        if spec == "linear" or spec == ("linear",):
+            return 0, None, None
+        if spec[0] == "upwind":
+            return 1, np.ascontiguousarray(spec[1], dtype=np.float64), None
+        if spec[0] == "limited_linear":
+            return 2, np.ascontiguousarray(spec[1], dtype=np.float64), np.ascontiguousarray(spec[2],
+                                                                                             dtype=np.float64)
+        raise ValueError(spec)
+
+    def hbya_ainv(self, row_ptr, col, val, mrhs, velocity):
+        """compute_hbya_ainv (hbya.rs:15-50).  Returns (hbya (n, dim), ainv (n,))."""
+        n = self.n_cells
+        rp, ci = np.ascontiguousarray(row_ptr, np.int32), np.ascontiguousarray(col, np.int32)
+        va = np.ascontiguousarray(val, np.float64)
+        mr = np.ascontiguousarray(mrhs, np.float64)
+        ve = np.ascontiguousarray(velocity, np.float64)
+        hb, ai = np.empty((n, self.dim)), np.empty(n)
+        if lib().orc_hbya_ainv(self.h, n, _p(rp), _p(ci), _p(va), _p(mr), _p(ve), _p(hb), _p(ai)):
+            raise RuntimeError("oracle hbya_ainv: " + _err())
+        return hb, ai
+
+    def interpolate_hbya_ainv_faces(self, hbya, ainv, hbya_interp, ainv_interp, u_bc_i, u_bc_e, p_bc_i, p_bc_e):
+        """intepolate_hbya_ainv_faces (hbya.rs:56-102).  Returns (hbyan_face, ainv_face)."""
+        hk, hf, hl = self._interp(hbya_interp)
+        ak, af, al = self._interp(ainv_interp)
+        arr = lambda a: np.ascontiguousarray(a, np.float64)
+        hb, ai, ubi, ube, pbi, pbe = (arr(a) for a in (hbya, ainv, u_bc_i, u_bc_e, p_bc_i, p_bc_e))
+        hn, af_out = np.empty(self.n_faces), np.empty(self.n_faces)
+        if lib().orc_interpolate_hbya_ainv_faces(self.h, _p(hb), _p(ai), hk, _p(hf), _p(hl), ak, _p(af), _p(al),
+                                                 _p(ubi), _p(ube), _p(pbi), _p(pbe), _p(hn), _p(af_out)):
+            raise RuntimeError("oracle interpolate_hbya_ainv_faces: " + _err())
+        return hn, af_out
+
+    def correct_phi(self, hbyan_face, ainv_face, pressure, p_bc_i, p_bc_e, corrected_grads=None, reduction=1.0):
+        """correct_phi (hbya.rs:108-157) with Orthogonal, or Corrected on corrected_grads."""
+        arr = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)
+        hn, af, pr, cg, pbi, pbe = (arr(a) for a in (hbyan_face, ainv_face, pressure, corrected_grads, p_bc_i,
+                                                     p_bc_e))
+        phi = np.empty(self.n_faces)
+        lib().orc_correct_phi(self.h, _p(hn), _p(af), _p(pr), _p(cg), reduction, _p(pbi), _p(pbe), _p(phi))
+        return phi
+
+    def correct_velocity(self, hbya, ainv, pressure_gradient):
+        """correct_velocity (hbya.rs:161-179).  Returns velocity (n, dim)."""
+        arr = lambda a: np.ascontiguousarray(a, np.float64)
+        hb, ai, pg = arr(hbya), arr(ainv), arr(pressure_gradient)
+        v = np.empty((self.n_cells, self.dim))
+        lib().orc_correct_velocity(self.h, _p(hb), _p(ai), _p(pg), _p(v))
+        return v
+
+    def wall_face_viscosity(self, grads):
+        """wall_distance.rs:92-103 (p = 3): |Linear interpolation of grads| per face."""
+        g = np.ascontiguousarray(grads, np.float64)
+        out = np.empty(self.n_faces)
+        lib().orc_wall_face_viscosity(self.h, _p(g), _p(out))
+        return out
+
+    def wall_normalize(self, grads, dist):
+        """wall_distance.rs:107-116 (p = 3)."""
+        g = np.ascontiguousarray(grads, np.float64)
+        d = np.array(dist, dtype=np.float64)
+        lib().orc_wall_normalize(self.h, _p(g), _p(d))
+        return d
+
+    def wall_distance(self, wall, max_iterations, tolerance, rtol, atol, max_iter, initial=None):
+        """compute_wall_distance (wall_distance.rs:8-119); wall: bool per boundary face.
+        Returns (distance, {solves, iterations, initial_residual, final_residual})."""
+        w = np.ascontiguousarray(wall, dtype=np.uint8)
+        d = np.zeros(self.n_cells) if initial is None else np.array(initial, dtype=np.float64)
+        info = np.zeros(4)
+        if lib().orc_wall_distance(self.h, _p(w), int(max_iterations), tolerance, rtol, atol, int(max_iter), _p(d),
+                                   _p(info)):
+            raise RuntimeError("oracle wall_distance: " + _err())
+        return d, dict(solves=int(info[0]), iterations=int(info[1]), initial_residual=info[2],
+                       final_residual=info[3])
 
     def assemble(self, program, face_fields, cell_fields, bc_i, bc_e):
         """program: list of (op, arg0, arg1, arg2, scalar). Returns (row_ptr, col, values, rhs)."""

@@ -822,3 +822,164 @@ def test_user_term_reproduces_builtin_terms(mesh):
     assert not np.array_equal(swapped[2], ref[2])
     swapped = om.assemble(prog, [li, lj, r], [-f, np.zeros(n)], bci, bce)
     assert not np.array_equal(swapped[3], ref[3])
+
+
+# ---- SIMPLE helpers (tools/hbya.rs) and the wall distance (tools/wall_distance.rs) ----
+
+def _momentum_like(m, seed):
+    """A diagonally dominant matrix on the mesh pattern: time(Euler) + convection(Upwind)
+    - laplacian(Orthogonal), assembled by the oracle."""
+    om = O.OracleMesh(m)
+    rng = np.random.default_rng(seed)
+    n, F, nb = om.n_cells, om.n_faces, om.n_bnd
+    OP = O.OP
+    prog = [(OP["TIME_EULER"], 0, -1, -1, 0.01), (OP["CONVECTION_UPWIND"], 0, 0, 0, 0.0), (OP["ADD"], 0, 0, 0, 0.0),
+            (OP["LAPLACIAN_ORTHOGONAL"], 1, 0, 0, 0.0), (OP["SUB"], 0, 0, 0, 0.0)]
+    rp, ci, va, b = om.assemble(prog, [rng.normal(size=F), rng.uniform(0.5, 2.0, F)], [rng.normal(size=n)],
+                                rng.uniform(0, 1, nb), rng.normal(size=nb))
+    return om, rp, ci, va
+
+
+@pytest.mark.parametrize("mesh", ["tri", "tet"])
+def test_hbya_reproduces_the_velocity_that_solves_the_momentum_rows(mesh):
+    """compute_hbya_ainv (hbya.rs:15-50): hbya_i = (mrhs_i - sum_{j != i} a_ij u_j) / a_ii.
+    With mrhs = A u (component by component) that is u itself, up to rounding; ainv is
+    vol_i / a_ii.  Dropping the diagonal exclusion or a sign breaks the first."""
+    m = meshgen.tri2d(8, seed=2) if mesh == "tri" else meshgen.tet3d(3, seed=2)
+    om, rp, ci, va = _momentum_like(m, 31)
+    n, D = om.n_cells, om.dim
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    u = np.random.default_rng(32).normal(size=(n, D))
+    hb, ai = om.hbya_ainv(rp, ci, va, A @ u, u)
+    assert np.allclose(hb, u, rtol=1e-11, atol=1e-12)
+    assert np.array_equal(ai, om.geometry()["cell_volume"] / A.diagonal())
+    with pytest.raises(RuntimeError):     # a row without its diagonal: the reference's index panics
+        keep = ci != np.repeat(np.arange(n), np.diff(rp))
+        rp2 = np.concatenate([[0], np.cumsum([keep[rp[i]:rp[i + 1]].sum() for i in range(n)])]).astype(np.int32)
+        om.hbya_ainv(rp2, ci[keep], va[keep], A @ u, u)
+
+
+@pytest.mark.parametrize("mesh", ["tri", "tet"])
+def test_hbya_face_interpolation_closed_forms(mesh):
+    """intepolate_hbya_ainv_faces (hbya.rs:56-102): a constant hbya U interpolates to U.n on
+    every face with Linear and LimitedLinear weights (they sum to one), a constant ainv to
+    itself; Upwind takes the upwind cell's value exactly; on a boundary face the owner's
+    U.n unless both closures describe a wall ((0, u.n = 0) and (1, 0)), where it is 0."""
+    m = meshgen.tri2d(8, seed=3) if mesh == "tri" else meshgen.tet3d(3, seed=3)
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    n, F, nb, D = om.n_cells, om.n_faces, om.n_bnd, om.dim
+    rng = np.random.default_rng(33)
+    U = rng.normal(size=D)
+    hb = np.tile(U, (n, 1))
+    ai = np.full(n, 0.37)
+    walls = rng.uniform(size=nb) < 0.5
+    ubi, ube = np.where(walls, 0.0, 1.0), np.zeros((nb, D))
+    pbi, pbe = np.ones(nb), np.zeros(nb)
+    pbi[~walls] = 0.0                       # outflow-like faces: fixed pressure, not a wall
+    bf = np.nonzero(m["face_patch"] >= 0)[0]
+    un = g["face_normal"] @ U
+    flux, lim = rng.normal(size=F), rng.uniform(0, 1, F)
+    for scheme in ("linear", ("limited_linear", flux, lim)):
+        hn, af = om.interpolate_hbya_ainv_faces(hb, ai, scheme, scheme, ubi, ube, pbi, pbe)
+        want = un.copy()
+        want[bf[walls]] = 0.0
+        assert np.allclose(hn, want, rtol=1e-12, atol=1e-13)
+        assert np.allclose(af, 0.37, rtol=1e-14, atol=0)
+        assert np.all(hn[bf[walls]] == 0.0) and np.array_equal(af[bf], np.full(nb, 0.37))
+    hbv = rng.normal(size=(n, D))
+    hn, af = om.interpolate_hbya_ainv_faces(hbv, rng.normal(size=n), ("upwind", flux), "linear", ubi, ube, pbi, pbe)
+    inner = g["neighbor"] >= 0
+    up = np.where(flux > 0, g["owner"], np.maximum(g["neighbor"], 0))
+    want = (hbv[up] * g["face_normal"]).sum(1)
+    assert np.allclose(hn[inner], want[inner], rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("mesh", ["tri", "tet"])
+def test_correct_phi_with_the_exact_pressure_gradient(mesh):
+    """correct_phi (hbya.rs:108-157) with Corrected and the exact gradient of a linear
+    pressure p = a.x (exact Dirichlet values on the boundary) takes the face-normal
+    gradient a.n on every face, so phi = hbyan - ainv a.n; Orthogonal does not on these
+    non-orthogonal meshes."""
+    m = meshgen.tri2d(8, seed=4) if mesh == "tri" else meshgen.tet3d(3, seed=4)
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    n, F, nb, D = om.n_cells, om.n_faces, om.n_bnd, om.dim
+    rng = np.random.default_rng(34)
+    a = rng.normal(size=D)
+    p = g["cell_center"] @ a
+    bf = np.nonzero(m["face_patch"] >= 0)[0]
+    pbi, pbe = np.zeros(nb), g["face_center"][bf] @ a
+    hn, af = rng.normal(size=F), rng.uniform(0.5, 2.0, F)
+    grads = np.tile(a, (n, 1))
+    phi = om.correct_phi(hn, af, p, pbi, pbe, corrected_grads=grads, reduction=1.0)
+    want = hn - af * (g["face_normal"] @ a)
+    assert np.allclose(phi, want, rtol=1e-10, atol=1e-11)
+    phi_o = om.correct_phi(hn, af, p, pbi, pbe)
+    assert not np.allclose(phi_o, want, rtol=1e-6, atol=1e-8)
+
+
+def test_correct_velocity_undoes_the_pressure_correction():
+    """correct_velocity (hbya.rs:161-179): velocity = hbya - ainv * grad p, so hbya = u +
+    ainv * g gives back u."""
+    om = O.OracleMesh(meshgen.tet3d(3, seed=5))
+    n, D = om.n_cells, om.dim
+    rng = np.random.default_rng(35)
+    u, gp, ai = rng.normal(size=(n, D)), rng.normal(size=(n, D)), rng.uniform(0.1, 2.0, n)
+    v = om.correct_velocity(u + ai[:, None] * gp, ai, gp)
+    assert np.allclose(v, u, rtol=1e-14, atol=1e-14)
+
+
+def test_wall_face_viscosity_and_normalisation_closed_forms():
+    """wall_distance.rs:92-116 with p = 3.  The face diffusivity is |Linear interpolation
+    of the gradients|, so a constant gradient a gives |a| on every face.  The final
+    normalisation -|g|^2 + (3/2 u + |g|^3)^(2/3) is exact for the 1D p-Poisson solution
+    with a wall at y = 0 and zero gradient at y = h: u = (2/3)(h^1.5 - (h - y)^1.5),
+    |u'| = (h - y)^0.5, which it maps to y."""
+    m = meshgen.tri2d(10, seed=6)
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    n = om.n_cells
+    a = np.array([0.3, -1.2])
+    visc = om.wall_face_viscosity(np.tile(a, (n, 1)))
+    assert np.allclose(visc, np.linalg.norm(a), rtol=1e-14, atol=0)
+    h = 1.25
+    y = g["cell_center"][:, 1]
+    u = (2.0 / 3.0) * (h ** 1.5 - (h - y) ** 1.5)
+    grads = np.stack([np.zeros(n), np.sqrt(h - y)], axis=1)
+    d = om.wall_normalize(grads, u)
+    assert np.allclose(d, y, rtol=1e-12, atol=1e-13)
+
+
+def test_wall_distance_loop_against_independent_solves():
+    """compute_wall_distance (wall_distance.rs:59-105): round 1 solves
+    -laplacian(Corrected(0, 1), 1) + source(-1) with walls at y = 0 and y = 1 (closure
+    (0, 0)) and zero gradient elsewhere ((1, 0)); round 2 the same with the new gradients
+    and face diffusivity |grad_f|.  Each round is checked against scipy's direct solve of
+    the oracle-assembled system, and the result against the normalisation of the last
+    round's solution with the gradients taken after it.  A tolerance above the first
+    initial residual stops after one solve, with zero gradients in the normalisation."""
+    m = meshgen.tri2d(12, seed=7)
+    om = O.OracleMesh(m)
+    n, F, nb = om.n_cells, om.n_faces, om.n_bnd
+    patch = m["face_patch"][m["face_patch"] >= 0]
+    wall = (patch == 0) | (patch == 2)
+    bci, bce = np.where(wall, 0.0, 1.0), np.zeros(nb)
+    OP = O.OP
+    prog = [(OP["LAPLACIAN_CORRECTED"], 0, 1, 0, 1.0), (OP["NEG"], 0, 0, 0, 0.0), (OP["SOURCE"], 0, 0, 0, 0.0),
+            (OP["ADD"], 0, 0, 0, 0.0)]
+    visc, grads, src = np.ones(F), np.zeros(n * 2), np.full(n, -1.0)
+    for rounds in (1, 2):
+        rp, ci, va, b = om.assemble(prog, [visc], [src, grads], bci, bce)
+        u = spl.spsolve(sp.csr_matrix((va, ci, rp), shape=(n, n)).tocsc(), b)
+        grads = om.gradients(u, bci, bce).ravel()
+        visc = om.wall_face_viscosity(grads.reshape(n, 2))
+        d, info = om.wall_distance(wall, rounds, 1e-30, 1e-12, 0.0, 2000)
+        assert info["solves"] == rounds
+        want = om.wall_normalize(grads.reshape(n, 2), u)
+        assert np.allclose(d, want, rtol=1e-8, atol=1e-10), rounds
+    d1, info = om.wall_distance(wall, 5, 10.0, 1e-12, 0.0, 2000)
+    rp, ci, va, b = om.assemble(prog, [np.ones(F)], [src, np.zeros(n * 2)], bci, bce)
+    u1 = spl.spsolve(sp.csr_matrix((va, ci, rp), shape=(n, n)).tocsc(), b)
+    assert info["solves"] == 1
+    assert np.allclose(d1, (1.5 * u1) ** (2.0 / 3.0), rtol=1e-8, atol=1e-10)
