@@ -82,9 +82,15 @@ def leg_batched(args, fvg, workloads):
     ctx.sync()
     res = {}
     xs = {}
-    for name, prec in (("fp64", fvg.PRECISION_FP64), ("mixed", fvg.PRECISION_MIXED)):
+    # fp64 and mixed_one_workgroup share the schedule (one workgroup per system, exact
+    # level-scheduled sweeps), so their ratio is the precision alone; mixed spreads each
+    # system over a cluster of workgroups with Jacobi-iterated factor applies
+    for name, prec in (("fp64", fvg.PRECISION_FP64), ("mixed_one_workgroup", fvg.PRECISION_MIXED),
+                       ("mixed", fvg.PRECISION_MIXED)):
         opts = fvg.LinearSolverOptions(1e-8, 0.0, 5000, precision=prec)
         bt = fvg.Batch(ctx, A, M, "cg", opts)
+        if name == "mixed_one_workgroup":
+            bt.set_cluster(0)
         tune = bt.tune(B, X)
         times = []
         for _ in range(max(1, args.steps)):
@@ -109,10 +115,13 @@ def leg_batched(args, fvg, workloads):
                      "autotune_ms": {str(k): 1e3 * v for k, v in tune.items()},
                      "iterations_max": int(it.max()), "iterations_mean": float(it.mean()),
                      "worst_true_relative_residual": worst}
+        if bt.rows_per_group == 0:
+            res[name]["phases"] = batch_phase_rates(bt, A, ns, B, X, best, 8 if prec == fvg.PRECISION_FP64 else 4)
         bt.close()
-    err = max(float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(xs["mixed"], xs["fp64"]))
-    res["mixed"]["speedup_vs_fp64"] = res["mixed"]["solves_per_s"] / res["fp64"]["solves_per_s"]
-    res["mixed"]["max_rel_error_vs_fp64_solution"] = err
+    for name in ("mixed_one_workgroup", "mixed"):
+        err = max(float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(xs[name], xs["fp64"]))
+        res[name]["speedup_vs_fp64"] = res[name]["solves_per_s"] / res["fp64"]["solves_per_s"]
+        res[name]["max_rel_error_vs_fp64_solution"] = err
     res["systems"] = len(A)
     res["total_cells"] = int(sum(ns))
     res["largest_system"] = int(max(ns))
@@ -121,6 +130,35 @@ def leg_batched(args, fvg, workloads):
                        "launch per batch")
     ctx.close()
     return res
+
+
+def batch_phase_rates(bt, A, ns, B, X, batch_s, w):
+    """SpMV and triangular-sweep rates of a one-workgroup-per-system batch from its phase
+    timers (one more solve with timing on): per system, (iterations + 1) SpMVs and lower
+    and upper IC(0) sweeps of the algorithmic bytes of spmv_bytes / sweep_bytes at value
+    width w.  per_workgroup: a phase's bytes over its time summed over the systems (the
+    rate one CU reaches); device: all systems' bytes of the phase over the batch's wall
+    time (what the device moved for that phase while the batch ran)."""
+    bt.set_phase_timing(True)
+    for x in X:
+        x.fill(0.0)
+    infos = bt.solve(B, X, history=False)
+    us = bt.phase_times()
+    bt.set_phase_timing(False)
+    out = {}
+    for j, name in ((0, "spmv"), (1, "lower_sweep"), (2, "upper_sweep")):
+        tot_b, tot_us = 0.0, 0.0
+        for k, (a, n) in enumerate(zip(A, ns)):
+            calls = infos[k].iterations + 1
+            nnz_s = (a.nnz - n) // 2
+            tot_b += calls * (spmv_bytes(n, a.nnz, w) if j == 0 else sweep_bytes(n, nnz_s, w))
+            tot_us += us[k, j]
+        per_wg = tot_b / (tot_us * 1e-6) / 1e9 if tot_us > 0 else 0.0
+        dev = tot_b / batch_s / 1e9
+        out[name] = {"algorithmic_bytes": tot_b, "busy_ms_summed": 1e-3 * tot_us, "per_workgroup_GB/s": per_wg,
+                     "device_GB/s": dev, "frac": dev / HBM_PEAK_GBS}
+    out["solve_ms_largest_system"] = 1e-3 * float(us[int(np.argmax(ns)), 3])
+    return out
 
 
 def leg_serving(args, fvg, workloads):
@@ -216,8 +254,14 @@ def leg_convdiff(args, fvg, workloads):
                        "gradient; assemble + ILU(0) + BiCGStab to 1e-8 rel. residual, x0 = 0",
            "n_cells": mesh.n_cells, "nnz": mesh.nnz, "mesh_build_ms": time_mesh_build(fvg, ctx, raw)}
     xs = {}
-    for name, prec in (("fp64", fvg.PRECISION_FP64), ("mixed", fvg.PRECISION_MIXED)):
+    n, nnz = mesh.n_cells, mesh.nnz
+    nnz_s = (nnz - n) // 2
+    # mixed: FP32 inner iterations applying ILU(0) by Jacobi-iterated sweeps (the default);
+    # mixed_exact_sweeps: the same with exact level-scheduled FP32 sweeps
+    for name, prec, exact in (("fp64", fvg.PRECISION_FP64, 0), ("mixed", fvg.PRECISION_MIXED, 0),
+                              ("mixed_exact_sweeps", fvg.PRECISION_MIXED, 1)):
         opts = fvg.LinearSolverOptions(1e-8, 0.0, 4000, precision=prec)
+        ctx.set_tuning(fvg.TUNE_INNER_EXACT, exact)
 
         def step():
             asm.run(lhs, rhs, sync=False)
@@ -227,6 +271,9 @@ def leg_convdiff(args, fvg, workloads):
 
         step()
         ctx.sync()
+        if name == "fp64":
+            ctx.kernel_stats(reset=True)
+            ctx.enable_timing(True, every=TIMING_EVERY)
         t = time.perf_counter()
         steps = max(1, min(args.steps, 3))
         for _ in range(steps):
@@ -239,9 +286,26 @@ def leg_convdiff(args, fvg, workloads):
         xs[name] = x.numpy()
         out[name] = {"solves_per_s": 1.0 / el, "ms_per_solve": 1e3 * el, "iterations": info.iterations,
                      "true_relative_residual": float(np.linalg.norm(bh - y.numpy()) / np.linalg.norm(bh))}
-    out["mixed"]["speedup_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
-    out["mixed"]["rel_error_vs_fp64_solution"] = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) /
-                                                       np.linalg.norm(xs["fp64"]))
+        if name == "fp64":
+            ctx.enable_timing(False)
+            ks = ctx.kernel_stats(reset=True)
+            sp_ms = ks["spmv_ms"] / max(1, ks["spmv_calls"])
+            sw_ms = ks["sweep_ms"] / max(1, ks["sweep_calls"])
+            # ILU(0) sweeps alternate lower (unit diagonal) and upper: their mean bytes
+            sw_b = 0.5 * ((8 + 4) * nnz_s + 4 * (n + 1) + 2 * 8 * n) + 0.5 * sweep_bytes(n, nnz_s)
+            out["fp64"]["kernels"] = {
+                "spmv": {"avg_ms": sp_ms, "algorithmic_bytes": spmv_bytes(n, nnz),
+                         "GB/s": spmv_bytes(n, nnz) / (sp_ms * 1e-3) / 1e9,
+                         "frac": spmv_bytes(n, nnz) / (sp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                "ilu0_sweep": {"span": "one triangular solve (lower or upper), HIP events around one launch group "
+                                       f"in {TIMING_EVERY}", "avg_ms": sw_ms, "algorithmic_bytes": sw_b,
+                               "GB/s": sw_b / (sw_ms * 1e-3) / 1e9, "frac": sw_b / (sw_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                               "levels": list(ilu.levels())}}
+    ctx.set_tuning(fvg.TUNE_INNER_EXACT, 0)
+    for name in ("mixed", "mixed_exact_sweeps"):
+        out[name]["speedup_vs_fp64"] = out[name]["solves_per_s"] / out["fp64"]["solves_per_s"]
+        out[name]["rel_error_vs_fp64_solution"] = float(np.linalg.norm(xs[name] - xs["fp64"]) /
+                                                        np.linalg.norm(xs["fp64"]))
     ctx.close()
     return out
 
@@ -539,6 +603,17 @@ def main():
     m_elapsed = time.perf_counter() - t0
     m_true = true_residual()
     m_err = float(np.linalg.norm(x.numpy() - x_fp64) / np.linalg.norm(x_fp64))
+    # the same with exact level-scheduled FP32 sweeps in the inner iterations
+    ctx.set_tuning(fvg.TUNE_INNER_EXACT, 1)
+    step(mixed=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    einfos = [step(mixed=True) for _ in range(args.steps)]
+    ctx.sync()
+    e_elapsed = time.perf_counter() - t0
+    e_true = true_residual()
+    e_err = float(np.linalg.norm(x.numpy() - x_fp64) / np.linalg.norm(x_fp64))
+    ctx.set_tuning(fvg.TUNE_INNER_EXACT, 0)
 
     spmv_avg_ms = ks["spmv_ms"] / max(1, ks["spmv_calls"])
     sweep_avg_ms = ks["sweep_ms"] / max(1, ks["sweep_calls"])
@@ -577,7 +652,13 @@ def main():
         "mixed": {"solves_per_s": args.steps / m_elapsed, "ms_per_step": 1e3 * m_elapsed / args.steps,
                   "speedup_vs_fp64": (args.steps / m_elapsed) / value, "inner_iterations": minfos[-1].iterations,
                   "outer_iterations": minfos[-1].outer_iterations, "true_relative_residual": m_true,
-                  "rel_error_vs_fp64_solution": m_err},
+                  "rel_error_vs_fp64_solution": m_err, "inner_apply": "IC(0) by Jacobi-iterated FP32 sweeps",
+                  "exact_sweeps": {"solves_per_s": args.steps / e_elapsed, "ms_per_step": 1e3 * e_elapsed / args.steps,
+                                   "speedup_vs_fp64": (args.steps / e_elapsed) / value,
+                                   "inner_iterations": einfos[-1].iterations,
+                                   "outer_iterations": einfos[-1].outer_iterations, "true_relative_residual": e_true,
+                                   "rel_error_vs_fp64_solution": e_err,
+                                   "inner_apply": "IC(0) by exact level-scheduled FP32 sweeps"}},
         "mesh": {"build_ms": mesh_build_ms, "assembly_ms": asm_ms,
                  "assembly_algorithmic_bytes": assembly_bytes(mesh, pr),
                  "assembly_GB/s": assembly_bytes(mesh, pr) / (asm_ms * 1e-3) / 1e9},
