@@ -348,6 +348,10 @@ int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
         if (value > 2) return fail(FVG_E_INVALID, "assemble kernel must be 0, 1 or 2");
         c->tune_assemble = value;
         return FVG_OK;
+    case FVG_TUNE_CG_FUSION:
+        if (value > 1) return fail(FVG_E_INVALID, "CG fusion must be 0 or 1");
+        c->tune_cg_fusion = value;
+        return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }
@@ -367,6 +371,7 @@ int fvg_ctx_get_tuning(const fvg_ctx* c, int32_t knob, int64_t* value) {
     case FVG_TUNE_SWEEP_WITHHOLD: *value = c->tune_withhold; return FVG_OK;
     case FVG_TUNE_SOLVE_ORDER: *value = c->tune_solve_order; return FVG_OK;
     case FVG_TUNE_ASSEMBLE: *value = c->tune_assemble; return FVG_OK;
+    case FVG_TUNE_CG_FUSION: *value = c->tune_cg_fusion; return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }

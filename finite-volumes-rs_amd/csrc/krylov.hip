@@ -468,6 +468,10 @@ int KRun::finish(fvg_solver_info* infos, double* const* histories, int64_t cap) 
 #define SCALAR(step, part0)                                                                                \
     hipLaunchKernelGGL(k_scalar<step>, dim3(L.n_inst), dim3(kScalarBlock), 0, c->stream, K.st, part0, stride, \
                        nb, L.blk_ptr, K.hist, K.hist_cap, K.flags, K.flags + 1)
+// the same over `count` partials (one per block of a fused sweep)
+#define SCALARN(step, part0, count)                                                                        \
+    hipLaunchKernelGGL(k_scalar<step>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part0, stride,      \
+                       (int)(count), nullptr, K.hist, K.hist_cap, K.flags, K.flags + 1)
 
 // iterations enqueued between two polls of the "all systems done" flag
 constexpr int kPollBatch = 4;
@@ -537,6 +541,18 @@ static int run_cg_pos(fvg_ctx* c, const fvg_prec* p, const double* b, double* x,
     hipLaunchKernelGGL(k_cg_start_pos, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, r, pp, part, P);
     SCALAR(CG_RZ0, part);
     FVG_HIP(hipGetLastError());
+    // the residual update with r.r and the r.z product run inside the two sweeps
+    const int64_t lo_nb = p->lower.nblk, up_nb = p->upper.nblk;
+    const bool fuse = c->tune_cg_fusion == 0 && lo_nb <= stride && up_nb <= stride && pos_fusable(c, p);
+    SweepFuse lf, uf;
+    lf.alpha = &K.st[0].alpha;
+    lf.ap = ap;
+    lf.x = xp;
+    lf.pp = pp;
+    lf.r = r;
+    lf.partials = part;
+    uf.r = r;
+    uf.partials = part;
     for (int64_t it = 0;; ++it) {
         SpmvEpilogue e1;   // ap = A p with p.ap
         e1.dotw = pp;
@@ -546,11 +562,20 @@ static int run_cg_pos(fvg_ctx* c, const fvg_prec* p, const double* b, double* x,
         e1.dd = true;
         FVG_TRY(launch_spmv(c, P.pa, pp, ap, e1));
         SCALAR(CG_ALPHA, part);
-        hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, nullptr, pp, ap, xp, r, part);
-        SCALAR(CG_ERROR, part);
-        FVG_TRY(apply_pos(c, p, r, all_done));
-        hipLaunchKernelGGL(k_cg_rz_pos, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, r, (const double*)P.up_x, part);
-        SCALAR(CG_BETA, part);
+        if (fuse) {
+            FVG_TRY(apply_pos_fused(c, p, r, all_done, lf, uf, [&]() {
+                SCALARN(CG_ERROR, part, lo_nb);
+                return FVG_OK;
+            }));
+            SCALARN(CG_BETA, part, up_nb);
+        } else {
+            hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, nullptr, pp, ap, xp, r, part);
+            SCALAR(CG_ERROR, part);
+            FVG_TRY(apply_pos(c, p, r, all_done));
+            hipLaunchKernelGGL(k_cg_rz_pos, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, r, (const double*)P.up_x,
+                               part);
+            SCALAR(CG_BETA, part);
+        }
         hipLaunchKernelGGL(k_cg_p_pos, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, pp, P);
         FVG_HIP(hipGetLastError());
         if ((it + 1) % kPollBatch == 0) {

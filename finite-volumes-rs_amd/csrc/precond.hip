@@ -472,13 +472,16 @@ __global__ void k_gather_rows(int64_t n, const int32_t* __restrict__ rows, const
     if (p < n) out[p] = v[rows[p]];
 }
 
-template <class T, int K>
+template <class T, int K, int FUSE = 0>
 struct BlockRow {
     int p;
     bool ok;
     uint64_t dw[K / 4];   // K 16-bit LDS slots, unpacked when the row is solved
     double2 a[K / 2];     // K coefficients
     T bv, sc;
+    // fused CG work (kFuseXR: ap, x and p of this position; kFuseRZ: r of its lower
+    // position)
+    T f[FUSE == 1 ? 3 : FUSE == 2 ? 1 : 0];
 };
 
 // SPIN: the wait limit is the spin_limit argument (a tuning override) instead of the
@@ -487,20 +490,23 @@ struct BlockRow {
 // limit: the forced time-out of the tests.  The default launch keeps the constant limit
 // and always publishes (a runtime limit in the polling loop measured 12 us slower per
 // sweep).
-template <class T, bool SCALE, int K, bool SPIN>
+template <class T, bool SCALE, int K, bool SPIN, int FUSE>
 __global__ void __launch_bounds__(1024)
     k_sweep_blocked(int nlevels, int64_t n, int W, int X, const int32_t* __restrict__ brange,
                     const int32_t* __restrict__ blvl, const int32_t* __restrict__ bxptr,
                     const int32_t* __restrict__ bxidx, const uint16_t* __restrict__ bdep,
                     const T* __restrict__ bval, const T* __restrict__ bpos,
                     const T* __restrict__ spos, T* xpos, int32_t* __restrict__ err,
-                    const int32_t* __restrict__ skip, int spin_limit, int withhold, int brev) {
+                    const int32_t* __restrict__ skip, int spin_limit, int withhold, int brev, SweepFuse F) {
     using U = typename Bits<T>::U;
     extern __shared__ __align__(16) unsigned char smem[];
     if (skip && *skip) return;
     const int blk = blockIdx.x;
     const int Lb = brange[2 * blk], Le = brange[2 * blk + 1];
-    if (Lb >= Le) return;
+    if (Lb >= Le) {
+        if (FUSE != kFuseNone && threadIdx.x == 0) put_dd(F.partials, 0, 0, blk, DD{0.0, 0.0});
+        return;
+    }
     T* ring = reinterpret_cast<T*>(smem);
     T* stage = ring + kBlockRing * W;
     int* s_lv = reinterpret_cast<int*>(stage + 2 * X);
@@ -525,6 +531,7 @@ __global__ void __launch_bounds__(1024)
     // both loops run to a multiple of kBlockAhead levels (the steps past Le only meet the
     // barrier), so the unrolled compute loop has no exit but its back edge
     const int Lend = Lb + (Le - Lb + kBlockAhead - 1) / kBlockAhead * kBlockAhead;
+    DD acc{0.0, 0.0};   // fused dot product of this thread's rows
 
     if (loader) {
         // The outside values of level l into staging slot l & 1: each lane issues all of its
@@ -567,15 +574,16 @@ __global__ void __launch_bounds__(1024)
             if (l + 1 < Le) fetch(l + 1);
             __syncthreads();
         }
-        return;
-    }
+        if constexpr (FUSE == kFuseNone) return;
+    } else {
 
     // compute threads: the data of this thread's row of level l, all in position order.
     // The loads are unconditional (indices clamped into range) so that each level issues
     // the same number of them and the waits before use count only what is older.  (A
     // gather of b by row index inside this kernel, two or three levels ahead, measured
     // 20 % slower than the separate gather into position order.)
-    auto prefetch = [&](int l, BlockRow<T, K>& r) {
+    const T alpha = FUSE == kFuseXR ? *F.alpha : T(0);
+    auto prefetch = [&](int l, BlockRow<T, K, FUSE>& r) {
         const int lc = l < Le ? l : Le - 1;
         const int p0 = s_lv[lc], p1 = s_lv[lc + 1];
         const bool ok = l < Le && p0 + ct < p1;
@@ -588,15 +596,30 @@ __global__ void __launch_bounds__(1024)
         for (int g = 0; g < K / 4; ++g) r.dw[g] = dw[g];
 #pragma unroll
         for (int h = 0; h < K / 2; ++h) r.a[h] = a2[h];
-        r.bv = bpos[brev ? n - 1 - p : p];   // brev: the right-hand side in reverse order
+        // brev: the right-hand side in reverse order; kFuseXR reads r through F (it writes it)
+        r.bv = FUSE == kFuseXR ? F.r[p] : bpos[brev ? n - 1 - p : p];
         r.sc = SCALE ? spos[p] : T(1);
         r.ok = ok;
+        if constexpr (FUSE == kFuseXR) {
+            r.f[0] = F.ap[p];
+            r.f[1] = F.x[p];
+            r.f[2] = F.pp[p];
+        } else if constexpr (FUSE == kFuseRZ) {
+            r.f[0] = F.r[n - 1 - p];
+        }
     };
-    auto solve = [&](int l, const BlockRow<T, K>& r) {
+    auto solve = [&](int l, const BlockRow<T, K, FUSE>& r) {
         if (!r.ok) return;
         // (reading only the row's used slots measured faster than issuing all K reads at
         // once with the unused ones pointed at slot 0)
         T s = r.bv;
+        if constexpr (FUSE == kFuseXR) {   // conjugate_gradient.rs:77-80, as k_cg_xr
+            F.x[r.p] = r.f[1] + r.f[2] * alpha;
+            const T ri = r.bv - r.f[0] * alpha;
+            F.r[r.p] = ri;
+            acc = dd_add(acc, ri * ri);
+            s = ri;
+        }
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             const unsigned d = (unsigned)(r.dw[q / 4] >> (16 * (q % 4))) & 0xFFFFu;
@@ -604,12 +627,13 @@ __global__ void __launch_bounds__(1024)
             if (d != kNoSlot) s -= aq * ring[d];
         }
         const T res = SCALE ? r.sc * s : s;
+        if constexpr (FUSE == kFuseRZ) acc = dd_add(acc, r.f[0] * res);   // r.z, as k_cg_rz_pos
         ring[(l % kBlockRing) * W + ct] = res;
         if (!SPIN || !withhold)
             __hip_atomic_store(reinterpret_cast<U*>(xpos) + r.p, Bits<T>::bits(res), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     };
-    BlockRow<T, K> r0, r1, r2, r3;
+    BlockRow<T, K, FUSE> r0, r1, r2, r3;
     prefetch(Lb, r0);
     prefetch(Lb + 1, r1);
     prefetch(Lb + 2, r2);
@@ -630,6 +654,23 @@ __global__ void __launch_bounds__(1024)
         FVG_BLOCK_STEP(r3, 3)
     }
 #undef FVG_BLOCK_STEP
+    }   // compute waves
+    if constexpr (FUSE != kFuseNone) {
+        // the block's partial of the fused dot product (the loader wave adds zero)
+        __shared__ double s_red[2 * 16];
+        const DD v = wave_sum(acc);
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        if (lane == 0) {
+            s_red[2 * w] = v.hi;
+            s_red[2 * w + 1] = v.lo;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            DD t{0.0, 0.0};
+            for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t = dd_add(t, DD{s_red[2 * q], s_red[2 * q + 1]});
+            put_dd(F.partials, 0, 0, blk, t);
+        }
+    }
 }
 
 // ---- FP32 inner preconditioner: Jacobi-iterated triangular solves ----------------------
@@ -1508,13 +1549,26 @@ static int launch_sweep(fvg_ctx* c, const Sweep& sw, const T* val, const T* scal
     return FVG_OK;
 }
 
+template <bool SCALE, int FUSE>
+static const void* blocked_kernel_f(int K, bool spin) {
+    if constexpr (FUSE != kFuseNone) {   // fused CG work with 4 slots per row only (8 would spill)
+        if (K > 4) return nullptr;
+        return spin ? (const void*)k_sweep_blocked<double, SCALE, 4, true, FUSE>
+                    : (const void*)k_sweep_blocked<double, SCALE, 4, false, FUSE>;
+    } else {
+        if (spin)
+            return K <= 4 ? (const void*)k_sweep_blocked<double, SCALE, 4, true, FUSE>
+                          : (const void*)k_sweep_blocked<double, SCALE, 8, true, FUSE>;
+        return K <= 4 ? (const void*)k_sweep_blocked<double, SCALE, 4, false, FUSE>
+                      : (const void*)k_sweep_blocked<double, SCALE, 8, false, FUSE>;
+    }
+}
+
 template <bool SCALE>
-static const void* blocked_kernel(int K, bool spin) {
-    if (spin)
-        return K <= 4 ? (const void*)k_sweep_blocked<double, SCALE, 4, true>
-                      : (const void*)k_sweep_blocked<double, SCALE, 8, true>;
-    return K <= 4 ? (const void*)k_sweep_blocked<double, SCALE, 4, false>
-                  : (const void*)k_sweep_blocked<double, SCALE, 8, false>;
+static const void* blocked_kernel(int K, bool spin, int fuse = kFuseNone) {
+    if (fuse == kFuseXR) return blocked_kernel_f<SCALE, kFuseXR>(K, spin);
+    if (fuse == kFuseRZ) return blocked_kernel_f<SCALE, kFuseRZ>(K, spin);
+    return blocked_kernel_f<SCALE, kFuseNone>(K, spin);
 }
 
 static size_t blocked_lds(const Sweep& sw) {
@@ -1523,15 +1577,18 @@ static size_t blocked_lds(const Sweep& sw) {
 }
 
 // whether every block of the blocked layout can be resident at once (decided on first use)
-static bool blocked_fits(fvg_ctx* c, const Sweep& sw) {
-    if (sw.nblk == 0) return false;
-    if (sw.bOK >= 0) return sw.bOK == 1;
+// the kernels of one family (fuse: kFuseNone, or both fused CG variants), scaled and not,
+// with and without the test's spin limit, all co-resident for this layout
+static bool blocked_family_fits(fvg_ctx* c, const Sweep& sw, bool fused) {
     const int threads = ((sw.bW + 63) / 64 + 1) * 64;
     const size_t lds = blocked_lds(sw);
     bool ok = threads <= 1024 && lds <= 160 * 1024;
-    for (int v = 0; ok && v < 4; ++v) {
-        const bool scaled = v & 1, spin = v >> 1;
-        const void* fn = scaled ? blocked_kernel<true>(sw.bK, spin) : blocked_kernel<false>(sw.bK, spin);
+    const int nf = fused ? 2 : 1;
+    for (int v = 0; ok && v < 4 * nf; ++v) {
+        const bool scaled = v & 1, spin = (v >> 1) & 1;
+        const int fuse = fused ? (v >> 2 ? kFuseRZ : kFuseXR) : kFuseNone;
+        const void* fn =
+            scaled ? blocked_kernel<true>(sw.bK, spin, fuse) : blocked_kernel<false>(sw.bK, spin, fuse);
         if (lds > 64 * 1024 &&
             hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
             ok = false;
@@ -1540,8 +1597,19 @@ static bool blocked_fits(fvg_ctx* c, const Sweep& sw) {
                    (int64_t)per_cu * c->num_cus < sw.nblk))
             ok = false;
     }
-    sw.bOK = ok ? 1 : 0;
     return ok;
+}
+
+static bool blocked_fits(fvg_ctx* c, const Sweep& sw) {
+    if (sw.nblk == 0) return false;
+    if (sw.bOK < 0) sw.bOK = blocked_family_fits(c, sw, false) ? 1 : 0;
+    return sw.bOK == 1;
+}
+
+static bool blocked_fused_fits(fvg_ctx* c, const Sweep& sw) {
+    if (!blocked_fits(c, sw) || sw.bK > 4) return false;
+    if (sw.bOKfuse < 0) sw.bOKfuse = blocked_family_fits(c, sw, true) ? 1 : 0;
+    return sw.bOKfuse == 1;
 }
 
 // One blocked triangular solve.  The right-hand side is b[brow[p]] (natural order), or,
@@ -1549,7 +1617,8 @@ static bool blocked_fits(fvg_ctx* c, const Sweep& sw) {
 // leaves the values in sw.xpos only (a paired upper sweep reads them there).
 static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, const double* b, double* x, int32_t* err,
                           const int32_t* skip, const int32_t* from = nullptr, bool scatter = true,
-                          bool prepare = true, const double* bsrc = nullptr, int brev = 0) {
+                          bool prepare = true, const double* bsrc = nullptr, int brev = 0, int fuse = kFuseNone,
+                          const SweepFuse* fz = nullptr) {
     // the timed span holds one whole triangular solve: set-up, sweep and scatter kernels
     // (without `prepare`, the caller has written the pending fill, and the right-hand side
     // in position order is sw.bpos, or bsrc, read in reverse with brev)
@@ -1561,11 +1630,12 @@ static int launch_blocked(fvg_ctx* c, const Sweep& sw, const double* scale, cons
     int64_t n = sw.n;
     const double* bp = bsrc ? bsrc : sw.bpos;
     const bool custom = c->tune_spin_limit > 0 || withhold;
-    const void* fn = scale ? blocked_kernel<true>(sw.bK, custom) : blocked_kernel<false>(sw.bK, custom);
+    const void* fn = scale ? blocked_kernel<true>(sw.bK, custom, fuse) : blocked_kernel<false>(sw.bK, custom, fuse);
+    SweepFuse F = fz ? *fz : SweepFuse{};
     void* args[] = {&nlev, &n, &W, &X, (void*)&sw.brange, (void*)&sw.blvl, (void*)&sw.bxptr, (void*)&sw.bxidx,
                     (void*)&sw.bdep, (void*)&sw.bval, (void*)&bp,
                     (void*)&sw.spos, (void*)&sw.xpos, (void*)&err, (void*)&skip, (void*)&spin, (void*)&withhold,
-                    (void*)&brev};
+                    (void*)&brev, (void*)&F};
     const int threads = ((sw.bW + 63) / 64 + 1) * 64;
     FVG_HIP(launch_coresident(c, fn, (unsigned)sw.nblk, (unsigned)threads, args, blocked_lds(sw)));
     if (scatter)
@@ -1850,6 +1920,22 @@ int apply_pos(fvg_ctx* c, const fvg_prec* p, const double* r_pos, const int32_t*
     FVG_TRY(launch_blocked(c, lo, lo_scale, nullptr, nullptr, p->d_flags + 3, skip, nullptr, false, false, r_pos, 0));
     return launch_blocked(c, up, p->d_inv_diag, nullptr, nullptr, p->d_flags + 3, skip, nullptr, false, false, lo.xpos,
                           1);
+}
+
+int apply_pos_fused(fvg_ctx* c, const fvg_prec* p, double* r_pos, const int32_t* skip, const SweepFuse& lo_f,
+                    const SweepFuse& up_f, const std::function<int()>& between) {
+    const Sweep& lo = p->lower;
+    const Sweep& up = p->upper;
+    const double* lo_scale = p->kind == kIC ? p->d_inv_diag : nullptr;
+    FVG_TRY(launch_blocked(c, lo, lo_scale, nullptr, nullptr, p->d_flags + 3, skip, nullptr, false, false, r_pos, 0,
+                           kFuseXR, &lo_f));
+    FVG_TRY(between());
+    return launch_blocked(c, up, p->d_inv_diag, nullptr, nullptr, p->d_flags + 3, skip, nullptr, false, false, lo.xpos,
+                          1, kFuseRZ, &up_f);
+}
+
+bool pos_fusable(fvg_ctx* c, const fvg_prec* p) {
+    return blocked_fused_fits(c, p->lower) && blocked_fused_fits(c, p->upper);
 }
 
 int apply_prec(fvg_ctx* c, const fvg_prec* p, const double* r, double* z, const int32_t* skip) {

@@ -1,6 +1,7 @@
 // precond.hpp -- preconditioner objects (internal).
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "internal.hpp"
@@ -45,6 +46,7 @@ struct Sweep {
     int32_t bK = 0;                     // most dependencies of one row
     int32_t bF = 0;                     // most fetched rows of one block (all levels)
     mutable int32_t bOK = -1;           // blocked launch fits co-resident: -1 unknown, 0 no, 1 yes
+    mutable int32_t bOKfuse = -1;       // the same for the kernels with fused CG work
                                         // (decided on first use, cached)
     int32_t* brange = nullptr;          // per block: first level, one past the last (2 x nblk)
     int32_t* blvl = nullptr;            // nblk x (nlevels + 1): first position of each level
@@ -153,6 +155,20 @@ void team_plan_release(fvg_ctx* ctx, fvg_prec* p);
 // one's values in reverse, and the result sits reversed in up_x (z at lower position q is
 // up_x[n - 1 - q]).  Before each apply the caller writes the pending pattern where polled:
 // lo_x[q] where lo_poll[q], up_x[q] where up_poll[q].
+// Work of the CG iteration fused into the two sweeps of a position-order apply: the lower
+// sweep forms its right-hand side r - ap*alpha itself (writing r, x += p*alpha and the
+// partials of r.r), the upper sweep leaves the partials of r.z (conjugate_gradient.rs:
+// 77-94).  One double-double partial per block of the sweep.
+enum { kFuseNone = 0, kFuseXR = 1, kFuseRZ = 2 };
+struct SweepFuse {
+    const double* alpha = nullptr;   // kFuseXR: the step length in the solver state
+    const double* ap = nullptr;      // kFuseXR: A p, by position
+    double* x = nullptr;             // kFuseXR
+    const double* pp = nullptr;      // kFuseXR: p
+    double* r = nullptr;             // kFuseXR: read and written; kFuseRZ: read
+    double* partials = nullptr;
+};
+
 struct PosSolve {
     int64_t n = 0;
     const int32_t* row = nullptr;       // row of each lower position
@@ -168,6 +184,12 @@ struct PosSolve {
 int pos_solve(fvg_ctx* ctx, const fvg_prec* p, const fvg_csr* a, bool allow_alloc, PosSolve* out, bool* ok);
 // z = M^-1 r, r in position order: both sweeps (no host synchronisation)
 int apply_pos(fvg_ctx* ctx, const fvg_prec* p, const double* r_pos, const int32_t* skip);
+// apply_pos with the CG iteration's vector work fused into the sweeps; `between` enqueues
+// what runs between the lower and the upper sweep (the convergence test)
+int apply_pos_fused(fvg_ctx* ctx, const fvg_prec* p, double* r_pos, const int32_t* skip, const SweepFuse& lo_f,
+                    const SweepFuse& up_f, const std::function<int()>& between);
+// whether the fused sweep kernels can run this preconditioner's layouts co-resident
+bool pos_fusable(fvg_ctx* ctx, const fvg_prec* p);
 // pending value bits of an FP64 sweep output
 constexpr uint64_t kPendingBits = 0x7FF4DEADBEEF0F64ull;
 }  // namespace fvg

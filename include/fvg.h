@@ -108,7 +108,11 @@ enum {
      * off-diagonal slot belongs to one face (a workgroup's rows built in LDS and written
      * out whole), else one lane per row; 1 always one lane per row; 2 the entry kernel
      * storing each value straight to its slot.  The same values every way. */
-    FVG_TUNE_ASSEMBLE = 12
+    FVG_TUNE_ASSEMBLE = 12,
+    /* FP64 CG in position order: 0 (default) its residual update and r.r are done by the
+     * lower sweep and r.z by the upper sweep of each preconditioner apply; 1 separate
+     * kernels.  The same values either way. */
+    FVG_TUNE_CG_FUSION = 13
 };
 int fvg_ctx_set_tuning(fvg_ctx* ctx, int32_t knob, int64_t value);
 int fvg_ctx_get_tuning(const fvg_ctx* ctx, int32_t knob, int64_t* value);

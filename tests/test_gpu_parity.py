@@ -21,6 +21,7 @@ MESHES = {
     "tri64": lambda: meshgen.tri2d(64, seed=1),
     "tet4": lambda: meshgen.tet3d(4, seed=5),
     "tet10": lambda: meshgen.tet3d(10, seed=7),
+    "tet16": lambda: meshgen.tet3d(16, seed=9),
     "prism8": lambda: meshgen.prism3d(8, layers=4, seed=2),
     "prism24": lambda: meshgen.prism3d(24, layers=12, seed=2),
 }
@@ -429,3 +430,37 @@ def test_position_order_solve_matches_natural_order(ctx, name, kind, solver, pre
     assert np.linalg.norm(xp - xn) <= 1e-12 * np.linalg.norm(xn)
     assert spos["sweep_launches"] == spos["sweep_calls"]          # one kernel per triangular solve
     assert snat["sweep_launches"] > snat["sweep_calls"]
+
+
+@pytest.mark.parametrize("name", ["tet10", "tet16"])
+def test_fused_cg_iteration_matches_separate_kernels(ctx, name):
+    """In position order, FP64 CG does its residual update and r.r inside the lower sweep
+    and r.z inside the upper sweep (FVG_TUNE_CG_FUSION 0, the default) instead of in
+    separate kernels (1).  The arithmetic per element is the same and the dot products are
+    double-double sums, so x, the iteration count and the residual history are the same
+    bits, and both meet the oracle's bars."""
+    rp, ci, va, b = _oracle_system(name, "sym")
+    n = rp.shape[0] - 1
+    A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+    M = fvg.IncompleteCholesky.from_matrix(A, 0)
+    lay = M.sweep_layout()
+    assert lay["mirrored"] == 1
+    B = fvg.DistributedVector.from_data(ctx, b)
+    opts = fvg.LinearSolverOptions(1e-10, 0.0, 2000)
+    out = {}
+    try:
+        for mode in (1, 0):
+            ctx.set_tuning(fvg.TUNE_CG_FUSION, mode)
+            x = fvg.DistributedVector.from_data(ctx, np.full(n, 0.25))
+            info = fvg.conjugate_gradient(x, A, B, M, opts)
+            out[mode] = (x.numpy(), info)
+    finally:
+        ctx.set_tuning(fvg.TUNE_CG_FUSION, 0)
+    (xs, isep), (xf, ifus) = out[1], out[0]
+    assert ifus.iterations == isep.iterations
+    assert np.array_equal(ifus.history, isep.history)
+    assert np.array_equal(xf, xs)
+    P = O.OraclePrec("ic", rp, ci, va, 0)
+    xo, io = O.solve("cg", rp, ci, va, P, b, x0=np.full(n, 0.25), rtol=1e-10, atol=0.0, max_iter=2000)
+    assert abs(ifus.iterations - io["iterations"]) <= 2
+    assert np.linalg.norm(xf - xo) <= 1e-6 * np.linalg.norm(xo)
