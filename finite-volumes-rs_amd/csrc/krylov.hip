@@ -468,10 +468,6 @@ int KRun::finish(fvg_solver_info* infos, double* const* histories, int64_t cap) 
 #define SCALAR(step, part0)                                                                                \
     hipLaunchKernelGGL(k_scalar<step>, dim3(L.n_inst), dim3(kScalarBlock), 0, c->stream, K.st, part0, stride, \
                        nb, L.blk_ptr, K.hist, K.hist_cap, K.flags, K.flags + 1)
-// the same over `count` partials (one per block of a fused sweep)
-#define SCALARN(step, part0, count)                                                                        \
-    hipLaunchKernelGGL(k_scalar<step>, dim3(1), dim3(kScalarBlock), 0, c->stream, K.st, part0, stride,      \
-                       (int)(count), nullptr, K.hist, K.hist_cap, K.flags, K.flags + 1)
 
 // iterations enqueued between two polls of the "all systems done" flag
 constexpr int kPollBatch = 4;
@@ -550,9 +546,17 @@ static int run_cg_pos(fvg_ctx* c, const fvg_prec* p, const double* b, double* x,
     lf.x = xp;
     lf.pp = pp;
     lf.r = r;
-    lf.partials = part;
     uf.r = r;
-    uf.partials = part;
+    for (SweepFuse* f : {&lf, &uf}) {   // the sweeps' last blocks run the scalar steps
+        f->partials = part;
+        f->counter = K.flags + 2;
+        f->st = K.st;
+        f->hist = K.hist;
+        f->active = K.flags;
+        f->all_done = K.flags + 1;
+    }
+    lf.step = CG_ERROR;
+    uf.step = CG_BETA;
     for (int64_t it = 0;; ++it) {
         SpmvEpilogue e1;   // ap = A p with p.ap
         e1.dotw = pp;
@@ -563,11 +567,7 @@ static int run_cg_pos(fvg_ctx* c, const fvg_prec* p, const double* b, double* x,
         FVG_TRY(launch_spmv(c, P.pa, pp, ap, e1));
         SCALAR(CG_ALPHA, part);
         if (fuse) {
-            FVG_TRY(apply_pos_fused(c, p, r, all_done, lf, uf, [&]() {
-                SCALARN(CG_ERROR, part, lo_nb);
-                return FVG_OK;
-            }));
-            SCALARN(CG_BETA, part, up_nb);
+            FVG_TRY(apply_pos_fused(c, p, r, all_done, lf, uf));
         } else {
             hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, nullptr, pp, ap, xp, r, part);
             SCALAR(CG_ERROR, part);

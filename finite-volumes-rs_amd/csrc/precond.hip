@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "kstate.hpp"
 #include "precond.hpp"
 #include "reduce.hpp"
 
@@ -490,6 +491,36 @@ struct BlockRow {
 // limit: the forced time-out of the tests.  The default launch keeps the constant limit
 // and always publishes (a runtime limit in the polling loop measured 12 us slower per
 // sweep).
+// One block's partial of a fused sweep (t, valid in thread 0; called by every thread of
+// the block).  With a counter, the partial is published with agent-scope stores and the
+// block counts itself in; the last block to arrive sums all partials (one per lane of its
+// first wave, agent-scope loads: coherent across XCDs without a cache flush) and runs
+// the solver's scalar step.
+__device__ inline void sweep_fuse_finish(const SweepFuse& F, int blk, DD t, int* s_last) {
+    if (!F.counter) {
+        if (threadIdx.x == 0) put_dd(F.partials, 0, 0, blk, t);
+        return;
+    }
+    const int nb = (int)gridDim.x;
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(F.partials + 2 * blk, t.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(F.partials + 2 * blk + 1, t.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);   // both stores have completed before the count
+        *s_last = __hip_atomic_fetch_add(F.counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nb - 1;
+    }
+    __syncthreads();
+    if (!*s_last || threadIdx.x >= 64) return;
+    DD a{0.0, 0.0};
+    for (int q = (int)threadIdx.x; q < nb; q += 64)
+        a = dd_add(a, DD{__hip_atomic_load(F.partials + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                         __hip_atomic_load(F.partials + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)});
+    a = wave_sum(a);
+    if (threadIdx.x == 0) {
+        scalar_step(F.step, F.st, F.hist, a.hi, 0.0, F.active, F.all_done);
+        __hip_atomic_store(F.counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 template <class T, bool SCALE, int K, bool SPIN, int FUSE>
 __global__ void __launch_bounds__(1024)
     k_sweep_blocked(int nlevels, int64_t n, int W, int X, const int32_t* __restrict__ brange,
@@ -504,7 +535,10 @@ __global__ void __launch_bounds__(1024)
     const int blk = blockIdx.x;
     const int Lb = brange[2 * blk], Le = brange[2 * blk + 1];
     if (Lb >= Le) {
-        if (FUSE != kFuseNone && threadIdx.x == 0) put_dd(F.partials, 0, 0, blk, DD{0.0, 0.0});
+        if constexpr (FUSE != kFuseNone) {
+            __shared__ int s_last0;
+            sweep_fuse_finish(F, blk, DD{0.0, 0.0}, &s_last0);
+        }
         return;
     }
     T* ring = reinterpret_cast<T*>(smem);
@@ -658,6 +692,7 @@ __global__ void __launch_bounds__(1024)
     if constexpr (FUSE != kFuseNone) {
         // the block's partial of the fused dot product (the loader wave adds zero)
         __shared__ double s_red[2 * 16];
+        __shared__ int s_last;
         const DD v = wave_sum(acc);
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
         if (lane == 0) {
@@ -665,11 +700,10 @@ __global__ void __launch_bounds__(1024)
             s_red[2 * w + 1] = v.lo;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            DD t{0.0, 0.0};
+        DD t{0.0, 0.0};
+        if (threadIdx.x == 0)
             for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t = dd_add(t, DD{s_red[2 * q], s_red[2 * q + 1]});
-            put_dd(F.partials, 0, 0, blk, t);
-        }
+        sweep_fuse_finish(F, blk, t, &s_last);
     }
 }
 
@@ -1923,13 +1957,12 @@ int apply_pos(fvg_ctx* c, const fvg_prec* p, const double* r_pos, const int32_t*
 }
 
 int apply_pos_fused(fvg_ctx* c, const fvg_prec* p, double* r_pos, const int32_t* skip, const SweepFuse& lo_f,
-                    const SweepFuse& up_f, const std::function<int()>& between) {
+                    const SweepFuse& up_f) {
     const Sweep& lo = p->lower;
     const Sweep& up = p->upper;
     const double* lo_scale = p->kind == kIC ? p->d_inv_diag : nullptr;
     FVG_TRY(launch_blocked(c, lo, lo_scale, nullptr, nullptr, p->d_flags + 3, skip, nullptr, false, false, r_pos, 0,
                            kFuseXR, &lo_f));
-    FVG_TRY(between());
     return launch_blocked(c, up, p->d_inv_diag, nullptr, nullptr, p->d_flags + 3, skip, nullptr, false, false, lo.xpos,
                           1, kFuseRZ, &up_f);
 }

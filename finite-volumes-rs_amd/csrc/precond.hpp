@@ -1,7 +1,6 @@
 // precond.hpp -- preconditioner objects (internal).
 #pragma once
 
-#include <functional>
 #include <vector>
 
 #include "internal.hpp"
@@ -160,6 +159,7 @@ void team_plan_release(fvg_ctx* ctx, fvg_prec* p);
 // partials of r.r), the upper sweep leaves the partials of r.z (conjugate_gradient.rs:
 // 77-94).  One double-double partial per block of the sweep.
 enum { kFuseNone = 0, kFuseXR = 1, kFuseRZ = 2 };
+struct KState;
 struct SweepFuse {
     const double* alpha = nullptr;   // kFuseXR: the step length in the solver state
     const double* ap = nullptr;      // kFuseXR: A p, by position
@@ -167,6 +167,14 @@ struct SweepFuse {
     const double* pp = nullptr;      // kFuseXR: p
     double* r = nullptr;             // kFuseXR: read and written; kFuseRZ: read
     double* partials = nullptr;
+    // with `counter`, the last block to finish sums the partials and runs the solver's
+    // scalar step `step` (the convergence test after the lower sweep, beta after the upper)
+    int32_t* counter = nullptr;      // zero before the sweep; left zero after it
+    KState* st = nullptr;
+    double* hist = nullptr;
+    int32_t* active = nullptr;
+    int32_t* all_done = nullptr;
+    int32_t step = 0;
 };
 
 struct PosSolve {
@@ -184,10 +192,9 @@ struct PosSolve {
 int pos_solve(fvg_ctx* ctx, const fvg_prec* p, const fvg_csr* a, bool allow_alloc, PosSolve* out, bool* ok);
 // z = M^-1 r, r in position order: both sweeps (no host synchronisation)
 int apply_pos(fvg_ctx* ctx, const fvg_prec* p, const double* r_pos, const int32_t* skip);
-// apply_pos with the CG iteration's vector work fused into the sweeps; `between` enqueues
-// what runs between the lower and the upper sweep (the convergence test)
+// apply_pos with the CG iteration's vector work and scalar steps fused into the sweeps
 int apply_pos_fused(fvg_ctx* ctx, const fvg_prec* p, double* r_pos, const int32_t* skip, const SweepFuse& lo_f,
-                    const SweepFuse& up_f, const std::function<int()>& between);
+                    const SweepFuse& up_f);
 // whether the fused sweep kernels can run this preconditioner's layouts co-resident
 bool pos_fusable(fvg_ctx* ctx, const fvg_prec* p);
 // pending value bits of an FP64 sweep output
