@@ -352,6 +352,10 @@ int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
         if (value > 1) return fail(FVG_E_INVALID, "CG fusion must be 0 or 1");
         c->tune_cg_fusion = value;
         return FVG_OK;
+    case FVG_TUNE_SWEEP_GAP:
+        if (value == 1 || value > 8) return fail(FVG_E_INVALID, "sweep gap must be 0 or 2 to 8");
+        c->tune_sweep_gap = value;
+        return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }
@@ -372,6 +376,7 @@ int fvg_ctx_get_tuning(const fvg_ctx* c, int32_t knob, int64_t* value) {
     case FVG_TUNE_SOLVE_ORDER: *value = c->tune_solve_order; return FVG_OK;
     case FVG_TUNE_ASSEMBLE: *value = c->tune_assemble; return FVG_OK;
     case FVG_TUNE_CG_FUSION: *value = c->tune_cg_fusion; return FVG_OK;
+    case FVG_TUNE_SWEEP_GAP: *value = c->tune_sweep_gap; return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }

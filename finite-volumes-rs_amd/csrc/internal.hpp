@@ -77,6 +77,7 @@ struct fvg_ctx {
     int64_t tune_solve_order = 0;      // FVG_TUNE_SOLVE_ORDER
     int64_t tune_assemble = 0;         // FVG_TUNE_ASSEMBLE
     int64_t tune_cg_fusion = 0;        // FVG_TUNE_CG_FUSION
+    int64_t tune_sweep_gap = 0;        // FVG_TUNE_SWEEP_GAP
     int batches_in_flight = 0;         // batches of this context launched and not yet finished
     bool sweep_timed_out = false;      // a sweep wait gave up here: co-resident launches stay cooperative
     int last_sweep_launch = 0;         // 1 ordinary, 2 cooperative (0: none yet)

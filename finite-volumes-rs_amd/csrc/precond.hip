@@ -17,6 +17,7 @@
 // incomplete_lower_upper.rs:291-328).
 #include <algorithm>
 #include <cmath>
+#include <numeric>
 #include <cstdlib>
 #include <cstring>
 
@@ -424,11 +425,12 @@ __global__ void __launch_bounds__(kChunk * kWavesPerGroup)
 // One workgroup per block of the blocked layout, all co-resident (cooperative launch).
 // A block walks its levels in order with one barrier per level.  The layout guarantees
 // that a dependency between blocks spans at least two levels (blocks of whole gap-1
-// components, or delayed levels), so a row's dependencies are either values its own
-// block made one or two levels before (kept in an LDS ring of kBlockRing levels) or
-// values from other blocks (or older ones of its own) at least two levels before.  The
-// last wave of the workgroup is the loader: during level l it polls, with device-scope
-// loads, the outside values level l + 1 will read and puts them in LDS.  The other
+// components, or delayed levels; by default three), so a row's dependencies are either
+// values its own block made one or two levels before (kept in an LDS ring of kBlockRing
+// levels) or values from other blocks (or older ones of its own) at least two levels
+// before.  The last wave of the workgroup is the loader: during level l it awaits, with
+// device-scope loads, the outside values level l + 1 will read, whose polls it issued
+// during level l - 1, puts them in LDS and issues the polls of level l + 2.  The other
 // waves only read LDS for their operands; their own loads are the static data of each
 // row (packed LDS slots, coefficients, right-hand side, scale: all in position order),
 // fetched kBlockAhead levels before use, so no load of theirs waits on a poll.  Solved
@@ -544,16 +546,9 @@ __global__ void __launch_bounds__(1024)
     T* ring = reinterpret_cast<T*>(smem);
     T* stage = ring + kBlockRing * W;
     int* s_lv = reinterpret_cast<int*>(stage + 2 * X);
-    int* s_xp = s_lv + nlevels + 1;
-    int* s_xi = s_xp + nlevels + 1;
     const int32_t* lv = blvl + (int64_t)blk * (nlevels + 1);
     const int32_t* xp = bxptr + (int64_t)blk * (nlevels + 1);
-    const int x0 = xp[0], nx = xp[nlevels] - x0;
-    for (int t = threadIdx.x; t <= nlevels; t += blockDim.x) {
-        s_lv[t] = lv[t];
-        s_xp[t] = xp[t] - x0;
-    }
-    for (int t = threadIdx.x; t < nx; t += blockDim.x) s_xi[t] = bxidx[x0 + t];
+    for (int t = threadIdx.x; t <= nlevels; t += blockDim.x) s_lv[t] = lv[t];
     __syncthreads();
     // The loader is the last wave.  The test is made wave-uniform so that the loader and
     // the compute waves run separate loops (each with one barrier per level), and the
@@ -568,44 +563,113 @@ __global__ void __launch_bounds__(1024)
     DD acc{0.0, 0.0};   // fused dot product of this thread's rows
 
     if (loader) {
-        // The outside values of level l into staging slot l & 1: each lane issues all of its
-        // polls (kFetchBatch at a time) before it waits, then re-polls only the pending ones.
-        auto fetch = [&](int l) {
-            const int k0 = s_xp[l], k1 = s_xp[l + 1];
-            T* st = stage + (l & 1) * X - k0;
-            for (int kb = k0; kb < k1; kb += 64 * kFetchBatch) {
-                int jj[kFetchBatch];
-                U w[kFetchBatch];
+        // The outside values of level l go into staging slot l & 1, by a pipeline of loads
+        // that keeps the loader's only LDS accesses to those staging writes.  At the start of
+        // step s (after the barrier that ends level s - 1) the loader issues the polls of
+        // level s + 2, from the positions it loaded during step s - 1, loads the positions
+        // polled for level s + 3 (from the offsets it loaded during step s - 1) and the
+        // offsets of level s + 4; then it awaits the polls of level s + 1, issued a whole
+        // step before, re-polling only the pending ones, and writes them to LDS.  Where the
+        // layout puts three or more levels between blocks, each value was published before
+        // its poll was issued.  A level's polls past the first 64 * kFetchBatch are issued
+        // and awaited when the level is awaited.
+        int pj1[kFetchBatch], ij2[kFetchBatch];   // positions polled for levels s + 1, s + 2
+        U pw1[kFetchBatch];                         // the polls of level s + 1 in flight
+        int a1, b1, a2, b2, a3, b3;                 // bxidx offsets of levels s + 1 .. s + 3
+        auto offsets = [&](int l, int& a, int& b) {   // empty past the block's last level
+            const int lc = l < Le ? l : Le - 1;
+            a = xp[lc];
+            b = xp[lc + 1];
+            if (l >= Le) b = a;
+        };
+        // Every load of the pipeline is unconditional (indices clamped into range, unused
+        // results replaced afterwards), so each step issues the same number of them and the
+        // waits before use count only what is older.
+        auto positions = [&](int a, int b, int (&jj)[kFetchBatch]) {
 #pragma unroll
-                for (int q = 0; q < kFetchBatch; ++q) {
-                    const int k = kb + q * 64 + lt;
-                    jj[q] = k < k1 ? s_xi[k] : -1;
-                }
+            for (int q = 0; q < kFetchBatch; ++q) {
+                const int k = a + q * 64 + lt;
+                const int v = bxidx[k < b ? k : (b > 0 ? b - 1 : 0)];
+                jj[q] = k < b ? v : -1;
+            }
+        };
+        auto poll = [&](const int (&jj)[kFetchBatch], U (&w)[kFetchBatch]) {
 #pragma unroll
-                for (int q = 0; q < kFetchBatch; ++q) w[q] = jj[q] >= 0 ? load_word(xpos, jj[q]) : U(0);
-                for (int spins = 0;; ++spins) {
-                    bool pend = false;
+            for (int q = 0; q < kFetchBatch; ++q) w[q] = load_word(xpos, jj[q] >= 0 ? jj[q] : 0);
+        };
+        auto settle = [&](const int (&jj)[kFetchBatch], U (&w)[kFetchBatch]) {
 #pragma unroll
-                    for (int q = 0; q < kFetchBatch; ++q) pend |= w[q] == Bits<T>::pending;
-                    if (!pend) break;
-                    if (spins > (SPIN ? spin_limit : kSpinLimit) ||
-                        ((spins & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        break;
-                    }
+            for (int q = 0; q < kFetchBatch; ++q)
+                if (jj[q] < 0) w[q] = U(0);
+            for (int spins = 0;; ++spins) {
+                bool pend = false;
 #pragma unroll
-                    for (int q = 0; q < kFetchBatch; ++q)
-                        if (w[q] == Bits<T>::pending) w[q] = load_word(xpos, jj[q]);
+                for (int q = 0; q < kFetchBatch; ++q) pend |= w[q] == Bits<T>::pending;
+                if (!pend) break;
+                if (spins > (SPIN ? spin_limit : kSpinLimit) ||
+                    ((spins & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
                 }
 #pragma unroll
                 for (int q = 0; q < kFetchBatch; ++q)
-                    if (jj[q] >= 0) st[kb + q * 64 + lt] = Bits<T>::value(w[q]);
+                    if (w[q] == Bits<T>::pending) w[q] = load_word(xpos, jj[q]);
             }
         };
-        fetch(Lb);
+        // level l (offsets a, b) into its staging slot: the polls in flight, then the rest
+        auto stage_level = [&](int l, int a, int b, const int (&jj)[kFetchBatch], U (&w)[kFetchBatch]) {
+            T* st = stage + (l & 1) * X;
+            settle(jj, w);
+#pragma unroll
+            for (int q = 0; q < kFetchBatch; ++q)
+                if (jj[q] >= 0) st[q * 64 + lt] = Bits<T>::value(w[q]);
+            for (int kb = a + 64 * kFetchBatch; kb < b; kb += 64 * kFetchBatch) {
+                int jr[kFetchBatch];
+                U wr[kFetchBatch];
+                positions(kb, b, jr);
+                poll(jr, wr);
+                settle(jr, wr);
+#pragma unroll
+                for (int q = 0; q < kFetchBatch; ++q)
+                    if (jr[q] >= 0) st[kb - a + q * 64 + lt] = Bits<T>::value(wr[q]);
+            }
+        };
+        {   // level Lb at once; the pipeline primed with levels Lb + 1 .. Lb + 3
+            int a0, b0, j0[kFetchBatch];
+            U w0[kFetchBatch];
+            offsets(Lb, a0, b0);
+            offsets(Lb + 1, a1, b1);
+            offsets(Lb + 2, a2, b2);
+            offsets(Lb + 3, a3, b3);
+            positions(a0, b0, j0);
+            poll(j0, w0);
+            positions(a1, b1, pj1);
+            poll(pj1, pw1);
+            positions(a2, b2, ij2);
+            stage_level(Lb, a0, b0, j0, w0);
+        }
         __syncthreads();
         for (int l = Lb; l < Lend; ++l) {
-            if (l + 1 < Le) fetch(l + 1);
+            int pj2[kFetchBatch], ij3[kFetchBatch], a4, b4;
+            U pw2[kFetchBatch];
+            offsets(l + 4, a4, b4);
+            positions(a3, b3, ij3);            // level l + 3
+#pragma unroll
+            for (int q = 0; q < kFetchBatch; ++q) pj2[q] = ij2[q];
+            poll(pj2, pw2);                    // level l + 2
+            stage_level(l + 1, a1, b1, pj1, pw1);
+#pragma unroll
+            for (int q = 0; q < kFetchBatch; ++q) {
+                pj1[q] = pj2[q];
+                pw1[q] = pw2[q];
+                ij2[q] = ij3[q];
+            }
+            a1 = a2;
+            b1 = b2;
+            a2 = a3;
+            b2 = b3;
+            a3 = a4;
+            b3 = b4;
             __syncthreads();
         }
         if constexpr (FUSE == kFuseNone) return;
@@ -842,6 +906,7 @@ static void free_sweep(fvg_ctx* c, Sweep& s) {
 }
 
 constexpr int kBlockMaxDeps = 8;
+constexpr int32_t kBlockGrowCap = 128;   // rows of one level a grown block may reach
 constexpr int kBlockMaxRows = 960;   // 15 compute waves + the loader wave
 // Rows packed by component into at most `target` blocks (see build_blocked): the block
 // of each component; returns the number of blocks used.  A block's rows of one level are
@@ -941,11 +1006,158 @@ static int32_t pack_components(int64_t n, const std::vector<int32_t>& rp, const 
     return used;
 }
 
+// Blocks of components for a layout that puts three or more levels between blocks (see
+// assign_blocked); returns the number of blocks.  Delaying every dependency between blocks
+// lengthens any dependency chain that crosses between blocks often, so blocks are made
+// compact: first each component is paired with the one it shares the most dependencies
+// with (on extruded-like orderings those two span the same levels side by side, and a
+// chain can step back and forth between them), then the pairs are grown into blocks of
+// about n / target rows.  A block starts from the lowest unassigned pair and adds, among
+// the unassigned pairs that share a dependency with it, the one that raises its fullest
+// level least (the most shared dependencies on a tie), while that stays within `cap`
+// rows.  (On 1,572,864 Kuhn tetrahedra: 88 blocks and 269 levels at a gap of three,
+// against 319 for the blocks of pack_components.)
+static int32_t grow_blocks(int64_t n, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, bool upper,
+                           const std::vector<int32_t>& comp_of, size_t C, const std::vector<int32_t>& lev, int target,
+                           int32_t cap, std::vector<int32_t>& blk_of_comp) {
+    auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
+    // dependencies between components, as (weight, a, b) with a < b
+    std::vector<uint64_t> keys;
+    for (int64_t i = 0; i < n; ++i)
+        for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
+            const int32_t j = col[(size_t)k];
+            if (!take(i, j)) continue;
+            uint32_t a = (uint32_t)comp_of[(size_t)i], b = (uint32_t)comp_of[(size_t)j];
+            if (a == b) continue;
+            if (a > b) std::swap(a, b);
+            keys.push_back((uint64_t)a << 32 | b);
+        }
+    std::sort(keys.begin(), keys.end());
+    struct Link {
+        int32_t w, a, b;
+    };
+    std::vector<Link> links;
+    for (size_t k = 0; k < keys.size();) {
+        size_t e = k;
+        while (e < keys.size() && keys[e] == keys[k]) ++e;
+        links.push_back({(int32_t)(e - k), (int32_t)(keys[k] >> 32), (int32_t)(keys[k] & 0xFFFFFFFFu)});
+        k = e;
+    }
+    keys.clear();
+    keys.shrink_to_fit();
+    // pairs along the heaviest links (ties: lower components first)
+    std::vector<Link> order(links);
+    std::stable_sort(order.begin(), order.end(), [](const Link& x, const Link& y) { return x.w > y.w; });
+    std::vector<int32_t> grp(C, -1);
+    int32_t G = 0;
+    for (const Link& l : order)
+        if (grp[(size_t)l.a] < 0 && grp[(size_t)l.b] < 0) grp[(size_t)l.a] = grp[(size_t)l.b] = G++;
+    for (size_t q = 0; q < C; ++q)
+        if (grp[q] < 0) grp[q] = G++;
+    // pairs in order of their lowest component
+    {
+        std::vector<int32_t> first((size_t)G, INT32_MAX);
+        for (size_t q = 0; q < C; ++q) first[(size_t)grp[q]] = std::min(first[(size_t)grp[q]], (int32_t)q);
+        std::vector<int32_t> rank((size_t)G);
+        std::iota(rank.begin(), rank.end(), 0);
+        std::sort(rank.begin(), rank.end(), [&](int32_t x, int32_t y) { return first[(size_t)x] < first[(size_t)y]; });
+        std::vector<int32_t> renum((size_t)G);
+        for (int32_t r = 0; r < G; ++r) renum[(size_t)rank[(size_t)r]] = r;
+        for (size_t q = 0; q < C; ++q) grp[q] = renum[(size_t)grp[q]];
+    }
+    // rows of each pair per level (sorted (level, count) runs) and the pairs' links
+    std::vector<int64_t> gl;
+    gl.reserve((size_t)n);
+    int nlev = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        gl.push_back((int64_t)grp[(size_t)comp_of[(size_t)i]] << 32 | (uint32_t)lev[(size_t)i]);
+        nlev = std::max(nlev, lev[(size_t)i] + 1);
+    }
+    std::sort(gl.begin(), gl.end());
+    std::vector<int64_t> gp((size_t)G + 1, 0), gsize((size_t)G, 0);
+    std::vector<int32_t> glev, gcnt;
+    for (size_t k = 0; k < gl.size();) {
+        size_t e = k;
+        while (e < gl.size() && gl[e] == gl[k]) ++e;
+        const int32_t g = (int32_t)(gl[k] >> 32);
+        glev.push_back((int32_t)(gl[k] & 0xFFFFFFFF));
+        gcnt.push_back((int32_t)(e - k));
+        gp[(size_t)g + 1]++;
+        gsize[(size_t)g] += (int64_t)(e - k);
+        k = e;
+    }
+    for (int32_t g = 0; g < G; ++g) gp[(size_t)g + 1] += gp[(size_t)g];
+    std::vector<int64_t> ap((size_t)G + 1, 0);
+    for (const Link& l : links)
+        if (grp[(size_t)l.a] != grp[(size_t)l.b]) {
+            ap[(size_t)grp[(size_t)l.a] + 1]++;
+            ap[(size_t)grp[(size_t)l.b] + 1]++;
+        }
+    for (int32_t g = 0; g < G; ++g) ap[(size_t)g + 1] += ap[(size_t)g];
+    std::vector<std::pair<int32_t, int32_t>> adj((size_t)ap[(size_t)G]);   // (pair, weight)
+    {
+        std::vector<int64_t> nx(ap.begin(), ap.end() - 1);
+        for (const Link& l : links) {
+            const int32_t x = grp[(size_t)l.a], y = grp[(size_t)l.b];
+            if (x == y) continue;
+            adj[(size_t)nx[(size_t)x]++] = {y, l.w};
+            adj[(size_t)nx[(size_t)y]++] = {x, l.w};
+        }
+    }
+    // growth
+    const int64_t goal = (n + target - 1) / target;
+    std::vector<int32_t> blk_of_grp((size_t)G, -1), load((size_t)nlev, 0), touched;
+    std::vector<int32_t> front;
+    int32_t nb = 0;
+    for (int32_t seed = 0; seed < G; ++seed) {
+        if (blk_of_grp[(size_t)seed] >= 0) continue;
+        const int32_t b = nb++;
+        int64_t rows = 0;
+        for (int32_t l : touched) load[(size_t)l] = 0;
+        touched.clear();
+        front.clear();
+        auto add = [&](int32_t g) {
+            blk_of_grp[(size_t)g] = b;
+            for (int64_t k = gp[(size_t)g]; k < gp[(size_t)g + 1]; ++k) {
+                if (load[(size_t)glev[(size_t)k]] == 0) touched.push_back(glev[(size_t)k]);
+                load[(size_t)glev[(size_t)k]] += gcnt[(size_t)k];
+            }
+            rows += gsize[(size_t)g];
+            for (int64_t k = ap[(size_t)g]; k < ap[(size_t)g + 1]; ++k)
+                if (blk_of_grp[(size_t)adj[(size_t)k].first] < 0) front.push_back(adj[(size_t)k].first);
+        };
+        add(seed);
+        while (rows < goal) {
+            int32_t best = -1, best_peak = INT32_MAX, best_w = 0;
+            for (int32_t g : front) {
+                if (blk_of_grp[(size_t)g] >= 0) continue;
+                int32_t peak = 0, wsum = 0;
+                for (int64_t k = gp[(size_t)g]; k < gp[(size_t)g + 1]; ++k)
+                    peak = std::max(peak, load[(size_t)glev[(size_t)k]] + gcnt[(size_t)k]);
+                for (int64_t k = ap[(size_t)g]; k < ap[(size_t)g + 1]; ++k)
+                    if (blk_of_grp[(size_t)adj[(size_t)k].first] == b) wsum += adj[(size_t)k].second;
+                if (peak < best_peak || (peak == best_peak && (wsum > best_w || (wsum == best_w && g < best)))) {
+                    best = g;
+                    best_peak = peak;
+                    best_w = wsum;
+                }
+            }
+            if (best < 0 || best_peak > cap) break;
+            add(best);
+            front.erase(std::remove_if(front.begin(), front.end(),
+                                       [&](int32_t g) { return blk_of_grp[(size_t)g] >= 0; }),
+                        front.end());
+        }
+    }
+    for (size_t q = 0; q < C; ++q) blk_of_comp[q] = blk_of_grp[(size_t)grp[q]];
+    return nb;
+}
+
 // Delayed levels of a sweep whose rows are grouped by comp_of: a row's level is one past
-// its deepest dependency in its own group and two past one in another group; returns
+// its deepest dependency in its own group and `gap` past one in another group; returns
 // the number of levels.
 static int delayed_levels(int64_t n, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, bool upper,
-                          const std::vector<int32_t>& comp_of, std::vector<int32_t>& dlev) {
+                          const std::vector<int32_t>& comp_of, int gap, std::vector<int32_t>& dlev) {
     auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
     dlev.assign((size_t)n, 0);
     int maxl = 0;
@@ -955,7 +1167,7 @@ static int delayed_levels(int64_t n, const std::vector<int32_t>& rp, const std::
         for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
             const int32_t j = col[(size_t)k];
             if (take(i, j))
-                l = std::max(l, dlev[(size_t)j] + (comp_of[(size_t)j] == comp_of[(size_t)i] ? 1 : 2));
+                l = std::max(l, dlev[(size_t)j] + (comp_of[(size_t)j] == comp_of[(size_t)i] ? 1 : gap));
         }
         dlev[(size_t)i] = l;
         maxl = std::max(maxl, l);
@@ -965,15 +1177,18 @@ static int delayed_levels(int64_t n, const std::vector<int32_t>& rp, const std::
 
 // The block and level assignment of a blocked sweep layout (see k_sweep_blocked).  Rows
 // joined by a dependency one level apart are kept in one block (union-find over those
-// edges) and the components are packed into at most `target` blocks (pack_components);
+// edges) and the components are packed into at most `target` blocks (pack_components), or,
+// with `gap` > 2, grown into compact blocks of about n / target rows (grow_blocks);
 // when one component holds more than an eighth of the rows, the blocks are instead
-// target_consecutive ranges of consecutive rows with delayed levels (delayed_levels).
-// False (no blocked layout) when that would cost more than half again the levels or keep
-// fewer than 8 blocks busy per level, or when a row has more than kBlockMaxDeps
-// dependencies: the persistent sweep then serves.
+// target_consecutive ranges of consecutive rows.  Levels are then delayed (delayed_levels)
+// so that every dependency between blocks spans at least `gap` levels (gap >= 2): with
+// three or more, a block's loader wave has a whole level to fetch each value from outside
+// (the components alone give two).  False (no blocked layout) when consecutive blocks
+// would cost more than half again the levels or keep fewer than 8 blocks busy per level,
+// or when a row has more than kBlockMaxDeps dependencies: the persistent sweep then serves.
 static bool assign_blocked(const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n, bool upper,
                            const std::vector<int32_t>& lev_in, int nlevels_in, int target, int target_consecutive,
-                           BlockAssign& A) {
+                           int gap, BlockAssign& A) {
     auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
     if (n < 4096 || target < 2) return false;
     std::vector<int32_t> par((size_t)n);
@@ -1021,11 +1236,11 @@ static bool assign_blocked(const std::vector<int32_t>& rp, const std::vector<int
         if ((int64_t)sz * 8 > n) contiguous = true;
     // One component holding much of the matrix (every dependency one level deep, as on
     // extruded and 2D meshes): blocks of consecutive rows instead, with delayed levels.
-    // A row's level is then one past its deepest dependency in its own block and two
-    // past one in another block, so every dependency between blocks spans at least two
-    // levels, as the kernel requires.  Built only when that costs at most half again the
-    // levels and keeps at least 8 blocks busy per level on average (on a 2D mesh the
-    // consecutive blocks would run one after another).
+    // A row's level is then one past its deepest dependency in its own block and `gap`
+    // past one in another block (two, the least the kernel requires, where `gap` would
+    // cost more than half again the levels).  Built only when that costs at most half
+    // again the levels and keeps at least 8 blocks busy per level on average (on a 2D mesh
+    // the consecutive blocks would run one after another).
     std::vector<int32_t> dlev;
     int nlevels = nlevels_in;
     if (contiguous) {
@@ -1035,7 +1250,9 @@ static bool assign_blocked(const std::vector<int32_t>& rp, const std::vector<int
             comp_of[(size_t)i] = (int32_t)(i * nbc / n);
             comp_size[(size_t)comp_of[(size_t)i]]++;
         }
-        nlevels = delayed_levels(n, rp, col, upper, comp_of, dlev);
+        nlevels = delayed_levels(n, rp, col, upper, comp_of, gap, dlev);
+        if (gap > 2 && (int64_t)nlevels * 2 > (int64_t)nlevels_in * 3)   // the least delay instead
+            nlevels = delayed_levels(n, rp, col, upper, comp_of, gap = 2, dlev);
         if ((int64_t)nlevels * 2 > (int64_t)nlevels_in * 3) return false;
         std::vector<int64_t> pairs((size_t)n);
         for (int64_t i = 0; i < n; ++i) pairs[(size_t)i] = (int64_t)comp_of[(size_t)i] * nlevels + dlev[(size_t)i];
@@ -1043,7 +1260,6 @@ static bool assign_blocked(const std::vector<int32_t>& rp, const std::vector<int
         const int64_t busy = (int64_t)(std::unique(pairs.begin(), pairs.end()) - pairs.begin());
         if (busy < 8 * (int64_t)nlevels) return false;
     }
-    const std::vector<int32_t>& lev = contiguous ? dlev : lev_in;
     // blocks: the consecutive row ranges as they are, or the components packed
     const size_t C = comp_size.size();
     int32_t nb = 0;
@@ -1051,15 +1267,24 @@ static bool assign_blocked(const std::vector<int32_t>& rp, const std::vector<int
     if (contiguous) {
         for (size_t q = 0; q < C; ++q) blk_of_comp[q] = (int32_t)q;
         nb = (int32_t)C;
+    } else if (gap > 2) {
+        nb = grow_blocks(n, rp, col, upper, comp_of, C, lev_in, target, kBlockGrowCap, blk_of_comp);
     } else {
-        nb = pack_components(n, rp, col, upper, comp_of, C, lev, nlevels, target, blk_of_comp);
+        nb = pack_components(n, rp, col, upper, comp_of, C, lev_in, nlevels_in, target, blk_of_comp);
     }
     A.nb = nb;
-    A.nlevels = nlevels;
     A.kmax = kmax;
-    A.lev = lev;
     A.blk.resize((size_t)n);
     for (int64_t i = 0; i < n; ++i) A.blk[(size_t)i] = blk_of_comp[(size_t)comp_of[(size_t)i]];
+    if (contiguous) {
+        A.nlevels = nlevels;
+        A.lev = std::move(dlev);
+    } else if (gap > 2) {   // the components give two levels between blocks; delay for more
+        A.nlevels = delayed_levels(n, rp, col, upper, A.blk, gap, A.lev);
+    } else {
+        A.nlevels = nlevels_in;
+        A.lev = lev_in;
+    }
     return true;
 }
 
@@ -1182,6 +1407,7 @@ static int materialize_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const
     std::vector<int32_t> fill(bxidx);
     std::sort(fill.begin(), fill.end());
     fill.erase(std::unique(fill.begin(), fill.end()), fill.end());
+    if (bxidx.empty()) bxidx.push_back(0);   // the loader's clamped loads read one entry
     sw.h_brow = order;
     sw.bL = nlevels;
     sw.nblk = nb;
@@ -1211,9 +1437,9 @@ static int materialize_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const
 // The assignment is kept (sw.h_assign) so that a mirrored layout can be made from it.
 static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
                          bool upper, const std::vector<int32_t>* src_map, const std::vector<int32_t>& lev_in,
-                         int nlevels_in, int target, int target_consecutive, Sweep& sw) {
+                         int nlevels_in, int target, int target_consecutive, int gap, Sweep& sw) {
     BlockAssign A;
-    if (!assign_blocked(rp, col, n, upper, lev_in, nlevels_in, target, target_consecutive, A)) return FVG_OK;
+    if (!assign_blocked(rp, col, n, upper, lev_in, nlevels_in, target, target_consecutive, gap, A)) return FVG_OK;
     FVG_TRY(materialize_blocked(c, rp, col, n, upper, src_map, A, false, sw));
     if (sw.nblk > 0) sw.h_assign = std::move(A);
     return FVG_OK;
@@ -1327,15 +1553,19 @@ static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::ve
         (s = upload(c, sw.ptr, ptr.data(), sw.npos + 1)) || (s = upload(c, sw.col, ecol.data(), sw.nnz + 1)) ||
         (s = upload(c, sw.val + sw.nnz, &zero, 1)) || (s = upload(c, sw.src, esrc.data(), sw.nnz)))
         return s;
-    // packed components: about one block per CU; consecutive rows: two per CU (measured on
+    // packed components: about one block per CU, or with a gap of three or more, grown
+    // blocks of about n / (CUs / 4) rows (on 1,572,864 tetrahedra ~95 blocks whose widest
+    // level reaches 128 rows or one pair's); consecutive rows: two per CU (measured on
     // 2,097,152 prisms: 597 us per ILU(0) apply with 512 blocks, 621 with 256, 608 with 1024)
-    const int packed = c->tune_sweep_blocks ? (int)c->tune_sweep_blocks : std::max(2, c->num_cus);
+    const int gap = c->tune_sweep_gap ? (int)c->tune_sweep_gap : kBlockGap;
+    const int packed = c->tune_sweep_blocks ? (int)c->tune_sweep_blocks
+                                            : std::max(2, gap > 2 ? c->num_cus / 4 : c->num_cus);
     const int consecutive = c->tune_sweep_blocks ? (int)c->tune_sweep_blocks : std::max(2, 2 * c->num_cus);
     if (upper && mirror && mirror->nblk > 0) {
         FVG_TRY(build_mirrored(c, rp, col, n, src_map, *mirror, sw));
         if (sw.nblk > 0) return FVG_OK;
     }
-    return build_blocked(c, rp, col, n, upper, src_map, lev, sw.nlevels, packed, consecutive, sw);
+    return build_blocked(c, rp, col, n, upper, src_map, lev, sw.nlevels, packed, consecutive, gap, sw);
 }
 
 // When both layouts of a factor are blocked, the upper set-up reads the lower sweep's
@@ -1606,8 +1836,7 @@ static const void* blocked_kernel(int K, bool spin, int fuse = kFuseNone) {
 }
 
 static size_t blocked_lds(const Sweep& sw) {
-    return (size_t)(kBlockRing * sw.bW + 2 * sw.bX) * sizeof(double) +
-           (size_t)(2 * (sw.bL + 1) + sw.bF) * sizeof(int32_t);
+    return (size_t)(kBlockRing * sw.bW + 2 * sw.bX) * sizeof(double) + (size_t)(sw.bL + 1) * sizeof(int32_t);
 }
 
 // whether every block of the blocked layout can be resident at once (decided on first use)
@@ -2309,7 +2538,7 @@ int fvg_prec_sweep_layout(const fvg_prec* p, int32_t* info, int32_t count) {
     if (!p || (count > 0 && !info)) return fail(FVG_E_INVALID, "null argument");
     const int64_t v[] = {p->lower.nblk, p->upper.nblk, p->lower.nlevels, p->lower.bW, p->lower.bX,
                          p->lower.nbx,  p->upper.bW,   p->upper.bX,       p->lower.bK, p->upper.bK,
-                         p->upper.mirrored ? 1 : 0};
+                         p->upper.mirrored ? 1 : 0,    p->lower.nblk ? p->lower.bL : 0};
     for (int k = 0; k < count && k < (int)(sizeof(v) / sizeof(v[0])); ++k)
         info[k] = (int32_t)std::min<int64_t>(v[k], INT32_MAX);
     return FVG_OK;

@@ -79,6 +79,7 @@ struct Sweep {
     uint8_t* bpolled = nullptr;
 };
 constexpr int kBlockRing = 3;           // levels of values a block keeps in LDS
+constexpr int kBlockGap = 3;            // least levels between blocks of a blocked layout (default)
 constexpr uint16_t kNoSlot = 0xFFFF;    // an unused dependency slot of the blocked layout
 
 // One strict triangular factor as a CSR in natural row order with FP32 values, for the

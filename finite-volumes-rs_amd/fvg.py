@@ -37,7 +37,7 @@ OP_ADD, OP_SUB, OP_NEG = 16, 17, 18
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 (TUNE_SWEEP_MODE, TUNE_SWEEP_WAVES, TUNE_SWEEP_POLL, TUNE_INNER_EXACT, TUNE_INNER_SWEEPS, TUNE_GRAPHS,
  TUNE_SWEEP_BLOCKS, TUNE_SPIN_LIMIT, TUNE_SWEEP_LAUNCH, TUNE_SWEEP_WITHHOLD, TUNE_SOLVE_ORDER, TUNE_ASSEMBLE,
- TUNE_CG_FUSION) = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13)
+ TUNE_CG_FUSION, TUNE_SWEEP_GAP) = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14)
 
 
 class FvgError(RuntimeError):
@@ -1145,11 +1145,13 @@ class Preconditioner:
         levels, lower_rows / lower_fetch (most rows of a block in one level, most values it
         fetches from outside for one level), lower_fetch_total, upper_rows, upper_fetch,
         lower_slots / upper_slots (dependency slots per row: 4 or 8), mirrored (1: the upper
-        layout is the lower one reversed; FP64 solves then iterate in position order)."""
-        info = np.zeros(11, dtype=np.int32)
-        check(lib().fvg_prec_sweep_layout(self.h, info.ctypes.data, 11))
+        layout is the lower one reversed; FP64 solves then iterate in position order),
+        block_levels (levels of the blocked lower layout, more than `levels` where it delays
+        dependencies between blocks; 0: not built)."""
+        info = np.zeros(12, dtype=np.int32)
+        check(lib().fvg_prec_sweep_layout(self.h, info.ctypes.data, 12))
         keys = ("lower_blocks", "upper_blocks", "levels", "lower_rows", "lower_fetch", "lower_fetch_total",
-                "upper_rows", "upper_fetch", "lower_slots", "upper_slots", "mirrored")
+                "upper_rows", "upper_fetch", "lower_slots", "upper_slots", "mirrored", "block_levels")
         return dict(zip(keys, (int(v) for v in info)))
 
     def refactor_async(self, matrix):

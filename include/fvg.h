@@ -112,7 +112,13 @@ enum {
     /* FP64 CG in position order: 0 (default) its residual update and r.r are done by the
      * lower sweep and r.z by the upper sweep of each preconditioner apply; 1 separate
      * kernels.  The same values either way. */
-    FVG_TUNE_CG_FUSION = 13
+    FVG_TUNE_CG_FUSION = 13,
+    /* Least number of levels between a value of one workgroup of the blocked triangular
+     * sweep and its use in another, in the layouts of preconditioners created afterwards:
+     * 0 (default) 3, so a workgroup fetches each such value a whole level before it needs
+     * it; 2 to 8 that many (2: no delay beyond what the dependency components give).  More
+     * levels for fewer waits on the fetches.  The same values every way. */
+    FVG_TUNE_SWEEP_GAP = 14
 };
 int fvg_ctx_set_tuning(fvg_ctx* ctx, int32_t knob, int64_t value);
 int fvg_ctx_get_tuning(const fvg_ctx* ctx, int32_t knob, int64_t* value);
@@ -405,13 +411,15 @@ int fvg_ic_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out)
  * fvg_precondition, not batches (FVG_E_UNSUPPORTED there). */
 int fvg_ilu_create(fvg_ctx* ctx, const fvg_csr* a, int32_t level, fvg_prec** out);
 /* Diagnostics (no reference counterpart): the shape of the blocked sweep layouts, the
- * first `count` (at most 11) of: blocks of the lower and of the upper factor (0: not
+ * first `count` (at most 12) of: blocks of the lower and of the upper factor (0: not
  * built, the persistent sweep serves), dependency levels of the lower sweep, the lower
  * layout's most rows of one block in one level and most values one block fetches from
  * outside for one level, its total number of such fetches per sweep, the upper layout's
  * most rows and most fetches, the dependency slots per row of the lower and of the
- * upper layout (4 or 8), and 1 if the upper layout is the lower one mirrored (then FP64
- * solves iterate in the lower layout's position order, FVG_TUNE_SOLVE_ORDER). */
+ * upper layout (4 or 8), 1 if the upper layout is the lower one mirrored (then FP64
+ * solves iterate in the lower layout's position order, FVG_TUNE_SOLVE_ORDER), and the
+ * levels of the blocked lower layout (more than the dependency levels where it delays
+ * the dependencies between workgroups, FVG_TUNE_SWEEP_GAP; 0 when not built). */
 int fvg_prec_sweep_layout(const fvg_prec* prec, int32_t* info, int32_t count);
 /* Re-run the numeric factorisation for new values on the same pattern. */
 int fvg_prec_refactor(fvg_ctx* ctx, fvg_prec* p, const fvg_csr* a);

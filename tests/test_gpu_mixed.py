@@ -108,6 +108,45 @@ def test_sweep_launch_shapes_give_identical_bits(ctx, mode, waves, poll):
         ctx.set_tuning(fvg.TUNE_SWEEP_POLL, 0)
 
 
+@pytest.mark.parametrize("name,kind,prec", [("tet10", "sym", "ic"), ("tet10", "nonsym", "ilu"),
+                                           ("prism24", "nonsym", "ilu"), ("prism24", "sym", "ic")])
+def test_blocked_sweep_gaps_give_identical_bits(ctx, name, kind, prec):
+    """Layouts that put 2, 3 (the default), 4 or 6 levels between a value of one workgroup
+    and its use in another (components packed, or paired and grown into compact blocks, or
+    consecutive rows) all give the oracle's apply bits, and FP64 solves with them the same
+    iterations and solution bits: the layout changes the schedule, never the arithmetic.
+    A delayed layout has at least the dependency levels."""
+    rp, ci, va, b = _oracle_system(name, kind)
+    n = rp.shape[0] - 1
+    A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+    B = fvg.DistributedVector.from_data(ctx, b)
+    r = np.random.default_rng(5).normal(size=n)
+    ref = O.OraclePrec(prec, rp, ci, va, 0).apply(r)
+    fn = fvg.conjugate_gradient if prec == "ic" else fvg.bi_conjugate_gradient_stab
+    opts = fvg.LinearSolverOptions(1e-8, 0.0, 2000)
+    for bad in (1, 9, -1):
+        with pytest.raises(fvg.FvgError):
+            ctx.set_tuning(fvg.TUNE_SWEEP_GAP, bad)
+    solves = []
+    try:
+        for gap in (2, 3, 4, 6):
+            ctx.set_tuning(fvg.TUNE_SWEEP_GAP, gap)
+            P = (fvg.IncompleteCholesky if prec == "ic" else fvg.IncompleteLowerUpper).from_matrix(A, 0)
+            lay = P.sweep_layout()
+            assert lay["lower_blocks"] > 0 and lay["block_levels"] >= lay["levels"], (gap, lay)
+            z = fvg.DistributedVector(ctx, n)
+            P.precondition(z, fvg.DistributedVector.from_data(ctx, r))
+            assert np.array_equal(z.numpy(), ref), gap
+            x = fvg.DistributedVector(ctx, n)
+            info = fn(x, A, B, P, opts)
+            solves.append((gap, info.iterations, x.numpy()))
+            P.close()
+    finally:
+        ctx.set_tuning(fvg.TUNE_SWEEP_GAP, 0)
+    for gap, it, xg in solves[1:]:
+        assert it == solves[0][1] and np.array_equal(xg, solves[0][2]), gap
+
+
 @pytest.mark.parametrize("name,kind,prec,blocked", [("tet10", "sym", "ic", True), ("tet10", "nonsym", "ilu", True),
                                                     ("prism24", "nonsym", "ilu", True), ("prism24", "sym", "ic", True),
                                                     ("tri64", "sym", "ic", False)])
