@@ -573,19 +573,24 @@ __global__ void __launch_bounds__(1024)
         // layout puts three or more levels between blocks, each value was published before
         // its poll was issued.  A level's polls past the first 64 * kFetchBatch are issued
         // and awaited when the level is awaited.
-        int pj1[kFetchBatch], ij2[kFetchBatch];   // positions polled for levels s + 1, s + 2
-        U pw1[kFetchBatch];                         // the polls of level s + 1 in flight
-        int a1, b1, a2, b2, a3, b3;                 // bxidx offsets of levels s + 1 .. s + 3
-        auto offsets = [&](int l, int& a, int& b) {   // empty past the block's last level
+        // A level's loads pass through one of four register slots over four steps (offsets,
+        // positions, polls, staging); the loop is unrolled by four so that every slot keeps
+        // registers of its own and no value still in flight is ever copied.
+        struct Slot {
+            int a, b;                // bxidx offsets of the slot's level
+            int jj[kFetchBatch];     // positions it polls (-1: none)
+            U w[kFetchBatch];        // the polls
+        };
+        auto offsets = [&](int l, Slot& S) {   // empty past the block's last level
             const int lc = l < Le ? l : Le - 1;
-            a = xp[lc];
-            b = xp[lc + 1];
-            if (l >= Le) b = a;
+            S.a = xp[lc];
+            S.b = xp[lc + 1];
+            if (l >= Le) S.b = S.a;
         };
         // Every load of the pipeline is unconditional (indices clamped into range, unused
         // results replaced afterwards), so each step issues the same number of them and the
         // waits before use count only what is older.
-        auto positions = [&](int a, int b, int (&jj)[kFetchBatch]) {
+        auto positions_at = [&](int a, int b, int (&jj)[kFetchBatch]) {
 #pragma unroll
             for (int q = 0; q < kFetchBatch; ++q) {
                 const int k = a + q * 64 + lt;
@@ -616,61 +621,52 @@ __global__ void __launch_bounds__(1024)
                     if (w[q] == Bits<T>::pending) w[q] = load_word(xpos, jj[q]);
             }
         };
-        // level l (offsets a, b) into its staging slot: the polls in flight, then the rest
-        auto stage_level = [&](int l, int a, int b, const int (&jj)[kFetchBatch], U (&w)[kFetchBatch]) {
+        // level l's values into its staging slot: the polls in flight, then any further ones
+        auto stage_level = [&](int l, Slot& S) {
             T* st = stage + (l & 1) * X;
-            settle(jj, w);
+            settle(S.jj, S.w);
 #pragma unroll
             for (int q = 0; q < kFetchBatch; ++q)
-                if (jj[q] >= 0) st[q * 64 + lt] = Bits<T>::value(w[q]);
-            for (int kb = a + 64 * kFetchBatch; kb < b; kb += 64 * kFetchBatch) {
+                if (S.jj[q] >= 0) st[q * 64 + lt] = Bits<T>::value(S.w[q]);
+            for (int kb = S.a + 64 * kFetchBatch; kb < S.b; kb += 64 * kFetchBatch) {
                 int jr[kFetchBatch];
                 U wr[kFetchBatch];
-                positions(kb, b, jr);
+                positions_at(kb, S.b, jr);
                 poll(jr, wr);
                 settle(jr, wr);
 #pragma unroll
                 for (int q = 0; q < kFetchBatch; ++q)
-                    if (jr[q] >= 0) st[kb - a + q * 64 + lt] = Bits<T>::value(wr[q]);
+                    if (jr[q] >= 0) st[kb - S.a + q * 64 + lt] = Bits<T>::value(wr[q]);
             }
         };
-        {   // level Lb at once; the pipeline primed with levels Lb + 1 .. Lb + 3
-            int a0, b0, j0[kFetchBatch];
-            U w0[kFetchBatch];
-            offsets(Lb, a0, b0);
-            offsets(Lb + 1, a1, b1);
-            offsets(Lb + 2, a2, b2);
-            offsets(Lb + 3, a3, b3);
-            positions(a0, b0, j0);
-            poll(j0, w0);
-            positions(a1, b1, pj1);
-            poll(pj1, pw1);
-            positions(a2, b2, ij2);
-            stage_level(Lb, a0, b0, j0, w0);
-        }
-        __syncthreads();
-        for (int l = Lb; l < Lend; ++l) {
-            int pj2[kFetchBatch], ij3[kFetchBatch], a4, b4;
-            U pw2[kFetchBatch];
-            offsets(l + 4, a4, b4);
-            positions(a3, b3, ij3);            // level l + 3
-#pragma unroll
-            for (int q = 0; q < kFetchBatch; ++q) pj2[q] = ij2[q];
-            poll(pj2, pw2);                    // level l + 2
-            stage_level(l + 1, a1, b1, pj1, pw1);
-#pragma unroll
-            for (int q = 0; q < kFetchBatch; ++q) {
-                pj1[q] = pj2[q];
-                pw1[q] = pw2[q];
-                ij2[q] = ij3[q];
-            }
-            a1 = a2;
-            b1 = b2;
-            a2 = a3;
-            b2 = b3;
-            a3 = a4;
-            b3 = b4;
+        // the step of level l: offsets of l + 4 (into the slot level l left), positions of
+        // l + 3, polls of l + 2, and level l + 1 staged
+        auto step = [&](int l, Slot& S4, Slot& S3, Slot& S2, Slot& S1) {
+            offsets(l + 4, S4);
+            positions_at(S3.a, S3.b, S3.jj);
+            poll(S2.jj, S2.w);
+            stage_level(l + 1, S1);
             __syncthreads();
+        };
+        Slot S0, S1, S2, S3;
+        // level Lb at once; the pipeline primed with levels Lb + 1 .. Lb + 3
+        offsets(Lb, S0);
+        offsets(Lb + 1, S1);
+        offsets(Lb + 2, S2);
+        offsets(Lb + 3, S3);
+        positions_at(S0.a, S0.b, S0.jj);
+        positions_at(S1.a, S1.b, S1.jj);
+        positions_at(S2.a, S2.b, S2.jj);
+        poll(S0.jj, S0.w);
+        poll(S1.jj, S1.w);
+        stage_level(Lb, S0);
+        __syncthreads();
+        static_assert(kBlockAhead % 4 == 0, "the loader's loop runs whole groups of four steps to Lend");
+        for (int L0 = Lb; L0 < Lend; L0 += 4) {   // slot of level v: S[(v - Lb) % 4]
+            step(L0, S0, S3, S2, S1);
+            step(L0 + 1, S1, S0, S3, S2);
+            step(L0 + 2, S2, S1, S0, S3);
+            step(L0 + 3, S3, S2, S1, S0);
         }
         if constexpr (FUSE == kFuseNone) return;
     } else {
