@@ -39,6 +39,13 @@ def sweep_bytes(n, nnz_strict, w=8):
     return (w + 4) * nnz_strict + 4 * (n + 1) + 3 * w * n
 
 
+# Bytes per row a fused CG sweep streams besides its triangular solve, averaged over the
+# pair: the lower sweep forms its right-hand side r - ap * alpha itself and updates x, so
+# it also reads ap, x and p and writes x and r (5 * 8 = 40); the upper one reads r for r.z
+# (8).
+FUSED_CG_BYTES_PER_ROW = (40 + 8) // 2
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -618,9 +625,11 @@ def main():
     spmv_avg_ms = ks["spmv_ms"] / max(1, ks["spmv_calls"])
     sweep_avg_ms = ks["sweep_ms"] / max(1, ks["sweep_calls"])
     spmv_gbs = spmv_bytes(n, nnz) / (spmv_avg_ms * 1e-3) / 1e9
-    sweep_gbs = sweep_bytes(n, nnz_s) / (sweep_avg_ms * 1e-3) / 1e9
+    fused = bool(infos[-1].flags & fvg.INFO_FUSED_SWEEPS)
+    sweep_alg = sweep_bytes(n, nnz_s) + (FUSED_CG_BYTES_PER_ROW * n if fused else 0)
+    sweep_gbs = sweep_alg / (sweep_avg_ms * 1e-3) / 1e9
     if ks["sweep_ms"] >= ks["spmv_ms"]:
-        dom = dict(kernel="ic0_sweep", bytes=sweep_bytes(n, nnz_s), gbs=sweep_gbs, ms=sweep_avg_ms)
+        dom = dict(kernel="ic0_sweep", bytes=sweep_alg, gbs=sweep_gbs, ms=sweep_avg_ms)
     else:
         dom = dict(kernel="spmv", bytes=spmv_bytes(n, nnz), gbs=spmv_gbs, ms=spmv_avg_ms)
     lo_lv, up_lv = ic.levels()
@@ -671,7 +680,12 @@ def main():
             "ic0_sweep": {"span": "one triangular solve: one k_sweep_blocked launch (the solve iterates in the "
                                   "lower sweep's position order with the upper layout mirrored, so no set-up or "
                                   f"scatter kernel); HIP events around one launch group in {TIMING_EVERY}",
-                          "avg_ms": sweep_avg_ms, "algorithmic_bytes": sweep_bytes(n, nnz_s), "GB/s": sweep_gbs,
+                          "fused_cg_work": fused,
+                          "algorithmic_bytes_note": "triangular solve (values, columns, row offsets, b, x, scale) "
+                                                    + (f"plus the fused CG vectors, {FUSED_CG_BYTES_PER_ROW} B per "
+                                                       "row on average over the lower and upper sweep" if fused
+                                                       else ""),
+                          "avg_ms": sweep_avg_ms, "algorithmic_bytes": sweep_alg, "GB/s": sweep_gbs,
                           "frac": sweep_gbs / HBM_PEAK_GBS, "calls": ks["sweep_calls"], "launches": ks["sweep_launches"],
                           "total_ms": ks["sweep_ms"], "levels": [lo_lv, up_lv]},
         },

@@ -455,6 +455,7 @@ int KRun::finish(fvg_solver_info* infos, double* const* histories, int64_t cap) 
             info.final_residual = s.error;
             info.history_len = s.hist_len;
             info.restarts = s.restarts;
+            info.flags = path;
         }
         if (histories && histories[i] && cap > 0) {
             int64_t m = s.hist_len < cap ? s.hist_len : cap;
@@ -557,6 +558,7 @@ static int run_cg_pos(fvg_ctx* c, const fvg_prec* p, const double* b, double* x,
     }
     lf.step = CG_ERROR;
     uf.step = CG_BETA;
+    K.path = FVG_INFO_POSITION_ORDER | (fuse ? FVG_INFO_FUSED_SWEEPS : 0);
     for (int64_t it = 0;; ++it) {
         SpmvEpilogue e1;   // ap = A p with p.ap
         e1.dotw = pp;
@@ -621,6 +623,7 @@ static int run_bicgstab_pos(fvg_ctx* c, const fvg_prec* p, const double* b, doub
     SCALAR(BI_INIT, part);
     hipLaunchKernelGGL(k_pos_pending, dim3(nb), dim3(kBlock), 0, c->stream, n, K.st, P);
     FVG_HIP(hipGetLastError());
+    K.path = FVG_INFO_POSITION_ORDER;
     for (int64_t it = 0;; ++it) {
         FVG_TRY(apply_pos(c, p, pp, all_done));   // y = M^-1 p (reversed in up_x)
         SpmvEpilogue e1;                            // v = A y, rh.v

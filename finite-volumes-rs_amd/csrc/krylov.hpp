@@ -24,6 +24,7 @@ struct KRun {
     int64_t hist_cap = 0;
     int32_t* flags = nullptr;   // [0] members still iterating, [1] all done
     bool owned = true;          // st, hist and flags were allocated by init()
+    int32_t path = 0;           // FVG_INFO_POSITION_ORDER / FVG_INFO_FUSED_SWEEPS of the run
     int init(fvg_ctx* ctx, int ninst, const fvg_solver_options* o);
     // the same state in caller-provided device memory (nothing allocated, nothing freed):
     // ninst states, ninst * (max_iterations + 2) history slots, 4 flags

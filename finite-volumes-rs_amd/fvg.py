@@ -35,6 +35,8 @@ INTERP_LINEAR, INTERP_UPWIND, INTERP_LIMITED_LINEAR = 0, 1, 2
 FNGRAD_ORTHOGONAL, FNGRAD_CORRECTED = 0, 1
 OP_ADD, OP_SUB, OP_NEG = 16, 17, 18
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+# fvg_solver_info.flags
+INFO_FP64_FALLBACK, INFO_POSITION_ORDER, INFO_FUSED_SWEEPS = 1, 2, 4
 (TUNE_SWEEP_MODE, TUNE_SWEEP_WAVES, TUNE_SWEEP_POLL, TUNE_INNER_EXACT, TUNE_INNER_SWEEPS, TUNE_GRAPHS,
  TUNE_SWEEP_BLOCKS, TUNE_SPIN_LIMIT, TUNE_SWEEP_LAUNCH, TUNE_SWEEP_WITHHOLD, TUNE_SOLVE_ORDER, TUNE_ASSEMBLE,
  TUNE_CG_FUSION, TUNE_SWEEP_GAP) = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14)

@@ -473,9 +473,13 @@ typedef struct fvg_solver_info {
                                    FVG_E_TIMEOUT when a multi-workgroup barrier wait expired) */
     int32_t flags;              /* FVG_INFO_FP64_FALLBACK: a mixed solve whose FP32 iterations stalled
                                    needed FP64 iterations (the FP64 solver from the refined solution,
-                                   or FP64 inner iterations in multi-workgroup batches) */
+                                   or FP64 inner iterations in multi-workgroup batches);
+                                   FVG_INFO_POSITION_ORDER: an FP64 solve iterated with its vectors in
+                                   the lower sweep's position order (FVG_TUNE_SOLVE_ORDER);
+                                   FVG_INFO_FUSED_SWEEPS: an FP64 CG did its residual update and dot
+                                   products inside the triangular sweeps (FVG_TUNE_CG_FUSION) */
 } fvg_solver_info;
-enum { FVG_INFO_FP64_FALLBACK = 1 };
+enum { FVG_INFO_FP64_FALLBACK = 1, FVG_INFO_POSITION_ORDER = 2, FVG_INFO_FUSED_SWEEPS = 4 };
 
 /* conjugate_gradient (solvers/conjugate_gradient.rs:12-112).  x is the initial
  * guess on entry and the solution on exit.  history (may be NULL) receives up to
