@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-qps", type=float, default=80.0)
+    ap.add_argument("--serve-fresh", type=int, default=60, help="requests on meshes of their own (serving leg)")
+    ap.add_argument("--serve-fresh-qps", type=float, default=10.0)
     ap.add_argument("--serve-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain device copy of known size (scales rocprof FETCH_SIZE / WRITE_SIZE "
@@ -173,7 +175,8 @@ def leg_serving(args, fvg, workloads):
     process of its own (its streams then get hardware queues of their own)."""
     import subprocess
     cmd = [sys.executable, os.path.abspath(__file__), "--serve-child", "--serve-requests", str(args.serve_requests),
-           "--serve-qps", str(args.serve_qps)]
+           "--serve-qps", str(args.serve_qps), "--serve-fresh", str(args.serve_fresh),
+           "--serve-fresh-qps", str(args.serve_fresh_qps)]
     # the server keeps several streams busy at once (per lane: preparation and batches;
     # allocation, result copies): give each a hardware queue of its own (read when the HIP
     # runtime starts)
@@ -216,6 +219,31 @@ def _serve(args, fvg, workloads):
     hs = serving.latency_summary(hi, hwall)
     hs["offered_qps"] = 3 * args.serve_qps
     hs["failed"] = int(sum(1 for r in hi if r.error is not None or not (r.info.true_residual <= 1e-8)))
+    # requests on meshes of their own: the server builds each mesh, assembles its pattern and
+    # analyses its factor inside the request's latency (the client made the fields with the
+    # geometry from a context of its own, before the run)
+    client = fvg.Context(0)
+
+    def geometry_of_raw(raw):
+        m = fvg.Mesh(client, raw)
+        g = m.geometry()
+        m.close()
+        return g
+
+    fresh = workloads.serving_fresh_requests(geometry_of_raw, args.serve_fresh, args.serve_fresh_qps, seed=14)
+    client.close()
+    st0 = dict(srv.stats)
+    fwall = srv.run(fresh)
+    fs = serving.latency_summary(fresh, fwall)
+    built = srv.stats.get("meshes_built", 0) - st0.get("meshes_built", 0)
+    fs.update({"offered_qps": args.serve_fresh_qps, "meshes_built": built,
+               "mesh_build_ms_mean": 1e3 * (srv.stats.get("mesh_build_s", 0.0) - st0.get("mesh_build_s", 0.0)) / max(1, built),
+               "slot_setup_ms_mean": 1e3 * (srv.stats["slot_create_s"] - st0["slot_create_s"])
+               / max(1, srv.stats["slots_created"] - st0["slots_created"]),
+               "failed": int(sum(1 for r in fresh if r.error is not None or not (r.info.true_residual <= 1e-8))),
+               "workload": "the same mix, each request on a new mesh of its own (sizes log-uniform as above); "
+                           "latency includes the device mesh build, CSR pattern, assembler set-up and the "
+                           "factor's layout analysis"})
     by = {}
     for name, sel in (("tri", lambda r: r.mesh.startswith("tri")), ("tet", lambda r: r.mesh.startswith("tet"))):
         sub = [r for r in reqs if sel(r)]
@@ -235,6 +263,7 @@ def _serve(args, fvg, workloads):
               "worst_true_relative_residual": float(max(r.info.true_residual for r in reqs)),
               "fp64_fallbacks": int(sum(1 for r in reqs if r.info.flags & 1)),
               "high_load": hs,
+              "fresh_meshes": fs,
               "workload": "serving: Poisson arrivals; 70% 2D triangle meshes (1k-64k cells), 30% 3D Kuhn tet meshes "
                           "(50k-500k cells) from a pool of 16; 70% diffusion (ln k ~ N(0,1), CG + IC(0)), 30% "
                           "convection-diffusion (cell Peclet U[1,50], BiCGStab + ILU(0)); mixed precision to 1e-8; "

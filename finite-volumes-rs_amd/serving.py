@@ -1,7 +1,8 @@
 """Serving loop: continuous batching of assemble-and-solve requests on one GPU.
 
-A request names a registered mesh and carries its own fields (diffusivity and source,
-or velocity flux) and solver kind.  The server keeps *slots* resident in HBM: per mesh
+A request names a registered mesh, or carries a mesh of its own (its topology, built on
+the device when the request is placed and dropped once it is answered), and its own
+fields (diffusivity and source, or velocity flux) and solver kind.  The server keeps *slots* resident in HBM: per mesh
 and solver kind, a matrix on the mesh's pattern, its IC(0) or ILU(0) preconditioner
 (with the batched-solve plan cached on it), an assembler whose field buffers are reused,
 and right-hand-side / solution vectors.  Serving a request is: upload its fields,
@@ -62,6 +63,7 @@ class Request:
     info: object = None
     error: object = None
     batch_size: int = 0
+    raw: dict | None = None       # a mesh of the request's own (topology), when not registered
 
     @property
     def latency(self):
@@ -110,14 +112,26 @@ def form_group(pending, can_place, max_batch, allow_large=True, large_rows=100_0
 # device side
 # ---------------------------------------------------------------------------------
 
+def raw_cells(raw):
+    """Cells of a mesh given as topology arrays."""
+    return int(raw["cell_ptr"].shape[0] - 1)
+
+
+def raw_nnz_bound(raw):
+    """An upper bound of the CSR entries of a mesh given as topology arrays: the diagonal
+    plus one entry per cell-face incidence."""
+    return raw_cells(raw) + int(raw["cell_ptr"][-1])
+
+
 class _MeshEntry:
-    def __init__(self, ctx, key, raw):
+    def __init__(self, ctx, key, raw, transient=False):
         self.key = key
         self.raw = raw
         self.mesh = fvg.Mesh(ctx, raw)
         self.n = self.mesh.n_cells
         self.nnz = self.mesh.nnz
         self.slot_bytes = None    # measured when the first slot is made
+        self.transient = transient   # a request's own mesh: dropped with its last slot
 
 
 class _Slot:
@@ -217,9 +231,44 @@ class Server:
         self.t0 = time.perf_counter()
 
     # -- registration / submission -------------------------------------------------
-    def register_mesh(self, key, raw):
-        self.meshes[key] = _MeshEntry(self.ctx, key, raw)
+    def register_mesh(self, key, raw, transient=False):
+        self.meshes[key] = _MeshEntry(self.ctx, key, raw, transient)
         return self.meshes[key]
+
+    def _rows(self, req):
+        e = self.meshes.get(req.mesh)
+        return e.n if e is not None else raw_cells(req.raw)
+
+    def _need(self, req):
+        """HBM bytes a new slot for req is expected to take (measured once a slot of its
+        mesh exists; else about 1200 bytes per CSR entry, plus the mesh of its own)."""
+        e = self.meshes.get(req.mesh)
+        if e is not None:
+            return e.slot_bytes if e.slot_bytes is not None else 1200 * e.nnz
+        return 1600 * raw_nnz_bound(req.raw)
+
+    def _entry(self, req):
+        """The mesh entry of req, built on the device from the request's own topology the
+        first time (the build counts in the request's latency)."""
+        e = self.meshes.get(req.mesh)
+        if e is None:
+            if req.raw is None:
+                raise KeyError(f"mesh {req.mesh} is not registered and the request carries none")
+            t = time.perf_counter()
+            e = self.register_mesh(req.mesh, req.raw, transient=True)
+            self.stats["mesh_build_s"] = self.stats.get("mesh_build_s", 0.0) + time.perf_counter() - t
+            self.stats["meshes_built"] = self.stats.get("meshes_built", 0) + 1
+        return e
+
+    def _drop_transient(self, slots):
+        """Close the idle slots of requests' own meshes, and those meshes once unused."""
+        for s in slots:
+            if s.entry.transient and not s.busy and s in self.slots:
+                s.close()
+                self.slots.remove(s)
+        for key in [k for k, e in self.meshes.items() if e.transient]:
+            if not any(s.entry.key == key for s in self.slots):
+                self.meshes.pop(key).mesh.close()
 
     def submit(self, req):
         self.pending.append(req)
@@ -251,7 +300,7 @@ class Server:
         have = sum(1 for s in self.slots if s.entry.key == template.mesh and s.kind == template.kind)
         for _ in range(max(0, count - have)):
             entry = self.meshes[template.mesh]
-            need = entry.slot_bytes if entry.slot_bytes is not None else 1200 * entry.nnz
+            need = self._need(template)
             if self.ctx.bytes_in_use() + need > self.budget:
                 break
             t = time.perf_counter()
@@ -273,11 +322,10 @@ class Server:
         s = self._free_slot(req)
         if s is not None:
             return s
-        entry = self.meshes[req.mesh]
-        need = entry.slot_bytes if entry.slot_bytes is not None else 1200 * entry.nnz
-        if not self._evict_for(need):
+        if not self._evict_for(self._need(req)):
             return None
         try:
+            entry = self._entry(req)
             t = time.perf_counter()
             s = _Slot(self, entry, req.kind, req)
             self.stats["slot_create_s"] += time.perf_counter() - t
@@ -316,6 +364,7 @@ class Server:
             for s in lane.slots:
                 s.busy = False
                 s.last_used = time.perf_counter()
+            self._drop_transient(lane.slots)
             lane.batch, lane.reqs, lane.slots, lane.reported = None, [], [], None
             return True
         return bool(flags.any())
@@ -385,14 +434,11 @@ class Server:
                     reserved.add(id(s))
                     return True
                 if s is None:
-                    entry = self.meshes[r.mesh]
-                    need = entry.slot_bytes if entry.slot_bytes is not None else 1200 * entry.nnz
                     idle = sum(x.bytes for x in self.slots if not x.busy and id(x) not in reserved)
-                    return self.ctx.bytes_in_use() + need <= self.budget + idle
+                    return self.ctx.bytes_in_use() + self._need(r) <= self.budget + idle
                 return False
 
-            group = form_group(self.pending, can_place, room, allow_large, self.large_rows,
-                               lambda r: self.meshes[r.mesh].n)
+            group = form_group(self.pending, can_place, room, allow_large, self.large_rows, self._rows)
             if not group:
                 if all(l.batch is None for l in self.lanes) and not any(s.busy for s in self.slots):
                     r = self.pending[0]
@@ -411,7 +457,7 @@ class Server:
                 if all(l.batch is None for l in self.lanes) and not any(s.busy for s in self.slots):
                     raise RuntimeError(f"request {group[0].rid} on mesh {group[0].mesh} does not fit the HBM budget")
                 break
-            large = any(self.meshes[r.mesh].n >= self.large_rows for r in placed)
+            large = any(self._rows(r) >= self.large_rows for r in placed)
             self._launch(lane, placed, slots, large)
             inflight += len(placed)
             if large:

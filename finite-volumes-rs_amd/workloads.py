@@ -170,4 +170,38 @@ def serving_requests(pool, geometry_of, n, qps, frac_3d=0.3, frac_conv=0.3, seed
     return out
 
 
+def serving_fresh_requests(geometry_of_raw, n, qps, frac_3d=0.3, frac_conv=0.3, seed=3, pe=(1.0, 50.0),
+                           tri=(1000, 64000), tet=(50000, 500000)):
+    """n requests with Poisson arrivals at rate qps, each on a mesh of its own: a jittered
+    Kuhn tetrahedral mesh with probability frac_3d, else a jittered triangle mesh, cell
+    counts log-uniform in the given ranges; fields as serving_requests.  The mesh travels
+    with the request (Request.raw), so the server builds it, and analyses the factor's
+    pattern, inside the request's latency.  geometry_of_raw(raw) returns the
+    Mesh::compute geometry the client uses to make the fields."""
+    from serving import Request, poisson_arrivals
+    rng = np.random.default_rng(seed)
+    arrivals = poisson_arrivals(n, qps, seed + 1)
+    out = []
+    for i in range(n):
+        s = int(rng.integers(1 << 30))
+        if rng.uniform() < frac_3d:
+            t = np.exp(rng.uniform(np.log(tet[0]), np.log(tet[1])))
+            raw = meshgen.tet3d(max(2, int(round((t / 6.0) ** (1.0 / 3.0)))), jitter=0.2, seed=s)
+        else:
+            t = np.exp(rng.uniform(np.log(tri[0]), np.log(tri[1])))
+            raw = meshgen.tri2d(max(2, int(round(np.sqrt(t / 2.0)))), jitter=0.25, seed=s)
+        geom = geometry_of_raw(raw)
+        nc = raw["cell_ptr"].shape[0] - 1
+        if rng.uniform() < frac_conv:
+            f = convdiff_fields(raw, geom, pe=pe, seed=s + 2)
+            kind = "bicgstab"
+        else:
+            kc = meshgen.lognormal_cells(nc, 1.0, s + 3)
+            f = diffusion_fields(raw, geom, kc, np.random.default_rng(s + 4).standard_normal(nc), bc="zero")
+            kind = "cg"
+        out.append(Request(i, f"fresh{seed}_{i}", kind, f["prog"], f["face_fields"], f["cell_fields"], f["bc_i"],
+                           f["bc_e"], arrival=float(arrivals[i]), raw=raw))
+    return out
+
+
 WORKLOADS = {"diffusion_2d": diffusion_2d, "diffusion_3d": diffusion_3d, "convdiff_prism": convdiff_prism}

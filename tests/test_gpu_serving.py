@@ -138,3 +138,34 @@ def test_mixed_bicgstab_recovers_from_an_fp32_breakdown():
     assert np.all(np.isfinite(xs))
     assert np.linalg.norm(b - S @ xs) / np.linalg.norm(b) <= 1e-8
     batch.close()
+
+
+def test_requests_on_their_own_meshes():
+    """Requests that carry their own mesh: the server builds each mesh when it places the
+    request (inside its latency), answers to the accuracy bar, drops the mesh and its slot
+    once the request is answered (nothing of theirs stays resident), and gives the same bits
+    as the same request on a registered mesh."""
+    srv = Server(budget_bytes=2 * 10**9, lanes=2, max_batch=4)
+    try:
+        base = srv.ctx.bytes_in_use()
+        reqs = workloads.serving_fresh_requests(lambda raw: O.OracleMesh(raw).geometry(), 10, qps=2000.0,
+                                                frac_3d=0.3, frac_conv=0.4, seed=21, tri=(200, 2000),
+                                                tet=(1296, 6000))
+        srv.run(reqs)
+        assert srv.stats["meshes_built"] == len(reqs)
+        assert not any(e.transient for e in srv.meshes.values())
+        assert not srv.slots and srv.ctx.bytes_in_use() <= base + (1 << 20)
+        for r in reqs:
+            rp, ci, va, b = _oracle_system(r.raw, r)
+            n = rp.shape[0] - 1
+            S = sp.csr_matrix((va, ci, rp), shape=(n, n))
+            assert np.linalg.norm(b - S @ r.x) / np.linalg.norm(b) <= 1e-8, r.rid
+        # the same request on a registered mesh: the same bits
+        r = reqs[0]
+        srv.register_mesh("kept", r.raw)
+        again = copy.copy(r)
+        again.mesh, again.raw, again.arrival, again.done, again.x = "kept", None, 0.0, -1.0, None
+        srv.run([again])
+        assert np.array_equal(again.x, r.x)
+    finally:
+        srv.close()

@@ -1,7 +1,10 @@
 """Host logic of the serving loop (no GPU): arrivals, statistics and batch formation."""
 import numpy as np
 
-from serving import Request, form_group, latency_summary, poisson_arrivals
+import fvg
+import meshgen
+import oracle as O
+from serving import Request, form_group, latency_summary, poisson_arrivals, raw_cells, raw_nnz_bound
 
 
 def _req(i, kind="cg", mesh="m", arrival=0.0):
@@ -42,3 +45,15 @@ def test_form_group_keeps_kind_order_capacity_and_placement():
     g = form_group(pending, lambda r: r.rid == 1, max_batch=10)
     assert [r.rid for r in g] == [1]                      # an unplaceable head does not block the rest
     assert form_group(pending, lambda r: False, max_batch=4) == []
+
+
+def test_raw_mesh_size_estimates_bound_the_pattern():
+    """A request carrying its own mesh is admitted on estimates from its topology alone:
+    the cell count is exact and the CSR entry bound is at least the oracle's pattern."""
+    for raw in (meshgen.tri2d(9, seed=1), meshgen.tet3d(4, seed=2), meshgen.prism3d(5, layers=3, seed=3)):
+        om = O.OracleMesh(raw)
+        assert raw_cells(raw) == om.n_cells
+        eq = -fvg.laplacian(fvg.Orthogonal(), np.ones(om.n_faces)) - fvg.source(np.ones(om.n_cells))
+        ops, ff, cf = eq.compile()
+        rp = om.assemble(ops, ff, cf, np.zeros(om.n_bnd), np.zeros(om.n_bnd))[0]
+        assert raw_nnz_bound(raw) >= int(rp[-1])
