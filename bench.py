@@ -101,6 +101,21 @@ def leg_batched(args, fvg, workloads):
         if name == "mixed_one_workgroup":
             bt.set_cluster(0)
         tune = bt.tune(B, X)
+        bucket = None
+        if name == "mixed":
+            # rows per workgroup per size bucket, each bucket's systems timed on their own;
+            # kept when the whole batch runs faster than with the best single choice
+            single = bt.rows_per_group
+            btimes = bt.tune_buckets(B, X)
+            t_b = min(bt._time_solve(B, X, 1) for _ in range(2))
+            kept = t_b < min(tune.values())
+            bucket = {"limits": bt.buckets[0], "rows": bt.buckets[1] + [bt.rows_per_group],
+                      "ms_per_batch": 1e3 * t_b, "kept": kept,
+                      "bucket_autotune_ms": {str(k): {str(r): 1e3 * v for r, v in t.items()}
+                                             for k, t in btimes.items()}}
+            if not kept:
+                bt.set_cluster_buckets([], [])
+                bt.set_cluster(single, bt.max_workers)
         times = []
         for _ in range(max(1, args.steps)):
             for x in X:
@@ -120,6 +135,8 @@ def leg_batched(args, fvg, workloads):
         best = min(times)
         shape = ({"rows_per_workgroup": bt.rows_per_group, "resident_workgroups": bt.workers}
                  if bt.rows_per_group > 0 else {"threads_per_system": bt.threads})
+        if bucket is not None:
+            shape["size_buckets"] = bucket
         res[name] = {"solves_per_s": len(A) / best, "ms_per_batch": 1e3 * best, "launch_shape": shape,
                      "autotune_ms": {str(k): 1e3 * v for k, v in tune.items()},
                      "iterations_max": int(it.max()), "iterations_mean": float(it.mean()),

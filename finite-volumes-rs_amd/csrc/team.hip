@@ -1188,6 +1188,10 @@ struct fvg_batch {
     fvg_ctx* flight_ctx = nullptr;    // the context whose stream a launch followed (counts it in flight)
     // multi-workgroup solves of mixed-precision batches
     int cl_rows = 4096;                    // rows per workgroup (0: one workgroup per system)
+    // per size bucket: systems of at most cl_limit[i] rows (ascending) get cl_brows[i] rows
+    // per workgroup; larger ones cl_rows (fvg_batch_set_cluster_buckets)
+    std::vector<int64_t> cl_limit;
+    std::vector<int32_t> cl_brows;
     int cl_workers = 0;                    // worker pool (0: as many as can be co-resident)
     int cl_resident = 1;                   // workgroups of the multi-workgroup kernel that fit the device
     std::vector<int32_t> cl_groups;        // workgroups of each launch slot
@@ -1596,7 +1600,13 @@ static void cluster_plan(fvg_batch* B) {
     if (!B->cluster()) return;
     for (int slot = 0; slot < B->n_inst; ++slot) {
         const int64_t n = B->plan[(size_t)B->order[(size_t)slot]]->L.n;
-        const int64_t k = (n + B->cl_rows - 1) / B->cl_rows;
+        int64_t rows = B->cl_rows;
+        for (size_t q = 0; q < B->cl_limit.size(); ++q)
+            if (n <= B->cl_limit[q]) {
+                rows = B->cl_brows[q];
+                break;
+            }
+        const int64_t k = (n + rows - 1) / rows;
         // a system's workgroups must all run at once: never more than the launch's pool
         const int kmax = std::min(kClMaxGroups, std::max(B->cl_workers > 0 ? std::min(B->cl_workers, B->cl_resident)
                                                                             : B->cl_resident,
@@ -1914,6 +1924,21 @@ int fvg_batch_set_cluster(fvg_ctx* c, fvg_batch* B, int32_t rows_per_group, int3
     if (B->launched) return fail(FVG_E_INVALID, "batch in flight");
     B->cl_rows = rows_per_group;
     B->cl_workers = max_workers;
+    cluster_plan(B);
+    return FVG_OK;
+}
+
+int fvg_batch_set_cluster_buckets(fvg_ctx* c, fvg_batch* B, int32_t count, const int64_t* max_rows,
+                                  const int32_t* rows_per_group) {
+    if (!c || !B || count < 0 || (count > 0 && (!max_rows || !rows_per_group)))
+        return fail(FVG_E_INVALID, "null argument or negative count");
+    if (B->launched) return fail(FVG_E_INVALID, "batch in flight");
+    for (int32_t q = 0; q < count; ++q) {
+        if (rows_per_group[q] <= 0 || max_rows[q] <= 0) return fail(FVG_E_INVALID, "bucket sizes must be positive");
+        if (q > 0 && max_rows[q] <= max_rows[q - 1]) return fail(FVG_E_INVALID, "bucket limits must ascend");
+    }
+    B->cl_limit.assign(max_rows, max_rows + count);
+    B->cl_brows.assign(rows_per_group, rows_per_group + count);
     cluster_plan(B);
     return FVG_OK;
 }

@@ -206,6 +206,7 @@ def lib():
         "fvg_batch_get_threads": [P, C.POINTER(C.c_int32)],
         "fvg_batch_set_cluster": [P, P, C.c_int32, C.c_int32],
         "fvg_batch_get_cluster": [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+        "fvg_batch_set_cluster_buckets": [P, P, C.c_int32, P, P],
         "fvg_batch_set_phase_timing": [P, P, C.c_int32],
         "fvg_batch_phase_times": [P, P, P],
     }
@@ -1360,6 +1361,7 @@ class Batch:
         self._inflight = None
         self.rows_per_group = 4096 if self.options.precision == PRECISION_MIXED else 0
         self.max_workers = 0
+        self.buckets = ([], [])   # size buckets of set_cluster_buckets
 
     def set_cluster(self, rows_per_group=4096, max_workers=0):
         """Mixed precision: workgroups per system = ceil(n / rows_per_group) (0: one per
@@ -1367,6 +1369,17 @@ class Batch:
         check(lib().fvg_batch_set_cluster(self.ctx.h, self.h, int(rows_per_group), int(max_workers)))
         self.rows_per_group = int(rows_per_group) if self.options.precision == PRECISION_MIXED else 0
         self.max_workers = int(max_workers)
+
+    def set_cluster_buckets(self, max_rows, rows_per_group):
+        """Mixed precision: a system of at most max_rows[i] rows (ascending) runs on
+        ceil(n / rows_per_group[i]) workgroups; larger systems keep set_cluster's rows.
+        Empty lists remove the buckets."""
+        lim = np.ascontiguousarray(max_rows, dtype=np.int64)
+        rows = np.ascontiguousarray(rows_per_group, dtype=np.int32)
+        if lim.size != rows.size:
+            raise ValueError("one rows_per_group per bucket")
+        check(lib().fvg_batch_set_cluster_buckets(self.ctx.h, self.h, int(lim.size), _ptr(lim), _ptr(rows)))
+        self.buckets = (lim.tolist(), rows.tolist())
 
     @property
     def workers(self):
@@ -1430,6 +1443,39 @@ class Batch:
             self.threads = th
             times[th] = self._time_solve(rhs, solutions, repeats)
         self.threads = min(times, key=times.get)
+        return times
+
+    def tune_buckets(self, rhs, solutions, limits=(8192, 32768), candidates=(2048, 4096, 8192, 16384), repeats=1):
+        """Mixed precision: autotune the rows per workgroup per system size bucket.  The
+        systems of each bucket (at most limits[0] rows, at most limits[1], ..., the rest)
+        are timed as a batch of their own with each candidate, from x = 0, and the fastest
+        is kept for the bucket; the last bucket's choice becomes the batch's rows_per_group.
+        Overwrites the solutions.  Returns {bucket upper limit (None: the rest): {candidate:
+        seconds}}."""
+        if self.rows_per_group == 0:
+            raise ValueError("bucket tuning applies to mixed-precision clusters")
+        mats, precs = self._keep
+        sizes = [m.nrows for m in mats]
+        solver = "cg" if self.solver == SOLVER_CG else "bicgstab"
+        times, chosen, lo = {}, [], 0
+        for hi in list(limits) + [None]:
+            idx = [k for k, n in enumerate(sizes) if n > lo and (hi is None or n <= hi)]
+            lo = hi if hi is not None else lo
+            if not idx:
+                chosen.append(chosen[-1] if chosen else self.rows_per_group)
+                continue
+            sub = Batch(self.ctx, [mats[k] for k in idx], [precs[k] for k in idx], solver, self.options)
+            try:
+                t = {}
+                for rows in candidates:
+                    sub.set_cluster(rows, self.max_workers)
+                    t[rows] = sub._time_solve([rhs[k] for k in idx], [solutions[k] for k in idx], repeats)
+            finally:
+                sub.close()
+            times[hi] = t
+            chosen.append(min(t, key=t.get))
+        self.set_cluster_buckets(list(limits), chosen[:-1])
+        self.set_cluster(chosen[-1], self.max_workers)
         return times
 
     def set_phase_timing(self, enable=True):

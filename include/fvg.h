@@ -575,6 +575,13 @@ int fvg_batch_get_threads(const fvg_batch* batch, int32_t* threads);
  * each a share).  A result depends on its system and its workgroup count only.
  * No counterpart in the reference. */
 int fvg_batch_set_cluster(fvg_ctx* ctx, fvg_batch* batch, int32_t rows_per_group, int32_t max_workers);
+/* Per size bucket (mixed-precision clusters; the autotuner's choice per system size): a
+ * system of at most max_rows[i] rows (ascending) is solved by ceil(n / rows_per_group[i])
+ * workgroups, the first bucket that holds it deciding; larger systems keep the batch's
+ * rows_per_group (fvg_batch_set_cluster).  count 0 removes the buckets.  No counterpart in
+ * the reference. */
+int fvg_batch_set_cluster_buckets(fvg_ctx* ctx, fvg_batch* batch, int32_t count, const int64_t* max_rows,
+                                  const int32_t* rows_per_group);
 /* The rows per workgroup in effect (0: one workgroup per system) and the worker pool a
  * launch uses (a system never gets more workgroups than the pool). */
 int fvg_batch_get_cluster(const fvg_batch* batch, int32_t* rows_per_group, int32_t* workers);

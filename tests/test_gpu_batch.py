@@ -140,6 +140,41 @@ def test_batch_result_is_the_same_alone_and_in_the_mix(ctx, solver, kind, prec, 
             assert np.array_equal(alone[1].history, other[1].history)
 
 
+def test_batch_size_buckets(ctx):
+    """Rows per workgroup per size bucket: each system of a bucketed mixed batch gives the
+    bits it gives alone with its bucket's rows per workgroup (a result depends on its system
+    and its workgroup count only), every answer meets the accuracy bar, the bucket autotuner
+    sets one choice per bucket, and malformed buckets are refused."""
+    systems = _systems("sym", _sym_meshes(), 70)
+    limits, rows, rest = [500, 1500], [64, 256], 4096
+
+    def rows_of(n):
+        return next((r for lim, r in zip(limits, rows) if n <= lim), rest)
+
+    B = _Batch(ctx, systems, "ic", "cg", fvg.PRECISION_MIXED, rows=rest)
+    B.batch.set_cluster_buckets(limits, rows)
+    res = B.solve()
+    for k, (rp, ci, va, b) in enumerate(systems):
+        n = rp.shape[0] - 1
+        alone = _Batch(ctx, [systems[k]], "ic", "cg", fvg.PRECISION_MIXED, rows=rows_of(n)).solve()[0]
+        assert np.array_equal(alone[0], res[k][0]), (k, n)
+        S = sp.csr_matrix((va, ci, rp), shape=(n, n))
+        assert np.linalg.norm(b - S @ res[k][0]) / np.linalg.norm(b) <= 1e-8, k
+    with pytest.raises(fvg.FvgError):
+        B.batch.set_cluster_buckets([1500, 500], [64, 256])
+    with pytest.raises(fvg.FvgError):
+        B.batch.set_cluster_buckets([500], [0])
+    times = B.batch.tune_buckets(B.b, B.x, limits=(500, 1500), candidates=(64, 256, 1024))
+    assert set(times) == {500, 1500, None} and all(len(t) == 3 for t in times.values())
+    assert B.batch.buckets[0] == [500, 1500] and len(B.batch.buckets[1]) == 2
+    assert B.batch.rows_per_group in (64, 256, 1024)
+    res = B.solve()
+    for k, (rp, ci, va, b) in enumerate(systems):
+        n = rp.shape[0] - 1
+        S = sp.csr_matrix((va, ci, rp), shape=(n, n))
+        assert np.linalg.norm(b - S @ res[k][0]) / np.linalg.norm(b) <= 1e-8, k
+
+
 @pytest.mark.parametrize("rows", [0, 64, 256, 4096])
 @pytest.mark.parametrize("solver,kind,prec", [("cg", "sym", "ic"), ("bicgstab", "nonsym", "ilu")])
 def test_batch_mixed_precision_reaches_fp64_accuracy(ctx, solver, kind, prec, rows):
