@@ -19,13 +19,26 @@ int fail(int status, const std::string& msg) {
     return status;
 }
 
+void release_work_cache(fvg_ctx* ctx) {
+    if (ctx->work_cache.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& b : ctx->work_cache) dev_free(ctx, b.first, (size_t)b.second * sizeof(double));
+    ctx->work_cache.clear();
+}
+
 int dev_alloc(fvg_ctx* ctx, size_t bytes, void** out) {
     *out = nullptr;
+    if (ctx->budget && ctx->in_use + bytes > ctx->budget) release_work_cache(ctx);
     if (ctx->budget && ctx->in_use + bytes > ctx->budget)
         return fail(FVG_E_BUDGET, "allocation of " + std::to_string(bytes) + " bytes exceeds the HBM budget (" +
                                       std::to_string(ctx->in_use) + " of " + std::to_string(ctx->budget) +
                                       " in use)");
     hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess && !ctx->work_cache.empty()) {
+        (void)hipGetLastError();
+        release_work_cache(ctx);
+        e = hipMalloc(out, bytes);
+    }
     if (e != hipSuccess) {
         *out = nullptr;
         return fail(FVG_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -265,6 +278,7 @@ int fvg_ctx_destroy(fvg_ctx* c) {
         (void)hipEventDestroy(s.b);
     }
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+    release_work_cache(c);
     dev_free_n(c, c->partials, c->partials_cap * kPartialSlots);
     dev_free_n(c, c->scalar, 64);
     if (c->pinned) (void)hipHostFree(c->pinned);

@@ -78,6 +78,9 @@ struct fvg_ctx {
     int64_t tune_assemble = 0;         // FVG_TUNE_ASSEMBLE
     int64_t tune_cg_fusion = 0;        // FVG_TUNE_CG_FUSION
     int64_t tune_sweep_gap = 0;        // FVG_TUNE_SWEEP_GAP
+    // work vectors of finished solves, kept for the next solve on this context's stream
+    // (length in doubles); released when an allocation would not fit otherwise
+    std::vector<std::pair<double*, int64_t>> work_cache;
     int batches_in_flight = 0;         // batches of this context launched and not yet finished
     bool sweep_timed_out = false;      // a sweep wait gave up here: co-resident launches stay cooperative
     int last_sweep_launch = 0;         // 1 ordinary, 2 cooperative (0: none yet)
@@ -108,6 +111,8 @@ namespace fvg {
 
 // ---- device memory accounted against the context budget ---------------------------
 int dev_alloc(fvg_ctx* ctx, size_t bytes, void** out);
+// frees the context's cached solver work vectors (after its stream has drained)
+void release_work_cache(fvg_ctx* ctx);
 void dev_free(fvg_ctx* ctx, void* p, size_t bytes);
 template <class T>
 int dev_alloc_n(fvg_ctx* ctx, int64_t n, T** out) {

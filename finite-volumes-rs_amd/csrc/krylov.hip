@@ -12,6 +12,7 @@
 // A batch is a block-diagonal system whose members start on 256-row boundaries, so
 // every 256-row workgroup belongs to exactly one member and each member's partial
 // sums (and so every result bit) are the same as when that member is solved alone.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -350,16 +351,42 @@ __global__ void __launch_bounds__(kBlock) k_bi_p_pos(int64_t n, const KState* __
 
 // ---- host drivers ----------------------------------------------------------------------
 
+// Work vectors of one solve.  They come from the context's cache of earlier solves' work
+// vectors when one is large enough (the smallest such), and go back to it afterwards: the
+// next solve on the same stream reuses them in stream order, with no allocation or free
+// (a free waits for the device).  The cache keeps at most kWorkCache vectors.
+constexpr size_t kWorkCache = 16;
 struct Workspace {
     fvg_ctx* c;
     std::vector<std::pair<double*, int64_t>> bufs;
     ~Workspace() {
-        (void)hipStreamSynchronize(c->stream);
-        for (auto& b : bufs) dev_free_n(c, b.first, b.second);
+        for (auto& b : bufs) c->work_cache.push_back(b);
+        if (c->work_cache.size() > kWorkCache) {   // keep the largest
+            (void)hipStreamSynchronize(c->stream);
+            std::sort(c->work_cache.begin(), c->work_cache.end(),
+                      [](const std::pair<double*, int64_t>& x, const std::pair<double*, int64_t>& y) {
+                          return x.second > y.second;
+                      });
+            for (size_t k = kWorkCache; k < c->work_cache.size(); ++k)
+                dev_free_n(c, c->work_cache[k].first, c->work_cache[k].second);
+            c->work_cache.resize(kWorkCache);
+        }
     }
     int get(int64_t n, double** out) {
-        FVG_TRY(dev_alloc_n(c, n, out));
-        bufs.emplace_back(*out, n);
+        size_t best = c->work_cache.size();
+        for (size_t k = 0; k < c->work_cache.size(); ++k)
+            if (c->work_cache[k].second >= n &&
+                (best == c->work_cache.size() || c->work_cache[k].second < c->work_cache[best].second))
+                best = k;
+        if (best < c->work_cache.size()) {
+            bufs.push_back(c->work_cache[best]);
+            c->work_cache.erase(c->work_cache.begin() + (ptrdiff_t)best);
+        } else {
+            double* p = nullptr;
+            FVG_TRY(dev_alloc_n(c, n, &p));
+            bufs.emplace_back(p, n);
+        }
+        *out = bufs.back().first;
         return FVG_OK;
     }
 };
@@ -505,8 +532,9 @@ static int solve_buffers(fvg_ctx* c, int64_t n, int count, const KBuffers* kb, W
     return FVG_OK;
 }
 
-// CG in the lower sweep's position order (PosSolve): 9 kernels per iteration, none of
-// which reorders a vector.
+// CG in the lower sweep's position order (PosSolve): none of its kernels reorders a
+// vector; with the sweeps fused, an iteration is five kernels (SpMV, alpha, the two
+// sweeps, the direction update).
 static int run_cg_pos(fvg_ctx* c, const fvg_prec* p, const double* b, double* x, const Layout& L, KRun& K,
                       const KBuffers* kb, const PosSolve& P) {
     const int64_t n = P.n;
