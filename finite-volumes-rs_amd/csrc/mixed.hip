@@ -8,9 +8,11 @@
 //                      FP64, pivots included, then rounded), FP32 vectors and dots
 //                      x += d              (FP64 update)
 //
-// The inner solve runs until its residual has dropped by `inner_tolerance` (default
-// 1e-4) or the total inner iteration budget (max_iterations) is spent.  Everything
-// scalar lives on the device, as in the FP64 solvers.
+// An inner solve runs until its residual has dropped by the factor still needed to meet
+// the caller's tolerance (with a margin of 0.3), by at most 0.1 and at least
+// `inner_tolerance` (default 1e-3 for CG, 1e-4 for BiCGStab), or until the total inner
+// iteration budget (max_iterations) is spent.  Everything scalar lives on the device, as
+// in the FP64 solvers.
 #include <cmath>
 #include <cstring>
 
@@ -29,6 +31,7 @@ constexpr int kGraphIters = 4; // inner iterations between host checks (one capt
 
 struct MState {
     double bnorm, init_res, true_res, atol, rtol, inner_tol;
+    double inner_red;    // reduction asked of the current inner solve
     double rin0, rin, rz, alpha, beta, rho, omega;
     int64_t inner_iter, total_inner, max_inner, outer, restarts;
     int32_t done;        // outer loop finished
@@ -141,6 +144,10 @@ __global__ void __launch_bounds__(kScalarBlock) k_mscalar(MState* __restrict__ s
         if ((e < st->atol) || (e / st->init_res < st->rtol) || st->total_inner >= st->max_inner) st->done = 1;
         st->inner_done = st->done;
         st->inner_iter = 0;
+        // the remaining factor to the caller's tolerance with a margin of 0.3, at most 0.1
+        // and at least inner_tol (an inner solve asked for more only overshoots the goal)
+        const double goal = fmax(st->rtol * st->init_res, st->atol);
+        st->inner_red = fmax(fmin(0.3 * goal / e, 0.1), st->inner_tol);
         break;
     }
     case M_IN_INIT:
@@ -158,7 +165,7 @@ __global__ void __launch_bounds__(kScalarBlock) k_mscalar(MState* __restrict__ s
         st->rin = sqrt(s0);
         st->inner_iter += 1;
         st->total_inner += 1;
-        if (st->rin <= st->inner_tol * st->rin0 || st->total_inner >= st->max_inner) st->inner_done = 1;
+        if (st->rin <= st->inner_red * st->rin0 || st->total_inner >= st->max_inner) st->inner_done = 1;
         break;
     case M_CG_BETA:
         st->beta = s0 / st->rz;
@@ -170,7 +177,7 @@ __global__ void __launch_bounds__(kScalarBlock) k_mscalar(MState* __restrict__ s
         break;
     case M_BI_S: {
         const double ss = sqrt(s0);
-        if (ss <= st->inner_tol * st->rin0) {
+        if (ss <= st->inner_red * st->rin0) {
             st->rin = ss;
             st->inner_iter += 1;
             st->total_inner += 1;
@@ -190,7 +197,7 @@ __global__ void __launch_bounds__(kScalarBlock) k_mscalar(MState* __restrict__ s
         st->rin = sqrt(s1);
         st->inner_iter += 1;
         st->total_inner += 1;
-        if (st->rin <= st->inner_tol * st->rin0 || st->total_inner >= st->max_inner) {
+        if (st->rin <= st->inner_red * st->rin0 || st->total_inner >= st->max_inner) {
             st->inner_done = 1;
             break;
         }
@@ -444,7 +451,7 @@ int run_mixed(fvg_ctx* c, fvg_csr* a, fvg_prec* p, const fvg_vec* b, fvg_vec* x,
     h.bnorm = bnorm;
     h.atol = o->absolute_tolerance;
     h.rtol = o->relative_tolerance;
-    h.inner_tol = o->inner_tolerance > 0.0 ? o->inner_tolerance : 1e-4;
+    h.inner_tol = o->inner_tolerance > 0.0 ? o->inner_tolerance : (bicgstab ? 1e-4 : 1e-3);
     h.max_inner = o->max_iterations > 0 ? o->max_iterations : 1;
     FVG_HIP(hipMemcpyAsync(st, &h, sizeof(MState), hipMemcpyHostToDevice, c->stream));
     const int32_t* inner_skip = &st->inner_done;

@@ -456,7 +456,10 @@ typedef struct fvg_solver_options {
                                    FVG_PRECISION_MIXED: FP32 inner preconditioned iterations
                                    inside an FP64 iterative-refinement loop. */
     int32_t reserved;
-    double inner_tolerance;     /* mixed only: relative reduction per inner solve (0 = default 1e-4) */
+    double inner_tolerance;     /* mixed only: the deepest relative reduction asked of one inner
+                                   solve; an inner solve asks for less once the caller's
+                                   tolerance is within reach (0 = default: 1e-3 for the
+                                   single-system CG, 1e-4 for its BiCGStab and for batches) */
 } fvg_solver_options;
 
 /* LinearSolverInfo (solvers/mod.rs:11-17) plus counters. */
