@@ -71,6 +71,14 @@ def parse():
     return ap.parse_args()
 
 
+def layout_summary(prec):
+    """The blocked sweep layout of a preconditioner: workgroups, levels (the dependency
+    levels, and those of the blocked layout) and the widest level of one workgroup."""
+    lay = prec.sweep_layout()
+    return {"workgroups": lay["lower_blocks"], "dependency_levels": lay["levels"],
+            "block_levels": lay["block_levels"], "widest_rows": lay["lower_rows"]}
+
+
 def leg_batched(args, fvg, workloads):
     """256 independent 2D diffusion systems (cells log-uniform in [1k, 64k]) solved by one
     graph-launched batch: FP64 (one workgroup per system, workgroup size autotuned) and
@@ -353,7 +361,7 @@ def leg_convdiff(args, fvg, workloads):
                 "ilu0_sweep": {"span": "one triangular solve (lower or upper), HIP events around one launch group "
                                        f"in {TIMING_EVERY}", "avg_ms": sw_ms, "algorithmic_bytes": sw_b,
                                "GB/s": sw_b / (sw_ms * 1e-3) / 1e9, "frac": sw_b / (sw_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                               "levels": list(ilu.levels())}}
+                               "levels": list(ilu.levels()), "layout": layout_summary(ilu)}}
     ctx.set_tuning(fvg.TUNE_INNER_EXACT, 0)
     for name in ("mixed", "mixed_exact_sweeps"):
         out[name]["speedup_vs_fp64"] = out[name]["solves_per_s"] / out["fp64"]["solves_per_s"]
@@ -438,7 +446,7 @@ def leg_transient(args, fvg, workloads):
     out = {"workload": "ex4 step on convdiff_prism: 2,097,152 prisms, LimitedLinear(1) limiters (Upwind, "
                        "Corrected), time(Euler, dt=0.01) + convection(LimitedLinear) - laplacian(Corrected) "
                        "- source (unit source in a ball of radius 0.1), ILU(1) + BiCGStab to 1e-8",
-           "ilu_level": 1, "factor_nnz": int(ilu.factor_pattern()[0][-1]),
+           "ilu_level": 1, "factor_nnz": int(ilu.factor_pattern()[0][-1]), "layout": layout_summary(ilu),
            "n_cells": n, "n_faces": F, "step_ms": step_ms, "steps_per_s": 1e3 / step_ms,
            "iterations": info.iterations,
            "gradients": {"avg_ms": grad_ms, "algorithmic_bytes": grad_bytes,
@@ -733,7 +741,7 @@ def main():
                                                        else ""),
                           "avg_ms": sweep_avg_ms, "algorithmic_bytes": sweep_alg, "GB/s": sweep_gbs,
                           "frac": sweep_gbs / HBM_PEAK_GBS, "calls": ks["sweep_calls"], "launches": ks["sweep_launches"],
-                          "total_ms": ks["sweep_ms"], "levels": [lo_lv, up_lv]},
+                          "total_ms": ks["sweep_ms"], "levels": [lo_lv, up_lv], "layout": layout_summary(ic)},
         },
     }
     # each leg below runs as the only live context on the device, as the headline did (the

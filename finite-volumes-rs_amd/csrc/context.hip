@@ -370,6 +370,10 @@ int fvg_ctx_set_tuning(fvg_ctx* c, int32_t knob, int64_t value) {
         if (value == 1 || value > 8) return fail(FVG_E_INVALID, "sweep gap must be 0 or 2 to 8");
         c->tune_sweep_gap = value;
         return FVG_OK;
+    case FVG_TUNE_SWEEP_BOXES:
+        if (value > 1024) return fail(FVG_E_INVALID, "sweep boxes must be 0, 1 or 2 to 1024");
+        c->tune_sweep_boxes = value;
+        return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }
@@ -391,6 +395,7 @@ int fvg_ctx_get_tuning(const fvg_ctx* c, int32_t knob, int64_t* value) {
     case FVG_TUNE_ASSEMBLE: *value = c->tune_assemble; return FVG_OK;
     case FVG_TUNE_CG_FUSION: *value = c->tune_cg_fusion; return FVG_OK;
     case FVG_TUNE_SWEEP_GAP: *value = c->tune_sweep_gap; return FVG_OK;
+    case FVG_TUNE_SWEEP_BOXES: *value = c->tune_sweep_boxes; return FVG_OK;
     }
     return fail(FVG_E_INVALID, "unknown tuning knob");
 }

@@ -263,6 +263,16 @@ int fvg_csr_upload_values(fvg_ctx* c, fvg_csr* a, const double* values) {
     return FVG_OK;
 }
 
+int fvg_csr_set_row_coords(fvg_csr* a, int32_t dim, const double* coords) {
+    if (!a || dim < 0 || dim > 3 || (dim > 0 && !coords && a->nrows > 0)) return fail(FVG_E_INVALID, "bad argument");
+    const size_t count = (size_t)a->nrows * (size_t)dim;
+    a->h_coords.assign(count, 0.f);
+    for (size_t k = 0; k < count; ++k) a->h_coords[k] = (float)coords[k];
+    a->coord_dim = dim;
+    if (dim == 0) a->h_coords.shrink_to_fit();
+    return FVG_OK;
+}
+
 int fvg_spmv(fvg_ctx* c, const fvg_csr* a, const fvg_vec* x, fvg_vec* y) {
     if (!c || !a || !x || !y) return fail(FVG_E_INVALID, "null argument");
     if (x->n < a->ncols || y->n != a->nrows) return fail(FVG_E_INVALID, "vector lengths do not match the matrix");

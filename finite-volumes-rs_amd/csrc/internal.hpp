@@ -78,6 +78,7 @@ struct fvg_ctx {
     int64_t tune_assemble = 0;         // FVG_TUNE_ASSEMBLE
     int64_t tune_cg_fusion = 0;        // FVG_TUNE_CG_FUSION
     int64_t tune_sweep_gap = 0;        // FVG_TUNE_SWEEP_GAP
+    int64_t tune_sweep_boxes = 0;      // FVG_TUNE_SWEEP_BOXES
     // work vectors of finished solves, kept for the next solve on this context's stream
     // (length in doubles); released when an allocation would not fit otherwise
     std::vector<std::pair<double*, int64_t>> work_cache;
@@ -105,6 +106,10 @@ struct fvg_csr {
     uint64_t version = 0;              // bumped whenever the values change
     float* val32 = nullptr;            // FP32 copy of the values for mixed precision (made on use)
     uint64_t val32_version = ~0ull;
+    // coordinates of the rows (cell centres; coord_dim per row, 0: none), used by the
+    // preconditioner analysis to shape its blocked layouts
+    std::vector<float> h_coords;
+    int32_t coord_dim = 0;
 };
 
 namespace fvg {

@@ -895,7 +895,22 @@ int fvg_csr_from_mesh(fvg_ctx* c, const fvg_mesh* m, fvg_csr** out) {
     // every row holds its diagonal, so max_minor + 1 (dmatrix.rs:47) is the cell count
     FVG_TRY(csr_from_device_pattern(c, m->n_cells, m->n_cells, m->nnz, m->d_row_ptr, m->d_col, out));
     (*out)->pattern_id = m->pattern_id;
-    return FVG_OK;
+    // the cell centres as row coordinates, for the preconditioner layout analysis
+    std::vector<double> centre((size_t)m->n_cells * (size_t)m->dim);
+    auto fetch = [&]() -> int {
+        if (centre.empty()) return FVG_OK;
+        FVG_HIP(hipMemcpyAsync(centre.data(), m->d_cell_center, centre.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+        FVG_HIP(hipStreamSynchronize(c->stream));
+        return FVG_OK;
+    };
+    int s = fetch();
+    if (s == FVG_OK) s = fvg_csr_set_row_coords(*out, m->dim, centre.data());
+    if (s != FVG_OK) {
+        (void)fvg_csr_destroy(c, *out);
+        *out = nullptr;
+    }
+    return s;
 }
 
 }  // extern "C"

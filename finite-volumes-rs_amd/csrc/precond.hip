@@ -1171,20 +1171,106 @@ static int delayed_levels(int64_t n, const std::vector<int32_t>& rp, const std::
     return maxl + 1;
 }
 
+// Boxes of a coordinate grid: the bounding box of the rows' coordinates cut into equal
+// slabs along each axis, `count` boxes (a power of two) in the most nearly cubic grid for
+// the extents, boxes ordered z-major.  A row's slab along each axis is then raised to the
+// highest slab among its dependencies (lower sweep, rows ascending; the lowest, rows
+// descending, for an upper sweep), so that along every chain of dependencies the slab
+// indices never go back and a chain crosses each cut at most once: a chain then meets at
+// most (s0 - 1) + (s1 - 1) + (s2 - 1) delays, however long it is.  Returns the box count.
+static int32_t coordinate_boxes(const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
+                                bool upper, const RowCoords& X, int count, std::vector<int32_t>& blk) {
+    auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
+    const int dim = std::min(X.dim, 3);
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int a = 0; a < dim; ++a) {
+        lo[a] = hi[a] = X.v[a];
+        for (int64_t i = 0; i < n; ++i) {
+            lo[a] = std::min(lo[a], (double)X.v[(size_t)i * X.dim + a]);
+            hi[a] = std::max(hi[a], (double)X.v[(size_t)i * X.dim + a]);
+        }
+    }
+    // the grid: s[a] = 2^e[a], sum e = log2(count); least box aspect ratio, ties to more
+    // cuts along the later axes
+    int k = 0;
+    while ((2 << k) <= count) ++k;
+    int s[3] = {1, 1, 1};
+    double best = 1e300;
+    for (int e0 = 0; e0 <= k; ++e0)
+        for (int e1 = 0; e0 + e1 <= k; ++e1) {
+            const int e[3] = {e0, e1, k - e0 - e1};
+            double mx = 0.0, mn = 1e300;
+            bool ok = true;
+            for (int a = 0; a < 3; ++a) {
+                const double ext = a < dim ? hi[a] - lo[a] : 0.0;
+                if (!(ext > 0.0)) {
+                    ok = ok && e[a] == 0;
+                    continue;
+                }
+                const double side = ext / (double)(1 << e[a]);
+                mx = std::max(mx, side);
+                mn = std::min(mn, side);
+            }
+            if (!ok || mn > mx) continue;
+            const double aspect = mx / mn;
+            if (aspect < best) {
+                best = aspect;
+                for (int a = 0; a < 3; ++a) s[a] = 1 << e[a];
+            }
+        }
+    std::vector<int32_t> slab[3];
+    for (int a = 0; a < 3; ++a) {
+        slab[a].assign((size_t)n, 0);
+        if (s[a] == 1) continue;
+        const double scale = (double)s[a] / (hi[a] - lo[a]);
+        for (int64_t i = 0; i < n; ++i) {
+            const int v = (int)(((double)X.v[(size_t)i * X.dim + a] - lo[a]) * scale);
+            slab[a][(size_t)i] = std::min(std::max(v, 0), s[a] - 1);
+        }
+        for (int64_t t = 0; t < n; ++t) {
+            const int64_t i = upper ? n - 1 - t : t;
+            int32_t v = slab[a][(size_t)i];
+            for (int32_t q = rp[(size_t)i]; q < rp[(size_t)i + 1]; ++q)
+                if (take(i, col[(size_t)q]))
+                    v = upper ? std::min(v, slab[a][(size_t)col[(size_t)q]]) : std::max(v, slab[a][(size_t)col[(size_t)q]]);
+            slab[a][(size_t)i] = v;
+        }
+    }
+    blk.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+        blk[(size_t)i] = (slab[2][(size_t)i] * s[1] + slab[1][(size_t)i]) * s[0] + slab[0][(size_t)i];
+    return s[0] * s[1] * s[2];
+}
+
+// The widest block-level (rows) of an assignment and how many block-levels hold rows.
+static void block_level_stats(int64_t n, const std::vector<int32_t>& blk, int32_t nb, const std::vector<int32_t>& lev,
+                              int nlevels, int32_t& widest, int64_t& busy) {
+    std::vector<int32_t> cnt((size_t)nb * (size_t)nlevels, 0);
+    widest = 0;
+    busy = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t v = ++cnt[(size_t)blk[(size_t)i] * (size_t)nlevels + (size_t)lev[(size_t)i]];
+        widest = std::max(widest, v);
+        busy += v == 1;
+    }
+}
+
 // The block and level assignment of a blocked sweep layout (see k_sweep_blocked).  Rows
 // joined by a dependency one level apart are kept in one block (union-find over those
 // edges) and the components are packed into at most `target` blocks (pack_components), or,
 // with `gap` > 2, grown into compact blocks of about n / target rows (grow_blocks);
 // when one component holds more than an eighth of the rows, the blocks are instead
-// target_consecutive ranges of consecutive rows.  Levels are then delayed (delayed_levels)
-// so that every dependency between blocks spans at least `gap` levels (gap >= 2): with
-// three or more, a block's loader wave has a whole level to fetch each value from outside
-// (the components alone give two).  False (no blocked layout) when consecutive blocks
-// would cost more than half again the levels or keep fewer than 8 blocks busy per level,
-// or when a row has more than kBlockMaxDeps dependencies: the persistent sweep then serves.
+// target_consecutive ranges of consecutive rows, or boxes of the rows' coordinates
+// (coordinate_boxes) where X has them and they cost less.  Levels are then delayed
+// (delayed_levels) so that every dependency between blocks spans at least `gap` levels
+// (gap >= 2): with three or more, a block's loader wave has a whole level to fetch each
+// value from outside (the components alone give two).  False (no blocked layout) when
+// neither the consecutive blocks nor the boxes keep within half again the levels, 8
+// blocks busy per level and kBlockMaxRows rows per block-level, or when a row has more
+// than kBlockMaxDeps dependencies: the persistent sweep then serves.
 static bool assign_blocked(const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n, bool upper,
                            const std::vector<int32_t>& lev_in, int nlevels_in, int target, int target_consecutive,
-                           int gap, BlockAssign& A) {
+                           int gap, const RowCoords& X, const std::vector<int>& box_counts, BlockAssign& A) {
     auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
     if (n < 4096 || target < 2) return false;
     std::vector<int32_t> par((size_t)n);
@@ -1246,15 +1332,43 @@ static bool assign_blocked(const std::vector<int32_t>& rp, const std::vector<int
             comp_of[(size_t)i] = (int32_t)(i * nbc / n);
             comp_size[(size_t)comp_of[(size_t)i]]++;
         }
+        const int gap_asked = gap;
         nlevels = delayed_levels(n, rp, col, upper, comp_of, gap, dlev);
         if (gap > 2 && (int64_t)nlevels * 2 > (int64_t)nlevels_in * 3)   // the least delay instead
             nlevels = delayed_levels(n, rp, col, upper, comp_of, gap = 2, dlev);
-        if ((int64_t)nlevels * 2 > (int64_t)nlevels_in * 3) return false;
-        std::vector<int64_t> pairs((size_t)n);
-        for (int64_t i = 0; i < n; ++i) pairs[(size_t)i] = (int64_t)comp_of[(size_t)i] * nlevels + dlev[(size_t)i];
-        std::sort(pairs.begin(), pairs.end());
-        const int64_t busy = (int64_t)(std::unique(pairs.begin(), pairs.end()) - pairs.begin());
-        if (busy < 8 * (int64_t)nlevels) return false;
+        // A layout is usable when it costs at most half again the levels, keeps 8 blocks
+        // busy per level on average and no block-level exceeds kBlockMaxRows rows; a level
+        // costs about 19 + (waves of the widest block-level) units (on 2,097,152 prisms an
+        // ILU(0) apply took 603.7 us with 512 blocks of consecutive rows, 459 levels of at
+        // most 128 rows; 500.8 us with 64 boxes, 337 levels of 528 rows; 439.4 us with 128
+        // boxes, 345 levels of 272 rows).
+        auto cost = [&](const std::vector<int32_t>& blk, int32_t nbk, const std::vector<int32_t>& lv, int L) -> int64_t {
+            if ((int64_t)L * 2 > (int64_t)nlevels_in * 3) return INT64_MAX;
+            int32_t w = 0;
+            int64_t busy = 0;
+            block_level_stats(n, blk, nbk, lv, L, w, busy);
+            if (busy < 8 * (int64_t)L || w > kBlockMaxRows) return INT64_MAX;
+            return (int64_t)L * (19 + (w + 63) / 64);
+        };
+        int64_t best = cost(comp_of, nbc, dlev, nlevels);
+        // boxes of the rows' coordinates, where the matrix has them: kept when they cost less
+        if (X.v && X.dim > 0) {
+            std::vector<int32_t> bblk, blev;
+            for (int count : box_counts) {
+                const int32_t nbx = coordinate_boxes(rp, col, n, upper, X, count, bblk);
+                if (nbx < 2) continue;
+                const int L = delayed_levels(n, rp, col, upper, bblk, gap_asked, blev);
+                const int64_t k = cost(bblk, nbx, blev, L);
+                if (k >= best) continue;
+                best = k;
+                nlevels = L;
+                comp_of.swap(bblk);
+                dlev.swap(blev);
+                comp_size.assign((size_t)nbx, 0);
+                for (int64_t i = 0; i < n; ++i) comp_size[(size_t)comp_of[(size_t)i]]++;
+            }
+        }
+        if (best == INT64_MAX) return false;
     }
     // blocks: the consecutive row ranges as they are, or the components packed
     const size_t C = comp_size.size();
@@ -1433,9 +1547,11 @@ static int materialize_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const
 // The assignment is kept (sw.h_assign) so that a mirrored layout can be made from it.
 static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
                          bool upper, const std::vector<int32_t>* src_map, const std::vector<int32_t>& lev_in,
-                         int nlevels_in, int target, int target_consecutive, int gap, Sweep& sw) {
+                         int nlevels_in, int target, int target_consecutive, int gap, const RowCoords& X,
+                         const std::vector<int>& box_counts, Sweep& sw) {
     BlockAssign A;
-    if (!assign_blocked(rp, col, n, upper, lev_in, nlevels_in, target, target_consecutive, gap, A)) return FVG_OK;
+    if (!assign_blocked(rp, col, n, upper, lev_in, nlevels_in, target, target_consecutive, gap, X, box_counts, A))
+        return FVG_OK;
     FVG_TRY(materialize_blocked(c, rp, col, n, upper, src_map, A, false, sw));
     if (sw.nblk > 0) sw.h_assign = std::move(A);
     return FVG_OK;
@@ -1487,7 +1603,8 @@ static int build_mirrored(fvg_ctx* c, const std::vector<int32_t>& rp, const std:
 // whose dependencies the caller has checked are these transposed) takes the mirrored
 // layout where it can.
 static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
-                       bool upper, const std::vector<int32_t>* src_map, Sweep& sw, const Sweep* mirror = nullptr) {
+                       bool upper, const std::vector<int32_t>* src_map, Sweep& sw, const Sweep* mirror = nullptr,
+                       const RowCoords& X = RowCoords()) {
     std::vector<int32_t> lev((size_t)n, 0);
     int maxlev = 0;
     auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
@@ -1561,7 +1678,15 @@ static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::ve
         FVG_TRY(build_mirrored(c, rp, col, n, src_map, *mirror, sw));
         if (sw.nblk > 0) return FVG_OK;
     }
-    return build_blocked(c, rp, col, n, upper, src_map, lev, sw.nlevels, packed, consecutive, gap, sw);
+    // boxes of the rows' coordinates: a grid of about CUs/2 boxes (a power of two; on
+    // 2,097,152 prisms 64 boxes gave 337 levels of up to 528 rows, 128 boxes 345 of 272,
+    // 256 boxes 353 of 272)
+    std::vector<int> box_counts;
+    if (c->tune_sweep_boxes >= 2)
+        box_counts.push_back((int)c->tune_sweep_boxes);
+    else if (c->tune_sweep_boxes == 0)
+        box_counts.push_back(std::max(2, c->num_cus / 2));
+    return build_blocked(c, rp, col, n, upper, src_map, lev, sw.nlevels, packed, consecutive, gap, X, box_counts, sw);
 }
 
 // When both layouts of a factor are blocked, the upper set-up reads the lower sweep's
@@ -2443,17 +2568,22 @@ static int create_factored(fvg_ctx* c, const fvg_csr* a, int32_t level, int kind
     // factor entry
     if (s == FVG_OK) s = dev_alloc_n(c, p->nnz + 1, &p->d_fac);
     if (s == FVG_OK) s = dev_alloc_n(c, n, &p->d_tmp);
-    if (s == FVG_OK) s = build_sweep(c, p->h_rp, p->h_col, n, false, nullptr, p->lower);
+    RowCoords X;
+    if (a->coord_dim > 0 && (int64_t)a->h_coords.size() == n * a->coord_dim) {
+        X.v = a->h_coords.data();
+        X.dim = a->coord_dim;
+    }
+    if (s == FVG_OK) s = build_sweep(c, p->h_rp, p->h_col, n, false, nullptr, p->lower, nullptr, X);
     if (s == FVG_OK) {
         if (kind == kIC && level > 0) {
             for (int32_t& v : F.usrc)
                 if (v < 0) v = (int32_t)p->nnz;
-            s = build_sweep(c, F.urp, F.ucol, n, true, &F.usrc, p->upper, F.transposed ? &p->lower : nullptr);
+            s = build_sweep(c, F.urp, F.ucol, n, true, &F.usrc, p->upper, F.transposed ? &p->lower : nullptr, X);
         } else {
             // IC(0): the upper entries are the lower ones transposed; ILU: when the pattern is symmetric
             const bool tr = kind == kIC || structurally_symmetric(p->h_rp, p->h_col, n);
             s = build_sweep(c, p->h_rp, p->h_col, n, true, kind == kIC ? &p->h_tpos : nullptr, p->upper,
-                            tr ? &p->lower : nullptr);
+                            tr ? &p->lower : nullptr, X);
         }
     }
     if (s == FVG_OK) s = pair_layouts(c, p->lower, p->upper);

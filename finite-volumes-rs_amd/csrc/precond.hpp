@@ -13,6 +13,12 @@ namespace fvg {
 // level is a contiguous range of chunks and its rows' entries are contiguous in HBM.
 constexpr int kChunk = 64;              // rows per wave64 chunk
 
+// Coordinates of the rows (cell centres, `dim` floats per row) for the layout analysis.
+struct RowCoords {
+    const float* v = nullptr;
+    int dim = 0;
+};
+
 // Block and level of every row of a blocked sweep layout (host).
 struct BlockAssign {
     int32_t nb = 0;                     // blocks

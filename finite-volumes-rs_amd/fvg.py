@@ -39,7 +39,7 @@ PRECISION_FP64, PRECISION_MIXED = 0, 1
 INFO_FP64_FALLBACK, INFO_POSITION_ORDER, INFO_FUSED_SWEEPS = 1, 2, 4
 (TUNE_SWEEP_MODE, TUNE_SWEEP_WAVES, TUNE_SWEEP_POLL, TUNE_INNER_EXACT, TUNE_INNER_SWEEPS, TUNE_GRAPHS,
  TUNE_SWEEP_BLOCKS, TUNE_SPIN_LIMIT, TUNE_SWEEP_LAUNCH, TUNE_SWEEP_WITHHOLD, TUNE_SOLVE_ORDER, TUNE_ASSEMBLE,
- TUNE_CG_FUSION, TUNE_SWEEP_GAP) = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14)
+ TUNE_CG_FUSION, TUNE_SWEEP_GAP, TUNE_SWEEP_BOXES) = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15)
 
 
 class FvgError(RuntimeError):
@@ -164,6 +164,7 @@ def lib():
         "fvg_csr_sizes": [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
         "fvg_csr_download": [P, P, P, P, P],
         "fvg_csr_upload_values": [P, P, P],
+        "fvg_csr_set_row_coords": [P, C.c_int32, P],
         "fvg_spmv": [P, P, P, P],
         "fvg_spmv_time": [P, P, P, P, C.c_int32, C.POINTER(C.c_double)],
         "fvg_assemble": [P, P, P, C.c_int32, P, C.c_int32, P, C.c_int32, P, P, P, P],
@@ -616,6 +617,20 @@ class DistributedMatrix:
     def set_values(self, values):
         va = np.ascontiguousarray(values, dtype=np.float64)
         check(lib().fvg_csr_upload_values(self.ctx.h, self.h, _ptr(va)))
+
+    def set_row_coordinates(self, coords):
+        """Coordinates of the rows ((nrows, dim) with dim 1 to 3, e.g. cell centres; None
+        drops them) for the layout analysis of preconditioners created afterwards.  A
+        matrix made by from_mesh carries its mesh's cell centres already."""
+        if coords is None:
+            check(lib().fvg_csr_set_row_coords(self.h, 0, None))
+            return
+        xy = np.ascontiguousarray(coords, dtype=np.float64)
+        if xy.ndim == 1:
+            xy = xy[:, None]
+        if xy.shape[0] != self.nrows:
+            raise ValueError("one coordinate row per matrix row")
+        check(lib().fvg_csr_set_row_coords(self.h, xy.shape[1], _ptr(xy)))
 
     def imul(self, result, rhs):
         """result = A rhs (dmatrix.rs:136-146)."""

@@ -118,7 +118,14 @@ enum {
      * 0 (default) 3, so a workgroup fetches each such value a whole level before it needs
      * it; 2 to 8 that many (2: no delay beyond what the dependency components give).  More
      * levels for fewer waits on the fetches.  The same values every way. */
-    FVG_TUNE_SWEEP_GAP = 14
+    FVG_TUNE_SWEEP_GAP = 14,
+    /* Blocked layouts of matrices whose rows carry coordinates (fvg_csr_from_mesh,
+     * fvg_csr_set_row_coords) where the dependencies form one large component (extruded
+     * and 2D meshes), for preconditioners created afterwards: 0 (default) boxes of the
+     * coordinates in a grid of about CUs/2 boxes, kept where their levels (and widths)
+     * cost less than those of blocks of consecutive rows; 1 never; 2 to 1024 a grid of that
+     * many boxes (rounded down to a power of two).  The same values every way. */
+    FVG_TUNE_SWEEP_BOXES = 15
 };
 int fvg_ctx_set_tuning(fvg_ctx* ctx, int32_t knob, int64_t value);
 int fvg_ctx_get_tuning(const fvg_ctx* ctx, int32_t knob, int64_t* value);
@@ -237,7 +244,8 @@ int fvg_pvtu_writer_destroy(fvg_pvtu_writer* writer);
 typedef struct fvg_csr fvg_csr;
 
 /* DistributedMatrix::from_cut_sparsity(mesh.cell_to_cell_sparsity(), n_cells)
- * (dmatrix.rs:38-50): sorted pattern, zero values. */
+ * (dmatrix.rs:38-50): sorted pattern, zero values.  The matrix keeps a host copy of the
+ * cell centres as its row coordinates (see fvg_csr_set_row_coords). */
 int fvg_csr_from_mesh(fvg_ctx* ctx, const fvg_mesh* mesh, fvg_csr** out);
 /* General CSR (columns sorted per row).  values may be NULL (zeros). */
 int fvg_csr_create(fvg_ctx* ctx, int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* row_ptr,
@@ -246,6 +254,13 @@ int fvg_csr_destroy(fvg_ctx* ctx, fvg_csr* a);
 int fvg_csr_sizes(const fvg_csr* a, int64_t* nrows, int64_t* ncols, int64_t* nnz);
 int fvg_csr_download(fvg_ctx* ctx, const fvg_csr* a, int32_t* row_ptr, int32_t* col_idx, double* values);
 int fvg_csr_upload_values(fvg_ctx* ctx, fvg_csr* a, const double* values);
+/* Coordinates of the rows (no counterpart in the reference): `dim` (1 to 3) values per
+ * row, row-major, e.g. the cell centres of the mesh the matrix was assembled on; dim 0
+ * (coords NULL) drops them.  Only the layout analysis of preconditioners created
+ * afterwards reads them (FVG_TUNE_SWEEP_BOXES): where the sweep dependencies form one
+ * large component, boxes of the coordinates can give a blocked sweep fewer levels.
+ * Solutions are the same bits with or without them. */
+int fvg_csr_set_row_coords(fvg_csr* a, int32_t dim, const double* coords);
 /* DistributedMatrix::imul (dmatrix.rs:136-146): y = A x, each row summed in
  * ascending column order, bit-identical to the sequential loop. */
 int fvg_spmv(fvg_ctx* ctx, const fvg_csr* a, const fvg_vec* x, fvg_vec* y);

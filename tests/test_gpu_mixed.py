@@ -147,6 +147,85 @@ def test_blocked_sweep_gaps_give_identical_bits(ctx, name, kind, prec):
         assert it == solves[0][1] and np.array_equal(xg, solves[0][2]), gap
 
 
+@pytest.mark.parametrize("name,kind,prec", [("prism24", "nonsym", "ilu"), ("prism24", "sym", "ic"),
+                                           ("tri64", "sym", "ic")])
+def test_coordinate_boxes_give_identical_bits(ctx, name, kind, prec):
+    """Rows that carry their cell centres may take a blocked layout of boxes of the
+    coordinates (one dependency component: extruded and 2D meshes).  Without boxes, with
+    the default choice and with grids of 16 and 64 boxes the preconditioner applies give the
+    oracle's bits and the FP64 solves the same iterations and solution bits.  On the prism
+    mesh a forced grid of 64 boxes is built as 64 blocks, and boxes off gives the layout
+    of a matrix without coordinates."""
+    from test_gpu_parity import _mk
+    rp, ci, va, b = _oracle_system(name, kind)
+    n = rp.shape[0] - 1
+    centre = _mk(name)[2]["cell_center"]
+    A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+    B = fvg.DistributedVector.from_data(ctx, b)
+    r = np.random.default_rng(7).normal(size=n)
+    ref = O.OraclePrec(prec, rp, ci, va, 0).apply(r)
+    cls = fvg.IncompleteCholesky if prec == "ic" else fvg.IncompleteLowerUpper
+    fn = fvg.conjugate_gradient if prec == "ic" else fvg.bi_conjugate_gradient_stab
+    opts = fvg.LinearSolverOptions(1e-8, 0.0, 2000)
+    for bad in (-1, 1025):
+        with pytest.raises(fvg.FvgError):
+            ctx.set_tuning(fvg.TUNE_SWEEP_BOXES, bad)
+    with pytest.raises(ValueError):
+        A.set_row_coordinates(centre[:-1])
+    plain = cls.from_matrix(A, 0).sweep_layout()
+    A.set_row_coordinates(centre)
+    solves, layouts = [], {}
+    try:
+        for boxes in (1, 0, 16, 64):
+            ctx.set_tuning(fvg.TUNE_SWEEP_BOXES, boxes)
+            P = cls.from_matrix(A, 0)
+            layouts[boxes] = P.sweep_layout()
+            z = fvg.DistributedVector(ctx, n)
+            P.precondition(z, fvg.DistributedVector.from_data(ctx, r))
+            assert np.array_equal(z.numpy(), ref), boxes
+            x = fvg.DistributedVector(ctx, n)
+            info = fn(x, A, B, P, opts)
+            solves.append((boxes, info.iterations, x.numpy()))
+            P.close()
+    finally:
+        ctx.set_tuning(fvg.TUNE_SWEEP_BOXES, 0)
+    for boxes, it, xg in solves[1:]:
+        assert it == solves[0][1] and np.array_equal(xg, solves[0][2]), boxes
+    assert layouts[1] == plain
+    if name == "prism24":
+        assert layouts[64]["lower_blocks"] == 64 and layouts[64]["upper_blocks"] == 64, layouts[64]
+        assert layouts[64]["block_levels"] < plain["block_levels"], (layouts[64], plain)
+    A.set_row_coordinates(None)
+    assert cls.from_matrix(A, 0).sweep_layout() == plain
+
+
+def test_mesh_matrices_carry_cell_centres(ctx):
+    """A matrix made from a device mesh carries the cell centres: on the 13,824-prism mesh
+    its ILU(0) takes a layout of 64 coordinate boxes when asked for that grid (fewer levels
+    than the same matrix made from CSR arrays) and applies with the oracle's bits."""
+    from test_gpu_parity import _mk
+    import problems
+    raw, om, geom = _mk("prism24")
+    prob = problems.convection_diffusion(raw, geom, seed=3)
+    mesh = fvg.Mesh(ctx, raw)
+    lhs, rhs = fvg.assemble_program(mesh, prob["prog"], prob["face_fields"], prob["cell_fields"], prob["bc_i"],
+                                    prob["bc_e"])
+    rp, ci, va = lhs.csr()
+    n = rp.shape[0] - 1
+    try:
+        ctx.set_tuning(fvg.TUNE_SWEEP_BOXES, 64)
+        boxed = fvg.IncompleteLowerUpper.from_matrix(lhs, 0)
+        plain = fvg.IncompleteLowerUpper.from_matrix(fvg.DistributedMatrix.from_csr(ctx, rp, ci, va), 0)
+    finally:
+        ctx.set_tuning(fvg.TUNE_SWEEP_BOXES, 0)
+    lb, lp = boxed.sweep_layout(), plain.sweep_layout()
+    assert lb["lower_blocks"] == 64 and lb["block_levels"] < lp["block_levels"], (lb, lp)
+    r = np.random.default_rng(8).normal(size=n)
+    z = fvg.DistributedVector(ctx, n)
+    boxed.precondition(z, fvg.DistributedVector.from_data(ctx, r))
+    assert np.array_equal(z.numpy(), O.OraclePrec("ilu", rp, ci, va, 0).apply(r))
+
+
 @pytest.mark.parametrize("name,kind,prec,blocked", [("tet10", "sym", "ic", True), ("tet10", "nonsym", "ilu", True),
                                                     ("prism24", "nonsym", "ilu", True), ("prism24", "sym", "ic", True),
                                                     ("tri64", "sym", "ic", False)])
