@@ -269,6 +269,8 @@ int fvg_csr_set_row_coords(fvg_csr* a, int32_t dim, const double* coords) {
     a->h_coords.assign(count, 0.f);
     for (size_t k = 0; k < count; ++k) a->h_coords[k] = (float)coords[k];
     a->coord_dim = dim;
+    a->coord_mesh = nullptr;
+    a->coord_mesh_id = 0;
     if (dim == 0) a->h_coords.shrink_to_fit();
     return FVG_OK;
 }

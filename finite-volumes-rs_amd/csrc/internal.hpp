@@ -106,10 +106,13 @@ struct fvg_csr {
     uint64_t version = 0;              // bumped whenever the values change
     float* val32 = nullptr;            // FP32 copy of the values for mixed precision (made on use)
     uint64_t val32_version = ~0ull;
-    // coordinates of the rows (cell centres; coord_dim per row, 0: none), used by the
-    // preconditioner analysis to shape its blocked layouts
+    // coordinates of the rows (coord_dim per row, 0: none), used by the preconditioner
+    // analysis to shape its blocked layouts; a matrix made from a mesh instead names the
+    // mesh, whose cell centres the analysis reads while it is alive
     std::vector<float> h_coords;
     int32_t coord_dim = 0;
+    const fvg_mesh* coord_mesh = nullptr;
+    uint64_t coord_mesh_id = 0;
 };
 
 namespace fvg {

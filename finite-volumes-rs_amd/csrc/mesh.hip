@@ -32,7 +32,9 @@
 // reference's.
 #include <atomic>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 
 #include "mesh.hpp"
 
@@ -670,6 +672,13 @@ std::string mesh_error(unsigned long long key) {
 
 std::atomic<uint64_t> g_pattern_counter{1};
 
+// Meshes alive in this process, with their pattern ids: a matrix made from a mesh reads
+// the cell centres from it later only while that mesh (not another at the same address)
+// is alive.  A reader holds the lock for its whole copy, so a destroy waits for it.
+std::mutex g_live_mu;
+std::unordered_map<const fvg_mesh*, uint64_t> g_live;
+
+
 void free_mesh(fvg_ctx* c, fvg_mesh* m) {
     const int64_t F = m->n_faces, N = m->n_cells, D = m->dim;
     dev_free_n(c, m->d_area, F);
@@ -816,6 +825,21 @@ int build_mesh(fvg_ctx* c, fvg_mesh* m, int64_t n_nodes, const double* nodes, co
 
 }  // namespace
 
+int mesh_cell_centres(fvg_ctx* c, const fvg_mesh* m, uint64_t id, std::vector<float>& out, int& dim) {
+    out.clear();
+    dim = 0;
+    std::lock_guard<std::mutex> lock(g_live_mu);
+    const auto it = g_live.find(m);
+    if (it == g_live.end() || it->second != id) return FVG_OK;
+    std::vector<double> centre((size_t)m->n_cells * (size_t)m->dim);
+    FVG_HIP(hipMemcpyAsync(centre.data(), m->d_cell_center, centre.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    FVG_HIP(hipStreamSynchronize(c->stream));
+    out.assign(centre.begin(), centre.end());
+    dim = m->dim;
+    return FVG_OK;
+}
+
 int csr_from_device_pattern(fvg_ctx* c, int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* d_row_ptr,
                             const int32_t* d_col, fvg_csr** out);
 
@@ -851,12 +875,20 @@ int fvg_mesh_create(fvg_ctx* c, int dim, int64_t n_nodes, const double* nodes, i
         return s;
     }
     m->pattern_id = g_pattern_counter.fetch_add(1);
+    {
+        std::lock_guard<std::mutex> lock(g_live_mu);
+        g_live[m] = m->pattern_id;
+    }
     *out = m;
     return FVG_OK;
 }
 
 int fvg_mesh_destroy(fvg_ctx* c, fvg_mesh* m) {
     if (!m) return FVG_OK;
+    {
+        std::lock_guard<std::mutex> lock(g_live_mu);
+        g_live.erase(m);
+    }
     if (c) {
         (void)hipStreamSynchronize(c->stream);
         free_mesh(c, m);
@@ -895,22 +927,11 @@ int fvg_csr_from_mesh(fvg_ctx* c, const fvg_mesh* m, fvg_csr** out) {
     // every row holds its diagonal, so max_minor + 1 (dmatrix.rs:47) is the cell count
     FVG_TRY(csr_from_device_pattern(c, m->n_cells, m->n_cells, m->nnz, m->d_row_ptr, m->d_col, out));
     (*out)->pattern_id = m->pattern_id;
-    // the cell centres as row coordinates, for the preconditioner layout analysis
-    std::vector<double> centre((size_t)m->n_cells * (size_t)m->dim);
-    auto fetch = [&]() -> int {
-        if (centre.empty()) return FVG_OK;
-        FVG_HIP(hipMemcpyAsync(centre.data(), m->d_cell_center, centre.size() * sizeof(double), hipMemcpyDeviceToHost,
-                               c->stream));
-        FVG_HIP(hipStreamSynchronize(c->stream));
-        return FVG_OK;
-    };
-    int s = fetch();
-    if (s == FVG_OK) s = fvg_csr_set_row_coords(*out, m->dim, centre.data());
-    if (s != FVG_OK) {
-        (void)fvg_csr_destroy(c, *out);
-        *out = nullptr;
-    }
-    return s;
+    // the cell centres as row coordinates: read from the mesh when a preconditioner
+    // analysis asks for them (mesh_cell_centres)
+    (*out)->coord_mesh = m;
+    (*out)->coord_mesh_id = m->pattern_id;
+    return FVG_OK;
 }
 
 }  // extern "C"

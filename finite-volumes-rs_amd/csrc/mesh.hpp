@@ -45,6 +45,9 @@ struct fvg_mesh {
 };
 
 namespace fvg {
+// The cell centres of mesh m (dim floats per cell) into out, if m is still alive with
+// pattern id `id`; out is left empty otherwise.
+int mesh_cell_centres(fvg_ctx* c, const fvg_mesh* m, uint64_t id, std::vector<float>& out, int& dim);
 // entry roles of the per-cell assembly list
 constexpr int kRoleOwner = 0;       // interior face, this cell owns it: .y = slot (i, j)
 constexpr int kRoleNeighbour = 1;   // interior face, this cell is its neighbour: .y = slot (j, i)

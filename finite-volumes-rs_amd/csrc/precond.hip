@@ -22,6 +22,7 @@
 #include <cstring>
 
 #include "kstate.hpp"
+#include "mesh.hpp"
 #include "precond.hpp"
 #include "reduce.hpp"
 
@@ -1270,7 +1271,7 @@ static void block_level_stats(int64_t n, const std::vector<int32_t>& blk, int32_
 // than kBlockMaxDeps dependencies: the persistent sweep then serves.
 static bool assign_blocked(const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n, bool upper,
                            const std::vector<int32_t>& lev_in, int nlevels_in, int target, int target_consecutive,
-                           int gap, const RowCoords& X, const std::vector<int>& box_counts, BlockAssign& A) {
+                           int gap, const CoordSource& coords, const std::vector<int>& box_counts, BlockAssign& A) {
     auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
     if (n < 4096 || target < 2) return false;
     std::vector<int32_t> par((size_t)n);
@@ -1352,6 +1353,7 @@ static bool assign_blocked(const std::vector<int32_t>& rp, const std::vector<int
         };
         int64_t best = cost(comp_of, nbc, dlev, nlevels);
         // boxes of the rows' coordinates, where the matrix has them: kept when they cost less
+        const RowCoords X = coords && !box_counts.empty() ? coords() : RowCoords();
         if (X.v && X.dim > 0) {
             std::vector<int32_t> bblk, blev;
             for (int count : box_counts) {
@@ -1547,10 +1549,10 @@ static int materialize_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const
 // The assignment is kept (sw.h_assign) so that a mirrored layout can be made from it.
 static int build_blocked(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
                          bool upper, const std::vector<int32_t>* src_map, const std::vector<int32_t>& lev_in,
-                         int nlevels_in, int target, int target_consecutive, int gap, const RowCoords& X,
+                         int nlevels_in, int target, int target_consecutive, int gap, const CoordSource& coords,
                          const std::vector<int>& box_counts, Sweep& sw) {
     BlockAssign A;
-    if (!assign_blocked(rp, col, n, upper, lev_in, nlevels_in, target, target_consecutive, gap, X, box_counts, A))
+    if (!assign_blocked(rp, col, n, upper, lev_in, nlevels_in, target, target_consecutive, gap, coords, box_counts, A))
         return FVG_OK;
     FVG_TRY(materialize_blocked(c, rp, col, n, upper, src_map, A, false, sw));
     if (sw.nblk > 0) sw.h_assign = std::move(A);
@@ -1604,7 +1606,7 @@ static int build_mirrored(fvg_ctx* c, const std::vector<int32_t>& rp, const std:
 // layout where it can.
 static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
                        bool upper, const std::vector<int32_t>* src_map, Sweep& sw, const Sweep* mirror = nullptr,
-                       const RowCoords& X = RowCoords()) {
+                       const CoordSource& coords = CoordSource()) {
     std::vector<int32_t> lev((size_t)n, 0);
     int maxlev = 0;
     auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
@@ -1686,7 +1688,8 @@ static int build_sweep(fvg_ctx* c, const std::vector<int32_t>& rp, const std::ve
         box_counts.push_back((int)c->tune_sweep_boxes);
     else if (c->tune_sweep_boxes == 0)
         box_counts.push_back(std::max(2, c->num_cus / 2));
-    return build_blocked(c, rp, col, n, upper, src_map, lev, sw.nlevels, packed, consecutive, gap, X, box_counts, sw);
+    return build_blocked(c, rp, col, n, upper, src_map, lev, sw.nlevels, packed, consecutive, gap, coords, box_counts,
+                         sw);
 }
 
 // When both layouts of a factor are blocked, the upper set-up reads the lower sweep's
@@ -2568,11 +2571,22 @@ static int create_factored(fvg_ctx* c, const fvg_csr* a, int32_t level, int kind
     // factor entry
     if (s == FVG_OK) s = dev_alloc_n(c, p->nnz + 1, &p->d_fac);
     if (s == FVG_OK) s = dev_alloc_n(c, n, &p->d_tmp);
-    RowCoords X;
-    if (a->coord_dim > 0 && (int64_t)a->h_coords.size() == n * a->coord_dim) {
-        X.v = a->h_coords.data();
-        X.dim = a->coord_dim;
-    }
+    // the rows' coordinates: the matrix's own, or its mesh's cell centres, read once and
+    // only if a layout analysis prices boxes of them
+    std::vector<float> centre;
+    int centre_dim = 0;
+    bool centre_read = false;
+    const CoordSource X = [&]() -> RowCoords {
+        if (a->coord_dim > 0 && (int64_t)a->h_coords.size() == n * a->coord_dim)
+            return RowCoords{a->h_coords.data(), a->coord_dim};
+        if (a->coord_mesh && !centre_read) {
+            centre_read = true;
+            if (mesh_cell_centres(c, a->coord_mesh, a->coord_mesh_id, centre, centre_dim) != FVG_OK ||
+                (int64_t)centre.size() != n * centre_dim)
+                centre.clear();
+        }
+        return centre.empty() ? RowCoords() : RowCoords{centre.data(), centre_dim};
+    };
     if (s == FVG_OK) s = build_sweep(c, p->h_rp, p->h_col, n, false, nullptr, p->lower, nullptr, X);
     if (s == FVG_OK) {
         if (kind == kIC && level > 0) {

@@ -1,6 +1,7 @@
 // precond.hpp -- preconditioner objects (internal).
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "internal.hpp"
@@ -13,11 +14,14 @@ namespace fvg {
 // level is a contiguous range of chunks and its rows' entries are contiguous in HBM.
 constexpr int kChunk = 64;              // rows per wave64 chunk
 
-// Coordinates of the rows (cell centres, `dim` floats per row) for the layout analysis.
+// Coordinates of the rows (cell centres, `dim` floats per row) for the layout analysis,
+// and their source, called only where the analysis uses them (it may copy them from a
+// mesh first).
 struct RowCoords {
     const float* v = nullptr;
     int dim = 0;
 };
+using CoordSource = std::function<RowCoords()>;
 
 // Block and level of every row of a blocked sweep layout (host).
 struct BlockAssign {

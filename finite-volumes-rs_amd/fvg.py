@@ -621,7 +621,8 @@ class DistributedMatrix:
     def set_row_coordinates(self, coords):
         """Coordinates of the rows ((nrows, dim) with dim 1 to 3, e.g. cell centres; None
         drops them) for the layout analysis of preconditioners created afterwards.  A
-        matrix made by from_mesh carries its mesh's cell centres already."""
+        matrix made by from_mesh uses its mesh's cell centres already, while the mesh is
+        alive."""
         if coords is None:
             check(lib().fvg_csr_set_row_coords(self.h, 0, None))
             return

@@ -244,8 +244,9 @@ int fvg_pvtu_writer_destroy(fvg_pvtu_writer* writer);
 typedef struct fvg_csr fvg_csr;
 
 /* DistributedMatrix::from_cut_sparsity(mesh.cell_to_cell_sparsity(), n_cells)
- * (dmatrix.rs:38-50): sorted pattern, zero values.  The matrix keeps a host copy of the
- * cell centres as its row coordinates (see fvg_csr_set_row_coords). */
+ * (dmatrix.rs:38-50): sorted pattern, zero values.  The matrix remembers the mesh: while
+ * the mesh is alive, a preconditioner analysis may read its cell centres as the row
+ * coordinates (see fvg_csr_set_row_coords, which replaces them). */
 int fvg_csr_from_mesh(fvg_ctx* ctx, const fvg_mesh* mesh, fvg_csr** out);
 /* General CSR (columns sorted per row).  values may be NULL (zeros). */
 int fvg_csr_create(fvg_ctx* ctx, int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* row_ptr,

@@ -200,9 +200,11 @@ def test_coordinate_boxes_give_identical_bits(ctx, name, kind, prec):
 
 
 def test_mesh_matrices_carry_cell_centres(ctx):
-    """A matrix made from a device mesh carries the cell centres: on the 13,824-prism mesh
-    its ILU(0) takes a layout of 64 coordinate boxes when asked for that grid (fewer levels
-    than the same matrix made from CSR arrays) and applies with the oracle's bits."""
+    """A matrix made from a device mesh uses the mesh's cell centres: on the 13,824-prism
+    mesh its ILU(0) takes a layout of 64 coordinate boxes when asked for that grid (fewer
+    levels than the same matrix made from CSR arrays) and applies with the oracle's bits.
+    Once the mesh is destroyed the matrix has no coordinates: the layout of a matrix made
+    from CSR arrays."""
     from test_gpu_parity import _mk
     import problems
     raw, om, geom = _mk("prism24")
@@ -223,6 +225,15 @@ def test_mesh_matrices_carry_cell_centres(ctx):
     r = np.random.default_rng(8).normal(size=n)
     z = fvg.DistributedVector(ctx, n)
     boxed.precondition(z, fvg.DistributedVector.from_data(ctx, r))
+    assert np.array_equal(z.numpy(), O.OraclePrec("ilu", rp, ci, va, 0).apply(r))
+    mesh.close()
+    try:
+        ctx.set_tuning(fvg.TUNE_SWEEP_BOXES, 64)
+        after = fvg.IncompleteLowerUpper.from_matrix(lhs, 0)
+    finally:
+        ctx.set_tuning(fvg.TUNE_SWEEP_BOXES, 0)
+    assert after.sweep_layout() == lp
+    after.precondition(z, fvg.DistributedVector.from_data(ctx, r))
     assert np.array_equal(z.numpy(), O.OraclePrec("ilu", rp, ci, va, 0).apply(r))
 
 
