@@ -1178,7 +1178,8 @@ static int delayed_levels(int64_t n, const std::vector<int32_t>& rp, const std::
 // highest slab among its dependencies (lower sweep, rows ascending; the lowest, rows
 // descending, for an upper sweep), so that along every chain of dependencies the slab
 // indices never go back and a chain crosses each cut at most once: a chain then meets at
-// most (s0 - 1) + (s1 - 1) + (s2 - 1) delays, however long it is.  Returns the box count.
+// most (s0 - 1) + (s1 - 1) + (s2 - 1) delays, however long it is.  Returns the box count
+// (1, no boxes, where a coordinate is not finite).
 static int32_t coordinate_boxes(const std::vector<int32_t>& rp, const std::vector<int32_t>& col, int64_t n,
                                 bool upper, const RowCoords& X, int count, std::vector<int32_t>& blk) {
     auto take = [&](int64_t i, int32_t j) { return upper ? (j > i && j < n) : (j < i); };
@@ -1187,8 +1188,10 @@ static int32_t coordinate_boxes(const std::vector<int32_t>& rp, const std::vecto
     for (int a = 0; a < dim; ++a) {
         lo[a] = hi[a] = X.v[a];
         for (int64_t i = 0; i < n; ++i) {
-            lo[a] = std::min(lo[a], (double)X.v[(size_t)i * X.dim + a]);
-            hi[a] = std::max(hi[a], (double)X.v[(size_t)i * X.dim + a]);
+            const double v = X.v[(size_t)i * X.dim + a];
+            if (!std::isfinite(v)) return 1;
+            lo[a] = std::min(lo[a], v);
+            hi[a] = std::max(hi[a], v);
         }
     }
     // the grid: s[a] = 2^e[a], sum e = log2(count); least box aspect ratio, ties to more

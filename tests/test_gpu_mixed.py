@@ -154,8 +154,9 @@ def test_coordinate_boxes_give_identical_bits(ctx, name, kind, prec):
     coordinates (one dependency component: extruded and 2D meshes).  Without boxes, with
     the default choice and with grids of 16 and 64 boxes the preconditioner applies give the
     oracle's bits and the FP64 solves the same iterations and solution bits.  On the prism
-    mesh a forced grid of 64 boxes is built as 64 blocks, and boxes off gives the layout
-    of a matrix without coordinates."""
+    mesh a forced grid of 64 boxes is built as 64 blocks, and boxes off, coordinates
+    dropped or a coordinate that is not finite give the layout of a matrix without
+    coordinates."""
     from test_gpu_parity import _mk
     rp, ci, va, b = _oracle_system(name, kind)
     n = rp.shape[0] - 1
@@ -196,6 +197,11 @@ def test_coordinate_boxes_give_identical_bits(ctx, name, kind, prec):
         assert layouts[64]["lower_blocks"] == 64 and layouts[64]["upper_blocks"] == 64, layouts[64]
         assert layouts[64]["block_levels"] < plain["block_levels"], (layouts[64], plain)
     A.set_row_coordinates(None)
+    assert cls.from_matrix(A, 0).sweep_layout() == plain
+    # a coordinate that is not finite: no boxes
+    bad = np.array(centre, dtype=np.float64)
+    bad[n // 2, 0] = np.nan
+    A.set_row_coordinates(bad)
     assert cls.from_matrix(A, 0).sweep_layout() == plain
 
 
