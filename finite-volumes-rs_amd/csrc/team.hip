@@ -216,9 +216,16 @@ __device__ __forceinline__ void sweep_compute(const SweepRow<T>& R, const TeamSw
         if (R.lvl == L) {
             T s = R.b;
             if (ring_ok) {
+                // every slot's read issued at once (a slot past the row's count holds the
+                // row's last dependency), each product kept only for the row's own slots
+                T v[kES];
 #pragma unroll
-                for (int e = 0; e < kES; ++e)
-                    if (e < R.cnt) s -= R.val[e] * ring[R.dep[e] & (RING - 1)];
+                for (int e = 0; e < kES; ++e) v[e] = ring[R.dep[e] & (RING - 1)];
+#pragma unroll
+                for (int e = 0; e < kES; ++e) {
+                    const T t = s - R.val[e] * v[e];
+                    s = e < R.cnt ? t : s;
+                }
                 for (int k = R.kb + kES; k < R.kb + R.cnt; ++k) s -= val[k] * ring[S.dep[k] & (RING - 1)];
             } else {
 #pragma unroll
