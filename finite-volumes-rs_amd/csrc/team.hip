@@ -105,6 +105,25 @@ struct ProfScope {
     }
 };
 
+// ---- typed memory access -------------------------------------------------------------------
+// The descriptor lives in LDS, so every pointer read from it is generic, and generic
+// (flat) accesses count against both the vector-memory and the LDS wait counters: a wait
+// for an LDS value then also waits for every global load in flight, which undoes the
+// prefetching.  The hot loops therefore access memory through pointers typed global (gm)
+// or LDS (lm).
+template <class T>
+using gptr_t = __attribute__((address_space(1))) T*;
+template <class T>
+using lptr_t = __attribute__((address_space(3))) T*;
+template <class T>
+__device__ __forceinline__ gptr_t<T> gm(T* p) {
+    return (gptr_t<T>)p;
+}
+template <class T>
+__device__ __forceinline__ lptr_t<T> lm(T* p) {
+    return (lptr_t<T>)p;
+}
+
 // ---- workgroup reductions (result in every thread, fixed order) -------------------------
 template <int NT, class T>
 struct Reducer {
@@ -112,7 +131,7 @@ struct Reducer {
     int par;
     __device__ T sum(T v) {
         v = wave_sum(v);
-        T* r = buf + par * 2 * (NT / 64);
+        const lptr_t<T> r = lm(buf) + par * 2 * (NT / 64);
         par ^= 1;
         if ((threadIdx.x & 63) == 0) r[threadIdx.x >> 6] = v;
         __syncthreads();
@@ -124,7 +143,7 @@ struct Reducer {
     __device__ void sum2(T& a, T& b) {
         a = wave_sum(a);
         b = wave_sum(b);
-        T* r = buf + par * 2 * (NT / 64);
+        const lptr_t<T> r = lm(buf) + par * 2 * (NT / 64);
         par ^= 1;
         if ((threadIdx.x & 63) == 0) {
             r[threadIdx.x >> 6] = a;
@@ -170,8 +189,8 @@ template <int NT>
 __device__ __forceinline__ void sweep_ptr(const TeamSweep& S, int n, int g, int& kb, int& ke) {
     const int q = g * NT + (int)threadIdx.x;
     const int qc = min(q, n - 1);
-    kb = S.ptr[qc];
-    ke = S.ptr[qc + 1];
+    kb = gm(S.ptr)[qc];
+    ke = gm(S.ptr)[qc + 1];
     if (q >= n) ke = kb;
 }
 
@@ -184,16 +203,16 @@ __device__ __forceinline__ void sweep_row(SweepRow<T>& R, const TeamSweep& S, co
     const int pos = S.upper ? n - 1 - qc : qc;
     R.kb = kb;
     R.cnt = ke - kb;
-    const int lv = S.lvl[qc];
+    const int lv = gm(S.lvl)[qc];
     R.lvl = q < n ? lv : -1;
-    R.b = b[pos];
-    R.sc = sc[pos];
+    R.b = gm(b)[pos];
+    R.sc = gm(sc)[pos];
     const int last = max(R.cnt - 1, 0);
 #pragma unroll
     for (int e = 0; e < kES; ++e) {
         const int k = min(kb + min(e, last), S.nnz1);
-        R.dep[e] = S.dep[k];
-        R.val[e] = val[k];
+        R.dep[e] = gm(S.dep)[k];
+        R.val[e] = gm(val)[k];
     }
 }
 
@@ -202,8 +221,8 @@ template <int NT>
 __device__ __forceinline__ void group_levels(const TeamSweep& S, int n, int g, int& la, int& lb) {
     const int q0 = g * NT;
     const int q1 = min(q0 + NT, n);
-    la = S.lvl[q0];
-    lb = S.lvl[q1 - 1];
+    la = gm(S.lvl)[q0];
+    lb = gm(S.lvl)[q1 - 1];
 }
 
 template <int NT, class T, int RING>
@@ -220,22 +239,22 @@ __device__ __forceinline__ void sweep_compute(const SweepRow<T>& R, const TeamSw
                 // row's last dependency), each product kept only for the row's own slots
                 T v[kES];
 #pragma unroll
-                for (int e = 0; e < kES; ++e) v[e] = ring[R.dep[e] & (RING - 1)];
+                for (int e = 0; e < kES; ++e) v[e] = lm(ring)[R.dep[e] & (RING - 1)];
 #pragma unroll
                 for (int e = 0; e < kES; ++e) {
                     const T t = s - R.val[e] * v[e];
                     s = e < R.cnt ? t : s;
                 }
-                for (int k = R.kb + kES; k < R.kb + R.cnt; ++k) s -= val[k] * ring[S.dep[k] & (RING - 1)];
+                for (int k = R.kb + kES; k < R.kb + R.cnt; ++k) s -= gm(val)[k] * lm(ring)[gm(S.dep)[k] & (RING - 1)];
             } else {
 #pragma unroll
                 for (int e = 0; e < kES; ++e)
-                    if (e < R.cnt) s -= R.val[e] * out[R.dep[e]];
-                for (int k = R.kb + kES; k < R.kb + R.cnt; ++k) s -= val[k] * out[S.dep[k]];
+                    if (e < R.cnt) s -= R.val[e] * gm(out)[R.dep[e]];
+                for (int k = R.kb + kES; k < R.kb + R.cnt; ++k) s -= gm(val)[k] * gm(out)[gm(S.dep)[k]];
             }
             const T r = has_sc ? R.sc * s : s;
-            if (ring_ok) ring[pos & (RING - 1)] = r;
-            out[pos] = r;
+            if (ring_ok) lm(ring)[pos & (RING - 1)] = r;
+            gm(out)[pos] = r;
         }
         __syncthreads();
     }
@@ -278,11 +297,11 @@ __device__ void team_prec(const TeamDesc& D, const T* r, T* z, T* tmp, T* ring) 
     const T* sc = pick<T>(D.sc, D.sc32);
     switch (D.kind) {
     case kIdentity:
-        for (int p = threadIdx.x; p < D.n; p += NT) z[p] = r[p];
+        for (int p = threadIdx.x; p < D.n; p += NT) gm(z)[p] = gm(r)[p];
         __syncthreads();
         break;
     case kDiagonal:
-        for (int p = threadIdx.x; p < D.n; p += NT) z[p] = sc[p] * r[p];
+        for (int p = threadIdx.x; p < D.n; p += NT) gm(z)[p] = gm(sc)[p] * gm(r)[p];
         __syncthreads();
         break;
     case kIC:
@@ -322,8 +341,8 @@ template <int NT>
 __device__ __forceinline__ void spmv_ptr(const SpA& M, int g, int& kb, int& ke) {
     const int q = g * NT + (int)threadIdx.x;
     const int qc = min(q, M.n - 1);
-    kb = M.ptr[qc];
-    ke = M.ptr[qc + 1];
+    kb = gm(M.ptr)[qc];
+    ke = gm(M.ptr)[qc + 1];
     if (q >= M.n) ke = kb;
 }
 
@@ -337,11 +356,11 @@ __device__ __forceinline__ void spmv_row(SpRow<T>& R, const SpA& M, const T* __r
 #pragma unroll
     for (int e = 0; e < kEA; ++e) {
         const int k = min(kb + min(e, last), M.nnz1);
-        R.dep[e] = M.dep[k];
-        R.val[e] = val[k];
+        R.dep[e] = gm(M.dep)[k];
+        R.val[e] = gm(val)[k];
     }
-    R.aux = aux[min(q, M.npad - 1)];
-    R.xs = x[min(q + NT + M.w, M.npad - 1)];   // window slice this tile adds for the next one
+    R.aux = gm(aux)[min(q, M.npad - 1)];
+    R.xs = gm(x)[min(q + NT + M.w, M.npad - 1)];   // window slice this tile adds for the next one
 }
 
 template <int NT, class T, int RING, class Epi>
@@ -352,13 +371,13 @@ __device__ __forceinline__ void spmv_compute(const SpRow<T>& R, const SpA& M, co
     if (win) {
 #pragma unroll
         for (int e = 0; e < kEA; ++e)
-            if (e < R.cnt) s += R.val[e] * ring[R.dep[e] & (RING - 1)];
-        for (int k = R.kb + kEA; k < R.kb + R.cnt; ++k) s += val[k] * ring[M.dep[k] & (RING - 1)];
+            if (e < R.cnt) s += R.val[e] * lm(ring)[R.dep[e] & (RING - 1)];
+        for (int k = R.kb + kEA; k < R.kb + R.cnt; ++k) s += gm(val)[k] * lm(ring)[gm(M.dep)[k] & (RING - 1)];
     } else {
 #pragma unroll
         for (int e = 0; e < kEA; ++e)
-            if (e < R.cnt) s += R.val[e] * x[R.dep[e]];
-        for (int k = R.kb + kEA; k < R.kb + R.cnt; ++k) s += val[k] * x[M.dep[k]];
+            if (e < R.cnt) s += R.val[e] * gm(x)[R.dep[e]];
+        for (int k = R.kb + kEA; k < R.kb + R.cnt; ++k) s += gm(val)[k] * gm(x)[gm(M.dep)[k]];
     }
     epi(q, s, R.aux);
 }
@@ -373,7 +392,7 @@ __device__ void team_spmv(const TeamDesc& Dd, const T* __restrict__ val, const T
     const int G = (n + NT - 1) / NT;
     if (win) {   // window for tile 0: positions [0, NT + w)
         const int e0 = min(M.npad, NT + M.w);
-        for (int p = threadIdx.x; p < e0; p += NT) ring[p & (RING - 1)] = x[p];
+        for (int p = threadIdx.x; p < e0; p += NT) lm(ring)[p & (RING - 1)] = gm(x)[p];
     }
     SpRow<T> A, B;
     int akb, ake, bkb, bke;
@@ -389,7 +408,7 @@ __device__ void team_spmv(const TeamDesc& Dd, const T* __restrict__ val, const T
         if (g + 1 >= G) break;
         if (win) {   // tile g+1 needs positions up to (g+2) NT + w: the slice A fetched
             const int pa = (g + 1) * NT + M.w + (int)threadIdx.x;
-            if (pa < M.npad) ring[pa & (RING - 1)] = A.xs;
+            if (pa < M.npad) lm(ring)[pa & (RING - 1)] = A.xs;
             __syncthreads();
         }
         spmv_row<NT, T>(A, M, val, aux, x, g2, akb, ake);
@@ -397,7 +416,7 @@ __device__ void team_spmv(const TeamDesc& Dd, const T* __restrict__ val, const T
         spmv_compute<NT, T, RING>(B, M, val, x, ring, win, g + 1, epi);
         if (win && g + 2 < G) {
             const int pb = (g + 2) * NT + M.w + (int)threadIdx.x;
-            if (pb < M.npad) ring[pb & (RING - 1)] = B.xs;
+            if (pb < M.npad) lm(ring)[pb & (RING - 1)] = B.xs;
             __syncthreads();
         }
     }
@@ -409,12 +428,15 @@ struct alignas(4 * sizeof(T)) V4 {
     T e[4];
 };
 template <class T>
+using ev4 = T __attribute__((ext_vector_type(4)));
+template <class T>
 __device__ __forceinline__ V4<T> ld4(const T* p, int v) {
-    return *reinterpret_cast<const V4<T>*>(p + 4 * v);
+    const ev4<T> a = *reinterpret_cast<gptr_t<const ev4<T>>>(gm(p + 4 * v));
+    return V4<T>{{a.x, a.y, a.z, a.w}};
 }
 template <class T>
 __device__ __forceinline__ void st4(T* p, int v, const V4<T>& a) {
-    *reinterpret_cast<V4<T>*>(p + 4 * v) = a;
+    *reinterpret_cast<gptr_t<ev4<T>>>(gm(p + 4 * v)) = ev4<T>{a.e[0], a.e[1], a.e[2], a.e[3]};
 }
 
 // p = z; returns partial r.z
@@ -545,7 +567,7 @@ __device__ void pass_bi_p(int nv, const double* __restrict__ r, const double* __
 
 // ---- per-system bookkeeping ----------------------------------------------------------------
 __device__ __forceinline__ void team_push(const TeamDesc& D, int64_t& len, double e) {
-    if (threadIdx.x == 0 && len < D.hcap) D.hist[len] = e;
+    if (threadIdx.x == 0 && len < D.hcap) gm(D.hist)[len] = e;
     len += 1;
 }
 
@@ -560,19 +582,19 @@ __device__ double team_gather(const TeamDesc& D, double* __restrict__ b, double*
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const int p = p0 + u * NT + (int)threadIdx.x;
-            row[u] = p < D.n ? perm[p] : 0;
+            row[u] = p < D.n ? gm(perm)[p] : 0;
         }
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            bb[u] = D.b[row[u]];
-            xx[u] = D.x[row[u]];
+            bb[u] = gm(D.b)[row[u]];
+            xx[u] = gm(D.x)[row[u]];
         }
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const int p = p0 + u * NT + (int)threadIdx.x;
             if (p < D.n) {
-                b[p] = bb[u];
-                x[p] = xx[u];
+                gm(b)[p] = bb[u];
+                gm(x)[p] = xx[u];
                 q += bb[u] * bb[u];
             }
         }
@@ -582,7 +604,7 @@ __device__ double team_gather(const TeamDesc& D, double* __restrict__ b, double*
 
 template <int NT>
 __device__ void team_scatter(const TeamDesc& D, const double* __restrict__ x) {
-    for (int p = threadIdx.x; p < D.n; p += NT) D.x[D.perm[p]] = x[p];
+    for (int p = threadIdx.x; p < D.n; p += NT) gm(D.x)[gm(D.perm)[p]] = gm(x)[p];
 }
 
 __device__ void team_info(const TeamDesc& D, int64_t iters, double init, double fin, int64_t hlen, int64_t restarts,
@@ -605,7 +627,7 @@ __device__ void team_info(const TeamDesc& D, int64_t iters, double init, double 
 template <int NT>
 __device__ bool team_zero_rhs(const TeamDesc& D, double bnorm, bool mixed) {
     if (!(bnorm < 1e-14)) return false;
-    for (int p = threadIdx.x; p < D.n; p += NT) D.x[D.perm[p]] = 0.0;
+    for (int p = threadIdx.x; p < D.n; p += NT) gm(D.x)[gm(D.perm)[p]] = 0.0;
     int64_t len = 0;
     team_push(D, len, bnorm);
     team_info(D, 0, bnorm, bnorm, len, 0, 0, mixed ? bnorm : 0.0);
@@ -629,7 +651,7 @@ __device__ void team_cg64(const TeamDesc& D, Reducer<NT, double>& R, double* rin
     double q = 0.0;
     team_spmv<NT, double, kRing64>(D, D.a_val, x, b, ring, [&](int i, double s, double bi) {
         const double ri = (s - bi) * -1.0;
-        r[i] = ri;
+        gm(r)[i] = ri;
         q += ri * ri;
     });
     const double init_res = sqrt(R.sum(q)) / bnorm;
@@ -641,7 +663,7 @@ __device__ void team_cg64(const TeamDesc& D, Reducer<NT, double>& R, double* rin
         for (;;) {
             q = 0.0;   // ap = A p, p.ap (:72-74)
             team_spmv<NT, double, kRing64>(D, D.a_val, p, p, ring, [&](int i, double s, double pi) {
-                ap[i] = s;
+                gm(ap)[i] = s;
                 q += pi * s;
             });
             const double alpha = rz / R.sum(q);
@@ -687,9 +709,9 @@ __device__ void team_bi64(const TeamDesc& D, Reducer<NT, double>& R, double* rin
     double q = 0.0;   // r = b - A x, rh = p = r (:165-190)
     team_spmv<NT, double, kRing64>(D, D.a_val, x, b, ring, [&](int i, double sv, double bi) {
         const double ri = (sv - bi) * -1.0;
-        r[i] = ri;
-        rh[i] = ri;
-        p[i] = ri;
+        gm(r)[i] = ri;
+        gm(rh)[i] = ri;
+        gm(p)[i] = ri;
         q += ri * ri;
     });
     const double rr0 = R.sum(q);
@@ -709,7 +731,7 @@ __device__ void team_bi64(const TeamDesc& D, Reducer<NT, double>& R, double* rin
         team_prec<NT, double, kRing64>(D, p, y, t, ring);   // y = M^-1 p
         q = 0.0;                                                 // v = A y, rh.v (:208-209)
         team_spmv<NT, double, kRing64>(D, D.a_val, y, rh, ring, [&](int i, double sv, double rhi) {
-            v[i] = sv;
+            gm(v)[i] = sv;
             q += rhi * sv;
         });
         alpha = rho / R.sum(q);
@@ -723,7 +745,7 @@ __device__ void team_bi64(const TeamDesc& D, Reducer<NT, double>& R, double* rin
         team_prec<NT, double, kRing64>(D, s, z, t, ring);   // z = M^-1 s
         double q0 = 0.0, q1 = 0.0;                               // t = A z; t.s, t.t (:239-242)
         team_spmv<NT, double, kRing64>(D, D.a_val, z, s, ring, [&](int i, double tv, double si) {
-            t[i] = tv;
+            gm(t)[i] = tv;
             q0 += si * tv;
             q1 += tv * tv;
         });
@@ -805,8 +827,8 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
         double q = 0.0;
         team_spmv<NT, double, kRing64>(D, D.a_val, x, b, (double*)nullptr, [&](int i, double sv, double bi) {
             const double ri = (sv - bi) * -1.0;
-            r[i] = (float)ri;
-            d[i] = 0.f;
+            gm(r)[i] = (float)ri;
+            gm(d)[i] = 0.f;
             q += ri * ri;
         });
         const double e = sqrt(R.sum(q)) / bnorm;
@@ -815,7 +837,7 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
         // refinement keeps only corrections that lower the true residual: an FP32 inner
         // solve that stagnated or diverged is taken back and retried with a shorter budget
         if (check && !(e < e_acc)) {
-            for (int i = tid; i < np; i += NT) x[i] = xa[i];
+            for (int i = tid; i < np; i += NT) gm(x)[i] = gm(xa)[i];
             __syncthreads();
             check = false;
             cap = cap / 4 > 8 ? cap / 4 : 8;
@@ -844,7 +866,7 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
         check = false;
         team_push(D, hlen, e);
         if ((e < D.atol) || (e / init_res < D.rtol) || total >= max_inner) break;
-        for (int i = tid; i < np; i += NT) xa[i] = x[i];
+        for (int i = tid; i < np; i += NT) gm(xa)[i] = gm(x)[i];
         const int64_t inner_end = total + cap;
         const double tol = inner_target(D, e, init_res);
         if (!bicg) {
@@ -866,7 +888,7 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
             for (;;) {
                 float qa = 0.f;
                 team_spmv<NT, float, kRing32>(D, D.a_val32, p, p, ring, [&](int i, float sv, float pi) {
-                    ap[i] = sv;
+                    gm(ap)[i] = sv;
                     qa += pi * sv;
                 });
                 const double alpha_d = rz / (double)F.sum(qa);
@@ -919,7 +941,7 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
                 team_prec<NT, float, kRing32>(D, p, y, t, ring);
                 float qv = 0.f;
                 team_spmv<NT, float, kRing32>(D, D.a_val32, y, rh, ring, [&](int i, float sv, float rhi) {
-                    v[i] = sv;
+                    gm(v)[i] = sv;
                     qv += rhi * sv;
                 });
                 alpha = rho / (double)F.sum(qv);
@@ -946,7 +968,7 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
                 team_prec<NT, float, kRing32>(D, s, z, t, ring);
                 float qts = 0.f, qtt = 0.f;
                 team_spmv<NT, float, kRing32>(D, D.a_val32, z, s, ring, [&](int i, float tv, float si) {
-                    t[i] = tv;
+                    gm(t)[i] = tv;
                     qts += tv * si;
                     qtt += tv * tv;
                 });
@@ -1004,14 +1026,14 @@ __device__ void team_mixed(const TeamDesc& D, Reducer<NT, double>& R, Reducer<NT
         }
         for (int vi = threadIdx.x; vi < np / 2; vi += NT) {   // FP64 solution update x += d
             const int i = 2 * vi;
-            x[i] += (double)d[i];
-            x[i + 1] += (double)d[i + 1];
+            gm(x)[i] += (double)gm(d)[i];
+            gm(x)[i + 1] += (double)gm(d)[i + 1];
         }
         check = true;
         __syncthreads();
     }
     if (check) {   // the loop ended on an unjudged correction (outer-step bound): keep the accepted x
-        for (int i = tid; i < np; i += NT) x[i] = xa[i];
+        for (int i = tid; i < np; i += NT) gm(x)[i] = gm(xa)[i];
         __syncthreads();
     }
     if (fallback) {
