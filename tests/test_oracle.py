@@ -952,12 +952,12 @@ def test_wall_face_viscosity_and_normalisation_closed_forms():
 
 
 def test_wall_distance_loop_against_independent_solves():
-    """compute_wall_distance (wall_distance.rs:59-105): round 1 solves
+    """compute_wall_distance (wall_distance.rs:59-105): the first pass solves
     -laplacian(Corrected(0, 1), 1) + source(-1) with walls at y = 0 and y = 1 (closure
-    (0, 0)) and zero gradient elsewhere ((1, 0)); round 2 the same with the new gradients
-    and face diffusivity |grad_f|.  Each round is checked against scipy's direct solve of
-    the oracle-assembled system, and the result against the normalisation of the last
-    round's solution with the gradients taken after it.  A tolerance above the first
+    (0, 0)) and zero gradient elsewhere ((1, 0)); the second pass the same with the new
+    gradients and face diffusivity |grad_f|.  Each pass is checked against scipy's direct
+    solve of the oracle-assembled system, and the result against the normalisation of the
+    last pass's solution with the gradients taken after it.  A tolerance above the first
     initial residual stops after one solve, with zero gradients in the normalisation."""
     m = meshgen.tri2d(12, seed=7)
     om = O.OracleMesh(m)
