@@ -118,3 +118,36 @@ def test_batches_refuse_fill_levels(ctx):
     with pytest.raises(fvg.FvgError) as e:
         fvg.Batch(ctx, [A], [M], "cg", fvg.LinearSolverOptions(1e-8, 0.0, 100))
     assert fvg.STATUS[e.value.status] == "FVG_E_UNSUPPORTED"
+
+
+@pytest.mark.parametrize("kind", ["ilu", "ic"])
+def test_fill_layouts_with_coordinate_boxes(ctx, kind):
+    """IC(1) and ILU(1) of the prism mesh whose rows carry their cell centres: with the
+    coordinate boxes off, at the default and forced to grids of 16 and 64, every apply
+    gives the oracle's bits.  ILU(1)'s fill pattern is not symmetric, so its upper layout
+    is assigned on its own, with the slab indices closed along the upper dependencies."""
+    from test_gpu_parity import _mk
+    centre = _mk("prism24")[2]["cell_center"]
+    rp, ci, va, b = _oracle_system("prism24", "nonsym" if kind == "ilu" else "sym")
+    n = rp.shape[0] - 1
+    A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+    A.set_row_coordinates(centre)
+    r = np.random.default_rng(11).normal(size=n)
+    want = O.OraclePrec(kind, rp, ci, va, 1).apply(r)
+    Prec = fvg.IncompleteCholesky if kind == "ic" else fvg.IncompleteLowerUpper
+    layouts = {}
+    try:
+        for boxes in (1, 0, 16, 64):
+            ctx.set_tuning(fvg.TUNE_SWEEP_BOXES, boxes)
+            M = Prec.from_matrix(A, 1)
+            layouts[boxes] = M.sweep_layout()
+            z = fvg.DistributedVector(ctx, n)
+            M.precondition(z, fvg.DistributedVector.from_data(ctx, r))
+            assert np.array_equal(z.numpy(), want), (kind, boxes, layouts[boxes])
+            M.close()
+    finally:
+        ctx.set_tuning(fvg.TUNE_SWEEP_BOXES, 0)
+    # a grid of 64 boxes is built for both factors: mirrored for IC(1), on its own for ILU(1)
+    assert layouts[64]["lower_blocks"] == 64 and layouts[64]["upper_blocks"] == 64, layouts[64]
+    assert layouts[64]["mirrored"] == (1 if kind == "ic" else 0), layouts[64]
+    assert layouts[64]["block_levels"] < layouts[1]["block_levels"], (layouts[64], layouts[1])
