@@ -464,8 +464,10 @@ int run_mixed(fvg_ctx* c, fvg_csr* a, fvg_prec* p, const fvg_vec* b, fvg_vec* x,
             hipLaunchKernelGGL(k_mscalar<M_CG_ALPHA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
             hipLaunchKernelGGL(k_m_cg_dr, dim3(nb), dim3(kBlock), 0, c->stream, n, st, pp, ap, d, r, part);
             hipLaunchKernelGGL(k_mscalar<M_CG_ERR>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
-            FVG_TRY(apply_prec32(c, p, r, z, inner_skip));
-            hipLaunchKernelGGL(k_m_dot, dim3(nb), dim3(kBlock), 0, c->stream, n, st, 1, r, z, part);
+            // r.z: fused into the preconditioner's last sweep where it can be
+            bool dot_done = false;
+            FVG_TRY(apply_prec32(c, p, r, z, inner_skip, DotFuse{r, part, nb, &dot_done}));
+            if (!dot_done) hipLaunchKernelGGL(k_m_dot, dim3(nb), dim3(kBlock), 0, c->stream, n, st, 1, r, z, part);
             hipLaunchKernelGGL(k_mscalar<M_CG_BETA>, dim3(1), dim3(kScalarBlock), 0, c->stream, st, part, stride, nb);
             hipLaunchKernelGGL(k_m_cg_p, dim3(nb), dim3(kBlock), 0, c->stream, n, st, z, pp);
         } else {

@@ -797,7 +797,8 @@ template <bool FIRST, bool SCALE>
 __global__ void __launch_bounds__(kJacRows)
     k_jacobi32(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col, const float* __restrict__ val,
                const float* __restrict__ scale, const float* __restrict__ v, const float* __restrict__ yin,
-               float* __restrict__ yout, const int32_t* __restrict__ skip) {
+               float* __restrict__ yout, const int32_t* __restrict__ skip, const float* __restrict__ dotw,
+               double* __restrict__ dpart) {
     __shared__ float prod[kJacStage];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int r0 = tile * kJacRows;
@@ -840,7 +841,13 @@ __global__ void __launch_bounds__(kJacRows)
             sum += val[k] * ((FIRST && SCALE) ? scale[j] * src[j] : src[j]);
         }
     }
-    if (r < r1) yout[r] = SCALE ? sr * (vr - sum) : vr - sum;
+    const float y = SCALE ? sr * (vr - sum) : vr - sum;
+    if (r < r1) yout[r] = y;
+    if (dotw) {   // dotw . yout over the tile's rows, at the tile's slot (as a row-ordered dot kernel)
+        __shared__ float red[kJacRows / 64];
+        const float q = block_sum<kJacRows>(r < r1 ? dotw[r] * y : 0.f, red);
+        if (threadIdx.x == 0) dpart[tile] = (double)q;
+    }
 }
 
 __global__ void k_gather_f32(int64_t nnz, const int32_t* __restrict__ src, const double* __restrict__ fac,
@@ -2134,29 +2141,32 @@ int ensure_prec32(fvg_ctx* c, fvg_prec* p) {
 
 // m Jacobi sweeps of one triangular factor (natural row order); sweep k writes `odd` for
 // odd k and `even` for even k, so the result lands in `odd` when m is odd and in `even`
-// otherwise
+// otherwise.  With dotw, the last sweep also leaves the partials of dotw . result, one per
+// tile of kJacRows rows at the tile's index (the layout of a one-row-per-thread dot kernel
+// with workgroups of kJacRows threads).
 static void jacobi32(fvg_ctx* c, const Tri32& t, int m, const float* scale, const float* v, float* odd, float* even,
-                     const int32_t* skip) {
+                     const int32_t* skip, const float* dotw = nullptr, double* dpart = nullptr) {
     if (t.n == 0) return;
     const dim3 grid((unsigned)((t.n + kJacRows - 1) / kJacRows)), block(kJacRows);
     const int n = (int)t.n;
     const float* prev = nullptr;
     for (int k = 1; k <= m; ++k) {
         float* out = (k & 1) ? odd : even;
+        const float* dw = k == m ? dotw : nullptr;
         if (k == 1) {
             if (scale)
                 hipLaunchKernelGGL((k_jacobi32<true, true>), grid, block, 0, c->stream, n, t.ptr, t.col, t.val, scale,
-                                   v, nullptr, out, skip);
+                                   v, nullptr, out, skip, dw, dpart);
             else
                 hipLaunchKernelGGL((k_jacobi32<true, false>), grid, block, 0, c->stream, n, t.ptr, t.col, t.val,
-                                   nullptr, v, nullptr, out, skip);
+                                   nullptr, v, nullptr, out, skip, dw, dpart);
         } else {
             if (scale)
                 hipLaunchKernelGGL((k_jacobi32<false, true>), grid, block, 0, c->stream, n, t.ptr, t.col, t.val, scale,
-                                   v, prev, out, skip);
+                                   v, prev, out, skip, dw, dpart);
             else
                 hipLaunchKernelGGL((k_jacobi32<false, false>), grid, block, 0, c->stream, n, t.ptr, t.col, t.val,
-                                   nullptr, v, prev, out, skip);
+                                   nullptr, v, prev, out, skip, dw, dpart);
         }
         prev = out;
     }
@@ -2165,7 +2175,7 @@ static void jacobi32(fvg_ctx* c, const Tri32& t, int m, const float* scale, cons
 // the FP32 inner solver's factor application: exact level-scheduled sweeps, or m Jacobi
 // sweeps per triangular factor (the default)
 static int apply_factor32(fvg_ctx* c, const fvg_prec* p, const float* lo_scale, const float* r, float* z,
-                          const int32_t* skip) {
+                          const int32_t* skip, const DotFuse& dot) {
     if (c->tune_inner_exact) {
         FVG_TRY(sweep32(c, p->lower, lo_scale, r, p->d_tmp32, p->d_flags + 3, skip));
         return sweep32(c, p->upper, p->d_inv_diag32, p->d_tmp32, z, p->d_flags + 3, skip);
@@ -2175,12 +2185,18 @@ static int apply_factor32(fvg_ctx* c, const fvg_prec* p, const float* lo_scale, 
     jacobi32(c, p->jlo, m, lo_scale, r, p->d_tmp32, p->d_tmp32b, skip);
     const float* y = (m & 1) ? p->d_tmp32 : p->d_tmp32b;
     float* other = (m & 1) ? p->d_tmp32b : p->d_tmp32;
-    jacobi32(c, p->jup, m, p->d_inv_diag32, y, (m & 1) ? z : other, (m & 1) ? other : z, skip);
+    // the dot with z fused into the upper factor's last sweep where its partials land as the
+    // caller's (one per kJacRows rows)
+    const bool fuse = dot.w && dot.done && p->jup.n > 0 && (p->jup.n + kJacRows - 1) / kJacRows == dot.count;
+    jacobi32(c, p->jup, m, p->d_inv_diag32, y, (m & 1) ? z : other, (m & 1) ? other : z, skip,
+             fuse ? dot.w : nullptr, fuse ? dot.part : nullptr);
+    if (fuse) *dot.done = true;
     FVG_HIP(hipGetLastError());
     return FVG_OK;
 }
 
-int apply_prec32(fvg_ctx* c, const fvg_prec* p, const float* r, float* z, const int32_t* skip) {
+int apply_prec32(fvg_ctx* c, const fvg_prec* p, const float* r, float* z, const int32_t* skip, const DotFuse& dot) {
+    if (dot.done) *dot.done = false;
     switch (p->kind) {
     case kIdentity:
         hipLaunchKernelGGL(k_scale32, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, nullptr, r, z, skip);
@@ -2190,9 +2206,9 @@ int apply_prec32(fvg_ctx* c, const fvg_prec* p, const float* r, float* z, const 
                            skip);
         break;
     case kIC:
-        return apply_factor32(c, p, p->d_inv_diag32, r, z, skip);
+        return apply_factor32(c, p, p->d_inv_diag32, r, z, skip, dot);
     case kILU:
-        return apply_factor32(c, p, nullptr, r, z, skip);
+        return apply_factor32(c, p, nullptr, r, z, skip, dot);
     default:
         return fail(FVG_E_INVALID, "unknown preconditioner");
     }

@@ -155,7 +155,16 @@ int check_sweep_status(fvg_ctx* ctx, const fvg_prec* p);
 // FP32 factor copies for the mixed-precision inner solver (refreshed after a refactor).
 int ensure_prec32(fvg_ctx* ctx, fvg_prec* p);
 // solution = M^-1 rhs with FP32 factors, vectors and arithmetic.
-int apply_prec32(fvg_ctx* ctx, const fvg_prec* p, const float* rhs, float* solution, const int32_t* skip);
+// Optional dot product w . solution fused into an FP32 apply: `count` partials expected (one
+// per 256 rows, at partials[0 .. count)); *done tells whether the apply left them.
+struct DotFuse {
+    const float* w = nullptr;
+    double* part = nullptr;
+    int64_t count = 0;
+    bool* done = nullptr;
+};
+int apply_prec32(fvg_ctx* ctx, const fvg_prec* p, const float* rhs, float* solution, const int32_t* skip,
+                 const DotFuse& dot = DotFuse());
 // frees the batched-solve plan cached on p (team.hip)
 void team_plan_release(fvg_ctx* ctx, fvg_prec* p);
 
