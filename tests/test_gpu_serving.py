@@ -169,3 +169,28 @@ def test_requests_on_their_own_meshes():
         assert np.array_equal(again.x, r.x)
     finally:
         srv.close()
+
+
+def test_slots_skip_the_device_wide_sweep_layout():
+    """A slot is only ever solved in batches, so its preconditioner is made without the
+    blocked layout of device-wide sweeps (most of its set-up time), the context's tuning
+    is left as it was, and the answers meet the accuracy bar."""
+    import fvg
+    srv, pool = _server(2 * 10**9, lanes=2, max_batch=4)
+    try:
+        srv.ctx.set_tuning(fvg.TUNE_SWEEP_BLOCKS, 0)
+        reqs = _requests(srv, pool, 8, seed=4)
+        srv.run(reqs)
+        assert srv.ctx.get_tuning(fvg.TUNE_SWEEP_BLOCKS) == 0
+        slots = list(srv.slots)
+        assert slots
+        for s in slots:
+            lay = s.M.sweep_layout()
+            assert lay["lower_blocks"] == 0 and lay["upper_blocks"] == 0, lay
+        for r in reqs:
+            rp, ci, va, b = _oracle_system(pool[r.mesh], r)
+            n = rp.shape[0] - 1
+            S = sp.csr_matrix((va, ci, rp), shape=(n, n))
+            assert np.linalg.norm(b - S @ r.x) / np.linalg.norm(b) <= 1e-8, r.rid
+    finally:
+        srv.close()
