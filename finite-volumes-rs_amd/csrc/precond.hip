@@ -793,9 +793,11 @@ __global__ void __launch_bounds__(1024)
 constexpr int kJacRows = 256;
 constexpr int kJacStage = 2048;
 constexpr int kJacUnroll = 4;
-template <bool FIRST, bool SCALE>
+// DELTA: columns read as 16-bit offsets from the tile's first row (dcol) instead of col
+template <bool FIRST, bool SCALE, bool DELTA>
 __global__ void __launch_bounds__(kJacRows)
-    k_jacobi32(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col, const float* __restrict__ val,
+    k_jacobi32(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+               const int16_t* __restrict__ dcol, const float* __restrict__ val,
                const float* __restrict__ scale, const float* __restrict__ v, const float* __restrict__ yin,
                float* __restrict__ yout, const int32_t* __restrict__ skip, const float* __restrict__ dotw,
                double* __restrict__ dpart) {
@@ -820,7 +822,8 @@ __global__ void __launch_bounds__(kJacRows)
 #pragma unroll
             for (int u = 0; u < kJacUnroll; ++u) {
                 const int k = min(k0 + u * kJacRows, cnt - 1);
-                c[u] = __builtin_nontemporal_load(col + base + k);
+                c[u] = DELTA ? r0 + (int)__builtin_nontemporal_load(dcol + base + k)
+                             : __builtin_nontemporal_load(col + base + k);
                 a[u] = __builtin_nontemporal_load(val + base + k);
             }
 #pragma unroll
@@ -837,7 +840,7 @@ __global__ void __launch_bounds__(kJacRows)
             for (int k = rb - base; k < re - base; ++k) sum += prod[k];
     } else if (r < r1) {
         for (int k = rb; k < re; ++k) {
-            const int j = col[k];
+            const int j = DELTA ? r0 + (int)dcol[k] : col[k];
             sum += val[k] * ((FIRST && SCALE) ? scale[j] * src[j] : src[j]);
         }
     }
@@ -1743,6 +1746,7 @@ static int pair_layouts(fvg_ctx* c, Sweep& lo, Sweep& up) {
 static void free_tri32(fvg_ctx* c, Tri32& t) {
     dev_free_n(c, t.ptr, t.n + 1);
     dev_free_n(c, t.col, t.nnz);
+    if (t.dcol) dev_free_n(c, t.dcol, t.nnz);
     dev_free_n(c, t.src, t.nnz);
     dev_free_n(c, t.val, t.nnz);
     t = Tri32{};
@@ -2100,12 +2104,28 @@ static int build_tri32(fvg_ctx* c, const fvg_prec* p, bool upper, Tri32& t) {
     }
     t.n = n;
     t.nnz = (int64_t)col.size();
+    // 16-bit column offsets from each tile's first row, where they all fit: the sweeps then
+    // stream 2 bytes per entry for the columns instead of 4
+    std::vector<int16_t> dcol((size_t)t.nnz);
+    bool fits = t.nnz > 0;
+    for (int64_t i = 0; i < n && fits; ++i) {
+        const int64_t r0 = i / kJacRows * kJacRows;
+        for (int32_t k = ptr[(size_t)i]; k < ptr[(size_t)i + 1]; ++k) {
+            const int64_t d = (int64_t)col[(size_t)k] - r0;
+            if (d < INT16_MIN || d > INT16_MAX) {
+                fits = false;
+                break;
+            }
+            dcol[(size_t)k] = (int16_t)d;
+        }
+    }
     int s;
     if ((s = dev_alloc_n(c, n + 1, &t.ptr)) || (s = dev_alloc_n(c, t.nnz, &t.col)) ||
         (s = dev_alloc_n(c, t.nnz, &t.src)) || (s = dev_alloc_n(c, t.nnz, &t.val)) ||
         (s = upload(c, t.ptr, ptr.data(), n + 1)) || (s = upload(c, t.col, col.data(), t.nnz)) ||
         (s = upload(c, t.src, src.data(), t.nnz)))
         return s;
+    if (fits && ((s = dev_alloc_n(c, t.nnz, &t.dcol)) || (s = upload(c, t.dcol, dcol.data(), t.nnz)))) return s;
     FVG_HIP(hipStreamSynchronize(c->stream));   // the host vectors go out of scope
     return FVG_OK;
 }
@@ -2144,6 +2164,18 @@ int ensure_prec32(fvg_ctx* c, fvg_prec* p) {
 // otherwise.  With dotw, the last sweep also leaves the partials of dotw . result, one per
 // tile of kJacRows rows at the tile's index (the layout of a one-row-per-thread dot kernel
 // with workgroups of kJacRows threads).
+template <bool FIRST, bool SCALE>
+static void launch_jacobi32(fvg_ctx* c, dim3 grid, dim3 block, int n, const Tri32& t, const float* scale,
+                            const float* v, const float* yin, float* out, const int32_t* skip, const float* dw,
+                            double* dp) {
+    if (t.dcol)
+        hipLaunchKernelGGL((k_jacobi32<FIRST, SCALE, true>), grid, block, 0, c->stream, n, t.ptr, t.col, t.dcol, t.val,
+                           scale, v, yin, out, skip, dw, dp);
+    else
+        hipLaunchKernelGGL((k_jacobi32<FIRST, SCALE, false>), grid, block, 0, c->stream, n, t.ptr, t.col, t.dcol, t.val,
+                           scale, v, yin, out, skip, dw, dp);
+}
+
 static void jacobi32(fvg_ctx* c, const Tri32& t, int m, const float* scale, const float* v, float* odd, float* even,
                      const int32_t* skip, const float* dotw = nullptr, double* dpart = nullptr) {
     if (t.n == 0) return;
@@ -2155,18 +2187,14 @@ static void jacobi32(fvg_ctx* c, const Tri32& t, int m, const float* scale, cons
         const float* dw = k == m ? dotw : nullptr;
         if (k == 1) {
             if (scale)
-                hipLaunchKernelGGL((k_jacobi32<true, true>), grid, block, 0, c->stream, n, t.ptr, t.col, t.val, scale,
-                                   v, nullptr, out, skip, dw, dpart);
+                launch_jacobi32<true, true>(c, grid, block, n, t, scale, v, nullptr, out, skip, dw, dpart);
             else
-                hipLaunchKernelGGL((k_jacobi32<true, false>), grid, block, 0, c->stream, n, t.ptr, t.col, t.val,
-                                   nullptr, v, nullptr, out, skip, dw, dpart);
+                launch_jacobi32<true, false>(c, grid, block, n, t, nullptr, v, nullptr, out, skip, dw, dpart);
         } else {
             if (scale)
-                hipLaunchKernelGGL((k_jacobi32<false, true>), grid, block, 0, c->stream, n, t.ptr, t.col, t.val, scale,
-                                   v, prev, out, skip, dw, dpart);
+                launch_jacobi32<false, true>(c, grid, block, n, t, scale, v, prev, out, skip, dw, dpart);
             else
-                hipLaunchKernelGGL((k_jacobi32<false, false>), grid, block, 0, c->stream, n, t.ptr, t.col, t.val,
-                                   nullptr, v, prev, out, skip, dw, dpart);
+                launch_jacobi32<false, false>(c, grid, block, n, t, nullptr, v, prev, out, skip, dw, dpart);
         }
         prev = out;
     }

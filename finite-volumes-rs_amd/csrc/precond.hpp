@@ -98,6 +98,8 @@ struct Tri32 {
     int64_t n = 0, nnz = 0;
     int32_t* ptr = nullptr;    // n + 1
     int32_t* col = nullptr;    // nnz, ascending within a row
+    int16_t* dcol = nullptr;   // nnz: col minus the first row of the row's 256-row tile, where
+                               // every such offset fits 16 bits (else null)
     int32_t* src = nullptr;    // factor slot of each entry
     float* val = nullptr;      // nnz, gathered from the factor after each refactorisation
 };
