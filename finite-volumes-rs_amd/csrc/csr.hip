@@ -141,6 +141,7 @@ int launch_spmv(fvg_ctx* ctx, const fvg_csr* a, const double* x, double* y, cons
 
 static void free_csr(fvg_ctx* c, fvg_csr* a) {
     dev_free_n(c, a->val32, a->nnz);
+    if (a->dcol32) dev_free_n(c, a->dcol32, a->nnz);
     dev_free_n(c, a->row_ptr, a->nrows + 1);
     dev_free_n(c, a->col, a->nnz);
     dev_free_n(c, a->val, a->nnz);

@@ -106,6 +106,10 @@ struct fvg_csr {
     uint64_t version = 0;              // bumped whenever the values change
     float* val32 = nullptr;            // FP32 copy of the values for mixed precision (made on use)
     uint64_t val32_version = ~0ull;
+    // the columns as 16-bit offsets from the first row of each 256-row tile, for the FP32
+    // SpMV (made on use where every offset fits; state 0 not tried, 1 made, -1 does not fit)
+    int16_t* dcol32 = nullptr;
+    int32_t dcol32_state = 0;
     // coordinates of the rows (coord_dim per row, 0: none), used by the preconditioner
     // analysis to shape its blocked layouts; a matrix made from a mesh instead names the
     // mesh, whose cell centres the analysis reads while it is alive

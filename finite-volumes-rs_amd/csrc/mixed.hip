@@ -41,9 +41,12 @@ struct MState {
 };
 
 // ---- FP32 SpMV with optional fused dots (partials written as doubles) ---------------
-// y = A32 x; slot 0: sum dotw[r]*y[r]; slot 1: sum y[r]^2.
+// y = A32 x; slot 0: sum dotw[r]*y[r]; slot 1: sum y[r]^2.  DELTA: the columns are read as
+// 16-bit offsets from the tile's first row (dci) instead of ci.
+template <bool DELTA>
 __global__ void __launch_bounds__(kRows32) k_spmv32(int n, const int32_t* __restrict__ rp,
-                                                    const int32_t* __restrict__ ci, const float* __restrict__ v,
+                                                    const int32_t* __restrict__ ci,
+                                                    const int16_t* __restrict__ dci, const float* __restrict__ v,
                                                     const float* __restrict__ x, float* __restrict__ y,
                                                     const float* __restrict__ dotw, int dot_self,
                                                     double* __restrict__ partials, int64_t pstride,
@@ -68,7 +71,8 @@ __global__ void __launch_bounds__(kRows32) k_spmv32(int n, const int32_t* __rest
 #pragma unroll
             for (int u = 0; u < kUnroll32; ++u) {
                 const int k = min(k0 + u * kRows32, cnt - 1);
-                c[u] = __builtin_nontemporal_load(ci + base + k);   // streamed once: keep x in L2
+                c[u] = DELTA ? r0 + (int)__builtin_nontemporal_load(dci + base + k)
+                             : __builtin_nontemporal_load(ci + base + k);   // streamed once: keep x in L2
                 a[u] = __builtin_nontemporal_load(v + base + k);
             }
 #pragma unroll
@@ -84,7 +88,7 @@ __global__ void __launch_bounds__(kRows32) k_spmv32(int n, const int32_t* __rest
         }
     } else if (r < r1) {
         const int e = rp[r + 1];
-        for (int k = rp[r]; k < e; ++k) sum += v[k] * x[ci[k]];
+        for (int k = rp[r]; k < e; ++k) sum += v[k] * x[DELTA ? r0 + (int)dci[k] : ci[k]];
     }
     float d0 = 0.f, d1 = 0.f;
     if (r < r1) {
@@ -361,6 +365,29 @@ static int ensure_val32(fvg_ctx* c, fvg_csr* a) {
         hipLaunchKernelGGL(k_vals_f32, dim3(grid_for(a->nnz)), dim3(kBlock), 0, c->stream, a->nnz, a->val, a->val32);
     FVG_HIP(hipGetLastError());
     a->val32_version = a->version;
+    if (a->dcol32_state == 0 && a->nnz > 0 && (int64_t)a->h_row_ptr.size() == a->nrows + 1) {
+        // 16-bit column offsets from each tile's first row, where they all fit: the FP32
+        // SpMV then streams 2 bytes per entry for the columns instead of 4
+        std::vector<int16_t> d((size_t)a->nnz);
+        bool fits = true;
+        for (int64_t i = 0; i < a->nrows && fits; ++i) {
+            const int64_t r0 = i / kRows32 * kRows32;
+            for (int32_t k = a->h_row_ptr[(size_t)i]; k < a->h_row_ptr[(size_t)i + 1]; ++k) {
+                const int64_t off = (int64_t)a->h_col[(size_t)k] - r0;
+                if (off < INT16_MIN || off > INT16_MAX) {
+                    fits = false;
+                    break;
+                }
+                d[(size_t)k] = (int16_t)off;
+            }
+        }
+        a->dcol32_state = fits ? 1 : -1;
+        if (fits) {
+            FVG_TRY(dev_alloc_n(c, a->nnz, &a->dcol32));
+            FVG_TRY(upload(c, a->dcol32, d.data(), a->nnz));
+            FVG_HIP(hipStreamSynchronize(c->stream));   // d goes out of scope
+        }
+    }
     return FVG_OK;
 }
 
@@ -390,8 +417,12 @@ static int launch_spmv32(fvg_ctx* c, const fvg_csr* a, const float* x, float* y,
     const int64_t n = a->nrows;
     const int g = grid_for(n, kRows32);
     const int tok = timing_begin(c);
-    hipLaunchKernelGGL(k_spmv32, dim3(g), dim3(kRows32), 0, c->stream, (int)n, a->row_ptr, a->col, a->val32, x, y,
-                       dotw, dot_self, c->partials, c->partials_cap, skip);
+    if (a->dcol32)
+        hipLaunchKernelGGL(k_spmv32<true>, dim3(g), dim3(kRows32), 0, c->stream, (int)n, a->row_ptr, a->col,
+                           a->dcol32, a->val32, x, y, dotw, dot_self, c->partials, c->partials_cap, skip);
+    else
+        hipLaunchKernelGGL(k_spmv32<false>, dim3(g), dim3(kRows32), 0, c->stream, (int)n, a->row_ptr, a->col,
+                           a->dcol32, a->val32, x, y, dotw, dot_self, c->partials, c->partials_cap, skip);
     timing_end(c, tok, kSpanSpmv, 1);
     FVG_HIP(hipGetLastError());
     return FVG_OK;
