@@ -369,9 +369,16 @@ __device__ __forceinline__ void spmv_compute(const SpRow<T>& R, const SpA& M, co
     const int q = g * NT + (int)threadIdx.x;
     T s = T(0);
     if (win) {
+        // every slot's window read issued at once (a slot past the row's count holds its
+        // last entry), each product kept only for the row's own slots
+        T xv[kEA];
 #pragma unroll
-        for (int e = 0; e < kEA; ++e)
-            if (e < R.cnt) s += R.val[e] * lm(ring)[R.dep[e] & (RING - 1)];
+        for (int e = 0; e < kEA; ++e) xv[e] = lm(ring)[R.dep[e] & (RING - 1)];
+#pragma unroll
+        for (int e = 0; e < kEA; ++e) {
+            const T t = s + R.val[e] * xv[e];
+            s = e < R.cnt ? t : s;
+        }
         for (int k = R.kb + kEA; k < R.kb + R.cnt; ++k) s += gm(val)[k] * lm(ring)[gm(M.dep)[k] & (RING - 1)];
     } else {
 #pragma unroll
