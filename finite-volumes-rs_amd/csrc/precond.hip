@@ -126,6 +126,51 @@ __global__ void k_ilu0_level(int64_t begin, int64_t end, const int32_t* __restri
     ilu0_row(i, rp, col, diag, a, lu, flags);
 }
 
+// ILU(0) rows of one level, one wave per row (rows of at most 64 entries): lane t keeps
+// entry s + t of row i in a register, and for each lower entry k, in column order, the
+// lanes read the pivot row's entries right of its diagonal as one wave-wide load and take
+// them through shuffles.  Every entry receives the same updates in the same order as in
+// ilu0_row, so the factor has the same bits; what changes is that no merge walks the two
+// rows' columns one dependent load at a time.
+__global__ void __launch_bounds__(kBlock) k_ilu0_level_wave(int64_t begin, int64_t end,
+                                                            const int32_t* __restrict__ rows,
+                                                            const int32_t* __restrict__ rp,
+                                                            const int32_t* __restrict__ col,
+                                                            const int32_t* __restrict__ diag,
+                                                            const double* __restrict__ a, double* lu,
+                                                            int32_t* __restrict__ flags) {
+    const int64_t p = begin + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    if (p >= end) return;   // the same for all lanes of a wave
+    const int i = rows[p];
+    if (i < 0) return;
+    const int lane = threadIdx.x & 63;
+    const int s = rp[i], e = rp[i + 1], d = diag[i];
+    const bool mine = s + lane < e;
+    const int cx = mine ? col[s + lane] : -1;
+    double v = mine ? a[s + lane] : 0.0;
+    for (int k = s; k < d; ++k) {
+        const int kk = __shfl(cx, k - s);
+        const int dk = diag[kk], ek = rp[kk + 1];
+        const double ukk = lu[dk];
+        if (lane == 0 && fabs(ukk) < 1e-16) atomicAdd(&flags[0], 1);
+        const double inv = 1.0 / ukk;
+        if (lane == k - s) v = inv * v;
+        const double lik = __shfl(v, k - s);
+        for (int y0 = dk + 1; y0 < ek; y0 += 64) {
+            const int m = min(64, ek - y0);
+            const int cy = lane < m ? col[y0 + lane] : -2;
+            const double uy = lane < m ? lu[y0 + lane] : 0.0;
+            for (int t = 0; t < m; ++t) {
+                const int c = __shfl(cy, t);
+                const double u = __shfl(uy, t);
+                if (mine && lane > k - s && cx == c) v -= lik * u;
+            }
+        }
+    }
+    const double tol = fabs(__shfl(v, d - s)) * 1e-6;   // the drop rule of ilu0_row
+    if (mine) lu[s + lane] = (s + lane != d && !(fabs(v) > tol)) ? 0.0 : v;
+}
+
 // One workgroup refactorises one system of a group, level by level (a workgroup barrier
 // between levels instead of a launch per level), then makes the inverse pivots (ILU)
 // and gathers the sweep values: the same per-row arithmetic as the per-level launches.
@@ -1865,10 +1910,20 @@ static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a, bool check = tru
                                p->d_col, p->d_diag, av, p->d_fac, p->d_inv_diag, p->d_flags);
         }
     } else {
+        if (p->max_row < 0) {
+            int32_t w = 0;
+            for (int64_t i = 0; i < p->n; ++i) w = std::max(w, p->h_rp[(size_t)i + 1] - p->h_rp[(size_t)i]);
+            p->max_row = w;
+        }
+        const bool wave = p->max_row <= 64;   // a wave per row holds the row in its lanes
         for (int l = 0; l < lo.nlevels; ++l) {
             const int64_t b = lo.lvl_ptr[(size_t)l], e = lo.lvl_ptr[(size_t)l + 1];
-            hipLaunchKernelGGL(k_ilu0_level, dim3(grid_for(e - b)), dim3(kBlock), 0, c->stream, b, e, lo.rows,
-                               p->d_rp, p->d_col, p->d_diag, av, p->d_fac, p->d_flags);
+            if (wave)
+                hipLaunchKernelGGL(k_ilu0_level_wave, dim3(grid_for(64 * (e - b))), dim3(kBlock), 0, c->stream, b,
+                                   e, lo.rows, p->d_rp, p->d_col, p->d_diag, av, p->d_fac, p->d_flags);
+            else
+                hipLaunchKernelGGL(k_ilu0_level, dim3(grid_for(e - b)), dim3(kBlock), 0, c->stream, b, e, lo.rows,
+                                   p->d_rp, p->d_col, p->d_diag, av, p->d_fac, p->d_flags);
         }
         hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(p->n)), dim3(kBlock), 0, c->stream, p->n, p->d_diag, p->d_fac,
                            p->d_inv_diag);

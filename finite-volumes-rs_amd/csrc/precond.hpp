@@ -145,6 +145,7 @@ struct fvg_prec {
     mutable int32_t* pa_src = nullptr;          // slot of the original matrix of each of its entries
     mutable const fvg_csr* pa_of = nullptr;     // the matrix whose pattern was checked and whose values it holds
     mutable uint64_t pa_ver = ~0ull;            // version of those values
+    int32_t max_row = -1;               // longest factor row (set on first factorisation)
     uint64_t pattern_id = 0;            // mesh pattern the analysis was made for (0: not a mesh pattern)
 };
 
