@@ -151,3 +151,47 @@ def test_fill_layouts_with_coordinate_boxes(ctx, kind):
     assert layouts[64]["lower_blocks"] == 64 and layouts[64]["upper_blocks"] == 64, layouts[64]
     assert layouts[64]["mirrored"] == (1 if kind == "ic" else 0), layouts[64]
     assert layouts[64]["block_levels"] < layouts[1]["block_levels"], (layouts[64], layouts[1])
+
+
+def _banded(n, half, arrow, seed):
+    """Nonsymmetric, diagonally dominant CSR: a band of half-width `half`, plus a dense
+    first row and column when `arrow` (a row of n entries)."""
+    rng = np.random.default_rng(seed)
+    rp, ci, va = [0], [], []
+    for i in range(n):
+        cols = sorted(set(range(max(0, i - half), min(n, i + half + 1))) | ({0} if arrow else set())
+                      | (set(range(n)) if arrow and i == 0 else set()))
+        vals = rng.uniform(-1.0, 1.0, size=len(cols))
+        vals[cols.index(i)] = 2.0 * len(cols) + (n if arrow and i == 0 else 0)
+        ci += cols
+        va += list(vals)
+        rp.append(len(ci))
+    return np.array(rp, np.int32), np.array(ci, np.int32), np.array(va, np.float64)
+
+
+@pytest.mark.parametrize("n,half,arrow", [(300, 30, False), (200, 2, True)])
+def test_ilu0_rows_near_and_past_a_wave(ctx, n, half, arrow):
+    """The ILU refactorisation holds a row in the lanes of one wave when every row has at
+    most 64 entries (61 here) and falls back to a thread per row otherwise (an arrow matrix
+    whose first row holds all 200 columns): both give the oracle's factor and apply bits."""
+    rp, ci, va = _banded(n, half, arrow, seed=n)
+    A = fvg.DistributedMatrix.from_csr(ctx, rp, ci, va)
+    M = fvg.IncompleteLowerUpper.from_matrix(A, 0)
+    P = O.OraclePrec("ilu", rp, ci, va, 0)
+    orp, oci, ova, oinv = P.factor()
+    frp, fci = M.factor_pattern()
+    gva, ginv = M.factor_values()
+    assert np.array_equal(frp, rp) and np.array_equal(fci, ci)
+    for i in range(n):   # the oracle leaves out what the drop rule zeroes
+        cols, vals = fci[frp[i]:frp[i + 1]], gva[frp[i]:frp[i + 1]]
+        keep = np.isin(cols, oci[orp[i]:orp[i + 1]])
+        assert np.array_equal(cols[keep], oci[orp[i]:orp[i + 1]]), i
+        assert np.array_equal(vals[keep], ova[orp[i]:orp[i + 1]]), i
+        assert np.all(vals[~keep] == 0.0), i
+    r = np.random.default_rng(1).normal(size=n)
+    z = fvg.DistributedVector(ctx, n)
+    M.precondition(z, fvg.DistributedVector.from_data(ctx, r))
+    assert np.array_equal(z.numpy(), P.apply(r))
+    M.refactor(A)   # a second factorisation over the first gives the same bits
+    M.precondition(z, fvg.DistributedVector.from_data(ctx, r))
+    assert np.array_equal(z.numpy(), P.apply(r))
