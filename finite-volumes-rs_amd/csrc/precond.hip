@@ -126,6 +126,68 @@ __global__ void k_ilu0_level(int64_t begin, int64_t end, const int32_t* __restri
     ilu0_row(i, rp, col, diag, a, lu, flags);
 }
 
+// IC(0) rows of one level, one wave per row (rows of at most 64 entries): lane t keeps
+// l[s + t].  For each strictly lower entry k, the lanes of row i's earlier entries find
+// their column among row j's lower entries (one wave-wide load of that row, shuffled) and
+// form their products; every lane then subtracts the products in column order, so the
+// sum has ic0_row's order and bits.
+__global__ void __launch_bounds__(kBlock) k_ic0_level_wave(int64_t begin, int64_t end,
+                                                           const int32_t* __restrict__ rows,
+                                                           const int32_t* __restrict__ rp,
+                                                           const int32_t* __restrict__ col,
+                                                           const int32_t* __restrict__ diag,
+                                                           const double* __restrict__ a, double* l,
+                                                           double* inv_diag, int32_t* __restrict__ flags) {
+    const int64_t p = begin + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    if (p >= end) return;   // the same for all lanes of a wave
+    const int i = rows[p];
+    if (i < 0) return;
+    const int lane = threadIdx.x & 63;
+    const int s = rp[i], d = diag[i];
+    const bool mine = s + lane <= d;
+    const int cx = mine ? col[s + lane] : -1;
+    const double ax = mine ? a[s + lane] : 0.0;
+    double lx = 0.0;
+    for (int k = s; k < d; ++k) {
+        const int j = __shfl(cx, k - s);
+        const int yb = rp[j], ye = diag[j];
+        const double inv = inv_diag[j];
+        double prod = 0.0;
+        bool hit = false;
+        for (int y0 = yb; y0 < ye; y0 += 64) {
+            const int m = min(64, ye - y0);
+            const int cy = lane < m ? col[y0 + lane] : -2;
+            const double ly = lane < m ? l[y0 + lane] : 0.0;
+            for (int t = 0; t < m; ++t) {
+                const int c = __shfl(cy, t);
+                const double u = __shfl(ly, t);
+                if (lane < k - s && cx == c) {
+                    prod = lx * u;
+                    hit = true;
+                }
+            }
+        }
+        double v = __shfl(ax, k - s);
+        for (int x = 0; x < k - s; ++x) {
+            const double px = __shfl(prod, x);
+            if (__shfl((int)hit, x)) v -= px;
+        }
+        if (lane == k - s) lx = inv * v;
+    }
+    double dd = __shfl(ax, d - s);
+    for (int x = 0; x < d - s; ++x) {
+        const double q = __shfl(lx, x);
+        dd -= q * q;
+    }
+    if (lane == 0 && !(dd > 0.0)) atomicAdd(&flags[1], 1);
+    const double lkk = sqrt(dd);
+    if (s + lane < d) l[s + lane] = lx;
+    if (s + lane == d) {
+        l[d] = lkk;
+        inv_diag[i] = 1.0 / lkk;
+    }
+}
+
 // ILU(0) rows of one level, one wave per row (rows of at most 64 entries): lane t keeps
 // entry s + t of row i in a register, and for each lower entry k, in column order, the
 // lanes read the pivot row's entries right of its diagonal as one wave-wide load and take
@@ -1892,6 +1954,16 @@ static int read_flags(fvg_ctx* c, const fvg_prec* p, int32_t* host4) {
 // whole factorisation is only enqueued and the flags are read by fvg_prec_status.
 static const double* factor_input(fvg_ctx* c, const fvg_prec* p, const fvg_csr* a);
 
+// entries of the factor pattern's longest row (counted on first use)
+static int32_t longest_row(fvg_prec* p) {
+    if (p->max_row < 0) {
+        int32_t w = 0;
+        for (int64_t i = 0; i < p->n; ++i) w = std::max(w, p->h_rp[(size_t)i + 1] - p->h_rp[(size_t)i]);
+        p->max_row = w;
+    }
+    return p->max_row;
+}
+
 static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a, bool check = true) {
     FVG_HIP(hipMemsetAsync(p->d_flags, 0, 4 * sizeof(int32_t), c->stream));
     const Sweep& lo = p->lower;
@@ -1904,18 +1976,18 @@ static int factorize(fvg_ctx* c, fvg_prec* p, const fvg_csr* a, bool check = tru
             FVG_TRY(read_flags(c, p, fl));
             if (fl[2]) return fail(FVG_E_NOT_SYMMETRIC, "IncompleteCholesky requires a symmetric matrix");
         }
+        const bool wave = longest_row(p) <= 64;   // a wave per row holds the row in its lanes
         for (int l = 0; l < lo.nlevels; ++l) {
             const int64_t b = lo.lvl_ptr[(size_t)l], e = lo.lvl_ptr[(size_t)l + 1];
-            hipLaunchKernelGGL(k_ic0_level, dim3(grid_for(e - b)), dim3(kBlock), 0, c->stream, b, e, lo.rows, p->d_rp,
-                               p->d_col, p->d_diag, av, p->d_fac, p->d_inv_diag, p->d_flags);
+            if (wave)
+                hipLaunchKernelGGL(k_ic0_level_wave, dim3(grid_for(64 * (e - b))), dim3(kBlock), 0, c->stream, b, e,
+                                   lo.rows, p->d_rp, p->d_col, p->d_diag, av, p->d_fac, p->d_inv_diag, p->d_flags);
+            else
+                hipLaunchKernelGGL(k_ic0_level, dim3(grid_for(e - b)), dim3(kBlock), 0, c->stream, b, e, lo.rows,
+                                   p->d_rp, p->d_col, p->d_diag, av, p->d_fac, p->d_inv_diag, p->d_flags);
         }
     } else {
-        if (p->max_row < 0) {
-            int32_t w = 0;
-            for (int64_t i = 0; i < p->n; ++i) w = std::max(w, p->h_rp[(size_t)i + 1] - p->h_rp[(size_t)i]);
-            p->max_row = w;
-        }
-        const bool wave = p->max_row <= 64;   // a wave per row holds the row in its lanes
+        const bool wave = longest_row(p) <= 64;
         for (int l = 0; l < lo.nlevels; ++l) {
             const int64_t b = lo.lvl_ptr[(size_t)l], e = lo.lvl_ptr[(size_t)l + 1];
             if (wave)
